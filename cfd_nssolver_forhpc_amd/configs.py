@@ -1,0 +1,95 @@
+"""Workload definitions: the reference's native case and the five cylinder
+configurations (128x64x4 CPU case, 512x256x64 FP64 and mixed, 256x128x32
+adjoint case, 1024x512x128 stretched case)."""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+from . import capi
+from .capi import CfdParams, GridArrays
+from .solver import axis_tanh, default_params, native_grid, uniform_grid
+from .capi import GridArrays as _GA
+
+
+@dataclass
+class Workload:
+    name: str
+    grid: GridArrays
+    params: CfdParams
+    steps: int
+    noise: float
+    seed: int
+
+
+def reference_native() -> Workload:
+    """The case the reference compiles in: 190x210x40 SML grid, cavity
+    table, cylinder (6.5, 6) R 0.5, Re 40, dt 0.002, BiCGSTAB to 1e-3."""
+    return Workload("native-190x210x40", native_grid(), default_params(), 20000, 0.0, 0)
+
+
+def cylinder_params(Re: float, dt: float, cx: float, cy: float, D: float, tol: float,
+                    precision: int = capi.CFD_FP64) -> CfdParams:
+    """Flow past a z-cylinder: uniform inflow u = 1 on x-, zero-gradient
+    outflow on x+, slip walls on y and z; pressure Neumann except p = 0 on
+    the outflow face; source term added at the cell it belongs to."""
+    p = default_params()
+    p.Re = Re
+    p.dt = dt
+    p.set_bc(capi.CHANNEL_SLIP_DIR, capi.CHANNEL_SLIP_NUM)
+    for f in range(6):
+        p.p_dirichlet[f] = 0
+    p.p_dirichlet[1] = 1
+    p.source_index_quirk = 0
+    p.inner_loops = 3
+    p.solver = capi.CFD_SOLVER_BICGSTAB
+    p.op = capi.CFD_OP_ELL
+    p.precision = precision
+    p.tol = tol
+    p.tol_inner = 1e-5
+    p.maxit = 5000
+    p.ib_on = 1
+    p.ib_cx, p.ib_cy, p.ib_r = cx, cy, 0.5 * D
+    p.ib_nsub = 100
+    p.u_init[0], p.u_init[1], p.u_init[2] = 1.0, 0.0, 0.0
+    return p
+
+
+def config_a() -> Workload:
+    """128x64x4 over 20D x 10D x 0.5D, D = 1 at (5, 5), Re 100, dt 0.002, 100 steps, FP64, tol 1e-10."""
+    g = uniform_grid((128, 64, 4), (20.0, 10.0, 0.5))
+    return Workload("cyl-128x64x4-Re100", g, cylinder_params(100.0, 0.002, 5.0, 5.0, 1.0, 1e-10), 100, 1e-3, 1)
+
+
+def config_b(precision: int = capi.CFD_FP64) -> Workload:
+    """512x256x64 over 20 x 10 x 2.5 (D = 25 cells), Re 200, dt 5e-4, 200 steps, tol 1e-10."""
+    g = uniform_grid((512, 256, 64), (20.0, 10.0, 2.5))
+    name = "cyl-512x256x64-Re200" + ("-mixed" if precision == capi.CFD_MIXED else "-fp64")
+    return Workload(name, g, cylinder_params(200.0, 5e-4, 5.0, 5.0, 1.0, 1e-10, precision), 200, 1e-3, 1)
+
+
+def config_d() -> Workload:
+    """256x128x32 over 20 x 10 x 2.5, Re 100, dt 1e-3, 400 steps (adjoint case)."""
+    g = uniform_grid((256, 128, 32), (20.0, 10.0, 2.5))
+    return Workload("cyl-256x128x32-Re100", g, cylinder_params(100.0, 1e-3, 5.0, 5.0, 1.0, 1e-10), 400, 1e-3, 1)
+
+
+def config_e(precision: int = capi.CFD_MIXED) -> Workload:
+    """1024x512x128, x/y stretched (growth 1.05) around a D = 40-cell cylinder, Re 300, dt 2.5e-4, 100 steps."""
+    D = 1.0
+    h0 = D / 40.0
+    ax = axis_tanh(1024, 40.0, 10.0, h0, 4.0 * D, 1.05)
+    ay = axis_tanh(512, 20.0, 10.0, h0, 4.0 * D, 1.05)
+    from .solver import axis_uniform
+    az = axis_uniform(128, 128 * h0)
+    g = _GA((1024, 512, 128), ax[0], ay[0], az[0], ax[1], ay[1], az[1], ax[2], ay[2], az[2], 40.0, 20.0, 128 * h0)
+    return Workload("cyl-1024x512x128-Re300", g, cylinder_params(300.0, 2.5e-4, 10.0, 10.0, D, 1e-10, precision),
+                    100, 1e-3, 1)
+
+
+def small_cylinder(ncal=(32, 16, 8), lengths=(8.0, 4.0, 2.0), quirk=0, tol=1e-10, inner=3) -> Workload:
+    """A few-thousand-cell cylinder case for parity tests."""
+    g = uniform_grid(ncal, lengths)
+    p = cylinder_params(100.0, 0.002, lengths[0] / 4.0, lengths[1] / 2.0, 1.0, tol)
+    p.source_index_quirk = quirk
+    p.inner_loops = inner
+    return Workload("small", g, p, 10, 1e-3, 7)

@@ -1,0 +1,163 @@
+// cfd_internal.h -- context layout and kernel launchers shared by the HIP
+// translation units of libcfd_hip.so.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/cfd_capi.h"
+
+namespace cfd {
+
+constexpr int GC = 2;           // ghost layers per side (grid::gC)
+constexpr int KSLOTS = 7;       // ELL slots per row (createPressureMatrix resize(N, 7))
+constexpr int RED_THREADS = 256;
+constexpr int RED_MAX_BLOCKS = 2048;  // upper bound of partial sums per reduction
+
+// Grid extents and 1-D metrics on the device.
+struct DevGrid {
+  int nx, ny, nz;        // ghosted extents
+  int nxc, nyc, nzc;     // interior extents
+  int64_t G, N;          // ghosted / interior element counts
+  const double *X, *Y, *Z;
+  const double *Dx, *Dy, *Dz, *Dxs, *Dys, *Dzs;
+  double lx, ly, lz;
+};
+
+__host__ __device__ inline int64_t icel(const DevGrid &g, int i, int j, int k) {
+  return ((int64_t)i * g.ny + j) * g.nz + k;
+}
+__host__ __device__ inline int64_t icelcal(const DevGrid &g, int i, int j, int k) {
+  return ((int64_t)(i - GC) * g.nyc + (j - GC)) * g.nzc + (k - GC);
+}
+
+// Device-resident scalars of the Krylov solvers: stream-ordered, never
+// round-tripped to the host inside an iteration.
+struct KrylovScalars {
+  double rho, pre_rho, alpha, omega, beta, norm0, rnorm, relres;
+  double tol, restart_tol;
+  int iters, maxit, done, hit_maxit, restart, timestep, restarts, pad;
+};
+
+// Reduction workspace: per-block partials plus a completion counter per
+// reduction site; the last block to finish combines the partials in block
+// order (deterministic) and resets its counter.
+struct RedWork {
+  double *partials;      // [RED_SITES][3][RED_MAX_BLOCKS]
+  unsigned *counters;    // [RED_SITES]
+  double *out;           // [RED_SITES][4] combined results
+};
+enum RedSite { RS_SPMV_DOT = 0, RS_SPMV_DOT2, RS_XR, RS_INIT, RS_NORM, RS_DRAG, RS_MAX, RS_MISC, RED_SITES };
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  cfd_params prm{};
+  DevGrid g{};
+  // host copies of the metrics
+  std::vector<double> hX, hY, hZ, hDx, hDy, hDz, hDxs, hDys, hDzs;
+  // fields
+  double *f[CFD_NFIELDS] = {};
+  // ELL (slot-major), FP32 copy, colour-permuted copy for RB-SOR
+  int32_t *col = nullptr;
+  double *val = nullptr;
+  float *val32 = nullptr;
+  int32_t *colP = nullptr;
+  double *valP = nullptr;
+  float *valP32 = nullptr;
+  int32_t *perm = nullptr;  // natural row -> permuted position
+  int64_t ncolour[2] = {0, 0};
+  bool ell_ready = false, spe_ready = false;
+  // SPE coefficients (xm, xp over nxc; ym, yp over nyc; zm, zp over nzc) and jp
+  double *spe_coef = nullptr;
+  double *spe_jp = nullptr;
+  // solver vectors (FP64)
+  double *r0 = nullptr, *r1 = nullptr, *p1 = nullptr, *s1 = nullptr, *ap = nullptr, *as = nullptr;
+  double *px1 = nullptr, *px2 = nullptr;
+  // FP32 inner-solve vectors (mixed precision) and permuted SOR vectors
+  float *q32[8] = {};
+  double *xP = nullptr, *bP = nullptr, *rP = nullptr;
+  float *xP32 = nullptr, *bP32 = nullptr;
+  // device scalars and reduction workspace
+  KrylovScalars *ks = nullptr;   // FP64 outer / plain solve
+  KrylovScalars *ks32 = nullptr; // FP32 inner solve
+  KrylovScalars *hks = nullptr;  // pinned host mirror
+  RedWork red{};
+  double *hred = nullptr;        // pinned host mirror of red.out
+  double *stepbuf = nullptr;     // per-pass drag sums and max differences of one step
+  int solves = 0;                // bicgstabRe2::timestep_ (solve count)
+  int solves32 = 0;
+  int loop = 1;
+  double time = 0.0;
+  bool p_from_x = false;         // ghosted p interior lags X (step mode)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+inline double *red_part_host(const Ctx &c, int site) {
+  return c.red.partials + (size_t)site * 3 * RED_MAX_BLOCKS;
+}
+
+// error reporting
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+#define CFD_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return ::cfd::fail(CFD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+inline int blocks_for(int64_t n, int threads, int cap) {
+  int64_t b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (int)b;
+}
+
+// ---- launchers (return hipError_t of the launch) -------------------------
+// k_flow.hip
+hipError_t launch_init_fields(Ctx &c, double noise_amp, uint64_t seed);
+hipError_t launch_eta_cylinder_z(Ctx &c, double inv_n2);
+hipError_t launch_predictor(Ctx &c, const double *u0, const double *v0, const double *w0, double *u1,
+                            double *v1, double *w1, const double *src, int src_mode);
+hipError_t launch_source_add(Ctx &c, double *u1, double *v1, double *w1, const double *src, int quirk);
+hipError_t launch_source_quirk_fixup(Ctx &c, double *u1, double *v1, double *w1, const double *src);
+hipError_t launch_rhs(Ctx &c, const double *u1, const double *v1, const double *w1, const double *jp, double *b);
+hipError_t launch_scatter_p(Ctx &c, const double *X, double *p);
+// mode bits for the DFIB update
+enum { UPD_WRITE_T3 = 1, UPD_WRITE_T0 = 2, UPD_SRC_SET = 4, UPD_SRC_ADD = 8, UPD_DRAG = 16, UPD_MAXDIFF = 32,
+       UPD_P_FROM_X = 64 };
+hipError_t launch_dfib_update_into(Ctx &c, int mode, const double *p_or_x, double *drag_out, double *max_out);
+hipError_t launch_source_accumulate(Ctx &c);
+hipError_t launch_bc(Ctx &c, bool main, bool copy, bool pressure);
+hipError_t launch_drag(Ctx &c);
+hipError_t launch_maxdiff(Ctx &c);
+hipError_t launch_copy(Ctx &c, const double *src, double *dst, int64_t n);
+hipError_t launch_fill(Ctx &c, double *dst, double v, int64_t n);
+// k_ell.hip
+hipError_t launch_ell_assemble(Ctx &c);
+hipError_t launch_ell_permute(Ctx &c);
+hipError_t launch_spe_assemble(Ctx &c);
+hipError_t launch_spmv(Ctx &c, int op, int precision, const double *x, double *y);
+hipError_t launch_spmv_ell_f32(Ctx &c, const float *x, float *y);
+hipError_t launch_cast32(Ctx &c, const double *a, float *b, int64_t n);
+int stream_blocks(int64_t n);
+// k_krylov.hip
+int bicgstab_solve(Ctx &c, const double *b, double *x, double tol, int maxit, int *iters, double *relres);
+int defect_correction(Ctx &c, const double *b, double *x, double tol, double tol_inner, int max_outer,
+                      int *outer, int *inner_total, double *relres);
+hipError_t launch_bicg_iteration(Ctx &c);
+hipError_t launch_bicg_timing_setup(Ctx &c);
+// k_sor.hip
+int rbsor_solve(Ctx &c, const double *b, double *x, double omega, double tol, int maxit, int check_every,
+                int *iters, double *relres);
+hipError_t launch_rbsor_sweep(Ctx &c, double omega);
+int rbsor_solve_f32(Ctx &c, const float *b, float *x, double omega, double tol, int maxit, int check_every,
+                    int *iters, double *relres);
+hipError_t launch_copy_kernel_bench(Ctx &c, const double *src, double *dst, int64_t n);
+
+}  // namespace cfd

@@ -1,0 +1,97 @@
+// dev_spmv.h -- row kernels of the pressure operator, inlined into the
+// SpMV, Krylov and SOR kernels.
+//
+// ELL rows are read slot-major (val[s*N + row]): lane r of a wave reads
+// element r of each slot slab, so every slot load of a wave64 is one
+// contiguous 512 B (FP64) / 256 B (FP32) segment.  The matrix streams once
+// per SpMV and is loaded non-temporally so that it does not evict the
+// x vector, whose seven gathers per row (row, row +- 1, +- nz, +- ny*nz)
+// are served from L2 / Infinity Cache.  The accumulation is an FMA chain in
+// slot order, which is exactly what the reference build computes for
+// r[i] += VAL * x[IDX] (inc/matrix/ELL_sparseMatrix.hpp:477-492).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cfd_internal.h"
+
+namespace cfd {
+
+template <typename T>
+__device__ inline T ntload(const T *p) {
+  return __builtin_nontemporal_load(p);
+}
+
+template <typename TV, typename TX>
+__device__ inline TX ell_row(const int32_t *__restrict__ col, const TV *__restrict__ val, int64_t N, int64_t r,
+                             const TX *__restrict__ x) {
+  TX acc = TX(0);
+#pragma unroll
+  for (int s = 0; s < KSLOTS; ++s) {
+    const int32_t c = ntload(col + (int64_t)s * N + r);
+    const TX v = (TX)ntload(val + (int64_t)s * N + r);
+    acc = fma(v, x[c], acc);
+  }
+  return acc;
+}
+
+// SPE_matrix1::spmv_ompJP (inc/matrix/SPE_sparseMatrix1.hpp:126-144): the
+// Jacobi-scaled operator applied matrix-free; coef = xm[nxc] xp[nxc]
+// ym[nyc] yp[nyc] zm[nzc] zp[nzc]; neighbour terms in the order xm, ym, zm,
+// diagonal, zp, yp, xp, each contracted to an FMA like the reference build.
+__device__ inline double spe_row(const double *__restrict__ coef, const double *__restrict__ jp, int nxc, int nyc,
+                                 int nzc, int64_t ic, const double *__restrict__ x) {
+  const int64_t nynz = (int64_t)nyc * nzc;
+  const int i = (int)(ic / nynz);
+  const int64_t rem = ic - (int64_t)i * nynz;
+  const int j = (int)(rem / nzc);
+  const int k = (int)(rem - (int64_t)j * nzc);
+  const double *xm = coef, *xp = xm + nxc, *ym = xp + nxc, *yp = ym + nyc, *zm = yp + nyc, *zp = zm + nzc;
+  const double J = jp[ic];
+  double sum = 0.0;
+  if (i != 0) sum = fma(-xm[i] * J, x[ic - nynz], sum);
+  if (j != 0) sum = fma(-ym[j] * J, x[ic - nzc], sum);
+  if (k != 0) sum = fma(-zm[k] * J, x[ic - 1], sum);
+  sum += x[ic];
+  if (k != nzc - 1) sum = fma(-zp[k] * J, x[ic + 1], sum);
+  if (j != nyc - 1) sum = fma(-yp[j] * J, x[ic + nzc], sum);
+  if (i != nxc - 1) sum = fma(-xp[i] * J, x[ic + nynz], sum);
+  return sum;
+}
+
+// Operator handle passed by value to the kernels.
+struct OpArgs {
+  int op;                    // CFD_OP_ELL / CFD_OP_SPE
+  int64_t N;
+  const int32_t *col;
+  const double *val;
+  const float *val32;
+  const double *coef, *jp;   // SPE
+  int nxc, nyc, nzc;
+};
+
+__device__ inline double op_row(const OpArgs &a, int64_t r, const double *__restrict__ x) {
+  if (a.op == CFD_OP_SPE) return spe_row(a.coef, a.jp, a.nxc, a.nyc, a.nzc, r, x);
+  return ell_row<double, double>(a.col, a.val, a.N, r, x);
+}
+
+__device__ inline float op_row32(const OpArgs &a, int64_t r, const float *__restrict__ x) {
+  return ell_row<float, float>(a.col, a.val32, a.N, r, x);
+}
+
+inline OpArgs op_args(const Ctx &c, int op) {
+  OpArgs a;
+  a.op = op;
+  a.N = c.g.N;
+  a.col = c.col;
+  a.val = c.val;
+  a.val32 = c.val32;
+  a.coef = c.spe_coef;
+  a.jp = c.spe_jp;
+  a.nxc = c.g.nxc;
+  a.nyc = c.g.nyc;
+  a.nzc = c.g.nzc;
+  return a;
+}
+
+}  // namespace cfd

@@ -1,0 +1,193 @@
+// k_ell.hip -- pressure-operator assembly and standalone SpMV kernels.
+#include "dev_common.h"
+#include "dev_spmv.h"
+
+namespace cfd {
+
+namespace {
+
+// createPressureMatrix (inc/matrix/PressureMatrix.hpp:5-117) one row per
+// thread, written straight into the slot-major layout: slot 0 diagonal
+// val*jp, then -xm*jp, -xp*jp, -ym*jp, -yp*jp, -zm*jp, -zp*jp for the
+// neighbours that exist, then padding (N-1, 0) -- the slot order
+// mat::ELL_matrix::set leaves (ELL_sparseMatrix.hpp:146-160, 246-320).
+// Divisions and the 9-term diagonal sum are evaluated in the reference's
+// order, so every stored value is bit-identical to the reference's.
+struct PBC {
+  int pd[6];
+};
+
+__global__ void __launch_bounds__(256) ell_assemble_kernel(DevGrid g, PBC b, int32_t *__restrict__ col,
+                                                           double *__restrict__ val, float *__restrict__ val32,
+                                                           double *__restrict__ jp) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t N = g.N;
+  if (r >= N) return;
+  const int64_t nynz = (int64_t)g.nyc * g.nzc;
+  const int i = (int)(r / nynz) + GC;
+  const int j = (int)((r / g.nzc) % g.nyc) + GC;
+  const int k = (int)(r % g.nzc) + GC;
+  const int xBeg = GC, yBeg = GC, zBeg = GC, xEnd = g.nx - GC - 1, yEnd = g.ny - GC - 1, zEnd = g.nz - GC - 1;
+  const double xp = 1.0 / g.Dx[i] / g.Dxs[i], yp = 1.0 / g.Dy[j] / g.Dys[j], zp = 1.0 / g.Dz[k] / g.Dzs[k];
+  const double xm = 1.0 / g.Dx[i] / g.Dxs[i - 1], ym = 1.0 / g.Dy[j] / g.Dys[j - 1],
+               zm = 1.0 / g.Dz[k] / g.Dzs[k - 1];
+  double aa = 0.0, bb = 0.0, cc = 0.0;
+  if (i == xBeg && !b.pd[0]) aa = -xm;
+  if (i == xEnd && !b.pd[1]) aa = -xp;
+  if (j == yBeg && !b.pd[2]) bb = -ym;
+  if (j == yEnd && !b.pd[3]) bb = -yp;
+  if (k == zBeg && !b.pd[4]) cc = -zm;
+  if (k == zEnd && !b.pd[5]) cc = -zp;
+  double vd = xm + xp;
+  vd = vd + ym;
+  vd = vd + yp;
+  vd = vd + zm;
+  vd = vd + zp;
+  vd = vd + aa;
+  vd = vd + bb;
+  vd = vd + cc;
+  const double jr = 1. / vd;
+  jp[r] = jr;
+  int32_t cs[KSLOTS];
+  double vs[KSLOTS];
+  int s = 0;
+  cs[s] = (int32_t)r; vs[s] = vd * jr; ++s;
+  if (i != xBeg) { cs[s] = (int32_t)(r - nynz); vs[s] = -xm * jr; ++s; }
+  if (i != xEnd) { cs[s] = (int32_t)(r + nynz); vs[s] = -xp * jr; ++s; }
+  if (j != yBeg) { cs[s] = (int32_t)(r - g.nzc); vs[s] = -ym * jr; ++s; }
+  if (j != yEnd) { cs[s] = (int32_t)(r + g.nzc); vs[s] = -yp * jr; ++s; }
+  if (k != zBeg) { cs[s] = (int32_t)(r - 1); vs[s] = -zm * jr; ++s; }
+  if (k != zEnd) { cs[s] = (int32_t)(r + 1); vs[s] = -zp * jr; ++s; }
+  for (; s < KSLOTS; ++s) { cs[s] = (int32_t)(N - 1); vs[s] = 0.0; }
+#pragma unroll
+  for (int q = 0; q < KSLOTS; ++q) {
+    col[(int64_t)q * N + r] = cs[q];
+    val[(int64_t)q * N + r] = vs[q];
+    if (val32) val32[(int64_t)q * N + r] = (float)vs[q];
+  }
+}
+
+// Colour-ordered copy of the ELL matrix for red-black SOR: row r moves to
+// perm[r] (colour-0 rows first, natural order within a colour) and its
+// columns are renumbered the same way, so each colour sweep streams a
+// contiguous half of the matrix.
+__global__ void __launch_bounds__(256) ell_permute_kernel(int64_t N, const int32_t *__restrict__ perm,
+                                                          const int32_t *__restrict__ col,
+                                                          const double *__restrict__ val, int32_t *__restrict__ colP,
+                                                          double *__restrict__ valP, float *__restrict__ valP32) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  const int64_t q = perm[r];
+#pragma unroll
+  for (int s = 0; s < KSLOTS; ++s) {
+    colP[(int64_t)s * N + q] = perm[col[(int64_t)s * N + r]];
+    const double v = val[(int64_t)s * N + r];
+    valP[(int64_t)s * N + q] = v;
+    if (valP32) valP32[(int64_t)s * N + q] = (float)v;
+  }
+}
+
+// SPE_matrix1::setupPressure / set_Jp_pc diagonal (SPE_sparseMatrix1.hpp:78-121)
+__global__ void __launch_bounds__(256) spe_jp_kernel(int nxc, int nyc, int nzc, PBC b, const double *__restrict__ coef,
+                                                     double *__restrict__ jp) {
+  const int64_t ic = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nynz = (int64_t)nyc * nzc;
+  if (ic >= nynz * nxc) return;
+  const int i = (int)(ic / nynz), j = (int)((ic / nzc) % nyc), k = (int)(ic % nzc);
+  const double *xm = coef, *xp = xm + nxc, *ym = xp + nxc, *yp = ym + nyc, *zm = yp + nyc, *zp = zm + nzc;
+  double ijk = xm[i] + ym[j];
+  ijk = ijk + zm[k];
+  ijk = ijk + xp[i];
+  ijk = ijk + yp[j];
+  ijk = ijk + zp[k];
+  if (i == 0 && !b.pd[0]) ijk -= xm[i];
+  if (i == nxc - 1 && !b.pd[1]) ijk -= xp[i];
+  if (j == 0 && !b.pd[2]) ijk -= ym[j];
+  if (j == nyc - 1 && !b.pd[3]) ijk -= yp[j];
+  if (k == 0 && !b.pd[4]) ijk -= zm[k];
+  if (k == nzc - 1 && !b.pd[5]) ijk -= zp[k];
+  jp[ic] = 1.0 / ijk;
+}
+
+__global__ void __launch_bounds__(256) spmv_kernel(OpArgs a, const double *__restrict__ x, double *__restrict__ y) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) y[r] = op_row(a, r, x);
+}
+
+// FP32 values and vectors, FP32 accumulation (the mixed-precision inner operator)
+__global__ void __launch_bounds__(256) spmv32_kernel(OpArgs a, const float *__restrict__ x, float *__restrict__ y) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) y[r] = op_row32(a, r, x);
+}
+
+// FP32 matrix applied to an FP64 vector: x rounded to FP32 on load, result widened
+__global__ void __launch_bounds__(256) spmv32_from64_kernel(OpArgs a, const double *__restrict__ x,
+                                                            double *__restrict__ y) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int s = 0; s < KSLOTS; ++s) {
+      const int32_t c = ntload(a.col + (int64_t)s * a.N + r);
+      acc = fmaf(ntload(a.val32 + (int64_t)s * a.N + r), (float)x[c], acc);
+    }
+    y[r] = (double)acc;
+  }
+}
+
+__global__ void __launch_bounds__(256) cast32_kernel(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = (float)a[t];
+}
+
+PBC pbc_of(const Ctx &c) {
+  PBC b;
+  for (int q = 0; q < 6; ++q) b.pd[q] = c.prm.p_dirichlet[q];
+  return b;
+}
+
+}  // namespace
+
+// grid size of the streaming kernels: enough waves to cover HBM latency on
+// 256 CUs, capped so per-thread loops stay long enough to amortise setup
+int stream_blocks(int64_t n) { return blocks_for(n, 256, 8192); }
+
+hipError_t launch_ell_assemble(Ctx &c) {
+  ell_assemble_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g, pbc_of(c), c.col, c.val, c.val32,
+                                                                             c.f[CFD_JP]);
+  return hipGetLastError();
+}
+
+hipError_t launch_ell_permute(Ctx &c) {
+  ell_permute_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g.N, c.perm, c.col, c.val, c.colP,
+                                                                            c.valP, c.valP32);
+  return hipGetLastError();
+}
+
+hipError_t launch_cast32(Ctx &c, const double *a, float *b, int64_t n) {
+  cast32_kernel<<<blocks_for(n, 256, 1 << 30), 256, 0, c.stream>>>(a, b, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_spe_assemble(Ctx &c) {
+  spe_jp_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g.nxc, c.g.nyc, c.g.nzc, pbc_of(c),
+                                                                       c.spe_coef, c.spe_jp);
+  return hipGetLastError();
+}
+
+hipError_t launch_spmv(Ctx &c, int op, int precision, const double *x, double *y) {
+  const OpArgs a = op_args(c, op);
+  if (precision == CFD_MIXED && op == CFD_OP_ELL)
+    spmv32_from64_kernel<<<stream_blocks(c.g.N), 256, 0, c.stream>>>(a, x, y);
+  else
+    spmv_kernel<<<stream_blocks(c.g.N), 256, 0, c.stream>>>(a, x, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_spmv_ell_f32(Ctx &c, const float *x, float *y) {
+  const OpArgs a = op_args(c, CFD_OP_ELL);
+  spmv32_kernel<<<stream_blocks(c.g.N), 256, 0, c.stream>>>(a, x, y);
+  return hipGetLastError();
+}
+
+}  // namespace cfd

@@ -1,0 +1,696 @@
+// k_flow.hip -- gfx950 kernels of the velocity side of the timestep:
+// QUICK convection-diffusion predictor, source term, pressure RHS, DFIB
+// forcing/correction, boundary conditions, IB mask, initial state.
+//
+// All fields use the reference's ghosted layout icel = (i*ny + j)*nz + k
+// (k fastest), so consecutive lanes of a wave64 touch consecutive k and every
+// stencil read of a wave is a contiguous (coalesced) segment; i/j neighbours
+// are reused through L2.  These kernels are HBM-bound (0.3-2 flop/B), so
+// they are written for one pass over memory per phase, with the phases of a
+// pass fused where the data dependence allows.
+#include "dev_common.h"
+
+namespace cfd {
+
+namespace {
+
+struct Cell {
+  int i, j, k;
+};
+
+// interior flat index t in [0, N) -> ghosted (i, j, k)
+__device__ inline Cell decode_interior(const DevGrid &g, int64_t t) {
+  Cell c;
+  const int64_t nzc = g.nzc, nyc = g.nyc;
+  const int64_t jk = t % (nyc * nzc);
+  c.i = (int)(t / (nyc * nzc)) + GC;
+  c.j = (int)(jk / nzc) + GC;
+  c.k = (int)(jk % nzc) + GC;
+  return c;
+}
+
+// QuickUnitA (inc/source/convection_and_difussion.hpp:20-49)
+__device__ inline void quickA(double Up, double Un, int64_t pp, int64_t p, int64_t ic, int64_t n, int64_t nn,
+                              const double *__restrict__ Di, const double *__restrict__ Ds, int I,
+                              const double *__restrict__ phi, double &phiP, double &phiN) {
+  const double fc = phi[ic], fp = phi[p], fn = phi[n];
+  if (Up > 0.0)
+    phiP = 0.5 * (fc + fp) - 0.125 * Di[I + 1] * Di[I + 1] / Ds[I] * ((fp - fc) / Di[I + 1] - (fc - fn) / Di[I]);
+  else
+    phiP = 0.5 * (fc + fp) -
+           0.125 * Di[I + 1] * Di[I + 1] / Ds[I + 1] * ((phi[pp] - fp) / Di[I + 2] - (fp - fc) / Di[I + 1]);
+  if (Un > 0.0)
+    phiN = 0.5 * (fn + fc) - 0.125 * Di[I] * Di[I] / Ds[I - 1] * ((fc - fn) / Di[I] - (fn - phi[nn]) / Di[I - 1]);
+  else
+    phiN = 0.5 * (fn + fc) - 0.125 * Di[I] * Di[I] / Ds[I] * ((fp - fc) / Di[I + 1] - (fc - fn) / Di[I]);
+}
+
+// QuickUnitB (inc/source/convection_and_difussion.hpp:53-84)
+__device__ inline void quickB(double Up, double Un, int64_t pp, int64_t p, int64_t ic, int64_t n, int64_t nn,
+                              const double *__restrict__ Di, const double *__restrict__ Ds, int I,
+                              const double *__restrict__ phi, double &phiP, double &phiN) {
+  const double fc = phi[ic], fp = phi[p], fn = phi[n];
+  if (Up > 0.0)
+    phiP = 0.5 * (fc + fp) - 0.125 * Ds[I] * Ds[I] / Di[I] * ((fp - fc) / Ds[I] - (fc - fn) / Ds[I - 1]);
+  else
+    phiP = 0.5 * (fc + fp) -
+           0.125 * Ds[I] * Ds[I] / Di[I + 1] * ((phi[pp] - fp) / Ds[I + 1] - (fp - fc) / Ds[I]);
+  if (Un > 0.0)
+    phiN = 0.5 * (fn + fc) -
+           0.125 * Ds[I - 1] * Ds[I - 1] / Di[I - 1] * ((fc - fn) / Ds[I - 1] - (fn - phi[nn]) / Ds[I - 2]);
+  else
+    phiN = 0.5 * (fn + fc) - 0.125 * Ds[I - 1] * Ds[I - 1] / Di[I] * ((fp - fc) / Ds[I] - (fc - fn) / Ds[I - 1]);
+}
+
+// src_mode: 0 none, 1 add src[icelCal] at the cell, 2 reference index
+// mapping (src[m] added at ghosted m = icel when m < N)
+__global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, double dt, const double *__restrict__ u,
+                                                        const double *__restrict__ v, const double *__restrict__ w,
+                                                        double *__restrict__ u1, double *__restrict__ v1,
+                                                        double *__restrict__ w1, const double *__restrict__ src,
+                                                        int src_mode) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  const Cell c = decode_interior(g, t);
+  const int i = c.i, j = c.j, k = c.k;
+  const int64_t ic = icel(g, i, j, k);
+  const int64_t sx = (int64_t)g.ny * g.nz, sy = g.nz, sz = 1;
+  const int64_t xm = ic - sx, xp = ic + sx, ym = ic - sy, yp = ic + sy, zm = ic - sz, zp = ic + sz;
+  const int64_t xmm = ic - 2 * sx, xpp = ic + 2 * sx, ymm = ic - 2 * sy, ypp = ic + 2 * sy, zmm = ic - 2, zpp = ic + 2;
+  const double *__restrict__ Dx = g.Dx, *__restrict__ Dy = g.Dy, *__restrict__ Dz = g.Dz;
+  const double *__restrict__ Dxs = g.Dxs, *__restrict__ Dys = g.Dys, *__restrict__ Dzs = g.Dzs;
+  const double nudt = nu * dt;
+  const int64_t N = g.N;
+
+  if (i < g.nx - GC - 1) {  // u (:196-269)
+    const double IT = (Dxs[i] * 0.5) / Dx[i];
+    const double uc = u[ic];
+    const double Up = 0.5 * (u[xp] + uc);
+    const double Un = 0.5 * (u[xm] + uc);
+    const double Vp = v[ic] + (v[xp] - v[ic]) * IT;
+    const double Vn = v[ym] + (v[ym + sx] - v[ym]) * IT;
+    const double Wp = w[ic] + (w[xp] - w[ic]) * IT;
+    const double Wn = w[zm] + (w[zm + sx] - w[zm]) * IT;
+    double uPx, uNx, uPy, uNy, uPz, uNz;
+    quickA(Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, u, uPx, uNx);
+    quickB(Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, u, uPy, uNy);
+    quickB(Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, u, uPz, uNz);
+    const double conv =
+        -dt * ((uPx * uPx - uNx * uNx) / Dxs[i] + (uPy * Vp - uNy * Vn) / Dy[j] + (uPz * Wp - uNz * Wn) / Dz[k]);
+    const double diff = nudt * (((u[xp] - uc) / Dx[i + 1] - (uc - u[xm]) / Dx[i]) / Dxs[i] +
+                                ((u[yp] - uc) / Dys[j] - (uc - u[ym]) / Dys[j - 1]) / Dy[j] +
+                                ((u[zp] - uc) / Dzs[k] - (uc - u[zm]) / Dzs[k - 1]) / Dz[k]);
+    double r = uc + conv + diff;
+    if (src_mode == 1) r += src[t];
+    else if (src_mode == 2 && ic < N) r += src[ic];
+    u1[ic] = r;
+  }
+  if (j < g.ny - GC - 1) {  // v (:278-351)
+    const double IT = (Dys[j] * 0.5) / Dy[j];
+    const double vc = v[ic];
+    const double Vp = 0.5 * (v[yp] + vc);
+    const double Vn = 0.5 * (v[ym] + vc);
+    const double Up = u[ic] + (u[yp] - u[ic]) * IT;
+    const double Un = u[xm] + (u[xm + sy] - u[xm]) * IT;
+    const double Wp = w[ic] + (w[yp] - w[ic]) * IT;
+    const double Wn = w[zm] + (w[zm + sy] - w[zm]) * IT;
+    double vPx, vNx, vPy, vNy, vPz, vNz;
+    quickB(Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, v, vPx, vNx);
+    quickA(Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, v, vPy, vNy);
+    quickB(Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, v, vPz, vNz);
+    const double conv =
+        -dt * ((Up * vPx - Un * vNx) / Dx[i] + (vPy * vPy - vNy * vNy) / Dys[j] + (Wp * vPz - Wn * vNz) / Dz[k]);
+    const double diff = nudt * (((v[xp] - vc) / Dxs[i] - (vc - v[xm]) / Dxs[i - 1]) / Dx[i] +
+                                ((v[yp] - vc) / Dy[j + 1] - (vc - v[ym]) / Dy[j]) / Dys[j] +
+                                ((v[zp] - vc) / Dzs[k] - (vc - v[zm]) / Dzs[k - 1]) / Dz[k]);
+    double r = vc + conv + diff;
+    if (src_mode == 1) r += src[N + t];
+    else if (src_mode == 2 && ic < N) r += src[N + ic];
+    v1[ic] = r;
+  }
+  if (k < g.nz - GC - 1) {  // w (:360-434)
+    const double IT = (Dzs[k] * 0.5) / Dz[k];
+    const double wc = w[ic];
+    const double Wp = 0.5 * (w[zp] + wc);
+    const double Wn = 0.5 * (w[zm] + wc);
+    const double Up = u[ic] + (u[zp] - u[ic]) * IT;
+    const double Un = u[xm] + (u[xm + 1] - u[xm]) * IT;
+    const double Vp = v[ic] + (v[zp] - v[ic]) * IT;
+    const double Vn = v[ym] + (v[ym + 1] - v[ym]) * IT;
+    double wPx, wNx, wPy, wNy, wPz, wNz;
+    quickB(Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, w, wPx, wNx);
+    quickB(Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, w, wPy, wNy);
+    quickA(Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, w, wPz, wNz);
+    const double conv =
+        -dt * ((Up * wPx - Un * wNx) / Dx[i] + (Vp * wPy - Vn * wNy) / Dy[j] + (wPz * wPz - wNz * wNz) / Dzs[k]);
+    const double diff = nudt * (((w[xp] - wc) / Dxs[i] - (wc - w[xm]) / Dxs[i - 1]) / Dx[i] +
+                                ((w[yp] - wc) / Dys[j] - (wc - w[ym]) / Dys[j - 1]) / Dy[j] +
+                                ((w[zp] - wc) / Dz[k + 1] - (wc - w[zm]) / Dz[k]) / Dzs[k]);
+    double r = wc + conv + diff;
+    if (src_mode == 1) r += src[2 * N + t];
+    else if (src_mode == 2 && ic < N) r += src[2 * N + ic];
+    w1[ic] = r;
+  }
+}
+
+// T1 += source_termF at positions the predictor did not write (reference
+// index mapping): ghosted m in [0, N) outside each component's update range.
+__global__ void __launch_bounds__(256) source_quirk_fixup_kernel(DevGrid g, double *__restrict__ u1,
+                                                                 double *__restrict__ v1, double *__restrict__ w1,
+                                                                 const double *__restrict__ src) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= g.N) return;
+  const int64_t nzny = (int64_t)g.ny * g.nz;
+  const int i = (int)(m / nzny);
+  const int j = (int)((m / g.nz) % g.ny);
+  const int k = (int)(m % g.nz);
+  const bool in_i = i >= GC && i < g.nx - GC, in_j = j >= GC && j < g.ny - GC, in_k = k >= GC && k < g.nz - GC;
+  const bool cd_u = in_j && in_k && i >= GC && i < g.nx - GC - 1;
+  const bool cd_v = in_i && in_k && j >= GC && j < g.ny - GC - 1;
+  const bool cd_w = in_i && in_j && k >= GC && k < g.nz - GC - 1;
+  if (!cd_u) u1[m] += src[m];
+  if (!cd_v) v1[m] += src[g.N + m];
+  if (!cd_w) w1[m] += src[2 * g.N + m];
+}
+
+// T1 += source_termF over all interior cells (RUN_CPU_PC_METHOD.hpp:309-316)
+__global__ void __launch_bounds__(256) source_add_kernel(DevGrid g, double *__restrict__ u1, double *__restrict__ v1,
+                                                         double *__restrict__ w1, const double *__restrict__ src,
+                                                         int quirk) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  int64_t m = t;
+  if (!quirk) {
+    const Cell c = decode_interior(g, t);
+    m = icel(g, c.i, c.j, c.k);
+  }
+  u1[m] += src[t];
+  v1[m] += src[g.N + t];
+  w1[m] += src[2 * g.N + t];
+}
+
+// createBMatrix_omp, JACOBI_PC branch (inc/matrix/b_matrix.hpp:51-80)
+__global__ void __launch_bounds__(256) rhs_kernel(DevGrid g, double ddt, const double *__restrict__ u1,
+                                                  const double *__restrict__ v1, const double *__restrict__ w1,
+                                                  const double *__restrict__ jp, double *__restrict__ b) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  const Cell c = decode_interior(g, t);
+  const int64_t ic = icel(g, c.i, c.j, c.k);
+  const int64_t sx = (int64_t)g.ny * g.nz;
+  b[t] = -jp[t] * ddt *
+         ((u1[ic] - u1[ic - sx]) / g.Dx[c.i] + (v1[ic] - v1[ic - g.nz]) / g.Dy[c.j] +
+          (w1[ic] - w1[ic - 1]) / g.Dz[c.k]);
+}
+
+// Pressure_transform_X_result (inc/pressure/Transform.hpp:8-24)
+__global__ void __launch_bounds__(256) scatter_kernel(DevGrid g, const double *__restrict__ X, double *__restrict__ p) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  const Cell c = decode_interior(g, t);
+  p[icel(g, c.i, c.j, c.k)] = X[t];
+}
+
+// update_UandF_omp (inc/dfib/updateUandF.hpp:95-192) with the phases that
+// follow it in a pass fused in: source accumulation (RUN_CPU_PC_METHOD.hpp:
+// 393-401), drag partial sums (:403-435), the steady-state max difference
+// (CheckSteadystate.hpp:165-219) and, in the corrector, the write of the new
+// velocity straight into T0 (the T0 <- T3 copy, physicalVariables.hpp:84-102).
+__global__ void __launch_bounds__(256) dfib_update_kernel(DevGrid g, double dt, int mode,
+                                                          const double *__restrict__ eta,
+                                                          const double *__restrict__ p,
+                                                          const double *__restrict__ u1,
+                                                          const double *__restrict__ v1,
+                                                          const double *__restrict__ w1, double *__restrict__ u3,
+                                                          double *__restrict__ v3, double *__restrict__ w3,
+                                                          double *__restrict__ u0, double *__restrict__ v0,
+                                                          double *__restrict__ w0, double *__restrict__ f,
+                                                          double *__restrict__ src, double *part_drag,
+                                                          double *part_max) {
+  double drag[2] = {0.0, 0.0};
+  double mx[3] = {0.0, 0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < g.N; t += stride) {
+    const Cell c = decode_interior(g, t);
+    const int i = c.i, j = c.j, k = c.k;
+    const int64_t ic = icel(g, i, j, k);
+    const int64_t sx = (int64_t)g.ny * g.nz, sy = g.nz;
+    const int64_t N = g.N;
+    const bool from_x = mode & UPD_P_FROM_X;
+    // pressure at the cell and its +x/+y/+z neighbours
+    const double pc = from_x ? p[t] : p[ic];
+    const double eC = eta[ic];
+    const double comp_in[3] = {u1[ic], v1[ic], w1[ic]};
+    double *const outs3[3] = {u3, v3, w3};
+    double *const outs0[3] = {u0, v0, w0};
+    const bool active[3] = {i < g.nx - GC - 1, j < g.ny - GC - 1, k < g.nz - GC - 1};
+    const int64_t step_ghost[3] = {sx, sy, 1};
+    const int64_t step_int[3] = {(int64_t)g.nyc * g.nzc, g.nzc, 1};
+    const double *Ds[3] = {g.Dxs, g.Dys, g.Dzs};
+    const double *Dd[3] = {g.Dx, g.Dy, g.Dz};
+    const int idx[3] = {i, j, k};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (active[q]) {
+        const double pP = from_x ? p[t + step_int[q]] : p[ic + step_ghost[q]];
+        const double ds = Ds[q][idx[q]];
+        const double T2 = comp_in[q] - dt * (pP - pc) / ds;
+        const double e = eC + (eta[ic + step_ghost[q]] - eC) * ((ds * 0.5) / Dd[q][idx[q]]);
+        const double nv = eC * 0.0 + (1.0 - e) * T2;
+        const double fv = (nv - T2) / dt;
+        f[q * N + t] = fv;
+        if (mode & UPD_SRC_SET) src[q * N + t] = dt * fv;
+        else if (mode & UPD_SRC_ADD) src[q * N + t] += dt * fv;
+        if (mode & UPD_MAXDIFF) mx[q] = fmax(mx[q], outs0[q][ic] - nv);
+        if (mode & UPD_WRITE_T3) outs3[q][ic] = nv;
+        if (mode & UPD_WRITE_T0) outs0[q][ic] = nv;
+        if (q < 2) drag[q] += fv * (g.Dx[i] * g.Dy[j] * g.Dz[k]);
+      } else {
+        // last face: the update never writes T3 here, and the reference's
+        // T0 <- T3 copy carries that value into T0, where the x- and y-face
+        // ghost fills of BC_staggered_main read it before the face's own BC
+        // resets it
+        if (mode & UPD_MAXDIFF) mx[q] = fmax(mx[q], outs0[q][ic] - outs3[q][ic]);
+        if (mode & UPD_WRITE_T0) outs0[q][ic] = outs3[q][ic];
+      }
+    }
+  }
+  if (mode & UPD_DRAG) block_partials<2>(drag, part_drag);
+  if (mode & UPD_MAXDIFF) block_partials<3, true>(mx, part_max);
+}
+
+// source_termF += dt * f (RUN_CPU_PC_METHOD.hpp:393-401)
+__global__ void __launch_bounds__(256) source_accumulate_kernel(int64_t n3, double dt, const double *__restrict__ f,
+                                                                double *__restrict__ src) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n3) src[t] += dt * f[t];
+}
+
+// Boundary faces normal to x (faces 0 and 1) of BC_staggered_main
+// (Boundary_Condition.hpp:72-93, 148-154) and BC_staggered_copy
+// (updateBC.hpp:32-54); one thread per (j, k).
+struct BCArgs {
+  double dir[6][3], num[6][3];
+  int pd[6];
+};
+
+__device__ inline double nrm(const BCArgs &a, int b, int c, double x) { return a.dir[b][c] + a.num[b][c] * x; }
+__device__ inline double tan_(const BCArgs &a, int b, int c, double x) {
+  return a.dir[b][c] * 2. + (2.0 * a.num[b][c] - 1.0) * x;
+}
+
+__global__ void __launch_bounds__(256) bc_x_kernel(DevGrid g, BCArgs a, int do_main, int do_copy, int do_p,
+                                                   double *__restrict__ u, double *__restrict__ v,
+                                                   double *__restrict__ w, double *__restrict__ p,
+                                                   double *__restrict__ u1, double *__restrict__ v1,
+                                                   double *__restrict__ w1) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nj = g.nyc, nk = g.nzc;
+  if (t >= nj * nk) return;
+  const int j = (int)(t / nk) + GC, k = (int)(t % nk) + GC;
+  const int nx = g.nx;
+  const int64_t m0 = icel(g, 0, j, k), m1 = icel(g, 1, j, k), m2 = icel(g, 2, j, k);
+  const int64_t n1 = icel(g, nx - 1, j, k), n2 = icel(g, nx - 2, j, k), n3 = icel(g, nx - 3, j, k),
+                n4 = icel(g, nx - 4, j, k);
+  if (do_main) {
+    double x;
+    x = nrm(a, 0, 0, u[m2]); u[m1] = x; u[m0] = x;
+    x = tan_(a, 0, 1, v[m2]); v[m1] = x; v[m0] = x;
+    x = tan_(a, 0, 2, w[m2]); w[m1] = x; w[m0] = x;
+    x = nrm(a, 1, 0, u[n4]); u[n3] = x; u[n2] = x; u[n1] = x;
+    x = tan_(a, 1, 1, v[n3]); v[n2] = x; v[n1] = x;
+    x = tan_(a, 1, 2, w[n3]); w[n2] = x; w[n1] = x;
+  }
+  if (do_p) {
+    const double lo = a.pd[0] ? 0.0 : p[m2], hi = a.pd[1] ? 0.0 : p[n3];
+    p[m1] = lo; p[m0] = lo; p[n2] = hi; p[n1] = hi;
+  }
+  if (do_copy) {
+    const int64_t ms[4] = {m0, m1, n2, n1};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { u1[ms[q]] = u[ms[q]]; v1[ms[q]] = v[ms[q]]; w1[ms[q]] = w[ms[q]]; }
+    u1[n3] = u[n3];
+  }
+}
+
+__global__ void __launch_bounds__(256) bc_y_kernel(DevGrid g, BCArgs a, int do_main, int do_copy, int do_p,
+                                                   double *__restrict__ u, double *__restrict__ v,
+                                                   double *__restrict__ w, double *__restrict__ p,
+                                                   double *__restrict__ u1, double *__restrict__ v1,
+                                                   double *__restrict__ w1) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t ni = g.nxc, nk = g.nzc;
+  if (t >= ni * nk) return;
+  const int i = (int)(t / nk) + GC, k = (int)(t % nk) + GC;
+  const int ny = g.ny;
+  const int64_t m0 = icel(g, i, 0, k), m1 = icel(g, i, 1, k), m2 = icel(g, i, 2, k);
+  const int64_t n1 = icel(g, i, ny - 1, k), n2 = icel(g, i, ny - 2, k), n3 = icel(g, i, ny - 3, k),
+                n4 = icel(g, i, ny - 4, k);
+  if (do_main) {
+    double x;
+    x = tan_(a, 2, 0, u[m2]); u[m1] = x; u[m0] = x;
+    x = nrm(a, 2, 1, v[m2]); v[m1] = x; v[m0] = x;
+    x = tan_(a, 2, 2, w[m2]); w[m1] = x; w[m0] = x;
+    x = nrm(a, 3, 1, v[n4]); v[n3] = x; v[n2] = x; v[n1] = x;
+    x = tan_(a, 3, 0, u[n3]); u[n2] = x; u[n1] = x;
+    x = tan_(a, 3, 2, w[n3]); w[n2] = x; w[n1] = x;
+  }
+  if (do_p) {
+    const double lo = a.pd[2] ? 0.0 : p[m2], hi = a.pd[3] ? 0.0 : p[n3];
+    p[m1] = lo; p[m0] = lo; p[n2] = hi; p[n1] = hi;
+  }
+  if (do_copy) {
+    const int64_t ms[4] = {m0, m1, n2, n1};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { u1[ms[q]] = u[ms[q]]; v1[ms[q]] = v[ms[q]]; w1[ms[q]] = w[ms[q]]; }
+    v1[n3] = v[n3];
+  }
+}
+
+__global__ void __launch_bounds__(256) bc_z_kernel(DevGrid g, BCArgs a, int do_main, int do_copy, int do_p,
+                                                   double *__restrict__ u, double *__restrict__ v,
+                                                   double *__restrict__ w, double *__restrict__ p,
+                                                   double *__restrict__ u1, double *__restrict__ v1,
+                                                   double *__restrict__ w1) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t ni = g.nxc, nj = g.nyc;
+  if (t >= ni * nj) return;
+  const int i = (int)(t / nj) + GC, j = (int)(t % nj) + GC;
+  const int nz = g.nz;
+  const int64_t m0 = icel(g, i, j, 0), m1 = icel(g, i, j, 1), m2 = icel(g, i, j, 2);
+  const int64_t n1 = icel(g, i, j, nz - 1), n2 = icel(g, i, j, nz - 2), n3 = icel(g, i, j, nz - 3),
+                n4 = icel(g, i, j, nz - 4);
+  if (do_main) {
+    double x;
+    x = tan_(a, 4, 0, u[m2]); u[m1] = x; u[m0] = x;
+    x = tan_(a, 4, 1, v[m2]); v[m1] = x; v[m0] = x;
+    x = nrm(a, 4, 2, w[m2]); w[m1] = x; w[m0] = x;
+    x = nrm(a, 5, 2, w[n4]); w[n3] = x; w[n2] = x; w[n1] = x;
+    x = tan_(a, 5, 0, u[n3]); u[n2] = x; u[n1] = x;
+    x = tan_(a, 5, 1, v[n3]); v[n2] = x; v[n1] = x;
+  }
+  if (do_p) {
+    const double lo = a.pd[4] ? 0.0 : p[m2], hi = a.pd[5] ? 0.0 : p[n3];
+    p[m1] = lo; p[m0] = lo; p[n2] = hi; p[n1] = hi;
+  }
+  if (do_copy) {
+    const int64_t ms[4] = {m0, m1, n2, n1};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { u1[ms[q]] = u[ms[q]]; v1[ms[q]] = v[ms[q]]; w1[ms[q]] = w[ms[q]]; }
+    w1[n3] = w[n3];
+  }
+}
+
+// DFIB_CylinderZ (inc/dfib/setEta.hpp:136-204), plane k = 0; one thread per
+// (i, j).  The multiply-adds the reference build contracts are written as fma.
+__global__ void __launch_bounds__(64) eta_plane_kernel(DevGrid g, double cx, double cy, double R, int nsub,
+                                                       double inv_n2, double *__restrict__ eta) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t ni = g.nxc, nj = g.nyc;
+  if (t >= ni * nj) return;
+  const int i = (int)(t / nj) + GC, j = (int)(t % nj) + GC;
+  const double Dxi = g.Dx[i], Dyj = g.Dy[j], Xi = g.X[i], Yj = g.Y[j];
+  const double Diagonal = sqrt(fma(Dxi, Dxi, Dyj * Dyj)) * 0.5;
+  const double ax = ((Xi + g.X[i + 1]) * 0.5) - cx;
+  const double ay = ((Yj + g.Y[j + 1]) * 0.5) - cy;
+  const double Distance = sqrt(fma(ax, ax, ay * ay));
+  double e;
+  if (fabs(Distance - R) < Diagonal) {
+    const double dyg = Dyj / (double)nsub, dxg = Dxi / (double)nsub;
+    long long xi = 0;
+    for (int ii = 0; ii < nsub; ++ii) {
+      const double sx0 = fma((double)ii, dxg, Xi), sx1 = fma((double)(ii + 1), dxg, Xi);
+      const double bx = (sx0 + sx1) * 0.5 - cx;
+      const double bx2 = bx * bx;
+      for (int jj = 0; jj < nsub; ++jj) {
+        const double sy0 = fma((double)jj, dyg, Yj), sy1 = fma((double)(jj + 1), dyg, Yj);
+        const double by = (sy0 + sy1) * 0.5 - cy;
+        const double Sdist = sqrt(fma(bx, bx, by * by));
+        xi += (Sdist <= R) ? 1 : 0;
+      }
+      (void)bx2;
+    }
+    e = (double)xi * inv_n2;
+  } else if (Distance > R) {
+    e = 0.0;
+  } else {
+    e = 1.0;
+  }
+  eta[icel(g, i, j, 0)] = e;
+}
+
+// eta(i, j, k) = eta(i, j, 0) for i < nx-1, j < ny-1, 1 <= k < nz-1 (setEta.hpp:200-203)
+__global__ void __launch_bounds__(256) eta_extrude_kernel(DevGrid g, double *__restrict__ eta) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 2);
+  if (t >= n) return;
+  const int64_t nk = g.nz - 2, nj = g.ny - 1;
+  const int k = (int)(t % nk) + 1;
+  const int j = (int)((t / nk) % nj);
+  const int i = (int)(t / (nk * nj));
+  eta[icel(g, i, j, k)] = eta[icel(g, i, j, 0)];
+}
+
+__device__ inline uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+// iniU (physicalVariables.hpp:53-75) plus seeded interior noise; T1 = T3 = T0
+__global__ void __launch_bounds__(256) init_vel_kernel(DevGrid g, double u0, double v0, double w0, double amp,
+                                                       uint64_t seed, double *__restrict__ U0, double *__restrict__ V0,
+                                                       double *__restrict__ W0, double *__restrict__ U1,
+                                                       double *__restrict__ V1, double *__restrict__ W1,
+                                                       double *__restrict__ U3, double *__restrict__ V3,
+                                                       double *__restrict__ W3) {
+#pragma clang fp contract(off)  // noise = amp * (2u - 1) rounded, then added: identical on every host
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= g.G) return;
+  double val[3] = {u0, v0, w0};
+  if (amp != 0.0) {
+    const int64_t nzny = (int64_t)g.ny * g.nz;
+    const int i = (int)(m / nzny), j = (int)((m / g.nz) % g.ny), k = (int)(m % g.nz);
+    if (i >= GC && i < g.nx - GC && j >= GC && j < g.ny - GC && k >= GC && k < g.nz - GC) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const uint64_t z = splitmix64(seed ^ ((uint64_t)c << 56) ^ (uint64_t)m);
+        const double u01 = (double)(z >> 11) * 0x1.0p-53;
+        val[c] += amp * (2.0 * u01 - 1.0);
+      }
+    }
+  }
+  U0[m] = val[0]; V0[m] = val[1]; W0[m] = val[2];
+  U1[m] = val[0]; V1[m] = val[1]; W1[m] = val[2];
+  U3[m] = val[0]; V3[m] = val[1]; W3[m] = val[2];
+}
+
+__global__ void __launch_bounds__(256) copy_kernel(const double *__restrict__ a, double *__restrict__ b, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = a[t];
+}
+
+__global__ void __launch_bounds__(256) fill_kernel(double *__restrict__ a, double v, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+
+// standalone drag / max-difference reductions
+__global__ void __launch_bounds__(256) drag_kernel(DevGrid g, const double *__restrict__ f, double *part) {
+  double v[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < g.N; t += stride) {
+    const Cell c = decode_interior(g, t);
+    const double dv = g.Dx[c.i] * g.Dy[c.j] * g.Dz[c.k];
+    v[0] += f[t] * dv;
+    v[1] += f[g.N + t] * dv;
+  }
+  block_partials<2>(v, part);
+}
+
+__global__ void __launch_bounds__(256) maxdiff_kernel(DevGrid g, const double *__restrict__ u0,
+                                                      const double *__restrict__ v0, const double *__restrict__ w0,
+                                                      const double *__restrict__ u3, const double *__restrict__ v3,
+                                                      const double *__restrict__ w3, double *part) {
+  double v[3] = {0.0, 0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < g.N; t += stride) {
+    const Cell c = decode_interior(g, t);
+    const int64_t ic = icel(g, c.i, c.j, c.k);
+    v[0] = fmax(v[0], u0[ic] - u3[ic]);
+    v[1] = fmax(v[1], v0[ic] - v3[ic]);
+    v[2] = fmax(v[2], w0[ic] - w3[ic]);
+  }
+  block_partials<3, true>(v, part);
+}
+
+template <int NV, bool MAX>
+__global__ void __launch_bounds__(256) finalize_kernel(const double *part, int nblk, double *out) {
+  double tot[NV];
+  combine_partials<NV, MAX>(part, nblk, tot);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NV; ++q) out[q] = tot[q];
+}
+
+BCArgs bc_args(const Ctx &c) {
+  BCArgs a;
+  for (int b = 0; b < 6; ++b) {
+    for (int q = 0; q < 3; ++q) {
+      a.dir[b][q] = c.prm.bc_dir[b][q];
+      a.num[b][q] = c.prm.bc_num[b][q];
+    }
+    a.pd[b] = c.prm.p_dirichlet[b];
+  }
+  return a;
+}
+
+}  // namespace
+
+hipError_t launch_init_fields(Ctx &c, double noise_amp, uint64_t seed) {
+  const DevGrid &g = c.g;
+  init_vel_kernel<<<blocks_for(g.G, 256, 1 << 30), 256, 0, c.stream>>>(
+      g, c.prm.u_init[0], c.prm.u_init[1], c.prm.u_init[2], noise_amp, seed, c.f[CFD_T0_U], c.f[CFD_T0_V],
+      c.f[CFD_T0_W], c.f[CFD_T1_U], c.f[CFD_T1_V], c.f[CFD_T1_W], c.f[CFD_T3_U], c.f[CFD_T3_V], c.f[CFD_T3_W]);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if ((e = launch_fill(c, c.f[CFD_P], 0.0, g.G)) != hipSuccess) return e;
+  if ((e = launch_fill(c, c.f[CFD_X], 0.0, g.N)) != hipSuccess) return e;
+  if ((e = launch_fill(c, c.f[CFD_F], 0.0, 3 * g.N)) != hipSuccess) return e;
+  if ((e = launch_fill(c, c.f[CFD_SRC], 0.0, 3 * g.N)) != hipSuccess) return e;
+  return launch_bc(c, true, true, true);
+}
+
+hipError_t launch_eta_cylinder_z(Ctx &c, double inv_n2) {
+  const DevGrid &g = c.g;
+  hipError_t e = launch_fill(c, c.f[CFD_ETA], 0.0, g.G);
+  if (e != hipSuccess) return e;
+  if (c.prm.ib_on) {
+    const int64_t n = (int64_t)g.nxc * g.nyc;
+    eta_plane_kernel<<<blocks_for(n, 64, 1 << 30), 64, 0, c.stream>>>(g, c.prm.ib_cx, c.prm.ib_cy, c.prm.ib_r,
+                                                                      c.prm.ib_nsub, inv_n2, c.f[CFD_ETA]);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int64_t m = (int64_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 2);
+    eta_extrude_kernel<<<blocks_for(m, 256, 1 << 30), 256, 0, c.stream>>>(g, c.f[CFD_ETA]);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+hipError_t launch_predictor(Ctx &c, const double *u0, const double *v0, const double *w0, double *u1, double *v1,
+                            double *w1, const double *src, int src_mode) {
+  predictor_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g, 1.0 / c.prm.Re, c.prm.dt, u0, v0, w0,
+                                                                          u1, v1, w1, src, src_mode);
+  return hipGetLastError();
+}
+
+hipError_t launch_source_quirk_fixup(Ctx &c, double *u1, double *v1, double *w1, const double *src) {
+  source_quirk_fixup_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g, u1, v1, w1, src);
+  return hipGetLastError();
+}
+
+hipError_t launch_source_add(Ctx &c, double *u1, double *v1, double *w1, const double *src, int quirk) {
+  source_add_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g, u1, v1, w1, src, quirk);
+  return hipGetLastError();
+}
+
+hipError_t launch_rhs(Ctx &c, const double *u1, const double *v1, const double *w1, const double *jp, double *b) {
+  rhs_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g, 1.0 / c.prm.dt, u1, v1, w1, jp, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_p(Ctx &c, const double *X, double *p) {
+  scatter_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g, X, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_dfib_update_into(Ctx &c, int mode, const double *p_or_x, double *drag_out, double *max_out) {
+  if (!drag_out) mode &= ~UPD_DRAG;
+  if (!max_out) mode &= ~UPD_MAXDIFF;
+  const int nblk = (mode & (UPD_DRAG | UPD_MAXDIFF)) ? blocks_for(c.g.N, 256, RED_MAX_BLOCKS)
+                                                     : blocks_for(c.g.N, 256, 1 << 30);
+  dfib_update_kernel<<<nblk, 256, 0, c.stream>>>(
+      c.g, c.prm.dt, mode, c.f[CFD_ETA], p_or_x, c.f[CFD_T1_U], c.f[CFD_T1_V], c.f[CFD_T1_W], c.f[CFD_T3_U],
+      c.f[CFD_T3_V], c.f[CFD_T3_W], c.f[CFD_T0_U], c.f[CFD_T0_V], c.f[CFD_T0_W], c.f[CFD_F], c.f[CFD_SRC],
+      red_part_host(c, RS_DRAG), red_part_host(c, RS_MAX));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (mode & UPD_DRAG) {
+    finalize_kernel<2, false><<<1, 256, 0, c.stream>>>(red_part_host(c, RS_DRAG), nblk, drag_out);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  if (mode & UPD_MAXDIFF) {
+    finalize_kernel<3, true><<<1, 256, 0, c.stream>>>(red_part_host(c, RS_MAX), nblk, max_out);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+hipError_t launch_source_accumulate(Ctx &c) {
+  const int64_t n3 = 3 * c.g.N;
+  source_accumulate_kernel<<<blocks_for(n3, 256, 1 << 30), 256, 0, c.stream>>>(n3, c.prm.dt, c.f[CFD_F],
+                                                                               c.f[CFD_SRC]);
+  return hipGetLastError();
+}
+
+hipError_t launch_bc(Ctx &c, bool main, bool copy, bool pressure) {
+  const DevGrid &g = c.g;
+  const BCArgs a = bc_args(c);
+  double *u = c.f[CFD_T0_U], *v = c.f[CFD_T0_V], *w = c.f[CFD_T0_W], *p = c.f[CFD_P];
+  double *u1 = c.f[CFD_T1_U], *v1 = c.f[CFD_T1_V], *w1 = c.f[CFD_T1_W];
+  const int m = main ? 1 : 0, cp = copy ? 1 : 0, pp = pressure ? 1 : 0;
+  bc_x_kernel<<<blocks_for((int64_t)g.nyc * g.nzc, 256, 1 << 30), 256, 0, c.stream>>>(g, a, m, cp, pp, u, v, w, p,
+                                                                                       u1, v1, w1);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  bc_y_kernel<<<blocks_for((int64_t)g.nxc * g.nzc, 256, 1 << 30), 256, 0, c.stream>>>(g, a, m, cp, pp, u, v, w, p,
+                                                                                       u1, v1, w1);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  bc_z_kernel<<<blocks_for((int64_t)g.nxc * g.nyc, 256, 1 << 30), 256, 0, c.stream>>>(g, a, m, cp, pp, u, v, w, p,
+                                                                                       u1, v1, w1);
+  return hipGetLastError();
+}
+
+hipError_t launch_drag(Ctx &c) {
+  const int nblk = blocks_for(c.g.N, 256, RED_MAX_BLOCKS);
+  drag_kernel<<<nblk, 256, 0, c.stream>>>(c.g, c.f[CFD_F], red_part_host(c, RS_DRAG));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  finalize_kernel<2, false><<<1, 256, 0, c.stream>>>(red_part_host(c, RS_DRAG), nblk, c.red.out + RS_DRAG * 4);
+  return hipGetLastError();
+}
+
+hipError_t launch_maxdiff(Ctx &c) {
+  const int nblk = blocks_for(c.g.N, 256, RED_MAX_BLOCKS);
+  maxdiff_kernel<<<nblk, 256, 0, c.stream>>>(c.g, c.f[CFD_T0_U], c.f[CFD_T0_V], c.f[CFD_T0_W], c.f[CFD_T3_U],
+                                             c.f[CFD_T3_V], c.f[CFD_T3_W], red_part_host(c, RS_MAX));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  finalize_kernel<3, true><<<1, 256, 0, c.stream>>>(red_part_host(c, RS_MAX), nblk, c.red.out + RS_MAX * 4);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(Ctx &c, const double *src, double *dst, int64_t n) {
+  copy_kernel<<<blocks_for(n, 256, 1 << 30), 256, 0, c.stream>>>(src, dst, n);
+  return hipGetLastError();
+}
+
+// bandwidth calibration: a grid-stride copy of n doubles (16 B moved per element)
+__global__ void __launch_bounds__(256) stream_copy_kernel(const double *__restrict__ a, double *__restrict__ b,
+                                                          int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(a + t), b + t);
+}
+
+hipError_t launch_copy_kernel_bench(Ctx &c, const double *src, double *dst, int64_t n) {
+  stream_copy_kernel<<<blocks_for(n, 256, 8192), 256, 0, c.stream>>>(src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(Ctx &c, double *dst, double v, int64_t n) {
+  fill_kernel<<<blocks_for(n, 256, 1 << 30), 256, 0, c.stream>>>(dst, v, n);
+  return hipGetLastError();
+}
+
+}  // namespace cfd

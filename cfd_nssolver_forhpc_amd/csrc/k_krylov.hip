@@ -1,0 +1,407 @@
+// k_krylov.hip -- restarted BiCGSTAB (solver::bicgstabRe2,
+// inc/solver/bicgstabRe2.hpp:91-189) and FP32-inner / FP64-outer defect
+// correction on gfx950.
+//
+// One iteration is five streaming kernels, each fusing the reference's
+// BLAS-1 work with the reduction that follows it:
+//   K1  ap = A p,               partial (ap . r0)           -> F1: alpha
+//   K2  s  = r - alpha ap
+//   K3  as = A s,               partials (as . s), (as . as) -> F2: omega
+//   K4  x += alpha p + omega s; r = s - omega as;
+//                               partials (r . r), (r . r0)   -> F3: stop test, beta, restart rule
+//   K5  p  = r + beta (p - omega ap)   (or r0 = p = r on restart)
+// F1-F3 are one-block kernels combining the per-block partials in a fixed
+// order (deterministic results) and updating the solver's scalars in device
+// memory; every kernel returns at once when the device-side `done` flag is
+// set, so the host launches iterations in batches and reads the flag once per
+// batch instead of once per dot product.  FP64 dots are kept for every
+// scalar, also when the vectors are FP32.
+#include <algorithm>
+#include <cmath>
+
+#include "dev_common.h"
+#include "dev_spmv.h"
+
+namespace cfd {
+
+namespace {
+
+template <typename T>
+__device__ inline T op_apply(const OpArgs &a, int64_t r, const T *__restrict__ x);
+template <>
+__device__ inline double op_apply<double>(const OpArgs &a, int64_t r, const double *__restrict__ x) {
+  return op_row(a, r, x);
+}
+template <>
+__device__ inline float op_apply<float>(const OpArgs &a, int64_t r, const float *__restrict__ x) {
+  return op_row32(a, r, x);
+}
+
+__global__ void ks_setup_kernel(KrylovScalars *ks, double tol, double restart_tol, int maxit, int timestep) {
+  ks->tol = tol;
+  ks->restart_tol = restart_tol;
+  ks->maxit = maxit;
+  ks->timestep = timestep;
+  ks->done = 0;
+  ks->hit_maxit = 0;
+  ks->iters = 0;
+  ks->restart = 0;
+}
+
+// r0 = b - A x; r1 = p1 = r0; px2 = px1; px1 = x; partials (b.b), (r0.r0)
+template <typename T>
+__global__ void __launch_bounds__(256) init_kernel(OpArgs a, const T *__restrict__ b, const T *__restrict__ x,
+                                                   T *__restrict__ r0, T *__restrict__ r1, T *__restrict__ p1,
+                                                   double *__restrict__ px1, double *__restrict__ px2,
+                                                   double *part) {
+  double acc[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) {
+    const T ax = op_apply<T>(a, r, x);
+    const T br = b[r];
+    const T rr = br - ax;
+    r0[r] = rr;
+    r1[r] = rr;
+    p1[r] = rr;
+    if (px1) {
+      px2[r] = px1[r];
+      px1[r] = (double)x[r];
+    }
+    acc[0] += (double)br * (double)br;
+    acc[1] += (double)rr * (double)rr;
+  }
+  block_partials<2>(acc, part);
+}
+
+__global__ void __launch_bounds__(256) init_finalize(const double *part, int nblk, KrylovScalars *ks) {
+  double tot[2];
+  combine_partials<2>(part, nblk, tot);
+  if (threadIdx.x == 0) {
+    ks->norm0 = sqrt(tot[0]);
+    ks->rho = tot[1];
+    ks->relres = sqrt(tot[1]) / ks->norm0;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) spmv_dot_kernel(OpArgs a, const KrylovScalars *__restrict__ ks,
+                                                       const T *__restrict__ p1, T *__restrict__ ap,
+                                                       const T *__restrict__ r0, double *part) {
+  if (ks->done) return;
+  double acc[1] = {0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) {
+    const T y = op_apply<T>(a, r, p1);
+    ap[r] = y;
+    acc[0] += (double)y * (double)r0[r];
+  }
+  block_partials<1>(acc, part);
+}
+
+__global__ void __launch_bounds__(256) alpha_finalize(const double *part, int nblk, KrylovScalars *ks) {
+  if (ks->done) return;
+  double tot[1];
+  combine_partials<1>(part, nblk, tot);
+  if (threadIdx.x == 0) ks->alpha = ks->rho / tot[0];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) s_update_kernel(int64_t N, const KrylovScalars *__restrict__ ks,
+                                                       const T *__restrict__ r1, const T *__restrict__ ap,
+                                                       T *__restrict__ s1) {
+  if (ks->done) return;
+  const double alpha = ks->alpha;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += stride)
+    s1[r] = (T)((double)r1[r] - alpha * (double)ap[r]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) spmv_dot2_kernel(OpArgs a, const KrylovScalars *__restrict__ ks,
+                                                        const T *__restrict__ s1, T *__restrict__ as, double *part) {
+  if (ks->done) return;
+  double acc[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) {
+    const T y = op_apply<T>(a, r, s1);
+    as[r] = y;
+    acc[0] += (double)y * (double)s1[r];
+    acc[1] += (double)y * (double)y;
+  }
+  block_partials<2>(acc, part);
+}
+
+__global__ void __launch_bounds__(256) omega_finalize(const double *part, int nblk, KrylovScalars *ks) {
+  if (ks->done) return;
+  double tot[2];
+  combine_partials<2>(part, nblk, tot);
+  if (threadIdx.x == 0) ks->omega = tot[0] / tot[1];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) xr_update_kernel(int64_t N, const KrylovScalars *__restrict__ ks,
+                                                        T *__restrict__ x, const T *__restrict__ p1,
+                                                        const T *__restrict__ s1, const T *__restrict__ as,
+                                                        T *__restrict__ r1, const T *__restrict__ r0, double *part) {
+  if (ks->done) return;
+  const double alpha = ks->alpha, omega = ks->omega;
+  double acc[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += stride) {
+    const double s = (double)s1[r];
+    x[r] = (T)((double)x[r] + (alpha * (double)p1[r] + omega * s));
+    const T rn = (T)(s - omega * (double)as[r]);
+    r1[r] = rn;
+    acc[0] += (double)rn * (double)rn;
+    acc[1] += (double)rn * (double)r0[r];
+  }
+  block_partials<2>(acc, part);
+}
+
+// stop test, beta and the restart rule (bicgstabRe2.hpp:137-183)
+__global__ void __launch_bounds__(256) conv_finalize(const double *part, int nblk, KrylovScalars *ks) {
+  if (ks->done) return;
+  double tot[2];
+  combine_partials<2>(part, nblk, tot);
+  if (threadIdx.x == 0) {
+    ks->rnorm = sqrt(tot[0]);
+    ks->relres = ks->rnorm / ks->norm0;
+    if (ks->relres < ks->tol) {
+      ks->done = 1;
+    } else if (ks->iters >= ks->maxit) {
+      ks->done = 1;
+      ks->hit_maxit = 1;
+    } else {
+      ks->pre_rho = ks->rho;
+      ks->rho = tot[1];
+      ks->beta = (ks->alpha / ks->omega) * (ks->rho / ks->pre_rho);
+      ks->restart = (ks->timestep < 10 || fabs(ks->pre_rho) > ks->restart_tol) ? 0 : 1;
+      ks->restarts += ks->restart;
+      ks->iters += 1;
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) p_update_kernel(int64_t N, const KrylovScalars *__restrict__ ks,
+                                                       const T *__restrict__ r1, T *__restrict__ p1,
+                                                       const T *__restrict__ ap, T *__restrict__ r0) {
+  if (ks->done) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  if (ks->restart) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += stride) {
+      r0[r] = r1[r];
+      p1[r] = r1[r];
+    }
+    return;
+  }
+  const double beta = ks->beta, omega = ks->omega;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += stride)
+    p1[r] = (T)((double)r1[r] + beta * ((double)p1[r] - omega * (double)ap[r]));
+}
+
+// NaN / Inf reset after iters_max (bicgstabRe2.hpp:143-163)
+__global__ void __launch_bounds__(256) nan_reset_kernel(int64_t N, double *__restrict__ x,
+                                                        const double *__restrict__ px1, double *part) {
+  double acc[1] = {0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += stride) {
+    const double v = x[r];
+    if (isnan(v) || isinf(v)) {
+      x[r] = px1[r];
+      acc[0] += 1.0;
+    }
+  }
+  block_partials<1>(acc, part);
+}
+
+template <int NV>
+__global__ void __launch_bounds__(256) sum_finalize(const double *part, int nblk, double *out) {
+  double tot[NV];
+  combine_partials<NV>(part, nblk, tot);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NV; ++q) out[q] = tot[q];
+}
+
+// defect correction: r = b - A64 x in FP64, r32 = (float) r, partials (r.r), (b.b)
+__global__ void __launch_bounds__(256) dc_residual_kernel(OpArgs a, const double *__restrict__ b,
+                                                          const double *__restrict__ x, float *__restrict__ r32,
+                                                          double *part) {
+  double acc[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) {
+    const double br = b[r];
+    const double rr = br - op_row(a, r, x);
+    r32[r] = (float)rr;
+    acc[0] += rr * rr;
+    acc[1] += br * br;
+  }
+  block_partials<2>(acc, part);
+}
+
+__global__ void __launch_bounds__(256) dc_update_kernel(int64_t N, double *__restrict__ x, float *__restrict__ d) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += stride) {
+    x[r] += (double)d[r];
+    d[r] = 0.0f;
+  }
+}
+
+__global__ void __launch_bounds__(256) fill32_kernel(float *__restrict__ a, float v, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+
+template <typename T>
+struct Vecs {
+  T *x, *r0, *r1, *p1, *s1, *ap, *as;
+  const T *b;
+  double *px1, *px2;
+};
+
+template <typename T>
+hipError_t launch_iters(Ctx &c, const OpArgs &a, const Vecs<T> &v, KrylovScalars *ks, int nit) {
+  const int64_t N = c.g.N;
+  const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
+  double *p1 = red_part_host(c, RS_SPMV_DOT), *p2 = red_part_host(c, RS_SPMV_DOT2), *p3 = red_part_host(c, RS_XR);
+  for (int it = 0; it < nit; ++it) {
+    spmv_dot_kernel<T><<<nb, 256, 0, c.stream>>>(a, ks, v.p1, v.ap, v.r0, p1);
+    alpha_finalize<<<1, 256, 0, c.stream>>>(p1, nb, ks);
+    s_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.r1, v.ap, v.s1);
+    spmv_dot2_kernel<T><<<nb, 256, 0, c.stream>>>(a, ks, v.s1, v.as, p2);
+    omega_finalize<<<1, 256, 0, c.stream>>>(p2, nb, ks);
+    xr_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.x, v.p1, v.s1, v.as, v.r1, v.r0, p3);
+    conv_finalize<<<1, 256, 0, c.stream>>>(p3, nb, ks);
+    p_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.r1, v.p1, v.ap, v.r0);
+  }
+  return hipGetLastError();
+}
+
+// Generic driver: set up scalars, r0 = b - A x, then batches of iterations
+// with one host read of the scalars per batch.  The batch length follows
+// the observed convergence rate so that few no-op launches follow the
+// converged iteration.
+template <typename T>
+int solve_bicg(Ctx &c, const OpArgs &a, const Vecs<T> &v, KrylovScalars *ks, int timestep, double tol, int maxit,
+               int *iters, double *relres) {
+  const int64_t N = c.g.N;
+  const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
+  ks_setup_kernel<<<1, 1, 0, c.stream>>>(ks, tol, 1e-7 * tol, maxit, timestep);
+  init_kernel<T><<<nb, 256, 0, c.stream>>>(a, v.b, v.x, v.r0, v.r1, v.p1, v.px1, v.px2, red_part_host(c, RS_INIT));
+  init_finalize<<<1, 256, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, ks);
+  CFD_HIP(hipGetLastError());
+  int batch = 2;
+  double prev_rel = -1.0;
+  int prev_it = 0;
+  for (;;) {
+    CFD_HIP(launch_iters<T>(c, a, v, ks, batch));
+    CFD_HIP(hipMemcpyAsync(c.hks, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c.stream));
+    CFD_HIP(hipStreamSynchronize(c.stream));
+    const KrylovScalars h = *c.hks;
+    if (h.done) break;
+    // predict the iterations still needed from the last batch's rate
+    int next = batch * 2;
+    if (prev_rel > 0.0 && h.relres > 0.0 && h.relres < prev_rel && h.iters > prev_it) {
+      const double rate = std::log(h.relres / prev_rel) / (double)(h.iters - prev_it);
+      const double need = std::log(h.tol / h.relres) / rate;
+      if (need > 0.0 && std::isfinite(need)) next = (int)std::ceil(need * 1.05) + 1;
+    }
+    prev_rel = h.relres;
+    prev_it = h.iters;
+    next = std::max(1, std::min(next, 64));
+    next = std::min(next, maxit - h.iters + 1);
+    batch = std::max(next, 1);
+  }
+  const KrylovScalars h = *c.hks;
+  if (h.hit_maxit && v.px1) {
+    double *part = red_part_host(c, RS_MISC);
+    nan_reset_kernel<<<nb, 256, 0, c.stream>>>(N, (double *)v.x, v.px1, part);
+    sum_finalize<1><<<1, 256, 0, c.stream>>>(part, nb, c.red.out + RS_MISC * 4);
+    CFD_HIP(hipMemcpyAsync(c.hred, c.red.out + RS_MISC * 4, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    CFD_HIP(hipStreamSynchronize(c.stream));
+    if (c.hred[0] > (double)N * 0.1)
+      CFD_HIP(hipMemcpyAsync(v.x, v.px2, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+  }
+  *iters = h.iters;
+  *relres = h.relres;
+  return CFD_OK;
+}
+
+}  // namespace
+
+int bicgstab_solve(Ctx &c, const double *b, double *x, double tol, int maxit, int *iters, double *relres) {
+  if (c.prm.op == CFD_OP_ELL && !c.ell_ready) return fail(CFD_ERR_STATE, "ELL operator not assembled");
+  if (c.prm.op == CFD_OP_SPE && !c.spe_ready) return fail(CFD_ERR_STATE, "SPE operator not assembled");
+  c.solves++;
+  Vecs<double> v{x, c.r0, c.r1, c.p1, c.s1, c.ap, c.as, b, c.px1, c.px2};
+  return solve_bicg<double>(c, op_args(c, c.prm.op), v, c.ks, c.solves, tol, maxit, iters, relres);
+}
+
+// FP64 defect correction around FP32 BiCGSTAB on the FP32 ELL copy:
+// r = b - A x (FP64), solve A32 d = r32 to tol_inner, x += d, until
+// ||b - A x|| / ||b|| < tol.
+int defect_correction(Ctx &c, const double *b, double *x, double tol, double tol_inner, int max_outer, int *outer,
+                      int *inner_total, double *relres) {
+  if (!c.ell_ready || !c.val32) return fail(CFD_ERR_STATE, "FP32 ELL operator not assembled");
+  const int64_t N = c.g.N;
+  const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
+  const OpArgs a64 = op_args(c, c.prm.op == CFD_OP_SPE && c.spe_ready ? CFD_OP_SPE : CFD_OP_ELL);
+  const OpArgs a32 = op_args(c, CFD_OP_ELL);
+  float *r32 = c.q32[7], *d32 = c.q32[6];
+  Vecs<float> v{d32, c.q32[0], c.q32[1], c.q32[2], c.q32[3], c.q32[4], c.q32[5], r32, nullptr, nullptr};
+  fill32_kernel<<<blocks_for(N, 256, 1 << 30), 256, 0, c.stream>>>(d32, 0.0f, N);
+  CFD_HIP(hipGetLastError());
+  int k = 0, inner = 0;
+  double rel = 0.0;
+  c.solves++;
+  for (;;) {
+    double *part = red_part_host(c, RS_NORM);
+    dc_residual_kernel<<<nb, 256, 0, c.stream>>>(a64, b, x, r32, part);
+    sum_finalize<2><<<1, 256, 0, c.stream>>>(part, nb, c.red.out + RS_NORM * 4);
+    CFD_HIP(hipGetLastError());
+    CFD_HIP(hipMemcpyAsync(c.hred, c.red.out + RS_NORM * 4, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    CFD_HIP(hipStreamSynchronize(c.stream));
+    rel = std::sqrt(c.hred[0]) / std::sqrt(c.hred[1]);
+    if (!(rel >= tol) || k >= max_outer) break;
+    int it = 0;
+    double rin = 0.0;
+    c.solves32++;
+    if (c.prm.solver == CFD_SOLVER_RBSOR) {
+      int rc = rbsor_solve_f32(c, r32, d32, c.prm.sor_omega, tol_inner, c.prm.maxit, c.prm.sor_check, &it, &rin);
+      if (rc != CFD_OK) return rc;
+    } else {
+      int rc = solve_bicg<float>(c, a32, v, c.ks32, c.solves32, tol_inner, c.prm.maxit, &it, &rin);
+      if (rc != CFD_OK) return rc;
+    }
+    inner += it + 1;
+    dc_update_kernel<<<nb, 256, 0, c.stream>>>(N, x, d32);
+    CFD_HIP(hipGetLastError());
+    ++k;
+  }
+  *outer = k;
+  *inner_total = inner;
+  *relres = rel;
+  return CFD_OK;
+}
+
+// Timing of one iteration: the solver runs on scratch vectors (x = VEC_B
+// starting from 0) with the stop test disabled; the warm-start history of the
+// real solver (px1, px2, X) is not touched.
+hipError_t launch_bicg_iteration(Ctx &c) {
+  Vecs<double> v{c.f[CFD_VEC_B], c.r0, c.r1, c.p1, c.s1, c.ap, c.as, c.f[CFD_RHS], nullptr, nullptr};
+  return launch_iters<double>(c, op_args(c, c.prm.op), v, c.ks32, 1);
+}
+
+hipError_t launch_bicg_timing_setup(Ctx &c) {
+  const int64_t N = c.g.N;
+  const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
+  hipError_t e = launch_fill(c, c.f[CFD_VEC_B], 0.0, N);
+  if (e != hipSuccess) return e;
+  ks_setup_kernel<<<1, 1, 0, c.stream>>>(c.ks32, 0.0, 0.0, 1 << 30, 100);
+  init_kernel<double><<<nb, 256, 0, c.stream>>>(op_args(c, c.prm.op), c.f[CFD_RHS], c.f[CFD_VEC_B], c.r0, c.r1, c.p1,
+                                                nullptr, nullptr, red_part_host(c, RS_INIT));
+  init_finalize<<<1, 256, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, c.ks32);
+  return hipGetLastError();
+}
+
+}  // namespace cfd

@@ -1,0 +1,245 @@
+/*
+ * cfd_capi.h -- C-ABI of the MI355X (gfx950) hot path of the staggered-grid
+ * incompressible Navier-Stokes solver: QUICK convection-diffusion predictor,
+ * direct-forcing immersed boundary (DFIB) forcing and velocity correction,
+ * pressure right-hand side, ELL / matrix-free ("SPE") pressure operator,
+ * BiCGSTAB and red-black SOR pressure solves, boundary conditions, and the
+ * prediction-correction timestep that sequences them.
+ *
+ * Plain C: no C++ or torch types cross this boundary.  Every entry point
+ * returns an int status (CFD_OK == 0, negative on error); the message of the
+ * last failure on the calling thread is available from cfd_last_error().
+ *
+ * Array layouts follow the reference solver exactly:
+ *   ghosted fields (u, v, w, p, eta): length G = nx*ny*nz with nx = nx_cal+4
+ *     (two ghost layers per side), index icel(i,j,k) = (i*ny + j)*nz + k
+ *     (inc/grid/structureGrid.hpp:159-161); u(i,j,k) lives on the +x face of
+ *     cell (i,j,k), v on the +y face, w on the +z face.
+ *   interior vectors (pressure solution, RHS, jp): length N = nx_cal*ny_cal*
+ *     nz_cal, index icelCal = ((i-2)*ny_cal + (j-2))*nz_cal + (k-2)
+ *     (inc/grid/structureGrid.hpp:182-185).
+ *   DFIB force f and the accumulated source term: 3*N, component-major
+ *     (inc/resize_and_simuSetting.hpp:20, inc/dfib/updateUandF.hpp:112-114).
+ *   ELL matrix: slot-major structure of arrays col[s*N + row] (int32),
+ *     val[s*N + row] (fp64), s in [0, 7); slot 0 is the diagonal, slots 1..6
+ *     hold xm, xp, ym, yp, zm, zp with absent boundary neighbours skipped and
+ *     padding (col = N-1, val = 0) at the end of each row, exactly the slot
+ *     order mat::ELL_matrix<T>::set produces (inc/matrix/PressureMatrix.hpp:71-89,
+ *     inc/matrix/ELL_sparseMatrix.hpp:146-160, 246-320).
+ *
+ * Threading: a context owns one HIP stream and its device buffers; calls on
+ * one context are stream-ordered and must come from one thread at a time.
+ * Calls that return host scalars synchronise the stream.  Distinct contexts
+ * are independent.
+ */
+#ifndef CFD_CAPI_H
+#define CFD_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFD_ABI_VERSION 1
+
+/* status codes */
+#define CFD_OK 0
+#define CFD_ERR_ARG (-1)      /* invalid argument (reference: std::invalid_argument) */
+#define CFD_ERR_HIP (-2)      /* HIP runtime failure */
+#define CFD_ERR_STATE (-3)    /* call out of order (e.g. solve before assembly) */
+#define CFD_ERR_NOMEM (-4)    /* device allocation failed */
+#define CFD_ERR_RUNTIME (-5)  /* reference: std::runtime_error */
+
+/* pressure solvers (inc/controlPanel.hpp:17-29) */
+#define CFD_SOLVER_BICGSTAB 0 /* solver::bicgstabRe2 */
+#define CFD_SOLVER_RBSOR 1    /* red-black SOR on the ELL operator */
+
+/* pressure operators */
+#define CFD_OP_ELL 0 /* mat::ELL_matrix (P_SOLVER_BICG_ELL) */
+#define CFD_OP_SPE 1 /* mat::SPE_matrix1, matrix-free (P_SOLVER_BICG_SPE, the reference default) */
+
+/* arithmetic of the pressure solve */
+#define CFD_FP64 0
+#define CFD_MIXED 1 /* FP32 inner Krylov/SOR solves inside FP64 defect correction */
+
+/* device arrays addressable through upload / download */
+#define CFD_T0_U 0 /* old velocity (ghosted) */
+#define CFD_T0_V 1
+#define CFD_T0_W 2
+#define CFD_T1_U 3 /* predicted velocity u* (ghosted) */
+#define CFD_T1_V 4
+#define CFD_T1_W 5
+#define CFD_T3_U 6 /* corrected velocity (ghosted) */
+#define CFD_T3_V 7
+#define CFD_T3_W 8
+#define CFD_P 9      /* pressure (ghosted) */
+#define CFD_ETA 10   /* solid fraction (ghosted) */
+#define CFD_F 11     /* DFIB force, 3*N */
+#define CFD_SRC 12   /* accumulated source term source_termF, 3*N */
+#define CFD_RHS 13   /* matB, N */
+#define CFD_X 14     /* X_result, N */
+#define CFD_JP 15    /* Jacobi scaling 1/diag, N */
+#define CFD_VEC_A 16 /* scratch vectors for standalone operator calls, N */
+#define CFD_VEC_B 17
+#define CFD_NFIELDS 18
+
+typedef struct cfd_grid_desc {
+  int nx_cal, ny_cal, nz_cal; /* interior cells per axis (grid::nxCal..) */
+  const double *X, *Y, *Z;    /* node coordinates, n+1 entries (grid::X..) */
+  const double *Dx, *Dy, *Dz; /* cell widths, n entries (grid::Dx..) */
+  const double *Dxs, *Dys, *Dzs; /* centre-to-centre distances, n entries (grid::Dxs..) */
+  double lx, ly, lz;          /* domain lengths (grid::lx..) */
+} cfd_grid_desc;
+
+typedef struct cfd_params {
+  double Re;             /* nu = 1/Re (inc/simuClass.hpp:39-44) */
+  double dt;             /* simuClass::dt */
+  double bc_dir[6][3];   /* velocity Dirichlet table Dir[face][component] (Boundary_Condition.hpp:4-31) */
+  double bc_num[6][3];   /* velocity Neumann table Num[face][component] */
+  int p_dirichlet[6];    /* pressure face type: 0 Neumann (P_NEUMANN.hpp), 1 Dirichlet p = 0 */
+  int source_index_quirk;/* 1: source_termF[icelCal] added at T1[icelCal] as in
+                            inc/run/RUN_CPU_PC_METHOD.hpp:314,465; 0: added at T1[icel] */
+  int inner_loops;       /* predictor passes before the corrector: 3 (RUN_CPU_PC_METHOD.hpp:291), 0 (RUN_CPU.hpp) */
+  int solver;            /* CFD_SOLVER_* */
+  int op;                /* CFD_OP_* */
+  int precision;         /* CFD_FP64 / CFD_MIXED */
+  double tol;            /* simuClass::p_criteria; relative residual for BiCGSTAB / RB-SOR */
+  double tol_inner;      /* CFD_MIXED: tolerance of each FP32 inner solve */
+  int maxit;             /* bicgstabRe2::iters_max_ (5000) / RB-SOR sweep cap */
+  double sor_omega;      /* simuClass::p_sor_omega (1.8) */
+  int sor_check;         /* RB-SOR: sweeps between true-residual checks */
+  int ib_on;             /* DFIB on (simuClass::DfibMethod != "OFF") */
+  double ib_cx, ib_cy, ib_r; /* z-cylinder centre and radius (setEta.hpp:145-147) */
+  int ib_nsub;           /* sub-cells per cut-cell edge (setEta.hpp:152) */
+  double u_init[3];      /* initial velocity (RUN_CPU_PC_METHOD.hpp:77) */
+} cfd_params;
+
+/* per-timestep report; pass index 0..inner_loops-1 are predictor passes, the last is the corrector */
+#define CFD_MAX_PASSES 8
+typedef struct cfd_step_stats {
+  int loop;                       /* simuClass::loop of the step just taken (1-based) */
+  int n_passes;
+  int iters[CFD_MAX_PASSES];      /* solver iterations per pass, -1 when no solve (loop == 1) */
+  double relres[CFD_MAX_PASSES];  /* final relative residual per pass */
+  double cd[CFD_MAX_PASSES];      /* -2 sum(f_x dV)/lz per pass (RUN_CPU_PC_METHOD.hpp:403-435) */
+  double cl[CFD_MAX_PASSES];
+  double maxdiff[3];              /* CheckSteadyStatebyMaxVal (CheckSteadystate.hpp:165-219) */
+  int inner_iters[CFD_MAX_PASSES];/* CFD_MIXED: total FP32 inner iterations per pass */
+} cfd_step_stats;
+
+typedef struct cfd_ctx cfd_ctx;
+
+/* ---- library ---------------------------------------------------------- */
+int cfd_abi_version(void);
+const char *cfd_last_error(void);
+/* default parameters: the reference's native settings (src/CFD_MX.cpp:35-57,
+   inc/simuClass.hpp:74-80, BC_Cavity_A.hpp table, all-Neumann pressure) */
+int cfd_params_default(cfd_params *out);
+/* number of visible HIP devices (0 on a host without GPU) */
+int cfd_device_count(int *n);
+
+/* ---- grid generation (host only; one axis each; X has n+1 entries, D and Ds
+   n entries, n = ncal + 4) ------------------------------------------------- */
+/* grid::init_uniform (inc/grid/structureGrid.hpp:409-440) */
+int cfd_axis_uniform(int ncal, double L, double *X, double *D, double *Ds);
+/* grid::unit_SML + init_gC_dx (structureGrid.hpp:326-386, 506-559); the
+   reference's std::invalid_argument cases return CFD_ERR_ARG */
+int cfd_axis_sml(int ncal, double L, double lS, int nS, int off_nS, double *X, double *D, double *Ds);
+/* stretched axis of the large configuration (no reference counterpart):
+   width h0 over a band of width `band` around xc, widths growing by `ratio`
+   per cell outside it, capped so that the widths sum to L; then init_gC_dx */
+int cfd_axis_tanh(int ncal, double L, double xc, double h0, double band, double ratio, double *X, double *D,
+                  double *Ds);
+
+/* ---- context ---------------------------------------------------------- */
+/* Allocates device buffers for the grid; replaces the reference's
+   resize_variable (inc/resize_and_simuSetting.hpp:5-31) and grid metric upload. */
+int cfd_create(const cfd_grid_desc *grid, const cfd_params *params, int device, cfd_ctx **out);
+int cfd_destroy(cfd_ctx *ctx);
+int cfd_set_params(cfd_ctx *ctx, const cfd_params *params);
+int cfd_sizes(const cfd_ctx *ctx, size_t *G, size_t *N);
+
+/* host <-> device copies of the arrays named by CFD_* field ids; len is in elements */
+int cfd_upload(cfd_ctx *ctx, int field, const double *host, size_t len);
+int cfd_download(cfd_ctx *ctx, int field, double *host, size_t len);
+
+/* ---- set-up ----------------------------------------------------------- */
+/* T0 = T1 = T3 = u_init (+ seeded uniform noise of amplitude noise_amp on
+   interior cells), p = 0, X = 0, then BC_staggered_main + BC_staggered_copy
+   (RUN_CPU_PC_METHOD.hpp:77-119). noise_amp = 0 reproduces the reference. */
+int cfd_init_state(cfd_ctx *ctx, double noise_amp, uint64_t seed);
+/* DFIB_CylinderZ (inc/dfib/setEta.hpp:136-204); eta = 0 everywhere when ib_on == 0 */
+int cfd_set_eta_cylinder_z(cfd_ctx *ctx);
+/* createPressureMatrix (inc/matrix/PressureMatrix.hpp:5-117): ELL values, jp,
+   plus the FP32 value copy when precision == CFD_MIXED */
+int cfd_ell_assemble(cfd_ctx *ctx);
+/* SPE_matrix1::setupPressure + set_Jp_pc (inc/matrix/SPE_sparseMatrix1.hpp:40-121) */
+int cfd_spe_assemble(cfd_ctx *ctx);
+int cfd_ell_download(cfd_ctx *ctx, int32_t *col, double *val, size_t n_rows, int k);
+int cfd_ell_upload(cfd_ctx *ctx, const int32_t *col, const double *val, size_t n_rows, int k);
+
+/* ---- operators -------------------------------------------------------- */
+/* y = A x on device vectors named by field ids (N-sized), op = CFD_OP_*,
+   precision = CFD_FP64 or CFD_MIXED (FP32 values and vectors, result widened)
+   (ELL_matrix::multiply_omp, ELL_sparseMatrix.hpp:477-492;
+    SPE_matrix1::spmv_ompJP, SPE_sparseMatrix1.hpp:126-144) */
+int cfd_spmv(cfd_ctx *ctx, int op, int precision, int x_field, int y_field);
+
+/* ---- pressure solves (matB -> X_result, warm start from X_result) ------ */
+/* solver::bicgstabRe2::solve (inc/solver/bicgstabRe2.hpp:91-189), FP64 */
+int cfd_bicgstab(cfd_ctx *ctx, double tol, int maxit, int *iters, double *relres);
+/* red-black SOR on the ELL operator to ||b-Ax||/||b|| < tol, checked every
+   check_every sweeps; iters = sweeps taken */
+int cfd_rbsor(cfd_ctx *ctx, double omega, double tol, int maxit, int check_every, int *iters, double *relres);
+/* FP64 defect correction around FP32 BiCGSTAB (or RB-SOR) inner solves */
+int cfd_defect_correction(cfd_ctx *ctx, double tol, double tol_inner, int max_outer,
+                          int *outer, int *inner_total, double *relres);
+/* the configured solve (params.solver / precision / op): pSolver(Mx.matB, Mx.X_result) */
+int cfd_solve_pressure(cfd_ctx *ctx, int *iters, double *relres);
+
+/* ---- timestep phases (reference semantics, one call = one reference call) */
+/* ConvectionDifussion (inc/source/convection_and_difussion.hpp:168-446): T0 -> T1 */
+int cfd_predictor(cfd_ctx *ctx);
+/* T1 += source_termF (RUN_CPU_PC_METHOD.hpp:309-316), index per source_index_quirk */
+int cfd_source_add(cfd_ctx *ctx);
+/* createBMatrix_omp (inc/matrix/b_matrix.hpp:51-102): T1 -> matB */
+int cfd_build_rhs(cfd_ctx *ctx);
+/* Pressure_transform_X_result (inc/pressure/Transform.hpp:8-24): X_result -> p */
+int cfd_pressure_scatter(cfd_ctx *ctx);
+/* update_UandF_omp (inc/dfib/updateUandF.hpp:95-192): T1, p, eta -> T3, f */
+int cfd_dfib_update(cfd_ctx *ctx);
+/* source_termF += dt * f (RUN_CPU_PC_METHOD.hpp:393-401) */
+int cfd_source_accumulate(cfd_ctx *ctx);
+/* BC_staggered_main (Boundary_Condition.hpp:37-175) on T0 and p */
+int cfd_bc_main(cfd_ctx *ctx);
+/* BC_staggered_copy (updateBC.hpp:6-109): boundary faces of T0 -> T1 */
+int cfd_bc_copy(cfd_ctx *ctx);
+/* virtualF_Int x and y (virtualForceIntergrator.hpp:5-37): cD, cL of f */
+int cfd_drag(cfd_ctx *ctx, double *cd, double *cl);
+/* CheckSteadyStatebyMaxVal_omp (CheckSteadystate.hpp:165-219): max(T0 - T3) per component */
+int cfd_maxdiff(cfd_ctx *ctx, double out[3]);
+/* velocity::copy_omp T0 <- T3 (physicalVariables.hpp:84-102) */
+int cfd_copy_t3_to_t0(cfd_ctx *ctx);
+
+/* ---- whole timestep --------------------------------------------------- */
+/* one iteration of runSeqOmp_predictionMethod's main loop
+   (inc/run/RUN_CPU_PC_METHOD.hpp:287-627), or of runSeqOmp (RUN_CPU.hpp:290-460)
+   when inner_loops == 0; advances simuClass::loop and time. */
+int cfd_step(cfd_ctx *ctx, cfd_step_stats *stats);
+int cfd_loop(const cfd_ctx *ctx, int *loop, double *time);
+int cfd_set_loop(cfd_ctx *ctx, int loop, double time);
+
+/* ---- measurement ------------------------------------------------------ */
+/* average device time (ms) of one launch of the kernel named by `what`, over
+   nrep launches bracketed by HIP events on the context stream:
+   what = 0 ELL SpMV FP64, 1 ELL SpMV FP32, 2 SPE SpMV FP64, 3 RB-SOR sweep,
+   4 BiCGSTAB iteration (FP64), 5 predictor, 6 copy kernel (2*N doubles moved) */
+int cfd_time_kernel(cfd_ctx *ctx, int what, int nrep, double *ms_avg);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFD_CAPI_H */

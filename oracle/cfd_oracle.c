@@ -1,0 +1,1120 @@
+/*
+ * cfd_oracle.c -- TEST INFRASTRUCTURE ONLY: FP64 CPU restatement of the
+ * reference hot path (jimmy3769tw/CFD_NSsolver_forHPC).  Parity checker for
+ * the HIP library and timed CPU baseline; never linked into the product.
+ *
+ * Built with -ffp-contract=off: every rounding below is the one written in
+ * the source.  Where the reference build contracts a multiply-add into an FMA
+ * and the result is a bit-exact target (IB mask sub-sampling, ELL SpMV), the
+ * fma() is written out and commented.
+ *
+ * Parity status per function (pins live in tests/):
+ *   orc_axis_*              pinned: closed form + compiled reference (native SML grid)
+ *   orc_eta_cylinder_z      pinned: compiled reference hash + area identity
+ *   orc_ell_assemble        pinned: compiled reference hash + row-sum invariant
+ *   orc_spmv_ell / _spe     pinned: dense brute force + compiled reference
+ *   orc_predictor           pinned: compiled reference + closed forms
+ *   orc_rhs, orc_dfib_update, orc_bc_*, orc_drag, orc_maxdiff, orc_source_*:
+ *                           pinned: compiled reference step dumps + identities
+ *   orc_bicg_*              pinned: compiled reference trajectory + dense solve
+ *   orc_rbsor               pinned: fixed point equals the dense solve
+ *   orc_sor_lex             pinned: fixed point equals the dense solve of the
+ *                           symmetric form
+ *   orc_sim_step            pinned: compiled reference PC-method loop
+ *   orc_axis_tanh           no reference counterpart (large config only):
+ *                           pinned by its invariants (sum = L, ratio bound)
+ */
+#include "cfd_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define GC 2
+
+typedef struct {
+  int nx, ny, nz;    /* ghosted sizes */
+  int nxc, nyc, nzc; /* interior sizes */
+  size_t nzny, G, N;
+} dims_t;
+
+static dims_t dims_of(const cfd_grid_desc *g) {
+  dims_t d;
+  d.nxc = g->nx_cal; d.nyc = g->ny_cal; d.nzc = g->nz_cal;
+  d.nx = d.nxc + 2 * GC; d.ny = d.nyc + 2 * GC; d.nz = d.nzc + 2 * GC;
+  d.nzny = (size_t)d.ny * d.nz;
+  d.G = (size_t)d.nx * d.nzny;
+  d.N = (size_t)d.nxc * d.nyc * d.nzc;
+  return d;
+}
+
+/* grid::icel (structureGrid.hpp:159-161) and grid::icelCal (:182-185) */
+#define IC(d, i, j, k) (((size_t)(i) * (d).ny + (size_t)(j)) * (d).nz + (size_t)(k))
+#define ICAL(d, i, j, k) \
+  (((size_t)((i) - GC) * (d).nyc + (size_t)((j) - GC)) * (d).nzc + (size_t)((k) - GC))
+
+/* ========================================================================
+ * grid
+ * ===================================================================== */
+
+/* grid::init_uniform, one axis (structureGrid.hpp:409-440) */
+void orc_axis_uniform(int ncal, double L, double *X, double *D, double *Ds) {
+  const int n = ncal + 2 * GC;
+  const double d = L / (n - 2.0 * GC);
+  X[0] = 0.0 - 2.0 * d;
+  for (int i = 1; i < n + 1; ++i) X[i] = X[i - 1] + d;
+  for (int i = 0; i < n; ++i) { D[i] = d; Ds[i] = d; }
+}
+
+/* grid::init_gC_dx, one axis (structureGrid.hpp:506-559) */
+void orc_axis_gc_dx(int n, double *X, double *D, double *Ds) {
+  X[1] = 2 * X[2] - X[3];
+  X[0] = 2 * X[1] - X[2];
+  X[n - 1] = 2 * X[n - 2] - X[n - 3];
+  X[n] = 2 * X[n - 1] - X[n - 2];
+  for (int i = GC; i < n - GC - 1; ++i) Ds[i] = (X[i + 2] - X[i]) / 2.0;
+  for (int i = GC; i < n - GC - 1; ++i) D[i] = (X[i + 1] - X[i]);
+  D[0] = D[1] = D[2];
+  Ds[0] = Ds[1] = Ds[2];
+  D[n - 1] = D[n - 2] = D[n - 3] = X[n - 2] - X[n - 3];
+  Ds[n - 1] = Ds[n - 2] = Ds[n - 3] = Ds[n - 4];
+}
+
+/* grid::unit_SML (structureGrid.hpp:326-386) for one axis */
+int orc_axis_sml(int ncal, double L, double lS, int nS, int off_nS, double *X, double *D, double *Ds) {
+  const int n = ncal + 2 * GC;
+  memset(X, 0, sizeof(double) * (size_t)(n + 1));
+  X[GC] = 0.0;
+  if (nS != 0) {
+    const double dS = lS / nS;
+    const double d = (L - lS) / (ncal - nS);
+    if ((n - nS) % 2 != 0) return -1;
+    int nlEnd0 = (ncal - nS) / 2;
+    int nsEnd0 = nlEnd0 + nS;
+    const int nlEnd1 = nsEnd0 + nlEnd0;
+    nlEnd0 += off_nS;
+    nsEnd0 += off_nS;
+    if (nlEnd0 <= 0) return -1;
+    for (int i = 0; i < nlEnd0; ++i) X[i + GC + 1] = X[i + GC] + d;
+    for (int i = nlEnd0; i < nsEnd0; ++i) X[i + GC + 1] = X[i + GC] + dS;
+    for (int i = nsEnd0; i < nlEnd1; ++i) X[i + GC + 1] = X[i + GC] + d;
+  } else {
+    for (int i = GC; i < n - GC; ++i) X[i + 1] = X[i] + L / ncal;
+  }
+  orc_axis_gc_dx(n, X, D, Ds);
+  return 0;
+}
+
+/* Stretched axis for the large configuration (no reference counterpart):
+   uniform spacing h0 inside a band of width `band` centred on xc, widths
+   growing by `ratio` per cell outside it, capped at hcap chosen by bisection
+   so that the ncal widths sum to L; the last width absorbs the rounding. */
+static double stretch_sum(int ncal, int ic0, int ic1, double h0, double ratio, double hcap, double *w) {
+  double s = 0.0;
+  for (int c = 0; c < ncal; ++c) {
+    int dist = c < ic0 ? ic0 - c : (c > ic1 ? c - ic1 : 0);
+    double h = h0 * pow(ratio, (double)dist);
+    if (h > hcap) h = hcap;
+    if (w) w[c] = h;
+    s += h;
+  }
+  return s;
+}
+
+int orc_axis_tanh(int ncal, double L, double xc, double h0, double band, double ratio,
+                  double *X, double *D, double *Ds) {
+  const int n = ncal + 2 * GC;
+  if (ncal < 4 || !(L > 0) || !(h0 > 0) || !(ratio >= 1.0)) return -1;
+  if (h0 * ncal > L) return -1; /* cannot reach L with widths >= h0 */
+  int nb = (int)floor(band / h0 + 0.5);
+  if (nb < 1) nb = 1;
+  if (nb > ncal) nb = ncal;
+  /* place the band so that its centre is at xc when the far field is uniform */
+  double *w = (double *)malloc(sizeof(double) * (size_t)ncal);
+  if (!w) return -1;
+  int best_ic0 = 0;
+  double lo = h0, hi = L; /* hcap */
+  for (int pass = 0; pass < 2; ++pass) {
+    lo = h0; hi = L;
+    for (int it = 0; it < 200; ++it) {
+      double mid = 0.5 * (lo + hi);
+      double s = stretch_sum(ncal, best_ic0, best_ic0 + nb - 1, h0, ratio, mid, NULL);
+      if (s > L) hi = mid; else lo = mid;
+    }
+    if (pass == 0) {
+      /* choose ic0 so the band centre lands closest to xc */
+      stretch_sum(ncal, 0, nb - 1, h0, ratio, lo, w);
+      double best = 1e300;
+      for (int ic0 = 0; ic0 + nb <= ncal; ++ic0) {
+        stretch_sum(ncal, ic0, ic0 + nb - 1, h0, ratio, lo, w);
+        double x = 0.0;
+        for (int c = 0; c < ic0; ++c) x += w[c];
+        double centre = x + 0.5 * nb * h0;
+        if (fabs(centre - xc) < best) { best = fabs(centre - xc); best_ic0 = ic0; }
+      }
+    }
+  }
+  stretch_sum(ncal, best_ic0, best_ic0 + nb - 1, h0, ratio, lo, w);
+  memset(X, 0, sizeof(double) * (size_t)(n + 1));
+  X[GC] = 0.0;
+  for (int c = 0; c < ncal; ++c) X[GC + c + 1] = X[GC + c] + w[c];
+  X[GC + ncal] = L;
+  free(w);
+  orc_axis_gc_dx(n, X, D, Ds);
+  return 0;
+}
+
+/* ========================================================================
+ * reductions and noise
+ * ===================================================================== */
+
+#define ORC_NBLK 256
+
+/* Fixed 256-way blocked sum with a serial combine: the same bits for every
+   OpenMP thread count (the reference's omp reduction order is thread-dependent,
+   inc/solver/math.hpp:37-51). */
+double orc_dot(size_t n, const double *a, const double *b) {
+  double part[ORC_NBLK];
+#pragma omp parallel for schedule(static)
+  for (int blk = 0; blk < ORC_NBLK; ++blk) {
+    size_t lo = n * (size_t)blk / ORC_NBLK, hi = n * (size_t)(blk + 1) / ORC_NBLK;
+    double s = 0.0;
+    for (size_t i = lo; i < hi; ++i) s += a[i] * b[i];
+    part[blk] = s;
+  }
+  double r = 0.0;
+  for (int blk = 0; blk < ORC_NBLK; ++blk) r += part[blk];
+  return r;
+}
+
+static uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+double orc_noise(uint64_t seed, int component, uint64_t icel, double amp) {
+  uint64_t z = splitmix64(seed ^ ((uint64_t)component << 56) ^ icel);
+  double u01 = (double)(z >> 11) * 0x1.0p-53;
+  return amp * (2.0 * u01 - 1.0);
+}
+
+/* ========================================================================
+ * DFIB_CylinderZ (inc/dfib/setEta.hpp:136-204)
+ * ===================================================================== */
+void orc_eta_cylinder_z(const cfd_grid_desc *g, double cx, double cy, double R, int nsub, double *eta) {
+  const dims_t d = dims_of(g);
+  const double inv_n2 = pow((double)nsub, -2.0);
+  memset(eta, 0, sizeof(double) * d.G);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int i = GC; i < d.nx - GC; ++i) {
+    double *sdx = (double *)malloc(sizeof(double) * (size_t)(nsub + 1));
+    double *sdy = (double *)malloc(sizeof(double) * (size_t)(nsub + 1));
+    for (int j = GC; j < d.ny - GC; ++j) {
+      const size_t icel = (size_t)i * d.nz * d.ny + (size_t)j * d.nz;
+      /* the reference build contracts a*a + b*b into fma(a, a, b*b) */
+      const double Diagonal = sqrt(fma(g->Dx[i], g->Dx[i], g->Dy[j] * g->Dy[j])) * 0.5;
+      const double ax = ((g->X[i] + g->X[i + 1]) * 0.5) - cx;
+      const double ay = ((g->Y[j] + g->Y[j + 1]) * 0.5) - cy;
+      const double Distance = sqrt(fma(ax, ax, ay * ay));
+      if (fabs(Distance - R) < Diagonal) {
+        size_t xi = 0;
+        const double dyg = g->Dy[j] / (double)nsub;
+        const double dxg = g->Dx[i] / (double)nsub;
+        for (int q = 0; q < nsub + 1; ++q) {
+          sdy[q] = fma((double)q, dyg, g->Y[j]); /* contracted Y[j] + jj*dyg */
+          sdx[q] = fma((double)q, dxg, g->X[i]);
+        }
+        for (int ii = 0; ii < nsub; ++ii) {
+          const double bx = (sdx[ii] + sdx[ii + 1]) * 0.5 - cx;
+          for (int jj = 0; jj < nsub; ++jj) {
+            const double by = (sdy[jj] + sdy[jj + 1]) * 0.5 - cy;
+            const double Sdist = sqrt(fma(bx, bx, by * by));
+            if (Sdist <= R) ++xi;
+          }
+        }
+        eta[icel] = (double)xi * inv_n2;
+      } else if (Distance > R) {
+        eta[icel] = 0.;
+      } else {
+        eta[icel] = 1.;
+      }
+    }
+    free(sdx);
+    free(sdy);
+  }
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < d.nx - 1; ++i)
+    for (int j = 0; j < d.ny - 1; ++j)
+      for (int k = 1; k < d.nz - 1; ++k) eta[IC(d, i, j, k)] = eta[IC(d, i, j, 0)];
+}
+
+/* ========================================================================
+ * QUICK convection + central diffusion, explicit Euler
+ * (inc/source/convection_and_difussion.hpp:20-84, 168-446)
+ * ===================================================================== */
+
+/* QuickUnitA (:20-49): face values in the component's own direction */
+static inline void quickA(double Up, double Un, size_t pp, size_t p, size_t ic, size_t n, size_t nn,
+                          const double *Di, const double *Ds, int I, const double *phi,
+                          double *phiP, double *phiN) {
+  if (Up > 0.0)
+    *phiP = 0.5 * (phi[ic] + phi[p]) - 0.125 * Di[I + 1] * Di[I + 1] / Ds[I] *
+                                           ((phi[p] - phi[ic]) / Di[I + 1] - (phi[ic] - phi[n]) / Di[I]);
+  else
+    *phiP = 0.5 * (phi[ic] + phi[p]) - 0.125 * Di[I + 1] * Di[I + 1] / Ds[I + 1] *
+                                           ((phi[pp] - phi[p]) / Di[I + 2] - (phi[p] - phi[ic]) / Di[I + 1]);
+  if (Un > 0.0)
+    *phiN = 0.5 * (phi[n] + phi[ic]) - 0.125 * Di[I] * Di[I] / Ds[I - 1] *
+                                           ((phi[ic] - phi[n]) / Di[I] - (phi[n] - phi[nn]) / Di[I - 1]);
+  else
+    *phiN = 0.5 * (phi[n] + phi[ic]) - 0.125 * Di[I] * Di[I] / Ds[I] *
+                                           ((phi[p] - phi[ic]) / Di[I + 1] - (phi[ic] - phi[n]) / Di[I]);
+}
+
+/* QuickUnitB (:53-84): face values across the component's direction */
+static inline void quickB(double Up, double Un, size_t pp, size_t p, size_t ic, size_t n, size_t nn,
+                          const double *Di, const double *Ds, int I, const double *phi,
+                          double *phiP, double *phiN) {
+  if (Up > 0.0)
+    *phiP = 0.5 * (phi[ic] + phi[p]) - 0.125 * Ds[I] * Ds[I] / Di[I] *
+                                           ((phi[p] - phi[ic]) / Ds[I] - (phi[ic] - phi[n]) / Ds[I - 1]);
+  else
+    *phiP = 0.5 * (phi[ic] + phi[p]) - 0.125 * Ds[I] * Ds[I] / Di[I + 1] *
+                                           ((phi[pp] - phi[p]) / Ds[I + 1] - (phi[p] - phi[ic]) / Ds[I]);
+  if (Un > 0.0)
+    *phiN = 0.5 * (phi[n] + phi[ic]) - 0.125 * Ds[I - 1] * Ds[I - 1] / Di[I - 1] *
+                                           ((phi[ic] - phi[n]) / Ds[I - 1] - (phi[n] - phi[nn]) / Ds[I - 2]);
+  else
+    *phiN = 0.5 * (phi[n] + phi[ic]) - 0.125 * Ds[I - 1] * Ds[I - 1] / Di[I] *
+                                           ((phi[p] - phi[ic]) / Ds[I] - (phi[ic] - phi[n]) / Ds[I - 1]);
+}
+
+void orc_predictor(const cfd_grid_desc *g, double nu, double dt,
+                   const double *u, const double *v, const double *w,
+                   double *u1, double *v1, double *w1) {
+  const dims_t d = dims_of(g);
+  const size_t sx = d.nzny, sy = (size_t)d.nz, sz = 1;
+  const double *Dx = g->Dx, *Dy = g->Dy, *Dz = g->Dz, *Dxs = g->Dxs, *Dys = g->Dys, *Dzs = g->Dzs;
+  const int ib = GC, ie = d.nx - GC, jb = GC, je = d.ny - GC, kb = GC, ke = d.nz - GC;
+
+  /* u: i in [2, nx-3) -- the last face is excluded (:193-197) */
+#pragma omp parallel for schedule(static)
+  for (int i = ib; i < ie - 1; ++i)
+    for (int j = jb; j < je; ++j)
+      for (int k = kb; k < ke; ++k) {
+        const size_t ic = IC(d, i, j, k);
+        const size_t xm = ic - sx, xp = ic + sx, ym = ic - sy, yp = ic + sy, zm = ic - sz, zp = ic + sz;
+        const size_t xmm = ic - 2 * sx, xpp = ic + 2 * sx, ymm = ic - 2 * sy, ypp = ic + 2 * sy;
+        const size_t zmm = ic - 2 * sz, zpp = ic + 2 * sz;
+        const double IT = (Dxs[i] * 0.5) / Dx[i]; /* x_INTRPL (structureGrid.hpp:442) */
+        const double Up = 0.5 * (u[xp] + u[ic]);
+        const double Un = 0.5 * (u[xm] + u[ic]);
+        const double Vp = v[ic] + (v[xp] - v[ic]) * IT;
+        const double Vn = v[ym] + (v[ym + xp - ic] - v[ym]) * IT;
+        const double Wp = w[ic] + (w[xp] - w[ic]) * IT;
+        const double Wn = w[zm] + (w[zm + xp - ic] - w[zm]) * IT;
+        double uPx, uNx, uPy, uNy, uPz, uNz;
+        quickA(Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, u, &uPx, &uNx);
+        quickB(Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, u, &uPy, &uNy);
+        quickB(Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, u, &uPz, &uNz);
+        const double convection = -dt * ((uPx * uPx - uNx * uNx) / Dxs[i] + (uPy * Vp - uNy * Vn) / Dy[j] +
+                                         (uPz * Wp - uNz * Wn) / Dz[k]);
+        const double difussion =
+            nu * dt *
+            (((u[xp] - u[ic]) / Dx[i + 1] - (u[ic] - u[xm]) / Dx[i]) / Dxs[i] +
+             ((u[yp] - u[ic]) / Dys[j] - (u[ic] - u[ym]) / Dys[j - 1]) / Dy[j] +
+             ((u[zp] - u[ic]) / Dzs[k] - (u[ic] - u[zm]) / Dzs[k - 1]) / Dz[k]);
+        u1[ic] = u[ic] + convection + difussion;
+      }
+
+  /* v: j in [2, ny-3) (:274-351) */
+#pragma omp parallel for schedule(static)
+  for (int i = ib; i < ie; ++i)
+    for (int j = jb; j < je - 1; ++j)
+      for (int k = kb; k < ke; ++k) {
+        const size_t ic = IC(d, i, j, k);
+        const size_t xm = ic - sx, xp = ic + sx, ym = ic - sy, yp = ic + sy, zm = ic - sz, zp = ic + sz;
+        const size_t xmm = ic - 2 * sx, xpp = ic + 2 * sx, ymm = ic - 2 * sy, ypp = ic + 2 * sy;
+        const size_t zmm = ic - 2 * sz, zpp = ic + 2 * sz;
+        const double IT = (Dys[j] * 0.5) / Dy[j];
+        const double Vp = 0.5 * (v[yp] + v[ic]);
+        const double Vn = 0.5 * (v[ym] + v[ic]);
+        const double Up = u[ic] + (u[yp] - u[ic]) * IT;
+        const double Un = u[xm] + (u[xm + yp - ic] - u[xm]) * IT;
+        const double Wp = w[ic] + (w[yp] - w[ic]) * IT;
+        const double Wn = w[zm] + (w[zm + yp - ic] - w[zm]) * IT;
+        double vPx, vNx, vPy, vNy, vPz, vNz;
+        quickB(Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, v, &vPx, &vNx);
+        quickA(Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, v, &vPy, &vNy);
+        quickB(Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, v, &vPz, &vNz);
+        const double convection = -dt * ((Up * vPx - Un * vNx) / Dx[i] + (vPy * vPy - vNy * vNy) / Dys[j] +
+                                         (Wp * vPz - Wn * vNz) / Dz[k]);
+        const double difussion =
+            nu * dt *
+            (((v[xp] - v[ic]) / Dxs[i] - (v[ic] - v[xm]) / Dxs[i - 1]) / Dx[i] +
+             ((v[yp] - v[ic]) / Dy[j + 1] - (v[ic] - v[ym]) / Dy[j]) / Dys[j] +
+             ((v[zp] - v[ic]) / Dzs[k] - (v[ic] - v[zm]) / Dzs[k - 1]) / Dz[k]);
+        v1[ic] = v[ic] + convection + difussion;
+      }
+
+  /* w: k in [2, nz-3) (:357-434) */
+#pragma omp parallel for schedule(static)
+  for (int i = ib; i < ie; ++i)
+    for (int j = jb; j < je; ++j)
+      for (int k = kb; k < ke - 1; ++k) {
+        const size_t ic = IC(d, i, j, k);
+        const size_t xm = ic - sx, xp = ic + sx, ym = ic - sy, yp = ic + sy, zm = ic - sz, zp = ic + sz;
+        const size_t xmm = ic - 2 * sx, xpp = ic + 2 * sx, ymm = ic - 2 * sy, ypp = ic + 2 * sy;
+        const size_t zmm = ic - 2 * sz, zpp = ic + 2 * sz;
+        const double IT = (Dzs[k] * 0.5) / Dz[k];
+        const double Wp = 0.5 * (w[zp] + w[ic]);
+        const double Wn = 0.5 * (w[zm] + w[ic]);
+        const double Up = u[ic] + (u[zp] - u[ic]) * IT;
+        const double Un = u[xm] + (u[xm + zp - ic] - u[xm]) * IT;
+        const double Vp = v[ic] + (v[zp] - v[ic]) * IT;
+        const double Vn = v[ym] + (v[ym + zp - ic] - v[ym]) * IT;
+        double wPx, wNx, wPy, wNy, wPz, wNz;
+        quickB(Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, w, &wPx, &wNx);
+        quickB(Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, w, &wPy, &wNy);
+        quickA(Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, w, &wPz, &wNz);
+        const double convection = -dt * ((Up * wPx - Un * wNx) / Dx[i] + (Vp * wPy - Vn * wNy) / Dy[j] +
+                                         (wPz * wPz - wNz * wNz) / Dzs[k]);
+        const double difussion =
+            nu * dt *
+            (((w[xp] - w[ic]) / Dxs[i] - (w[ic] - w[xm]) / Dxs[i - 1]) / Dx[i] +
+             ((w[yp] - w[ic]) / Dys[j] - (w[ic] - w[ym]) / Dys[j - 1]) / Dy[j] +
+             ((w[zp] - w[ic]) / Dz[k + 1] - (w[ic] - w[zm]) / Dz[k]) / Dzs[k]);
+        w1[ic] = w[ic] + convection + difussion;
+      }
+}
+
+/* T1 += source_termF (RUN_CPU_PC_METHOD.hpp:309-316, 460-467).  The reference
+   indexes the ghosted T1 with the interior index icelCal (quirk = 1); quirk = 0
+   adds at the cell's own ghosted index. */
+void orc_source_add(const cfd_grid_desc *g, int quirk, const double *src,
+                    double *u1, double *v1, double *w1) {
+  const dims_t d = dims_of(g);
+  double *T[3] = {u1, v1, w1};
+  for (int c = 0; c < 3; ++c) {
+    double *t = T[c];
+    const double *s = src + d.N * (size_t)c;
+#pragma omp parallel for schedule(static)
+    for (int i = GC; i < d.nx - GC; ++i)
+      for (int j = GC; j < d.ny - GC; ++j)
+        for (int k = GC; k < d.nz - GC; ++k) {
+          const size_t r = ICAL(d, i, j, k);
+          t[quirk ? r : IC(d, i, j, k)] += s[r];
+        }
+  }
+}
+
+/* source_termF += dt * f (RUN_CPU_PC_METHOD.hpp:393-401) */
+void orc_source_accumulate(const cfd_grid_desc *g, double dt, const double *f, double *src) {
+  const dims_t d = dims_of(g);
+  const size_t n3 = 3 * d.N;
+#pragma omp parallel for schedule(static)
+  for (long long m = 0; m < (long long)n3; ++m) src[m] += dt * f[m];
+}
+
+/* createBMatrix_omp, JACOBI_PC branch (inc/matrix/b_matrix.hpp:51-80) */
+void orc_rhs(const cfd_grid_desc *g, double dt, const double *jp,
+             const double *u1, const double *v1, const double *w1, double *b) {
+  const dims_t d = dims_of(g);
+  const double ddt = 1.0 / dt;
+#pragma omp parallel for schedule(static)
+  for (int i = GC; i < d.nx - GC; ++i)
+    for (int j = GC; j < d.ny - GC; ++j)
+      for (int k = GC; k < d.nz - GC; ++k) {
+        const size_t icel = IC(d, i, j, k);
+        const size_t row = ICAL(d, i, j, k);
+        b[row] = -jp[row] * ddt *
+                 ((u1[icel] - u1[IC(d, i - 1, j, k)]) / g->Dx[i] + (v1[icel] - v1[IC(d, i, j - 1, k)]) / g->Dy[j] +
+                  (w1[icel] - w1[IC(d, i, j, k - 1)]) / g->Dz[k]);
+      }
+}
+
+/* ========================================================================
+ * Pressure operator
+ * ===================================================================== */
+
+/* createPressureMatrix (inc/matrix/PressureMatrix.hpp:5-117) into the ELL
+   layout of mat::ELL_matrix<double>::set (ELL_sparseMatrix.hpp:146-160,
+   246-320): slot 0 diagonal, then xm, xp, ym, yp, zm, zp without absent
+   neighbours, padding (N-1, 0.0).  Pressure faces marked Dirichlet keep the
+   neighbour coefficient in the diagonal (p = 0 in the ghost). */
+int orc_ell_assemble(const cfd_grid_desc *g, const int pd[6], int32_t *col, double *val, double *jp) {
+  const dims_t d = dims_of(g);
+  const size_t N = d.N;
+  if (N == 0 || N > 0x7fffffffULL) return -1;
+  const int xBeg = GC, yBeg = GC, zBeg = GC, xEnd = d.nx - GC - 1, yEnd = d.ny - GC - 1, zEnd = d.nz - GC - 1;
+  const double *Dx = g->Dx, *Dy = g->Dy, *Dz = g->Dz, *Dxs = g->Dxs, *Dys = g->Dys, *Dzs = g->Dzs;
+#pragma omp parallel for schedule(static)
+  for (int i = xBeg; i <= xEnd; ++i)
+    for (int j = yBeg; j <= yEnd; ++j)
+      for (int k = zBeg; k <= zEnd; ++k) {
+        const double xp = 1.0 / Dx[i] / Dxs[i], yp = 1.0 / Dy[j] / Dys[j], zp = 1.0 / Dz[k] / Dzs[k];
+        const double xm = 1.0 / Dx[i] / Dxs[i - 1], ym = 1.0 / Dy[j] / Dys[j - 1], zm = 1.0 / Dz[k] / Dzs[k - 1];
+        double aa = 0.0, bb = 0.0, cc = 0.0;
+        if (i == xBeg && !pd[0]) aa = -xm;
+        if (i == xEnd && !pd[1]) aa = -xp;
+        if (j == yBeg && !pd[2]) bb = -ym;
+        if (j == yEnd && !pd[3]) bb = -yp;
+        if (k == zBeg && !pd[4]) cc = -zm;
+        if (k == zEnd && !pd[5]) cc = -zp;
+        const size_t row = ICAL(d, i, j, k);
+        const double vdiag = xm + xp + ym + yp + zm + zp + aa + bb + cc;
+        const double jr = 1. / vdiag;
+        jp[row] = jr;
+        int s = 0;
+        col[(size_t)s * N + row] = (int32_t)row; val[(size_t)s * N + row] = vdiag * jr; ++s;
+        if (i != xBeg) { col[(size_t)s * N + row] = (int32_t)ICAL(d, i - 1, j, k); val[(size_t)s * N + row] = -xm * jr; ++s; }
+        if (i != xEnd) { col[(size_t)s * N + row] = (int32_t)ICAL(d, i + 1, j, k); val[(size_t)s * N + row] = -xp * jr; ++s; }
+        if (j != yBeg) { col[(size_t)s * N + row] = (int32_t)ICAL(d, i, j - 1, k); val[(size_t)s * N + row] = -ym * jr; ++s; }
+        if (j != yEnd) { col[(size_t)s * N + row] = (int32_t)ICAL(d, i, j + 1, k); val[(size_t)s * N + row] = -yp * jr; ++s; }
+        if (k != zBeg) { col[(size_t)s * N + row] = (int32_t)ICAL(d, i, j, k - 1); val[(size_t)s * N + row] = -zm * jr; ++s; }
+        if (k != zEnd) { col[(size_t)s * N + row] = (int32_t)ICAL(d, i, j, k + 1); val[(size_t)s * N + row] = -zp * jr; ++s; }
+        for (; s < 7; ++s) { col[(size_t)s * N + row] = (int32_t)(N - 1); val[(size_t)s * N + row] = 0.0; }
+      }
+  return 0;
+}
+
+/* ELL_matrix::multiply_omp (ELL_sparseMatrix.hpp:477-492).  The reference
+   build contracts r[i] += VAL * x[IDX] into an FMA chain in slot order. */
+void orc_spmv_ell(size_t N, int K, const int32_t *col, const double *val, const double *x, double *y) {
+#pragma omp parallel for schedule(static)
+  for (long long r = 0; r < (long long)N; ++r) {
+    double acc = 0.0;
+    for (int s = 0; s < K; ++s) acc = fma(val[(size_t)s * N + r], x[col[(size_t)s * N + r]], acc);
+    y[r] = acc;
+  }
+}
+
+/* SPE_matrix1::setupPressure + set_Jp_pc (SPE_sparseMatrix1.hpp:40-121) */
+void orc_spe_setup(const cfd_grid_desc *g, const int pd[6], double *coef, double *jp) {
+  const dims_t d = dims_of(g);
+  double *xm = coef, *xp = xm + d.nxc, *ym = xp + d.nxc, *yp = ym + d.nyc, *zm = yp + d.nyc, *zp = zm + d.nzc;
+  for (int i = 0; i < d.nxc; ++i) {
+    xp[i] = 1.0 / g->Dx[i + GC] / g->Dxs[i + GC];
+    xm[i] = 1.0 / g->Dx[i + GC] / g->Dxs[i + GC - 1];
+  }
+  for (int j = 0; j < d.nyc; ++j) {
+    yp[j] = 1.0 / g->Dy[j + GC] / g->Dys[j + GC];
+    ym[j] = 1.0 / g->Dy[j + GC] / g->Dys[j + GC - 1];
+  }
+  for (int k = 0; k < d.nzc; ++k) {
+    zp[k] = 1.0 / g->Dz[k + GC] / g->Dzs[k + GC];
+    zm[k] = 1.0 / g->Dz[k + GC] / g->Dzs[k + GC - 1];
+  }
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < d.nxc; ++i)
+    for (int j = 0; j < d.nyc; ++j)
+      for (int k = 0; k < d.nzc; ++k) {
+        const size_t ic = (size_t)k + (size_t)j * d.nzc + (size_t)i * d.nyc * d.nzc;
+        double ijk = xm[i] + ym[j] + zm[k] + xp[i] + yp[j] + zp[k];
+        if (i == 0 && !pd[0]) ijk -= xm[i];
+        if (i == d.nxc - 1 && !pd[1]) ijk -= xp[i];
+        if (j == 0 && !pd[2]) ijk -= ym[j];
+        if (j == d.nyc - 1 && !pd[3]) ijk -= yp[j];
+        if (k == 0 && !pd[4]) ijk -= zm[k];
+        if (k == d.nzc - 1 && !pd[5]) ijk -= zp[k];
+        jp[ic] = 1.0 / ijk;
+      }
+}
+
+/* SPE_matrix1::spmv_ompJP (SPE_sparseMatrix1.hpp:126-144); each
+   sum += (-c*jp) * x contracts to an FMA in the reference build */
+void orc_spmv_spe(const cfd_grid_desc *g, const double *coef, const double *jp, const double *x, double *y) {
+  const dims_t d = dims_of(g);
+  const double *xm = coef, *xp = xm + d.nxc, *ym = xp + d.nxc, *yp = ym + d.nyc, *zm = yp + d.nyc, *zp = zm + d.nzc;
+  const size_t nz_ = (size_t)d.nzc, nynz_ = (size_t)d.nyc * d.nzc;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < d.nxc; ++i)
+    for (int j = 0; j < d.nyc; ++j)
+      for (int k = 0; k < d.nzc; ++k) {
+        const size_t ic = (size_t)k + (size_t)j * nz_ + (size_t)i * nynz_;
+        const double J = jp[ic];
+        double sum = 0.0;
+        if (i != 0) sum = fma(-xm[i] * J, x[ic - nynz_], sum);
+        if (j != 0) sum = fma(-ym[j] * J, x[ic - nz_], sum);
+        if (k != 0) sum = fma(-zm[k] * J, x[ic - 1], sum);
+        sum += x[ic];
+        if (k != d.nzc - 1) sum = fma(-zp[k] * J, x[ic + 1], sum);
+        if (j != d.nyc - 1) sum = fma(-yp[j] * J, x[ic + nz_], sum);
+        if (i != d.nxc - 1) sum = fma(-xp[i] * J, x[ic + nynz_], sum);
+        y[ic] = sum;
+      }
+}
+
+/* Pressure_transform_X_result (inc/pressure/Transform.hpp:8-24) */
+void orc_pressure_scatter(const cfd_grid_desc *g, const double *X, double *p) {
+  const dims_t d = dims_of(g);
+#pragma omp parallel for schedule(static)
+  for (int i = GC; i < d.nx - GC; ++i)
+    for (int j = GC; j < d.ny - GC; ++j)
+      for (int k = GC; k < d.nz - GC; ++k) p[IC(d, i, j, k)] = X[ICAL(d, i, j, k)];
+}
+
+/* update_UandF_omp (inc/dfib/updateUandF.hpp:95-192), solid velocity 0 */
+void orc_dfib_update(const cfd_grid_desc *g, double dt, const double *eta, const double *p,
+                     const double *u1, const double *v1, const double *w1,
+                     double *u3, double *v3, double *w3, double *f) {
+  const dims_t d = dims_of(g);
+  const size_t N = d.N;
+  const double uS = 0.0, vS = 0.0, wS = 0.0;
+#pragma omp parallel for schedule(static)
+  for (int i = GC; i < d.nx - GC - 1; ++i)
+    for (int j = GC; j < d.ny - GC; ++j)
+      for (int k = GC; k < d.nz - GC; ++k) {
+        const size_t ic = IC(d, i, j, k), icP = IC(d, i + 1, j, k);
+        const double T2u = u1[ic] - dt * (p[icP] - p[ic]) / g->Dxs[i];
+        const double e = eta[ic] + (eta[icP] - eta[ic]) * ((g->Dxs[i] * 0.5) / g->Dx[i]);
+        u3[ic] = eta[ic] * uS + (1.0 - e) * T2u;
+        f[ICAL(d, i, j, k)] = (u3[ic] - T2u) / dt;
+      }
+#pragma omp parallel for schedule(static)
+  for (int i = GC; i < d.nx - GC; ++i)
+    for (int j = GC; j < d.ny - GC - 1; ++j)
+      for (int k = GC; k < d.nz - GC; ++k) {
+        const size_t ic = IC(d, i, j, k), icP = IC(d, i, j + 1, k);
+        const double T2v = v1[ic] - dt * (p[icP] - p[ic]) / g->Dys[j];
+        const double e = eta[ic] + (eta[icP] - eta[ic]) * ((g->Dys[j] * 0.5) / g->Dy[j]);
+        v3[ic] = eta[ic] * vS + (1.0 - e) * T2v;
+        f[ICAL(d, i, j, k) + N] = (v3[ic] - T2v) / dt;
+      }
+#pragma omp parallel for schedule(static)
+  for (int i = GC; i < d.nx - GC; ++i)
+    for (int j = GC; j < d.ny - GC; ++j)
+      for (int k = GC; k < d.nz - GC - 1; ++k) {
+        const size_t ic = IC(d, i, j, k), icP = IC(d, i, j, k + 1);
+        const double T2w = w1[ic] - dt * (p[icP] - p[ic]) / g->Dzs[k];
+        const double e = eta[ic] + (eta[icP] - eta[ic]) * ((g->Dzs[k] * 0.5) / g->Dz[k]);
+        w3[ic] = eta[ic] * wS + (1.0 - e) * T2w;
+        f[ICAL(d, i, j, k) + 2 * N] = (w3[ic] - T2w) / dt;
+      }
+}
+
+/* ========================================================================
+ * Boundary conditions
+ * ===================================================================== */
+
+/* BC_staggered_main (Boundary_Condition.hpp:37-175), faces in the order the
+   reference writes them; normal components: ghost = Dir + Num*interior,
+   tangential: ghost = 2 Dir + (2 Num - 1) interior; faces 1/3/5 also set the
+   last interior face.  Pressure ghosts copy the first interior cell, or are 0
+   on faces marked Dirichlet. */
+void orc_bc_main(const cfd_grid_desc *g, const double Dir[6][3], const double Num[6][3],
+                 const int pd[6], double *u, double *v, double *w, double *p) {
+  const dims_t d = dims_of(g);
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+#define NRM(b, c, x) (Dir[b][c] + Num[b][c] * (x))
+#define TAN(b, c, x) (Dir[b][c] * 2. + (2.0 * Num[b][c] - 1.0) * (x))
+  for (int j = GC; j < ny - GC; ++j)
+    for (int k = GC; k < nz - GC; ++k) { /* BC_0 */
+      u[IC(d, 0, j, k)] = u[IC(d, 1, j, k)] = NRM(0, 0, u[IC(d, 2, j, k)]);
+      v[IC(d, 0, j, k)] = v[IC(d, 1, j, k)] = TAN(0, 1, v[IC(d, 2, j, k)]);
+      w[IC(d, 0, j, k)] = w[IC(d, 1, j, k)] = TAN(0, 2, w[IC(d, 2, j, k)]);
+    }
+  for (int j = GC; j < ny - GC; ++j)
+    for (int k = GC; k < nz - GC; ++k) { /* BC_1 */
+      u[IC(d, nx - 1, j, k)] = u[IC(d, nx - 2, j, k)] = u[IC(d, nx - 3, j, k)] = NRM(1, 0, u[IC(d, nx - 4, j, k)]);
+      v[IC(d, nx - 1, j, k)] = v[IC(d, nx - 2, j, k)] = TAN(1, 1, v[IC(d, nx - 3, j, k)]);
+      w[IC(d, nx - 1, j, k)] = w[IC(d, nx - 2, j, k)] = TAN(1, 2, w[IC(d, nx - 3, j, k)]);
+    }
+  for (int i = GC; i < nx - GC; ++i)
+    for (int k = GC; k < nz - GC; ++k) { /* BC_2 */
+      u[IC(d, i, 0, k)] = u[IC(d, i, 1, k)] = TAN(2, 0, u[IC(d, i, 2, k)]);
+      v[IC(d, i, 0, k)] = v[IC(d, i, 1, k)] = NRM(2, 1, v[IC(d, i, 2, k)]);
+      w[IC(d, i, 0, k)] = w[IC(d, i, 1, k)] = TAN(2, 2, w[IC(d, i, 2, k)]);
+    }
+  for (int i = GC; i < nx - GC; ++i)
+    for (int k = GC; k < nz - GC; ++k) { /* BC_3 */
+      v[IC(d, i, ny - 1, k)] = v[IC(d, i, ny - 2, k)] = v[IC(d, i, ny - 3, k)] = NRM(3, 1, v[IC(d, i, ny - 4, k)]);
+      u[IC(d, i, ny - 1, k)] = u[IC(d, i, ny - 2, k)] = TAN(3, 0, u[IC(d, i, ny - 3, k)]);
+      w[IC(d, i, ny - 1, k)] = w[IC(d, i, ny - 2, k)] = TAN(3, 2, w[IC(d, i, ny - 3, k)]);
+    }
+  for (int i = GC; i < nx - GC; ++i)
+    for (int j = GC; j < ny - GC; ++j) { /* BC_4 */
+      u[IC(d, i, j, 0)] = u[IC(d, i, j, 1)] = TAN(4, 0, u[IC(d, i, j, 2)]);
+      v[IC(d, i, j, 0)] = v[IC(d, i, j, 1)] = TAN(4, 1, v[IC(d, i, j, 2)]);
+      w[IC(d, i, j, 0)] = w[IC(d, i, j, 1)] = NRM(4, 2, w[IC(d, i, j, 2)]);
+    }
+  for (int i = GC; i < nx - GC; ++i)
+    for (int j = GC; j < ny - GC; ++j) { /* BC_5 */
+      w[IC(d, i, j, nz - 1)] = w[IC(d, i, j, nz - 2)] = w[IC(d, i, j, nz - 3)] = NRM(5, 2, w[IC(d, i, j, nz - 4)]);
+      u[IC(d, i, j, nz - 1)] = u[IC(d, i, j, nz - 2)] = TAN(5, 0, u[IC(d, i, j, nz - 3)]);
+      v[IC(d, i, j, nz - 1)] = v[IC(d, i, j, nz - 2)] = TAN(5, 1, v[IC(d, i, j, nz - 3)]);
+    }
+#undef NRM
+#undef TAN
+  if (!p) return;
+  for (int j = GC; j < ny - GC; ++j)
+    for (int k = GC; k < nz - GC; ++k) {
+      p[IC(d, 0, j, k)] = p[IC(d, 1, j, k)] = pd[0] ? 0.0 : p[IC(d, 2, j, k)];
+      p[IC(d, nx - 1, j, k)] = p[IC(d, nx - 2, j, k)] = pd[1] ? 0.0 : p[IC(d, nx - 3, j, k)];
+    }
+  for (int i = GC; i < nx - GC; ++i)
+    for (int k = GC; k < nz - GC; ++k) {
+      p[IC(d, i, 0, k)] = p[IC(d, i, 1, k)] = pd[2] ? 0.0 : p[IC(d, i, 2, k)];
+      p[IC(d, i, ny - 1, k)] = p[IC(d, i, ny - 2, k)] = pd[3] ? 0.0 : p[IC(d, i, ny - 3, k)];
+    }
+  for (int i = GC; i < nx - GC; ++i)
+    for (int j = GC; j < ny - GC; ++j) {
+      p[IC(d, i, j, 0)] = p[IC(d, i, j, 1)] = pd[4] ? 0.0 : p[IC(d, i, j, 2)];
+      p[IC(d, i, j, nz - 1)] = p[IC(d, i, j, nz - 2)] = pd[5] ? 0.0 : p[IC(d, i, j, nz - 3)];
+    }
+}
+
+/* BC_staggered_copy (updateBC.hpp:6-109): boundary faces old -> current */
+void orc_bc_copy(const cfd_grid_desc *g, const double *u0, const double *v0, const double *w0,
+                 double *u1, double *v1, double *w1) {
+  const dims_t d = dims_of(g);
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  for (int j = GC; j < ny - GC; ++j)
+    for (int k = GC; k < nz - GC; ++k) {
+      const int gi[4] = {0, 1, nx - 2, nx - 1};
+      for (int q = 0; q < 4; ++q) {
+        const size_t m = IC(d, gi[q], j, k);
+        u1[m] = u0[m]; v1[m] = v0[m]; w1[m] = w0[m];
+      }
+      const size_t c = IC(d, nx - 3, j, k);
+      u1[c] = u0[c];
+    }
+  for (int i = GC; i < nx - GC; ++i)
+    for (int k = GC; k < nz - GC; ++k) {
+      const int gj[4] = {0, 1, ny - 2, ny - 1};
+      for (int q = 0; q < 4; ++q) {
+        const size_t m = IC(d, i, gj[q], k);
+        u1[m] = u0[m]; v1[m] = v0[m]; w1[m] = w0[m];
+      }
+      const size_t c = IC(d, i, ny - 3, k);
+      v1[c] = v0[c];
+    }
+  for (int i = GC; i < nx - GC; ++i)
+    for (int j = GC; j < ny - GC; ++j) {
+      const int gk[4] = {0, 1, nz - 2, nz - 1};
+      for (int q = 0; q < 4; ++q) {
+        const size_t m = IC(d, i, j, gk[q]);
+        u1[m] = u0[m]; v1[m] = v0[m]; w1[m] = w0[m];
+      }
+      const size_t c = IC(d, i, j, nz - 3);
+      w1[c] = w0[c];
+    }
+}
+
+/* virtualF_Int (virtualForceIntergrator.hpp:5-37) for x and y, with
+   delta = Dx*Dy*Dz (Generator.hpp:47-53); cD = -2 sum / lz (:123-127) */
+void orc_drag(const cfd_grid_desc *g, const double *f, double *cd, double *cl) {
+  const dims_t d = dims_of(g);
+  const size_t N = d.N;
+  double sums[2];
+  for (int c = 0; c < 2; ++c) {
+    double part[ORC_NBLK];
+#pragma omp parallel for schedule(static)
+    for (int blk = 0; blk < ORC_NBLK; ++blk) {
+      size_t lo = N * (size_t)blk / ORC_NBLK, hi = N * (size_t)(blk + 1) / ORC_NBLK;
+      double s = 0.0;
+      for (size_t r = lo; r < hi; ++r) {
+        const int i = (int)(r / ((size_t)d.nyc * d.nzc)) + GC;
+        const int j = (int)((r / (size_t)d.nzc) % (size_t)d.nyc) + GC;
+        const int k = (int)(r % (size_t)d.nzc) + GC;
+        s += f[(size_t)c * N + r] * (g->Dx[i] * g->Dy[j] * g->Dz[k]);
+      }
+      part[blk] = s;
+    }
+    double s = 0.0;
+    for (int blk = 0; blk < ORC_NBLK; ++blk) s += part[blk];
+    sums[c] = s;
+  }
+  *cd = -2.0 * sums[0] / g->lz;
+  *cl = -2.0 * sums[1] / g->lz;
+}
+
+/* CheckSteadyStatebyMaxVal_omp (CheckSteadystate.hpp:165-219) */
+void orc_maxdiff(const cfd_grid_desc *g, const double *u0, const double *v0, const double *w0,
+                 const double *u3, const double *v3, const double *w3, double out[3]) {
+  const dims_t d = dims_of(g);
+  const double *A[3] = {u0, v0, w0}, *B[3] = {u3, v3, w3};
+  for (int c = 0; c < 3; ++c) {
+    double m = 0.0;
+    for (int i = GC; i < d.nx - GC; ++i)
+      for (int j = GC; j < d.ny - GC; ++j)
+        for (int k = GC; k < d.nz - GC; ++k) {
+          const size_t ic = IC(d, i, j, k);
+          const double dif = A[c][ic] - B[c][ic];
+          if (dif > m) m = dif;
+        }
+    out[c] = m;
+  }
+}
+
+/* ========================================================================
+ * solver::bicgstabRe2 (inc/solver/bicgstabRe2.hpp:13-189)
+ * ===================================================================== */
+struct orc_bicg {
+  size_t N;
+  int op;
+  const int32_t *col;
+  const double *val;
+  const cfd_grid_desc *g;
+  const double *coef, *jp;
+  double *r0, *r1, *p1, *s1, *ap, *as, *px1, *px2;
+  int iters_max;
+  double tol, restart_factor, restart_tol;
+  int timestep, total_iters, resets, restarts;
+};
+
+static void bicg_matvec(orc_bicg *s, const double *x, double *y) {
+  if (s->op == CFD_OP_SPE) orc_spmv_spe(s->g, s->coef, s->jp, x, y);
+  else orc_spmv_ell(s->N, 7, s->col, s->val, x, y);
+}
+
+orc_bicg *orc_bicg_create(size_t N, int op, const int32_t *col, const double *val,
+                          const cfd_grid_desc *g, const double *spe_coef, const double *spe_jp) {
+  orc_bicg *s = (orc_bicg *)calloc(1, sizeof(orc_bicg));
+  if (!s) return NULL;
+  s->N = N; s->op = op; s->col = col; s->val = val; s->g = g; s->coef = spe_coef; s->jp = spe_jp;
+  double **vecs[8] = {&s->r0, &s->r1, &s->p1, &s->s1, &s->ap, &s->as, &s->px1, &s->px2};
+  for (int q = 0; q < 8; ++q) {
+    *vecs[q] = (double *)calloc(N ? N : 1, sizeof(double));
+    if (!*vecs[q]) { orc_bicg_destroy(s); return NULL; }
+  }
+  s->iters_max = 5000;
+  s->tol = 1e-3;
+  s->restart_factor = 1e-7;
+  s->restart_tol = s->restart_factor * s->tol;
+  return s;
+}
+
+void orc_bicg_destroy(orc_bicg *s) {
+  if (!s) return;
+  free(s->r0); free(s->r1); free(s->p1); free(s->s1); free(s->ap); free(s->as); free(s->px1); free(s->px2);
+  free(s);
+}
+
+void orc_bicg_set_tolerance(orc_bicg *s, double tol) { s->tol = tol; s->restart_tol = s->restart_factor * tol; }
+void orc_bicg_set_maxit(orc_bicg *s, int maxit) { s->iters_max = maxit; }
+int orc_bicg_restarts(const orc_bicg *s) { return s->restarts; }
+int orc_bicg_resets(const orc_bicg *s) { return s->resets; }
+
+static void vcopy(size_t n, const double *a, double *r) {
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)n; ++i) r[i] = a[i];
+}
+
+double orc_bicg_solve(orc_bicg *s, const double *rhs, double *x, int *iters_out) {
+  const size_t n = s->N;
+  s->timestep++;
+  double r1r0, pre_r1r0, a, w, b;
+  const double norm0 = sqrt(orc_dot(n, rhs, rhs));
+  bicg_matvec(s, x, s->r0);
+  vcopy(n, s->px1, s->px2);
+  vcopy(n, x, s->px1);
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)n; ++i) s->r0[i] = rhs[i] - s->r0[i];
+  vcopy(n, s->r0, s->r1);
+  vcopy(n, s->r0, s->p1);
+  r1r0 = orc_dot(n, s->r1, s->r0);
+  int numIter = 0;
+  for (;;) {
+    bicg_matvec(s, s->p1, s->ap);
+    a = r1r0 / orc_dot(n, s->ap, s->r0);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; ++i) s->s1[i] = s->r1[i] - a * s->ap[i];
+    bicg_matvec(s, s->s1, s->as);
+    w = orc_dot(n, s->as, s->s1) / orc_dot(n, s->as, s->as);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; ++i) {
+      x[i] += a * s->p1[i] + w * s->s1[i];
+      s->r1[i] = s->s1[i] - w * s->as[i];
+    }
+    if ((sqrt(orc_dot(n, s->r1, s->r1)) / norm0) < s->tol) break;
+    if (numIter >= s->iters_max) {
+      long long reset = 0;
+#pragma omp parallel for schedule(static) reduction(+ : reset)
+      for (long long i = 0; i < (long long)n; ++i)
+        if (isnan(x[i]) || isinf(x[i])) { x[i] = s->px1[i]; ++reset; }
+      if ((double)reset > (double)n * 0.1) { vcopy(n, s->px2, x); ++s->resets; }
+      else if (reset != 0) ++s->resets;
+      break;
+    }
+    pre_r1r0 = r1r0;
+    r1r0 = orc_dot(n, s->r1, s->r0);
+    b = (a / w) * (r1r0 / pre_r1r0);
+    if (s->timestep < 10 || fabs(pre_r1r0) > s->restart_tol) {
+#pragma omp parallel for schedule(static)
+      for (long long i = 0; i < (long long)n; ++i) s->p1[i] = s->r1[i] + b * (s->p1[i] - w * s->ap[i]);
+    } else {
+      vcopy(n, s->r1, s->r0);
+      vcopy(n, s->r1, s->p1);
+      ++s->restarts;
+    }
+    ++numIter;
+    ++s->total_iters;
+  }
+  *iters_out = numIter;
+  return sqrt(orc_dot(n, s->r1, s->r1)) / norm0;
+}
+
+/* ========================================================================
+ * red-black SOR on the ELL operator (no reference counterpart: the
+ * reference SOR is lexicographic and in place, SorPipeLine.seq.hpp)
+ * ===================================================================== */
+double orc_rbsor(size_t N, const cfd_grid_desc *g, const int32_t *col, const double *val,
+                 const double *b, double *x, double omega, double tol, int maxit,
+                 int check_every, int *iters) {
+  const dims_t d = dims_of(g);
+  double *r = (double *)malloc(sizeof(double) * (N ? N : 1));
+  const double nb = sqrt(orc_dot(N, b, b));
+  double rel = INFINITY;
+  int it = 0;
+  if (check_every < 1) check_every = 1;
+  while (it < maxit) {
+    for (int colour = 0; colour < 2; ++colour) {
+#pragma omp parallel for schedule(static)
+      for (int ic = 0; ic < d.nxc; ++ic)
+        for (int jc = 0; jc < d.nyc; ++jc)
+          for (int kc = ((ic + jc + colour) & 1); kc < d.nzc; kc += 2) {
+            const size_t row = ((size_t)ic * d.nyc + jc) * d.nzc + kc;
+            double acc = 0.0;
+            for (int s = 0; s < 7; ++s) acc = fma(val[(size_t)s * N + row], x[col[(size_t)s * N + row]], acc);
+            x[row] = x[row] + omega * (b[row] - acc) / val[row];
+          }
+    }
+    ++it;
+    if (it % check_every == 0 || it == maxit) {
+      orc_spmv_ell(N, 7, col, val, x, r);
+#pragma omp parallel for schedule(static)
+      for (long long q = 0; q < (long long)N; ++q) r[q] = b[q] - r[q];
+      rel = sqrt(orc_dot(N, r, r)) / nb;
+      if (rel < tol) break;
+    }
+  }
+  free(r);
+  *iters = it;
+  return rel;
+}
+
+/* SorPipeLine_seq (inc/pressure/solver/SorPipeLine.seq.hpp:4-122) with its
+   coefficients computed on every call (the reference computes them only when
+   simuClass::loop == 1, a loop in which the driver never calls it) */
+double orc_sor_lex(const cfd_grid_desc *g, double dt, double omega, double crit, int itmax,
+                   const double *u1, const double *v1, const double *w1, double *p, int *iters) {
+  const dims_t d = dims_of(g);
+  const size_t N = d.N;
+  double *cf = (double *)malloc(sizeof(double) * 8 * (N ? N : 1));
+  const double rdt = 1.0 / dt;
+  const double *Dx = g->Dx, *Dy = g->Dy, *Dz = g->Dz, *Dxs = g->Dxs, *Dys = g->Dys, *Dzs = g->Dzs;
+  for (int i = GC; i < d.nx - GC; ++i)
+    for (int j = GC; j < d.ny - GC; ++j)
+      for (int k = GC; k < d.nz - GC; ++k) {
+        const size_t ii = ICAL(d, i, j, k) * 8;
+        cf[ii + 0] = Dy[j] * Dz[k] / Dxs[i];
+        cf[ii + 1] = Dy[j] * Dz[k] / Dxs[i - 1];
+        cf[ii + 2] = Dx[i] * Dz[k] / Dys[j];
+        cf[ii + 3] = Dx[i] * Dz[k] / Dys[j - 1];
+        cf[ii + 4] = Dx[i] * Dy[j] / Dzs[k];
+        cf[ii + 5] = Dx[i] * Dy[j] / Dzs[k - 1];
+        const double coef = Dy[j] * Dz[k] / Dxs[i] + Dy[j] * Dz[k] / Dxs[i - 1] + Dx[i] * Dz[k] / Dys[j] +
+                            Dx[i] * Dz[k] / Dys[j - 1] + Dx[i] * Dy[j] / Dzs[k] + Dx[i] * Dy[j] / Dzs[k - 1];
+        cf[ii + 7] = -1.0 / coef; /* grid::coef (structureGrid.hpp:278-287) */
+        const size_t icel = IC(d, i, j, k);
+        const double mChange = (u1[icel] - u1[IC(d, i - 1, j, k)]) * Dy[j] * Dz[k] +
+                               (v1[icel] - v1[IC(d, i, j - 1, k)]) * Dx[i] * Dz[k] +
+                               (w1[icel] - w1[IC(d, i, j, k - 1)]) * Dx[i] * Dy[j];
+        cf[ii + 6] = mChange * rdt;
+      }
+  int it = 0;
+  double residual;
+  do {
+    residual = 0.0;
+    for (int i = GC; i < d.nx - GC; ++i)
+      for (int j = GC; j < d.ny - GC; ++j)
+        for (int k = GC; k < d.nz - GC; ++k) {
+          const size_t icel = IC(d, i, j, k);
+          const size_t ii8 = ICAL(d, i, j, k) * 8;
+          const double pNEW = (-p[IC(d, i + 1, j, k)] * cf[ii8] - p[IC(d, i - 1, j, k)] * cf[ii8 + 1] -
+                               p[IC(d, i, j + 1, k)] * cf[ii8 + 2] - p[IC(d, i, j - 1, k)] * cf[ii8 + 3] -
+                               p[IC(d, i, j, k + 1)] * cf[ii8 + 4] - p[IC(d, i, j, k - 1)] * cf[ii8 + 5] +
+                               cf[ii8 + 6]) * cf[ii8 + 7];
+          const double pChange = fabs(pNEW - p[icel]);
+          p[icel] += omega * (pNEW - p[icel]);
+          if (pChange > residual) residual = pChange;
+        }
+    ++it;
+  } while (residual > crit && it < itmax);
+  free(cf);
+  *iters = it - 1;
+  return residual;
+}
+
+/* ========================================================================
+ * runSeqOmp_predictionMethod main loop (RUN_CPU_PC_METHOD.hpp:287-627)
+ * ===================================================================== */
+struct orc_sim {
+  cfd_grid_desc g;
+  dims_t d;
+  cfd_params prm;
+  double *T0[3], *T1[3], *T3[3], *p, *eta, *f, *src, *b, *X, *jp, *va, *vb;
+  int32_t *col;
+  double *val;
+  double *spe_coef, *spe_jp;
+  orc_bicg *bicg;
+  int loop;
+  double time;
+};
+
+orc_sim *orc_sim_create(const cfd_grid_desc *g, const cfd_params *prm) {
+  orc_sim *s = (orc_sim *)calloc(1, sizeof(orc_sim));
+  if (!s) return NULL;
+  s->g = *g;
+  s->d = dims_of(g);
+  s->prm = *prm;
+  const size_t G = s->d.G, N = s->d.N;
+  for (int c = 0; c < 3; ++c) {
+    s->T0[c] = (double *)calloc(G, sizeof(double));
+    s->T1[c] = (double *)calloc(G, sizeof(double));
+    s->T3[c] = (double *)calloc(G, sizeof(double));
+  }
+  s->p = (double *)calloc(G, sizeof(double));
+  s->eta = (double *)calloc(G, sizeof(double));
+  s->f = (double *)calloc(3 * N, sizeof(double));
+  s->src = (double *)calloc(3 * N, sizeof(double));
+  s->b = (double *)calloc(N, sizeof(double));
+  s->X = (double *)calloc(N, sizeof(double));
+  s->jp = (double *)calloc(N, sizeof(double));
+  s->va = (double *)calloc(N, sizeof(double));
+  s->vb = (double *)calloc(N, sizeof(double));
+  s->col = (int32_t *)calloc(7 * N, sizeof(int32_t));
+  s->val = (double *)calloc(7 * N, sizeof(double));
+  s->spe_coef = (double *)calloc((size_t)2 * (s->d.nxc + s->d.nyc + s->d.nzc), sizeof(double));
+  s->spe_jp = (double *)calloc(N, sizeof(double));
+  s->loop = 1;
+  s->time = 0.0;
+  return s;
+}
+
+void orc_sim_destroy(orc_sim *s) {
+  if (!s) return;
+  for (int c = 0; c < 3; ++c) { free(s->T0[c]); free(s->T1[c]); free(s->T3[c]); }
+  free(s->p); free(s->eta); free(s->f); free(s->src); free(s->b); free(s->X); free(s->jp);
+  free(s->va); free(s->vb); free(s->col); free(s->val); free(s->spe_coef); free(s->spe_jp);
+  orc_bicg_destroy(s->bicg);
+  free(s);
+}
+
+double *orc_sim_field(orc_sim *s, int field, size_t *len) {
+  const size_t G = s->d.G, N = s->d.N;
+  double *ptr = NULL;
+  size_t n = G;
+  switch (field) {
+    case CFD_T0_U: case CFD_T0_V: case CFD_T0_W: ptr = s->T0[field - CFD_T0_U]; break;
+    case CFD_T1_U: case CFD_T1_V: case CFD_T1_W: ptr = s->T1[field - CFD_T1_U]; break;
+    case CFD_T3_U: case CFD_T3_V: case CFD_T3_W: ptr = s->T3[field - CFD_T3_U]; break;
+    case CFD_P: ptr = s->p; break;
+    case CFD_ETA: ptr = s->eta; break;
+    case CFD_F: ptr = s->f; n = 3 * N; break;
+    case CFD_SRC: ptr = s->src; n = 3 * N; break;
+    case CFD_RHS: ptr = s->b; n = N; break;
+    case CFD_X: ptr = s->X; n = N; break;
+    case CFD_JP: ptr = s->jp; n = N; break;
+    case CFD_VEC_A: ptr = s->va; n = N; break;
+    case CFD_VEC_B: ptr = s->vb; n = N; break;
+    default: n = 0; break;
+  }
+  if (len) *len = n;
+  return ptr;
+}
+
+int32_t *orc_sim_ell_col(orc_sim *s) { return s->col; }
+int orc_sim_loop(const orc_sim *s) { return s->loop; }
+void orc_sim_set_loop(orc_sim *s, int loop, double time) { s->loop = loop; s->time = time; }
+
+int orc_sim_init_state(orc_sim *s, double noise_amp, uint64_t seed) {
+  const dims_t d = s->d;
+  for (int c = 0; c < 3; ++c) {
+    double *t = s->T0[c];
+    const double u0 = s->prm.u_init[c];
+    for (size_t m = 0; m < d.G; ++m) t[m] = u0;
+    if (noise_amp != 0.0) {
+      for (int i = GC; i < d.nx - GC; ++i)
+        for (int j = GC; j < d.ny - GC; ++j)
+          for (int k = GC; k < d.nz - GC; ++k) {
+            const size_t ic = IC(d, i, j, k);
+            t[ic] += orc_noise(seed, c, ic, noise_amp);
+          }
+    }
+    memcpy(s->T1[c], t, sizeof(double) * d.G);
+    memcpy(s->T3[c], t, sizeof(double) * d.G);
+  }
+  memset(s->p, 0, sizeof(double) * d.G);
+  memset(s->X, 0, sizeof(double) * d.N);
+  memset(s->f, 0, sizeof(double) * 3 * d.N);
+  memset(s->src, 0, sizeof(double) * 3 * d.N);
+  orc_bc_main(&s->g, s->prm.bc_dir, s->prm.bc_num, s->prm.p_dirichlet, s->T0[0], s->T0[1], s->T0[2], s->p);
+  orc_bc_copy(&s->g, s->T0[0], s->T0[1], s->T0[2], s->T1[0], s->T1[1], s->T1[2]);
+  s->loop = 1;
+  s->time = 0.0;
+  return 0;
+}
+
+int orc_sim_setup(orc_sim *s) {
+  const cfd_params *P = &s->prm;
+  if (orc_ell_assemble(&s->g, P->p_dirichlet, s->col, s->val, s->jp) != 0) return -1;
+  orc_spe_setup(&s->g, P->p_dirichlet, s->spe_coef, s->spe_jp);
+  if (P->op == CFD_OP_SPE) memcpy(s->jp, s->spe_jp, sizeof(double) * s->d.N); /* gA.jp = set_Jp_pc() */
+  orc_bicg_destroy(s->bicg);
+  s->bicg = orc_bicg_create(s->d.N, P->op, s->col, s->val, &s->g, s->spe_coef, s->spe_jp);
+  if (!s->bicg) return -1;
+  orc_bicg_set_tolerance(s->bicg, P->tol);
+  orc_bicg_set_maxit(s->bicg, P->maxit);
+  if (P->ib_on) orc_eta_cylinder_z(&s->g, P->ib_cx, P->ib_cy, P->ib_r, P->ib_nsub, s->eta);
+  else memset(s->eta, 0, sizeof(double) * s->d.G);
+  return 0;
+}
+
+static void sim_pass(orc_sim *s, int pass, int accumulate, cfd_step_stats *st) {
+  const cfd_params *P = &s->prm;
+  const cfd_grid_desc *g = &s->g;
+  orc_predictor(g, 1.0 / P->Re, P->dt, s->T0[0], s->T0[1], s->T0[2], s->T1[0], s->T1[1], s->T1[2]);
+  orc_source_add(g, P->source_index_quirk, s->src, s->T1[0], s->T1[1], s->T1[2]);
+  int it = -1;
+  double rel = 0.0;
+  if (s->loop > 1) {
+    orc_rhs(g, P->dt, s->jp, s->T1[0], s->T1[1], s->T1[2], s->b);
+    if (P->solver == CFD_SOLVER_RBSOR)
+      rel = orc_rbsor(s->d.N, g, s->col, s->val, s->b, s->X, P->sor_omega, P->tol, P->maxit,
+                      P->sor_check, &it);
+    else
+      rel = orc_bicg_solve(s->bicg, s->b, s->X, &it);
+    orc_pressure_scatter(g, s->X, s->p);
+  }
+  orc_dfib_update(g, P->dt, s->eta, s->p, s->T1[0], s->T1[1], s->T1[2], s->T3[0], s->T3[1], s->T3[2], s->f);
+  if (accumulate) orc_source_accumulate(g, P->dt, s->f, s->src);
+  if (st && pass < CFD_MAX_PASSES) {
+    st->iters[pass] = it;
+    st->relres[pass] = rel;
+    orc_drag(g, s->f, &st->cd[pass], &st->cl[pass]);
+  }
+}
+
+int orc_sim_step(orc_sim *s, cfd_step_stats *st) {
+  const cfd_params *P = &s->prm;
+  const cfd_grid_desc *g = &s->g;
+  const dims_t d = s->d;
+  if (st) { memset(st, 0, sizeof(*st)); st->loop = s->loop; st->n_passes = P->inner_loops + 1; }
+  memset(s->src, 0, sizeof(double) * 3 * d.N);
+  for (int pass = 0; pass < P->inner_loops; ++pass) sim_pass(s, pass, 1, st);
+  sim_pass(s, P->inner_loops, 0, st);
+  double md[3];
+  orc_maxdiff(g, s->T0[0], s->T0[1], s->T0[2], s->T3[0], s->T3[1], s->T3[2], md);
+  if (st) memcpy(st->maxdiff, md, sizeof(md));
+  for (int c = 0; c < 3; ++c) vcopy(d.G, s->T3[c], s->T0[c]);
+  orc_bc_main(g, P->bc_dir, P->bc_num, P->p_dirichlet, s->T0[0], s->T0[1], s->T0[2], s->p);
+  orc_bc_copy(g, s->T0[0], s->T0[1], s->T0[2], s->T1[0], s->T1[1], s->T1[2]);
+  /* simuClass::finishloop (simuClass.hpp:152-169) */
+  s->time += P->dt;
+  s->loop++;
+  return 0;
+}

@@ -1,0 +1,125 @@
+/*
+ * cfd_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * FP64 CPU restatement of the reference solver's per-timestep hot path
+ * (jimmy3769tw/CFD_NSsolver_forHPC, inc/...), used as the parity checker for
+ * the HIP library and as the timed CPU baseline.  Only tests/, the smoke entry
+ * point and the bench's CPU-baseline leg may load it; the product never does.
+ *
+ * Every function cites the reference file:line it restates.  Floating-point
+ * contraction is off for this translation unit; where the reference build
+ * (-O3 -march=native, GCC's default -ffp-contract=fast) contracts a
+ * multiply-add whose rounding matters for a bit-exact target, the restatement
+ * writes fma() explicitly and says so.
+ *
+ * Parity pins (see tests/): grid, ELL assembly and IB mask against the
+ * compiled reference (oracle/_ref) and committed fixtures; predictor, RHS,
+ * DFIB update, BC, SpMV, BiCGSTAB and the PC-method step against the compiled
+ * reference at its native case and against closed forms.  Red-black SOR has
+ * no reference counterpart (the reference SOR is lexicographic): it is pinned
+ * by its fixed point (same solution as BiCGSTAB / a dense solve).
+ */
+#ifndef CFD_ORACLE_H
+#define CFD_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../include/cfd_capi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- grid (inc/grid/structureGrid.hpp) --------------------------------- */
+/* init_uniform, one axis (structureGrid.hpp:409-440): X has n+1 entries,
+   D and Ds n entries, n = ncal + 4 */
+void orc_axis_uniform(int ncal, double L, double *X, double *D, double *Ds);
+/* unit_SML + init_gC_dx, one axis (structureGrid.hpp:326-386, 506-559);
+   returns 0, or -1 where the reference throws std::invalid_argument */
+int orc_axis_sml(int ncal, double L, double lS, int nS, int off_nS, double *X, double *D, double *Ds);
+/* tanh clustering around xc (no reference counterpart; the large config's
+   stretched grid): widths grow by at most `ratio` per cell away from a band
+   of width `band` of uniform spacing h0 centred on xc, then init_gC_dx */
+int orc_axis_tanh(int ncal, double L, double xc, double h0, double band, double ratio,
+                  double *X, double *D, double *Ds);
+/* init_gC_dx for one axis (structureGrid.hpp:506-559) */
+void orc_axis_gc_dx(int n, double *X, double *D, double *Ds);
+
+/* ---- individual hot-path functions (ghosted arrays, layout of cfd_capi.h) */
+void orc_eta_cylinder_z(const cfd_grid_desc *g, double cx, double cy, double R, int nsub, double *eta);
+void orc_predictor(const cfd_grid_desc *g, double nu, double dt,
+                   const double *u0, const double *v0, const double *w0,
+                   double *u1, double *v1, double *w1);
+void orc_source_add(const cfd_grid_desc *g, int quirk, const double *src,
+                    double *u1, double *v1, double *w1);
+void orc_source_accumulate(const cfd_grid_desc *g, double dt, const double *f, double *src);
+void orc_rhs(const cfd_grid_desc *g, double dt, const double *jp,
+             const double *u1, const double *v1, const double *w1, double *b);
+/* createPressureMatrix; col/val slot-major [7][N]; returns 0 or -1 */
+int orc_ell_assemble(const cfd_grid_desc *g, const int p_dirichlet[6], int32_t *col, double *val, double *jp);
+void orc_spmv_ell(size_t N, int K, const int32_t *col, const double *val, const double *x, double *y);
+/* SPE_matrix1 setupPressure + set_Jp_pc: coef holds xm[nx],xp[nx],ym[ny],yp[ny],zm[nz],zp[nz] (interior
+   axis index) followed by jp[N] */
+void orc_spe_setup(const cfd_grid_desc *g, const int p_dirichlet[6], double *coef, double *jp);
+void orc_spmv_spe(const cfd_grid_desc *g, const double *coef, const double *jp, const double *x, double *y);
+void orc_pressure_scatter(const cfd_grid_desc *g, const double *X, double *p);
+void orc_dfib_update(const cfd_grid_desc *g, double dt, const double *eta, const double *p,
+                     const double *u1, const double *v1, const double *w1,
+                     double *u3, double *v3, double *w3, double *f);
+void orc_bc_main(const cfd_grid_desc *g, const double dir[6][3], const double num[6][3],
+                 const int p_dirichlet[6], double *u, double *v, double *w, double *p);
+void orc_bc_copy(const cfd_grid_desc *g, const double *u0, const double *v0, const double *w0,
+                 double *u1, double *v1, double *w1);
+void orc_drag(const cfd_grid_desc *g, const double *f, double *cd, double *cl);
+void orc_maxdiff(const cfd_grid_desc *g, const double *u0, const double *v0, const double *w0,
+                 const double *u3, const double *v3, const double *w3, double out[3]);
+/* deterministic dot product (fixed 256-way blocking, serial combine) */
+double orc_dot(size_t n, const double *a, const double *b);
+/* seeded uniform noise in [-amp, amp): splitmix64(seed ^ (component<<56) ^ icel) */
+double orc_noise(uint64_t seed, int component, uint64_t icel, double amp);
+
+/* ---- solvers ------------------------------------------------------------ */
+typedef struct orc_bicg orc_bicg;
+/* op: CFD_OP_ELL (col/val) or CFD_OP_SPE (g, coef, jp) */
+orc_bicg *orc_bicg_create(size_t N, int op, const int32_t *col, const double *val,
+                          const cfd_grid_desc *g, const double *spe_coef, const double *spe_jp);
+void orc_bicg_destroy(orc_bicg *s);
+void orc_bicg_set_tolerance(orc_bicg *s, double tol);
+void orc_bicg_set_maxit(orc_bicg *s, int maxit);
+/* bicgstabRe2::solve (inc/solver/bicgstabRe2.hpp:91-189); returns relres, *iters */
+double orc_bicg_solve(orc_bicg *s, const double *b, double *x, int *iters);
+int orc_bicg_restarts(const orc_bicg *s);
+int orc_bicg_resets(const orc_bicg *s);
+
+/* red-black SOR on the ELL operator, natural ordering; colour = (i+j+k) & 1 */
+double orc_rbsor(size_t N, const cfd_grid_desc *g, const int32_t *col, const double *val,
+                 const double *b, double *x, double omega, double tol, int maxit,
+                 int check_every, int *iters);
+/* SorPipeLine_seq (inc/pressure/solver/SorPipeLine.seq.hpp:4-122): lexicographic
+   SOR on the ghosted p with the volume-weighted coefficients, stops on
+   max|dp| < crit; returns the final max|dp| and the reference's iteration count */
+double orc_sor_lex(const cfd_grid_desc *g, double dt, double omega, double crit, int itmax,
+                   const double *u1, const double *v1, const double *w1, double *p, int *iters);
+
+/* ---- the timestep loop (RUN_CPU_PC_METHOD.hpp:287-627 / RUN_CPU.hpp:290-460) */
+typedef struct orc_sim orc_sim;
+orc_sim *orc_sim_create(const cfd_grid_desc *g, const cfd_params *prm);
+void orc_sim_destroy(orc_sim *s);
+/* pointer to a CFD_* array of the simulation; *len receives its length */
+double *orc_sim_field(orc_sim *s, int field, size_t *len);
+int32_t *orc_sim_ell_col(orc_sim *s);
+/* iniU(u_init) + noise, p = 0, X = 0, T1 = T3 = T0, BC main + copy */
+int orc_sim_init_state(orc_sim *s, double noise_amp, uint64_t seed);
+/* DFIB_CylinderZ and the pressure operator (ELL and SPE) */
+int orc_sim_setup(orc_sim *s);
+int orc_sim_step(orc_sim *s, cfd_step_stats *st);
+/* phases of one pass, exposed for the CPU baseline's timing */
+int orc_sim_loop(const orc_sim *s);
+void orc_sim_set_loop(orc_sim *s, int loop, double time);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,235 @@
+// ref_harness.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// Drives the reference solver's own header-only functions, compiled from the
+// sources where they lie under /root/reference (see oracle/Makefile), at the
+// reference's hard-coded native case (190x210x40 SML grid, z-cylinder at
+// (6.5, 6), R = 0.5, BC_Cavity_A table, all-Neumann pressure, Re 40, dt 0.002)
+// and dumps raw arrays for the parity tests:
+//   grid.bin        X,Y,Z (n+1 each) then Dx,Dy,Dz,Dxs,Dys,Dzs (n each)
+//   eta.bin         DFIB_CylinderZ result (G doubles)
+//   ell_col.bin     createPressureMatrix ELL columns, slot-major int32 [7][N]
+//   ell_val.bin     ELL values, slot-major [7][N]
+//   jp.bin          gA.jp after createPressureMatrix (N)
+//   spe_jp.bin      SPE_matrix1::set_Jp_pc (N)
+//   step_XXX_*.bin  state after each timestep (T0 u,v,w, p, X, f)
+//   fn_*.bin        inputs/outputs of each hot-path function in pass 1 of step FN_STEP
+//   stats.txt       per pass: step pass iters error cD cL, per step: maxdiff
+//
+// The time loop below calls the reference functions in the order of
+// runSeqOmp_predictionMethod (inc/run/RUN_CPU_PC_METHOD.hpp:287-627); it
+// restates only the sequencing, never the arithmetic.
+//
+// usage: ref_harness OUTDIR NSTEPS TOL OP(ell|spe) FN_STEP
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <ctime>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <utility>
+#include <vector>
+#include <omp.h>
+#include <dirent.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+// ELL_matrix keeps its (column, value) slots private; expose them to dump the
+// slot layout without touching the reference sources.
+#define private public
+#include "controlPanel.hpp"
+#include "run/general.hpp"
+#undef private
+
+static std::string g_out;
+
+template <typename T>
+static void dump(const std::string &name, const std::vector<T> &v) {
+  std::ofstream f(g_out + "/" + name, std::ios::binary);
+  f.write(reinterpret_cast<const char *>(v.data()), sizeof(T) * v.size());
+}
+
+static void dump_vel(const std::string &prefix, const velocity &V) {
+  dump(prefix + "_u.bin", V.u);
+  dump(prefix + "_v.bin", V.v);
+  dump(prefix + "_w.bin", V.w);
+}
+
+int main(int argc, char **argv) {
+  if (argc < 6) {
+    std::cerr << "usage: ref_harness OUTDIR NSTEPS TOL OP(ell|spe) FN_STEP\n";
+    return 2;
+  }
+  g_out = argv[1];
+  const int nsteps = std::atoi(argv[2]);
+  const double tol = std::atof(argv[3]);
+  const bool use_spe = std::string(argv[4]) == "spe";
+  const int fn_step = std::atoi(argv[5]);
+  mkdir(g_out.c_str(), 0755);
+
+  // src/CFD_MX.cpp:28-57
+  simuClass simu;
+  grid gA;
+  simu.set_time_max(40);
+  simu.set_dt(0.002);
+  simu.set_Re(40.0);
+  simu.init_fileStep(1);
+  simu.DfibMethod = "DFIB_Cylinder-Z";
+  simu.Dfib_boolT = true;
+  simu.Locality = 1;
+  simu.p_criteria = tol;
+
+  // RUN_CPU_PC_METHOD.hpp:30-119
+  DfibArray Dfib;
+  pressure t1;
+  velocity T0, T1, T3;
+  shareMenory ShareM;
+  MxClass Mx;
+  calDomain dom;
+  std::vector<int> grid_size{gA.nx, gA.ny, gA.nz};
+  std::vector<int> dims{1, 1, 1};
+  dom.initTable(grid_size, dims);
+  dom.initLocal(0, 0, 0);
+  resize_variable(gA, t1, T0, T1, T3, Dfib);
+  T0.iniU_omp(1.0, 0.0, 0.0);
+  T1.iniU_omp(1.0, 0.0, 0.0);
+  T3.iniU_omp(1.0, 0.0, 0.0);
+  t1.init_p(0.0);
+  generateGride(simu, ShareM, dom, gA);
+  T1.u = T3.u = T0.u;
+  T1.v = T3.v = T0.v;
+  T1.w = T3.w = T0.w;
+  BC_staggered_main(dom, T0, t1, gA);
+  BC_staggered_copy(dom, T0, T1, gA);
+
+  {
+    std::vector<double> g;
+    for (auto *a : {&gA.X, &gA.Y, &gA.Z, &gA.Dx, &gA.Dy, &gA.Dz, &gA.Dxs, &gA.Dys, &gA.Dzs})
+      g.insert(g.end(), a->begin(), a->end());
+    dump("grid.bin", g);
+    dump("init_T0_u.bin", T0.u);
+  }
+
+  const size_t N = gA.iceltotCal;
+  Mx.X_result.resize(N, 0.0);
+
+  // ELL operator (RUN_CPU_PC_METHOD.hpp:137-148)
+  createPressureMatrix(Mx, simu, dom, gA);
+  {
+    std::vector<int> col(7 * N);
+    std::vector<double> val(7 * N);
+    for (size_t r = 0; r < N; ++r)
+      for (int s = 0; s < 7; ++s) {
+        auto &pr = Mx.matA_ell.Idx_val_[r * 7 + s];
+        col[s * N + r] = pr.first;
+        val[s * N + r] = pr.second;
+      }
+    dump("ell_col.bin", col);
+    dump("ell_val.bin", val);
+    dump("jp.bin", gA.jp);
+  }
+  std::vector<double> ell_jp = gA.jp;
+  // SPE operator (RUN_CPU_PC_METHOD.hpp:167-175)
+  Mx.matA_spe.resize(gA.nxCal, gA.nyCal, gA.nzCal);
+  Mx.matA_spe.setupPressure(gA.gC, gA.Dx, gA.Dy, gA.Dz, gA.Dxs, gA.Dys, gA.Dzs);
+  std::vector<double> spe_jp = Mx.matA_spe.set_Jp_pc();
+  dump("spe_jp.bin", spe_jp);
+  gA.jp = use_spe ? spe_jp : ell_jp;
+
+  solver::bicgstabRe2<ELL_type> ellSolver(Mx.matA_ell);
+  solver::bicgstabRe2<SPE_type> speSolver(Mx.matA_spe);
+  ellSolver.setTolerance(simu.p_criteria);
+  speSolver.setTolerance(simu.p_criteria);
+  auto solve = [&](std::vector<double> &b, std::vector<double> &x) {
+    return use_spe ? speSolver(b, x) : ellSolver(b, x);
+  };
+
+  // RUN_CPU_PC_METHOD.hpp:258-264
+  std::fill(Dfib.eta.begin(), Dfib.eta.end(), 0.0);
+  DFIB_CylinderZ(Dfib, dom, gA);
+  dump("eta.bin", Dfib.eta);
+  auto source_termF = Dfib.f;
+
+  std::ofstream stats(g_out + "/stats.txt");
+  stats << std::setprecision(17);
+
+  auto drag = [&](int dir) {
+    virtualF_Int(Dfib, dom, gA, dir);
+    return -2.0 * Dfib.ValSum[dir] / gA.lz;
+  };
+
+  for (simu.loop = 1; (int)simu.loop <= nsteps; simu.finishloop()) {
+    const bool fn = (int)simu.loop == fn_step;
+    char tag[64];
+    std::fill(source_termF.begin(), source_termF.end(), double());
+    for (int pass = 0; pass < 4; ++pass) {
+      const bool fnp = fn && pass == 1;
+      if (fnp) { dump_vel("fn_pred_in_T0", T0); dump_vel("fn_pred_in_T1", T1); }
+      ConvectionDifussion(simu, T0, T1, dom, gA);
+      if (fnp) { dump_vel("fn_pred_out_T1", T1); dump("fn_src.bin", source_termF); }
+      for (int c = 0; c < 3; c++)
+        for (auto i = dom.i_begin; i < dom.i_endof; ++i)
+          for (auto j = dom.j_begin; j < dom.j_endof; ++j)
+            for (auto k = dom.k_begin; k < dom.k_endof; ++k)
+              T1.U(c)[gA.icelCal(i, j, k)] += source_termF[gA.icelCal(i, j, k) + gA.iceltotCal * c];
+      if (fnp) dump_vel("fn_srcadd_out_T1", T1);
+      int iters = -1;
+      double err = 0.0;
+      if (simu.loop > 1) {
+        createBMatrix_omp(T1, Mx, simu, dom, gA);
+        if (fnp) { dump("fn_rhs_out_b.bin", Mx.matB); dump("fn_solve_in_x.bin", Mx.X_result); }
+        std::tie(iters, err) = solve(Mx.matB, Mx.X_result);
+        if (fnp) {
+          dump("fn_solve_out_x.bin", Mx.X_result);
+          std::vector<double> ax(N), ax2(N);
+          Mx.matA_ell.multiply_omp(Mx.X_result, ax);
+          Mx.matA_spe.multiply_omp(Mx.X_result, ax2);
+          dump("fn_spmv_ell.bin", ax);
+          dump("fn_spmv_spe.bin", ax2);
+          stats << "fn_solve " << iters << " " << err << "\n";
+        }
+        Pressure_transform_X_result(t1, Mx, dom, gA);
+      }
+      if (fnp) dump("fn_update_in_p.bin", t1.p);
+      update_UandF_omp(Dfib, simu, t1, T1, T3, dom, gA);
+      if (fnp) { dump_vel("fn_update_out_T3", T3); dump("fn_update_out_f.bin", Dfib.f); }
+      if (pass < 3)
+        for (int c = 0; c < 3; c++)
+          for (auto i = dom.i_begin; i < dom.i_endof; ++i)
+            for (auto j = dom.j_begin; j < dom.j_endof; ++j)
+              for (auto k = dom.k_begin; k < dom.k_endof; ++k)
+                source_termF[gA.icelCal(i, j, k) + gA.iceltotCal * c] +=
+                    simu.dt * Dfib.f[gA.icelCal(i, j, k) + gA.iceltotCal * c];
+      stats << "pass " << simu.loop << " " << pass << " " << iters << " " << err << " " << drag(0) << " "
+            << drag(1) << "\n";
+    }
+    auto [ud, vd, wd] = CheckSteadyStatebyMaxVal_omp(simu, ShareM, T0, T3, dom, gA);
+    stats << "maxdiff " << simu.loop << " " << ud << " " << vd << " " << wd << "\n";
+    T0.copy_omp(T3);
+    if (fn) { dump_vel("fn_bc_in_T0", T0); dump("fn_bc_in_p.bin", t1.p); dump_vel("fn_bccopy_in_T1", T1); }
+    BC_staggered_main(dom, T0, t1, gA);
+    BC_staggered_copy(dom, T0, T1, gA);
+    if (fn) { dump_vel("fn_bc_out_T0", T0); dump("fn_bc_out_p.bin", t1.p); dump_vel("fn_bccopy_out_T1", T1); }
+    std::snprintf(tag, sizeof(tag), "step_%03d", (int)simu.loop);
+    dump_vel(std::string(tag) + "_T0", T0);
+    dump_vel(std::string(tag) + "_T1", T1);
+    dump(std::string(tag) + "_p.bin", t1.p);
+    dump(std::string(tag) + "_X.bin", Mx.X_result);
+    dump(std::string(tag) + "_f.bin", Dfib.f);
+    stats.flush();
+  }
+  return 0;
+}

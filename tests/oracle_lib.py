@@ -1,0 +1,182 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure only: the oracle is the parity checker for the HIP
+library and never part of the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from cfd_nssolver_forhpc_amd.capi import CfdGridDesc, CfdParams, CfdStepStats, GridArrays
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_DIR = ROOT / "oracle"
+_LIB = None
+
+dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib(native: bool = False):
+    """Load (building if absent) the oracle shared library."""
+    global _LIB
+    if _LIB is not None and not native:
+        return _LIB
+    name = "liboracle_native.so" if native else "liboracle.so"
+    path = ORACLE_DIR / "_build" / name
+    if not path.exists():
+        subprocess.run(["make", "-C", str(ORACLE_DIR), "native" if native else "all"], check=True,
+                       stdout=subprocess.DEVNULL)
+    L = C.CDLL(str(path))
+    _declare(L)
+    if not native:
+        _LIB = L
+    return L
+
+
+def _declare(L):
+    G = C.POINTER(CfdGridDesc)
+    L.orc_axis_uniform.argtypes = [C.c_int, C.c_double, dp, dp, dp]
+    L.orc_axis_sml.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp, dp]
+    L.orc_axis_sml.restype = C.c_int
+    L.orc_axis_tanh.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp]
+    L.orc_axis_tanh.restype = C.c_int
+    L.orc_eta_cylinder_z.argtypes = [G, C.c_double, C.c_double, C.c_double, C.c_int, dp]
+    L.orc_predictor.argtypes = [G, C.c_double, C.c_double, dp, dp, dp, dp, dp, dp]
+    L.orc_source_add.argtypes = [G, C.c_int, dp, dp, dp, dp]
+    L.orc_source_accumulate.argtypes = [G, C.c_double, dp, dp]
+    L.orc_rhs.argtypes = [G, C.c_double, dp, dp, dp, dp, dp]
+    L.orc_ell_assemble.argtypes = [G, C.POINTER(C.c_int), ip, dp, dp]
+    L.orc_ell_assemble.restype = C.c_int
+    L.orc_spmv_ell.argtypes = [C.c_size_t, C.c_int, ip, dp, dp, dp]
+    L.orc_spe_setup.argtypes = [G, C.POINTER(C.c_int), dp, dp]
+    L.orc_spmv_spe.argtypes = [G, dp, dp, dp, dp]
+    L.orc_pressure_scatter.argtypes = [G, dp, dp]
+    L.orc_dfib_update.argtypes = [G, C.c_double, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+    L.orc_bc_main.argtypes = [G, C.POINTER((C.c_double * 3) * 6), C.POINTER((C.c_double * 3) * 6),
+                              C.POINTER(C.c_int), dp, dp, dp, dp]
+    L.orc_bc_copy.argtypes = [G, dp, dp, dp, dp, dp, dp]
+    L.orc_drag.argtypes = [G, dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.orc_maxdiff.argtypes = [G, dp, dp, dp, dp, dp, dp, C.POINTER(C.c_double * 3)]
+    L.orc_dot.argtypes = [C.c_size_t, dp, dp]
+    L.orc_dot.restype = C.c_double
+    L.orc_noise.argtypes = [C.c_uint64, C.c_int, C.c_uint64, C.c_double]
+    L.orc_noise.restype = C.c_double
+    L.orc_bicg_create.argtypes = [C.c_size_t, C.c_int, ip, dp, G, C.c_void_p, C.c_void_p]
+    L.orc_bicg_create.restype = C.c_void_p
+    L.orc_bicg_destroy.argtypes = [C.c_void_p]
+    L.orc_bicg_set_tolerance.argtypes = [C.c_void_p, C.c_double]
+    L.orc_bicg_set_maxit.argtypes = [C.c_void_p, C.c_int]
+    L.orc_bicg_solve.argtypes = [C.c_void_p, dp, dp, C.POINTER(C.c_int)]
+    L.orc_bicg_solve.restype = C.c_double
+    L.orc_bicg_restarts.argtypes = [C.c_void_p]
+    L.orc_bicg_resets.argtypes = [C.c_void_p]
+    L.orc_rbsor.argtypes = [C.c_size_t, G, ip, dp, dp, dp, C.c_double, C.c_double, C.c_int, C.c_int,
+                            C.POINTER(C.c_int)]
+    L.orc_rbsor.restype = C.c_double
+    L.orc_sor_lex.argtypes = [G, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp,
+                              C.POINTER(C.c_int)]
+    L.orc_sor_lex.restype = C.c_double
+    L.orc_sim_create.argtypes = [G, C.POINTER(CfdParams)]
+    L.orc_sim_create.restype = C.c_void_p
+    L.orc_sim_destroy.argtypes = [C.c_void_p]
+    L.orc_sim_field.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_size_t)]
+    L.orc_sim_field.restype = C.POINTER(C.c_double)
+    L.orc_sim_ell_col.argtypes = [C.c_void_p]
+    L.orc_sim_ell_col.restype = C.POINTER(C.c_int32)
+    L.orc_sim_init_state.argtypes = [C.c_void_p, C.c_double, C.c_uint64]
+    L.orc_sim_setup.argtypes = [C.c_void_p]
+    L.orc_sim_step.argtypes = [C.c_void_p, C.POINTER(CfdStepStats)]
+    L.orc_sim_loop.argtypes = [C.c_void_p]
+    L.orc_sim_set_loop.argtypes = [C.c_void_p, C.c_int, C.c_double]
+
+
+class Grid:
+    """Grid metrics held as numpy arrays plus the cfd_grid_desc pointing at them."""
+
+    def __init__(self, arrays: GridArrays):
+        self.a = arrays
+        self.desc = arrays.desc()
+        self.nxc, self.nyc, self.nzc = arrays.ncal
+        self.nx, self.ny, self.nz = self.nxc + 4, self.nyc + 4, self.nzc + 4
+        self.G = self.nx * self.ny * self.nz
+        self.N = self.nxc * self.nyc * self.nzc
+
+    @property
+    def ref(self):
+        return C.byref(self.desc)
+
+
+def oracle_axis_uniform(ncal, L):
+    n = ncal + 4
+    X, D, Ds = np.zeros(n + 1), np.zeros(n), np.zeros(n)
+    lib().orc_axis_uniform(ncal, L, X, D, Ds)
+    return X, D, Ds
+
+
+def oracle_axis_sml(ncal, L, lS, nS, off):
+    n = ncal + 4
+    X, D, Ds = np.zeros(n + 1), np.zeros(n), np.zeros(n)
+    rc = lib().orc_axis_sml(ncal, L, lS, nS, off, X, D, Ds)
+    if rc != 0:
+        raise ValueError("unit_SML rejected its arguments")
+    return X, D, Ds
+
+
+def oracle_axis_tanh(ncal, L, xc, h0, band, ratio):
+    n = ncal + 4
+    X, D, Ds = np.zeros(n + 1), np.zeros(n), np.zeros(n)
+    rc = lib().orc_axis_tanh(ncal, L, xc, h0, band, ratio, X, D, Ds)
+    if rc != 0:
+        raise ValueError("stretched axis rejected its arguments")
+    return X, D, Ds
+
+
+def ints6(v):
+    return (C.c_int * 6)(*[int(x) for x in v])
+
+
+class Sim:
+    """The oracle's runSeqOmp_predictionMethod loop."""
+
+    def __init__(self, grid: Grid, params: CfdParams):
+        self.grid = grid
+        self.params = params
+        self.h = lib().orc_sim_create(grid.ref, C.byref(params))
+        if not self.h:
+            raise MemoryError("orc_sim_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_sim_destroy(self.h)
+            self.h = None
+
+    def field(self, which: int) -> np.ndarray:
+        n = C.c_size_t()
+        ptr = lib().orc_sim_field(self.h, which, C.byref(n))
+        return np.ctypeslib.as_array(ptr, shape=(n.value,))
+
+    def ell(self):
+        N = self.grid.N
+        col = np.ctypeslib.as_array(lib().orc_sim_ell_col(self.h), shape=(7 * N,))
+        return col
+
+    def init_state(self, noise=0.0, seed=1):
+        lib().orc_sim_init_state(self.h, noise, seed)
+
+    def setup(self):
+        if lib().orc_sim_setup(self.h) != 0:
+            raise RuntimeError("orc_sim_setup failed")
+
+    def step(self) -> CfdStepStats:
+        st = CfdStepStats()
+        lib().orc_sim_step(self.h, C.byref(st))
+        return st
+
+    def set_loop(self, loop, time):
+        lib().orc_sim_set_loop(self.h, loop, time)

@@ -1,0 +1,269 @@
+"""HIP library against the CPU oracle, function by function and step by step.
+
+Bars: bit-exact for the ELL assembly, the IB mask, the SpMVs and red-black
+SOR (same FMA chains on both sides); 1e-13 relative L2 for the predictor
+(FMA contraction of a different compiler); 1e-10 relative L2 for pressure
+solves and whole timesteps (the FP64 bar of the north star).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from helpers import bc_tables, pd6, random_state, rel_l2
+from cfd_nssolver_forhpc_amd import capi, configs
+from cfd_nssolver_forhpc_amd.solver import Simulation, native_grid, uniform_grid, axis_sml
+from cfd_nssolver_forhpc_amd.capi import GridArrays
+
+pytestmark = pytest.mark.gpu
+
+
+def sml_small():
+    ax = axis_sml(30, 8.0, 2.4, 12, -2)
+    ay = axis_sml(22, 4.0, 1.2, 10, 0)
+    az = axis_sml(8, 2.0, 1.0, 0, 0)
+    return GridArrays((30, 22, 8), ax[0], ay[0], az[0], ax[1], ay[1], az[1], ax[2], ay[2], az[2], 8.0, 4.0, 2.0)
+
+
+GRIDS = {
+    "uniform": lambda: uniform_grid((24, 16, 8), (6.0, 4.0, 2.0)),
+    "sml": sml_small,
+}
+
+
+def make_sim(grid, quirk=1, pd=(0, 0, 0, 0, 0, 0), op=capi.CFD_OP_ELL, tol=1e-12, inner=3):
+    p = configs.cylinder_params(100.0, 0.002, grid.lx / 3.0, grid.ly / 2.0, 1.0, tol)
+    p.source_index_quirk = quirk
+    for f in range(6):
+        p.p_dirichlet[f] = pd[f]
+    p.op = op
+    p.inner_loops = inner
+    return Simulation(grid, p), p
+
+
+@pytest.mark.parametrize("gname", list(GRIDS))
+def test_eta_and_ell_bit_exact(require_gpu, gname):
+    grid = GRIDS[gname]()
+    sim, p = make_sim(grid, pd=(0, 1, 0, 0, 0, 0))
+    sim.init_state()
+    sim.DFIB_CylinderZ()
+    sim.createPressureMatrix()
+    sim.setupPressure()
+    g = O.Grid(grid)
+    eta = np.zeros(g.G)
+    O.lib().orc_eta_cylinder_z(g.ref, p.ib_cx, p.ib_cy, p.ib_r, p.ib_nsub, eta)
+    assert np.array_equal(sim.download(capi.ETA), eta)
+    assert (eta > 0).sum() > 0 and (eta == 1.0).sum() > 0
+    col, val = sim.ell()
+    ocol = np.zeros(7 * g.N, np.int32)
+    oval = np.zeros(7 * g.N)
+    ojp = np.zeros(g.N)
+    assert O.lib().orc_ell_assemble(g.ref, pd6(p), ocol, oval, ojp) == 0
+    assert np.array_equal(col, ocol)
+    assert np.array_equal(val, oval)
+    assert np.array_equal(sim.download(capi.JP), ojp)
+
+
+def test_native_grid_ell_and_eta(require_gpu):
+    """The reference's own case; the oracle side is pinned to the compiled reference."""
+    grid = native_grid()
+    p = capi.CfdParams()
+    from cfd_nssolver_forhpc_amd.solver import default_params
+    p = default_params()
+    sim = Simulation(grid, p)
+    sim.init_state()
+    sim.DFIB_CylinderZ()
+    sim.createPressureMatrix()
+    g = O.Grid(grid)
+    eta = np.zeros(g.G)
+    O.lib().orc_eta_cylinder_z(g.ref, 6.5, 6.0, 0.5, 100, eta)
+    assert np.array_equal(sim.download(capi.ETA), eta)
+    col, val = sim.ell()
+    ocol = np.zeros(7 * g.N, np.int32)
+    oval = np.zeros(7 * g.N)
+    ojp = np.zeros(g.N)
+    O.lib().orc_ell_assemble(g.ref, pd6(p), ocol, oval, ojp)
+    assert np.array_equal(col, ocol) and np.array_equal(val, oval)
+
+
+@pytest.mark.parametrize("gname", list(GRIDS))
+def test_spmv_bit_exact(require_gpu, gname):
+    grid = GRIDS[gname]()
+    sim, p = make_sim(grid)
+    sim.init_state()
+    sim.createPressureMatrix()
+    sim.setupPressure()
+    g = O.Grid(grid)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(g.N)
+    sim.upload(capi.X, x)
+    col, val = sim.ell()
+    y = np.zeros(g.N)
+    O.lib().orc_spmv_ell(g.N, 7, col, val, x, y)
+    sim.spmv(capi.X, capi.VEC_A, op=capi.CFD_OP_ELL)
+    assert np.array_equal(sim.download(capi.VEC_A), y)
+    coef = np.zeros(2 * (g.nxc + g.nyc + g.nzc))
+    jp = np.zeros(g.N)
+    O.lib().orc_spe_setup(g.ref, pd6(p), coef, jp)
+    O.lib().orc_spmv_spe(g.ref, coef, jp, x, y)
+    sim.spmv(capi.X, capi.VEC_B, op=capi.CFD_OP_SPE)
+    assert np.array_equal(sim.download(capi.VEC_B), y)
+
+
+@pytest.mark.parametrize("gname", list(GRIDS))
+def test_velocity_phases(require_gpu, gname):
+    grid = GRIDS[gname]()
+    sim, p = make_sim(grid, quirk=1)
+    sim.init_state()
+    sim.DFIB_CylinderZ()
+    sim.createPressureMatrix()
+    g = O.Grid(grid)
+    T0 = random_state(grid, 11)
+    T1 = random_state(grid, 12)
+    for c in range(3):
+        sim.upload(capi.T0_U + c, T0[c])
+        sim.upload(capi.T1_U + c, T1[c])
+    # predictor
+    sim.ConvectionDifussion()
+    o1 = [t.copy() for t in T1]
+    O.lib().orc_predictor(g.ref, 1.0 / p.Re, p.dt, *T0, *o1)
+    for c in range(3):
+        assert rel_l2(sim.download(capi.T1_U + c), o1[c]) < 1e-13
+    # source add with the reference's index mapping
+    rng = np.random.default_rng(5)
+    src = 1e-3 * rng.standard_normal(3 * g.N)
+    sim.upload(capi.SRC, src)
+    for c in range(3):
+        sim.upload(capi.T1_U + c, o1[c])
+    sim.add_source()
+    O.lib().orc_source_add(g.ref, 1, src, *o1)
+    for c in range(3):
+        assert np.array_equal(sim.download(capi.T1_U + c), o1[c])
+    # RHS
+    sim.createBMatrix()
+    b = np.zeros(g.N)
+    O.lib().orc_rhs(g.ref, p.dt, sim.download(capi.JP), *o1, b)
+    assert rel_l2(sim.download(capi.RHS), b) < 1e-15
+    # DFIB update from a given pressure
+    pres = rng.standard_normal(g.G)
+    sim.upload(capi.P, pres)
+    eta = sim.download(capi.ETA)
+    T3 = random_state(grid, 13)
+    for c in range(3):
+        sim.upload(capi.T3_U + c, T3[c])
+    f0 = rng.standard_normal(3 * g.N)
+    sim.upload(capi.F, f0)
+    sim.update_UandF()
+    f = f0.copy()
+    O.lib().orc_dfib_update(g.ref, p.dt, eta, pres, *o1, *T3, f)
+    for c in range(3):
+        assert rel_l2(sim.download(capi.T3_U + c), T3[c]) < 1e-15
+    assert rel_l2(sim.download(capi.F), f) < 1e-13
+    # drag
+    cd, cl = sim.drag()
+    ocd, ocl = C.c_double(), C.c_double()
+    O.lib().orc_drag(g.ref, f, C.byref(ocd), C.byref(ocl))
+    assert abs(cd - ocd.value) <= 1e-12 * max(1.0, abs(ocd.value))
+    assert abs(cl - ocl.value) <= 1e-12 * max(1.0, abs(ocl.value))
+    # source accumulate
+    sim.upload(capi.SRC, src)
+    sim.accumulate_source()
+    s2 = src.copy()
+    O.lib().orc_source_accumulate(g.ref, p.dt, f, s2)
+    assert rel_l2(sim.download(capi.SRC), s2) < 1e-15
+    # BC main + copy
+    Dir, Num = bc_tables(p)
+    sim.BC_staggered_main()
+    T0o = [t.copy() for t in T0]
+    O.lib().orc_bc_main(g.ref, C.byref(Dir), C.byref(Num), pd6(p), *T0o, pres)
+    for c in range(3):
+        assert np.array_equal(sim.download(capi.T0_U + c), T0o[c])
+    assert np.array_equal(sim.download(capi.P), pres)
+    sim.BC_staggered_copy()
+    O.lib().orc_bc_copy(g.ref, *T0o, *o1)
+    for c in range(3):
+        assert np.array_equal(sim.download(capi.T1_U + c), o1[c])
+    # max difference
+    md = sim.CheckSteadyStatebyMaxVal()
+    om = (C.c_double * 3)()
+    O.lib().orc_maxdiff(g.ref, *T0o, *T3, C.byref(om))
+    assert np.allclose(md, list(om), rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("op", [capi.CFD_OP_ELL, capi.CFD_OP_SPE])
+def test_bicgstab_matches_oracle(require_gpu, op):
+    grid = GRIDS["sml"]()
+    sim, p = make_sim(grid, pd=(0, 1, 0, 0, 0, 0), op=op)
+    sim.init_state()
+    sim.createPressureMatrix()
+    sim.setupPressure()
+    g = O.Grid(grid)
+    rng = np.random.default_rng(9)
+    b = rng.standard_normal(g.N)
+    sim.upload(capi.RHS, b)
+    sim.upload(capi.X, np.zeros(g.N))
+    it, rel = sim.bicgstab(tol=1e-12, maxit=5000)
+    x = sim.download(capi.X)
+    col, val = sim.ell()
+    coef = np.zeros(2 * (g.nxc + g.nyc + g.nzc))
+    jp = np.zeros(g.N)
+    O.lib().orc_spe_setup(g.ref, pd6(p), coef, jp)
+    s = O.lib().orc_bicg_create(g.N, op, col, val, g.ref, coef.ctypes.data, jp.ctypes.data)
+    O.lib().orc_bicg_set_tolerance(s, 1e-12)
+    ox = np.zeros(g.N)
+    oit = C.c_int()
+    orel = O.lib().orc_bicg_solve(s, b, ox, C.byref(oit))
+    O.lib().orc_bicg_destroy(s)
+    assert rel < 1e-12 and orel < 1e-12
+    assert abs(it - oit.value) <= max(3, oit.value // 10)
+    assert rel_l2(x, ox) < 1e-10
+
+
+def test_rbsor_matches_oracle(require_gpu):
+    grid = GRIDS["uniform"]()
+    sim, p = make_sim(grid, pd=(0, 1, 0, 0, 0, 0))
+    sim.init_state()
+    sim.createPressureMatrix()
+    g = O.Grid(grid)
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal(g.N)
+    sim.upload(capi.RHS, b)
+    sim.upload(capi.X, np.zeros(g.N))
+    it, rel = sim.rbsor(omega=1.8, tol=1e-300, maxit=40, check_every=10)
+    col, val = sim.ell()
+    ox = np.zeros(g.N)
+    oit = C.c_int()
+    orel = O.lib().orc_rbsor(g.N, g.ref, col, val, b, ox, 1.8, 1e-300, 40, 10, C.byref(oit))
+    assert it == oit.value == 40
+    assert np.array_equal(sim.download(capi.X), ox)
+    assert abs(rel - orel) <= 1e-12 * orel
+
+
+@pytest.mark.parametrize("quirk,inner,op", [(1, 3, capi.CFD_OP_ELL), (0, 3, capi.CFD_OP_ELL),
+                                            (1, 0, capi.CFD_OP_ELL), (1, 3, capi.CFD_OP_SPE)])
+def test_steps_match_oracle(require_gpu, quirk, inner, op):
+    grid = GRIDS["sml"]()
+    pd = (0, 1, 0, 0, 0, 0)
+    sim, p = make_sim(grid, quirk=quirk, pd=pd, op=op, tol=1e-13, inner=inner)
+    sim.setup(noise=1e-3, seed=3)
+    osim = O.Sim(O.Grid(grid), p)
+    osim.init_state(1e-3, 3)
+    osim.setup()
+    for c in range(3):
+        assert np.array_equal(sim.download(capi.T0_U + c), osim.field(capi.T0_U + c))
+    for k in range(4):
+        st = sim.step()
+        ost = osim.step()
+        for c in range(3):
+            e = rel_l2(sim.download(capi.T0_U + c), osim.field(capi.T0_U + c))
+            assert e < 1e-10, (k, c, e)
+        assert rel_l2(sim.download(capi.X), osim.field(capi.X)) < 1e-10
+        assert rel_l2(sim.download(capi.P), osim.field(capi.P)) < 1e-10
+        assert rel_l2(sim.download(capi.F), osim.field(capi.F)) < 1e-9
+        for q in range(st.n_passes):
+            assert abs(st.cd[q] - ost.cd[q]) <= 1e-9 * max(1.0, abs(ost.cd[q]))
+        for c in range(3):
+            assert abs(st.maxdiff[c] - ost.maxdiff[c]) <= 1e-10 * max(1e-3, abs(ost.maxdiff[c]))
