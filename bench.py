@@ -2,14 +2,17 @@
 
 Workload: 3-D flow past a z-cylinder, 512x256x64 uniform cells over
 20D x 10D x 2.5D (D = 25 cells), Re 200, dt 5e-4, pressure by BiCGSTAB on the
-Jacobi-scaled ELL 7-point operator to ||b - Ax|| / ||b|| < 1e-10 (FP64), and
-the same run with FP32 inner / FP64 defect-correction pressure solves.
+Jacobi-scaled ELL 7-point operator to ||b - Ax|| / ||b|| < 1e-10 (FP64).
 A step is one iteration of the reference's main loop: 3 predictor passes and
 a corrector, each with its pressure solve.
 
-Prints one JSON line.  `value` is FP64 timesteps per second; the mixed run,
-the ELL SpMV roofline and the CPU baseline (the FP64 oracle on the host's
-cores, a bounded sample of the same workload) are reported beside it.
+Prints one JSON line.  `value` is FP64 timesteps per second.  Reported beside
+it: the same run with mixed-precision pressure solves and with the
+reference's default matrix-free (SPE) operator, the standalone Poisson
+benchmark on that grid (seeded zero-mean N(0,1) RHS; BiCGSTAB FP64 and mixed,
+red-black SOR omega 1.8), the ELL SpMV roofline measured with HIP events on
+the library's stream, and the CPU baseline (the FP64 oracle on the host's
+cores, a bounded sample of the same workload).
 """
 from __future__ import annotations
 
@@ -31,31 +34,67 @@ from cfd_nssolver_forhpc_amd import capi, configs  # noqa: E402
 from cfd_nssolver_forhpc_amd.solver import Simulation  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-SPMV_BYTES_PER_ROW = {"fp64": 100, "fp32": 64}  # 7 x (val + col) + x + y
+# algorithmic bytes per row: ELL SpMV 7 x (val + col) + x + y; SPE x + jp + y;
+# BiCGSTAB iteration (5 fused kernels) 320; RB-SOR sweep 84 + x + b + x(w) + x(nbr)
+BYTES = {"spmv_fp64": 100, "spmv_fp32": 64, "spe_fp64": 24, "bicg_iter": 320, "sor_sweep": 116,
+         "copy": 16, "predictor": 48}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_steps(w, warmup, steps):
+def run_steps(w, warmup, steps, tag):
     sim = Simulation(w.grid, w.params)
-    t0 = time.perf_counter()
     sim.setup(noise=w.noise, seed=w.seed)
-    setup_s = time.perf_counter() - t0
-    stats = []
     for k in range(warmup):
         st = sim.step()
-        log(f"  warmup step {k + 1}: iters {list(st.iters[:st.n_passes])} inner {list(st.inner_iters[:st.n_passes])}")
+        log(f"  [{tag}] warmup step {k + 1}: iters {list(st.iters[:st.n_passes])}")
+    stats = []
     t0 = time.perf_counter()
-    for k in range(steps):
-        st = sim.step()
-        stats.append(st.as_dict())
+    for _ in range(steps):
+        stats.append(sim.step().as_dict())
     dt = time.perf_counter() - t0
     for k, s in enumerate(stats):
-        log(f"  step {warmup + k + 1}: iters {s['iters']} inner {s['inner_iters']} relres {max(s['relres']):.2e} "
-            f"cD {s['cd'][-1]:.5f}")
-    return sim, dt, stats, setup_s
+        log(f"  [{tag}] step {warmup + k + 1}: iters {s['iters']} inner {s['inner_iters']} "
+            f"relres {max(s['relres']):.2e} cD {s['cd'][-1]:.4f}")
+    return sim, dt, stats
+
+
+def per_step(stats, key):
+    return float(np.mean([sum(max(i, 0) for i in s[key]) for s in stats]))
+
+
+def poisson_benchmark(w):
+    """Standalone Poisson solve on the 512x256x64 grid: ELL operator with
+    Neumann walls and p = 0 on the outflow face, seeded zero-mean N(0,1) RHS."""
+    out = {}
+    rng = np.random.default_rng(2)
+    b = rng.standard_normal(w.grid.N)
+    b -= b.mean()
+    for tag, prec, solver in (("bicgstab_fp64", capi.CFD_FP64, 0), ("bicgstab_mixed", capi.CFD_MIXED, 0),
+                              ("rbsor_fp64", capi.CFD_FP64, 1)):
+        p = w.params.copy()
+        p.precision, p.solver, p.maxit = prec, solver, 20000
+        sim = Simulation(w.grid, p)
+        sim.init_state()
+        sim.createPressureMatrix()
+        sim.upload(capi.RHS, b)
+        sim.upload(capi.X, np.zeros(w.grid.N))
+        t0 = time.perf_counter()
+        if solver == 1:
+            it, rel = sim.rbsor(omega=1.8, tol=1e-10, maxit=2000, check_every=100)
+        else:
+            it, rel = sim.solve()
+        dt = time.perf_counter() - t0
+        rec = {"seconds": dt, "iters": it, "relres": rel}
+        if solver == 1:
+            rec["note"] = "capped at 2000 sweeps (omega 1.8 needs ~1e6 sweeps for 1e-10 at this size)"
+            rec["us_per_sweep"] = dt / max(it, 1) * 1e6
+        out[tag] = rec
+        log(f"[poisson] {tag}: {it} iters, relres {rel:.2e}, {dt:.2f} s")
+        sim.close()
+    return out
 
 
 def cpu_baseline(w, iters_per_step, threads):
@@ -64,10 +103,10 @@ def cpu_baseline(w, iters_per_step, threads):
     a fixed count of BiCGSTAB iterations on the same ELL operator; the step
     time is t_nonsolver + iters_per_step * t_iteration."""
     sys.path.insert(0, str(ROOT / "tests"))
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     import oracle_lib as O
+    flags = "-O3 -march=x86-64-v3 -fopenmp"
     lib = None
-    kind = "port"
     try:
         subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "native"], check=True,
                        stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=120)
@@ -75,7 +114,6 @@ def cpu_baseline(w, iters_per_step, threads):
         flags = "-O3 -march=native -fopenmp"
     except Exception:
         lib = O.lib()
-        flags = "-O3 -march=x86-64-v3 -fopenmp"
     O._LIB = lib
     g = O.Grid(w.grid)
     sim = O.Sim(g, w.params)
@@ -89,8 +127,7 @@ def cpu_baseline(w, iters_per_step, threads):
     val = np.zeros(7 * N)
     jp = np.zeros(N)
     lib.orc_ell_assemble(g.ref, O.ints6([w.params.p_dirichlet[f] for f in range(6)]), col, val, jp)
-    rng = np.random.default_rng(2)
-    b = rng.standard_normal(N)
+    b = np.random.default_rng(2).standard_normal(N)
     x = np.zeros(N)
     s = lib.orc_bicg_create(N, 0, col, val, g.ref, None, None)
     nit = 20
@@ -105,14 +142,31 @@ def cpu_baseline(w, iters_per_step, threads):
     return {
         "value": 1.0 / step_s,
         "unit": "timesteps/s",
-        "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
-        "kind": kind,
-        "sample": (f"FP64 oracle ({flags}) at 512x256x64: one PC-method step without solves "
-                   f"({t_nonsolver:.2f} s) + {nit} BiCGSTAB iterations ({t_iter * 1e3:.1f} ms each) x "
-                   f"{iters_per_step:.0f} iterations per step (the GPU run's count)"),
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"FP64 oracle ({flags}, {threads} OpenMP threads) at 512x256x64: one PC-method step without "
+                   f"solves ({t_nonsolver:.2f} s) + {nit} BiCGSTAB iterations ({t_iter * 1e3:.1f} ms each), "
+                   f"step = non-solver + {iters_per_step:.0f} iterations (the GPU run's count per step)"),
         "t_nonsolver_s": t_nonsolver,
         "t_iteration_ms": t_iter * 1e3,
+        "host": os.uname().nodename,
     }
+
+
+def kernel_table(sim, reps):
+    N = sim.N
+    rows = {}
+    for what, name in ((0, "spmv_fp64"), (1, "spmv_fp32"), (2, "spe_fp64"), (6, "copy"), (5, "predictor"),
+                       (4, "bicg_iter"), (3, "sor_sweep")):
+        try:
+            ms = sim.time_kernel(what, reps if what != 4 else max(10, reps // 4))
+        except Exception as e:  # an operator not assembled for this context
+            log(f"[bench] kernel {name}: {e}")
+            continue
+        gbs = BYTES[name] * N / (ms * 1e-3) / 1e9
+        rows[name] = {"avg_us": ms * 1e3, "gbs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS}
+        log(f"[bench] {name:10s} {ms * 1e3:9.1f} us  {gbs:7.0f} GB/s  ({gbs / HBM_PEAK_GBS:.2f} of peak)")
+    return rows
 
 
 def main():
@@ -121,52 +175,70 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-spe", action="store_true")
+    ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--spmv-reps", type=int, default=200)
+    ap.add_argument("--reps", type=int, default=200)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="time the operator kernels on the 512x256x64 grid and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
-              "(the path runs on a single MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
+              "(the hot path runs on a single MI355X; multi-GPU x-slab sharding is design-only)", file=sys.stderr)
         sys.exit(2)
     warmup = max(args.warmup, 2)  # step 1 of the reference loop has no pressure solve
 
     w64 = configs.config_b(capi.CFD_FP64)
-    log(f"[bench] FP64 {w64.name}: warmup {warmup}, timed {args.steps}")
-    sim64, t64, st64, setup_s = run_steps(w64, warmup, args.steps)
+    if args.kernels_only:
+        # operator kernels alone on the 512x256x64 grid (for counter collection)
+        sim = Simulation(w64.grid, w64.params)
+        sim.init_state()
+        sim.createPressureMatrix()
+        sim.setupPressure()
+        rng = np.random.default_rng(2)
+        sim.upload(capi.X, rng.standard_normal(sim.N))
+        sim.upload(capi.RHS, rng.standard_normal(sim.N))
+        print(json.dumps({"kernels": kernel_table(sim, args.reps)}), flush=True)
+        return
+    log(f"[bench] {w64.name}: warmup {warmup}, timed {args.steps}")
+    sim64, t64, st64 = run_steps(w64, warmup, args.steps, "fp64")
     tps64 = args.steps / t64
-    iters64 = float(np.mean([sum(max(i, 0) for i in s["iters"]) for s in st64]))
+    iters64 = per_step(st64, "iters")
 
-    # roofline of the dominant kernel: ELL SpMV, FP64, on the context stream
-    N = sim64.N
-    ms_spmv = sim64.time_kernel(0, args.spmv_reps)
-    ms_spmv32 = sim64.time_kernel(1, args.spmv_reps)
-    ms_copy = sim64.time_kernel(6, args.spmv_reps)
-    ms_iter = sim64.time_kernel(4, 50)
-    gbs = SPMV_BYTES_PER_ROW["fp64"] * N / (ms_spmv * 1e-3) / 1e9
-    gbs32 = SPMV_BYTES_PER_ROW["fp32"] * N / (ms_spmv32 * 1e-3) / 1e9
-    gbs_copy = 16 * N / (ms_copy * 1e-3) / 1e9
-    gbs_iter = 320 * N / (ms_iter * 1e-3) / 1e9
-    log(f"[bench] SpMV fp64 {ms_spmv * 1e3:.1f} us = {gbs:.0f} GB/s; fp32 {ms_spmv32 * 1e3:.1f} us = {gbs32:.0f} GB/s; "
-        f"copy {gbs_copy:.0f} GB/s; BiCGSTAB iteration {ms_iter * 1e3:.1f} us = {gbs_iter:.0f} GB/s")
+    # kernel roofline table; the dominant kernel is the ELL SpMV (FP64)
+    sim64.setupPressure()
+    kt = kernel_table(sim64, args.reps)
+    spmv = kt["spmv_fp64"]
 
     mixed = None
     if not args.no_mixed:
         wm = configs.config_b(capi.CFD_MIXED)
-        log(f"[bench] mixed {wm.name}")
-        simm, tm, stm, _ = run_steps(wm, warmup, args.steps)
+        simm, tm, stm = run_steps(wm, warmup, args.steps, "mixed")
         err = {}
         for name, fid in (("u", capi.T0_U), ("v", capi.T0_V), ("w", capi.T0_W), ("p", capi.X)):
             a, b = simm.download(fid), sim64.download(fid)
             err[name] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
         mixed = {
+            "scheme": "FP64 BiCGSTAB on FP32 ELL values (FP64 accumulation), FP64 residual replacement at 10x drops",
             "timesteps_per_s": args.steps / tm,
             "speedup_vs_fp64": (args.steps / tm) / tps64,
-            "outer_iters_per_step": float(np.mean([sum(max(i, 0) for i in s["iters"]) for s in stm])),
-            "inner_iters_per_step": float(np.mean([sum(s["inner_iters"]) for s in stm])),
-            "rel_l2_vs_fp64_after_steps": err,
-            "final_outer_relres": max(max(s["relres"]) for s in stm),
+            "replacements_per_step": per_step(stm, "iters"),
+            "iterations_per_step": per_step(stm, "inner_iters"),
+            "max_final_relres": max(max(s["relres"]) for s in stm),
+            "rel_l2_vs_fp64_after_timed_steps": err,
         }
         simm.close()
+
+    spe = None
+    if not args.no_spe:
+        ws = configs.config_b(capi.CFD_FP64)
+        ws.params.op = capi.CFD_OP_SPE
+        sims, ts, sts = run_steps(ws, warmup, args.steps, "spe")
+        spe = {"operator": "matrix-free SPE (the reference's default P_SOLVER_BICG_SPE)",
+               "timesteps_per_s": args.steps / ts, "bicgstab_iters_per_step": per_step(sts, "iters")}
+        sims.close()
+
+    poisson = None if args.no_poisson else poisson_benchmark(w64)
 
     cpu = None
     if not args.no_cpu:
@@ -174,9 +246,10 @@ def main():
             threads = len(os.sched_getaffinity(0))
         except Exception:
             threads = os.cpu_count() or 1
-        threads = min(threads, 16)
+        threads = max(1, min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)), 16))
         log(f"[bench] CPU baseline on {threads} threads")
         cpu = cpu_baseline(w64, iters64, threads)
+        cpu["gpu_speedup"] = tps64 / cpu["value"]
 
     out = {
         "metric": "timesteps/s @512x256x64 (mixed vs FP64, same residual); Poisson SpMV HBM GB/s",
@@ -190,17 +263,17 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded uniform +-1e-3 noise on the initial velocity)",
+        "data": "synthetic (uniform inflow, seeded +-1e-3 noise on the initial velocity; seeded N(0,1) Poisson RHS)",
         "config": {"workload": w64.name, "grid": [512, 256, 64], "Re": 200, "dt": 5e-4,
-                   "pressure": "BiCGSTAB, ELL, tol 1e-10", "passes_per_step": 4,
-                   "bicgstab_iters_per_step": iters64},
-        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "ELL SpMV FP64 (100 B/row algorithmic)", "avg_us": ms_spmv * 1e3},
-        "spmv_fp32_gbs": gbs32,
-        "copy_gbs": gbs_copy,
-        "bicgstab_iteration_gbs": gbs_iter,
+                   "pressure": "BiCGSTAB (bicgstabRe2), Jacobi-scaled ELL, tol 1e-10, maxit 5000",
+                   "passes_per_step": 4, "bicgstab_iters_per_step": iters64},
+        "roofline": {"bound": "hbm", "achieved": spmv["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": spmv["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "ELL SpMV FP64 (100 B/row x 8388608 rows per launch)", "avg_us": spmv["avg_us"]},
+        "kernels": kt,
         "mixed": mixed,
+        "spe": spe,
+        "poisson": poisson,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -30,6 +30,9 @@ CFD_OP_ELL = 0
 CFD_OP_SPE = 1
 CFD_FP64 = 0
 CFD_MIXED = 1
+CFD_MIXED_RELIABLE = 0
+CFD_MIXED_RESTARTED = 1
+CFD_MIXED_VALUES32 = 2
 
 # field ids
 T0_U, T0_V, T0_W, T1_U, T1_V, T1_W, T3_U, T3_V, T3_W = range(9)
@@ -58,7 +61,8 @@ class CfdParams(C.Structure):
         ("source_index_quirk", C.c_int),
         ("inner_loops", C.c_int),
         ("solver", C.c_int), ("op", C.c_int), ("precision", C.c_int),
-        ("tol", C.c_double), ("tol_inner", C.c_double), ("maxit", C.c_int),
+        ("tol", C.c_double), ("tol_inner", C.c_double), ("mixed_mode", C.c_int), ("mixed_delta", C.c_double),
+        ("maxit", C.c_int),
         ("sor_omega", C.c_double), ("sor_check", C.c_int),
         ("ib_on", C.c_int), ("ib_cx", C.c_double), ("ib_cy", C.c_double), ("ib_r", C.c_double),
         ("ib_nsub", C.c_int),

@@ -46,6 +46,11 @@ def cylinder_params(Re: float, dt: float, cx: float, cy: float, D: float, tol: f
     p.precision = precision
     p.tol = tol
     p.tol_inner = 1e-5
+    # FP32 operator values, FP64 Krylov vectors, FP64 residual replacement:
+    # the scheme that stays convergent at 8M+ unknowns (FP32 Krylov vectors
+    # stall near 3e-4 and diverge there; see the mixed-precision notes)
+    p.mixed_mode = capi.CFD_MIXED_VALUES32
+    p.mixed_delta = 0.1
     p.maxit = 5000
     p.ib_on = 1
     p.ib_cx, p.ib_cy, p.ib_r = cx, cy, 0.5 * D

@@ -6,6 +6,7 @@
 // sequencing of the fused timestep.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -63,6 +64,12 @@ int check_params(const cfd_params *p) {
   if (p->solver == CFD_SOLVER_RBSOR && p->op != CFD_OP_ELL)
     return fail(CFD_ERR_ARG, "red-black SOR runs on the ELL operator");
   if (p->maxit < 0) return fail(CFD_ERR_ARG, "maxit must be >= 0");
+  if (p->precision == CFD_MIXED && p->mixed_mode != CFD_MIXED_RELIABLE && p->mixed_mode != CFD_MIXED_RESTARTED &&
+      p->mixed_mode != CFD_MIXED_VALUES32)
+    return fail(CFD_ERR_ARG, "unknown mixed-precision scheme");
+  if (p->precision == CFD_MIXED && p->mixed_mode != CFD_MIXED_RESTARTED &&
+      !(p->mixed_delta > 0.0 && p->mixed_delta < 1.0))
+    return fail(CFD_ERR_ARG, "mixed_delta must be in (0, 1)");
   if (p->ib_on && (p->ib_nsub < 1 || !(p->ib_r > 0.0))) return fail(CFD_ERR_ARG, "invalid immersed body");
   return CFD_OK;
 }
@@ -181,7 +188,10 @@ int solve_configured(Ctx &c, int *iters, double *relres, int *inner) {
     }
     if (rc) return rc;
     int outer = 0;
-    rc = defect_correction(c, c.f[CFD_RHS], c.f[CFD_X], P.tol, P.tol_inner, P.maxit, &outer, inner, relres);
+    if (P.mixed_mode != CFD_MIXED_RESTARTED && P.solver == CFD_SOLVER_BICGSTAB)
+      rc = mixed_reliable_solve(c, c.f[CFD_RHS], c.f[CFD_X], P.tol, P.mixed_delta, P.maxit, &outer, inner, relres);
+    else
+      rc = defect_correction(c, c.f[CFD_RHS], c.f[CFD_X], P.tol, P.tol_inner, P.maxit, &outer, inner, relres);
     *iters = outer;
     return rc;
   }
@@ -230,6 +240,8 @@ int cfd_params_default(cfd_params *out) {
   out->precision = CFD_FP64;
   out->tol = 1e-3;        // simuClass::p_criteria (simuClass.hpp:74)
   out->tol_inner = 1e-5;
+  out->mixed_mode = CFD_MIXED_RELIABLE;
+  out->mixed_delta = 0.1;
   out->maxit = 5000;      // bicgstabRe2::iters_max_ (bicgstabRe2.hpp:57)
   out->sor_omega = 1.8;   // simuClass::p_sor_omega (simuClass.hpp:78)
   out->sor_check = 10;
@@ -382,6 +394,7 @@ int cfd_create(const cfd_grid_desc *grid, const cfd_params *params, int device, 
   Ctx *c = new (std::nothrow) Ctx();
   if (!c) return fail(CFD_ERR_NOMEM, "host allocation failed");
   c->device = device;
+  { const char *dbg = std::getenv("CFD_DEBUG"); c->debug = dbg && dbg[0] == '1'; }
   c->prm = *params;
   DevGrid &g = c->g;
   g.nxc = grid->nx_cal; g.nyc = grid->ny_cal; g.nzc = grid->nz_cal;

@@ -40,7 +40,9 @@ __host__ __device__ inline int64_t icelcal(const DevGrid &g, int i, int j, int k
 struct KrylovScalars {
   double rho, pre_rho, alpha, omega, beta, norm0, rnorm, relres;
   double tol, restart_tol;
-  int iters, maxit, done, hit_maxit, restart, timestep, restarts, pad;
+  double rmax, delta;  // mixed reliable updates: residual at the last replacement, trigger ratio
+  int iters, maxit, done, hit_maxit, restart, timestep, restarts, breakdown;
+  int replace, replacements, pad0, pad1;
 };
 
 // Reduction workspace: per-block partials plus a completion counter per
@@ -79,7 +81,8 @@ struct Ctx {
   double *r0 = nullptr, *r1 = nullptr, *p1 = nullptr, *s1 = nullptr, *ap = nullptr, *as = nullptr;
   double *px1 = nullptr, *px2 = nullptr;
   // FP32 inner-solve vectors (mixed precision) and permuted SOR vectors
-  float *q32[8] = {};
+  float *q32[8] = {};   // FP32 Krylov vectors r0 r1 p1 s1 ap as x r
+  double *xhi = nullptr; // FP64 residual of the mixed reliable-update solve
   double *xP = nullptr, *bP = nullptr, *rP = nullptr;
   float *xP32 = nullptr, *bP32 = nullptr;
   // device scalars and reduction workspace
@@ -95,6 +98,7 @@ struct Ctx {
   double time = 0.0;
   bool p_from_x = false;         // ghosted p interior lags X (step mode)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool debug = false;  // CFD_DEBUG=1: solver progress on stderr
 };
 
 inline double *red_part_host(const Ctx &c, int site) {
@@ -150,6 +154,8 @@ int stream_blocks(int64_t n);
 int bicgstab_solve(Ctx &c, const double *b, double *x, double tol, int maxit, int *iters, double *relres);
 int defect_correction(Ctx &c, const double *b, double *x, double tol, double tol_inner, int max_outer,
                       int *outer, int *inner_total, double *relres);
+int mixed_reliable_solve(Ctx &c, const double *b, double *x, double tol, double delta, int maxit, int *replacements,
+                         int *iters, double *relres);
 hipError_t launch_bicg_iteration(Ctx &c);
 hipError_t launch_bicg_timing_setup(Ctx &c);
 // k_sor.hip

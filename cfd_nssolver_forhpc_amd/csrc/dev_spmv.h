@@ -70,13 +70,28 @@ struct OpArgs {
   int nxc, nyc, nzc;
 };
 
+// internal operator id: the ELL operator with FP32 values applied to an FP64
+// vector (FP64 products and sum)
+constexpr int OP_ELL_VAL32 = 2;
+
 __device__ inline double op_row(const OpArgs &a, int64_t r, const double *__restrict__ x) {
   if (a.op == CFD_OP_SPE) return spe_row(a.coef, a.jp, a.nxc, a.nyc, a.nzc, r, x);
+  if (a.op == OP_ELL_VAL32) return ell_row<float, double>(a.col, a.val32, a.N, r, x);
   return ell_row<double, double>(a.col, a.val, a.N, r, x);
 }
 
+// FP32 values and vector; products are exact in FP64 and summed in FP64, so
+// the cancellation of the nearly row-sum-free operator does not cost FP32
+// digits (a pure FP32 accumulation loses them for smooth x and stalls the
+// Krylov iteration at 1e5+ unknowns).
 __device__ inline float op_row32(const OpArgs &a, int64_t r, const float *__restrict__ x) {
-  return ell_row<float, float>(a.col, a.val32, a.N, r, x);
+  double acc = 0.0;
+#pragma unroll
+  for (int s = 0; s < KSLOTS; ++s) {
+    const int32_t c = ntload(a.col + (int64_t)s * a.N + r);
+    acc = fma((double)ntload(a.val32 + (int64_t)s * a.N + r), (double)x[c], acc);
+  }
+  return (float)acc;
 }
 
 inline OpArgs op_args(const Ctx &c, int op) {

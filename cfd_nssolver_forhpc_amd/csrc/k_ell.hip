@@ -125,13 +125,13 @@ __global__ void __launch_bounds__(256) spmv32_from64_kernel(OpArgs a, const doub
                                                             double *__restrict__ y) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) {
-    float acc = 0.0f;
+    double acc = 0.0;
 #pragma unroll
     for (int s = 0; s < KSLOTS; ++s) {
       const int32_t c = ntload(a.col + (int64_t)s * a.N + r);
-      acc = fmaf(ntload(a.val32 + (int64_t)s * a.N + r), (float)x[c], acc);
+      acc = fma((double)ntload(a.val32 + (int64_t)s * a.N + r), (double)(float)x[c], acc);
     }
-    y[r] = (double)acc;
+    y[r] = acc;
   }
 }
 

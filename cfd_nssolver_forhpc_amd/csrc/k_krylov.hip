@@ -17,6 +17,8 @@
 // batch instead of once per dot product.  FP64 dots are kept for every
 // scalar, also when the vectors are FP32.
 #include <algorithm>
+#include <cstdio>
+#include <type_traits>
 #include <cmath>
 
 #include "dev_common.h"
@@ -402,6 +404,317 @@ hipError_t launch_bicg_timing_setup(Ctx &c) {
                                                 nullptr, nullptr, red_part_host(c, RS_INIT));
   init_finalize<<<1, 256, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, c.ks32);
   return hipGetLastError();
+}
+
+}  // namespace cfd
+
+// ---------------------------------------------------------------------------
+// Mixed precision by reliable updates: BiCGSTAB iterates in FP32 (FP32 ELL
+// values and vectors, FP64 accumulation in the operator and in every dot);
+// the FP32 iterate xl is folded into the FP64 solution x and the FP32
+// residual is replaced by the FP64 true residual b - A64 x whenever the
+// recurrence residual has fallen by `delta` since the last replacement, and
+// at the end.  The search direction and shadow residual are kept across a
+// replacement, so the Krylov space is not restarted; the stop test is on the
+// FP64 true residual, the same criterion as the FP64 solve.
+// ---------------------------------------------------------------------------
+namespace cfd {
+namespace {
+
+__global__ void set_delta_kernel(KrylovScalars *ks, double delta) { ks->delta = delta; }
+
+// r = b - A64 x (FP64); rv = r0 = p = (T) r; xl = 0; partials (b.b), (r.r), (rv.rv)
+template <typename T>
+__global__ void __launch_bounds__(256) mr_init_kernel(OpArgs a64, const double *__restrict__ b,
+                                                      const double *__restrict__ x, T *__restrict__ r32,
+                                                      T *__restrict__ r0, T *__restrict__ p, T *__restrict__ xl,
+                                                      double *part) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a64.N; r += stride) {
+    const double br = b[r];
+    const double rr = br - op_row(a64, r, x);
+    const T rf = (T)rr;
+    r32[r] = rf;
+    r0[r] = rf;
+    p[r] = rf;
+    if (xl) xl[r] = T(0);
+    acc[0] += br * br;
+    acc[1] += rr * rr;
+    acc[2] += (double)rf * (double)rf;
+  }
+  block_partials<3>(acc, part);
+}
+
+__global__ void __launch_bounds__(256) mr_init_finalize(const double *part, int nblk, KrylovScalars *ks) {
+  double tot[3];
+  combine_partials<3>(part, nblk, tot);
+  if (threadIdx.x == 0) {
+    ks->norm0 = sqrt(tot[0]);
+    ks->rmax = sqrt(tot[1]);
+    ks->relres = ks->rmax / ks->norm0;
+    ks->rho = tot[2];
+    ks->replace = 0;
+    ks->replacements = 0;
+    ks->breakdown = 0;
+    if (!(ks->relres >= ks->tol)) ks->done = 1;
+  }
+}
+
+__global__ void __launch_bounds__(256) mr_alpha_finalize(const double *part, int nblk, KrylovScalars *ks) {
+  if (ks->done) return;
+  double tot[1];
+  combine_partials<1>(part, nblk, tot);
+  if (threadIdx.x == 0) {
+    const double a = ks->rho / tot[0];
+    if (!isfinite(a)) {  // breakdown: fold what has been gained and re-anchor on the true residual
+      ks->breakdown += 1;
+      ks->replace = 1;
+      ks->alpha = 0.0;
+    } else {
+      ks->alpha = a;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) mr_omega_finalize(const double *part, int nblk, KrylovScalars *ks) {
+  if (ks->done || ks->replace) return;
+  double tot[2];
+  combine_partials<2>(part, nblk, tot);
+  if (threadIdx.x == 0) {
+    const double w = tot[0] / tot[1];
+    ks->omega = isfinite(w) ? w : 0.0;
+    if (!isfinite(w)) {
+      ks->breakdown += 1;
+      ks->replace = 1;
+    }
+  }
+}
+
+// recurrence residual test: trigger a replacement on a `delta` drop, on the
+// stop criterion, or at the iteration cap; otherwise beta and the restart rule
+__global__ void __launch_bounds__(256) mr_conv_finalize(const double *part, int nblk, KrylovScalars *ks) {
+  if (ks->done || ks->replace) return;
+  double tot[2];
+  combine_partials<2>(part, nblk, tot);
+  if (threadIdx.x == 0) {
+    const double rn = sqrt(tot[0]);
+    ks->rnorm = rn;
+    if (ks->iters >= ks->maxit) ks->hit_maxit = 1;
+    if (rn < ks->delta * ks->rmax || rn / ks->norm0 < ks->tol || ks->hit_maxit || !isfinite(rn)) {
+      ks->replace = 1;
+    } else {
+      ks->pre_rho = ks->rho;
+      ks->rho = tot[1];
+      ks->beta = (ks->alpha / ks->omega) * (ks->rho / ks->pre_rho);
+      ks->restart = (ks->timestep < 10 || fabs(ks->pre_rho) > ks->restart_tol) ? 0 : 1;
+      ks->restarts += ks->restart;
+      ks->iters += 1;
+    }
+  }
+}
+
+// x += xl; xl = 0 (only when a replacement is due)
+__global__ void __launch_bounds__(256) mr_fold_kernel(int64_t N, const KrylovScalars *__restrict__ ks,
+                                                      double *__restrict__ x, float *__restrict__ xl) {
+  if (ks->done || !ks->replace) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += stride) {
+    x[r] += (double)xl[r];
+    xl[r] = 0.0f;
+  }
+}
+
+// rv = (T)(b - A64 x); partials (r.r) in FP64, (rv.r0)
+template <typename T>
+__global__ void __launch_bounds__(256) mr_replace_kernel(OpArgs a64, const KrylovScalars *__restrict__ ks,
+                                                         const double *__restrict__ b, const double *__restrict__ x,
+                                                         T *__restrict__ r32, const T *__restrict__ r0,
+                                                         double *part) {
+  if (ks->done || !ks->replace) return;
+  double acc[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a64.N; r += stride) {
+    const double rr = b[r] - op_row(a64, r, x);
+    const T rf = (T)rr;
+    r32[r] = rf;
+    acc[0] += rr * rr;
+    acc[1] += (double)rf * (double)r0[r];
+  }
+  block_partials<2>(acc, part);
+}
+
+__global__ void __launch_bounds__(256) mr_replace_finalize(const double *part, int nblk, KrylovScalars *ks) {
+  if (ks->done || !ks->replace) return;
+  double tot[2];
+  combine_partials<2>(part, nblk, tot);
+  if (threadIdx.x == 0) {
+    const double rt = sqrt(tot[0]);
+    ks->replacements += 1;
+    ks->relres = rt / ks->norm0;
+    ks->replace = 0;
+    if (ks->relres < ks->tol || ks->hit_maxit || ks->breakdown > 8) {
+      ks->done = 1;
+      return;
+    }
+    const bool broke = !(ks->alpha != 0.0 && ks->omega != 0.0);
+    ks->rmax = rt;
+    ks->pre_rho = ks->rho;
+    ks->rho = tot[1];
+    ks->beta = broke ? 0.0 : (ks->alpha / ks->omega) * (ks->rho / ks->pre_rho);
+    ks->restart = broke ? 1 : ((ks->timestep < 10 || fabs(ks->pre_rho) > ks->restart_tol) ? 0 : 1);
+    ks->restarts += ks->restart;
+    ks->iters += 1;
+  }
+}
+
+// p update, shared by both branches: runs unless the solve is done
+template <typename T>
+__global__ void __launch_bounds__(256) mr_p_update_kernel(int64_t N, const KrylovScalars *__restrict__ ks,
+                                                          const T *__restrict__ r1, T *__restrict__ p1,
+                                                          const T *__restrict__ ap, T *__restrict__ r0) {
+  if (ks->done) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  if (ks->restart) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += stride) {
+      r0[r] = r1[r];
+      p1[r] = r1[r];
+    }
+    return;
+  }
+  const double beta = ks->beta, omega = ks->omega;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += stride)
+    p1[r] = (T)((double)r1[r] + beta * ((double)p1[r] - omega * (double)ap[r]));
+}
+
+// the kernels of one iteration skip their work while a replacement is pending
+template <typename T>
+__global__ void __launch_bounds__(256) mr_s_update_kernel(int64_t N, const KrylovScalars *__restrict__ ks,
+                                                          const T *__restrict__ r1, const T *__restrict__ ap,
+                                                          T *__restrict__ s1) {
+  if (ks->done || ks->replace) return;
+  const double alpha = ks->alpha;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += stride)
+    s1[r] = (T)((double)r1[r] - alpha * (double)ap[r]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) mr_spmv_dot2_kernel(OpArgs a, const KrylovScalars *__restrict__ ks,
+                                                           const T *__restrict__ s1, T *__restrict__ as,
+                                                           double *part) {
+  if (ks->done || ks->replace) return;
+  double acc[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) {
+    const T y = op_apply<T>(a, r, s1);
+    as[r] = y;
+    acc[0] += (double)y * (double)s1[r];
+    acc[1] += (double)y * (double)y;
+  }
+  block_partials<2>(acc, part);
+}
+
+template <typename T, typename TX>
+__global__ void __launch_bounds__(256) mr_xr_update_kernel(int64_t N, const KrylovScalars *__restrict__ ks,
+                                                           TX *__restrict__ xl, const T *__restrict__ p1,
+                                                           const T *__restrict__ s1, const T *__restrict__ as,
+                                                           T *__restrict__ r1, const T *__restrict__ r0,
+                                                           double *part) {
+  if (ks->done || ks->replace) return;
+  const double alpha = ks->alpha, omega = ks->omega;
+  double acc[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += stride) {
+    const double s = (double)s1[r];
+    xl[r] = (TX)((double)xl[r] + (alpha * (double)p1[r] + omega * s));
+    const T rn = (T)(s - omega * (double)as[r]);
+    r1[r] = rn;
+    acc[0] += (double)rn * (double)rn;
+    acc[1] += (double)rn * (double)r0[r];
+  }
+  block_partials<2>(acc, part);
+}
+
+// T = float: FP32 vectors, FP32 iterate xl folded into the FP64 x.
+// T = double: FP64 vectors on the FP32-valued operator; x updated directly.
+template <typename T>
+int reliable_solve(Ctx &c, const double *b, double *x, double tol, double delta, int maxit, int *replacements,
+                   int *iters, double *relres) {
+  if (!c.ell_ready || !c.val32) return fail(CFD_ERR_STATE, "FP32 ELL operator not assembled");
+  const int64_t N = c.g.N;
+  const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
+  const OpArgs a64 = op_args(c, c.prm.op == CFD_OP_SPE && c.spe_ready ? CFD_OP_SPE : CFD_OP_ELL);
+  const OpArgs ain = op_args(c, std::is_same<T, float>::value ? CFD_OP_ELL : OP_ELL_VAL32);
+  T *r0, *r1, *p1, *s1, *ap, *as;
+  if constexpr (std::is_same<T, float>::value) {
+    r0 = c.q32[0]; r1 = c.q32[1]; p1 = c.q32[2]; s1 = c.q32[3]; ap = c.q32[4]; as = c.q32[5];
+  } else {
+    r0 = c.r0; r1 = c.r1; p1 = c.p1; s1 = c.s1; ap = c.ap; as = c.as;
+  }
+  float *xl = std::is_same<T, float>::value ? c.q32[6] : nullptr;
+  KrylovScalars *ks = c.ks32;
+  double *P1 = red_part_host(c, RS_SPMV_DOT), *P2 = red_part_host(c, RS_SPMV_DOT2), *P3 = red_part_host(c, RS_XR),
+         *P4 = red_part_host(c, RS_NORM);
+  c.solves32++;
+  ks_setup_kernel<<<1, 1, 0, c.stream>>>(ks, tol, 1e-7 * tol, maxit, c.solves32);
+  set_delta_kernel<<<1, 1, 0, c.stream>>>(ks, delta);
+  mr_init_kernel<T><<<nb, 256, 0, c.stream>>>(a64, b, x, r1, r0, p1, (T *)xl, red_part_host(c, RS_INIT));
+  mr_init_finalize<<<1, 256, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, ks);
+  CFD_HIP(hipGetLastError());
+  int batch = 4;
+  double prev_rel = -1.0;
+  int prev_it = 0;
+  for (;;) {
+    for (int it = 0; it < batch; ++it) {
+      spmv_dot_kernel<T><<<nb, 256, 0, c.stream>>>(ain, ks, p1, ap, r0, P1);
+      mr_alpha_finalize<<<1, 256, 0, c.stream>>>(P1, nb, ks);
+      mr_s_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, r1, ap, s1);
+      mr_spmv_dot2_kernel<T><<<nb, 256, 0, c.stream>>>(ain, ks, s1, as, P2);
+      mr_omega_finalize<<<1, 256, 0, c.stream>>>(P2, nb, ks);
+      if constexpr (std::is_same<T, float>::value)
+        mr_xr_update_kernel<T, float><<<nb, 256, 0, c.stream>>>(N, ks, xl, p1, s1, as, r1, r0, P3);
+      else
+        mr_xr_update_kernel<T, double><<<nb, 256, 0, c.stream>>>(N, ks, x, p1, s1, as, r1, r0, P3);
+      mr_conv_finalize<<<1, 256, 0, c.stream>>>(P3, nb, ks);
+      if constexpr (std::is_same<T, float>::value) mr_fold_kernel<<<nb, 256, 0, c.stream>>>(N, ks, x, xl);
+      mr_replace_kernel<T><<<nb, 256, 0, c.stream>>>(a64, ks, b, x, r1, r0, P4);
+      mr_replace_finalize<<<1, 256, 0, c.stream>>>(P4, nb, ks);
+      mr_p_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, r1, p1, ap, r0);
+    }
+    CFD_HIP(hipGetLastError());
+    CFD_HIP(hipMemcpyAsync(c.hks, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c.stream));
+    CFD_HIP(hipStreamSynchronize(c.stream));
+    const KrylovScalars h = *c.hks;
+    if (c.debug)
+      fprintf(stderr, "[mixed] it %d rep %d brk %d restarts %d rn/nb %.3e true %.3e alpha %.3e omega %.3e rho %.3e\n",
+              h.iters, h.replacements, h.breakdown, h.restarts, h.rnorm / h.norm0, h.relres, h.alpha, h.omega, h.rho);
+    if (h.done) break;
+    const double cur = h.rnorm / h.norm0;
+    int next = batch * 2;
+    if (prev_rel > 0.0 && cur > 0.0 && cur < prev_rel && h.iters > prev_it) {
+      const double rate = std::log(cur / prev_rel) / (double)(h.iters - prev_it);
+      const double need = std::log(tol / cur) / rate;
+      if (need > 0.0 && std::isfinite(need)) next = (int)std::ceil(need * 1.05) + 1;
+    }
+    prev_rel = cur;
+    prev_it = h.iters;
+    batch = std::max(1, std::min(next, 64));
+  }
+  const KrylovScalars h = *c.hks;
+  *replacements = h.replacements;
+  *iters = h.iters + 1;
+  *relres = h.relres;
+  return CFD_OK;
+}
+
+}  // namespace
+
+int mixed_reliable_solve(Ctx &c, const double *b, double *x, double tol, double delta, int maxit, int *replacements,
+                         int *iters, double *relres) {
+  if (c.prm.mixed_mode == CFD_MIXED_VALUES32)
+    return reliable_solve<double>(c, b, x, tol, delta, maxit, replacements, iters, relres);
+  return reliable_solve<float>(c, b, x, tol, delta, maxit, replacements, iters, relres);
 }
 
 }  // namespace cfd

@@ -23,14 +23,14 @@ __global__ void __launch_bounds__(256) sweep_kernel(int64_t N, int64_t off, int6
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += stride) {
     const int64_t q = off + t;
-    TX acc = TX(0);
+    double acc = 0.0;
 #pragma unroll
     for (int s = 0; s < KSLOTS; ++s) {
       const int32_t cidx = ntload(colP + (int64_t)s * N + q);
-      acc = fma((TX)ntload(valP + (int64_t)s * N + q), xP[cidx], acc);
+      acc = fma((double)ntload(valP + (int64_t)s * N + q), (double)xP[cidx], acc);
     }
-    const TX d = (TX)valP[q];
-    xP[q] = xP[q] + (TX)(omega * (double)(bP[q] - acc)) / d;
+    const double d = (double)valP[q];
+    xP[q] = (TX)((double)xP[q] + omega * ((double)bP[q] - acc) / d);
   }
 }
 
@@ -41,13 +41,13 @@ __global__ void __launch_bounds__(256) resid_kernel(int64_t N, const int32_t *__
   double acc[2] = {0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += stride) {
-    TX ax = TX(0);
+    double ax = 0.0;
 #pragma unroll
     for (int s = 0; s < KSLOTS; ++s) {
       const int32_t cidx = ntload(colP + (int64_t)s * N + q);
-      ax = fma((TX)ntload(valP + (int64_t)s * N + q), xP[cidx], ax);
+      ax = fma((double)ntload(valP + (int64_t)s * N + q), (double)xP[cidx], ax);
     }
-    const double r = (double)bP[q] - (double)ax;
+    const double r = (double)bP[q] - ax;
     acc[0] += r * r;
     acc[1] += (double)bP[q] * (double)bP[q];
   }

@@ -63,6 +63,10 @@ extern "C" {
 /* arithmetic of the pressure solve */
 #define CFD_FP64 0
 #define CFD_MIXED 1 /* FP32 inner Krylov/SOR solves inside FP64 defect correction */
+/* mixed-precision schemes */
+#define CFD_MIXED_RELIABLE 0  /* FP32 BiCGSTAB with FP64 residual replacement (reliable updates) */
+#define CFD_MIXED_RESTARTED 1 /* FP64 defect correction around restarted FP32 inner solves */
+#define CFD_MIXED_VALUES32 2  /* FP64 BiCGSTAB on the FP32-valued ELL operator, FP64 residual replacement */
 
 /* device arrays addressable through upload / download */
 #define CFD_T0_U 0 /* old velocity (ghosted) */
@@ -106,7 +110,10 @@ typedef struct cfd_params {
   int op;                /* CFD_OP_* */
   int precision;         /* CFD_FP64 / CFD_MIXED */
   double tol;            /* simuClass::p_criteria; relative residual for BiCGSTAB / RB-SOR */
-  double tol_inner;      /* CFD_MIXED: tolerance of each FP32 inner solve */
+  double tol_inner;      /* CFD_MIXED, restarted mode: tolerance of each FP32 inner solve */
+  int mixed_mode;        /* CFD_MIXED: CFD_MIXED_RELIABLE (default) or CFD_MIXED_RESTARTED */
+  double mixed_delta;    /* CFD_MIXED_RELIABLE: FP64 residual replacement when the FP32
+                            residual has dropped by this factor (0.1) */
   int maxit;             /* bicgstabRe2::iters_max_ (5000) / RB-SOR sweep cap */
   double sor_omega;      /* simuClass::p_sor_omega (1.8) */
   int sor_check;         /* RB-SOR: sweeps between true-residual checks */

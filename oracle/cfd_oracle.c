@@ -200,6 +200,11 @@ double orc_noise(uint64_t seed, int component, uint64_t icel, double amp) {
   return amp * (2.0 * u01 - 1.0);
 }
 
+void orc_fill_noise(double *v, size_t n, uint64_t seed, int tag, double base, double amp) {
+#pragma omp parallel for schedule(static)
+  for (long long m = 0; m < (long long)n; ++m) v[m] = base + orc_noise(seed, tag, (uint64_t)m, amp);
+}
+
 /* ========================================================================
  * DFIB_CylinderZ (inc/dfib/setEta.hpp:136-204)
  * ===================================================================== */

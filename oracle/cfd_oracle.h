@@ -78,6 +78,8 @@ void orc_maxdiff(const cfd_grid_desc *g, const double *u0, const double *v0, con
 double orc_dot(size_t n, const double *a, const double *b);
 /* seeded uniform noise in [-amp, amp): splitmix64(seed ^ (component<<56) ^ icel) */
 double orc_noise(uint64_t seed, int component, uint64_t icel, double amp);
+/* v[m] = base + orc_noise(seed, tag, m, amp), the seeded inputs of the golden fixtures */
+void orc_fill_noise(double *v, size_t n, uint64_t seed, int tag, double base, double amp);
 
 /* ---- solvers ------------------------------------------------------------ */
 typedef struct orc_bicg orc_bicg;

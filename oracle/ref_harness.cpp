@@ -68,7 +68,145 @@ static void dump_vel(const std::string &prefix, const velocity &V) {
   dump(prefix + "_w.bin", V.w);
 }
 
+// Seeded inputs shared with the oracle's tests (oracle/cfd_oracle.c orc_noise):
+// value = base + amp * (2 u - 1), u from splitmix64(seed ^ (tag << 56) ^ index).
+static uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+static void fill_noise(std::vector<double> &v, uint64_t seed, int tag, double base, double amp) {
+  for (size_t m = 0; m < v.size(); ++m) {
+    const uint64_t z = splitmix64(seed ^ ((uint64_t)tag << 56) ^ (uint64_t)m);
+    const double u01 = (double)(z >> 11) * 0x1.0p-53;
+    volatile double nz = amp * (2.0 * u01 - 1.0);  // rounded before the add, never fused
+    v[m] = base + nz;
+  }
+}
+
+// usage: ref_harness OUTDIR synth -- every hot-path function once on seeded inputs
+static int synth_main() {
+  simuClass simu;
+  grid gA;
+  simu.set_time_max(40);
+  simu.set_dt(0.002);
+  simu.set_Re(40.0);
+  simu.Locality = 1;
+  DfibArray Dfib;
+  pressure t1;
+  velocity T0, T1, T3;
+  shareMenory ShareM;
+  MxClass Mx;
+  calDomain dom;
+  std::vector<int> grid_size{gA.nx, gA.ny, gA.nz};
+  std::vector<int> dims{1, 1, 1};
+  dom.initTable(grid_size, dims);
+  dom.initLocal(0, 0, 0);
+  resize_variable(gA, t1, T0, T1, T3, Dfib);
+  generateGride(simu, ShareM, dom, gA);
+  const size_t N = gA.iceltotCal;
+  createPressureMatrix(Mx, simu, dom, gA);
+  std::fill(Dfib.eta.begin(), Dfib.eta.end(), 0.0);
+  DFIB_CylinderZ(Dfib, dom, gA);
+  const uint64_t seed = 20240611ULL;
+  // predictor: T0 = (1, 0, 0) + 0.3 noise, T1 = 7 (sentinel for unwritten positions)
+  fill_noise(T0.u, seed, 1, 1.0, 0.3);
+  fill_noise(T0.v, seed, 2, 0.0, 0.3);
+  fill_noise(T0.w, seed, 3, 0.0, 0.3);
+  T1.u.assign(T1.u.size(), 7.0);
+  T1.v.assign(T1.v.size(), 7.0);
+  T1.w.assign(T1.w.size(), 7.0);
+  ConvectionDifussion(simu, T0, T1, dom, gA);
+  dump_vel("syn_pred_T1", T1);
+  // source add with the driver's indexing (RUN_CPU_PC_METHOD.hpp:309-316)
+  std::vector<double> src(3 * N);
+  fill_noise(src, seed, 4, 0.0, 1e-3);
+  for (int c = 0; c < 3; c++)
+    for (auto i = dom.i_begin; i < dom.i_endof; ++i)
+      for (auto j = dom.j_begin; j < dom.j_endof; ++j)
+        for (auto k = dom.k_begin; k < dom.k_endof; ++k)
+          T1.U(c)[gA.icelCal(i, j, k)] += src[gA.icelCal(i, j, k) + gA.iceltotCal * c];
+  dump_vel("syn_srcadd_T1", T1);
+  // RHS from the T1 just made
+  createBMatrix_omp(T1, Mx, simu, dom, gA);
+  dump("syn_rhs_b.bin", Mx.matB);
+  // SpMV of seeded x, ELL and SPE
+  std::vector<double> x(N), y(N);
+  fill_noise(x, seed, 5, 0.0, 1.0);
+  Mx.matA_ell.multiply_omp(x, y);
+  dump("syn_spmv_ell.bin", y);
+  Mx.matA_spe.resize(gA.nxCal, gA.nyCal, gA.nzCal);
+  Mx.matA_spe.setupPressure(gA.gC, gA.Dx, gA.Dy, gA.Dz, gA.Dxs, gA.Dys, gA.Dzs);
+  std::vector<double> spe_jp = Mx.matA_spe.set_Jp_pc();
+  Mx.matA_spe.multiply_omp(x, y);
+  dump("syn_spmv_spe.bin", y);
+  // DFIB update: p seeded, T3 = 9 sentinel
+  fill_noise(t1.p, seed, 6, 0.0, 1.0);
+  T3.u.assign(T3.u.size(), 9.0);
+  T3.v.assign(T3.v.size(), 9.0);
+  T3.w.assign(T3.w.size(), 9.0);
+  std::fill(Dfib.f.begin(), Dfib.f.end(), 0.0);
+  update_UandF_omp(Dfib, simu, t1, T1, T3, dom, gA);
+  dump_vel("syn_upd_T3", T3);
+  dump("syn_upd_f.bin", Dfib.f);
+  {
+    virtualF_Int(Dfib, dom, gA, 0);
+    virtualF_Int(Dfib, dom, gA, 1);
+    std::ofstream s(g_out + "/syn_scalars.txt");
+    s << std::setprecision(17) << "cd " << -2.0 * Dfib.ValSum[0] / gA.lz << "\ncl " << -2.0 * Dfib.ValSum[1] / gA.lz
+      << "\n";
+    auto [ud, vd, wd] = CheckSteadyStatebyMaxVal_omp(simu, ShareM, T0, T3, dom, gA);
+    s << "maxdiff " << ud << " " << vd << " " << wd << "\n";
+    // BiCGSTAB on the ELL operator from x = 0, reference tolerance 1e-3, on a
+    // consistent right-hand side b = A xt (the all-Neumann operator is singular)
+    std::vector<double> b(N), xs(N, 0.0), xt(N);
+    fill_noise(xt, seed, 7, 0.0, 1.0);
+    Mx.matA_ell.multiply_omp(xt, b);
+    solver::bicgstabRe2<ELL_type> sol(Mx.matA_ell);
+    sol.setTolerance(1e-3);
+    auto [it, err] = sol(b, xs);
+    s << "bicg_ell " << it << " " << err << "\n";
+    dump("syn_bicg_x.bin", xs);
+  }
+  // BC main on seeded T0 and p, then copy into seeded T1
+  fill_noise(T0.u, seed, 8, 1.0, 0.3);
+  fill_noise(T0.v, seed, 9, 0.0, 0.3);
+  fill_noise(T0.w, seed, 10, 0.0, 0.3);
+  fill_noise(T1.u, seed, 11, 1.0, 0.3);
+  fill_noise(T1.v, seed, 12, 0.0, 0.3);
+  fill_noise(T1.w, seed, 13, 0.0, 0.3);
+  BC_staggered_main(dom, T0, t1, gA);
+  BC_staggered_copy(dom, T0, T1, gA);
+  dump_vel("syn_bc_T0", T0);
+  dump_vel("syn_bc_T1", T1);
+  dump("syn_bc_p.bin", t1.p);
+  // SorPipeLine_seq: 12 lexicographic sweeps from p = 0 on the seeded T1
+  // (simu.loop == 1 so that the coefficients are computed)
+  {
+    SORcoefficient Sor;
+    fill_noise(T1.u, seed, 14, 1.0, 0.05);
+    fill_noise(T1.v, seed, 15, 0.0, 0.05);
+    fill_noise(T1.w, seed, 16, 0.0, 0.05);
+    t1.init_p(0.0);
+    simu.loop = 1;
+    simu.p_criteria = 0.0;
+    simu.p_sor_iter_max = 12;
+    SorPipeLine_seq(Sor, simu, T1, t1, dom, gA);
+    dump("syn_sor_p.bin", t1.p);
+    std::ofstream s(g_out + "/syn_scalars.txt", std::ios::app);
+    s << std::setprecision(17) << "sor_seq " << simu.iters << " " << simu.error << "\n";
+  }
+  return 0;
+}
+
 int main(int argc, char **argv) {
+  if (argc >= 3 && std::string(argv[2]) == "synth") {
+    g_out = argv[1];
+    mkdir(g_out.c_str(), 0755);
+    return synth_main();
+  }
   if (argc < 6) {
     std::cerr << "usage: ref_harness OUTDIR NSTEPS TOL OP(ell|spe) FN_STEP\n";
     return 2;

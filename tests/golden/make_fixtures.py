@@ -1,0 +1,81 @@
+"""Writes tests/golden/native_fixtures.json from the reference solver itself.
+
+Builds oracle/_ref (the reference's own header-only sources compiled where
+they lie, see oracle/Makefile), runs its harness at the reference's native
+case (190x210x40 SML grid, cavity table, Re 40, dt 0.002) and records:
+  * SHA-256 of the grid metrics, the IB mask, the ELL operator (columns,
+    values), gA.jp and the SPE Jacobi scaling;
+  * for every hot-path function applied once to seeded inputs (splitmix64
+    noise, reproduced bit for bit by the oracle's orc_fill_noise): SHA-256 of
+    its output, the L2 norm, and 256 seeded sample values;
+  * drag/lift, max-difference, BiCGSTAB (iterations, residual) and
+    SorPipeLine_seq scalars;
+  * the per-pass statistics of three PC-method timesteps at the reference
+    tolerance 1e-3.
+Only this data is committed; reference sources and binaries never are.
+usage: python tests/golden/make_fixtures.py  (needs /root/reference)
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+ORACLE = ROOT / "oracle"
+DUMPS = ORACLE / "_ref" / "dumps"
+OUT = Path(__file__).resolve().parent / "native_fixtures.json"
+
+
+def sha(path: Path) -> str:
+    return hashlib.sha256(path.read_bytes()).hexdigest()
+
+
+def samples(a: np.ndarray, n=256, seed=11):
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(a.size, size=min(n, a.size), replace=False))
+    return {"idx": idx.tolist(), "val": [float(v) for v in a[idx]]}
+
+
+def main():
+    if not Path("/root/reference/inc").is_dir():
+        sys.exit("needs the reference sources at /root/reference")
+    subprocess.run(["make", "-s", "-C", str(ORACLE), "ref"], check=True)
+    env = dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8"))
+    h = str(ORACLE / "_ref" / "ref_harness")
+    DUMPS.mkdir(parents=True, exist_ok=True)
+    subprocess.run([h, str(DUMPS / "native"), "3", "1e-3", "ell", "2"], check=True, env=env,
+                   stdout=subprocess.DEVNULL)
+    subprocess.run([h, str(DUMPS / "synth"), "synth"], check=True, env=env, stdout=subprocess.DEVNULL)
+    nat, syn = DUMPS / "native", DUMPS / "synth"
+    fx = {"case": "native 190x210x40 SML, BC_Cavity_A, Re 40, dt 0.002, cylinder (6.5, 6) R 0.5",
+          "seed": 20240611, "setup": {}, "synth": {}, "scalars": {}, "steps": []}
+    for name in ("grid.bin", "eta.bin", "ell_col.bin", "ell_val.bin", "jp.bin", "spe_jp.bin"):
+        fx["setup"][name] = sha(nat / name)
+    eta = np.fromfile(nat / "eta.bin")
+    fx["setup"]["eta_sum"] = float(eta.sum())
+    fx["setup"]["eta_cut_cells"] = int(((eta > 0) & (eta < 1)).sum())
+    for f in sorted(syn.glob("syn_*.bin")):
+        a = np.fromfile(f)
+        fx["synth"][f.name] = {"sha256": sha(f), "l2": float(np.linalg.norm(a)), **samples(a)}
+    for line in (syn / "syn_scalars.txt").read_text().splitlines():
+        k, *v = line.split()
+        fx["scalars"][k] = [float(x) for x in v]
+    for line in (nat / "stats.txt").read_text().splitlines():
+        k, *v = line.split()
+        if k == "pass":
+            fx["steps"].append({"step": int(v[0]), "pass": int(v[1]), "iters": int(v[2]), "err": float(v[3]),
+                                "cd": float(v[4]), "cl": float(v[5])})
+        elif k == "maxdiff":
+            fx["steps"].append({"step": int(v[0]), "maxdiff": [float(x) for x in v[1:]]})
+    OUT.write_text(json.dumps(fx, indent=0))
+    print(f"wrote {OUT} ({OUT.stat().st_size // 1024} KiB)")
+
+
+if __name__ == "__main__":
+    main()

@@ -67,6 +67,7 @@ def _declare(L):
     L.orc_dot.restype = C.c_double
     L.orc_noise.argtypes = [C.c_uint64, C.c_int, C.c_uint64, C.c_double]
     L.orc_noise.restype = C.c_double
+    L.orc_fill_noise.argtypes = [dp, C.c_size_t, C.c_uint64, C.c_int, C.c_double, C.c_double]
     L.orc_bicg_create.argtypes = [C.c_size_t, C.c_int, ip, dp, G, C.c_void_p, C.c_void_p]
     L.orc_bicg_create.restype = C.c_void_p
     L.orc_bicg_destroy.argtypes = [C.c_void_p]
@@ -135,6 +136,21 @@ def oracle_axis_tanh(ncal, L, xc, h0, band, ratio):
     if rc != 0:
         raise ValueError("stretched axis rejected its arguments")
     return X, D, Ds
+
+
+def noise(n, seed, tag, base, amp):
+    """Seeded input array shared with the reference harness (splitmix64)."""
+    v = np.empty(n)
+    lib().orc_fill_noise(v, n, seed, tag, base, amp)
+    return v
+
+
+def native_grid_oracle() -> "Grid":
+    """The reference's 190x210x40 'non_uniform-SML' grid built by the oracle."""
+    X, Dx, Dxs = oracle_axis_sml(190, 17.0, 0.3 * 17.0, 100, -4)
+    Y, Dy, Dys = oracle_axis_sml(210, 12.0, 0.3 * 12.0, 100, 0)
+    Z, Dz, Dzs = oracle_axis_sml(40, 3.0, 0.5 * 3.0, 0, 0)
+    return Grid(GridArrays((190, 210, 40), X, Y, Z, Dx, Dy, Dz, Dxs, Dys, Dzs, 17.0, 12.0, 3.0))
 
 
 def ints6(v):
