@@ -44,6 +44,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def profiled_traffic():
+    """HBM bytes per ELL SpMV launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration)."""
+    path = ROOT / "profiles" / "kernels_512x256x64.md"
+    try:
+        for line in path.read_text().splitlines():
+            if line.startswith("ELL SpMV FP64 traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def run_steps(w, warmup, steps, tag):
     sim = Simulation(w.grid, w.params)
     sim.setup(noise=w.noise, seed=w.seed)
@@ -268,7 +281,7 @@ def main():
                    "pressure": "BiCGSTAB (bicgstabRe2), Jacobi-scaled ELL, tol 1e-10, maxit 5000",
                    "passes_per_step": 4, "bicgstab_iters_per_step": iters64},
         "roofline": {"bound": "hbm", "achieved": spmv["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": spmv["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": spmv["gbs"] / HBM_PEAK_GBS, "traffic": profiled_traffic(),
                      "kernel": "ELL SpMV FP64 (100 B/row x 8388608 rows per launch)", "avg_us": spmv["avg_us"]},
         "kernels": kt,
         "mixed": mixed,
