@@ -1,0 +1,114 @@
+// The reference's prediction-correction loop (inc/run/RUN_CPU_PC_METHOD.hpp
+// :287-627) written against cfd_reference_shim.hpp -- host vectors, one
+// reference function per call -- must reproduce the device-resident fused
+// timestep (grid::step, cfd_step) on the same inputs.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "cfd_reference_shim.hpp"
+
+using namespace cfdgpu;
+
+static double rel_l2(const std::vector<double> &a, const std::vector<double> &b) {
+  double num = 0.0, den = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) {
+    num += (a[i] - b[i]) * (a[i] - b[i]);
+    den += b[i] * b[i];
+  }
+  return den > 0 ? std::sqrt(num / den) : std::sqrt(num);
+}
+
+int main() {
+  cfd_params prm;
+  check(cfd_params_default(&prm), "cfd_params_default");
+  // cylinder in a channel: inflow x-, outflow x+ (p = 0), slip y/z walls
+  const double dir[6][3] = {{1, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  const double num[6][3] = {{0, 0, 0}, {1, 1, 1}, {1, 0, 1}, {1, 0, 1}, {1, 1, 0}, {1, 1, 0}};
+  std::memcpy(prm.bc_dir, dir, sizeof(dir));
+  std::memcpy(prm.bc_num, num, sizeof(num));
+  for (int f = 0; f < 6; ++f) prm.p_dirichlet[f] = f == 1;
+  prm.Re = 100.0;
+  prm.dt = 0.002;
+  prm.tol = 1e-12;
+  prm.ib_cx = 2.0;
+  prm.ib_cy = 2.0;
+  prm.ib_r = 0.5;
+
+  // ---- reference-shaped driver, host vectors ----
+  grid gA(32, 16, 8, 8.0, 4.0, 2.0, prm);
+  simuClass simu;
+  simu.set_Re(prm.Re);
+  simu.set_time_max(3 * prm.dt);
+  simu.set_dt(prm.dt);
+  simu.p_criteria = prm.tol;
+  DfibArray Dfib;
+  pressure t1;
+  velocity T0, T1, T3;
+  MxClass Mx;
+  calDomain dom = gA.domain();
+  resize_variable(gA, t1, T0, T1, T3, Dfib);
+  std::fill(T0.u.begin(), T0.u.end(), 1.0);
+  T1 = T0;
+  T3 = T0;
+  BC_staggered_main(dom, T0, t1, gA);
+  BC_staggered_copy(dom, T0, T1, gA);
+  Mx.X_result.assign(gA.iceltotCal, 0.0);
+  createPressureMatrix(Mx, simu, dom, gA);
+  mat::ELL_matrix A(gA);
+  solver::bicgstabRe2<mat::ELL_matrix> pSolver(A);
+  pSolver.setTolerance(simu.p_criteria);
+  DFIB_CylinderZ(Dfib, dom, gA);
+  auto source_termF = Dfib.f;
+  const size_t N = gA.iceltotCal;
+  for (simu.loop = 1; simu.get_finishloop(); simu.finishloop()) {
+    std::fill(source_termF.begin(), source_termF.end(), 0.0);
+    for (int pass = 0; pass < 4; ++pass) {
+      ConvectionDifussion(simu, T0, T1, dom, gA);
+      if (pass > 0)  // (pass 0 adds zeros)
+        for (int c = 0; c < 3; ++c)
+          for (int i = dom.i_begin; i < dom.i_endof; ++i)
+            for (int j = dom.j_begin; j < dom.j_endof; ++j)
+              for (int k = dom.k_begin; k < dom.k_endof; ++k)
+                T1.U(c)[gA.icel(i, j, k)] += source_termF[gA.icelCal(i, j, k) + N * c];
+      if (simu.loop > 1) {
+        createBMatrix_omp(T1, Mx, simu, dom, gA);
+        std::tie(simu.iters, simu.error) = pSolver(Mx.matB, Mx.X_result);
+        Pressure_transform_X_result(t1, Mx, dom, gA);
+      }
+      update_UandF_omp(Dfib, simu, t1, T1, T3, dom, gA);
+      if (pass < 3)
+        for (size_t m = 0; m < 3 * N; ++m) source_termF[m] += simu.dt * Dfib.f[m];
+    }
+    CheckSteadyStatebyMaxVal_omp(simu, T0, T3, dom, gA);
+    T0 = T3;
+    BC_staggered_main(dom, T0, t1, gA);
+    BC_staggered_copy(dom, T0, T1, gA);
+    auto cdl = virtualF_Int(Dfib, dom, gA);
+    std::printf("loop %zu iters %d err %.3e cD %.6f\n", simu.loop, simu.iters, simu.error, cdl.first);
+  }
+
+  // ---- device-resident fused timestep ----
+  prm.source_index_quirk = 0;
+  grid gB(32, 16, 8, 8.0, 4.0, 2.0, prm);
+  check(cfd_init_state(gB.ctx(), 0.0, 0), "cfd_init_state");
+  check(cfd_ell_assemble(gB.ctx()), "cfd_ell_assemble");
+  check(cfd_set_eta_cylinder_z(gB.ctx()), "cfd_set_eta");
+  for (int s = 0; s < 3; ++s) gB.step();
+  velocity V;
+  gB.get(CFD_T0_U, V);
+  std::vector<double> x;
+  gB.get(CFD_X, x);
+  const double eu = rel_l2(V.u, T0.u), ev = rel_l2(V.v, T0.v), ew = rel_l2(V.w, T0.w), ex = rel_l2(x, Mx.X_result);
+  std::printf("shim vs fused step: u %.2e v %.2e w %.2e X %.2e\n", eu, ev, ew, ex);
+  // an out-of-range argument surfaces as the reference's exception type
+  bool threw = false;
+  try {
+    check(cfd_axis_sml(20, 1.0, 0.3, 7, 0, x.data(), x.data(), x.data()), "cfd_axis_sml");
+  } catch (const std::invalid_argument &) {
+    threw = true;
+  }
+  const bool ok = eu < 1e-11 && ev < 1e-11 && ew < 1e-11 && ex < 1e-11 && threw;
+  std::printf("%s\n", ok ? "PASS" : "FAIL");
+  return ok ? 0 : 1;
+}
