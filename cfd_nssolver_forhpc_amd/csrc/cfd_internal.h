@@ -16,7 +16,10 @@ namespace cfd {
 constexpr int GC = 2;           // ghost layers per side (grid::gC)
 constexpr int KSLOTS = 7;       // ELL slots per row (createPressureMatrix resize(N, 7))
 constexpr int RED_THREADS = 256;
-constexpr int RED_MAX_BLOCKS = 2048;  // upper bound of partial sums per reduction
+// upper bound of partial sums per reduction: one row per thread up to 8.4M
+// rows (256-thread blocks), which streams fastest on gfx950; larger vectors
+// fall back to a grid-stride loop over this many blocks
+constexpr int RED_MAX_BLOCKS = 32768;
 
 // Grid extents and 1-D metrics on the device.
 struct DevGrid {

@@ -58,14 +58,28 @@ __device__ inline void combine_partials(const double *part, int nblk, double (&t
   double v[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
-    double s = MAX ? 0.0 : 0.0;
-    bool first = true;
-    for (int b = threadIdx.x; b < nblk; b += RED_THREADS) {
-      const double x = part[q * RED_MAX_BLOCKS + b];
-      s = first ? x : (MAX ? fmax(s, x) : s + x);
-      first = false;
+    const double *pq = part + q * RED_MAX_BLOCKS;
+    if (MAX) {
+      double s = 0.0;
+      bool first = true;
+      for (int b = threadIdx.x; b < nblk; b += RED_THREADS) {
+        const double x = pq[b];
+        s = first ? x : fmax(s, x);
+        first = false;
+      }
+      v[q] = s;
+    } else {
+      // eight independent accumulators keep eight loads in flight per
+      // thread; the combination order is fixed, so the sum is deterministic
+      double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      int b = threadIdx.x;
+      for (; b + 7 * RED_THREADS < nblk; b += 8 * RED_THREADS) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s[u] += pq[b + u * RED_THREADS];
+      }
+      for (; b < nblk; b += RED_THREADS) s[0] += pq[b];
+      v[q] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
     }
-    v[q] = s;
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll

@@ -90,7 +90,7 @@ int rbsor_generic(Ctx &c, const TV *valP, const TX *bP, TX *xP, double omega, do
     for (int s = 0; s < chunk; ++s) {
       for (int colour = 0; colour < 2; ++colour) {
         const int64_t off = colour ? c.ncolour[0] : 0, cnt = c.ncolour[colour];
-        sweep_kernel<TV, TX><<<blocks_for(cnt, 256, 8192), 256, 0, c.stream>>>(N, off, cnt, c.colP, valP, bP, xP,
+        sweep_kernel<TV, TX><<<blocks_for(cnt, 256, 1 << 30), 256, 0, c.stream>>>(N, off, cnt, c.colP, valP, bP, xP,
                                                                                 omega);
       }
     }
@@ -145,7 +145,7 @@ hipError_t launch_rbsor_sweep(Ctx &c, double omega) {
   const int64_t N = c.g.N;
   for (int colour = 0; colour < 2; ++colour) {
     const int64_t off = colour ? c.ncolour[0] : 0, cnt = c.ncolour[colour];
-    sweep_kernel<double, double><<<blocks_for(cnt, 256, 8192), 256, 0, c.stream>>>(N, off, cnt, c.colP, c.valP,
+    sweep_kernel<double, double><<<blocks_for(cnt, 256, 1 << 30), 256, 0, c.stream>>>(N, off, cnt, c.colP, c.valP,
                                                                                     c.bP, c.xP, omega);
   }
   return hipGetLastError();

@@ -99,7 +99,16 @@ int main() {
   gB.get(CFD_T0_U, V);
   std::vector<double> x;
   gB.get(CFD_X, x);
-  const double eu = rel_l2(V.u, T0.u), ev = rel_l2(V.v, T0.v), ew = rel_l2(V.w, T0.w), ex = rel_l2(x, Mx.X_result);
+  // w carries only roundoff in this z-uniform flow, so its error is measured
+  // against |u|; X is a solve to 1e-12 relative residual on an operator with
+  // condition number ~1e3, so two solves may differ by ~1e-9
+  double dw = 0.0, nu2 = 0.0;
+  for (size_t m = 0; m < V.w.size(); ++m) {
+    dw += (V.w[m] - T0.w[m]) * (V.w[m] - T0.w[m]);
+    nu2 += T0.u[m] * T0.u[m];
+  }
+  const double ew = std::sqrt(dw / nu2);
+  const double eu = rel_l2(V.u, T0.u), ev = rel_l2(V.v, T0.v), ex = rel_l2(x, Mx.X_result);
   std::printf("shim vs fused step: u %.2e v %.2e w %.2e X %.2e\n", eu, ev, ew, ex);
   // an out-of-range argument surfaces as the reference's exception type
   bool threw = false;
@@ -108,7 +117,7 @@ int main() {
   } catch (const std::invalid_argument &) {
     threw = true;
   }
-  const bool ok = eu < 1e-11 && ev < 1e-11 && ew < 1e-11 && ex < 1e-11 && threw;
+  const bool ok = eu < 1e-11 && ev < 1e-11 && ew < 1e-11 && ex < 1e-8 && threw;
   std::printf("%s\n", ok ? "PASS" : "FAIL");
   return ok ? 0 : 1;
 }
