@@ -561,10 +561,13 @@ void orc_pressure_scatter(const cfd_grid_desc *g, const double *X, double *p) {
       for (int k = GC; k < d.nz - GC; ++k) p[IC(d, i, j, k)] = X[ICAL(d, i, j, k)];
 }
 
-/* update_UandF_omp (inc/dfib/updateUandF.hpp:95-192), solid velocity 0 */
-void orc_dfib_update(const cfd_grid_desc *g, double dt, const double *eta, const double *p,
-                     const double *u1, const double *v1, const double *w1,
-                     double *u3, double *v3, double *w3, double *f) {
+/* update_UandF_omp (inc/dfib/updateUandF.hpp:95-192), solid velocity 0.
+   gain scales the forcing, u_new = (1 - gain*eta_f) T2 (gain = 1 is the
+   reference, bit for bit); t2, when not NULL, receives the intermediate
+   velocity T2 = u* - dt dp/dx (3*N, interior order) for the adjoint. */
+void orc_dfib_update_ex(const cfd_grid_desc *g, double dt, double gain, const double *eta, const double *p,
+                        const double *u1, const double *v1, const double *w1,
+                        double *u3, double *v3, double *w3, double *f, double *t2) {
   const dims_t d = dims_of(g);
   const size_t N = d.N;
   const double uS = 0.0, vS = 0.0, wS = 0.0;
@@ -575,8 +578,9 @@ void orc_dfib_update(const cfd_grid_desc *g, double dt, const double *eta, const
         const size_t ic = IC(d, i, j, k), icP = IC(d, i + 1, j, k);
         const double T2u = u1[ic] - dt * (p[icP] - p[ic]) / g->Dxs[i];
         const double e = eta[ic] + (eta[icP] - eta[ic]) * ((g->Dxs[i] * 0.5) / g->Dx[i]);
-        u3[ic] = eta[ic] * uS + (1.0 - e) * T2u;
+        u3[ic] = eta[ic] * uS + (1.0 - gain * e) * T2u;
         f[ICAL(d, i, j, k)] = (u3[ic] - T2u) / dt;
+        if (t2) t2[ICAL(d, i, j, k)] = T2u;
       }
 #pragma omp parallel for schedule(static)
   for (int i = GC; i < d.nx - GC; ++i)
@@ -585,8 +589,9 @@ void orc_dfib_update(const cfd_grid_desc *g, double dt, const double *eta, const
         const size_t ic = IC(d, i, j, k), icP = IC(d, i, j + 1, k);
         const double T2v = v1[ic] - dt * (p[icP] - p[ic]) / g->Dys[j];
         const double e = eta[ic] + (eta[icP] - eta[ic]) * ((g->Dys[j] * 0.5) / g->Dy[j]);
-        v3[ic] = eta[ic] * vS + (1.0 - e) * T2v;
+        v3[ic] = eta[ic] * vS + (1.0 - gain * e) * T2v;
         f[ICAL(d, i, j, k) + N] = (v3[ic] - T2v) / dt;
+        if (t2) t2[ICAL(d, i, j, k) + N] = T2v;
       }
 #pragma omp parallel for schedule(static)
   for (int i = GC; i < d.nx - GC; ++i)
@@ -595,9 +600,16 @@ void orc_dfib_update(const cfd_grid_desc *g, double dt, const double *eta, const
         const size_t ic = IC(d, i, j, k), icP = IC(d, i, j, k + 1);
         const double T2w = w1[ic] - dt * (p[icP] - p[ic]) / g->Dzs[k];
         const double e = eta[ic] + (eta[icP] - eta[ic]) * ((g->Dzs[k] * 0.5) / g->Dz[k]);
-        w3[ic] = eta[ic] * wS + (1.0 - e) * T2w;
+        w3[ic] = eta[ic] * wS + (1.0 - gain * e) * T2w;
         f[ICAL(d, i, j, k) + 2 * N] = (w3[ic] - T2w) / dt;
+        if (t2) t2[ICAL(d, i, j, k) + 2 * N] = T2w;
       }
+}
+
+void orc_dfib_update(const cfd_grid_desc *g, double dt, const double *eta, const double *p,
+                     const double *u1, const double *v1, const double *w1,
+                     double *u3, double *v3, double *w3, double *f) {
+  orc_dfib_update_ex(g, dt, 1.0, eta, p, u1, v1, w1, u3, v3, w3, f, NULL);
 }
 
 /* ========================================================================
@@ -609,15 +621,18 @@ void orc_dfib_update(const cfd_grid_desc *g, double dt, const double *eta, const
    tangential: ghost = 2 Dir + (2 Num - 1) interior; faces 1/3/5 also set the
    last interior face.  Pressure ghosts copy the first interior cell, or are 0
    on faces marked Dirichlet. */
-void orc_bc_main(const cfd_grid_desc *g, const double Dir[6][3], const double Num[6][3],
-                 const int pd[6], double *u, double *v, double *w, double *p) {
+/* uin (NULL: the table value) replaces Dir[0][0] by a profile over y,
+   uin[j - 2] for interior row j: the inflow the adjoint differentiates. */
+void orc_bc_main_ex(const cfd_grid_desc *g, const double Dir[6][3], const double Num[6][3],
+                    const int pd[6], const double *uin, double *u, double *v, double *w, double *p) {
   const dims_t d = dims_of(g);
   const int nx = d.nx, ny = d.ny, nz = d.nz;
 #define NRM(b, c, x) (Dir[b][c] + Num[b][c] * (x))
 #define TAN(b, c, x) (Dir[b][c] * 2. + (2.0 * Num[b][c] - 1.0) * (x))
   for (int j = GC; j < ny - GC; ++j)
     for (int k = GC; k < nz - GC; ++k) { /* BC_0 */
-      u[IC(d, 0, j, k)] = u[IC(d, 1, j, k)] = NRM(0, 0, u[IC(d, 2, j, k)]);
+      const double din = uin ? uin[j - GC] : Dir[0][0];
+      u[IC(d, 0, j, k)] = u[IC(d, 1, j, k)] = din + Num[0][0] * u[IC(d, 2, j, k)];
       v[IC(d, 0, j, k)] = v[IC(d, 1, j, k)] = TAN(0, 1, v[IC(d, 2, j, k)]);
       w[IC(d, 0, j, k)] = w[IC(d, 1, j, k)] = TAN(0, 2, w[IC(d, 2, j, k)]);
     }
@@ -669,6 +684,11 @@ void orc_bc_main(const cfd_grid_desc *g, const double Dir[6][3], const double Nu
       p[IC(d, i, j, 0)] = p[IC(d, i, j, 1)] = pd[4] ? 0.0 : p[IC(d, i, j, 2)];
       p[IC(d, i, j, nz - 1)] = p[IC(d, i, j, nz - 2)] = pd[5] ? 0.0 : p[IC(d, i, j, nz - 3)];
     }
+}
+
+void orc_bc_main(const cfd_grid_desc *g, const double Dir[6][3], const double Num[6][3],
+                 const int pd[6], double *u, double *v, double *w, double *p) {
+  orc_bc_main_ex(g, Dir, Num, pd, NULL, u, v, w, p);
 }
 
 /* BC_staggered_copy (updateBC.hpp:6-109): boundary faces old -> current */
@@ -955,6 +975,280 @@ double orc_sor_lex(const cfd_grid_desc *g, double dt, double omega, double crit,
 }
 
 /* ========================================================================
+ * Discrete adjoint of the timestep (no reference counterpart).
+ *
+ * Every function below is the transpose of the forward function named in
+ * its comment, written as a scatter over the forward loop: where the forward
+ * pass computes y = sum_m a_m x_m, the adjoint adds a_m * ly to lx_m.  A
+ * forward assignment that overwrites y also clears ly.  The predictor is
+ * nonlinear and its transpose is the transpose of its Jacobian at the
+ * forward state (QUICK's upwind choice held fixed), so its pin is the
+ * dot-product identity <ly, J dx> = <J^T ly, dx> with J dx from central
+ * differences of orc_predictor itself.
+ * ===================================================================== */
+
+/* transpose of BC_staggered_main's velocity part (orc_bc_main_ex), faces in
+   the reverse of the forward order; the adjoint of the face-0 u Dirichlet
+   value is added to g_uin[j - 2] (when g_uin is not NULL) */
+void orc_bc_main_adjoint(const cfd_grid_desc *g, const double Num[6][3], double *g_uin,
+                         double *lu, double *lv, double *lw) {
+  const dims_t d = dims_of(g);
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  double *L[3] = {lu, lv, lw};
+  /* y[m0] = y[m1] = a*x[m2] + c  ->  lx[m2] += a (ly[m0] + ly[m1]); ly[m0..] = 0 */
+#define ADJ2(c, a, m0, m1, m2)                 \
+  do {                                         \
+    const double s_ = L[c][m0] + L[c][m1];     \
+    L[c][m0] = 0.0; L[c][m1] = 0.0;            \
+    L[c][m2] += (a) * s_;                      \
+  } while (0)
+#define ADJ3(c, a, m0, m1, m2, m3)                      \
+  do {                                                  \
+    const double s_ = L[c][m0] + L[c][m1] + L[c][m2];   \
+    L[c][m0] = 0.0; L[c][m1] = 0.0; L[c][m2] = 0.0;     \
+    L[c][m3] += (a) * s_;                               \
+  } while (0)
+#define AN(b, c) (Num[b][c])
+#define AT(b, c) (2.0 * Num[b][c] - 1.0)
+  for (int i = GC; i < nx - GC; ++i)
+    for (int j = GC; j < ny - GC; ++j) { /* BC_5, reverse order of its writes */
+      ADJ2(1, AT(5, 1), IC(d, i, j, nz - 1), IC(d, i, j, nz - 2), IC(d, i, j, nz - 3));
+      ADJ2(0, AT(5, 0), IC(d, i, j, nz - 1), IC(d, i, j, nz - 2), IC(d, i, j, nz - 3));
+      ADJ3(2, AN(5, 2), IC(d, i, j, nz - 1), IC(d, i, j, nz - 2), IC(d, i, j, nz - 3), IC(d, i, j, nz - 4));
+    }
+  for (int i = GC; i < nx - GC; ++i)
+    for (int j = GC; j < ny - GC; ++j) { /* BC_4 */
+      ADJ2(2, AN(4, 2), IC(d, i, j, 0), IC(d, i, j, 1), IC(d, i, j, 2));
+      ADJ2(1, AT(4, 1), IC(d, i, j, 0), IC(d, i, j, 1), IC(d, i, j, 2));
+      ADJ2(0, AT(4, 0), IC(d, i, j, 0), IC(d, i, j, 1), IC(d, i, j, 2));
+    }
+  for (int i = GC; i < nx - GC; ++i)
+    for (int k = GC; k < nz - GC; ++k) { /* BC_3 */
+      ADJ2(2, AT(3, 2), IC(d, i, ny - 1, k), IC(d, i, ny - 2, k), IC(d, i, ny - 3, k));
+      ADJ2(0, AT(3, 0), IC(d, i, ny - 1, k), IC(d, i, ny - 2, k), IC(d, i, ny - 3, k));
+      ADJ3(1, AN(3, 1), IC(d, i, ny - 1, k), IC(d, i, ny - 2, k), IC(d, i, ny - 3, k), IC(d, i, ny - 4, k));
+    }
+  for (int i = GC; i < nx - GC; ++i)
+    for (int k = GC; k < nz - GC; ++k) { /* BC_2 */
+      ADJ2(2, AT(2, 2), IC(d, i, 0, k), IC(d, i, 1, k), IC(d, i, 2, k));
+      ADJ2(1, AN(2, 1), IC(d, i, 0, k), IC(d, i, 1, k), IC(d, i, 2, k));
+      ADJ2(0, AT(2, 0), IC(d, i, 0, k), IC(d, i, 1, k), IC(d, i, 2, k));
+    }
+  for (int j = GC; j < ny - GC; ++j)
+    for (int k = GC; k < nz - GC; ++k) { /* BC_1 */
+      ADJ2(2, AT(1, 2), IC(d, nx - 1, j, k), IC(d, nx - 2, j, k), IC(d, nx - 3, j, k));
+      ADJ2(1, AT(1, 1), IC(d, nx - 1, j, k), IC(d, nx - 2, j, k), IC(d, nx - 3, j, k));
+      ADJ3(0, AN(1, 0), IC(d, nx - 1, j, k), IC(d, nx - 2, j, k), IC(d, nx - 3, j, k), IC(d, nx - 4, j, k));
+    }
+  for (int j = GC; j < ny - GC; ++j)
+    for (int k = GC; k < nz - GC; ++k) { /* BC_0 */
+      ADJ2(2, AT(0, 2), IC(d, 0, j, k), IC(d, 1, j, k), IC(d, 2, j, k));
+      ADJ2(1, AT(0, 1), IC(d, 0, j, k), IC(d, 1, j, k), IC(d, 2, j, k));
+      const size_t m0 = IC(d, 0, j, k), m1 = IC(d, 1, j, k);
+      const double s_ = lu[m0] + lu[m1];
+      if (g_uin) g_uin[j - GC] += s_;
+      lu[m0] = 0.0; lu[m1] = 0.0;
+      lu[IC(d, 2, j, k)] += AN(0, 0) * s_;
+    }
+#undef ADJ2
+#undef ADJ3
+#undef AN
+#undef AT
+}
+
+/* transpose of orc_dfib_update_ex at its forward input (T2 recorded by the
+   forward pass): from the adjoints of the new velocity (lu3.., ghosted; NULL
+   when the pass's velocity is overwritten later) and of f (lf, 3*N) to the
+   adjoints of u* (added into lu1.. at the update positions), of the
+   interior pressure (added into lX, N) and of the gain (added to *g_gain) */
+void orc_dfib_adjoint(const cfd_grid_desc *g, double dt, double gain, const double *eta, const double *t2,
+                      const double *lu3, const double *lv3, const double *lw3, const double *lf,
+                      double *lu1, double *lv1, double *lw1, double *lX, double *g_gain) {
+  const dims_t d = dims_of(g);
+  const size_t N = d.N;
+  const double *L3[3] = {lu3, lv3, lw3};
+  double *L1[3] = {lu1, lv1, lw1};
+  const double *Ds[3] = {g->Dxs, g->Dys, g->Dzs}, *Dd[3] = {g->Dx, g->Dy, g->Dz};
+  const size_t sg[3] = {d.nzny, (size_t)d.nz, 1};
+  const size_t si[3] = {(size_t)d.nyc * d.nzc, (size_t)d.nzc, 1};
+  double gg = 0.0;
+  for (int q = 0; q < 3; ++q) {
+    const int iend = d.nx - GC - (q == 0), jend = d.ny - GC - (q == 1), kend = d.nz - GC - (q == 2);
+    for (int i = GC; i < iend; ++i)
+      for (int j = GC; j < jend; ++j)
+        for (int k = GC; k < kend; ++k) {
+          const size_t ic = IC(d, i, j, k), t = ICAL(d, i, j, k);
+          const int I = q == 0 ? i : (q == 1 ? j : k);
+          const double ds = Ds[q][I];
+          const double e = eta[ic] + (eta[ic + sg[q]] - eta[ic]) * ((ds * 0.5) / Dd[q][I]);
+          const double T2 = t2[q * N + t];
+          /* u3 = (1 - gain e) T2; f = (u3 - T2)/dt */
+          const double lu3t = (L3[q] ? L3[q][ic] : 0.0) + lf[q * N + t] / dt;
+          const double lT2 = (1.0 - gain * e) * lu3t - lf[q * N + t] / dt;
+          gg += -e * T2 * lu3t;
+          L1[q][ic] += lT2;
+          /* T2 = u* - dt (p[+] - p) / ds */
+          lX[t] += dt / ds * lT2;
+          lX[t + si[q]] -= dt / ds * lT2;
+        }
+  }
+  *g_gain += gg;
+}
+
+/* transpose of createBMatrix_omp (orc_rhs): lb (N) -> lu1.. (ghosted, added) */
+void orc_rhs_adjoint(const cfd_grid_desc *g, double dt, const double *jp, const double *lb,
+                     double *lu1, double *lv1, double *lw1) {
+  const dims_t d = dims_of(g);
+  const double ddt = 1.0 / dt;
+  for (int i = GC; i < d.nx - GC; ++i)
+    for (int j = GC; j < d.ny - GC; ++j)
+      for (int k = GC; k < d.nz - GC; ++k) {
+        const size_t ic = IC(d, i, j, k), row = ICAL(d, i, j, k);
+        const double s = -jp[row] * ddt * lb[row];
+        lu1[ic] += s / g->Dx[i];
+        lu1[IC(d, i - 1, j, k)] -= s / g->Dx[i];
+        lv1[ic] += s / g->Dy[j];
+        lv1[IC(d, i, j - 1, k)] -= s / g->Dy[j];
+        lw1[ic] += s / g->Dz[k];
+        lw1[IC(d, i, j, k - 1)] -= s / g->Dz[k];
+      }
+}
+
+/* Coefficients of a QUICK face pair (quickA / quickB above) with respect to
+   phi at (pp, p, c, n, nn), for the upwind branches taken at (Up, Un). */
+static void quick_coef(int typeA, double Up, double Un, const double *Di, const double *Ds, int I,
+                       double cP[5], double cN[5]) {
+  for (int m = 0; m < 5; ++m) cP[m] = cN[m] = 0.0;
+  if (Up > 0.0) { /* phiP = (c+p)/2 - a ((p-c)/h1 - (c-n)/h0) */
+    const double a = typeA ? 0.125 * Di[I + 1] * Di[I + 1] / Ds[I] : 0.125 * Ds[I] * Ds[I] / Di[I];
+    const double h1 = typeA ? Di[I + 1] : Ds[I], h0 = typeA ? Di[I] : Ds[I - 1];
+    cP[1] = 0.5 - a / h1; cP[2] = 0.5 + a / h1 + a / h0; cP[3] = -a / h0;
+  } else {        /* phiP = (c+p)/2 - a ((pp-p)/h2 - (p-c)/h1) */
+    const double a = typeA ? 0.125 * Di[I + 1] * Di[I + 1] / Ds[I + 1] : 0.125 * Ds[I] * Ds[I] / Di[I + 1];
+    const double h2 = typeA ? Di[I + 2] : Ds[I + 1], h1 = typeA ? Di[I + 1] : Ds[I];
+    cP[0] = -a / h2; cP[1] = 0.5 + a / h2 + a / h1; cP[2] = 0.5 - a / h1;
+  }
+  if (Un > 0.0) { /* phiN = (n+c)/2 - b ((c-n)/h0 - (n-nn)/hm) */
+    const double b = typeA ? 0.125 * Di[I] * Di[I] / Ds[I - 1] : 0.125 * Ds[I - 1] * Ds[I - 1] / Di[I - 1];
+    const double h0 = typeA ? Di[I] : Ds[I - 1], hm = typeA ? Di[I - 1] : Ds[I - 2];
+    cN[2] = 0.5 - b / h0; cN[3] = 0.5 + b / h0 + b / hm; cN[4] = -b / hm;
+  } else {        /* phiN = (n+c)/2 - b ((p-c)/h1 - (c-n)/h0) */
+    const double b = typeA ? 0.125 * Di[I] * Di[I] / Ds[I] : 0.125 * Ds[I - 1] * Ds[I - 1] / Di[I];
+    const double h1 = typeA ? Di[I + 1] : Ds[I], h0 = typeA ? Di[I] : Ds[I - 1];
+    cN[1] = -b / h1; cN[2] = 0.5 + b / h1 + b / h0; cN[3] = 0.5 - b / h0;
+  }
+}
+
+/* face values of quickA / quickB from their coefficients */
+static double dot5(const double c[5], const double *phi, const size_t m[5]) {
+  double s = 0.0;
+  for (int q = 0; q < 5; ++q)
+    if (c[q] != 0.0) s += c[q] * phi[m[q]];
+  return s;
+}
+
+/* transpose of the Jacobian of orc_predictor at (u, v, w): ly1.. hold the
+   adjoints of the predicted velocity at the positions the predictor writes;
+   the adjoints of its inputs are added into lu, lv, lw (ghosted) */
+void orc_predictor_adjoint(const cfd_grid_desc *g, double nu, double dt,
+                           const double *u, const double *v, const double *w,
+                           const double *lu1, const double *lv1, const double *lw1,
+                           double *lu, double *lv, double *lw) {
+  const dims_t d = dims_of(g);
+  const size_t sa[3] = {d.nzny, (size_t)d.nz, 1};
+  const double *Di[3] = {g->Dx, g->Dy, g->Dz}, *Ds[3] = {g->Dxs, g->Dys, g->Dzs};
+  const double *F[3] = {u, v, w};
+  const double *LY[3] = {lu1, lv1, lw1};
+  double *LX[3] = {lu, lv, lw};
+  const double nudt = nu * dt;
+  for (int c = 0; c < 3; ++c) {
+    const int iend = d.nx - GC - (c == 0), jend = d.ny - GC - (c == 1), kend = d.nz - GC - (c == 2);
+    for (int i = GC; i < iend; ++i)
+      for (int j = GC; j < jend; ++j)
+        for (int k = GC; k < kend; ++k) {
+          const size_t ic = IC(d, i, j, k);
+          const double ly = LY[c][ic];
+          if (ly == 0.0) continue;
+          const int idx[3] = {i, j, k};
+          const double *phi = F[c];
+          double *lphi = LX[c];
+          lphi[ic] += ly; /* u1 = u + ... */
+          for (int a = 0; a < 3; ++a) { /* flux direction a */
+            const size_t s = sa[a];
+            const int I = idx[a];
+            const size_t m[5] = {ic + 2 * s, ic + s, ic, ic - s, ic - 2 * s}; /* pp p c n nn */
+            const int own = a == c;
+            /* convecting velocities on the +a / -a faces of the component's cell */
+            double Vp, Vn;
+            int vc = a; /* the velocity component normal to direction a */
+            double IT = 0.0;
+            size_t vpm0 = 0, vpm1 = 0, vnm0 = 0, vnm1 = 0;
+            if (own) {
+              Vp = 0.5 * (phi[ic + s] + phi[ic]);
+              Vn = 0.5 * (phi[ic - s] + phi[ic]);
+            } else {
+              /* interpolation of velocity a to the c-face, along direction c */
+              const int Ic = idx[c];
+              IT = (Ds[c][Ic] * 0.5) / Di[c][Ic];
+              const size_t sc = sa[c];
+              vpm0 = ic; vpm1 = ic + sc;           /* Vp = F[a][ic] + (F[a][ic+sc] - F[a][ic]) IT */
+              vnm0 = ic - s; vnm1 = ic - s + sc;   /* Vn at the -a neighbour */
+              Vp = F[vc][vpm0] + (F[vc][vpm1] - F[vc][vpm0]) * IT;
+              Vn = F[vc][vnm0] + (F[vc][vnm1] - F[vc][vnm0]) * IT;
+            }
+            double cP[5], cN[5];
+            quick_coef(own, Vp, Vn, Di[a], Ds[a], I, cP, cN);
+            const double phP = dot5(cP, phi, m), phN = dot5(cN, phi, m);
+            /* conv_a = -dt (phP*Vp' - phN*Vn') / H, own: Vp' = phP, H = Ds; else H = Di */
+            const double H = own ? Ds[a][I] : Di[a][I];
+            double lP, lN;
+            if (own) {
+              lP = -dt / H * 2.0 * phP * ly;
+              lN = dt / H * 2.0 * phN * ly;
+            } else {
+              lP = -dt / H * Vp * ly;
+              lN = dt / H * Vn * ly;
+              const double lVp = -dt / H * phP * ly, lVn = dt / H * phN * ly;
+              LX[vc][vpm0] += (1.0 - IT) * lVp;
+              LX[vc][vpm1] += IT * lVp;
+              LX[vc][vnm0] += (1.0 - IT) * lVn;
+              LX[vc][vnm1] += IT * lVn;
+            }
+            for (int q = 0; q < 5; ++q) lphi[m[q]] += cP[q] * lP + cN[q] * lN;
+            /* diffusion along a: ((phi+ - phi)/hp - (phi - phi-)/hm) / H2 */
+            double hp, hm, H2;
+            if (own) { hp = Di[a][I + 1]; hm = Di[a][I]; H2 = Ds[a][I]; }
+            else { hp = Ds[a][I]; hm = Ds[a][I - 1]; H2 = Di[a][I]; }
+            const double k1 = nudt * ly / H2;
+            lphi[ic + s] += k1 / hp;
+            lphi[ic] -= k1 / hp + k1 / hm;
+            lphi[ic - s] += k1 / hm;
+          }
+        }
+  }
+}
+
+/* transpose of a slot-major ELL matrix into the same layout (padding
+   entries, value 0, dropped; rows of the transpose padded with (N-1, 0)) */
+int orc_ell_transpose(size_t N, const int32_t *col, const double *val, int32_t *colT, double *valT) {
+  int *cnt = (int *)calloc(N ? N : 1, sizeof(int));
+  if (!cnt) return -1;
+  for (size_t q = 0; q < 7 * N; ++q) { colT[q] = (int32_t)(N - 1); valT[q] = 0.0; }
+  for (size_t r = 0; r < N; ++r)
+    for (int s = 0; s < 7; ++s) {
+      const double a = val[(size_t)s * N + r];
+      if (a == 0.0) continue;
+      const size_t c = (size_t)col[(size_t)s * N + r];
+      if (cnt[c] >= 7) { free(cnt); return -1; }
+      colT[(size_t)cnt[c] * N + c] = (int32_t)r;
+      valT[(size_t)cnt[c] * N + c] = a;
+      cnt[c]++;
+    }
+  free(cnt);
+  return 0;
+}
+
+/* ========================================================================
  * runSeqOmp_predictionMethod main loop (RUN_CPU_PC_METHOD.hpp:287-627)
  * ===================================================================== */
 struct orc_sim {
@@ -968,6 +1262,14 @@ struct orc_sim {
   orc_bicg *bicg;
   int loop;
   double time;
+  /* adjoint parameters and the forward record of an objective run */
+  double *uin;      /* inflow profile (ny_cal) or NULL */
+  double gain;      /* DFIB forcing gain */
+  double *t2_rec;   /* where sim_pass writes T2 (3*N), or NULL */
+  int adj_steps, adj_avg_from, adj_passes;
+  double *adj_x;    /* T0 at the start of every step: steps x 3G */
+  double *adj_t2;   /* T2 of every pass: steps x passes x 3N */
+  int *adj_solved;  /* loop > 1 for every step */
 };
 
 orc_sim *orc_sim_create(const cfd_grid_desc *g, const cfd_params *prm) {
@@ -997,7 +1299,15 @@ orc_sim *orc_sim_create(const cfd_grid_desc *g, const cfd_params *prm) {
   s->spe_jp = (double *)calloc(N, sizeof(double));
   s->loop = 1;
   s->time = 0.0;
+  s->gain = 1.0;
   return s;
+}
+
+static void adj_free(orc_sim *s) {
+  free(s->adj_x); free(s->adj_t2); free(s->adj_solved);
+  s->adj_x = s->adj_t2 = NULL;
+  s->adj_solved = NULL;
+  s->adj_steps = 0;
 }
 
 void orc_sim_destroy(orc_sim *s) {
@@ -1006,8 +1316,22 @@ void orc_sim_destroy(orc_sim *s) {
   free(s->p); free(s->eta); free(s->f); free(s->src); free(s->b); free(s->X); free(s->jp);
   free(s->va); free(s->vb); free(s->col); free(s->val); free(s->spe_coef); free(s->spe_jp);
   orc_bicg_destroy(s->bicg);
+  free(s->uin);
+  adj_free(s);
   free(s);
 }
+
+int orc_sim_set_inflow(orc_sim *s, const double *uin) {
+  free(s->uin);
+  s->uin = NULL;
+  if (!uin) return 0;
+  s->uin = (double *)malloc(sizeof(double) * (size_t)s->d.nyc);
+  if (!s->uin) return -1;
+  memcpy(s->uin, uin, sizeof(double) * (size_t)s->d.nyc);
+  return 0;
+}
+
+void orc_sim_set_gain(orc_sim *s, double gain) { s->gain = gain; }
 
 double *orc_sim_field(orc_sim *s, int field, size_t *len) {
   const size_t G = s->d.G, N = s->d.N;
@@ -1057,7 +1381,8 @@ int orc_sim_init_state(orc_sim *s, double noise_amp, uint64_t seed) {
   memset(s->X, 0, sizeof(double) * d.N);
   memset(s->f, 0, sizeof(double) * 3 * d.N);
   memset(s->src, 0, sizeof(double) * 3 * d.N);
-  orc_bc_main(&s->g, s->prm.bc_dir, s->prm.bc_num, s->prm.p_dirichlet, s->T0[0], s->T0[1], s->T0[2], s->p);
+  orc_bc_main_ex(&s->g, s->prm.bc_dir, s->prm.bc_num, s->prm.p_dirichlet, s->uin, s->T0[0], s->T0[1], s->T0[2],
+                 s->p);
   orc_bc_copy(&s->g, s->T0[0], s->T0[1], s->T0[2], s->T1[0], s->T1[1], s->T1[2]);
   s->loop = 1;
   s->time = 0.0;
@@ -1095,7 +1420,8 @@ static void sim_pass(orc_sim *s, int pass, int accumulate, cfd_step_stats *st) {
       rel = orc_bicg_solve(s->bicg, s->b, s->X, &it);
     orc_pressure_scatter(g, s->X, s->p);
   }
-  orc_dfib_update(g, P->dt, s->eta, s->p, s->T1[0], s->T1[1], s->T1[2], s->T3[0], s->T3[1], s->T3[2], s->f);
+  orc_dfib_update_ex(g, P->dt, s->gain, s->eta, s->p, s->T1[0], s->T1[1], s->T1[2], s->T3[0], s->T3[1], s->T3[2],
+                     s->f, s->t2_rec ? s->t2_rec + (size_t)pass * 3 * s->d.N : NULL);
   if (accumulate) orc_source_accumulate(g, P->dt, s->f, s->src);
   if (st && pass < CFD_MAX_PASSES) {
     st->iters[pass] = it;
@@ -1116,10 +1442,145 @@ int orc_sim_step(orc_sim *s, cfd_step_stats *st) {
   orc_maxdiff(g, s->T0[0], s->T0[1], s->T0[2], s->T3[0], s->T3[1], s->T3[2], md);
   if (st) memcpy(st->maxdiff, md, sizeof(md));
   for (int c = 0; c < 3; ++c) vcopy(d.G, s->T3[c], s->T0[c]);
-  orc_bc_main(g, P->bc_dir, P->bc_num, P->p_dirichlet, s->T0[0], s->T0[1], s->T0[2], s->p);
+  orc_bc_main_ex(g, P->bc_dir, P->bc_num, P->p_dirichlet, s->uin, s->T0[0], s->T0[1], s->T0[2], s->p);
   orc_bc_copy(g, s->T0[0], s->T0[1], s->T0[2], s->T1[0], s->T1[1], s->T1[2]);
   /* simuClass::finishloop (simuClass.hpp:152-169) */
   s->time += P->dt;
   s->loop++;
+  return 0;
+}
+
+/* ========================================================================
+ * Objective and gradient (no reference counterpart): J = mean of the
+ * corrector pass's cD over steps [avg_from, nsteps) of a run from the
+ * current state; gradient with respect to the inflow profile u_in(y) of the
+ * face-0 u Dirichlet value and to the DFIB forcing gain.  The forward run
+ * records every step's starting T0 and every pass's T2, so the reverse
+ * sweep needs no recomputation; the transposed pressure solve runs
+ * BiCGSTAB on the explicit transpose of the assembled ELL operator.
+ * ===================================================================== */
+
+double orc_sim_objective(orc_sim *s, int nsteps, int avg_from) {
+  const size_t G = s->d.G, N = s->d.N;
+  const int passes = s->prm.inner_loops + 1;
+  adj_free(s);
+  if (nsteps < 1 || avg_from < 0 || avg_from >= nsteps) return NAN;
+  s->adj_x = (double *)malloc(sizeof(double) * 3 * G * (size_t)nsteps);
+  s->adj_t2 = (double *)calloc(3 * N * (size_t)passes * (size_t)nsteps, sizeof(double));
+  s->adj_solved = (int *)calloc((size_t)nsteps, sizeof(int));
+  if (!s->adj_x || !s->adj_t2 || !s->adj_solved) { adj_free(s); return NAN; }
+  s->adj_steps = nsteps;
+  s->adj_avg_from = avg_from;
+  s->adj_passes = passes;
+  double J = 0.0;
+  for (int st = 0; st < nsteps; ++st) {
+    for (int c = 0; c < 3; ++c) memcpy(s->adj_x + ((size_t)st * 3 + c) * G, s->T0[c], sizeof(double) * G);
+    s->adj_solved[st] = s->loop > 1;
+    s->t2_rec = s->adj_t2 + (size_t)st * passes * 3 * N;
+    cfd_step_stats stats;
+    orc_sim_step(s, &stats);
+    s->t2_rec = NULL;
+    if (st >= avg_from) J += stats.cd[passes - 1];
+  }
+  return J / (double)(nsteps - avg_from);
+}
+
+/* transpose of BC_staggered_copy (orc_bc_copy): ly at the copied positions
+   added into lx */
+static void bc_copy_adjoint(const dims_t d, double *const ly[3], double *const lx[3]) {
+  for (int j = GC; j < d.ny - GC; ++j)
+    for (int k = GC; k < d.nz - GC; ++k) {
+      const int gi[4] = {0, 1, d.nx - 2, d.nx - 1};
+      for (int q = 0; q < 4; ++q)
+        for (int c = 0; c < 3; ++c) lx[c][IC(d, gi[q], j, k)] += ly[c][IC(d, gi[q], j, k)];
+      lx[0][IC(d, d.nx - 3, j, k)] += ly[0][IC(d, d.nx - 3, j, k)];
+    }
+  for (int i = GC; i < d.nx - GC; ++i)
+    for (int k = GC; k < d.nz - GC; ++k) {
+      const int gj[4] = {0, 1, d.ny - 2, d.ny - 1};
+      for (int q = 0; q < 4; ++q)
+        for (int c = 0; c < 3; ++c) lx[c][IC(d, i, gj[q], k)] += ly[c][IC(d, i, gj[q], k)];
+      lx[1][IC(d, i, d.ny - 3, k)] += ly[1][IC(d, i, d.ny - 3, k)];
+    }
+  for (int i = GC; i < d.nx - GC; ++i)
+    for (int j = GC; j < d.ny - GC; ++j) {
+      const int gk[4] = {0, 1, d.nz - 2, d.nz - 1};
+      for (int q = 0; q < 4; ++q)
+        for (int c = 0; c < 3; ++c) lx[c][IC(d, i, j, gk[q])] += ly[c][IC(d, i, j, gk[q])];
+      lx[2][IC(d, i, j, d.nz - 3)] += ly[2][IC(d, i, j, d.nz - 3)];
+    }
+}
+
+int orc_sim_gradient(orc_sim *s, double *g_uin, double *g_gain, int *adj_iters) {
+  const cfd_params *P = &s->prm;
+  const cfd_grid_desc *g = &s->g;
+  const dims_t d = s->d;
+  const size_t G = d.G, N = d.N;
+  if (!s->adj_steps || P->source_index_quirk || P->op != CFD_OP_ELL) return -1;
+  const int passes = s->adj_passes, L = passes - 1;
+  const double dt = P->dt, nu = 1.0 / P->Re;
+  const double wJ = 1.0 / (double)(s->adj_steps - s->adj_avg_from);
+  double *buf = (double *)calloc(12 * G + 6 * N + 3 * N + 7 * N, sizeof(double));
+  int32_t *colT = (int32_t *)malloc(sizeof(int32_t) * 7 * N);
+  if (!buf || !colT) { free(buf); free(colT); return -1; }
+  double *lx[3] = {buf, buf + G, buf + 2 * G};
+  double *lsum[3] = {buf + 3 * G, buf + 4 * G, buf + 5 * G};
+  double *lT1[3] = {buf + 6 * G, buf + 7 * G, buf + 8 * G};
+  double *Lam = buf + 12 * G, *lf = Lam + 3 * N, *lX = lf + 3 * N, *lb = lX + N, *zl = lb + N;
+  double *valT = zl + N;
+  (void)lb;
+  if (orc_ell_transpose(N, s->col, s->val, colT, valT) != 0) { free(buf); free(colT); return -1; }
+  orc_bicg *bt = orc_bicg_create(N, CFD_OP_ELL, colT, valT, g, NULL, NULL);
+  orc_bicg_set_tolerance(bt, P->tol);
+  orc_bicg_set_maxit(bt, P->maxit);
+  double gg = 0.0;
+  int total_it = 0;
+  for (int q = 0; q < d.nyc; ++q) g_uin[q] = 0.0;
+  for (int st = s->adj_steps - 1; st >= 0; --st) {
+    const double *x = s->adj_x + (size_t)st * 3 * G;
+    const double *t2 = s->adj_t2 + (size_t)st * passes * 3 * N;
+    /* T0' = BC(T3): lx <- adjoint of T3 */
+    orc_bc_main_adjoint(g, P->bc_num, g_uin, lx[0], lx[1], lx[2]);
+    memset(lsum[0], 0, sizeof(double) * 3 * G);
+    memset(Lam, 0, sizeof(double) * 3 * N);
+    for (int q = L; q >= 0; --q) {
+      memset(lT1[0], 0, sizeof(double) * 3 * G);
+      memset(lX, 0, sizeof(double) * N);
+      for (size_t m = 0; m < 3 * N; ++m) lf[m] = dt * Lam[m];
+      if (q == L && st >= s->adj_avg_from) /* cD = -2 sum f_x dV / lz */
+        for (int i = GC; i < d.nx - GC; ++i)
+          for (int j = GC; j < d.ny - GC; ++j)
+            for (int k = GC; k < d.nz - GC; ++k)
+              lf[ICAL(d, i, j, k)] += wJ * (-2.0 / g->lz) * (g->Dx[i] * g->Dy[j] * g->Dz[k]);
+      orc_dfib_adjoint(g, dt, s->gain, s->eta, t2 + (size_t)q * 3 * N, q == L ? lx[0] : NULL,
+                       q == L ? lx[1] : NULL, q == L ? lx[2] : NULL, lf, lT1[0], lT1[1], lT1[2], lX, &gg);
+      if (s->adj_solved[st]) {
+        int it = 0;
+        orc_bicg_solve(bt, lX, zl, &it); /* A^T zl = lX, warm start from the last adjoint solve */
+        total_it += it;
+        orc_rhs_adjoint(g, dt, s->jp, zl, lT1[0], lT1[1], lT1[2]);
+      }
+      /* T1 = P + S_q at every interior cell (source added at its own cell) */
+      for (int i = GC; i < d.nx - GC; ++i)
+        for (int j = GC; j < d.ny - GC; ++j)
+          for (int k = GC; k < d.nz - GC; ++k) {
+            const size_t m = IC(d, i, j, k), t = ICAL(d, i, j, k);
+            for (int c = 0; c < 3; ++c) Lam[(size_t)c * N + t] += lT1[c][m];
+          }
+      for (int c = 0; c < 3; ++c)
+        for (size_t m = 0; m < G; ++m) lsum[c][m] += lT1[c][m];
+    }
+    /* T1 = predictor(T0) at the predictor's positions, T0 copied elsewhere */
+    memset(lx[0], 0, sizeof(double) * 3 * G);
+    orc_predictor_adjoint(g, nu, dt, x, x + G, x + 2 * G, lsum[0], lsum[1], lsum[2], lx[0], lx[1], lx[2]);
+    bc_copy_adjoint(d, lsum, lx);
+  }
+  /* the initial state's ghosts: BC_staggered_main of the initial field */
+  orc_bc_main_adjoint(g, P->bc_num, g_uin, lx[0], lx[1], lx[2]);
+  *g_gain = gg;
+  if (adj_iters) *adj_iters = total_it;
+  orc_bicg_destroy(bt);
+  free(buf);
+  free(colT);
   return 0;
 }

@@ -120,6 +120,40 @@ int orc_sim_step(orc_sim *s, cfd_step_stats *st);
 int orc_sim_loop(const orc_sim *s);
 void orc_sim_set_loop(orc_sim *s, int loop, double time);
 
+/* ---- adjoint parameters (no reference counterpart) ---------------------- */
+/* update_UandF with forcing gain (u_new = (1 - gain*eta_f) T2; gain 1 is the
+   reference bit for bit); t2 (3*N, may be NULL) receives T2 */
+void orc_dfib_update_ex(const cfd_grid_desc *g, double dt, double gain, const double *eta, const double *p,
+                        const double *u1, const double *v1, const double *w1,
+                        double *u3, double *v3, double *w3, double *f, double *t2);
+/* BC_staggered_main with the face-0 u Dirichlet value replaced by uin[j-2] (uin may be NULL) */
+void orc_bc_main_ex(const cfd_grid_desc *g, const double dir[6][3], const double num[6][3],
+                    const int p_dirichlet[6], const double *uin, double *u, double *v, double *w, double *p);
+/* inflow profile (ny_cal values, NULL: table value) and gain of the simulation */
+int orc_sim_set_inflow(orc_sim *s, const double *uin);
+void orc_sim_set_gain(orc_sim *s, double gain);
+
+/* ---- discrete adjoint (transposes of the functions above) --------------- */
+void orc_bc_main_adjoint(const cfd_grid_desc *g, const double num[6][3], double *g_uin,
+                         double *lu, double *lv, double *lw);
+void orc_dfib_adjoint(const cfd_grid_desc *g, double dt, double gain, const double *eta, const double *t2,
+                      const double *lu3, const double *lv3, const double *lw3, const double *lf,
+                      double *lu1, double *lv1, double *lw1, double *lX, double *g_gain);
+void orc_rhs_adjoint(const cfd_grid_desc *g, double dt, const double *jp, const double *lb,
+                     double *lu1, double *lv1, double *lw1);
+void orc_predictor_adjoint(const cfd_grid_desc *g, double nu, double dt,
+                           const double *u, const double *v, const double *w,
+                           const double *lu1, const double *lv1, const double *lw1,
+                           double *lu, double *lv, double *lw);
+int orc_ell_transpose(size_t N, const int32_t *col, const double *val, int32_t *colT, double *valT);
+/* J = mean corrector-pass cD over steps [avg_from, nsteps) from the current
+   state, recording the run for orc_sim_gradient; NaN on bad arguments */
+double orc_sim_objective(orc_sim *s, int nsteps, int avg_from);
+/* dJ/du_in (ny_cal values) and dJ/dgain of the last objective run; -1 when
+   there is none or the configuration is not differentiable here (source
+   index quirk, matrix-free operator) */
+int orc_sim_gradient(orc_sim *s, double *g_uin, double *g_gain, int *adj_iters);
+
 #ifdef __cplusplus
 }
 #endif

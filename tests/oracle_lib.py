@@ -95,6 +95,24 @@ def _declare(L):
     L.orc_sim_step.argtypes = [C.c_void_p, C.POINTER(CfdStepStats)]
     L.orc_sim_loop.argtypes = [C.c_void_p]
     L.orc_sim_set_loop.argtypes = [C.c_void_p, C.c_int, C.c_double]
+    # adjoint
+    T = C.POINTER((C.c_double * 3) * 6)
+    L.orc_dfib_update_ex.argtypes = [G, C.c_double, C.c_double, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.c_void_p]
+    L.orc_bc_main_ex.argtypes = [G, T, T, C.POINTER(C.c_int), C.c_void_p, dp, dp, dp, dp]
+    L.orc_sim_set_inflow.argtypes = [C.c_void_p, C.c_void_p]
+    L.orc_sim_set_inflow.restype = C.c_int
+    L.orc_sim_set_gain.argtypes = [C.c_void_p, C.c_double]
+    L.orc_bc_main_adjoint.argtypes = [G, T, C.c_void_p, dp, dp, dp]
+    L.orc_dfib_adjoint.argtypes = [G, C.c_double, C.c_double, dp, dp, C.c_void_p, C.c_void_p, C.c_void_p, dp,
+                                   dp, dp, dp, dp, C.POINTER(C.c_double)]
+    L.orc_rhs_adjoint.argtypes = [G, C.c_double, dp, dp, dp, dp, dp]
+    L.orc_predictor_adjoint.argtypes = [G, C.c_double, C.c_double, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+    L.orc_ell_transpose.argtypes = [C.c_size_t, ip, dp, ip, dp]
+    L.orc_ell_transpose.restype = C.c_int
+    L.orc_sim_objective.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.orc_sim_objective.restype = C.c_double
+    L.orc_sim_gradient.argtypes = [C.c_void_p, dp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.orc_sim_gradient.restype = C.c_int
 
 
 class Grid:
@@ -196,3 +214,24 @@ class Sim:
 
     def set_loop(self, loop, time):
         lib().orc_sim_set_loop(self.h, loop, time)
+
+    def set_inflow(self, uin):
+        self._uin = None if uin is None else np.ascontiguousarray(uin, dtype=np.float64)
+        ptr = None if self._uin is None else self._uin.ctypes.data_as(C.c_void_p)
+        if lib().orc_sim_set_inflow(self.h, ptr) != 0:
+            raise MemoryError("orc_sim_set_inflow failed")
+
+    def set_gain(self, gain):
+        lib().orc_sim_set_gain(self.h, float(gain))
+
+    def objective(self, nsteps, avg_from) -> float:
+        return lib().orc_sim_objective(self.h, nsteps, avg_from)
+
+    def gradient(self):
+        """(dJ/du_in, dJ/dgain, adjoint BiCGSTAB iterations) of the last objective run"""
+        g_uin = np.zeros(self.grid.nyc)
+        g_gain = C.c_double()
+        it = C.c_int()
+        if lib().orc_sim_gradient(self.h, g_uin, C.byref(g_gain), C.byref(it)) != 0:
+            raise RuntimeError("orc_sim_gradient failed")
+        return g_uin, g_gain.value, it.value
