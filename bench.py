@@ -182,6 +182,50 @@ def kernel_table(sim, reps):
     return rows
 
 
+def adjoint_benchmark(steps, avg_from, every, fd_dirs):
+    """Discrete adjoint on the 256x128x32 Re-100 cylinder (dt 1e-3): J = mean
+    corrector cD over steps [avg_from, steps), gradient w.r.t. the seeded
+    smoothed inflow profile (128 values) and the DFIB forcing gain; timed
+    objective run (with checkpoints) and reverse sweep (with recomputation);
+    optionally central differences (h = 1e-6) in fd_dirs random directions."""
+    from cfd_nssolver_forhpc_amd.adjoint import DragProblem, smoothed_inflow
+    w = configs.config_d()
+    sim = Simulation(w.grid, w.params)
+    sim.init_state(w.noise, w.seed)
+    sim.DFIB_CylinderZ()
+    sim.createPressureMatrix()
+    pr = DragProblem(sim, steps, avg_from, every, noise=w.noise, seed=w.seed)
+    uin = smoothed_inflow(w.grid.ncal[1])
+    log(f"[bench] adjoint {w.name}: {steps} steps, J over [{avg_from}, {steps}), checkpoint every {every}")
+    t0 = time.perf_counter()
+    J = pr.run(uin, 1.0)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g_uin, g_gain, st = sim.adjoint_backward()
+    t_bwd = time.perf_counter() - t0
+    log(f"[bench] adjoint: J {J:.6f}, forward {t_fwd:.2f} s, reverse sweep {t_bwd:.2f} s")
+    checks = []
+    rng = np.random.default_rng(8)
+    h = 1e-6
+    for q in range(fd_dirs):
+        d = rng.standard_normal(uin.size + 1)
+        d /= np.linalg.norm(d)
+        Jp = pr.run(uin + h * d[:-1], 1.0 + h * d[-1])
+        Jm = pr.run(uin - h * d[:-1], 1.0 - h * d[-1])
+        fd = (Jp - Jm) / (2 * h)
+        ad = float(np.dot(g_uin, d[:-1]) + g_gain * d[-1])
+        checks.append({"fd": fd, "adjoint": ad, "rel_diff": abs(fd - ad) / max(abs(fd), abs(ad), 1e-300)})
+        log(f"[bench] adjoint FD direction {q}: fd {fd:.9e} adjoint {ad:.9e}")
+    sim.close()
+    return {
+        "workload": w.name, "steps": steps, "avg_from": avg_from, "checkpoint_every": every,
+        "J_mean_cd": J, "grad_uin_norm": float(np.linalg.norm(g_uin)), "grad_gain": g_gain,
+        "forward_s": t_fwd, "reverse_sweep_s": t_bwd, "reverse_over_forward": t_bwd / t_fwd,
+        "forward_ms_per_step": 1e3 * t_fwd / steps, "reverse_ms_per_step": 1e3 * t_bwd / steps,
+        "stats": st, "fd_checks": checks,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,7 +235,14 @@ def main():
     ap.add_argument("--no-spe", action="store_true")
     ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mg", action="store_true")
+    ap.add_argument("--mg-steps", type=int, default=20, help="timed steps of the multigrid-preconditioned runs")
     ap.add_argument("--reps", type=int, default=200)
+    ap.add_argument("--adjoint-steps", type=int, default=40,
+                    help="steps of the 256x128x32 adjoint run (0: skip); the drag is averaged over the second half")
+    ap.add_argument("--adjoint-every", type=int, default=20, help="checkpoint interval of the adjoint run")
+    ap.add_argument("--adjoint-fd", type=int, default=0, help="central-difference directions to check the gradient")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg and print its JSON")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time the operator kernels on the 512x256x64 grid and exit")
     args = ap.parse_args()
@@ -200,6 +251,12 @@ def main():
               "(the hot path runs on a single MI355X; multi-GPU x-slab sharding is design-only)", file=sys.stderr)
         sys.exit(2)
     warmup = max(args.warmup, 2)  # step 1 of the reference loop has no pressure solve
+
+    if args.adjoint_only:
+        steps = max(args.adjoint_steps, 2)
+        print(json.dumps({"adjoint": adjoint_benchmark(steps, steps // 2, args.adjoint_every, args.adjoint_fd)}),
+              flush=True)
+        return
 
     w64 = configs.config_b(capi.CFD_FP64)
     if args.kernels_only:
@@ -242,6 +299,37 @@ def main():
         }
         simm.close()
 
+    mg = None
+    if not args.no_mg:
+        mg = {}
+        for tag, prec in (("fp64_mg", capi.CFD_FP64), ("mixed_mg", capi.CFD_MIXED)):
+            wg = configs.config_b(prec)
+            wg.params.precond = capi.CFD_PRECOND_MG
+            wg.params.mixed_mode = capi.CFD_MIXED_RESTARTED
+            wg.params.tol_inner = 1e-5
+            wg.name += "-mg"
+            simg, tg, stg = run_steps(wg, warmup, args.mg_steps, tag)
+            rec = {"timesteps_per_s": args.mg_steps / tg, "speedup_vs_fp64": (args.mg_steps / tg) / tps64,
+                   "solver_iters_per_step": per_step(stg, "iters"),
+                   "inner_iters_per_step": per_step(stg, "inner_iters"),
+                   "max_final_relres": max(max(s["relres"]) for s in stg)}
+            mg[tag] = rec
+            mg[tag + "_sim"] = simg
+        # the mixed run against the FP64 run with the same preconditioner, after the same steps
+        err = {}
+        a, b_ = mg.pop("mixed_mg_sim"), mg.pop("fp64_mg_sim")
+        for name, fid in (("u", capi.T0_U), ("v", capi.T0_V), ("w", capi.T0_W), ("p", capi.X)):
+            x, y = a.download(fid), b_.download(fid)
+            err[name] = float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+        mg["mixed_mg"]["rel_l2_vs_fp64_mg_after_timed_steps"] = err
+        mg["mixed_mg"]["speedup_vs_fp64_mg"] = mg["mixed_mg"]["timesteps_per_s"] / mg["fp64_mg"]["timesteps_per_s"]
+        mg["note"] = ("BiCGSTAB right-preconditioned by a cell-centred multigrid V-cycle (red-black Gauss-Seidel 2+2, "
+                      "matrix-free levels); fp64_mg: FP64 throughout; mixed_mg: FP64 defect correction to 1e-10 "
+                      "around FP32 preconditioned BiCGSTAB (FP32 ELL values and V-cycle) to 1e-5; speedup_vs_fp64 "
+                      "is against the reference's Jacobi-scaled FP64 solve (the bench value)")
+        a.close()
+        b_.close()
+
     spe = None
     if not args.no_spe:
         ws = configs.config_b(capi.CFD_FP64)
@@ -252,6 +340,10 @@ def main():
         sims.close()
 
     poisson = None if args.no_poisson else poisson_benchmark(w64)
+
+    adjoint = None
+    if args.adjoint_steps > 1:
+        adjoint = adjoint_benchmark(args.adjoint_steps, args.adjoint_steps // 2, args.adjoint_every, args.adjoint_fd)
 
     cpu = None
     if not args.no_cpu:
@@ -285,8 +377,10 @@ def main():
                      "kernel": "ELL SpMV FP64 (100 B/row x 8388608 rows per launch)", "avg_us": spmv["avg_us"]},
         "kernels": kt,
         "mixed": mixed,
+        "multigrid": mg,
         "spe": spe,
         "poisson": poisson,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

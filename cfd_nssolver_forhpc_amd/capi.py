@@ -33,6 +33,8 @@ CFD_MIXED = 1
 CFD_MIXED_RELIABLE = 0
 CFD_MIXED_RESTARTED = 1
 CFD_MIXED_VALUES32 = 2
+CFD_PRECOND_JACOBI = 0
+CFD_PRECOND_MG = 1
 
 # field ids
 T0_U, T0_V, T0_W, T1_U, T1_V, T1_W, T3_U, T3_V, T3_W = range(9)
@@ -67,6 +69,7 @@ class CfdParams(C.Structure):
         ("ib_on", C.c_int), ("ib_cx", C.c_double), ("ib_cy", C.c_double), ("ib_r", C.c_double),
         ("ib_nsub", C.c_int),
         ("u_init", C.c_double * 3),
+        ("precond", C.c_int), ("mg_nu", C.c_int),
     ]
 
     def set_bc(self, dir_table, num_table):
@@ -101,6 +104,17 @@ class CfdStepStats(C.Structure):
             "maxdiff": list(self.maxdiff),
             "inner_iters": list(self.inner_iters[:n]),
         }
+
+
+class CfdAdjointStats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int), ("checkpoints", C.c_int), ("recomputed_steps", C.c_int),
+        ("adjoint_solves", C.c_int), ("adjoint_iters", C.c_int),
+        ("forward_ms", C.c_double), ("backward_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 # Velocity BC tables Dir[face][component], Num[face][component]; faces 0..5 =
@@ -230,6 +244,10 @@ def _declare(L: C.CDLL) -> None:
         "cfd_loop": ([vp, ip, dp], C.c_int),
         "cfd_set_loop": ([vp, C.c_int, C.c_double], C.c_int),
         "cfd_time_kernel": ([vp, C.c_int, C.c_int, dp], C.c_int),
+        "cfd_set_inflow_profile": ([vp, dp, C.c_size_t], C.c_int),
+        "cfd_set_forcing_gain": ([vp, C.c_double], C.c_int),
+        "cfd_adjoint_forward": ([vp, C.c_int, C.c_int, C.c_int, dp], C.c_int),
+        "cfd_adjoint_backward": ([vp, dp, C.c_size_t, dp, C.POINTER(CfdAdjointStats)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -29,7 +29,6 @@ int rbsor_solve_f32(Ctx &c, const float *b, float *x, double omega, double tol, 
                     int *iters, double *relres);
 int stream_blocks(int64_t n);
 hipError_t launch_cast32(Ctx &c, const double *a, float *b, int64_t n);
-hipError_t launch_dfib_update_into(Ctx &c, int mode, const double *p_or_x, double *drag_out, double *max_out);
 
 namespace {
 
@@ -71,6 +70,8 @@ int check_params(const cfd_params *p) {
       !(p->mixed_delta > 0.0 && p->mixed_delta < 1.0))
     return fail(CFD_ERR_ARG, "mixed_delta must be in (0, 1)");
   if (p->ib_on && (p->ib_nsub < 1 || !(p->ib_r > 0.0))) return fail(CFD_ERR_ARG, "invalid immersed body");
+  if (p->precond != CFD_PRECOND_JACOBI && p->precond != CFD_PRECOND_MG) return fail(CFD_ERR_ARG, "unknown preconditioner");
+  if (p->precond == CFD_PRECOND_MG && (p->mg_nu < 1 || p->mg_nu > 8)) return fail(CFD_ERR_ARG, "mg_nu must be in [1, 8]");
   return CFD_OK;
 }
 
@@ -85,6 +86,9 @@ int field_len(const Ctx &c, int field, size_t *len) {
 void free_ctx(Ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  adjoint_release(*c);
+  mg_release(*c);
+  dfree(c->uin);
   for (auto &p : c->f) dfree(p);
   dfree(c->col); dfree(c->val); dfree(c->val32); dfree(c->colP); dfree(c->valP); dfree(c->valP32); dfree(c->perm);
   dfree(c->spe_coef); dfree(c->spe_jp);
@@ -188,7 +192,7 @@ int solve_configured(Ctx &c, int *iters, double *relres, int *inner) {
     }
     if (rc) return rc;
     int outer = 0;
-    if (P.mixed_mode != CFD_MIXED_RESTARTED && P.solver == CFD_SOLVER_BICGSTAB)
+    if (P.mixed_mode != CFD_MIXED_RESTARTED && P.solver == CFD_SOLVER_BICGSTAB && P.precond != CFD_PRECOND_MG)
       rc = mixed_reliable_solve(c, c.f[CFD_RHS], c.f[CFD_X], P.tol, P.mixed_delta, P.maxit, &outer, inner, relres);
     else
       rc = defect_correction(c, c.f[CFD_RHS], c.f[CFD_X], P.tol, P.tol_inner, P.maxit, &outer, inner, relres);
@@ -200,6 +204,7 @@ int solve_configured(Ctx &c, int *iters, double *relres, int *inner) {
     if (rc) return rc;
     return rbsor_solve(c, c.f[CFD_RHS], c.f[CFD_X], P.sor_omega, P.tol, P.maxit, P.sor_check, iters, relres);
   }
+  if (P.precond == CFD_PRECOND_MG) return pbicgstab_solve64(c, c.f[CFD_RHS], c.f[CFD_X], P.tol, P.maxit, iters, relres);
   return bicgstab_solve(c, c.f[CFD_RHS], c.f[CFD_X], P.tol, P.maxit, iters, relres);
 }
 
@@ -251,6 +256,8 @@ int cfd_params_default(cfd_params *out) {
   out->ib_r = 0.5;
   out->ib_nsub = 100;
   out->u_init[0] = 1.0;   // RUN_CPU_PC_METHOD.hpp:77
+  out->precond = CFD_PRECOND_JACOBI;
+  out->mg_nu = 2;
   return CFD_OK;
 }
 
@@ -776,7 +783,14 @@ int cfd_copy_t3_to_t0(cfd_ctx *ctx) {
 // change inside a step, so every evaluation is bit-identical.
 int cfd_step(cfd_ctx *ctx, cfd_step_stats *stats) {
   CTX_CHECK(ctx);
-  Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  return step_impl(*reinterpret_cast<Ctx *>(ctx), stats);
+}
+
+}  // extern "C"
+
+namespace cfd {
+
+int step_impl(Ctx &c, cfd_step_stats *stats) {
   const cfd_params &P = c.prm;
   const bool solve = c.loop > 1;
   if (solve) {
@@ -812,7 +826,9 @@ int cfd_step(cfd_ctx *ctx, cfd_step_stats *stats) {
     if (corrector) mode |= UPD_WRITE_T3 | UPD_WRITE_T0 | UPD_MAXDIFF;
     else mode |= (pass == 0 ? UPD_SRC_SET : UPD_SRC_ADD);
     const double *pp = c.p_from_x ? c.f[CFD_X] : c.f[CFD_P];
-    RET_LAUNCH(launch_dfib_update_into(c, mode, pp, c.stepbuf + 2 * pass, corrector ? c.stepbuf + 32 : nullptr));
+    double *t2 = c.t2_rec ? c.t2_rec + (size_t)3 * c.g.N * pass : nullptr;
+    RET_LAUNCH(
+        launch_dfib_update_into(c, mode, pp, c.stepbuf + 2 * pass, corrector ? c.stepbuf + 32 : nullptr, t2));
   }
   RET_LAUNCH(launch_bc(c, true, true, !c.p_from_x));
   CFD_HIP(hipMemcpyAsync(c.hred, c.stepbuf, 40 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -827,6 +843,10 @@ int cfd_step(cfd_ctx *ctx, cfd_step_stats *stats) {
   if (stats) *stats = st;
   return CFD_OK;
 }
+
+}  // namespace cfd
+
+extern "C" {
 
 int cfd_loop(const cfd_ctx *ctx, int *loop, double *time) {
   if (!ctx) return fail(CFD_ERR_ARG, "context is NULL");

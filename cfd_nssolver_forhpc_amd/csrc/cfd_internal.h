@@ -102,7 +102,34 @@ struct Ctx {
   bool p_from_x = false;         // ghosted p interior lags X (step mode)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool debug = false;  // CFD_DEBUG=1: solver progress on stderr
+  // parameters of the discrete adjoint and its forward record
+  double gain = 1.0;             // DFIB forcing gain
+  double *uin = nullptr;         // face-0 u Dirichlet profile (ny_cal), device; NULL: table value
+  double *t2_rec = nullptr;      // cfd_step records T2 of pass q at t2_rec + 3*N*q when set
+  struct AdjState *adj = nullptr;
+  // multigrid preconditioner (hierarchy built on first use) and the
+  // preconditioned directions of the right-preconditioned BiCGSTAB
+  struct MGState *mg = nullptr;
+  double *ph64 = nullptr, *sh64 = nullptr;
+  float *ph32 = nullptr, *sh32 = nullptr;
 };
+
+// k_mg.hip
+int mg_ensure(Ctx &c, bool fp32);
+void mg_release(Ctx &c);
+int mg_apply64(Ctx &c, const double *r, double *out, const KrylovScalars *ks);
+int mg_apply32(Ctx &c, const float *r, float *out, const KrylovScalars *ks);
+const char *mg_shape(const Ctx &c);
+// k_krylov.hip: BiCGSTAB right-preconditioned by the V-cycle, FP64
+int pbicgstab_solve64(Ctx &c, const double *b, double *x, double tol, int maxit, int *iters, double *relres);
+
+// one timestep (cfd_step's body); adjoint.hip's recomputation calls it too
+int step_impl(Ctx &c, cfd_step_stats *stats);
+void adjoint_release(Ctx &c);
+// FP64 BiCGSTAB on the ELL operator that leaves the pressure solver's state
+// (solve count, NaN-reset history) untouched
+int bicgstab_solve_aux(Ctx &c, const double *b, double *x, double tol, int maxit, int timestep, int *iters,
+                       double *relres);
 
 inline double *red_part_host(const Ctx &c, int site) {
   return c.red.partials + (size_t)site * 3 * RED_MAX_BLOCKS;
@@ -138,7 +165,8 @@ hipError_t launch_scatter_p(Ctx &c, const double *X, double *p);
 // mode bits for the DFIB update
 enum { UPD_WRITE_T3 = 1, UPD_WRITE_T0 = 2, UPD_SRC_SET = 4, UPD_SRC_ADD = 8, UPD_DRAG = 16, UPD_MAXDIFF = 32,
        UPD_P_FROM_X = 64 };
-hipError_t launch_dfib_update_into(Ctx &c, int mode, const double *p_or_x, double *drag_out, double *max_out);
+hipError_t launch_dfib_update_into(Ctx &c, int mode, const double *p_or_x, double *drag_out, double *max_out,
+                                   double *t2out = nullptr);
 hipError_t launch_source_accumulate(Ctx &c);
 hipError_t launch_bc(Ctx &c, bool main, bool copy, bool pressure);
 hipError_t launch_drag(Ctx &c);

@@ -216,7 +216,10 @@ __global__ void __launch_bounds__(256) scatter_kernel(DevGrid g, const double *_
 // 393-401), drag partial sums (:403-435), the steady-state max difference
 // (CheckSteadystate.hpp:165-219) and, in the corrector, the write of the new
 // velocity straight into T0 (the T0 <- T3 copy, physicalVariables.hpp:84-102).
-__global__ void __launch_bounds__(256) dfib_update_kernel(DevGrid g, double dt, int mode,
+// gain scales the forcing (u_new = (1 - gain*eta_f) T2; 1 is the reference,
+// bit for bit); t2out, when set, records T2 for the discrete adjoint.
+__global__ void __launch_bounds__(256) dfib_update_kernel(DevGrid g, double dt, double gain, double *__restrict__ t2out,
+                                                          int mode,
                                                           const double *__restrict__ eta,
                                                           const double *__restrict__ p,
                                                           const double *__restrict__ u1,
@@ -256,9 +259,10 @@ __global__ void __launch_bounds__(256) dfib_update_kernel(DevGrid g, double dt, 
         const double ds = Ds[q][idx[q]];
         const double T2 = comp_in[q] - dt * (pP - pc) / ds;
         const double e = eC + (eta[ic + step_ghost[q]] - eC) * ((ds * 0.5) / Dd[q][idx[q]]);
-        const double nv = eC * 0.0 + (1.0 - e) * T2;
+        const double nv = eC * 0.0 + (1.0 - gain * e) * T2;
         const double fv = (nv - T2) / dt;
         f[q * N + t] = fv;
+        if (t2out) t2out[q * N + t] = T2;
         if (mode & UPD_SRC_SET) src[q * N + t] = dt * fv;
         else if (mode & UPD_SRC_ADD) src[q * N + t] += dt * fv;
         if (mode & UPD_MAXDIFF) mx[q] = fmax(mx[q], outs0[q][ic] - nv);
@@ -292,6 +296,7 @@ __global__ void __launch_bounds__(256) source_accumulate_kernel(int64_t n3, doub
 struct BCArgs {
   double dir[6][3], num[6][3];
   int pd[6];
+  const double *uin;  // face-0 u Dirichlet value per interior row j (NULL: dir[0][0])
 };
 
 __device__ inline double nrm(const BCArgs &a, int b, int c, double x) { return a.dir[b][c] + a.num[b][c] * x; }
@@ -314,7 +319,7 @@ __global__ void __launch_bounds__(256) bc_x_kernel(DevGrid g, BCArgs a, int do_m
                 n4 = icel(g, nx - 4, j, k);
   if (do_main) {
     double x;
-    x = nrm(a, 0, 0, u[m2]); u[m1] = x; u[m0] = x;
+    x = (a.uin ? a.uin[j - GC] : a.dir[0][0]) + a.num[0][0] * u[m2]; u[m1] = x; u[m0] = x;
     x = tan_(a, 0, 1, v[m2]); v[m1] = x; v[m0] = x;
     x = tan_(a, 0, 2, w[m2]); w[m1] = x; w[m0] = x;
     x = nrm(a, 1, 0, u[n4]); u[n3] = x; u[n2] = x; u[n1] = x;
@@ -542,6 +547,7 @@ BCArgs bc_args(const Ctx &c) {
     }
     a.pd[b] = c.prm.p_dirichlet[b];
   }
+  a.uin = c.uin;
   return a;
 }
 
@@ -604,13 +610,14 @@ hipError_t launch_scatter_p(Ctx &c, const double *X, double *p) {
   return hipGetLastError();
 }
 
-hipError_t launch_dfib_update_into(Ctx &c, int mode, const double *p_or_x, double *drag_out, double *max_out) {
+hipError_t launch_dfib_update_into(Ctx &c, int mode, const double *p_or_x, double *drag_out, double *max_out,
+                                   double *t2out) {
   if (!drag_out) mode &= ~UPD_DRAG;
   if (!max_out) mode &= ~UPD_MAXDIFF;
   const int nblk = (mode & (UPD_DRAG | UPD_MAXDIFF)) ? blocks_for(c.g.N, 256, RED_MAX_BLOCKS)
                                                      : blocks_for(c.g.N, 256, 1 << 30);
   dfib_update_kernel<<<nblk, 256, 0, c.stream>>>(
-      c.g, c.prm.dt, mode, c.f[CFD_ETA], p_or_x, c.f[CFD_T1_U], c.f[CFD_T1_V], c.f[CFD_T1_W], c.f[CFD_T3_U],
+      c.g, c.prm.dt, c.gain, t2out, mode, c.f[CFD_ETA], p_or_x, c.f[CFD_T1_U], c.f[CFD_T1_V], c.f[CFD_T1_W], c.f[CFD_T3_U],
       c.f[CFD_T3_V], c.f[CFD_T3_W], c.f[CFD_T0_U], c.f[CFD_T0_V], c.f[CFD_T0_W], c.f[CFD_F], c.f[CFD_SRC],
       red_part_host(c, RS_DRAG), red_part_host(c, RS_MAX));
   hipError_t e = hipGetLastError();

@@ -202,6 +202,44 @@ __global__ void __launch_bounds__(256) p_update_kernel(int64_t N, const KrylovSc
     p1[r] = (T)((double)r1[r] + beta * ((double)p1[r] - omega * (double)ap[r]));
 }
 
+// right-preconditioned variant: t = A sh, partials (t . s), (t . t)
+template <typename T>
+__global__ void __launch_bounds__(256) spmv_dot2_pc_kernel(OpArgs a, const KrylovScalars *__restrict__ ks,
+                                                           const T *__restrict__ sh, const T *__restrict__ s1,
+                                                           T *__restrict__ tv, double *part) {
+  if (ks->done) return;
+  double acc[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) {
+    const T y = op_apply<T>(a, r, sh);
+    tv[r] = y;
+    acc[0] += (double)y * (double)s1[r];
+    acc[1] += (double)y * (double)y;
+  }
+  block_partials<2>(acc, part);
+}
+
+// x += alpha ph + omega sh; r = s - omega t; partials (r . r), (r . r0)
+template <typename T>
+__global__ void __launch_bounds__(256) xr_update_pc_kernel(int64_t N, const KrylovScalars *__restrict__ ks,
+                                                           T *__restrict__ x, const T *__restrict__ ph,
+                                                           const T *__restrict__ sh, const T *__restrict__ s1,
+                                                           const T *__restrict__ tv, T *__restrict__ r1,
+                                                           const T *__restrict__ r0, double *part) {
+  if (ks->done) return;
+  const double alpha = ks->alpha, omega = ks->omega;
+  double acc[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += stride) {
+    x[r] = (T)((double)x[r] + (alpha * (double)ph[r] + omega * (double)sh[r]));
+    const T rn = (T)((double)s1[r] - omega * (double)tv[r]);
+    r1[r] = rn;
+    acc[0] += (double)rn * (double)rn;
+    acc[1] += (double)rn * (double)r0[r];
+  }
+  block_partials<2>(acc, part);
+}
+
 // NaN / Inf reset after iters_max (bicgstabRe2.hpp:143-163)
 __global__ void __launch_bounds__(256) nan_reset_kernel(int64_t N, double *__restrict__ x,
                                                         const double *__restrict__ px1, double *part) {
@@ -329,7 +367,91 @@ int solve_bicg(Ctx &c, const OpArgs &a, const Vecs<T> &v, KrylovScalars *ks, int
   return CFD_OK;
 }
 
+// BiCGSTAB right-preconditioned by the multigrid V-cycle (an addition; the
+// reference's bicgstabRe2 is Jacobi-scaled only):
+//   ph = M p; v = A ph; alpha = rho / (v . r0); s = r - alpha v;
+//   sh = M s; t = A sh; omega = (t . s) / (t . t);
+//   x += alpha ph + omega sh; r = s - omega t; beta; p = r + beta (p - omega v)
+// with the fused kernels and device-side scalars of the plain solve.
+template <typename T>
+struct PVecs {
+  T *x, *r0, *r1, *p1, *s1, *v, *t, *ph, *sh;
+  const T *b;
+};
+
+template <typename T>
+int mg_apply_t(Ctx &c, const T *r, T *out, const KrylovScalars *ks);
+template <>
+int mg_apply_t<double>(Ctx &c, const double *r, double *out, const KrylovScalars *ks) {
+  return mg_apply64(c, r, out, ks);
+}
+template <>
+int mg_apply_t<float>(Ctx &c, const float *r, float *out, const KrylovScalars *ks) {
+  return mg_apply32(c, r, out, ks);
+}
+
+template <typename T>
+int launch_pc_iters(Ctx &c, const OpArgs &a, const PVecs<T> &v, KrylovScalars *ks, int nit) {
+  const int64_t N = c.g.N;
+  const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
+  double *p1 = red_part_host(c, RS_SPMV_DOT), *p2 = red_part_host(c, RS_SPMV_DOT2), *p3 = red_part_host(c, RS_XR);
+  for (int it = 0; it < nit; ++it) {
+    int rc = mg_apply_t<T>(c, v.p1, v.ph, ks);
+    if (rc) return rc;
+    spmv_dot_kernel<T><<<nb, 256, 0, c.stream>>>(a, ks, v.ph, v.v, v.r0, p1);
+    alpha_finalize<<<1, 256, 0, c.stream>>>(p1, nb, ks);
+    s_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.r1, v.v, v.s1);
+    if ((rc = mg_apply_t<T>(c, v.s1, v.sh, ks))) return rc;
+    spmv_dot2_pc_kernel<T><<<nb, 256, 0, c.stream>>>(a, ks, v.sh, v.s1, v.t, p2);
+    omega_finalize<<<1, 256, 0, c.stream>>>(p2, nb, ks);
+    xr_update_pc_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.x, v.ph, v.sh, v.s1, v.t, v.r1, v.r0, p3);
+    conv_finalize<<<1, 256, 0, c.stream>>>(p3, nb, ks);
+    p_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.r1, v.p1, v.v, v.r0);
+    CFD_HIP(hipGetLastError());
+  }
+  return CFD_OK;
+}
+
+template <typename T>
+int solve_pbicg(Ctx &c, const OpArgs &a, const PVecs<T> &v, KrylovScalars *ks, double tol, int maxit, int *iters,
+                double *relres) {
+  const int64_t N = c.g.N;
+  const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
+  ks_setup_kernel<<<1, 1, 0, c.stream>>>(ks, tol, 0.0, maxit, 0);
+  init_kernel<T><<<nb, 256, 0, c.stream>>>(a, v.b, v.x, v.r0, v.r1, v.p1, nullptr, nullptr, red_part_host(c, RS_INIT));
+  init_finalize<<<1, 256, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, ks);
+  CFD_HIP(hipGetLastError());
+  CFD_HIP(hipMemcpyAsync(c.hks, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c.stream));
+  CFD_HIP(hipStreamSynchronize(c.stream));
+  if (!(c.hks->relres >= tol)) {  // already converged (or a zero right-hand side)
+    *iters = 0;
+    *relres = c.hks->norm0 > 0.0 ? c.hks->relres : 0.0;
+    return CFD_OK;
+  }
+  int batch = 2;
+  for (;;) {
+    int rc = launch_pc_iters<T>(c, a, v, ks, batch);
+    if (rc) return rc;
+    CFD_HIP(hipMemcpyAsync(c.hks, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c.stream));
+    CFD_HIP(hipStreamSynchronize(c.stream));
+    if (c.hks->done) break;
+    batch = std::min(batch * 2, 8);
+  }
+  *iters = c.hks->iters;
+  *relres = c.hks->relres;
+  return CFD_OK;
+}
+
 }  // namespace
+
+int pbicgstab_solve64(Ctx &c, const double *b, double *x, double tol, int maxit, int *iters, double *relres) {
+  if (!c.ell_ready) return fail(CFD_ERR_STATE, "ELL operator not assembled");
+  int rc = mg_ensure(c, false);
+  if (rc) return rc;
+  PVecs<double> v{x, c.r0, c.r1, c.p1, c.s1, c.ap, c.as, c.ph64, c.sh64, b};
+  c.solves++;
+  return solve_pbicg<double>(c, op_args(c, CFD_OP_ELL), v, c.ks, tol, maxit, iters, relres);
+}
 
 int bicgstab_solve(Ctx &c, const double *b, double *x, double tol, int maxit, int *iters, double *relres) {
   if (c.prm.op == CFD_OP_ELL && !c.ell_ready) return fail(CFD_ERR_STATE, "ELL operator not assembled");
@@ -337,6 +459,13 @@ int bicgstab_solve(Ctx &c, const double *b, double *x, double tol, int maxit, in
   c.solves++;
   Vecs<double> v{x, c.r0, c.r1, c.p1, c.s1, c.ap, c.as, b, c.px1, c.px2};
   return solve_bicg<double>(c, op_args(c, c.prm.op), v, c.ks, c.solves, tol, maxit, iters, relres);
+}
+
+int bicgstab_solve_aux(Ctx &c, const double *b, double *x, double tol, int maxit, int timestep, int *iters,
+                       double *relres) {
+  if (!c.ell_ready) return fail(CFD_ERR_STATE, "ELL operator not assembled");
+  Vecs<double> v{x, c.r0, c.r1, c.p1, c.s1, c.ap, c.as, b, nullptr, nullptr};
+  return solve_bicg<double>(c, op_args(c, CFD_OP_ELL), v, c.ks, timestep, tol, maxit, iters, relres);
 }
 
 // FP64 defect correction around FP32 BiCGSTAB on the FP32 ELL copy:
@@ -368,7 +497,13 @@ int defect_correction(Ctx &c, const double *b, double *x, double tol, double tol
     int it = 0;
     double rin = 0.0;
     c.solves32++;
-    if (c.prm.solver == CFD_SOLVER_RBSOR) {
+    if (c.prm.precond == CFD_PRECOND_MG && c.prm.solver == CFD_SOLVER_BICGSTAB) {
+      int rc = mg_ensure(c, true);
+      if (rc) return rc;
+      PVecs<float> pv{d32, c.q32[0], c.q32[1], c.q32[2], c.q32[3], c.q32[4], c.q32[5], c.ph32, c.sh32, r32};
+      rc = solve_pbicg<float>(c, a32, pv, c.ks32, tol_inner, c.prm.maxit, &it, &rin);
+      if (rc != CFD_OK) return rc;
+    } else if (c.prm.solver == CFD_SOLVER_RBSOR) {
       int rc = rbsor_solve_f32(c, r32, d32, c.prm.sor_omega, tol_inner, c.prm.maxit, c.prm.sor_check, &it, &rin);
       if (rc != CFD_OK) return rc;
     } else {

@@ -1,0 +1,424 @@
+// k_mg.hip -- cell-centred geometric multigrid V-cycle used as a
+// preconditioner of the pressure solve (an addition: the reference solves
+// with Jacobi-scaled BiCGSTAB only).
+//
+// The Jacobi-scaled operator is A = J L with J = diag(jp) and L the
+// finite-volume Laplacian of createPressureMatrix (inc/matrix/
+// PressureMatrix.hpp:5-117): per axis, the coefficient to the -/+ neighbour
+// is 1/(D[i] Ds[i-1]) / 1/(D[i] Ds[i]), Neumann faces drop theirs, Dirichlet
+// faces keep theirs in the diagonal.  M^-1 r = V-cycle(L, r / jp) is a cheap
+// approximation of A^-1 r.
+//
+// Every level is matrix-free: the grids are tensor products, so a level is
+// its extents plus per-axis arrays (coefficients to the -/+ neighbour, their
+// diagonal share, cell widths); a smoothing sweep streams only x and b.
+// Coarse levels halve every axis that is even and at least 4 cells long;
+// their coefficients are rediscretised from the merged cell widths (a
+// Dirichlet face keeps its mirror-ghost coefficient).  Restriction is the
+// volume-weighted sum of the children's residuals, prolongation adds the
+// parent's correction.  Smoother: red-black Gauss-Seidel, nu sweeps before
+// and after.  Levels of at most MG_SMALL cells run inside one 1024-thread
+// workgroup (one launch for the whole bottom of the cycle); the coarsest is
+// smoothed MG_COARSE_SWEEPS times.
+//
+// Every kernel returns at once when the Krylov solver's `done` flag is set,
+// so preconditioned iterations can be launched in batches like the plain
+// ones.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dev_common.h"
+
+namespace cfd {
+
+constexpr int MG_MAX_LEVELS = 16;
+constexpr int64_t MG_SMALL = 4096;
+constexpr int MG_COARSE_SWEEPS = 48;
+
+// per-level device view: ax = cm[nx] cp[nx] dd[nx] w[nx] (and ay, az)
+struct MGView {
+  int nx, ny, nz, fx, fy, fz;
+  int64_t n;
+  const double *ax, *ay, *az;
+};
+
+struct MGLevel {
+  MGView v{};
+  double *coef = nullptr;               // ax | ay | az
+  double *x64 = nullptr, *b64 = nullptr;
+  float *x32 = nullptr, *b32 = nullptr;
+};
+
+struct MGState {
+  std::vector<MGLevel> lv;
+  int small_from = 0;   // first level handled by the one-workgroup kernel
+  int nu = 2;
+  std::string shape;
+};
+
+namespace {
+
+template <typename T>
+int mg_alloc(T **p, size_t n) {
+  *p = nullptr;
+  hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(CFD_ERR_NOMEM, std::string("multigrid: hipMalloc failed: ") + hipGetErrorString(e));
+  }
+  return CFD_OK;
+}
+
+template <typename T>
+void mg_free(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+struct Cell3 {
+  int i, j, k;
+};
+
+__device__ inline Cell3 dec3(const MGView &L, int64_t t) {
+  Cell3 c;
+  c.k = (int)(t % L.nz);
+  const int64_t r = t / L.nz;
+  c.j = (int)(r % L.ny);
+  c.i = (int)(r / L.ny);
+  return c;
+}
+
+__device__ inline int64_t idx3(const MGView &L, int i, int j, int k) { return ((int64_t)i * L.ny + j) * L.nz + k; }
+
+template <typename T>
+__device__ inline T mg_diag(const MGView &L, int i, int j, int k) {
+  return (T)(L.ax[2 * L.nx + i] + L.ay[2 * L.ny + j] + L.az[2 * L.nz + k]);
+}
+
+// sum over the neighbours of coefficient * x
+template <typename T>
+__device__ inline T mg_off(const MGView &L, const T *__restrict__ x, int i, int j, int k, int64_t c) {
+  const int64_t sx = (int64_t)L.ny * L.nz, sy = L.nz;
+  T s = (T)0;
+  if (i > 0) s += (T)L.ax[i] * x[c - sx];
+  if (i < L.nx - 1) s += (T)L.ax[L.nx + i] * x[c + sx];
+  if (j > 0) s += (T)L.ay[j] * x[c - sy];
+  if (j < L.ny - 1) s += (T)L.ay[L.ny + j] * x[c + sy];
+  if (k > 0) s += (T)L.az[k] * x[c - 1];
+  if (k < L.nz - 1) s += (T)L.az[L.nz + k] * x[c + 1];
+  return s;
+}
+
+template <typename T>
+__device__ inline double mg_vol(const MGView &L, int i, int j, int k) {
+  return L.ax[3 * L.nx + i] * L.ay[3 * L.ny + j] * L.az[3 * L.nz + k];
+}
+
+// one colour of a red-black Gauss-Seidel sweep; `first`: the sweep starts
+// from x = 0 (this colour gets b/diag, the other colour is cleared)
+template <typename T>
+__device__ inline void rb_cell(const MGView &L, int64_t t, int colour, bool first, const T *__restrict__ b,
+                               T *__restrict__ x) {
+  const int half = (L.nz + 1) / 2;
+  const int k2 = (int)(t % half);
+  const int64_t r = t / half;
+  const int j = (int)(r % L.ny), i = (int)(r / L.ny);
+  const int k = 2 * k2 + ((i + j + colour) & 1);
+  if (first) {
+    const int ko = 2 * k2 + ((i + j + colour + 1) & 1);
+    if (k < L.nz) {
+      const int64_t c = idx3(L, i, j, k);
+      x[c] = b[c] / mg_diag<T>(L, i, j, k);
+    }
+    if (ko < L.nz) x[idx3(L, i, j, ko)] = (T)0;
+  } else if (k < L.nz) {
+    const int64_t c = idx3(L, i, j, k);
+    x[c] = (b[c] + mg_off<T>(L, x, i, j, k, c)) / mg_diag<T>(L, i, j, k);
+  }
+}
+
+template <typename T>
+__device__ inline void restrict_cell(const MGView &F, const MGView &C, int64_t t, const T *__restrict__ bf,
+                                     const T *__restrict__ xf, T *__restrict__ bc) {
+  const Cell3 q = dec3(C, t);
+  double acc = 0.0;
+  for (int di = 0; di < F.fx; ++di)
+    for (int dj = 0; dj < F.fy; ++dj)
+      for (int dk = 0; dk < F.fz; ++dk) {
+        const int i = q.i * F.fx + di, j = q.j * F.fy + dj, k = q.k * F.fz + dk;
+        const int64_t c = idx3(F, i, j, k);
+        const T r = bf[c] - (mg_diag<T>(F, i, j, k) * xf[c] - mg_off<T>(F, xf, i, j, k, c));
+        acc += mg_vol<T>(F, i, j, k) * (double)r;
+      }
+  bc[t] = (T)(acc / mg_vol<T>(C, q.i, q.j, q.k));
+}
+
+template <typename T>
+__device__ inline void prolong_cell(const MGView &F, const MGView &C, int64_t t, const T *__restrict__ xc,
+                                    T *__restrict__ xf) {
+  const Cell3 q = dec3(F, t);
+  xf[t] += xc[idx3(C, q.i / F.fx, q.j / F.fy, q.k / F.fz)];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) mg_scale_kernel(int64_t n, const T *__restrict__ r, const double *__restrict__ jp,
+                                                       T *__restrict__ b, const KrylovScalars *ks) {
+  if (ks && ks->done) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = (T)((double)r[t] / jp[t]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) mg_rb_kernel(MGView L, int colour, int first, const T *__restrict__ b,
+                                                    T *__restrict__ x, const KrylovScalars *ks) {
+  if (ks && ks->done) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)L.nx * L.ny * ((L.nz + 1) / 2)) return;
+  rb_cell<T>(L, t, colour, first != 0, b, x);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) mg_restrict_kernel(MGView F, MGView C, const T *__restrict__ bf,
+                                                          const T *__restrict__ xf, T *__restrict__ bc,
+                                                          const KrylovScalars *ks) {
+  if (ks && ks->done) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < C.n) restrict_cell<T>(F, C, t, bf, xf, bc);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) mg_prolong_kernel(MGView F, MGView C, const T *__restrict__ xc,
+                                                         T *__restrict__ xf, const KrylovScalars *ks) {
+  if (ks && ks->done) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < F.n) prolong_cell<T>(F, C, t, xc, xf);
+}
+
+template <typename T>
+struct SmallArgs {
+  int nlev, nu, coarse_sweeps;
+  MGView v[MG_MAX_LEVELS];
+  T *x[MG_MAX_LEVELS];
+  T *b[MG_MAX_LEVELS];
+};
+
+// the bottom of the V-cycle (levels of at most MG_SMALL cells) in one
+// workgroup; __syncthreads separates the phases, whose data stays in this
+// workgroup (L2-resident)
+template <typename T>
+__global__ void __launch_bounds__(1024) mg_small_kernel(SmallArgs<T> S, const KrylovScalars *ks) {
+  if (ks && ks->done) return;
+  const int nt = blockDim.x;
+  auto sweep = [&](int l, int nsw, bool from_zero) {
+    const MGView &L = S.v[l];
+    const int64_t cnt = (int64_t)L.nx * L.ny * ((L.nz + 1) / 2);
+    for (int s = 0; s < nsw; ++s)
+      for (int colour = 0; colour < 2; ++colour) {
+        const bool first = from_zero && s == 0 && colour == 0;
+        for (int64_t t = threadIdx.x; t < cnt; t += nt) rb_cell<T>(L, t, colour, first, S.b[l], S.x[l]);
+        __syncthreads();
+      }
+  };
+  const int last = S.nlev - 1;
+  for (int l = 0; l < last; ++l) {
+    sweep(l, S.nu, true);
+    for (int64_t t = threadIdx.x; t < S.v[l + 1].n; t += nt)
+      restrict_cell<T>(S.v[l], S.v[l + 1], t, S.b[l], S.x[l], S.b[l + 1]);
+    __syncthreads();
+  }
+  sweep(last, S.coarse_sweeps, true);
+  for (int l = last - 1; l >= 0; --l) {
+    for (int64_t t = threadIdx.x; t < S.v[l].n; t += nt) prolong_cell<T>(S.v[l], S.v[l + 1], t, S.x[l + 1], S.x[l]);
+    __syncthreads();
+    sweep(l, S.nu, false);
+  }
+}
+
+int nblk(int64_t n) { return blocks_for(n, 256, 1 << 30); }
+
+template <typename T>
+T *lx(MGLevel &L);
+template <>
+double *lx<double>(MGLevel &L) { return L.x64; }
+template <>
+float *lx<float>(MGLevel &L) { return L.x32; }
+template <typename T>
+T *lb(MGLevel &L);
+template <>
+double *lb<double>(MGLevel &L) { return L.b64; }
+template <>
+float *lb<float>(MGLevel &L) { return L.b32; }
+
+// out = V-cycle(L, r / jp)
+template <typename T>
+int mg_apply(Ctx &c, const T *r, T *out, const KrylovScalars *ks) {
+  MGState &M = *c.mg;
+  const int nl = (int)M.lv.size();
+  T *b0 = lb<T>(M.lv[0]);
+  if (!b0) return fail(CFD_ERR_STATE, "multigrid buffers of this precision are not allocated");
+  mg_scale_kernel<T><<<nblk(c.g.N), 256, 0, c.stream>>>(c.g.N, r, c.f[CFD_JP], b0, ks);
+  auto X = [&](int l) { return l == 0 ? out : lx<T>(M.lv[l]); };
+  auto rb = [&](int l, bool from_zero) {
+    const MGView &v = M.lv[l].v;
+    const int64_t cnt = (int64_t)v.nx * v.ny * ((v.nz + 1) / 2);
+    for (int s = 0; s < M.nu; ++s)
+      for (int colour = 0; colour < 2; ++colour)
+        mg_rb_kernel<T><<<nblk(cnt), 256, 0, c.stream>>>(v, colour, from_zero && s == 0 && colour == 0 ? 1 : 0,
+                                                         lb<T>(M.lv[l]), X(l), ks);
+  };
+  const int ls = M.small_from;
+  for (int l = 0; l < ls; ++l) {
+    rb(l, true);
+    mg_restrict_kernel<T><<<nblk(M.lv[l + 1].v.n), 256, 0, c.stream>>>(M.lv[l].v, M.lv[l + 1].v, lb<T>(M.lv[l]),
+                                                                       X(l), lb<T>(M.lv[l + 1]), ks);
+  }
+  SmallArgs<T> S;
+  std::memset(&S, 0, sizeof(S));
+  S.nlev = nl - ls;
+  S.nu = M.nu;
+  S.coarse_sweeps = MG_COARSE_SWEEPS;
+  for (int l = ls; l < nl; ++l) {
+    S.v[l - ls] = M.lv[l].v;
+    S.x[l - ls] = X(l);
+    S.b[l - ls] = lb<T>(M.lv[l]);
+  }
+  mg_small_kernel<T><<<1, 1024, 0, c.stream>>>(S, ks);
+  for (int l = ls - 1; l >= 0; --l) {
+    mg_prolong_kernel<T><<<nblk(M.lv[l].v.n), 256, 0, c.stream>>>(M.lv[l].v, M.lv[l + 1].v, X(l + 1), X(l), ks);
+    rb(l, false);
+  }
+  CFD_HIP(hipGetLastError());
+  return CFD_OK;
+}
+
+// per-axis level arrays from widths w (n) and the centre distances of the
+// finest level (ds_lo: centre of cell i-1 to i, for i = 0 the ghost
+// distance; ds_hi: centre of i to i+1, for i = n-1 the ghost distance)
+void axis_arrays(const std::vector<double> &w, const std::vector<double> &dlo, const std::vector<double> &dhi,
+                 bool dir_lo, bool dir_hi, std::vector<double> &out) {
+  const size_t n = w.size();
+  out.assign(4 * n, 0.0);
+  for (size_t i = 0; i < n; ++i) {
+    const double xm = 1.0 / w[i] / dlo[i], xp = 1.0 / w[i] / dhi[i];
+    out[i] = i > 0 ? xm : 0.0;
+    out[n + i] = i + 1 < n ? xp : 0.0;
+    out[2 * n + i] = ((i > 0 || dir_lo) ? xm : 0.0) + ((i + 1 < n || dir_hi) ? xp : 0.0);
+    out[3 * n + i] = w[i];
+  }
+}
+
+}  // namespace
+
+void mg_release(Ctx &c) {
+  if (!c.mg) return;
+  for (auto &L : c.mg->lv) {
+    mg_free(L.coef);
+    mg_free(L.x64);
+    mg_free(L.b64);
+    mg_free(L.x32);
+    mg_free(L.b32);
+  }
+  delete c.mg;
+  c.mg = nullptr;
+  mg_free(c.ph64);
+  mg_free(c.sh64);
+  mg_free(c.ph32);
+  mg_free(c.sh32);
+}
+
+// builds the hierarchy (once) and the buffers of the requested precision
+int mg_ensure(Ctx &c, bool fp32) {
+  if (!c.mg) {
+    MGState *M = new MGState();
+    c.mg = M;
+    const int nc[3] = {c.g.nxc, c.g.nyc, c.g.nzc};
+    const std::vector<double> *Dh[3] = {&c.hDx, &c.hDy, &c.hDz}, *Dsh[3] = {&c.hDxs, &c.hDys, &c.hDzs};
+    std::vector<double> w[3], dlo[3], dhi[3];
+    for (int a = 0; a < 3; ++a) {
+      for (int q = 0; q < nc[a]; ++q) {
+        const int I = q + GC;
+        w[a].push_back((*Dh[a])[I]);
+        dlo[a].push_back((*Dsh[a])[I - 1]);
+        dhi[a].push_back((*Dsh[a])[I]);
+      }
+    }
+    const bool dl[3] = {c.prm.p_dirichlet[0] != 0, c.prm.p_dirichlet[2] != 0, c.prm.p_dirichlet[4] != 0};
+    const bool dh[3] = {c.prm.p_dirichlet[1] != 0, c.prm.p_dirichlet[3] != 0, c.prm.p_dirichlet[5] != 0};
+    for (;;) {
+      MGLevel L;
+      L.v.nx = (int)w[0].size();
+      L.v.ny = (int)w[1].size();
+      L.v.nz = (int)w[2].size();
+      L.v.n = (int64_t)L.v.nx * L.v.ny * L.v.nz;
+      std::vector<double> arr[3], all;
+      for (int a = 0; a < 3; ++a) {
+        axis_arrays(w[a], dlo[a], dhi[a], dl[a], dh[a], arr[a]);
+        all.insert(all.end(), arr[a].begin(), arr[a].end());
+      }
+      int rc = mg_alloc(&L.coef, all.size());
+      if (rc) return rc;
+      CFD_HIP(hipMemcpy(L.coef, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+      L.v.ax = L.coef;
+      L.v.ay = L.coef + 4 * L.v.nx;
+      L.v.az = L.coef + 4 * L.v.nx + 4 * L.v.ny;
+      int f[3];
+      for (int a = 0; a < 3; ++a) f[a] = (w[a].size() % 2 == 0 && w[a].size() >= 4) ? 2 : 1;
+      const bool more = (f[0] > 1 || f[1] > 1 || f[2] > 1) && L.v.n > 64 && (int)M->lv.size() < MG_MAX_LEVELS - 1;
+      L.v.fx = more ? f[0] : 1;
+      L.v.fy = more ? f[1] : 1;
+      L.v.fz = more ? f[2] : 1;
+      M->lv.push_back(L);
+      if (!more) break;
+      // merged widths; centre distances from the widths, mirror ghosts at Dirichlet faces
+      for (int a = 0; a < 3; ++a) {
+        if (f[a] == 1) continue;
+        std::vector<double> wc;
+        for (size_t q = 0; q < w[a].size(); q += 2) wc.push_back(w[a][q] + w[a][q + 1]);
+        const size_t n = wc.size();
+        std::vector<double> lo(n), hi(n);
+        for (size_t q = 0; q < n; ++q) {
+          lo[q] = q > 0 ? 0.5 * (wc[q - 1] + wc[q]) : wc[q];
+          hi[q] = q + 1 < n ? 0.5 * (wc[q] + wc[q + 1]) : wc[q];
+        }
+        w[a] = wc;
+        dlo[a] = lo;
+        dhi[a] = hi;
+      }
+    }
+    M->small_from = (int)M->lv.size() - 1;
+    while (M->small_from > 0 && M->lv[M->small_from - 1].v.n <= MG_SMALL) M->small_from--;
+    M->nu = c.prm.mg_nu > 0 ? c.prm.mg_nu : 2;
+    for (auto &L : M->lv) M->shape += std::to_string(L.v.nx) + "x" + std::to_string(L.v.ny) + "x" +
+                                      std::to_string(L.v.nz) + " ";
+  }
+  MGState &M = *c.mg;
+  M.nu = c.prm.mg_nu > 0 ? c.prm.mg_nu : 2;
+  for (size_t l = 0; l < M.lv.size(); ++l) {
+    MGLevel &L = M.lv[l];
+    int rc = CFD_OK;
+    if (fp32) {
+      if (!L.b32 && (rc = mg_alloc(&L.b32, L.v.n))) return rc;
+      if (l > 0 && !L.x32 && (rc = mg_alloc(&L.x32, L.v.n))) return rc;
+    } else {
+      if (!L.b64 && (rc = mg_alloc(&L.b64, L.v.n))) return rc;
+      if (l > 0 && !L.x64 && (rc = mg_alloc(&L.x64, L.v.n))) return rc;
+    }
+  }
+  const size_t N = (size_t)c.g.N;
+  int rc = CFD_OK;
+  if (fp32) {
+    if ((!c.ph32 && (rc = mg_alloc(&c.ph32, N))) || (!c.sh32 && (rc = mg_alloc(&c.sh32, N)))) return rc;
+  } else {
+    if ((!c.ph64 && (rc = mg_alloc(&c.ph64, N))) || (!c.sh64 && (rc = mg_alloc(&c.sh64, N)))) return rc;
+  }
+  return CFD_OK;
+}
+
+int mg_apply64(Ctx &c, const double *r, double *out, const KrylovScalars *ks) { return mg_apply<double>(c, r, out, ks); }
+int mg_apply32(Ctx &c, const float *r, float *out, const KrylovScalars *ks) { return mg_apply<float>(c, r, out, ks); }
+
+const char *mg_shape(const Ctx &c) { return c.mg ? c.mg->shape.c_str() : ""; }
+
+}  // namespace cfd

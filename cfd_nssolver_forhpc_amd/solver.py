@@ -257,6 +257,33 @@ class Simulation:
     def set_loop(self, loop: int, time: float = 0.0):
         check(self.lib.cfd_set_loop(self.h, loop, time), "cfd_set_loop")
 
+    # ---- discrete adjoint -------------------------------------------------------------
+    def set_inflow_profile(self, u_in):
+        """Face-0 u Dirichlet value per interior row (ny_cal values); None: the table value."""
+        if u_in is None:
+            check(self.lib.cfd_set_inflow_profile(self.h, None, 0), "cfd_set_inflow_profile")
+            return
+        a = np.ascontiguousarray(u_in, dtype=np.float64)
+        check(self.lib.cfd_set_inflow_profile(self.h, dptr(a), a.size), "cfd_set_inflow_profile")
+
+    def set_forcing_gain(self, gain: float):
+        check(self.lib.cfd_set_forcing_gain(self.h, float(gain)), "cfd_set_forcing_gain")
+
+    def adjoint_forward(self, nsteps: int, avg_from: int, ckpt_every: int) -> float:
+        """Run nsteps timesteps with checkpoints; the mean corrector cD over [avg_from, nsteps)."""
+        J = C.c_double()
+        check(self.lib.cfd_adjoint_forward(self.h, nsteps, avg_from, ckpt_every, C.byref(J)), "cfd_adjoint_forward")
+        return J.value
+
+    def adjoint_backward(self):
+        """(dJ/du_in, dJ/dgain, stats) of the last adjoint_forward run."""
+        g = np.zeros(self.grid.ncal[1])
+        gg = C.c_double()
+        st = capi.CfdAdjointStats()
+        check(self.lib.cfd_adjoint_backward(self.h, dptr(g), g.size, C.byref(gg), C.byref(st)),
+              "cfd_adjoint_backward")
+        return g, gg.value, st.as_dict()
+
     # ---- measurement ------------------------------------------------------------------
     def time_kernel(self, what: int, nrep: int) -> float:
         ms = C.c_double()

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define CFD_ABI_VERSION 1
+#define CFD_ABI_VERSION 2
 
 /* status codes */
 #define CFD_OK 0
@@ -67,6 +67,12 @@ extern "C" {
 #define CFD_MIXED_RELIABLE 0  /* FP32 BiCGSTAB with FP64 residual replacement (reliable updates) */
 #define CFD_MIXED_RESTARTED 1 /* FP64 defect correction around restarted FP32 inner solves */
 #define CFD_MIXED_VALUES32 2  /* FP64 BiCGSTAB on the FP32-valued ELL operator, FP64 residual replacement */
+
+/* preconditioning of the BiCGSTAB pressure solve */
+#define CFD_PRECOND_JACOBI 0 /* the reference: Jacobi scaling only (bicgstabRe2 on the jp-scaled operator) */
+#define CFD_PRECOND_MG 1     /* right preconditioning by a geometric-multigrid V-cycle (no reference
+                                counterpart); with CFD_MIXED the V-cycle and the inner Krylov solve run in
+                                FP32 inside the FP64 defect correction */
 
 /* device arrays addressable through upload / download */
 #define CFD_T0_U 0 /* old velocity (ghosted) */
@@ -121,6 +127,9 @@ typedef struct cfd_params {
   double ib_cx, ib_cy, ib_r; /* z-cylinder centre and radius (setEta.hpp:145-147) */
   int ib_nsub;           /* sub-cells per cut-cell edge (setEta.hpp:152) */
   double u_init[3];      /* initial velocity (RUN_CPU_PC_METHOD.hpp:77) */
+  int precond;           /* CFD_PRECOND_JACOBI (default) or CFD_PRECOND_MG */
+  int mg_nu;             /* CFD_PRECOND_MG: red-black Gauss-Seidel sweeps before and after each coarse
+                            correction (default 2) */
 } cfd_params;
 
 /* per-timestep report; pass index 0..inner_loops-1 are predictor passes, the last is the corrector */
@@ -237,6 +246,33 @@ int cfd_copy_t3_to_t0(cfd_ctx *ctx);
 int cfd_step(cfd_ctx *ctx, cfd_step_stats *stats);
 int cfd_loop(const cfd_ctx *ctx, int *loop, double *time);
 int cfd_set_loop(cfd_ctx *ctx, int loop, double time);
+
+/* ---- discrete adjoint (no reference counterpart) ---------------------- */
+/* Objective J = mean of the corrector pass's cD (virtualF_Int,
+   virtualForceIntergrator.hpp:5-37) over steps [avg_from, nsteps) of a run
+   of cfd_step from the current state.  Parameters: the inflow profile u_in(y)
+   -- the face-0 u Dirichlet value of BC_staggered_main per interior row,
+   replacing bc_dir[0][0] -- and the DFIB forcing gain, u_new = (1 - gain *
+   eta_f) T2 in update_UandF (gain 1 reproduces the reference bit for bit).
+   Needs the ELL operator, BiCGSTAB and source_index_quirk = 0. */
+typedef struct cfd_adjoint_stats {
+  int steps;             /* forward steps of the objective run */
+  int checkpoints;       /* states kept by the forward run */
+  int recomputed_steps;  /* forward steps re-run by the reverse sweep */
+  int adjoint_solves;    /* transposed pressure solves */
+  int adjoint_iters;     /* their BiCGSTAB iterations */
+  double forward_ms;     /* device time of the objective run */
+  double backward_ms;    /* device time of the reverse sweep (recomputation included) */
+} cfd_adjoint_stats;
+/* u_in has ny_cal values; NULL restores the table value bc_dir[0][0] */
+int cfd_set_inflow_profile(cfd_ctx *ctx, const double *u_in, size_t n);
+int cfd_set_forcing_gain(cfd_ctx *ctx, double gain);
+/* run nsteps timesteps keeping a checkpoint every ckpt_every steps; *J receives the objective */
+int cfd_adjoint_forward(cfd_ctx *ctx, int nsteps, int avg_from, int ckpt_every, double *J);
+/* dJ/du_in (n = ny_cal values) and dJ/dgain of the last cfd_adjoint_forward
+   run, by the transposed timestep with recomputation from the checkpoints;
+   leaves the state where the forward run ended */
+int cfd_adjoint_backward(cfd_ctx *ctx, double *grad_uin, size_t n, double *grad_gain, cfd_adjoint_stats *stats);
 
 /* ---- measurement ------------------------------------------------------ */
 /* average device time (ms) of one launch of the kernel named by `what`, over

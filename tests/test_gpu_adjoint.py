@@ -1,0 +1,159 @@
+"""The GPU discrete adjoint (an addition with no reference counterpart).
+
+Pins: the objective and the gradient against the oracle's adjoint (which
+solves the transposed pressure system on the explicit transpose of the
+assembled ELL operator and records every step instead of recomputing from
+checkpoints), the gradient against central finite differences of the GPU
+forward run, independence from the checkpoint interval, the state left
+after the sweep, the torch.autograd.Function, and bit-identity of the
+forward step at the parameters' reference values.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as O
+from cfd_nssolver_forhpc_amd import capi, configs
+from cfd_nssolver_forhpc_amd.adjoint import DragObjective, DragProblem, smoothed_inflow
+from cfd_nssolver_forhpc_amd.solver import Simulation
+
+pytestmark = pytest.mark.gpu
+
+STEPS, AVG_FROM = 10, 5
+
+
+def case(tol=1e-13):
+    w = configs.small_cylinder(ncal=(24, 12, 4), lengths=(6.0, 3.0, 1.0), tol=tol)
+    w.params.maxit = 20000
+    return w, smoothed_inflow(12)
+
+
+def gpu_problem(w, every):
+    sim = Simulation(w.grid, w.params)
+    sim.init_state(1e-3, 1)
+    sim.DFIB_CylinderZ()
+    sim.createPressureMatrix()
+    return DragProblem(sim, STEPS, AVG_FROM, every, noise=1e-3, seed=1)
+
+
+def oracle_gradient(w, uin, gain):
+    g = O.Grid(w.grid)
+    s = O.Sim(g, w.params)
+    s.set_inflow(uin)
+    s.set_gain(gain)
+    s.init_state(1e-3, 1)
+    s.setup()
+    J = s.objective(STEPS, AVG_FROM)
+    g_uin, g_gain, _ = s.gradient()
+    return J, g_uin, g_gain
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(a), abs(b), 1e-300)
+
+
+def test_objective_and_gradient_match_the_oracle(require_gpu):
+    w, uin = case()
+    pr = gpu_problem(w, every=3)
+    J = pr.run(uin, 0.9)
+    g_uin, g_gain, st = pr.sim.adjoint_backward()
+    Jo, go_uin, go_gain = oracle_gradient(w, uin, 0.9)
+    print("J", J, Jo, "gain grad", g_gain, go_gain, "stats", st)
+    assert rel(J, Jo) < 1e-10
+    assert np.linalg.norm(g_uin - go_uin) <= 1e-8 * np.linalg.norm(go_uin)
+    assert rel(g_gain, go_gain) < 1e-8
+    assert st["checkpoints"] == 4 and st["recomputed_steps"] == STEPS and st["adjoint_solves"] > 0
+
+
+def test_gradient_matches_central_differences(require_gpu):
+    w, uin = case()
+    pr = gpu_problem(w, every=4)
+    pr.run(uin, 1.0)
+    g_uin, g_gain, _ = pr.sim.adjoint_backward()
+    rng = np.random.default_rng(21)
+    h = 1e-6
+    for _ in range(3):
+        d = rng.standard_normal(uin.size + 1)
+        d /= np.linalg.norm(d)
+        Jp = pr.run(uin + h * d[:-1], 1.0 + h * d[-1])
+        Jm = pr.run(uin - h * d[:-1], 1.0 - h * d[-1])
+        fd = (Jp - Jm) / (2 * h)
+        ad = float(np.dot(g_uin, d[:-1]) + g_gain * d[-1])
+        print("fd", fd, "adjoint", ad)
+        assert rel(fd, ad) < 1e-5
+
+
+def test_checkpoint_interval_does_not_change_the_gradient(require_gpu):
+    w, uin = case(tol=1e-10)
+    res = []
+    for every in (1, 4, STEPS):
+        pr = gpu_problem(w, every)
+        J = pr.run(uin, 1.0)
+        g_uin, g_gain, st = pr.sim.adjoint_backward()
+        res.append((J, g_uin, g_gain))
+        assert st["checkpoints"] == -(-STEPS // every)
+    for J, g_uin, g_gain in res[1:]:
+        assert J == res[0][0]
+        assert np.array_equal(g_uin, res[0][1]) and g_gain == res[0][2]
+
+
+def test_sweep_leaves_the_final_state(require_gpu):
+    w, uin = case(tol=1e-10)
+    pr = gpu_problem(w, every=3)
+    pr.run(uin, 1.0)
+    before = [pr.sim.download(f) for f in (capi.T0_U, capi.T0_V, capi.T0_W, capi.X)]
+    loop = pr.sim.loop
+    pr.sim.adjoint_backward()
+    after = [pr.sim.download(f) for f in (capi.T0_U, capi.T0_V, capi.T0_W, capi.X)]
+    assert pr.sim.loop == loop
+    assert all(np.array_equal(a, b) for a, b in zip(before, after))
+    # and the run continues as if the sweep had not happened
+    st = pr.sim.step()
+    assert st.loop == loop
+
+
+def test_autograd_function(require_gpu):
+    w, uin = case(tol=1e-10)
+    pr = gpu_problem(w, every=5)
+    u = torch.tensor(uin, dtype=torch.float64, requires_grad=True)
+    gn = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    J = DragObjective.apply(u, gn, pr)
+    (2.0 * J).backward()
+    pr2 = gpu_problem(w, every=5)
+    J2 = pr2.run(uin, 1.0)
+    g_uin, g_gain, _ = pr2.sim.adjoint_backward()
+    assert float(J) == J2
+    assert np.array_equal(u.grad.numpy(), 2.0 * g_uin) and float(gn.grad) == 2.0 * g_gain
+    assert pr.last_stats["steps"] == STEPS
+
+
+def test_reference_parameter_values_keep_the_step_bit_identical(require_gpu):
+    w, _ = case(tol=1e-10)
+    runs = []
+    for set_params in (False, True):
+        sim = Simulation(w.grid, w.params)
+        if set_params:
+            sim.set_inflow_profile(np.full(w.grid.ncal[1], w.params.bc_dir[0][0]))
+            sim.set_forcing_gain(1.0)
+        sim.init_state(1e-3, 1)
+        sim.DFIB_CylinderZ()
+        sim.createPressureMatrix()
+        for _ in range(3):
+            sim.step()
+        runs.append([sim.download(f) for f in (capi.T0_U, capi.T0_V, capi.T0_W, capi.F, capi.X)])
+    assert all(np.array_equal(a, b) for a, b in zip(*runs))
+
+
+def test_bad_adjoint_arguments_are_errors(require_gpu):
+    w, uin = case(tol=1e-10)
+    sim = Simulation(w.grid, w.params)
+    with pytest.raises(capi.CfdInvalidArgument):
+        sim.set_inflow_profile(np.ones(5))
+    with pytest.raises(capi.CfdError):
+        sim.adjoint_backward()  # no objective run yet
+    sim.init_state(1e-3, 1)
+    sim.createPressureMatrix()
+    with pytest.raises(capi.CfdInvalidArgument):
+        sim.adjoint_forward(5, 5, 1)

@@ -100,3 +100,82 @@ def test_mixed_step_close_to_fp64(require_gpu):
         assert max(sb.relres[:sb.n_passes]) < 1e-10
     for fid in (capi.T0_U, capi.T0_V, capi.T0_W):
         assert rel_l2(b.download(fid), a.download(fid)) < 1e-7
+
+
+# ---- multigrid-preconditioned solves (an addition: the reference is Jacobi-scaled only)
+def mg_sim(ncal=(64, 32, 16), precision=capi.CFD_FP64, nu=2):
+    sim, b = poisson_sim(ncal=ncal, precision=precision, mode=capi.CFD_MIXED_RESTARTED)
+    p = sim.params.copy()
+    p.precond = capi.CFD_PRECOND_MG
+    p.mg_nu = nu
+    p.tol_inner = 1e-5
+    sim.set_params(p)
+    return sim, b
+
+
+@pytest.mark.parametrize("nu", [1, 2])
+def test_mg_bicgstab_reaches_the_jacobi_solution(require_gpu, nu):
+    ref, b = poisson_sim(ncal=(64, 32, 16))
+    it_j, _ = ref.bicgstab(tol=1e-13, maxit=50000)
+    x64 = ref.download(capi.X)
+    sim, _ = mg_sim(nu=nu)
+    it, rel = sim.solve()
+    tr, x = true_relres(sim, b)
+    print("MG iterations", it, "Jacobi iterations", it_j, "true relres", tr)
+    assert rel < 1e-10 and tr < 2e-10
+    assert it <= 40 and it * 10 < it_j
+    assert rel_l2(x, x64) < 1e-7
+
+
+def test_mixed_mg_defect_correction(require_gpu):
+    ref, b = poisson_sim(ncal=(64, 32, 16))
+    ref.bicgstab(tol=1e-13, maxit=50000)
+    x64 = ref.download(capi.X)
+    sim, _ = mg_sim(precision=capi.CFD_MIXED)
+    outer, rel = sim.solve()
+    tr, x = true_relres(sim, b)
+    print("outer", outer, "true relres", tr)
+    assert rel < 1e-10 and tr < 1e-10 and outer <= 6
+    assert rel_l2(x, x64) < 1e-7
+
+
+def test_mg_on_odd_and_stretched_grids(require_gpu):
+    """Axes that cannot be halved stop coarsening there; a non-uniform grid
+    still converges to the FP64 residual."""
+    from cfd_nssolver_forhpc_amd.solver import axis_sml, axis_tanh
+    from cfd_nssolver_forhpc_amd.capi import GridArrays
+    ax = axis_tanh(48, 12.0, 4.0, 0.15, 2.0, 1.1)
+    ay = axis_sml(30, 6.0, 1.8, 10, 0)
+    az = axis_sml(10, 2.0, 1.0, 0, 0)
+    grid = GridArrays((48, 30, 10), ax[0], ay[0], az[0], ax[1], ay[1], az[1], ax[2], ay[2], az[2], 12.0, 6.0, 2.0)
+    p = configs.cylinder_params(100.0, 0.002, 4.0, 3.0, 1.0, 1e-10)
+    p.maxit = 50000
+    for prec in (capi.CFD_FP64, capi.CFD_MIXED):
+        q = p.copy()
+        q.precond, q.precision, q.mixed_mode, q.tol_inner = capi.CFD_PRECOND_MG, prec, capi.CFD_MIXED_RESTARTED, 1e-5
+        sim = Simulation(grid, q)
+        sim.init_state()
+        sim.createPressureMatrix()
+        b = np.random.default_rng(4).standard_normal(sim.N)
+        sim.upload(capi.RHS, b)
+        sim.upload(capi.X, np.zeros(sim.N))
+        it, rel = sim.solve()
+        tr, _ = true_relres(sim, b)
+        assert rel < 1e-10 and tr < 2e-10, (prec, it, rel, tr)
+
+
+def test_mg_step_matches_jacobi_step(require_gpu):
+    w = configs.small_cylinder(ncal=(32, 16, 8), lengths=(8.0, 4.0, 2.0), quirk=0, tol=1e-12)
+    w.params.maxit = 50000
+    a = Simulation(w.grid, w.params)
+    a.setup(noise=w.noise, seed=w.seed)
+    pm = w.params.copy()
+    pm.precond = capi.CFD_PRECOND_MG
+    b = Simulation(w.grid, pm)
+    b.setup(noise=w.noise, seed=w.seed)
+    for _ in range(3):
+        a.step()
+        sb = b.step()
+        assert max(sb.relres[:sb.n_passes]) < 1e-12
+    for fid in (capi.T0_U, capi.T0_V, capi.T0_W, capi.X):
+        assert rel_l2(b.download(fid), a.download(fid)) < 1e-9
