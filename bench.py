@@ -182,7 +182,7 @@ def kernel_table(sim, reps):
     return rows
 
 
-def adjoint_benchmark(steps, avg_from, every, fd_dirs):
+def adjoint_benchmark(steps, avg_from, every, fd_dirs, precond):
     """Discrete adjoint on the 256x128x32 Re-100 cylinder (dt 1e-3): J = mean
     corrector cD over steps [avg_from, steps), gradient w.r.t. the seeded
     smoothed inflow profile (128 values) and the DFIB forcing gain; timed
@@ -190,6 +190,7 @@ def adjoint_benchmark(steps, avg_from, every, fd_dirs):
     optionally central differences (h = 1e-6) in fd_dirs random directions."""
     from cfd_nssolver_forhpc_amd.adjoint import DragProblem, smoothed_inflow
     w = configs.config_d()
+    w.params.precond = precond
     sim = Simulation(w.grid, w.params)
     sim.init_state(w.noise, w.seed)
     sim.DFIB_CylinderZ()
@@ -219,6 +220,7 @@ def adjoint_benchmark(steps, avg_from, every, fd_dirs):
     sim.close()
     return {
         "workload": w.name, "steps": steps, "avg_from": avg_from, "checkpoint_every": every,
+        "pressure": "BiCGSTAB to 1e-10, " + ("multigrid-preconditioned" if precond else "Jacobi-scaled") + ", FP64",
         "J_mean_cd": J, "grad_uin_norm": float(np.linalg.norm(g_uin)), "grad_gain": g_gain,
         "forward_s": t_fwd, "reverse_sweep_s": t_bwd, "reverse_over_forward": t_bwd / t_fwd,
         "forward_ms_per_step": 1e3 * t_fwd / steps, "reverse_ms_per_step": 1e3 * t_bwd / steps,
@@ -237,12 +239,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mg", action="store_true")
     ap.add_argument("--mg-steps", type=int, default=20, help="timed steps of the multigrid-preconditioned runs")
+    ap.add_argument("--mg-only", action="store_true", help="run only the multigrid legs (FP64 reference speed from "
+                    "--ref-tps) and print their JSON")
+    ap.add_argument("--ref-tps", type=float, default=float("nan"),
+                    help="with --mg-only: the Jacobi FP64 timesteps/s to compare against")
     ap.add_argument("--reps", type=int, default=200)
     ap.add_argument("--adjoint-steps", type=int, default=40,
                     help="steps of the 256x128x32 adjoint run (0: skip); the drag is averaged over the second half")
     ap.add_argument("--adjoint-every", type=int, default=20, help="checkpoint interval of the adjoint run")
     ap.add_argument("--adjoint-fd", type=int, default=0, help="central-difference directions to check the gradient")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg and print its JSON")
+    ap.add_argument("--adjoint-jacobi", action="store_true",
+                    help="solve the adjoint run's pressure systems with the reference's Jacobi scaling (default: MG)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time the operator kernels on the 512x256x64 grid and exit")
     args = ap.parse_args()
@@ -254,7 +262,8 @@ def main():
 
     if args.adjoint_only:
         steps = max(args.adjoint_steps, 2)
-        print(json.dumps({"adjoint": adjoint_benchmark(steps, steps // 2, args.adjoint_every, args.adjoint_fd)}),
+        pc = capi.CFD_PRECOND_JACOBI if args.adjoint_jacobi else capi.CFD_PRECOND_MG
+        print(json.dumps({"adjoint": adjoint_benchmark(steps, steps // 2, args.adjoint_every, args.adjoint_fd, pc)}),
               flush=True)
         return
 
@@ -270,18 +279,20 @@ def main():
         sim.upload(capi.RHS, rng.standard_normal(sim.N))
         print(json.dumps({"kernels": kernel_table(sim, args.reps)}), flush=True)
         return
-    log(f"[bench] {w64.name}: warmup {warmup}, timed {args.steps}")
-    sim64, t64, st64 = run_steps(w64, warmup, args.steps, "fp64")
-    tps64 = args.steps / t64
-    iters64 = per_step(st64, "iters")
-
-    # kernel roofline table; the dominant kernel is the ELL SpMV (FP64)
-    sim64.setupPressure()
-    kt = kernel_table(sim64, args.reps)
-    spmv = kt["spmv_fp64"]
+    if args.mg_only:
+        sim64, tps64, iters64, kt, spmv = None, args.ref_tps, float("nan"), {}, {}
+    else:
+        log(f"[bench] {w64.name}: warmup {warmup}, timed {args.steps}")
+        sim64, t64, st64 = run_steps(w64, warmup, args.steps, "fp64")
+        tps64 = args.steps / t64
+        iters64 = per_step(st64, "iters")
+        # kernel roofline table; the dominant kernel is the ELL SpMV (FP64)
+        sim64.setupPressure()
+        kt = kernel_table(sim64, args.reps)
+        spmv = kt["spmv_fp64"]
 
     mixed = None
-    if not args.no_mixed:
+    if not args.no_mixed and not args.mg_only:
         wm = configs.config_b(capi.CFD_MIXED)
         simm, tm, stm = run_steps(wm, warmup, args.steps, "mixed")
         err = {}
@@ -302,33 +313,42 @@ def main():
     mg = None
     if not args.no_mg:
         mg = {}
-        for tag, prec in (("fp64_mg", capi.CFD_FP64), ("mixed_mg", capi.CFD_MIXED)):
+        sims = {}
+        variants = (("fp64_mg", capi.CFD_FP64, capi.CFD_MIXED_RESTARTED),
+                    ("mixed_mg", capi.CFD_MIXED, capi.CFD_MIXED_PC32),
+                    ("mixed_mg_dc", capi.CFD_MIXED, capi.CFD_MIXED_RESTARTED))
+        for tag, prec, mode in variants:
             wg = configs.config_b(prec)
             wg.params.precond = capi.CFD_PRECOND_MG
-            wg.params.mixed_mode = capi.CFD_MIXED_RESTARTED
+            wg.params.mixed_mode = mode
             wg.params.tol_inner = 1e-5
             wg.name += "-mg"
             simg, tg, stg = run_steps(wg, warmup, args.mg_steps, tag)
-            rec = {"timesteps_per_s": args.mg_steps / tg, "speedup_vs_fp64": (args.mg_steps / tg) / tps64,
-                   "solver_iters_per_step": per_step(stg, "iters"),
-                   "inner_iters_per_step": per_step(stg, "inner_iters"),
-                   "max_final_relres": max(max(s["relres"]) for s in stg)}
-            mg[tag] = rec
-            mg[tag + "_sim"] = simg
-        # the mixed run against the FP64 run with the same preconditioner, after the same steps
-        err = {}
-        a, b_ = mg.pop("mixed_mg_sim"), mg.pop("fp64_mg_sim")
-        for name, fid in (("u", capi.T0_U), ("v", capi.T0_V), ("w", capi.T0_W), ("p", capi.X)):
-            x, y = a.download(fid), b_.download(fid)
-            err[name] = float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
-        mg["mixed_mg"]["rel_l2_vs_fp64_mg_after_timed_steps"] = err
-        mg["mixed_mg"]["speedup_vs_fp64_mg"] = mg["mixed_mg"]["timesteps_per_s"] / mg["fp64_mg"]["timesteps_per_s"]
+            mg[tag] = {"timesteps_per_s": args.mg_steps / tg, "speedup_vs_fp64": (args.mg_steps / tg) / tps64,
+                       "solver_iters_per_step": per_step(stg, "iters"),
+                       "inner_iters_per_step": per_step(stg, "inner_iters"),
+                       "max_final_relres": max(max(s["relres"]) for s in stg)}
+            sims[tag] = simg
+        # mixed runs against the FP64 run with the same preconditioner, after the same steps
+        ref_mg = sims["fp64_mg"]
+        for tag in ("mixed_mg", "mixed_mg_dc"):
+            err = {}
+            for name, fid in (("u", capi.T0_U), ("v", capi.T0_V), ("w", capi.T0_W), ("p", capi.X)):
+                x, y = sims[tag].download(fid), ref_mg.download(fid)
+                err[name] = float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+            mg[tag]["rel_l2_vs_fp64_mg_after_timed_steps"] = err
+            mg[tag]["speedup_vs_fp64_mg"] = mg[tag]["timesteps_per_s"] / mg["fp64_mg"]["timesteps_per_s"]
+        for simg in sims.values():
+            simg.close()
         mg["note"] = ("BiCGSTAB right-preconditioned by a cell-centred multigrid V-cycle (red-black Gauss-Seidel 2+2, "
-                      "matrix-free levels); fp64_mg: FP64 throughout; mixed_mg: FP64 defect correction to 1e-10 "
-                      "around FP32 preconditioned BiCGSTAB (FP32 ELL values and V-cycle) to 1e-5; speedup_vs_fp64 "
-                      "is against the reference's Jacobi-scaled FP64 solve (the bench value)")
-        a.close()
-        b_.close()
+                      "matrix-free levels), every solve to ||b - Ax||/||b|| < 1e-10 in FP64. fp64_mg: FP64 throughout; "
+                      "mixed_mg: FP64 BiCGSTAB with the V-cycle in FP32; mixed_mg_dc: FP64 defect correction around "
+                      "FP32 preconditioned BiCGSTAB (FP32 ELL values) to 1e-5. speedup_vs_fp64 is against the "
+                      "reference's Jacobi-scaled FP64 solve (the bench value)")
+
+    if args.mg_only:
+        print(json.dumps({"multigrid": mg}), flush=True)
+        return
 
     spe = None
     if not args.no_spe:
@@ -343,7 +363,8 @@ def main():
 
     adjoint = None
     if args.adjoint_steps > 1:
-        adjoint = adjoint_benchmark(args.adjoint_steps, args.adjoint_steps // 2, args.adjoint_every, args.adjoint_fd)
+        pc = capi.CFD_PRECOND_JACOBI if args.adjoint_jacobi else capi.CFD_PRECOND_MG
+        adjoint = adjoint_benchmark(args.adjoint_steps, args.adjoint_steps // 2, args.adjoint_every, args.adjoint_fd, pc)
 
     cpu = None
     if not args.no_cpu:

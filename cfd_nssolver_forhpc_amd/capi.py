@@ -33,6 +33,7 @@ CFD_MIXED = 1
 CFD_MIXED_RELIABLE = 0
 CFD_MIXED_RESTARTED = 1
 CFD_MIXED_VALUES32 = 2
+CFD_MIXED_PC32 = 3
 CFD_PRECOND_JACOBI = 0
 CFD_PRECOND_MG = 1
 

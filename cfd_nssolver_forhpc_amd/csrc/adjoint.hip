@@ -623,7 +623,12 @@ int adjoint_step(Ctx &c, AdjState &A, const double *x, const double *t2, bool so
       int it = 0;
       double rel = 0.0;
       A.adj_solves++;
-      int rc = bicgstab_solve_aux(c, A.rhs, A.z, P.tol, P.maxit, A.adj_solves, &it, &rel);
+      int rc;
+      if (P.precond == CFD_PRECOND_MG)
+        rc = pbicgstab_solve_aux(c, A.rhs, A.z, P.tol, P.maxit,
+                                 P.precision == CFD_MIXED && P.mixed_mode == CFD_MIXED_PC32, &it, &rel);
+      else
+        rc = bicgstab_solve_aux(c, A.rhs, A.z, P.tol, P.maxit, A.adj_solves, &it, &rel);
       if (rc) return rc;
       A.adj_iters += it;
       adj_rhs_back_kernel<<<blocks(N), 256, 0, c.stream>>>(g, c.f[CFD_JP], A.z, A.lb);

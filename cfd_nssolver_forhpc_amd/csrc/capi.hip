@@ -64,9 +64,11 @@ int check_params(const cfd_params *p) {
     return fail(CFD_ERR_ARG, "red-black SOR runs on the ELL operator");
   if (p->maxit < 0) return fail(CFD_ERR_ARG, "maxit must be >= 0");
   if (p->precision == CFD_MIXED && p->mixed_mode != CFD_MIXED_RELIABLE && p->mixed_mode != CFD_MIXED_RESTARTED &&
-      p->mixed_mode != CFD_MIXED_VALUES32)
+      p->mixed_mode != CFD_MIXED_VALUES32 && p->mixed_mode != CFD_MIXED_PC32)
     return fail(CFD_ERR_ARG, "unknown mixed-precision scheme");
-  if (p->precision == CFD_MIXED && p->mixed_mode != CFD_MIXED_RESTARTED &&
+  if (p->precision == CFD_MIXED && p->mixed_mode == CFD_MIXED_PC32 && p->precond != CFD_PRECOND_MG)
+    return fail(CFD_ERR_ARG, "CFD_MIXED_PC32 needs precond = CFD_PRECOND_MG");
+  if (p->precision == CFD_MIXED && (p->mixed_mode == CFD_MIXED_RELIABLE || p->mixed_mode == CFD_MIXED_VALUES32) &&
       !(p->mixed_delta > 0.0 && p->mixed_delta < 1.0))
     return fail(CFD_ERR_ARG, "mixed_delta must be in (0, 1)");
   if (p->ib_on && (p->ib_nsub < 1 || !(p->ib_r > 0.0))) return fail(CFD_ERR_ARG, "invalid immersed body");
@@ -184,6 +186,11 @@ int sync_p(Ctx &c) {
 int solve_configured(Ctx &c, int *iters, double *relres, int *inner) {
   const cfd_params &P = c.prm;
   *inner = 0;
+  if (P.precond == CFD_PRECOND_MG && P.solver == CFD_SOLVER_BICGSTAB &&
+      (P.precision == CFD_FP64 || P.mixed_mode == CFD_MIXED_PC32)) {
+    // FP64 BiCGSTAB preconditioned by the V-cycle, the cycle in FP32 for CFD_MIXED_PC32
+    return pbicgstab_solve64(c, c.f[CFD_RHS], c.f[CFD_X], P.tol, P.maxit, P.precision == CFD_MIXED, iters, relres);
+  }
   if (P.precision == CFD_MIXED) {
     int rc = ensure_mixed(c);
     if (!rc && P.solver == CFD_SOLVER_RBSOR) {
@@ -204,7 +211,6 @@ int solve_configured(Ctx &c, int *iters, double *relres, int *inner) {
     if (rc) return rc;
     return rbsor_solve(c, c.f[CFD_RHS], c.f[CFD_X], P.sor_omega, P.tol, P.maxit, P.sor_check, iters, relres);
   }
-  if (P.precond == CFD_PRECOND_MG) return pbicgstab_solve64(c, c.f[CFD_RHS], c.f[CFD_X], P.tol, P.maxit, iters, relres);
   return bicgstab_solve(c, c.f[CFD_RHS], c.f[CFD_X], P.tol, P.maxit, iters, relres);
 }
 

@@ -119,9 +119,14 @@ int mg_ensure(Ctx &c, bool fp32);
 void mg_release(Ctx &c);
 int mg_apply64(Ctx &c, const double *r, double *out, const KrylovScalars *ks);
 int mg_apply32(Ctx &c, const float *r, float *out, const KrylovScalars *ks);
+int mg_apply64_fp32cycle(Ctx &c, const double *r, double *out, const KrylovScalars *ks);
 const char *mg_shape(const Ctx &c);
-// k_krylov.hip: BiCGSTAB right-preconditioned by the V-cycle, FP64
-int pbicgstab_solve64(Ctx &c, const double *b, double *x, double tol, int maxit, int *iters, double *relres);
+// k_krylov.hip: FP64 BiCGSTAB right-preconditioned by the V-cycle (run in
+// FP32 when pc32); the _aux form leaves the solve count untouched
+int pbicgstab_solve64(Ctx &c, const double *b, double *x, double tol, int maxit, bool pc32, int *iters,
+                      double *relres);
+int pbicgstab_solve_aux(Ctx &c, const double *b, double *x, double tol, int maxit, bool pc32, int *iters,
+                        double *relres);
 
 // one timestep (cfd_step's body); adjoint.hip's recomputation calls it too
 int step_impl(Ctx &c, cfd_step_stats *stats);

@@ -390,18 +390,28 @@ int mg_apply_t<float>(Ctx &c, const float *r, float *out, const KrylovScalars *k
   return mg_apply32(c, r, out, ks);
 }
 
+// the preconditioner: the V-cycle in the vectors' precision, or (pc32, FP64
+// vectors only) in FP32
 template <typename T>
-int launch_pc_iters(Ctx &c, const OpArgs &a, const PVecs<T> &v, KrylovScalars *ks, int nit) {
+int precondition(Ctx &c, const T *r, T *out, const KrylovScalars *ks, bool pc32) {
+  if constexpr (std::is_same<T, double>::value) {
+    if (pc32) return mg_apply64_fp32cycle(c, r, out, ks);
+  }
+  return mg_apply_t<T>(c, r, out, ks);
+}
+
+template <typename T>
+int launch_pc_iters(Ctx &c, const OpArgs &a, const PVecs<T> &v, KrylovScalars *ks, int nit, bool pc32) {
   const int64_t N = c.g.N;
   const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
   double *p1 = red_part_host(c, RS_SPMV_DOT), *p2 = red_part_host(c, RS_SPMV_DOT2), *p3 = red_part_host(c, RS_XR);
   for (int it = 0; it < nit; ++it) {
-    int rc = mg_apply_t<T>(c, v.p1, v.ph, ks);
+    int rc = precondition<T>(c, v.p1, v.ph, ks, pc32);
     if (rc) return rc;
     spmv_dot_kernel<T><<<nb, 256, 0, c.stream>>>(a, ks, v.ph, v.v, v.r0, p1);
     alpha_finalize<<<1, 256, 0, c.stream>>>(p1, nb, ks);
     s_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.r1, v.v, v.s1);
-    if ((rc = mg_apply_t<T>(c, v.s1, v.sh, ks))) return rc;
+    if ((rc = precondition<T>(c, v.s1, v.sh, ks, pc32))) return rc;
     spmv_dot2_pc_kernel<T><<<nb, 256, 0, c.stream>>>(a, ks, v.sh, v.s1, v.t, p2);
     omega_finalize<<<1, 256, 0, c.stream>>>(p2, nb, ks);
     xr_update_pc_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.x, v.ph, v.sh, v.s1, v.t, v.r1, v.r0, p3);
@@ -414,7 +424,7 @@ int launch_pc_iters(Ctx &c, const OpArgs &a, const PVecs<T> &v, KrylovScalars *k
 
 template <typename T>
 int solve_pbicg(Ctx &c, const OpArgs &a, const PVecs<T> &v, KrylovScalars *ks, double tol, int maxit, int *iters,
-                double *relres) {
+                double *relres, bool pc32 = false) {
   const int64_t N = c.g.N;
   const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
   ks_setup_kernel<<<1, 1, 0, c.stream>>>(ks, tol, 0.0, maxit, 0);
@@ -428,14 +438,28 @@ int solve_pbicg(Ctx &c, const OpArgs &a, const PVecs<T> &v, KrylovScalars *ks, d
     *relres = c.hks->norm0 > 0.0 ? c.hks->relres : 0.0;
     return CFD_OK;
   }
-  int batch = 2;
+  // Each preconditioned iteration is ~100 launches, so the batch follows the
+  // observed convergence rate: launches after the converged iteration are
+  // no-ops but still cost their launch latency.
+  int batch = 1;
+  double prev_rel = c.hks->relres;
+  int prev_it = 0;
   for (;;) {
-    int rc = launch_pc_iters<T>(c, a, v, ks, batch);
+    int rc = launch_pc_iters<T>(c, a, v, ks, batch, pc32);
     if (rc) return rc;
     CFD_HIP(hipMemcpyAsync(c.hks, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c.stream));
     CFD_HIP(hipStreamSynchronize(c.stream));
-    if (c.hks->done) break;
-    batch = std::min(batch * 2, 8);
+    const KrylovScalars h = *c.hks;
+    if (h.done) break;
+    int next = 1;
+    if (h.relres > 0.0 && h.relres < prev_rel && h.iters > prev_it) {
+      const double rate = std::log(h.relres / prev_rel) / (double)(h.iters - prev_it);
+      const double need = std::log(h.tol / h.relres) / rate;
+      if (need > 0.0 && std::isfinite(need)) next = (int)std::floor(need);
+    }
+    prev_rel = h.relres;
+    prev_it = h.iters;
+    batch = std::max(1, std::min(next, 8));
   }
   *iters = c.hks->iters;
   *relres = c.hks->relres;
@@ -444,13 +468,20 @@ int solve_pbicg(Ctx &c, const OpArgs &a, const PVecs<T> &v, KrylovScalars *ks, d
 
 }  // namespace
 
-int pbicgstab_solve64(Ctx &c, const double *b, double *x, double tol, int maxit, int *iters, double *relres) {
+int pbicgstab_solve_aux(Ctx &c, const double *b, double *x, double tol, int maxit, bool pc32, int *iters,
+                        double *relres) {
   if (!c.ell_ready) return fail(CFD_ERR_STATE, "ELL operator not assembled");
-  int rc = mg_ensure(c, false);
+  int rc = mg_ensure(c, pc32);
+  if (!rc && pc32) rc = mg_ensure(c, false);  // the FP64 preconditioned directions
   if (rc) return rc;
   PVecs<double> v{x, c.r0, c.r1, c.p1, c.s1, c.ap, c.as, c.ph64, c.sh64, b};
+  return solve_pbicg<double>(c, op_args(c, CFD_OP_ELL), v, c.ks, tol, maxit, iters, relres, pc32);
+}
+
+int pbicgstab_solve64(Ctx &c, const double *b, double *x, double tol, int maxit, bool pc32, int *iters,
+                      double *relres) {
   c.solves++;
-  return solve_pbicg<double>(c, op_args(c, CFD_OP_ELL), v, c.ks, tol, maxit, iters, relres);
+  return pbicgstab_solve_aux(c, b, x, tol, maxit, pc32, iters, relres);
 }
 
 int bicgstab_solve(Ctx &c, const double *b, double *x, double tol, int maxit, int *iters, double *relres) {

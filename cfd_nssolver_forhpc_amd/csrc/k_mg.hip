@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "dev_common.h"
@@ -162,12 +163,20 @@ __device__ inline void prolong_cell(const MGView &F, const MGView &C, int64_t t,
   xf[t] += xc[idx3(C, q.i / F.fx, q.j / F.fy, q.k / F.fz)];
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) mg_scale_kernel(int64_t n, const T *__restrict__ r, const double *__restrict__ jp,
+template <typename T, typename V>
+__global__ void __launch_bounds__(256) mg_scale_kernel(int64_t n, const V *__restrict__ r, const double *__restrict__ jp,
                                                        T *__restrict__ b, const KrylovScalars *ks) {
   if (ks && ks->done) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) b[t] = (T)((double)r[t] / jp[t]);
+}
+
+template <typename T, typename V>
+__global__ void __launch_bounds__(256) mg_widen_kernel(int64_t n, const T *__restrict__ x, V *__restrict__ out,
+                                                       const KrylovScalars *ks) {
+  if (ks && ks->done) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = (V)x[t];
 }
 
 template <typename T>
@@ -251,15 +260,18 @@ double *lb<double>(MGLevel &L) { return L.b64; }
 template <>
 float *lb<float>(MGLevel &L) { return L.b32; }
 
-// out = V-cycle(L, r / jp)
-template <typename T>
-int mg_apply(Ctx &c, const T *r, T *out, const KrylovScalars *ks) {
+// out = V-cycle(L, r / jp): the cycle runs in T, the vectors r and out are V
+// (an FP32 cycle inside an FP64 Krylov solve widens its result at the end)
+template <typename T, typename V>
+int mg_apply(Ctx &c, const V *r, V *out, const KrylovScalars *ks) {
   MGState &M = *c.mg;
   const int nl = (int)M.lv.size();
   T *b0 = lb<T>(M.lv[0]);
-  if (!b0) return fail(CFD_ERR_STATE, "multigrid buffers of this precision are not allocated");
-  mg_scale_kernel<T><<<nblk(c.g.N), 256, 0, c.stream>>>(c.g.N, r, c.f[CFD_JP], b0, ks);
-  auto X = [&](int l) { return l == 0 ? out : lx<T>(M.lv[l]); };
+  constexpr bool same = std::is_same<T, V>::value;
+  T *x0 = same ? reinterpret_cast<T *>(out) : lx<T>(M.lv[0]);
+  if (!b0 || !x0) return fail(CFD_ERR_STATE, "multigrid buffers of this precision are not allocated");
+  mg_scale_kernel<T, V><<<nblk(c.g.N), 256, 0, c.stream>>>(c.g.N, r, c.f[CFD_JP], b0, ks);
+  auto X = [&](int l) { return l == 0 ? x0 : lx<T>(M.lv[l]); };
   auto rb = [&](int l, bool from_zero) {
     const MGView &v = M.lv[l].v;
     const int64_t cnt = (int64_t)v.nx * v.ny * ((v.nz + 1) / 2);
@@ -289,6 +301,7 @@ int mg_apply(Ctx &c, const T *r, T *out, const KrylovScalars *ks) {
     mg_prolong_kernel<T><<<nblk(M.lv[l].v.n), 256, 0, c.stream>>>(M.lv[l].v, M.lv[l + 1].v, X(l + 1), X(l), ks);
     rb(l, false);
   }
+  if (!same) mg_widen_kernel<T, V><<<nblk(c.g.N), 256, 0, c.stream>>>(c.g.N, x0, out, ks);
   CFD_HIP(hipGetLastError());
   return CFD_OK;
 }
@@ -400,7 +413,7 @@ int mg_ensure(Ctx &c, bool fp32) {
     int rc = CFD_OK;
     if (fp32) {
       if (!L.b32 && (rc = mg_alloc(&L.b32, L.v.n))) return rc;
-      if (l > 0 && !L.x32 && (rc = mg_alloc(&L.x32, L.v.n))) return rc;
+      if (!L.x32 && (rc = mg_alloc(&L.x32, L.v.n))) return rc;  // level 0 too: the FP64 solve's FP32 cycle
     } else {
       if (!L.b64 && (rc = mg_alloc(&L.b64, L.v.n))) return rc;
       if (l > 0 && !L.x64 && (rc = mg_alloc(&L.x64, L.v.n))) return rc;
@@ -416,8 +429,13 @@ int mg_ensure(Ctx &c, bool fp32) {
   return CFD_OK;
 }
 
-int mg_apply64(Ctx &c, const double *r, double *out, const KrylovScalars *ks) { return mg_apply<double>(c, r, out, ks); }
-int mg_apply32(Ctx &c, const float *r, float *out, const KrylovScalars *ks) { return mg_apply<float>(c, r, out, ks); }
+int mg_apply64(Ctx &c, const double *r, double *out, const KrylovScalars *ks) {
+  return mg_apply<double, double>(c, r, out, ks);
+}
+int mg_apply32(Ctx &c, const float *r, float *out, const KrylovScalars *ks) { return mg_apply<float, float>(c, r, out, ks); }
+int mg_apply64_fp32cycle(Ctx &c, const double *r, double *out, const KrylovScalars *ks) {
+  return mg_apply<float, double>(c, r, out, ks);
+}
 
 const char *mg_shape(const Ctx &c) { return c.mg ? c.mg->shape.c_str() : ""; }
 

@@ -67,6 +67,7 @@ extern "C" {
 #define CFD_MIXED_RELIABLE 0  /* FP32 BiCGSTAB with FP64 residual replacement (reliable updates) */
 #define CFD_MIXED_RESTARTED 1 /* FP64 defect correction around restarted FP32 inner solves */
 #define CFD_MIXED_VALUES32 2  /* FP64 BiCGSTAB on the FP32-valued ELL operator, FP64 residual replacement */
+#define CFD_MIXED_PC32 3      /* with CFD_PRECOND_MG: FP64 BiCGSTAB whose multigrid V-cycle runs in FP32 */
 
 /* preconditioning of the BiCGSTAB pressure solve */
 #define CFD_PRECOND_JACOBI 0 /* the reference: Jacobi scaling only (bicgstabRe2 on the jp-scaled operator) */
@@ -117,7 +118,7 @@ typedef struct cfd_params {
   int precision;         /* CFD_FP64 / CFD_MIXED */
   double tol;            /* simuClass::p_criteria; relative residual for BiCGSTAB / RB-SOR */
   double tol_inner;      /* CFD_MIXED, restarted mode: tolerance of each FP32 inner solve */
-  int mixed_mode;        /* CFD_MIXED: CFD_MIXED_RELIABLE (default) or CFD_MIXED_RESTARTED */
+  int mixed_mode;        /* CFD_MIXED: CFD_MIXED_RELIABLE (default), _RESTARTED, _VALUES32 or _PC32 */
   double mixed_delta;    /* CFD_MIXED_RELIABLE: FP64 residual replacement when the FP32
                             residual has dropped by this factor (0.1) */
   int maxit;             /* bicgstabRe2::iters_max_ (5000) / RB-SOR sweep cap */

@@ -157,3 +157,24 @@ def test_bad_adjoint_arguments_are_errors(require_gpu):
     sim.createPressureMatrix()
     with pytest.raises(capi.CfdInvalidArgument):
         sim.adjoint_forward(5, 5, 1)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "pc32"])
+def test_multigrid_forward_and_adjoint_solves_match_the_oracle(require_gpu, mode):
+    """The same objective and gradient with every forward and transposed
+    pressure solve preconditioned by the multigrid V-cycle (to the same
+    tolerance)."""
+    w, uin = case()
+    w.params.precond = capi.CFD_PRECOND_MG
+    if mode == "pc32":
+        w.params.precision = capi.CFD_MIXED
+        w.params.mixed_mode = capi.CFD_MIXED_PC32
+    pr = gpu_problem(w, every=4)
+    J = pr.run(uin, 1.0)
+    g_uin, g_gain, st = pr.sim.adjoint_backward()
+    w0, _ = case()
+    Jo, go_uin, go_gain = oracle_gradient(w0, uin, 1.0)
+    print(mode, "J", J, Jo, "adjoint iterations", st["adjoint_iters"], "solves", st["adjoint_solves"])
+    assert rel(J, Jo) < 1e-9
+    assert np.linalg.norm(g_uin - go_uin) <= 1e-7 * np.linalg.norm(go_uin)
+    assert rel(g_gain, go_gain) < 1e-7

@@ -179,3 +179,20 @@ def test_mg_step_matches_jacobi_step(require_gpu):
         assert max(sb.relres[:sb.n_passes]) < 1e-12
     for fid in (capi.T0_U, capi.T0_V, capi.T0_W, capi.X):
         assert rel_l2(b.download(fid), a.download(fid)) < 1e-9
+
+
+def test_mixed_pc32_reaches_fp64_residual(require_gpu):
+    """FP64 BiCGSTAB with the V-cycle in FP32: the preconditioner's rounding
+    does not limit the attainable FP64 residual."""
+    ref, b = poisson_sim(ncal=(64, 32, 16))
+    ref.bicgstab(tol=1e-13, maxit=50000)
+    x64 = ref.download(capi.X)
+    sim, _ = mg_sim(precision=capi.CFD_MIXED)
+    p = sim.params.copy()
+    p.mixed_mode = capi.CFD_MIXED_PC32
+    sim.set_params(p)
+    it, rel = sim.solve()
+    tr, x = true_relres(sim, b)
+    print("PC32 iterations", it, "true relres", tr)
+    assert rel < 1e-10 and tr < 2e-10 and it <= 40
+    assert rel_l2(x, x64) < 1e-7
