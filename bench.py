@@ -182,7 +182,7 @@ def kernel_table(sim, reps):
     return rows
 
 
-def adjoint_benchmark(steps, avg_from, every, fd_dirs, precond):
+def adjoint_benchmark(steps, avg_from, every, fd_dirs, precond, tol=1e-10, h=1e-6):
     """Discrete adjoint on the 256x128x32 Re-100 cylinder (dt 1e-3): J = mean
     corrector cD over steps [avg_from, steps), gradient w.r.t. the seeded
     smoothed inflow profile (128 values) and the DFIB forcing gain; timed
@@ -191,6 +191,7 @@ def adjoint_benchmark(steps, avg_from, every, fd_dirs, precond):
     from cfd_nssolver_forhpc_amd.adjoint import DragProblem, smoothed_inflow
     w = configs.config_d()
     w.params.precond = precond
+    w.params.tol = tol
     sim = Simulation(w.grid, w.params)
     sim.init_state(w.noise, w.seed)
     sim.DFIB_CylinderZ()
@@ -207,7 +208,6 @@ def adjoint_benchmark(steps, avg_from, every, fd_dirs, precond):
     log(f"[bench] adjoint: J {J:.6f}, forward {t_fwd:.2f} s, reverse sweep {t_bwd:.2f} s")
     checks = []
     rng = np.random.default_rng(8)
-    h = 1e-6
     for q in range(fd_dirs):
         d = rng.standard_normal(uin.size + 1)
         d /= np.linalg.norm(d)
@@ -220,11 +220,11 @@ def adjoint_benchmark(steps, avg_from, every, fd_dirs, precond):
     sim.close()
     return {
         "workload": w.name, "steps": steps, "avg_from": avg_from, "checkpoint_every": every,
-        "pressure": "BiCGSTAB to 1e-10, " + ("multigrid-preconditioned" if precond else "Jacobi-scaled") + ", FP64",
+        "pressure": f"BiCGSTAB to {tol:g}, " + ("multigrid-preconditioned" if precond else "Jacobi-scaled") + ", FP64",
         "J_mean_cd": J, "grad_uin_norm": float(np.linalg.norm(g_uin)), "grad_gain": g_gain,
         "forward_s": t_fwd, "reverse_sweep_s": t_bwd, "reverse_over_forward": t_bwd / t_fwd,
         "forward_ms_per_step": 1e3 * t_fwd / steps, "reverse_ms_per_step": 1e3 * t_bwd / steps,
-        "stats": st, "fd_checks": checks,
+        "stats": st, "fd_step": h, "fd_checks": checks,
     }
 
 
@@ -249,6 +249,8 @@ def main():
     ap.add_argument("--adjoint-every", type=int, default=20, help="checkpoint interval of the adjoint run")
     ap.add_argument("--adjoint-fd", type=int, default=0, help="central-difference directions to check the gradient")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg and print its JSON")
+    ap.add_argument("--adjoint-tol", type=float, default=1e-10, help="pressure tolerance of the adjoint run")
+    ap.add_argument("--adjoint-h", type=float, default=1e-6, help="central-difference step of the gradient check")
     ap.add_argument("--adjoint-jacobi", action="store_true",
                     help="solve the adjoint run's pressure systems with the reference's Jacobi scaling (default: MG)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -263,8 +265,8 @@ def main():
     if args.adjoint_only:
         steps = max(args.adjoint_steps, 2)
         pc = capi.CFD_PRECOND_JACOBI if args.adjoint_jacobi else capi.CFD_PRECOND_MG
-        print(json.dumps({"adjoint": adjoint_benchmark(steps, steps // 2, args.adjoint_every, args.adjoint_fd, pc)}),
-              flush=True)
+        print(json.dumps({"adjoint": adjoint_benchmark(steps, steps // 2, args.adjoint_every, args.adjoint_fd, pc,
+                                                       args.adjoint_tol, args.adjoint_h)}), flush=True)
         return
 
     w64 = configs.config_b(capi.CFD_FP64)
@@ -364,7 +366,8 @@ def main():
     adjoint = None
     if args.adjoint_steps > 1:
         pc = capi.CFD_PRECOND_JACOBI if args.adjoint_jacobi else capi.CFD_PRECOND_MG
-        adjoint = adjoint_benchmark(args.adjoint_steps, args.adjoint_steps // 2, args.adjoint_every, args.adjoint_fd, pc)
+        adjoint = adjoint_benchmark(args.adjoint_steps, args.adjoint_steps // 2, args.adjoint_every, args.adjoint_fd, pc,
+                                    args.adjoint_tol, args.adjoint_h)
 
     cpu = None
     if not args.no_cpu:

@@ -91,6 +91,7 @@ void free_ctx(Ctx *c) {
   adjoint_release(*c);
   mg_release(*c);
   dfree(c->uin);
+  for (auto &q : c->pred) dfree(q);
   for (auto &p : c->f) dfree(p);
   dfree(c->col); dfree(c->val); dfree(c->val32); dfree(c->colP); dfree(c->valP); dfree(c->valP32); dfree(c->perm);
   dfree(c->spe_coef); dfree(c->spe_jp);
@@ -813,8 +814,22 @@ int step_impl(Ctx &c, cfd_step_stats *stats) {
     const bool corrector = pass == P.inner_loops;
     const bool with_src = pass > 0 && P.inner_loops > 0;
     const int src_mode = with_src ? (P.source_index_quirk ? 2 : 1) : 0;
-    RET_LAUNCH(launch_predictor(c, u0, v0, w0, u1, v1, w1, c.f[CFD_SRC], src_mode));
-    if (src_mode == 2) RET_LAUNCH(launch_source_quirk_fixup(c, u1, v1, w1, c.f[CFD_SRC]));
+    if (src_mode == 2 || P.inner_loops == 0) {
+      // the reference's index mapping adds the source outside the predictor's
+      // positions too: the predictor is re-evaluated every pass
+      RET_LAUNCH(launch_predictor(c, u0, v0, w0, u1, v1, w1, c.f[CFD_SRC], src_mode));
+      if (src_mode == 2) RET_LAUNCH(launch_source_quirk_fixup(c, u1, v1, w1, c.f[CFD_SRC]));
+    } else if (pass == 0) {
+      // T0 is the same in every pass: predict once, keep P for the other passes
+      for (int q = 0; q < 3; ++q)
+        if (!c.pred[q]) {
+          int rc = dmalloc(&c.pred[q], (size_t)c.g.G);
+          if (rc) return rc;
+        }
+      RET_LAUNCH(launch_predictor(c, u0, v0, w0, u1, v1, w1, c.f[CFD_SRC], 0, c.pred));
+    } else {
+      RET_LAUNCH(launch_pred_add(c, c.pred, c.f[CFD_SRC], u1, v1, w1));
+    }
     st.iters[pass] = -1;
     if (solve) {
       RET_LAUNCH(launch_rhs(c, u1, v1, w1, c.f[CFD_JP], c.f[CFD_RHS]));

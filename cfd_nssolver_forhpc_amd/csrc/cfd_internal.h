@@ -112,6 +112,8 @@ struct Ctx {
   struct MGState *mg = nullptr;
   double *ph64 = nullptr, *sh64 = nullptr;
   float *ph32 = nullptr, *sh32 = nullptr;
+  // the step's prediction (u, v, w; ghosted), kept by its first pass
+  double *pred[3] = {nullptr, nullptr, nullptr};
 };
 
 // k_mg.hip
@@ -162,7 +164,8 @@ inline int blocks_for(int64_t n, int threads, int cap) {
 hipError_t launch_init_fields(Ctx &c, double noise_amp, uint64_t seed);
 hipError_t launch_eta_cylinder_z(Ctx &c, double inv_n2);
 hipError_t launch_predictor(Ctx &c, const double *u0, const double *v0, const double *w0, double *u1,
-                            double *v1, double *w1, const double *src, int src_mode);
+                            double *v1, double *w1, const double *src, int src_mode, double *const keep[3] = nullptr);
+hipError_t launch_pred_add(Ctx &c, double *const keep[3], const double *src, double *u1, double *v1, double *w1);
 hipError_t launch_source_add(Ctx &c, double *u1, double *v1, double *w1, const double *src, int quirk);
 hipError_t launch_source_quirk_fixup(Ctx &c, double *u1, double *v1, double *w1, const double *src);
 hipError_t launch_rhs(Ctx &c, const double *u1, const double *v1, const double *w1, const double *jp, double *b);

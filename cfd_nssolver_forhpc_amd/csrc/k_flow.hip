@@ -68,7 +68,8 @@ __global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, do
                                                         const double *__restrict__ v, const double *__restrict__ w,
                                                         double *__restrict__ u1, double *__restrict__ v1,
                                                         double *__restrict__ w1, const double *__restrict__ src,
-                                                        int src_mode) {
+                                                        int src_mode, double *__restrict__ pu,
+                                                        double *__restrict__ pv, double *__restrict__ pw) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   const Cell c = decode_interior(g, t);
@@ -101,6 +102,7 @@ __global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, do
                                 ((u[yp] - uc) / Dys[j] - (uc - u[ym]) / Dys[j - 1]) / Dy[j] +
                                 ((u[zp] - uc) / Dzs[k] - (uc - u[zm]) / Dzs[k - 1]) / Dz[k]);
     double r = uc + conv + diff;
+    if (pu) pu[ic] = r;
     if (src_mode == 1) r += src[t];
     else if (src_mode == 2 && ic < N) r += src[ic];
     u1[ic] = r;
@@ -124,6 +126,7 @@ __global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, do
                                 ((v[yp] - vc) / Dy[j + 1] - (vc - v[ym]) / Dy[j]) / Dys[j] +
                                 ((v[zp] - vc) / Dzs[k] - (vc - v[zm]) / Dzs[k - 1]) / Dz[k]);
     double r = vc + conv + diff;
+    if (pv) pv[ic] = r;
     if (src_mode == 1) r += src[N + t];
     else if (src_mode == 2 && ic < N) r += src[N + ic];
     v1[ic] = r;
@@ -147,10 +150,28 @@ __global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, do
                                 ((w[yp] - wc) / Dys[j] - (wc - w[ym]) / Dys[j - 1]) / Dy[j] +
                                 ((w[zp] - wc) / Dz[k + 1] - (wc - w[zm]) / Dz[k]) / Dzs[k]);
     double r = wc + conv + diff;
+    if (pw) pw[ic] = r;
     if (src_mode == 1) r += src[2 * N + t];
     else if (src_mode == 2 && ic < N) r += src[2 * N + ic];
     w1[ic] = r;
   }
+}
+
+// T1 = P + source_termF at the predictor's positions, from the prediction P
+// kept by the step's first pass (T0, hence P, is the same in every pass; the
+// sum is the one predictor_kernel forms, so the result is bit-identical)
+__global__ void __launch_bounds__(256) pred_add_kernel(DevGrid g, const double *__restrict__ pu,
+                                                       const double *__restrict__ pv, const double *__restrict__ pw,
+                                                       const double *__restrict__ src, double *__restrict__ u1,
+                                                       double *__restrict__ v1, double *__restrict__ w1) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  const Cell c = decode_interior(g, t);
+  const int64_t ic = icel(g, c.i, c.j, c.k);
+  const int64_t N = g.N;
+  if (c.i < g.nx - GC - 1) u1[ic] = pu[ic] + src[t];
+  if (c.j < g.ny - GC - 1) v1[ic] = pv[ic] + src[N + t];
+  if (c.k < g.nz - GC - 1) w1[ic] = pw[ic] + src[2 * N + t];
 }
 
 // T1 += source_termF at positions the predictor did not write (reference
@@ -584,9 +605,16 @@ hipError_t launch_eta_cylinder_z(Ctx &c, double inv_n2) {
 }
 
 hipError_t launch_predictor(Ctx &c, const double *u0, const double *v0, const double *w0, double *u1, double *v1,
-                            double *w1, const double *src, int src_mode) {
-  predictor_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g, 1.0 / c.prm.Re, c.prm.dt, u0, v0, w0,
-                                                                          u1, v1, w1, src, src_mode);
+                            double *w1, const double *src, int src_mode, double *const keep[3]) {
+  predictor_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(
+      c.g, 1.0 / c.prm.Re, c.prm.dt, u0, v0, w0, u1, v1, w1, src, src_mode, keep ? keep[0] : nullptr,
+      keep ? keep[1] : nullptr, keep ? keep[2] : nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_pred_add(Ctx &c, double *const keep[3], const double *src, double *u1, double *v1, double *w1) {
+  pred_add_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g, keep[0], keep[1], keep[2], src, u1,
+                                                                         v1, w1);
   return hipGetLastError();
 }
 

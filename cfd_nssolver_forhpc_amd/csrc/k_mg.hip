@@ -25,6 +25,7 @@
 // so preconditioned iterations can be launched in batches like the plain
 // ones.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -57,6 +58,9 @@ struct MGState {
   int small_from = 0;   // first level handled by the one-workgroup kernel
   int nu = 2;
   std::string shape;
+  int dense_n = 0;      // > 0: the coarsest level (dense_n cells) is solved by its dense inverse
+  double *cinv64 = nullptr;
+  float *cinv32 = nullptr;
 };
 
 namespace {
@@ -117,15 +121,12 @@ __device__ inline double mg_vol(const MGView &L, int i, int j, int k) {
   return L.ax[3 * L.nx + i] * L.ay[3 * L.ny + j] * L.az[3 * L.nz + k];
 }
 
-// one colour of a red-black Gauss-Seidel sweep; `first`: the sweep starts
-// from x = 0 (this colour gets b/diag, the other colour is cleared)
+// one colour of a red-black Gauss-Seidel sweep at (i, j, k2), k = 2 k2 +
+// parity; `first`: the sweep starts from x = 0 (this colour gets b/diag,
+// the other colour is cleared)
 template <typename T>
-__device__ inline void rb_cell(const MGView &L, int64_t t, int colour, bool first, const T *__restrict__ b,
-                               T *__restrict__ x) {
-  const int half = (L.nz + 1) / 2;
-  const int k2 = (int)(t % half);
-  const int64_t r = t / half;
-  const int j = (int)(r % L.ny), i = (int)(r / L.ny);
+__device__ inline void rb_at(const MGView &L, int i, int j, int k2, int colour, bool first, const T *__restrict__ b,
+                             T *__restrict__ x) {
   const int k = 2 * k2 + ((i + j + colour) & 1);
   if (first) {
     const int ko = 2 * k2 + ((i + j + colour + 1) & 1);
@@ -140,27 +141,27 @@ __device__ inline void rb_cell(const MGView &L, int64_t t, int colour, bool firs
   }
 }
 
+// residual of the children of coarse cell (I, J, K), volume-weighted into bc
 template <typename T>
-__device__ inline void restrict_cell(const MGView &F, const MGView &C, int64_t t, const T *__restrict__ bf,
-                                     const T *__restrict__ xf, T *__restrict__ bc) {
-  const Cell3 q = dec3(C, t);
+__device__ inline void restrict_at(const MGView &F, const MGView &C, int I, int J, int K, const T *__restrict__ bf,
+                                   const T *__restrict__ xf, T *__restrict__ bc) {
   double acc = 0.0;
   for (int di = 0; di < F.fx; ++di)
     for (int dj = 0; dj < F.fy; ++dj)
       for (int dk = 0; dk < F.fz; ++dk) {
-        const int i = q.i * F.fx + di, j = q.j * F.fy + dj, k = q.k * F.fz + dk;
+        const int i = I * F.fx + di, j = J * F.fy + dj, k = K * F.fz + dk;
         const int64_t c = idx3(F, i, j, k);
         const T r = bf[c] - (mg_diag<T>(F, i, j, k) * xf[c] - mg_off<T>(F, xf, i, j, k, c));
         acc += mg_vol<T>(F, i, j, k) * (double)r;
       }
-  bc[t] = (T)(acc / mg_vol<T>(C, q.i, q.j, q.k));
+  bc[idx3(C, I, J, K)] = (T)(acc / mg_vol<T>(C, I, J, K));
 }
 
 template <typename T>
-__device__ inline void prolong_cell(const MGView &F, const MGView &C, int64_t t, const T *__restrict__ xc,
-                                    T *__restrict__ xf) {
-  const Cell3 q = dec3(F, t);
-  xf[t] += xc[idx3(C, q.i / F.fx, q.j / F.fy, q.k / F.fz)];
+__device__ inline void prolong_at(const MGView &F, const MGView &C, int i, int j, int k, const T *__restrict__ xc,
+                                  T *__restrict__ xf) {
+  const int I = F.fx == 2 ? i >> 1 : i, J = F.fy == 2 ? j >> 1 : j, K = F.fz == 2 ? k >> 1 : k;
+  xf[idx3(F, i, j, k)] += xc[idx3(C, I, J, K)];
 }
 
 template <typename T, typename V>
@@ -179,13 +180,14 @@ __global__ void __launch_bounds__(256) mg_widen_kernel(int64_t n, const T *__res
   if (t < n) out[t] = (V)x[t];
 }
 
+// 3-D launches (x: k or k2, y: j, z: i): no 64-bit index division per thread
 template <typename T>
 __global__ void __launch_bounds__(256) mg_rb_kernel(MGView L, int colour, int first, const T *__restrict__ b,
                                                     T *__restrict__ x, const KrylovScalars *ks) {
   if (ks && ks->done) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)L.nx * L.ny * ((L.nz + 1) / 2)) return;
-  rb_cell<T>(L, t, colour, first != 0, b, x);
+  const int k2 = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+  if (k2 >= (L.nz + 1) / 2 || j >= L.ny) return;
+  rb_at<T>(L, i, j, k2, colour, first != 0, b, x);
 }
 
 template <typename T>
@@ -193,56 +195,99 @@ __global__ void __launch_bounds__(256) mg_restrict_kernel(MGView F, MGView C, co
                                                           const T *__restrict__ xf, T *__restrict__ bc,
                                                           const KrylovScalars *ks) {
   if (ks && ks->done) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < C.n) restrict_cell<T>(F, C, t, bf, xf, bc);
+  const int K = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y * blockDim.y + threadIdx.y, I = blockIdx.z;
+  if (K >= C.nz || J >= C.ny) return;
+  restrict_at<T>(F, C, I, J, K, bf, xf, bc);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) mg_prolong_kernel(MGView F, MGView C, const T *__restrict__ xc,
                                                          T *__restrict__ xf, const KrylovScalars *ks) {
   if (ks && ks->done) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < F.n) prolong_cell<T>(F, C, t, xc, xf);
+  const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+  if (k >= F.nz || j >= F.ny) return;
+  prolong_at<T>(F, C, i, j, k, xc, xf);
 }
+
+constexpr int MG_SMALL_THREADS = 256;
 
 template <typename T>
 struct SmallArgs {
-  int nlev, nu, coarse_sweeps;
+  int nlev, nu, coarse_sweeps, dense;
   MGView v[MG_MAX_LEVELS];
   T *x[MG_MAX_LEVELS];
   T *b[MG_MAX_LEVELS];
+  const T *cinv;  // dense inverse of the coarsest operator (row-major), when dense
 };
 
 // the bottom of the V-cycle (levels of at most MG_SMALL cells) in one
 // workgroup; __syncthreads separates the phases, whose data stays in this
-// workgroup (L2-resident)
+// workgroup (L2-resident); the coarsest level is solved by its dense
+// inverse when it is nonsingular, else smoothed coarse_sweeps times
 template <typename T>
-__global__ void __launch_bounds__(1024) mg_small_kernel(SmallArgs<T> S, const KrylovScalars *ks) {
+__global__ void __launch_bounds__(MG_SMALL_THREADS) mg_small_kernel(SmallArgs<T> S, const KrylovScalars *ks) {
   if (ks && ks->done) return;
-  const int nt = blockDim.x;
+  const int nt = MG_SMALL_THREADS;
   auto sweep = [&](int l, int nsw, bool from_zero) {
     const MGView &L = S.v[l];
-    const int64_t cnt = (int64_t)L.nx * L.ny * ((L.nz + 1) / 2);
+    const int half = (L.nz + 1) / 2;
+    const int cnt = L.nx * L.ny * half;
     for (int s = 0; s < nsw; ++s)
       for (int colour = 0; colour < 2; ++colour) {
         const bool first = from_zero && s == 0 && colour == 0;
-        for (int64_t t = threadIdx.x; t < cnt; t += nt) rb_cell<T>(L, t, colour, first, S.b[l], S.x[l]);
+        for (int t = threadIdx.x; t < cnt; t += nt) {
+          const int k2 = t % half, r = t / half;
+          rb_at<T>(L, r / L.ny, r % L.ny, k2, colour, first, S.b[l], S.x[l]);
+        }
         __syncthreads();
       }
   };
   const int last = S.nlev - 1;
   for (int l = 0; l < last; ++l) {
     sweep(l, S.nu, true);
-    for (int64_t t = threadIdx.x; t < S.v[l + 1].n; t += nt)
-      restrict_cell<T>(S.v[l], S.v[l + 1], t, S.b[l], S.x[l], S.b[l + 1]);
+    const MGView &C = S.v[l + 1];
+    for (int t = threadIdx.x; t < (int)C.n; t += nt) {
+      const int K = t % C.nz, r = t / C.nz;
+      restrict_at<T>(S.v[l], C, r / C.ny, r % C.ny, K, S.b[l], S.x[l], S.b[l + 1]);
+    }
     __syncthreads();
   }
-  sweep(last, S.coarse_sweeps, true);
+  if (S.dense) {
+    const int n = (int)S.v[last].n;
+    const T *bl = S.b[last];
+    for (int t = threadIdx.x; t < n; t += nt) {
+      const T *row = S.cinv + (int64_t)t * n;
+      double acc = 0.0;
+      for (int q = 0; q < n; ++q) acc += (double)row[q] * (double)bl[q];
+      S.x[last][t] = (T)acc;
+    }
+    __syncthreads();
+  } else {
+    sweep(last, S.coarse_sweeps, true);
+  }
   for (int l = last - 1; l >= 0; --l) {
-    for (int64_t t = threadIdx.x; t < S.v[l].n; t += nt) prolong_cell<T>(S.v[l], S.v[l + 1], t, S.x[l + 1], S.x[l]);
+    const MGView &F = S.v[l];
+    for (int t = threadIdx.x; t < (int)F.n; t += nt) {
+      const int k = t % F.nz, r = t / F.nz;
+      prolong_at<T>(F, S.v[l + 1], r / F.ny, r % F.ny, k, S.x[l + 1], S.x[l]);
+    }
     __syncthreads();
     sweep(l, S.nu, false);
   }
+}
+
+// launch shape of a 3-D kernel over (nk, ny, nx) with 256-thread blocks
+struct Shape3 {
+  dim3 grid, block;
+};
+Shape3 shape3(int nk, int ny, int nx) {
+  int bx = 1;
+  while (bx < nk && bx < 64) bx <<= 1;
+  const int by = 256 / bx;
+  Shape3 s;
+  s.block = dim3(bx, by, 1);
+  s.grid = dim3((nk + bx - 1) / bx, (ny + by - 1) / by, nx);
+  return s;
 }
 
 int nblk(int64_t n) { return blocks_for(n, 256, 1 << 30); }
@@ -274,31 +319,37 @@ int mg_apply(Ctx &c, const V *r, V *out, const KrylovScalars *ks) {
   auto X = [&](int l) { return l == 0 ? x0 : lx<T>(M.lv[l]); };
   auto rb = [&](int l, bool from_zero) {
     const MGView &v = M.lv[l].v;
-    const int64_t cnt = (int64_t)v.nx * v.ny * ((v.nz + 1) / 2);
+    const Shape3 sh = shape3((v.nz + 1) / 2, v.ny, v.nx);
     for (int s = 0; s < M.nu; ++s)
       for (int colour = 0; colour < 2; ++colour)
-        mg_rb_kernel<T><<<nblk(cnt), 256, 0, c.stream>>>(v, colour, from_zero && s == 0 && colour == 0 ? 1 : 0,
-                                                         lb<T>(M.lv[l]), X(l), ks);
+        mg_rb_kernel<T><<<sh.grid, sh.block, 0, c.stream>>>(v, colour, from_zero && s == 0 && colour == 0 ? 1 : 0,
+                                                            lb<T>(M.lv[l]), X(l), ks);
   };
   const int ls = M.small_from;
   for (int l = 0; l < ls; ++l) {
     rb(l, true);
-    mg_restrict_kernel<T><<<nblk(M.lv[l + 1].v.n), 256, 0, c.stream>>>(M.lv[l].v, M.lv[l + 1].v, lb<T>(M.lv[l]),
-                                                                       X(l), lb<T>(M.lv[l + 1]), ks);
+    const MGView &C = M.lv[l + 1].v;
+    const Shape3 sh = shape3(C.nz, C.ny, C.nx);
+    mg_restrict_kernel<T><<<sh.grid, sh.block, 0, c.stream>>>(M.lv[l].v, C, lb<T>(M.lv[l]), X(l),
+                                                               lb<T>(M.lv[l + 1]), ks);
   }
   SmallArgs<T> S;
   std::memset(&S, 0, sizeof(S));
   S.nlev = nl - ls;
   S.nu = M.nu;
   S.coarse_sweeps = MG_COARSE_SWEEPS;
+  S.dense = M.dense_n > 0 ? 1 : 0;
+  S.cinv = std::is_same<T, double>::value ? reinterpret_cast<const T *>(M.cinv64) : reinterpret_cast<const T *>(M.cinv32);
   for (int l = ls; l < nl; ++l) {
     S.v[l - ls] = M.lv[l].v;
     S.x[l - ls] = X(l);
     S.b[l - ls] = lb<T>(M.lv[l]);
   }
-  mg_small_kernel<T><<<1, 1024, 0, c.stream>>>(S, ks);
+  mg_small_kernel<T><<<1, MG_SMALL_THREADS, 0, c.stream>>>(S, ks);
   for (int l = ls - 1; l >= 0; --l) {
-    mg_prolong_kernel<T><<<nblk(M.lv[l].v.n), 256, 0, c.stream>>>(M.lv[l].v, M.lv[l + 1].v, X(l + 1), X(l), ks);
+    const MGView &F = M.lv[l].v;
+    const Shape3 sh = shape3(F.nz, F.ny, F.nx);
+    mg_prolong_kernel<T><<<sh.grid, sh.block, 0, c.stream>>>(F, M.lv[l + 1].v, X(l + 1), X(l), ks);
     rb(l, false);
   }
   if (!same) mg_widen_kernel<T, V><<<nblk(c.g.N), 256, 0, c.stream>>>(c.g.N, x0, out, ks);
@@ -333,6 +384,8 @@ void mg_release(Ctx &c) {
     mg_free(L.x32);
     mg_free(L.b32);
   }
+  mg_free(c.mg->cinv64);
+  mg_free(c.mg->cinv32);
   delete c.mg;
   c.mg = nullptr;
   mg_free(c.ph64);
@@ -359,6 +412,7 @@ int mg_ensure(Ctx &c, bool fp32) {
     }
     const bool dl[3] = {c.prm.p_dirichlet[0] != 0, c.prm.p_dirichlet[2] != 0, c.prm.p_dirichlet[4] != 0};
     const bool dh[3] = {c.prm.p_dirichlet[1] != 0, c.prm.p_dirichlet[3] != 0, c.prm.p_dirichlet[5] != 0};
+    std::vector<double> last_arr[3];
     for (;;) {
       MGLevel L;
       L.v.nx = (int)w[0].size();
@@ -369,6 +423,7 @@ int mg_ensure(Ctx &c, bool fp32) {
       for (int a = 0; a < 3; ++a) {
         axis_arrays(w[a], dlo[a], dhi[a], dl[a], dh[a], arr[a]);
         all.insert(all.end(), arr[a].begin(), arr[a].end());
+        last_arr[a] = arr[a];
       }
       int rc = mg_alloc(&L.coef, all.size());
       if (rc) return rc;
@@ -402,6 +457,60 @@ int mg_ensure(Ctx &c, bool fp32) {
     }
     M->small_from = (int)M->lv.size() - 1;
     while (M->small_from > 0 && M->lv[M->small_from - 1].v.n <= MG_SMALL) M->small_from--;
+    // exact coarsest solve: the dense inverse of the coarsest operator when a
+    // Dirichlet face makes it nonsingular (all-Neumann keeps the sweeps)
+    const MGView &Lc = M->lv.back().v;
+    const bool any_dir = dl[0] || dl[1] || dl[2] || dh[0] || dh[1] || dh[2];
+    if (any_dir && Lc.n <= 512) {
+      const int n = (int)Lc.n, nx = Lc.nx, ny = Lc.ny, nz = Lc.nz;
+      std::vector<double> A((size_t)n * n, 0.0), Inv((size_t)n * n, 0.0);
+      const std::vector<double> &ax = last_arr[0], &ay = last_arr[1], &az = last_arr[2];
+      for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+          for (int k = 0; k < nz; ++k) {
+            const int r = (i * ny + j) * nz + k;
+            A[(size_t)r * n + r] = ax[2 * nx + i] + ay[2 * ny + j] + az[2 * nz + k];
+            if (i > 0) A[(size_t)r * n + r - ny * nz] = -ax[i];
+            if (i < nx - 1) A[(size_t)r * n + r + ny * nz] = -ax[nx + i];
+            if (j > 0) A[(size_t)r * n + r - nz] = -ay[j];
+            if (j < ny - 1) A[(size_t)r * n + r + nz] = -ay[ny + j];
+            if (k > 0) A[(size_t)r * n + r - 1] = -az[k];
+            if (k < nz - 1) A[(size_t)r * n + r + 1] = -az[nz + k];
+          }
+      for (int r = 0; r < n; ++r) Inv[(size_t)r * n + r] = 1.0;
+      bool ok = true;
+      for (int col = 0; col < n && ok; ++col) {  // Gauss-Jordan with partial pivoting
+        int piv = col;
+        for (int r = col + 1; r < n; ++r)
+          if (std::fabs(A[(size_t)r * n + col]) > std::fabs(A[(size_t)piv * n + col])) piv = r;
+        if (!(std::fabs(A[(size_t)piv * n + col]) > 0.0)) { ok = false; break; }
+        if (piv != col)
+          for (int q = 0; q < n; ++q) {
+            std::swap(A[(size_t)piv * n + q], A[(size_t)col * n + q]);
+            std::swap(Inv[(size_t)piv * n + q], Inv[(size_t)col * n + q]);
+          }
+        const double d = A[(size_t)col * n + col];
+        for (int q = 0; q < n; ++q) { A[(size_t)col * n + q] /= d; Inv[(size_t)col * n + q] /= d; }
+        for (int r = 0; r < n; ++r) {
+          if (r == col) continue;
+          const double f = A[(size_t)r * n + col];
+          if (f == 0.0) continue;
+          for (int q = 0; q < n; ++q) {
+            A[(size_t)r * n + q] -= f * A[(size_t)col * n + q];
+            Inv[(size_t)r * n + q] -= f * Inv[(size_t)col * n + q];
+          }
+        }
+      }
+      if (ok) {
+        std::vector<float> inv32(Inv.begin(), Inv.end());
+        int rc = mg_alloc(&M->cinv64, Inv.size());
+        if (!rc) rc = mg_alloc(&M->cinv32, inv32.size());
+        if (rc) return rc;
+        CFD_HIP(hipMemcpy(M->cinv64, Inv.data(), Inv.size() * sizeof(double), hipMemcpyHostToDevice));
+        CFD_HIP(hipMemcpy(M->cinv32, inv32.data(), inv32.size() * sizeof(float), hipMemcpyHostToDevice));
+        M->dense_n = n;
+      }
+    }
     M->nu = c.prm.mg_nu > 0 ? c.prm.mg_nu : 2;
     for (auto &L : M->lv) M->shape += std::to_string(L.v.nx) + "x" + std::to_string(L.v.ny) + "x" +
                                       std::to_string(L.v.nz) + " ";
