@@ -28,6 +28,7 @@ CFD_SOLVER_BICGSTAB = 0
 CFD_SOLVER_RBSOR = 1
 CFD_OP_ELL = 0
 CFD_OP_SPE = 1
+CFD_OP_CSR = 2
 CFD_FP64 = 0
 CFD_MIXED = 1
 CFD_MIXED_RELIABLE = 0
@@ -223,6 +224,9 @@ def _declare(L: C.CDLL) -> None:
         "cfd_set_eta_cylinder_z": ([vp], C.c_int),
         "cfd_ell_assemble": ([vp], C.c_int),
         "cfd_spe_assemble": ([vp], C.c_int),
+        "cfd_csr_assemble": ([vp], C.c_int),
+        "cfd_csr_nnz": ([vp, C.POINTER(C.c_size_t)], C.c_int),
+        "cfd_csr_download": ([vp, i32p, i32p, dp, C.c_size_t, C.c_size_t], C.c_int),
         "cfd_ell_download": ([vp, i32p, dp, C.c_size_t, C.c_int], C.c_int),
         "cfd_ell_upload": ([vp, i32p, dp, C.c_size_t, C.c_int], C.c_int),
         "cfd_spmv": ([vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),

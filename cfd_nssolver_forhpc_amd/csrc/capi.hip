@@ -58,7 +58,7 @@ int check_params(const cfd_params *p) {
   if (p->inner_loops < 0 || p->inner_loops > CFD_MAX_PASSES - 1)
     return fail(CFD_ERR_ARG, "inner_loops must be in [0, 7]");
   if (p->solver != CFD_SOLVER_BICGSTAB && p->solver != CFD_SOLVER_RBSOR) return fail(CFD_ERR_ARG, "unknown solver");
-  if (p->op != CFD_OP_ELL && p->op != CFD_OP_SPE) return fail(CFD_ERR_ARG, "unknown operator");
+  if (p->op != CFD_OP_ELL && p->op != CFD_OP_SPE && p->op != CFD_OP_CSR) return fail(CFD_ERR_ARG, "unknown operator");
   if (p->precision != CFD_FP64 && p->precision != CFD_MIXED) return fail(CFD_ERR_ARG, "unknown precision");
   if (p->solver == CFD_SOLVER_RBSOR && p->op != CFD_OP_ELL)
     return fail(CFD_ERR_ARG, "red-black SOR runs on the ELL operator");
@@ -95,6 +95,7 @@ void free_ctx(Ctx *c) {
   for (auto &p : c->f) dfree(p);
   dfree(c->col); dfree(c->val); dfree(c->val32); dfree(c->colP); dfree(c->valP); dfree(c->valP32); dfree(c->perm);
   dfree(c->spe_coef); dfree(c->spe_jp);
+  dfree(c->csr_ptr); dfree(c->csr_col); dfree(c->csr_val); dfree(c->csr_jp);
   dfree(c->r0); dfree(c->r1); dfree(c->p1); dfree(c->s1); dfree(c->ap); dfree(c->as); dfree(c->px1); dfree(c->px2);
   for (auto &p : c->q32) dfree(p);
   dfree(c->xP); dfree(c->bP); dfree(c->rP); dfree(c->xP32); dfree(c->bP32);
@@ -555,6 +556,7 @@ int cfd_ell_assemble(cfd_ctx *ctx) {
   if (rc) return rc;
   if (c.prm.op == CFD_OP_SPE && c.spe_ready)  // gA.jp follows the configured operator
     RET_LAUNCH(launch_copy(c, c.spe_jp, c.f[CFD_JP], c.g.N));
+  if (c.prm.op == CFD_OP_CSR && c.csr_ready) RET_LAUNCH(launch_copy(c, c.csr_jp, c.f[CFD_JP], c.g.N));
   CFD_HIP(hipStreamSynchronize(c.stream));
   return CFD_OK;
 }
@@ -582,6 +584,60 @@ int cfd_spe_assemble(cfd_ctx *ctx) {
   if (c.prm.op == CFD_OP_SPE) RET_LAUNCH(launch_copy(c, c.spe_jp, c.f[CFD_JP], g.N));
   CFD_HIP(hipStreamSynchronize(c.stream));
   c.spe_ready = true;
+  return CFD_OK;
+}
+
+int cfd_csr_assemble(cfd_ctx *ctx) {
+  CTX_CHECK(ctx);
+  Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  const DevGrid &g = c.g;
+  if (g.N * 7 > (int64_t)INT32_MAX) return fail(CFD_ERR_ARG, "CSR operator: nnz exceeds 32-bit row pointers");
+  // row pointers: one entry per row plus one per neighbour that exists
+  std::vector<int32_t> ptr((size_t)g.N + 1);
+  int64_t e = 0, r = 0;
+  ptr[0] = 0;
+  for (int i = 0; i < g.nxc; ++i)
+    for (int j = 0; j < g.nyc; ++j)
+      for (int k = 0; k < g.nzc; ++k) {
+        e += 1 + (i != 0) + (j != 0) + (k != 0) + (k != g.nzc - 1) + (j != g.nyc - 1) + (i != g.nxc - 1);
+        ptr[(size_t)++r] = (int32_t)e;
+      }
+  if (c.csr_nnz != e) {
+    dfree(c.csr_col); dfree(c.csr_val);
+    c.csr_nnz = 0;
+  }
+  int rc = CFD_OK;
+  if (!c.csr_ptr && (rc = dmalloc(&c.csr_ptr, (size_t)g.N + 1))) return rc;
+  if (!c.csr_jp && (rc = dmalloc(&c.csr_jp, (size_t)g.N))) return rc;
+  if (!c.csr_col && ((rc = dmalloc(&c.csr_col, (size_t)e)) || (rc = dmalloc(&c.csr_val, (size_t)e)))) return rc;
+  c.csr_nnz = e;
+  CFD_HIP(hipMemcpyAsync(c.csr_ptr, ptr.data(), sizeof(int32_t) * ptr.size(), hipMemcpyHostToDevice, c.stream));
+  RET_LAUNCH(launch_csr_assemble(c));
+  if (c.prm.op == CFD_OP_CSR) RET_LAUNCH(launch_copy(c, c.csr_jp, c.f[CFD_JP], g.N));
+  CFD_HIP(hipStreamSynchronize(c.stream));  // ptr's host copy must outlive the upload
+  c.csr_ready = true;
+  return CFD_OK;
+}
+
+int cfd_csr_nnz(cfd_ctx *ctx, size_t *nnz) {
+  CTX_CHECK(ctx);
+  Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  if (!nnz) return fail(CFD_ERR_ARG, "nnz is NULL");
+  if (!c.csr_ready) return fail(CFD_ERR_STATE, "CSR operator not assembled");
+  *nnz = (size_t)c.csr_nnz;
+  return CFD_OK;
+}
+
+int cfd_csr_download(cfd_ctx *ctx, int32_t *ptr, int32_t *col, double *val, size_t n_rows, size_t nnz) {
+  CTX_CHECK(ctx);
+  Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  if (!c.csr_ready) return fail(CFD_ERR_STATE, "CSR operator not assembled");
+  if (!ptr || !col || !val || n_rows != (size_t)c.g.N || nnz != (size_t)c.csr_nnz)
+    return fail(CFD_ERR_ARG, "CSR shape mismatch");
+  CFD_HIP(hipMemcpyAsync(ptr, c.csr_ptr, sizeof(int32_t) * (n_rows + 1), hipMemcpyDeviceToHost, c.stream));
+  CFD_HIP(hipMemcpyAsync(col, c.csr_col, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, c.stream));
+  CFD_HIP(hipMemcpyAsync(val, c.csr_val, sizeof(double) * nnz, hipMemcpyDeviceToHost, c.stream));
+  CFD_HIP(hipStreamSynchronize(c.stream));
   return CFD_OK;
 }
 
@@ -619,9 +675,8 @@ int cfd_spmv(cfd_ctx *ctx, int op, int precision, int x_field, int y_field) {
   auto is_vec = [&](int f) { return std::find(std::begin(vec_ok), std::end(vec_ok), f) != std::end(vec_ok); };
   if (!is_vec(x_field) || !is_vec(y_field) || x_field == y_field)
     return fail(CFD_ERR_ARG, "x and y must be distinct N-sized vector fields");
-  if (op == CFD_OP_ELL && !c.ell_ready) return fail(CFD_ERR_STATE, "ELL operator not assembled");
-  if (op == CFD_OP_SPE && !c.spe_ready) return fail(CFD_ERR_STATE, "SPE operator not assembled");
-  if (op != CFD_OP_ELL && op != CFD_OP_SPE) return fail(CFD_ERR_ARG, "unknown operator");
+  if (op != CFD_OP_ELL && op != CFD_OP_SPE && op != CFD_OP_CSR) return fail(CFD_ERR_ARG, "unknown operator");
+  if (int rc = op_ready(c, op)) return rc;
   if (precision == CFD_MIXED) {
     if (op != CFD_OP_ELL) return fail(CFD_ERR_ARG, "FP32 SpMV is defined for the ELL operator");
     int rc = ensure_mixed(c);
@@ -797,12 +852,18 @@ int cfd_step(cfd_ctx *ctx, cfd_step_stats *stats) {
 
 namespace cfd {
 
+int op_ready(const Ctx &c, int op) {
+  if (op == CFD_OP_ELL && !c.ell_ready) return fail(CFD_ERR_STATE, "ELL operator not assembled");
+  if (op == CFD_OP_SPE && !c.spe_ready) return fail(CFD_ERR_STATE, "SPE operator not assembled");
+  if (op == CFD_OP_CSR && !c.csr_ready) return fail(CFD_ERR_STATE, "CSR operator not assembled");
+  return CFD_OK;
+}
+
 int step_impl(Ctx &c, cfd_step_stats *stats) {
   const cfd_params &P = c.prm;
   const bool solve = c.loop > 1;
   if (solve) {
-    if (P.op == CFD_OP_ELL && !c.ell_ready) return fail(CFD_ERR_STATE, "ELL operator not assembled");
-    if (P.op == CFD_OP_SPE && !c.spe_ready) return fail(CFD_ERR_STATE, "SPE operator not assembled");
+    if (int rc = op_ready(c, P.op)) return rc;
   }
   cfd_step_stats st;
   std::memset(&st, 0, sizeof(st));
@@ -896,8 +957,9 @@ int cfd_time_kernel(cfd_ctx *ctx, int what, int nrep, double *ms_avg) {
     return fail(CFD_ERR_STATE, "ELL operator not assembled");
   if (what == 1 && (rc = ensure_mixed(c))) return rc;
   if (what == 2 && !c.spe_ready) return fail(CFD_ERR_STATE, "SPE operator not assembled");
+  if (what == 7 && !c.csr_ready) return fail(CFD_ERR_STATE, "CSR operator not assembled");
   if (what == 3 && (rc = ensure_perm(c))) return rc;
-  if (what < 0 || what > 6) return fail(CFD_ERR_ARG, "unknown kernel id");
+  if (what < 0 || what > 7) return fail(CFD_ERR_ARG, "unknown kernel id");
   auto body = [&]() -> hipError_t {
     switch (what) {
       case 0: return launch_spmv(c, CFD_OP_ELL, CFD_FP64, c.f[CFD_X], c.f[CFD_VEC_A]);
@@ -905,6 +967,7 @@ int cfd_time_kernel(cfd_ctx *ctx, int what, int nrep, double *ms_avg) {
       case 2: return launch_spmv(c, CFD_OP_SPE, CFD_FP64, c.f[CFD_X], c.f[CFD_VEC_A]);
       case 3: return launch_rbsor_sweep(c, c.prm.sor_omega);
       case 4: return launch_bicg_iteration(c);
+      case 7: return launch_spmv(c, CFD_OP_CSR, CFD_FP64, c.f[CFD_X], c.f[CFD_VEC_A]);
       case 5:
         return launch_predictor(c, c.f[CFD_T0_U], c.f[CFD_T0_V], c.f[CFD_T0_W], c.f[CFD_T1_U], c.f[CFD_T1_V],
                                 c.f[CFD_T1_W], nullptr, 0);

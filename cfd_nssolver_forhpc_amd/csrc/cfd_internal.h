@@ -80,6 +80,12 @@ struct Ctx {
   // SPE coefficients (xm, xp over nxc; ym, yp over nyc; zm, zp over nzc) and jp
   double *spe_coef = nullptr;
   double *spe_jp = nullptr;
+  // CSR operator (createPressureMatrixCSR): row pointers (N+1), columns,
+  // values and its own Jacobi scaling
+  int32_t *csr_ptr = nullptr, *csr_col = nullptr;
+  double *csr_val = nullptr, *csr_jp = nullptr;
+  int64_t csr_nnz = 0;
+  bool csr_ready = false;
   // solver vectors (FP64)
   double *r0 = nullptr, *r1 = nullptr, *p1 = nullptr, *s1 = nullptr, *ap = nullptr, *as = nullptr;
   double *px1 = nullptr, *px2 = nullptr;
@@ -185,6 +191,16 @@ hipError_t launch_fill(Ctx &c, double *dst, double v, int64_t n);
 hipError_t launch_ell_assemble(Ctx &c);
 hipError_t launch_ell_permute(Ctx &c);
 hipError_t launch_spe_assemble(Ctx &c);
+hipError_t launch_csr_assemble(Ctx &c);
+// CFD_OK when operator `op` is assembled, else the CFD_ERR_STATE failure
+int op_ready(const Ctx &c, int op);
+// the FP64 operator of the configured path: prm.op when it is assembled,
+// else the ELL operator
+inline int fp64_op(const Ctx &c) {
+  if (c.prm.op == CFD_OP_SPE && c.spe_ready) return CFD_OP_SPE;
+  if (c.prm.op == CFD_OP_CSR && c.csr_ready) return CFD_OP_CSR;
+  return CFD_OP_ELL;
+}
 hipError_t launch_spmv(Ctx &c, int op, int precision, const double *x, double *y);
 hipError_t launch_spmv_ell_f32(Ctx &c, const float *x, float *y);
 hipError_t launch_cast32(Ctx &c, const double *a, float *b, int64_t n);

@@ -59,23 +59,37 @@ __device__ inline double spe_row(const double *__restrict__ coef, const double *
   return sum;
 }
 
+// CSR_matrix::multiply_omp (inc/matrix/CSR_sparseMatrix.hpp:278-292): the
+// row's entries in stored (column) order, each product added as an FMA like
+// the reference build; one thread per row
+__device__ inline double csr_row(const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+                                 const double *__restrict__ val, int64_t r, const double *__restrict__ x) {
+  double acc = 0.0;
+  const int32_t e1 = ptr[r + 1];
+  for (int32_t e = ptr[r]; e < e1; ++e) acc = fma(val[e], x[col[e]], acc);
+  return acc;
+}
+
 // Operator handle passed by value to the kernels.
 struct OpArgs {
-  int op;                    // CFD_OP_ELL / CFD_OP_SPE
+  int op;                    // CFD_OP_ELL / CFD_OP_SPE / CFD_OP_CSR
   int64_t N;
   const int32_t *col;
   const double *val;
   const float *val32;
   const double *coef, *jp;   // SPE
   int nxc, nyc, nzc;
+  const int32_t *cptr, *ccol;  // CSR
+  const double *cval;
 };
 
 // internal operator id: the ELL operator with FP32 values applied to an FP64
 // vector (FP64 products and sum)
-constexpr int OP_ELL_VAL32 = 2;
+constexpr int OP_ELL_VAL32 = 16;
 
 __device__ inline double op_row(const OpArgs &a, int64_t r, const double *__restrict__ x) {
   if (a.op == CFD_OP_SPE) return spe_row(a.coef, a.jp, a.nxc, a.nyc, a.nzc, r, x);
+  if (a.op == CFD_OP_CSR) return csr_row(a.cptr, a.ccol, a.cval, r, x);
   if (a.op == OP_ELL_VAL32) return ell_row<float, double>(a.col, a.val32, a.N, r, x);
   return ell_row<double, double>(a.col, a.val, a.N, r, x);
 }
@@ -106,6 +120,9 @@ inline OpArgs op_args(const Ctx &c, int op) {
   a.nxc = c.g.nxc;
   a.nyc = c.g.nyc;
   a.nzc = c.g.nzc;
+  a.cptr = c.csr_ptr;
+  a.ccol = c.csr_col;
+  a.cval = c.csr_val;
   return a;
 }
 

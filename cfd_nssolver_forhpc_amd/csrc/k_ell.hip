@@ -87,6 +87,50 @@ __global__ void __launch_bounds__(256) ell_permute_kernel(int64_t N, const int32
   }
 }
 
+// grid::createPressureMatrixCSR (inc/grid/structureGrid.hpp:630-728) one row
+// per thread at ptr[row] (row pointers counted on the host): entries in
+// column order xm, ym, zm, diag, zp, yp, xp, the diagonal summed as
+// xm + ym + zm + xp + yp + zp + aa + bb + cc (the CSR builder's order, not
+// the ELL one), jp = 1/diag.  The builder closes every face with the Neumann
+// term whatever the pressure BC table says (:658-666), and so does this.
+__global__ void __launch_bounds__(256) csr_assemble_kernel(DevGrid g, const int32_t *__restrict__ ptr,
+                                                           int32_t *__restrict__ col, double *__restrict__ val,
+                                                           double *__restrict__ jp) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.N) return;
+  const int64_t nynz = (int64_t)g.nyc * g.nzc;
+  const int i = (int)(r / nynz), j = (int)((r / g.nzc) % g.nyc), k = (int)(r % g.nzc);
+  const int I = i + GC, J = j + GC, K = k + GC;
+  const double xp = 1.0 / g.Dx[I] / g.Dxs[I], yp = 1.0 / g.Dy[J] / g.Dys[J], zp = 1.0 / g.Dz[K] / g.Dzs[K];
+  const double xm = 1.0 / g.Dx[I] / g.Dxs[I - 1], ym = 1.0 / g.Dy[J] / g.Dys[J - 1],
+               zm = 1.0 / g.Dz[K] / g.Dzs[K - 1];
+  double aa = 0.0, bb = 0.0, cc = 0.0;
+  if (i == 0) aa = -xm;
+  if (i == g.nxc - 1) aa = -xp;
+  if (j == 0) bb = -ym;
+  if (j == g.nyc - 1) bb = -yp;
+  if (k == 0) cc = -zm;
+  if (k == g.nzc - 1) cc = -zp;
+  double ijk = xm + ym;
+  ijk = ijk + zm;
+  ijk = ijk + xp;
+  ijk = ijk + yp;
+  ijk = ijk + zp;
+  ijk = ijk + aa;
+  ijk = ijk + bb;
+  ijk = ijk + cc;
+  const double jr = 1.0 / ijk;
+  jp[r] = jr;
+  int32_t e = ptr[r];
+  if (i != 0) { col[e] = (int32_t)(r - nynz); val[e++] = -xm * jr; }
+  if (j != 0) { col[e] = (int32_t)(r - g.nzc); val[e++] = -ym * jr; }
+  if (k != 0) { col[e] = (int32_t)(r - 1); val[e++] = -zm * jr; }
+  col[e] = (int32_t)r; val[e++] = ijk * jr;
+  if (k != g.nzc - 1) { col[e] = (int32_t)(r + 1); val[e++] = -zp * jr; }
+  if (j != g.nyc - 1) { col[e] = (int32_t)(r + g.nzc); val[e++] = -yp * jr; }
+  if (i != g.nxc - 1) { col[e] = (int32_t)(r + nynz); val[e++] = -xp * jr; }
+}
+
 // SPE_matrix1::setupPressure / set_Jp_pc diagonal (SPE_sparseMatrix1.hpp:78-121)
 __global__ void __launch_bounds__(256) spe_jp_kernel(int nxc, int nyc, int nzc, PBC b, const double *__restrict__ coef,
                                                      double *__restrict__ jp) {
@@ -167,6 +211,12 @@ hipError_t launch_ell_permute(Ctx &c) {
 
 hipError_t launch_cast32(Ctx &c, const double *a, float *b, int64_t n) {
   cast32_kernel<<<blocks_for(n, 256, 1 << 30), 256, 0, c.stream>>>(a, b, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_csr_assemble(Ctx &c) {
+  csr_assemble_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g, c.csr_ptr, c.csr_col,
+                                                                             c.csr_val, c.csr_jp);
   return hipGetLastError();
 }
 

@@ -485,8 +485,7 @@ int pbicgstab_solve64(Ctx &c, const double *b, double *x, double tol, int maxit,
 }
 
 int bicgstab_solve(Ctx &c, const double *b, double *x, double tol, int maxit, int *iters, double *relres) {
-  if (c.prm.op == CFD_OP_ELL && !c.ell_ready) return fail(CFD_ERR_STATE, "ELL operator not assembled");
-  if (c.prm.op == CFD_OP_SPE && !c.spe_ready) return fail(CFD_ERR_STATE, "SPE operator not assembled");
+  if (int rc = op_ready(c, c.prm.op)) return rc;
   c.solves++;
   Vecs<double> v{x, c.r0, c.r1, c.p1, c.s1, c.ap, c.as, b, c.px1, c.px2};
   return solve_bicg<double>(c, op_args(c, c.prm.op), v, c.ks, c.solves, tol, maxit, iters, relres);
@@ -507,7 +506,7 @@ int defect_correction(Ctx &c, const double *b, double *x, double tol, double tol
   if (!c.ell_ready || !c.val32) return fail(CFD_ERR_STATE, "FP32 ELL operator not assembled");
   const int64_t N = c.g.N;
   const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
-  const OpArgs a64 = op_args(c, c.prm.op == CFD_OP_SPE && c.spe_ready ? CFD_OP_SPE : CFD_OP_ELL);
+  const OpArgs a64 = op_args(c, fp64_op(c));
   const OpArgs a32 = op_args(c, CFD_OP_ELL);
   float *r32 = c.q32[7], *d32 = c.q32[6];
   Vecs<float> v{d32, c.q32[0], c.q32[1], c.q32[2], c.q32[3], c.q32[4], c.q32[5], r32, nullptr, nullptr};
@@ -810,7 +809,7 @@ int reliable_solve(Ctx &c, const double *b, double *x, double tol, double delta,
   if (!c.ell_ready || !c.val32) return fail(CFD_ERR_STATE, "FP32 ELL operator not assembled");
   const int64_t N = c.g.N;
   const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
-  const OpArgs a64 = op_args(c, c.prm.op == CFD_OP_SPE && c.spe_ready ? CFD_OP_SPE : CFD_OP_ELL);
+  const OpArgs a64 = op_args(c, fp64_op(c));
   const OpArgs ain = op_args(c, std::is_same<T, float>::value ? CFD_OP_ELL : OP_ELL_VAL32);
   T *r0, *r1, *p1, *s1, *ap, *as;
   if constexpr (std::is_same<T, float>::value) {

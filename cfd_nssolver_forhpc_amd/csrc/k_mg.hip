@@ -122,54 +122,84 @@ __device__ inline double mg_vol(const MGView &L, int i, int j, int k) {
 }
 
 // one colour of a red-black Gauss-Seidel sweep at (i, j, k2), k = 2 k2 +
-// parity; `first`: the sweep starts from x = 0 (this colour gets b/diag,
-// the other colour is cleared)
+// parity (colour 0 = red: i + j + k even)
 template <typename T>
-__device__ inline void rb_at(const MGView &L, int i, int j, int k2, int colour, bool first, const T *__restrict__ b,
+__device__ inline void rb_at(const MGView &L, int i, int j, int k2, int colour, const T *__restrict__ b,
                              T *__restrict__ x) {
   const int k = 2 * k2 + ((i + j + colour) & 1);
-  if (first) {
-    const int ko = 2 * k2 + ((i + j + colour + 1) & 1);
-    if (k < L.nz) {
-      const int64_t c = idx3(L, i, j, k);
-      x[c] = b[c] / mg_diag<T>(L, i, j, k);
-    }
-    if (ko < L.nz) x[idx3(L, i, j, ko)] = (T)0;
-  } else if (k < L.nz) {
-    const int64_t c = idx3(L, i, j, k);
-    x[c] = (b[c] + mg_off<T>(L, x, i, j, k, c)) / mg_diag<T>(L, i, j, k);
-  }
+  if (k >= L.nz) return;
+  const int64_t c = idx3(L, i, j, k);
+  x[c] = (b[c] + mg_off<T>(L, x, i, j, k, c)) / mg_diag<T>(L, i, j, k);
 }
 
-// residual of the children of coarse cell (I, J, K), volume-weighted into bc
+// parent of fine cell (i, j, k) on the next coarser level
+__device__ inline int64_t parent(const MGView &F, const MGView &C, int i, int j, int k) {
+  return idx3(C, F.fx == 2 ? i >> 1 : i, F.fy == 2 ? j >> 1 : j, F.fz == 2 ? k >> 1 : k);
+}
+
+// the first red sweep after the coarse correction, with the prolongation
+// folded in: the black neighbours are read as x + xc[parent].  Red cells are
+// overwritten here and black cells by the black sweep that follows, so the
+// corrected values are only ever needed as neighbours.
+template <typename T>
+__device__ inline void rb_prolong_at(const MGView &F, const MGView &C, int i, int j, int k2, const T *__restrict__ b,
+                                     T *__restrict__ x, const T *__restrict__ xc) {
+  const int k = 2 * k2 + ((i + j) & 1);
+  if (k >= F.nz) return;
+  const int64_t c = idx3(F, i, j, k);
+  const int64_t sx = (int64_t)F.ny * F.nz, sy = F.nz;
+  T s = (T)0;
+  if (i > 0) s += (T)F.ax[i] * (x[c - sx] + xc[parent(F, C, i - 1, j, k)]);
+  if (i < F.nx - 1) s += (T)F.ax[F.nx + i] * (x[c + sx] + xc[parent(F, C, i + 1, j, k)]);
+  if (j > 0) s += (T)F.ay[j] * (x[c - sy] + xc[parent(F, C, i, j - 1, k)]);
+  if (j < F.ny - 1) s += (T)F.ay[F.ny + j] * (x[c + sy] + xc[parent(F, C, i, j + 1, k)]);
+  if (k > 0) s += (T)F.az[k] * (x[c - 1] + xc[parent(F, C, i, j, k - 1)]);
+  if (k < F.nz - 1) s += (T)F.az[F.nz + k] * (x[c + 1] + xc[parent(F, C, i, j, k + 1)]);
+  x[c] = (b[c] + s) / mg_diag<T>(F, i, j, k);
+}
+
+// a new right-hand side at cell c and the first red half-sweep from x = 0
+template <typename T>
+__device__ inline void set_rhs(const MGView &L, int i, int j, int k, T bv, T *__restrict__ b, T *__restrict__ x) {
+  const int64_t c = idx3(L, i, j, k);
+  b[c] = bv;
+  x[c] = ((i + j + k) & 1) == 0 ? bv / mg_diag<T>(L, i, j, k) : (T)0;
+}
+
+// residual of the children of coarse cell (I, J, K), volume-weighted into
+// the coarse right-hand side (which starts its own first red half-sweep)
+template <typename T>
+__device__ inline double child_res(const MGView &F, int i, int j, int k, const T *__restrict__ bf,
+                                   const T *__restrict__ xf) {
+  const int64_t c = idx3(F, i, j, k);
+  const T r = bf[c] - (mg_diag<T>(F, i, j, k) * xf[c] - mg_off<T>(F, xf, i, j, k, c));
+  return mg_vol<T>(F, i, j, k) * (double)r;
+}
+
 template <typename T>
 __device__ inline void restrict_at(const MGView &F, const MGView &C, int I, int J, int K, const T *__restrict__ bf,
-                                   const T *__restrict__ xf, T *__restrict__ bc) {
+                                   const T *__restrict__ xf, T *__restrict__ bc, T *__restrict__ xc) {
   double acc = 0.0;
-  for (int di = 0; di < F.fx; ++di)
-    for (int dj = 0; dj < F.fy; ++dj)
-      for (int dk = 0; dk < F.fz; ++dk) {
-        const int i = I * F.fx + di, j = J * F.fy + dj, k = K * F.fz + dk;
-        const int64_t c = idx3(F, i, j, k);
-        const T r = bf[c] - (mg_diag<T>(F, i, j, k) * xf[c] - mg_off<T>(F, xf, i, j, k, c));
-        acc += mg_vol<T>(F, i, j, k) * (double)r;
-      }
-  bc[idx3(C, I, J, K)] = (T)(acc / mg_vol<T>(C, I, J, K));
-}
-
-template <typename T>
-__device__ inline void prolong_at(const MGView &F, const MGView &C, int i, int j, int k, const T *__restrict__ xc,
-                                  T *__restrict__ xf) {
-  const int I = F.fx == 2 ? i >> 1 : i, J = F.fy == 2 ? j >> 1 : j, K = F.fz == 2 ? k >> 1 : k;
-  xf[idx3(F, i, j, k)] += xc[idx3(C, I, J, K)];
+  if (F.fx == 2 && F.fy == 2 && F.fz == 2) {
+#pragma unroll
+    for (int d = 0; d < 8; ++d) acc += child_res<T>(F, 2 * I + (d >> 2), 2 * J + ((d >> 1) & 1), 2 * K + (d & 1), bf, xf);
+  } else {
+    for (int di = 0; di < F.fx; ++di)
+      for (int dj = 0; dj < F.fy; ++dj)
+        for (int dk = 0; dk < F.fz; ++dk)
+          acc += child_res<T>(F, I * F.fx + di, J * F.fy + dj, K * F.fz + dk, bf, xf);
+  }
+  set_rhs<T>(C, I, J, K, (T)(acc / mg_vol<T>(C, I, J, K)), bc, xc);
 }
 
 template <typename T, typename V>
-__global__ void __launch_bounds__(256) mg_scale_kernel(int64_t n, const V *__restrict__ r, const double *__restrict__ jp,
-                                                       T *__restrict__ b, const KrylovScalars *ks) {
+__global__ void __launch_bounds__(256) mg_scale_kernel(MGView L, const V *__restrict__ r, const double *__restrict__ jp,
+                                                       T *__restrict__ b, T *__restrict__ x, const KrylovScalars *ks) {
   if (ks && ks->done) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) b[t] = (T)((double)r[t] / jp[t]);
+  const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+  if (k >= L.nz || j >= L.ny) return;
+  const int64_t c = idx3(L, i, j, k);
+  set_rhs<T>(L, i, j, k, (T)((double)r[c] / jp[c]), b, x);
 }
 
 template <typename T, typename V>
@@ -182,31 +212,32 @@ __global__ void __launch_bounds__(256) mg_widen_kernel(int64_t n, const T *__res
 
 // 3-D launches (x: k or k2, y: j, z: i): no 64-bit index division per thread
 template <typename T>
-__global__ void __launch_bounds__(256) mg_rb_kernel(MGView L, int colour, int first, const T *__restrict__ b,
-                                                    T *__restrict__ x, const KrylovScalars *ks) {
+__global__ void __launch_bounds__(256) mg_rb_kernel(MGView L, int colour, const T *__restrict__ b, T *__restrict__ x,
+                                                    const KrylovScalars *ks) {
   if (ks && ks->done) return;
   const int k2 = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
   if (k2 >= (L.nz + 1) / 2 || j >= L.ny) return;
-  rb_at<T>(L, i, j, k2, colour, first != 0, b, x);
+  rb_at<T>(L, i, j, k2, colour, b, x);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) mg_rb_prolong_kernel(MGView F, MGView C, const T *__restrict__ b,
+                                                            T *__restrict__ x, const T *__restrict__ xc,
+                                                            const KrylovScalars *ks) {
+  if (ks && ks->done) return;
+  const int k2 = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+  if (k2 >= (F.nz + 1) / 2 || j >= F.ny) return;
+  rb_prolong_at<T>(F, C, i, j, k2, b, x, xc);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) mg_restrict_kernel(MGView F, MGView C, const T *__restrict__ bf,
                                                           const T *__restrict__ xf, T *__restrict__ bc,
-                                                          const KrylovScalars *ks) {
+                                                          T *__restrict__ xc, const KrylovScalars *ks) {
   if (ks && ks->done) return;
   const int K = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y * blockDim.y + threadIdx.y, I = blockIdx.z;
   if (K >= C.nz || J >= C.ny) return;
-  restrict_at<T>(F, C, I, J, K, bf, xf, bc);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) mg_prolong_kernel(MGView F, MGView C, const T *__restrict__ xc,
-                                                         T *__restrict__ xf, const KrylovScalars *ks) {
-  if (ks && ks->done) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
-  if (k >= F.nz || j >= F.ny) return;
-  prolong_at<T>(F, C, i, j, k, xc, xf);
+  restrict_at<T>(F, C, I, J, K, bf, xf, bc, xc);
 }
 
 constexpr int MG_SMALL_THREADS = 256;
@@ -222,33 +253,41 @@ struct SmallArgs {
 
 // the bottom of the V-cycle (levels of at most MG_SMALL cells) in one
 // workgroup; __syncthreads separates the phases, whose data stays in this
-// workgroup (L2-resident); the coarsest level is solved by its dense
-// inverse when it is nonsingular, else smoothed coarse_sweeps times
+// workgroup (L2-resident).  The first level's right-hand side and first red
+// half-sweep were written by the launch before; the coarsest level is solved
+// by its dense inverse when it is nonsingular, else smoothed coarse_sweeps
+// times.
 template <typename T>
 __global__ void __launch_bounds__(MG_SMALL_THREADS) mg_small_kernel(SmallArgs<T> S, const KrylovScalars *ks) {
   if (ks && ks->done) return;
   const int nt = MG_SMALL_THREADS;
-  auto sweep = [&](int l, int nsw, bool from_zero) {
+  // nsw sweeps of level l; skip_red0: the first red half-sweep is done;
+  // prolong_from >= 0: the first red half-sweep folds in that level's correction
+  auto sweep = [&](int l, int nsw, bool skip_red0, int prolong_from) {
     const MGView &L = S.v[l];
     const int half = (L.nz + 1) / 2;
     const int cnt = L.nx * L.ny * half;
     for (int s = 0; s < nsw; ++s)
       for (int colour = 0; colour < 2; ++colour) {
-        const bool first = from_zero && s == 0 && colour == 0;
+        if (s == 0 && colour == 0 && skip_red0) continue;
+        const bool fold = s == 0 && colour == 0 && prolong_from >= 0;
         for (int t = threadIdx.x; t < cnt; t += nt) {
           const int k2 = t % half, r = t / half;
-          rb_at<T>(L, r / L.ny, r % L.ny, k2, colour, first, S.b[l], S.x[l]);
+          if (fold)
+            rb_prolong_at<T>(L, S.v[prolong_from], r / L.ny, r % L.ny, k2, S.b[l], S.x[l], S.x[prolong_from]);
+          else
+            rb_at<T>(L, r / L.ny, r % L.ny, k2, colour, S.b[l], S.x[l]);
         }
         __syncthreads();
       }
   };
   const int last = S.nlev - 1;
   for (int l = 0; l < last; ++l) {
-    sweep(l, S.nu, true);
+    sweep(l, S.nu, true, -1);
     const MGView &C = S.v[l + 1];
     for (int t = threadIdx.x; t < (int)C.n; t += nt) {
       const int K = t % C.nz, r = t / C.nz;
-      restrict_at<T>(S.v[l], C, r / C.ny, r % C.ny, K, S.b[l], S.x[l], S.b[l + 1]);
+      restrict_at<T>(S.v[l], C, r / C.ny, r % C.ny, K, S.b[l], S.x[l], S.b[l + 1], S.x[l + 1]);
     }
     __syncthreads();
   }
@@ -263,17 +302,9 @@ __global__ void __launch_bounds__(MG_SMALL_THREADS) mg_small_kernel(SmallArgs<T>
     }
     __syncthreads();
   } else {
-    sweep(last, S.coarse_sweeps, true);
+    sweep(last, S.coarse_sweeps, true, -1);
   }
-  for (int l = last - 1; l >= 0; --l) {
-    const MGView &F = S.v[l];
-    for (int t = threadIdx.x; t < (int)F.n; t += nt) {
-      const int k = t % F.nz, r = t / F.nz;
-      prolong_at<T>(F, S.v[l + 1], r / F.ny, r % F.ny, k, S.x[l + 1], S.x[l]);
-    }
-    __syncthreads();
-    sweep(l, S.nu, false);
-  }
+  for (int l = last - 1; l >= 0; --l) sweep(l, S.nu, false, l + 1);
 }
 
 // launch shape of a 3-D kernel over (nk, ny, nx) with 256-thread blocks
@@ -315,23 +346,36 @@ int mg_apply(Ctx &c, const V *r, V *out, const KrylovScalars *ks) {
   constexpr bool same = std::is_same<T, V>::value;
   T *x0 = same ? reinterpret_cast<T *>(out) : lx<T>(M.lv[0]);
   if (!b0 || !x0) return fail(CFD_ERR_STATE, "multigrid buffers of this precision are not allocated");
-  mg_scale_kernel<T, V><<<nblk(c.g.N), 256, 0, c.stream>>>(c.g.N, r, c.f[CFD_JP], b0, ks);
   auto X = [&](int l) { return l == 0 ? x0 : lx<T>(M.lv[l]); };
-  auto rb = [&](int l, bool from_zero) {
+  {
+    const MGView &v = M.lv[0].v;
+    const Shape3 sh = shape3(v.nz, v.ny, v.nx);
+    mg_scale_kernel<T, V><<<sh.grid, sh.block, 0, c.stream>>>(v, r, c.f[CFD_JP], b0, x0, ks);
+  }
+  // nu sweeps of level l: pre-smoothing skips the red half-sweep its
+  // right-hand side's producer did; post-smoothing folds the prolongation
+  // into its first red half-sweep
+  auto smooth = [&](int l, bool pre) {
     const MGView &v = M.lv[l].v;
     const Shape3 sh = shape3((v.nz + 1) / 2, v.ny, v.nx);
     for (int s = 0; s < M.nu; ++s)
-      for (int colour = 0; colour < 2; ++colour)
-        mg_rb_kernel<T><<<sh.grid, sh.block, 0, c.stream>>>(v, colour, from_zero && s == 0 && colour == 0 ? 1 : 0,
-                                                            lb<T>(M.lv[l]), X(l), ks);
+      for (int colour = 0; colour < 2; ++colour) {
+        if (s == 0 && colour == 0) {
+          if (pre) continue;
+          mg_rb_prolong_kernel<T><<<sh.grid, sh.block, 0, c.stream>>>(v, M.lv[l + 1].v, lb<T>(M.lv[l]), X(l),
+                                                                      X(l + 1), ks);
+          continue;
+        }
+        mg_rb_kernel<T><<<sh.grid, sh.block, 0, c.stream>>>(v, colour, lb<T>(M.lv[l]), X(l), ks);
+      }
   };
   const int ls = M.small_from;
   for (int l = 0; l < ls; ++l) {
-    rb(l, true);
+    smooth(l, true);
     const MGView &C = M.lv[l + 1].v;
     const Shape3 sh = shape3(C.nz, C.ny, C.nx);
     mg_restrict_kernel<T><<<sh.grid, sh.block, 0, c.stream>>>(M.lv[l].v, C, lb<T>(M.lv[l]), X(l),
-                                                               lb<T>(M.lv[l + 1]), ks);
+                                                               lb<T>(M.lv[l + 1]), X(l + 1), ks);
   }
   SmallArgs<T> S;
   std::memset(&S, 0, sizeof(S));
@@ -346,12 +390,7 @@ int mg_apply(Ctx &c, const V *r, V *out, const KrylovScalars *ks) {
     S.b[l - ls] = lb<T>(M.lv[l]);
   }
   mg_small_kernel<T><<<1, MG_SMALL_THREADS, 0, c.stream>>>(S, ks);
-  for (int l = ls - 1; l >= 0; --l) {
-    const MGView &F = M.lv[l].v;
-    const Shape3 sh = shape3(F.nz, F.ny, F.nx);
-    mg_prolong_kernel<T><<<sh.grid, sh.block, 0, c.stream>>>(F, M.lv[l + 1].v, X(l + 1), X(l), ks);
-    rb(l, false);
-  }
+  for (int l = ls - 1; l >= 0; --l) smooth(l, false);
   if (!same) mg_widen_kernel<T, V><<<nblk(c.g.N), 256, 0, c.stream>>>(c.g.N, x0, out, ks);
   CFD_HIP(hipGetLastError());
   return CFD_OK;

@@ -140,6 +140,18 @@ class Simulation:
               "cfd_ell_download")
         return col, val
 
+    def csr(self):
+        """(ptr, col, val) of the assembled CSR operator (createPressureMatrixCSR)."""
+        nnz = C.c_size_t()
+        check(self.lib.cfd_csr_nnz(self.h, C.byref(nnz)), "cfd_csr_nnz")
+        ptr = np.empty(self.N + 1, dtype=np.int32)
+        col = np.empty(nnz.value, dtype=np.int32)
+        val = np.empty(nnz.value, dtype=np.float64)
+        i32 = C.POINTER(C.c_int32)
+        check(self.lib.cfd_csr_download(self.h, ptr.ctypes.data_as(i32), col.ctypes.data_as(i32), dptr(val),
+                                        self.N, nnz.value), "cfd_csr_download")
+        return ptr, col, val
+
     def set_ell(self, col: np.ndarray, val: np.ndarray):
         col = np.ascontiguousarray(col, dtype=np.int32)
         val = np.ascontiguousarray(val, dtype=np.float64)
@@ -161,12 +173,18 @@ class Simulation:
         """SPE_matrix1::setupPressure + set_Jp_pc (matrix-free operator)."""
         check(self.lib.cfd_spe_assemble(self.h), "cfd_spe_assemble")
 
+    def createPressureMatrixCSR(self):
+        """grid::createPressureMatrixCSR on the device; jp follows it when op is CSR."""
+        check(self.lib.cfd_csr_assemble(self.h), "cfd_csr_assemble")
+
     def setup(self, noise: float = 0.0, seed: int = 1):
         """Everything the reference does before its main loop."""
         self.init_state(noise, seed)
         self.createPressureMatrix()
         if self.params.op == capi.CFD_OP_SPE:
             self.setupPressure()
+        elif self.params.op == capi.CFD_OP_CSR:
+            self.createPressureMatrixCSR()
         self.DFIB_CylinderZ()
 
     # ---- operators and solves ---------------------------------------------------

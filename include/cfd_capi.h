@@ -59,6 +59,7 @@ extern "C" {
 /* pressure operators */
 #define CFD_OP_ELL 0 /* mat::ELL_matrix (P_SOLVER_BICG_ELL) */
 #define CFD_OP_SPE 1 /* mat::SPE_matrix1, matrix-free (P_SOLVER_BICG_SPE, the reference default) */
+#define CFD_OP_CSR 2 /* mat::CSR_matrix from grid::createPressureMatrixCSR (P_SOLVER_BICG_CSR) */
 
 /* arithmetic of the pressure solve */
 #define CFD_FP64 0
@@ -194,6 +195,16 @@ int cfd_set_eta_cylinder_z(cfd_ctx *ctx);
 int cfd_ell_assemble(cfd_ctx *ctx);
 /* SPE_matrix1::setupPressure + set_Jp_pc (inc/matrix/SPE_sparseMatrix1.hpp:40-121) */
 int cfd_spe_assemble(cfd_ctx *ctx);
+/* grid::createPressureMatrixCSR (inc/grid/structureGrid.hpp:630-728) into
+   mat::CSR_matrix::set (inc/matrix/CSR_sparseMatrix.hpp:42-120): row
+   pointers, columns and values of the Jacobi-scaled operator with the
+   builder's own diagonal summation order and Neumann closure on every face;
+   its jp becomes gA.jp when op == CFD_OP_CSR (RUN_CPU_PC_METHOD.hpp:153-160) */
+int cfd_csr_assemble(cfd_ctx *ctx);
+/* copies of the CSR operator: ptr n_rows + 1 entries, col/val nnz entries
+   (nnz = ptr[n_rows]; cfd_csr_nnz reports it) */
+int cfd_csr_nnz(cfd_ctx *ctx, size_t *nnz);
+int cfd_csr_download(cfd_ctx *ctx, int32_t *ptr, int32_t *col, double *val, size_t n_rows, size_t nnz);
 int cfd_ell_download(cfd_ctx *ctx, int32_t *col, double *val, size_t n_rows, int k);
 int cfd_ell_upload(cfd_ctx *ctx, const int32_t *col, const double *val, size_t n_rows, int k);
 
@@ -201,7 +212,8 @@ int cfd_ell_upload(cfd_ctx *ctx, const int32_t *col, const double *val, size_t n
 /* y = A x on device vectors named by field ids (N-sized), op = CFD_OP_*,
    precision = CFD_FP64 or CFD_MIXED (FP32 values and vectors, result widened)
    (ELL_matrix::multiply_omp, ELL_sparseMatrix.hpp:477-492;
-    SPE_matrix1::spmv_ompJP, SPE_sparseMatrix1.hpp:126-144) */
+    SPE_matrix1::spmv_ompJP, SPE_sparseMatrix1.hpp:126-144;
+    CSR_matrix::multiply_omp, CSR_sparseMatrix.hpp:278-292; FP64 only) */
 int cfd_spmv(cfd_ctx *ctx, int op, int precision, int x_field, int y_field);
 
 /* ---- pressure solves (matB -> X_result, warm start from X_result) ------ */
@@ -279,7 +291,8 @@ int cfd_adjoint_backward(cfd_ctx *ctx, double *grad_uin, size_t n, double *grad_
 /* average device time (ms) of one launch of the kernel named by `what`, over
    nrep launches bracketed by HIP events on the context stream:
    what = 0 ELL SpMV FP64, 1 ELL SpMV FP32, 2 SPE SpMV FP64, 3 RB-SOR sweep,
-   4 BiCGSTAB iteration (FP64), 5 predictor, 6 copy kernel (2*N doubles moved) */
+   4 BiCGSTAB iteration (FP64), 5 predictor, 6 copy kernel (2*N doubles moved),
+   7 CSR SpMV FP64 */
 int cfd_time_kernel(cfd_ctx *ctx, int what, int nrep, double *ms_avg);
 
 #ifdef __cplusplus

@@ -7,7 +7,8 @@
 // DfibArray, MxClass, calDomain, ConvectionDifussion, createBMatrix_omp,
 // createPressureMatrix, update_UandF_omp, Pressure_transform_X_result,
 // BC_staggered_main, BC_staggered_copy, DFIB_CylinderZ, virtualF_Int,
-// CheckSteadyStatebyMaxVal_omp, solver::bicgstabRe2.  Fields stay in
+// CheckSteadyStatebyMaxVal_omp, grid::createPressureMatrixCSR, mat::ELL_matrix,
+// mat::CSR_matrix, solver::bicgstabRe2.  Fields stay in
 // caller-owned std::vector<double> in the reference layout; each call
 // uploads its inputs, runs on the GPU and downloads its outputs, so it is a
 // migration path, not the fast path.  The fast path keeps the state on the
@@ -131,6 +132,19 @@ class grid {
   }
   void put(int f0, const velocity &V) const { put(f0, V.u); put(f0 + 1, V.v); put(f0 + 2, V.w); }
   void get(int f0, velocity &V) const { get(f0, V.u); get(f0 + 1, V.v); get(f0 + 2, V.w); }
+
+  // structureGrid.hpp:630-728: assembles the CSR operator on the device and
+  // returns host copies of its row pointers, columns and values (the tuple
+  // the reference returns); jp follows it when prm.op == CFD_OP_CSR
+  std::tuple<std::vector<int>, std::vector<int>, std::vector<double>> createPressureMatrixCSR() {
+    check(cfd_csr_assemble(ctx_), "cfd_csr_assemble");
+    size_t nnz = 0;
+    check(cfd_csr_nnz(ctx_, &nnz), "cfd_csr_nnz");
+    std::vector<int> ptr(iceltotCal + 1), idx(nnz);
+    std::vector<double> val(nnz);
+    check(cfd_csr_download(ctx_, ptr.data(), idx.data(), val.data(), iceltotCal, nnz), "cfd_csr_download");
+    return {std::move(ptr), std::move(idx), std::move(val)};
+  }
 
   // device-resident timestep (runSeqOmp_predictionMethod's loop body)
   cfd_step_stats step() {
@@ -268,12 +282,42 @@ namespace mat {
 // inc/matrix/ELL_sparseMatrix.hpp:35-139: the assembled operator of a grid
 class ELL_matrix {
  public:
+  static constexpr int op = CFD_OP_ELL;
   explicit ELL_matrix(grid &g) : g_(&g) {}
   int row() const { return (int)g_->iceltotCal; }
   // r = A x (multiply_omp, :477-492)
   bool multiply_omp(std::vector<double> &x, std::vector<double> &r) {
     g_->put(CFD_VEC_A, x);
     check(cfd_spmv(g_->ctx(), CFD_OP_ELL, CFD_FP64, CFD_VEC_A, CFD_VEC_B), "cfd_spmv");
+    g_->get(CFD_VEC_B, r);
+    return true;
+  }
+  grid &owner() { return *g_; }
+
+ private:
+  grid *g_;
+};
+
+// inc/matrix/CSR_sparseMatrix.hpp:42-292: the CSR operator of a grid, made on
+// the device by grid::createPressureMatrixCSR
+class CSR_matrix {
+ public:
+  static constexpr int op = CFD_OP_CSR;
+  explicit CSR_matrix(grid &g) : g_(&g) {}
+  // set (:42-120): takes the tuple grid::createPressureMatrixCSR returned;
+  // the device already holds that operator, so this checks the shapes
+  void set(const std::tuple<std::vector<int>, std::vector<int>, std::vector<double>> &m) {
+    size_t nnz = 0;
+    check(cfd_csr_nnz(g_->ctx(), &nnz), "cfd_csr_nnz");
+    if (std::get<0>(m).size() != g_->iceltotCal + 1 || std::get<1>(m).size() != nnz ||
+        std::get<2>(m).size() != nnz)
+      throw std::invalid_argument("CSR_matrix::set: not the operator of this grid");
+  }
+  int row() const { return (int)g_->iceltotCal; }
+  // r = A x (multiply_omp, :278-292)
+  bool multiply_omp(std::vector<double> &x, std::vector<double> &r) {
+    g_->put(CFD_VEC_A, x);
+    check(cfd_spmv(g_->ctx(), CFD_OP_CSR, CFD_FP64, CFD_VEC_A, CFD_VEC_B), "cfd_spmv");
     g_->get(CFD_VEC_B, r);
     return true;
   }
@@ -294,6 +338,8 @@ class bicgstabRe2 {
   void setTolerance(double tol) { tol_ = tol; }
   std::pair<int, double> operator()(const std::vector<double> &rhs, std::vector<double> &x) {
     grid &g = A_->owner();
+    // the device solve runs on the operator the grid's parameters name
+    if (g.prm.op != matrixT::op) throw std::invalid_argument("bicgstabRe2: cfd_params.op names another operator");
     g.put(CFD_RHS, rhs);
     g.put(CFD_X, x);
     int it = 0;

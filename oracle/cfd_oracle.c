@@ -497,6 +497,59 @@ void orc_spmv_ell(size_t N, int K, const int32_t *col, const double *val, const 
 }
 
 /* SPE_matrix1::setupPressure + set_Jp_pc (SPE_sparseMatrix1.hpp:40-121) */
+/* grid::createPressureMatrixCSR (inc/grid/structureGrid.hpp:630-728): the
+   Jacobi-scaled rows with their entries in column order xm, ym, zm, diag,
+   zp, yp, xp (absent neighbours skipped), the diagonal summed in the order
+   xm + ym + zm + xp + yp + zp + aa + bb + cc (not the ELL order), and jp =
+   1/diag.  pd marks Dirichlet pressure faces, which keep their coefficient
+   in the diagonal as in the ELL assembly (the reference's CSR path has none).
+   ptr has N+1 entries; returns the number of entries. */
+long long orc_csr_assemble(const cfd_grid_desc *g, int32_t *ptr, int32_t *col, double *val, double *jp) {
+  const dims_t d = dims_of(g);
+  const long long nzny = (long long)d.nyc * d.nzc;
+  const double *Dx = g->Dx, *Dy = g->Dy, *Dz = g->Dz, *Dxs = g->Dxs, *Dys = g->Dys, *Dzs = g->Dzs;
+  long long e = 0, row = 0;
+  ptr[0] = 0;
+  for (int i = 0; i < d.nxc; ++i)
+    for (int j = 0; j < d.nyc; ++j)
+      for (int k = 0; k < d.nzc; ++k, ++row) {
+        const int I = i + GC, J = j + GC, K = k + GC;
+        const double xp = 1.0 / Dx[I] / Dxs[I], yp = 1.0 / Dy[J] / Dys[J], zp = 1.0 / Dz[K] / Dzs[K];
+        const double xm = 1.0 / Dx[I] / Dxs[I - 1], ym = 1.0 / Dy[J] / Dys[J - 1], zm = 1.0 / Dz[K] / Dzs[K - 1];
+        /* the Neumann closure on every face (:658-666 checks no pressure BC) */
+        double aa = 0.0, bb = 0.0, cc = 0.0;
+        if (i == 0) aa = -xm;
+        if (i == d.nxc - 1) aa = -xp;
+        if (j == 0) bb = -ym;
+        if (j == d.nyc - 1) bb = -yp;
+        if (k == 0) cc = -zm;
+        if (k == d.nzc - 1) cc = -zp;
+        const double ijk = xm + ym + zm + xp + yp + zp + aa + bb + cc;
+        const double J_ = 1.0 / ijk;
+        jp[row] = J_;
+        if (i != 0) { col[e] = (int32_t)(row - nzny); val[e++] = -xm * J_; }
+        if (j != 0) { col[e] = (int32_t)(row - d.nzc); val[e++] = -ym * J_; }
+        if (k != 0) { col[e] = (int32_t)(row - 1); val[e++] = -zm * J_; }
+        col[e] = (int32_t)row; val[e++] = ijk * J_;
+        if (k != d.nzc - 1) { col[e] = (int32_t)(row + 1); val[e++] = -zp * J_; }
+        if (j != d.nyc - 1) { col[e] = (int32_t)(row + d.nzc); val[e++] = -yp * J_; }
+        if (i != d.nxc - 1) { col[e] = (int32_t)(row + nzny); val[e++] = -xp * J_; }
+        ptr[row + 1] = (int32_t)e;
+      }
+  return e;
+}
+
+/* CSR_matrix::multiply_omp (inc/matrix/CSR_sparseMatrix.hpp:278-292): r[i] +=
+   VAL * x[IDX] in entry order, which the reference build contracts to fma */
+void orc_spmv_csr(size_t N, const int32_t *ptr, const int32_t *col, const double *val, const double *x, double *y) {
+#pragma omp parallel for schedule(static)
+  for (long long r = 0; r < (long long)N; ++r) {
+    double acc = 0.0;
+    for (int32_t e = ptr[r]; e < ptr[r + 1]; ++e) acc = fma(val[e], x[col[e]], acc);
+    y[r] = acc;
+  }
+}
+
 void orc_spe_setup(const cfd_grid_desc *g, const int pd[6], double *coef, double *jp) {
   const dims_t d = dims_of(g);
   double *xm = coef, *xp = xm + d.nxc, *ym = xp + d.nxc, *yp = ym + d.nyc, *zm = yp + d.nyc, *zp = zm + d.nzc;
@@ -780,6 +833,7 @@ void orc_maxdiff(const cfd_grid_desc *g, const double *u0, const double *v0, con
 struct orc_bicg {
   size_t N;
   int op;
+  const int32_t *ptr; /* CSR row pointers */
   const int32_t *col;
   const double *val;
   const cfd_grid_desc *g;
@@ -792,6 +846,7 @@ struct orc_bicg {
 
 static void bicg_matvec(orc_bicg *s, const double *x, double *y) {
   if (s->op == CFD_OP_SPE) orc_spmv_spe(s->g, s->coef, s->jp, x, y);
+  else if (s->op == CFD_OP_CSR) orc_spmv_csr(s->N, s->ptr, s->col, s->val, x, y);
   else orc_spmv_ell(s->N, 7, s->col, s->val, x, y);
 }
 
@@ -809,6 +864,12 @@ orc_bicg *orc_bicg_create(size_t N, int op, const int32_t *col, const double *va
   s->tol = 1e-3;
   s->restart_factor = 1e-7;
   s->restart_tol = s->restart_factor * s->tol;
+  return s;
+}
+
+orc_bicg *orc_bicg_create_csr(size_t N, const int32_t *ptr, const int32_t *col, const double *val) {
+  orc_bicg *s = orc_bicg_create(N, CFD_OP_CSR, col, val, NULL, NULL, NULL);
+  if (s) s->ptr = ptr;
   return s;
 }
 
@@ -1259,6 +1320,8 @@ struct orc_sim {
   int32_t *col;
   double *val;
   double *spe_coef, *spe_jp;
+  int32_t *csr_ptr, *csr_col;
+  double *csr_val, *csr_jp;
   orc_bicg *bicg;
   int loop;
   double time;
@@ -1297,6 +1360,10 @@ orc_sim *orc_sim_create(const cfd_grid_desc *g, const cfd_params *prm) {
   s->val = (double *)calloc(7 * N, sizeof(double));
   s->spe_coef = (double *)calloc((size_t)2 * (s->d.nxc + s->d.nyc + s->d.nzc), sizeof(double));
   s->spe_jp = (double *)calloc(N, sizeof(double));
+  s->csr_ptr = (int32_t *)calloc(N + 1, sizeof(int32_t));
+  s->csr_col = (int32_t *)calloc(7 * N, sizeof(int32_t));
+  s->csr_val = (double *)calloc(7 * N, sizeof(double));
+  s->csr_jp = (double *)calloc(N, sizeof(double));
   s->loop = 1;
   s->time = 0.0;
   s->gain = 1.0;
@@ -1315,6 +1382,7 @@ void orc_sim_destroy(orc_sim *s) {
   for (int c = 0; c < 3; ++c) { free(s->T0[c]); free(s->T1[c]); free(s->T3[c]); }
   free(s->p); free(s->eta); free(s->f); free(s->src); free(s->b); free(s->X); free(s->jp);
   free(s->va); free(s->vb); free(s->col); free(s->val); free(s->spe_coef); free(s->spe_jp);
+  free(s->csr_ptr); free(s->csr_col); free(s->csr_val); free(s->csr_jp);
   orc_bicg_destroy(s->bicg);
   free(s->uin);
   adj_free(s);
@@ -1395,7 +1463,14 @@ int orc_sim_setup(orc_sim *s) {
   orc_spe_setup(&s->g, P->p_dirichlet, s->spe_coef, s->spe_jp);
   if (P->op == CFD_OP_SPE) memcpy(s->jp, s->spe_jp, sizeof(double) * s->d.N); /* gA.jp = set_Jp_pc() */
   orc_bicg_destroy(s->bicg);
-  s->bicg = orc_bicg_create(s->d.N, P->op, s->col, s->val, &s->g, s->spe_coef, s->spe_jp);
+  if (P->op == CFD_OP_CSR) {
+    /* RUN_CPU_PC_METHOD.hpp:153-160: the CSR builder's jp becomes gA.jp */
+    orc_csr_assemble(&s->g, s->csr_ptr, s->csr_col, s->csr_val, s->csr_jp);
+    memcpy(s->jp, s->csr_jp, sizeof(double) * s->d.N);
+    s->bicg = orc_bicg_create_csr(s->d.N, s->csr_ptr, s->csr_col, s->csr_val);
+  } else {
+    s->bicg = orc_bicg_create(s->d.N, P->op, s->col, s->val, &s->g, s->spe_coef, s->spe_jp);
+  }
   if (!s->bicg) return -1;
   orc_bicg_set_tolerance(s->bicg, P->tol);
   orc_bicg_set_maxit(s->bicg, P->maxit);

@@ -59,6 +59,11 @@ void orc_rhs(const cfd_grid_desc *g, double dt, const double *jp,
 /* createPressureMatrix; col/val slot-major [7][N]; returns 0 or -1 */
 int orc_ell_assemble(const cfd_grid_desc *g, const int p_dirichlet[6], int32_t *col, double *val, double *jp);
 void orc_spmv_ell(size_t N, int K, const int32_t *col, const double *val, const double *x, double *y);
+/* createPressureMatrixCSR (structureGrid.hpp:630-728): Neumann closure on every
+   face whatever the pressure BC; ptr N+1, col/val up to 7N; returns nnz */
+long long orc_csr_assemble(const cfd_grid_desc *g, int32_t *ptr, int32_t *col, double *val, double *jp);
+/* CSR_matrix::multiply_omp (CSR_sparseMatrix.hpp:278-292) */
+void orc_spmv_csr(size_t N, const int32_t *ptr, const int32_t *col, const double *val, const double *x, double *y);
 /* SPE_matrix1 setupPressure + set_Jp_pc: coef holds xm[nx],xp[nx],ym[ny],yp[ny],zm[nz],zp[nz] (interior
    axis index) followed by jp[N] */
 void orc_spe_setup(const cfd_grid_desc *g, const int p_dirichlet[6], double *coef, double *jp);
@@ -86,6 +91,8 @@ typedef struct orc_bicg orc_bicg;
 /* op: CFD_OP_ELL (col/val) or CFD_OP_SPE (g, coef, jp) */
 orc_bicg *orc_bicg_create(size_t N, int op, const int32_t *col, const double *val,
                           const cfd_grid_desc *g, const double *spe_coef, const double *spe_jp);
+/* op CFD_OP_CSR: ptr/col/val from orc_csr_assemble */
+orc_bicg *orc_bicg_create_csr(size_t N, const int32_t *ptr, const int32_t *col, const double *val);
 void orc_bicg_destroy(orc_bicg *s);
 void orc_bicg_set_tolerance(orc_bicg *s, double tol);
 void orc_bicg_set_maxit(orc_bicg *s, int maxit);
@@ -113,7 +120,7 @@ double *orc_sim_field(orc_sim *s, int field, size_t *len);
 int32_t *orc_sim_ell_col(orc_sim *s);
 /* iniU(u_init) + noise, p = 0, X = 0, T1 = T3 = T0, BC main + copy */
 int orc_sim_init_state(orc_sim *s, double noise_amp, uint64_t seed);
-/* DFIB_CylinderZ and the pressure operator (ELL and SPE) */
+/* DFIB_CylinderZ and the pressure operator (ELL and SPE, plus CSR when op is CSR) */
 int orc_sim_setup(orc_sim *s);
 int orc_sim_step(orc_sim *s, cfd_step_stats *st);
 /* phases of one pass, exposed for the CPU baseline's timing */

@@ -11,6 +11,7 @@
 //   ell_val.bin     ELL values, slot-major [7][N]
 //   jp.bin          gA.jp after createPressureMatrix (N)
 //   spe_jp.bin      SPE_matrix1::set_Jp_pc (N)
+//   csr_*.bin       createPressureMatrixCSR: ptr (N+1), idx, val (int32/fp64), its jp (N)
 //   step_XXX_*.bin  state after each timestep (T0 u,v,w, p, X, f)
 //   fn_*.bin        inputs/outputs of each hot-path function in pass 1 of step FN_STEP
 //   stats.txt       per pass: step pass iters error cD cL, per step: maxdiff
@@ -142,6 +143,15 @@ static int synth_main() {
   std::vector<double> spe_jp = Mx.matA_spe.set_Jp_pc();
   Mx.matA_spe.multiply_omp(x, y);
   dump("syn_spmv_spe.bin", y);
+  {
+    // CSR operator (grid::createPressureMatrixCSR, structureGrid.hpp:630-728),
+    // CSR_matrix::multiply_omp (CSR_sparseMatrix.hpp:278-292); it rewrites gA.jp
+    const std::vector<double> keep_jp = gA.jp;
+    mat::CSR_matrix<double> A(gA.createPressureMatrixCSR());
+    A.multiply_omp(x, y);
+    dump("syn_spmv_csr.bin", y);
+    gA.jp = keep_jp;
+  }
   // DFIB update: p seeded, T3 = 9 sentinel
   fill_noise(t1.p, seed, 6, 0.0, 1.0);
   T3.u.assign(T3.u.size(), 9.0);
@@ -280,6 +290,15 @@ int main(int argc, char **argv) {
     dump("jp.bin", gA.jp);
   }
   std::vector<double> ell_jp = gA.jp;
+  {
+    // CSR operator (RUN_CPU_PC_METHOD.hpp:156, structureGrid.hpp:630-728)
+    auto [ptr, idx, val] = gA.createPressureMatrixCSR();
+    dump("csr_ptr.bin", ptr);
+    dump("csr_idx.bin", idx);
+    dump("csr_val.bin", val);
+    dump("csr_jp.bin", gA.jp);
+    gA.jp = ell_jp;
+  }
   // SPE operator (RUN_CPU_PC_METHOD.hpp:167-175)
   Mx.matA_spe.resize(gA.nxCal, gA.nyCal, gA.nzCal);
   Mx.matA_spe.setupPressure(gA.gC, gA.Dx, gA.Dy, gA.Dz, gA.Dxs, gA.Dys, gA.Dzs);

@@ -110,6 +110,34 @@ int main() {
   const double ew = std::sqrt(dw / nu2);
   const double eu = rel_l2(V.u, T0.u), ev = rel_l2(V.v, T0.v), ex = rel_l2(x, Mx.X_result);
   std::printf("shim vs fused step: u %.2e v %.2e w %.2e X %.2e\n", eu, ev, ew, ex);
+  // the CSR operator through the reference-shaped types: the tuple that
+  // createPressureMatrixCSR returns, applied on the host with the FMA chain
+  // of CSR_matrix::multiply_omp, equals the device product bit for bit
+  auto csr = gA.createPressureMatrixCSR();
+  mat::CSR_matrix Ac(gA);
+  Ac.set(csr);
+  const auto &cptr = std::get<0>(csr);
+  const auto &cidx = std::get<1>(csr);
+  const auto &cval = std::get<2>(csr);
+  std::vector<double> xs(N), yd, yh(N);
+  for (size_t r = 0; r < N; ++r) xs[r] = std::sin(0.37 * (double)r);
+  Ac.multiply_omp(xs, yd);
+  for (size_t r = 0; r < N; ++r) {
+    double acc = 0.0;
+    for (int e = cptr[r]; e < cptr[r + 1]; ++e) acc = std::fma(cval[e], xs[cidx[e]], acc);
+    yh[r] = acc;
+  }
+  const bool csr_ok = cptr.size() == N + 1 && (size_t)cptr.back() == cidx.size() && yd == yh;
+  // a solver over the CSR type on a grid configured for the ELL operator is refused
+  bool op_threw = false;
+  try {
+    solver::bicgstabRe2<mat::CSR_matrix> sc(Ac);
+    sc(Mx.matB, xs);
+  } catch (const std::invalid_argument &) {
+    op_threw = true;
+  }
+  std::printf("CSR: nnz %zu product %s, solver on another operator %s\n", cidx.size(), csr_ok ? "exact" : "DIFFERS",
+              op_threw ? "refused" : "NOT refused");
   // an out-of-range argument surfaces as the reference's exception type
   bool threw = false;
   try {
@@ -117,7 +145,7 @@ int main() {
   } catch (const std::invalid_argument &) {
     threw = true;
   }
-  const bool ok = eu < 1e-11 && ev < 1e-11 && ew < 1e-11 && ex < 1e-8 && threw;
+  const bool ok = eu < 1e-11 && ev < 1e-11 && ew < 1e-11 && ex < 1e-8 && threw && csr_ok && op_threw;
   std::printf("%s\n", ok ? "PASS" : "FAIL");
   return ok ? 0 : 1;
 }

@@ -4,7 +4,8 @@ Builds oracle/_ref (the reference's own header-only sources compiled where
 they lie, see oracle/Makefile), runs its harness at the reference's native
 case (190x210x40 SML grid, cavity table, Re 40, dt 0.002) and records:
   * SHA-256 of the grid metrics, the IB mask, the ELL operator (columns,
-    values), gA.jp and the SPE Jacobi scaling;
+    values), gA.jp, the SPE Jacobi scaling and the CSR operator (row
+    pointers, columns, values and its Jacobi scaling);
   * for every hot-path function applied once to seeded inputs (splitmix64
     noise, reproduced bit for bit by the oracle's orc_fill_noise): SHA-256 of
     its output, the L2 norm, and 256 seeded sample values;
@@ -46,7 +47,12 @@ def main():
     if not Path("/root/reference/inc").is_dir():
         sys.exit("needs the reference sources at /root/reference")
     subprocess.run(["make", "-s", "-C", str(ORACLE), "ref"], check=True)
-    env = dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8"))
+    # one thread: the reference's BC face loops (updateBC.hpp:14-81) are
+    # `omp for ... nowait`, so with several threads the edge cells that two
+    # face loops write race and its BiCGSTAB reductions change with the
+    # partition; serially every dump is reproducible and the BC follows the
+    # x -> y -> z face order its code spells out
+    env = dict(os.environ, OMP_NUM_THREADS="1")
     h = str(ORACLE / "_ref" / "ref_harness")
     DUMPS.mkdir(parents=True, exist_ok=True)
     subprocess.run([h, str(DUMPS / "native"), "3", "1e-3", "ell", "2"], check=True, env=env,
@@ -55,7 +61,8 @@ def main():
     nat, syn = DUMPS / "native", DUMPS / "synth"
     fx = {"case": "native 190x210x40 SML, BC_Cavity_A, Re 40, dt 0.002, cylinder (6.5, 6) R 0.5",
           "seed": 20240611, "setup": {}, "synth": {}, "scalars": {}, "steps": []}
-    for name in ("grid.bin", "eta.bin", "ell_col.bin", "ell_val.bin", "jp.bin", "spe_jp.bin"):
+    for name in ("grid.bin", "eta.bin", "ell_col.bin", "ell_val.bin", "jp.bin", "spe_jp.bin", "csr_ptr.bin",
+                 "csr_idx.bin", "csr_val.bin", "csr_jp.bin"):
         fx["setup"][name] = sha(nat / name)
     eta = np.fromfile(nat / "eta.bin")
     fx["setup"]["eta_sum"] = float(eta.sum())

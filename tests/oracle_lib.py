@@ -54,6 +54,9 @@ def _declare(L):
     L.orc_ell_assemble.argtypes = [G, C.POINTER(C.c_int), ip, dp, dp]
     L.orc_ell_assemble.restype = C.c_int
     L.orc_spmv_ell.argtypes = [C.c_size_t, C.c_int, ip, dp, dp, dp]
+    L.orc_csr_assemble.argtypes = [G, ip, ip, dp, dp]
+    L.orc_csr_assemble.restype = C.c_longlong
+    L.orc_spmv_csr.argtypes = [C.c_size_t, ip, ip, dp, dp, dp]
     L.orc_spe_setup.argtypes = [G, C.POINTER(C.c_int), dp, dp]
     L.orc_spmv_spe.argtypes = [G, dp, dp, dp, dp]
     L.orc_pressure_scatter.argtypes = [G, dp, dp]
@@ -70,6 +73,8 @@ def _declare(L):
     L.orc_fill_noise.argtypes = [dp, C.c_size_t, C.c_uint64, C.c_int, C.c_double, C.c_double]
     L.orc_bicg_create.argtypes = [C.c_size_t, C.c_int, ip, dp, G, C.c_void_p, C.c_void_p]
     L.orc_bicg_create.restype = C.c_void_p
+    L.orc_bicg_create_csr.argtypes = [C.c_size_t, ip, ip, dp]
+    L.orc_bicg_create_csr.restype = C.c_void_p
     L.orc_bicg_destroy.argtypes = [C.c_void_p]
     L.orc_bicg_set_tolerance.argtypes = [C.c_void_p, C.c_double]
     L.orc_bicg_set_maxit.argtypes = [C.c_void_p, C.c_int]

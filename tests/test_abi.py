@@ -109,3 +109,11 @@ def test_no_device_is_an_error_not_a_fallback():
         h = C.c_void_p()
         assert L.cfd_create(C.byref(desc), C.byref(p), 0, C.byref(h)) == capi.CFD_ERR_HIP
         assert not h.value
+
+
+def test_reference_shim_compiles():
+    """The header-only C++ shim and its driver compile against the C-ABI
+    header alone (no GPU, no library needed for a syntax check)."""
+    import subprocess
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I", str(ROOT / "include"),
+                    str(ROOT / "tests" / "cpp" / "test_shim.cpp")], check=True)

@@ -68,6 +68,48 @@ def test_setup_bit_exact(native):
     assert sha(native["spe_jp"]) == FX["setup"]["spe_jp.bin"]
 
 
+def csr_native(g):
+    N = g.N
+    ptr, col, val, jp = np.zeros(N + 1, np.int32), np.zeros(7 * N, np.int32), np.zeros(7 * N), np.zeros(N)
+    nnz = O.lib().orc_csr_assemble(g.ref, ptr, col, val, jp)
+    assert nnz == ptr[-1]
+    return ptr, col[:nnz].copy(), val[:nnz].copy(), jp
+
+
+def test_csr_operator_bit_exact(native):
+    """createPressureMatrixCSR and CSR_matrix::multiply_omp against the
+    compiled reference: row pointers, columns, values, its jp, and A x on
+    the seeded x."""
+    g = native["g"]
+    ptr, col, val, jp = csr_native(g)
+    assert sha(ptr) == FX["setup"]["csr_ptr.bin"]
+    assert sha(col) == FX["setup"]["csr_idx.bin"]
+    assert sha(val) == FX["setup"]["csr_val.bin"]
+    assert sha(jp) == FX["setup"]["csr_jp.bin"]
+    x = O.noise(g.N, SEED, 5, 0.0, 1.0)
+    y = np.zeros(g.N)
+    O.lib().orc_spmv_csr(g.N, ptr, col, val, x, y)
+    assert sha(y) == FX["synth"]["syn_spmv_csr.bin"]["sha256"]
+
+
+def test_csr_is_the_ell_operator(native):
+    """The two builders sum the diagonal in different orders and sort the
+    entries differently; the operator is the same: equal sparsity and values
+    within a few ulps (padding slots of the ELL copy hold exact zeros)."""
+    import scipy.sparse as sp
+    g = native["g"]
+    N = g.N
+    ptr, col, val, jp = csr_native(g)
+    A = sp.csr_matrix((val, col, ptr), shape=(N, N))
+    rows = np.tile(np.arange(N), 7)
+    E = sp.csr_matrix((native["val"], (rows, native["col"])), shape=(N, N))
+    E.eliminate_zeros()
+    assert A.nnz == E.nnz == len(val)
+    D = (A - E).tocoo()
+    assert np.abs(D.data).max(initial=0.0) <= 4 * np.finfo(float).eps * np.abs(val).max()
+    assert np.all(np.abs(jp - native["jp"]) <= 4 * np.finfo(float).eps * np.abs(jp))
+
+
 def test_functions_on_seeded_inputs(native):
     g = native["g"]
     N, G = g.N, g.G
@@ -78,11 +120,9 @@ def test_functions_on_seeded_inputs(native):
     for c, n in enumerate("uvw"):
         check_samples(f"syn_pred_T1_{n}.bin", T1[c], 1e-13)
     src = O.noise(3 * N, SEED, 4, 0.0, 1e-3)
-    T1r = [np.fromfile(p) if False else None for p in ()]  # (inputs are regenerated, never read)
-    # source add and RHS continue from the reference's own predictor output
-    # (the predictor differs by FMA contraction only; rebuild it from samples
-    # is impossible, so this stage starts from the oracle's T1 and the
-    # following bit-exact checks use stage-local comparisons)
+    # source add and RHS continue from the oracle's predictor output (the
+    # fixtures hold only samples of the reference's), so they are compared
+    # through samples at a tolerance rather than by hash
     L.orc_source_add(g.ref, 1, src, *T1)
     b = np.zeros(N)
     L.orc_rhs(g.ref, 0.002, native["jp"], *T1, b)
@@ -152,6 +192,29 @@ def test_bicgstab_and_sor(native):
     assert its.value == int(FX["scalars"]["sor_seq"][0])
     assert abs(res - FX["scalars"]["sor_seq"][1]) <= 1e-12 * FX["scalars"]["sor_seq"][1]
     check_samples("syn_sor_p.bin", pp, 1e-12)
+
+
+def test_bicgstab_csr_consistent_rhs(native):
+    """bicgstabRe2 on the CSR operator (P_SOLVER_BICG_CSR) recovers x_t from
+    b = A x_t to the tolerance the operator's conditioning allows, and stops
+    on the reference's relative-residual test."""
+    g = native["g"]
+    N = g.N
+    ptr, col, val, jp = csr_native(g)
+    xt = O.noise(N, SEED, 7, 0.0, 1.0)
+    xt -= xt.mean()
+    b = np.zeros(N)
+    O.lib().orc_spmv_csr(N, ptr, col, val, xt, b)
+    s = O.lib().orc_bicg_create_csr(N, ptr, col, val)
+    O.lib().orc_bicg_set_tolerance(s, 1e-9)
+    x = np.zeros(N)
+    it = C.c_int()
+    rel = O.lib().orc_bicg_solve(s, b, x, C.byref(it))
+    O.lib().orc_bicg_destroy(s)
+    r = np.zeros(N)
+    O.lib().orc_spmv_csr(N, ptr, col, val, x, r)
+    assert rel < 1e-9 and np.linalg.norm(b - r) <= 1.01e-9 * np.linalg.norm(b)
+    assert 0 < it.value < 5000
 
 
 def test_pc_loop_statistics(native):
