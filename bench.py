@@ -169,14 +169,24 @@ def cpu_baseline(w, iters_per_step, threads):
 def kernel_table(sim, reps):
     N = sim.N
     rows = {}
-    for what, name in ((0, "spmv_fp64"), (1, "spmv_fp32"), (2, "spe_fp64"), (6, "copy"), (5, "predictor"),
-                       (4, "bicg_iter"), (3, "sor_sweep")):
+    nbytes = dict(BYTES)
+    try:
+        # CSR SpMV: 12 B per stored entry (value + column), the row pointers,
+        # x and y (boundary rows hold fewer than 7 entries)
+        nnz = len(sim.csr()[1])
+        nbytes["spmv_csr"] = (12 * nnz + 4 * (N + 1) + 16 * N) / N
+    except Exception as e:  # CSR operator not assembled for this context
+        log(f"[bench] kernel spmv_csr: {e}")
+    for what, name in ((0, "spmv_fp64"), (1, "spmv_fp32"), (2, "spe_fp64"), (7, "spmv_csr"), (6, "copy"),
+                       (5, "predictor"), (4, "bicg_iter"), (3, "sor_sweep")):
+        if name not in nbytes:
+            continue
         try:
             ms = sim.time_kernel(what, reps if what != 4 else max(10, reps // 4))
         except Exception as e:  # an operator not assembled for this context
             log(f"[bench] kernel {name}: {e}")
             continue
-        gbs = BYTES[name] * N / (ms * 1e-3) / 1e9
+        gbs = nbytes[name] * N / (ms * 1e-3) / 1e9
         rows[name] = {"avg_us": ms * 1e3, "gbs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS}
         log(f"[bench] {name:10s} {ms * 1e3:9.1f} us  {gbs:7.0f} GB/s  ({gbs / HBM_PEAK_GBS:.2f} of peak)")
     return rows
@@ -276,6 +286,7 @@ def main():
         sim.init_state()
         sim.createPressureMatrix()
         sim.setupPressure()
+        sim.createPressureMatrixCSR()
         rng = np.random.default_rng(2)
         sim.upload(capi.X, rng.standard_normal(sim.N))
         sim.upload(capi.RHS, rng.standard_normal(sim.N))
@@ -290,6 +301,7 @@ def main():
         iters64 = per_step(st64, "iters")
         # kernel roofline table; the dominant kernel is the ELL SpMV (FP64)
         sim64.setupPressure()
+        sim64.createPressureMatrixCSR()
         kt = kernel_table(sim64, args.reps)
         spmv = kt["spmv_fp64"]
 

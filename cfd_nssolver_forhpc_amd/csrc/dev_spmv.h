@@ -94,6 +94,46 @@ __device__ inline double op_row(const OpArgs &a, int64_t r, const double *__rest
   return ell_row<double, double>(a.col, a.val, a.N, r, x);
 }
 
+// CSR rows of one 256-row block staged through LDS.  The block's entries are
+// contiguous, [ptr[base], ptr[base + 256]), so the block reads them with
+// coalesced non-temporal loads and each thread then runs its row's FMA chain
+// (the order of csr_row) out of LDS; a thread walking its own row straight
+// from memory spreads every load instruction of a wave over ~3.5 KB.  Only
+// CSR launches size the dynamic LDS (op_lds_bytes), so the other operators
+// keep their occupancy.
+constexpr int OP_BLOCK = 256;
+constexpr int CSR_STAGE = OP_BLOCK * KSLOTS;
+
+inline size_t op_lds_bytes(const OpArgs &a) {
+  return a.op == CFD_OP_CSR ? (size_t)CSR_STAGE * (sizeof(double) + sizeof(int32_t)) : 0;
+}
+
+// (A x)[base + threadIdx.x], 0 past the last row.  base is the block's first
+// row: the same in every thread of a 256-thread block, all of which call this.
+__device__ inline double op_row_block(const OpArgs &a, int64_t base, const double *__restrict__ x) {
+  const int64_t r = base + threadIdx.x;
+  if (a.op != CFD_OP_CSR) return r < a.N ? op_row(a, r, x) : 0.0;
+  extern __shared__ double csr_lds[];
+  double *sval = csr_lds;
+  int32_t *scol = reinterpret_cast<int32_t *>(csr_lds + CSR_STAGE);
+  const int64_t rend = base + OP_BLOCK < a.N ? base + OP_BLOCK : a.N;
+  const int32_t e0 = a.cptr[base];
+  const int32_t cnt = a.cptr[rend] - e0;
+  if (cnt > CSR_STAGE)  // more than 7 entries a row on average: not staged (block-uniform)
+    return r < a.N ? csr_row(a.cptr, a.ccol, a.cval, r, x) : 0.0;
+  __syncthreads();  // the previous block's rows have been read
+  for (int32_t q = threadIdx.x; q < cnt; q += OP_BLOCK) {
+    sval[q] = ntload(a.cval + e0 + q);
+    scol[q] = ntload(a.ccol + e0 + q);
+  }
+  __syncthreads();
+  if (r >= a.N) return 0.0;
+  const int32_t e1 = a.cptr[r + 1] - e0;
+  double acc = 0.0;
+  for (int32_t e = a.cptr[r] - e0; e < e1; ++e) acc = fma(sval[e], x[scol[e]], acc);
+  return acc;
+}
+
 // FP32 values and vector; products are exact in FP64 and summed in FP64, so
 // the cancellation of the nearly row-sum-free operator does not cost FP32
 // digits (a pure FP32 accumulation loses them for smooth x and stalls the

@@ -155,7 +155,10 @@ __global__ void __launch_bounds__(256) spe_jp_kernel(int nxc, int nyc, int nzc, 
 
 __global__ void __launch_bounds__(256) spmv_kernel(OpArgs a, const double *__restrict__ x, double *__restrict__ y) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) y[r] = op_row(a, r, x);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.N; base += stride) {
+    const double v = op_row_block(a, base, x);
+    if (base + threadIdx.x < a.N) y[base + threadIdx.x] = v;
+  }
 }
 
 // FP32 values and vectors, FP32 accumulation (the mixed-precision inner operator)
@@ -231,7 +234,7 @@ hipError_t launch_spmv(Ctx &c, int op, int precision, const double *x, double *y
   if (precision == CFD_MIXED && op == CFD_OP_ELL)
     spmv32_from64_kernel<<<stream_blocks(c.g.N), 256, 0, c.stream>>>(a, x, y);
   else
-    spmv_kernel<<<stream_blocks(c.g.N), 256, 0, c.stream>>>(a, x, y);
+    spmv_kernel<<<stream_blocks(c.g.N), 256, op_lds_bytes(a), c.stream>>>(a, x, y);
   return hipGetLastError();
 }
 

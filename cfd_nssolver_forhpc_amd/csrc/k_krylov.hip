@@ -39,6 +39,20 @@ __device__ inline float op_apply<float>(const OpArgs &a, int64_t r, const float 
   return op_row32(a, r, x);
 }
 
+// (A x)[base + threadIdx.x] for the block of rows starting at base (0 past
+// the last row); every thread of the block calls it with the same base
+template <typename T>
+__device__ inline T op_apply_block(const OpArgs &a, int64_t base, const T *__restrict__ x);
+template <>
+__device__ inline double op_apply_block<double>(const OpArgs &a, int64_t base, const double *__restrict__ x) {
+  return op_row_block(a, base, x);
+}
+template <>
+__device__ inline float op_apply_block<float>(const OpArgs &a, int64_t base, const float *__restrict__ x) {
+  const int64_t r = base + threadIdx.x;
+  return r < a.N ? op_row32(a, r, x) : 0.0f;
+}
+
 __global__ void ks_setup_kernel(KrylovScalars *ks, double tol, double restart_tol, int maxit, int timestep) {
   ks->tol = tol;
   ks->restart_tol = restart_tol;
@@ -58,8 +72,10 @@ __global__ void __launch_bounds__(256) init_kernel(OpArgs a, const T *__restrict
                                                    double *part) {
   double acc[2] = {0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) {
-    const T ax = op_apply<T>(a, r, x);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.N; base += stride) {
+    const T ax = op_apply_block<T>(a, base, x);
+    const int64_t r = base + threadIdx.x;
+    if (r >= a.N) continue;
     const T br = b[r];
     const T rr = br - ax;
     r0[r] = rr;
@@ -92,8 +108,10 @@ __global__ void __launch_bounds__(256) spmv_dot_kernel(OpArgs a, const KrylovSca
   if (ks->done) return;
   double acc[1] = {0.0};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) {
-    const T y = op_apply<T>(a, r, p1);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.N; base += stride) {
+    const T y = op_apply_block<T>(a, base, p1);
+    const int64_t r = base + threadIdx.x;
+    if (r >= a.N) continue;
     ap[r] = y;
     acc[0] += (double)y * (double)r0[r];
   }
@@ -124,8 +142,10 @@ __global__ void __launch_bounds__(256) spmv_dot2_kernel(OpArgs a, const KrylovSc
   if (ks->done) return;
   double acc[2] = {0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) {
-    const T y = op_apply<T>(a, r, s1);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.N; base += stride) {
+    const T y = op_apply_block<T>(a, base, s1);
+    const int64_t r = base + threadIdx.x;
+    if (r >= a.N) continue;
     as[r] = y;
     acc[0] += (double)y * (double)s1[r];
     acc[1] += (double)y * (double)y;
@@ -269,9 +289,12 @@ __global__ void __launch_bounds__(256) dc_residual_kernel(OpArgs a, const double
                                                           double *part) {
   double acc[2] = {0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.N; base += stride) {
+    const double ax = op_row_block(a, base, x);
+    const int64_t r = base + threadIdx.x;
+    if (r >= a.N) continue;
     const double br = b[r];
-    const double rr = br - op_row(a, r, x);
+    const double rr = br - ax;
     r32[r] = (float)rr;
     acc[0] += rr * rr;
     acc[1] += br * br;
@@ -305,10 +328,10 @@ hipError_t launch_iters(Ctx &c, const OpArgs &a, const Vecs<T> &v, KrylovScalars
   const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
   double *p1 = red_part_host(c, RS_SPMV_DOT), *p2 = red_part_host(c, RS_SPMV_DOT2), *p3 = red_part_host(c, RS_XR);
   for (int it = 0; it < nit; ++it) {
-    spmv_dot_kernel<T><<<nb, 256, 0, c.stream>>>(a, ks, v.p1, v.ap, v.r0, p1);
+    spmv_dot_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, ks, v.p1, v.ap, v.r0, p1);
     alpha_finalize<<<1, 256, 0, c.stream>>>(p1, nb, ks);
     s_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.r1, v.ap, v.s1);
-    spmv_dot2_kernel<T><<<nb, 256, 0, c.stream>>>(a, ks, v.s1, v.as, p2);
+    spmv_dot2_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, ks, v.s1, v.as, p2);
     omega_finalize<<<1, 256, 0, c.stream>>>(p2, nb, ks);
     xr_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.x, v.p1, v.s1, v.as, v.r1, v.r0, p3);
     conv_finalize<<<1, 256, 0, c.stream>>>(p3, nb, ks);
@@ -327,7 +350,8 @@ int solve_bicg(Ctx &c, const OpArgs &a, const Vecs<T> &v, KrylovScalars *ks, int
   const int64_t N = c.g.N;
   const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
   ks_setup_kernel<<<1, 1, 0, c.stream>>>(ks, tol, 1e-7 * tol, maxit, timestep);
-  init_kernel<T><<<nb, 256, 0, c.stream>>>(a, v.b, v.x, v.r0, v.r1, v.p1, v.px1, v.px2, red_part_host(c, RS_INIT));
+  init_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, v.b, v.x, v.r0, v.r1, v.p1, v.px1, v.px2,
+                                                      red_part_host(c, RS_INIT));
   init_finalize<<<1, 256, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, ks);
   CFD_HIP(hipGetLastError());
   int batch = 2;
@@ -408,7 +432,7 @@ int launch_pc_iters(Ctx &c, const OpArgs &a, const PVecs<T> &v, KrylovScalars *k
   for (int it = 0; it < nit; ++it) {
     int rc = precondition<T>(c, v.p1, v.ph, ks, pc32);
     if (rc) return rc;
-    spmv_dot_kernel<T><<<nb, 256, 0, c.stream>>>(a, ks, v.ph, v.v, v.r0, p1);
+    spmv_dot_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, ks, v.ph, v.v, v.r0, p1);
     alpha_finalize<<<1, 256, 0, c.stream>>>(p1, nb, ks);
     s_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.r1, v.v, v.s1);
     if ((rc = precondition<T>(c, v.s1, v.sh, ks, pc32))) return rc;
@@ -428,7 +452,8 @@ int solve_pbicg(Ctx &c, const OpArgs &a, const PVecs<T> &v, KrylovScalars *ks, d
   const int64_t N = c.g.N;
   const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
   ks_setup_kernel<<<1, 1, 0, c.stream>>>(ks, tol, 0.0, maxit, 0);
-  init_kernel<T><<<nb, 256, 0, c.stream>>>(a, v.b, v.x, v.r0, v.r1, v.p1, nullptr, nullptr, red_part_host(c, RS_INIT));
+  init_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, v.b, v.x, v.r0, v.r1, v.p1, nullptr, nullptr,
+                                                      red_part_host(c, RS_INIT));
   init_finalize<<<1, 256, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, ks);
   CFD_HIP(hipGetLastError());
   CFD_HIP(hipMemcpyAsync(c.hks, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c.stream));
@@ -517,7 +542,7 @@ int defect_correction(Ctx &c, const double *b, double *x, double tol, double tol
   c.solves++;
   for (;;) {
     double *part = red_part_host(c, RS_NORM);
-    dc_residual_kernel<<<nb, 256, 0, c.stream>>>(a64, b, x, r32, part);
+    dc_residual_kernel<<<nb, 256, op_lds_bytes(a64), c.stream>>>(a64, b, x, r32, part);
     sum_finalize<2><<<1, 256, 0, c.stream>>>(part, nb, c.red.out + RS_NORM * 4);
     CFD_HIP(hipGetLastError());
     CFD_HIP(hipMemcpyAsync(c.hred, c.red.out + RS_NORM * 4, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -565,8 +590,9 @@ hipError_t launch_bicg_timing_setup(Ctx &c) {
   hipError_t e = launch_fill(c, c.f[CFD_VEC_B], 0.0, N);
   if (e != hipSuccess) return e;
   ks_setup_kernel<<<1, 1, 0, c.stream>>>(c.ks32, 0.0, 0.0, 1 << 30, 100);
-  init_kernel<double><<<nb, 256, 0, c.stream>>>(op_args(c, c.prm.op), c.f[CFD_RHS], c.f[CFD_VEC_B], c.r0, c.r1, c.p1,
-                                                nullptr, nullptr, red_part_host(c, RS_INIT));
+  const OpArgs a = op_args(c, c.prm.op);
+  init_kernel<double><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, c.f[CFD_RHS], c.f[CFD_VEC_B], c.r0, c.r1, c.p1,
+                                                              nullptr, nullptr, red_part_host(c, RS_INIT));
   init_finalize<<<1, 256, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, c.ks32);
   return hipGetLastError();
 }
@@ -596,9 +622,12 @@ __global__ void __launch_bounds__(256) mr_init_kernel(OpArgs a64, const double *
                                                       double *part) {
   double acc[3] = {0.0, 0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a64.N; r += stride) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a64.N; base += stride) {
+    const double ax = op_row_block(a64, base, x);
+    const int64_t r = base + threadIdx.x;
+    if (r >= a64.N) continue;
     const double br = b[r];
-    const double rr = br - op_row(a64, r, x);
+    const double rr = br - ax;
     const T rf = (T)rr;
     r32[r] = rf;
     r0[r] = rf;
@@ -699,8 +728,11 @@ __global__ void __launch_bounds__(256) mr_replace_kernel(OpArgs a64, const Krylo
   if (ks->done || !ks->replace) return;
   double acc[2] = {0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a64.N; r += stride) {
-    const double rr = b[r] - op_row(a64, r, x);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a64.N; base += stride) {
+    const double ax = op_row_block(a64, base, x);
+    const int64_t r = base + threadIdx.x;
+    if (r >= a64.N) continue;
+    const double rr = b[r] - ax;
     const T rf = (T)rr;
     r32[r] = rf;
     acc[0] += rr * rr;
@@ -824,7 +856,8 @@ int reliable_solve(Ctx &c, const double *b, double *x, double tol, double delta,
   c.solves32++;
   ks_setup_kernel<<<1, 1, 0, c.stream>>>(ks, tol, 1e-7 * tol, maxit, c.solves32);
   set_delta_kernel<<<1, 1, 0, c.stream>>>(ks, delta);
-  mr_init_kernel<T><<<nb, 256, 0, c.stream>>>(a64, b, x, r1, r0, p1, (T *)xl, red_part_host(c, RS_INIT));
+  mr_init_kernel<T><<<nb, 256, op_lds_bytes(a64), c.stream>>>(a64, b, x, r1, r0, p1, (T *)xl,
+                                                               red_part_host(c, RS_INIT));
   mr_init_finalize<<<1, 256, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, ks);
   CFD_HIP(hipGetLastError());
   int batch = 4;
@@ -832,7 +865,7 @@ int reliable_solve(Ctx &c, const double *b, double *x, double tol, double delta,
   int prev_it = 0;
   for (;;) {
     for (int it = 0; it < batch; ++it) {
-      spmv_dot_kernel<T><<<nb, 256, 0, c.stream>>>(ain, ks, p1, ap, r0, P1);
+      spmv_dot_kernel<T><<<nb, 256, op_lds_bytes(ain), c.stream>>>(ain, ks, p1, ap, r0, P1);
       mr_alpha_finalize<<<1, 256, 0, c.stream>>>(P1, nb, ks);
       mr_s_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, r1, ap, s1);
       mr_spmv_dot2_kernel<T><<<nb, 256, 0, c.stream>>>(ain, ks, s1, as, P2);
@@ -843,7 +876,7 @@ int reliable_solve(Ctx &c, const double *b, double *x, double tol, double delta,
         mr_xr_update_kernel<T, double><<<nb, 256, 0, c.stream>>>(N, ks, x, p1, s1, as, r1, r0, P3);
       mr_conv_finalize<<<1, 256, 0, c.stream>>>(P3, nb, ks);
       if constexpr (std::is_same<T, float>::value) mr_fold_kernel<<<nb, 256, 0, c.stream>>>(N, ks, x, xl);
-      mr_replace_kernel<T><<<nb, 256, 0, c.stream>>>(a64, ks, b, x, r1, r0, P4);
+      mr_replace_kernel<T><<<nb, 256, op_lds_bytes(a64), c.stream>>>(a64, ks, b, x, r1, r0, P4);
       mr_replace_finalize<<<1, 256, 0, c.stream>>>(P4, nb, ks);
       mr_p_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, r1, p1, ap, r0);
     }
