@@ -9,5 +9,5 @@ HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
   -Wall -Wno-unused-function -Wno-unused-variable \
   -I "$HERE/../include" \
   -o "$HERE/_lib/libcfd_hip.so.tmp" \
-  "$HERE"/csrc/capi.hip "$HERE"/csrc/k_flow.hip "$HERE"/csrc/k_ell.hip "$HERE"/csrc/k_krylov.hip "$HERE"/csrc/k_sor.hip "$HERE"/csrc/adjoint.hip "$HERE"/csrc/k_mg.hip
+  "$HERE"/csrc/capi.hip "$HERE"/csrc/k_flow.hip "$HERE"/csrc/k_ell.hip "$HERE"/csrc/k_krylov.hip "$HERE"/csrc/k_sor.hip "$HERE"/csrc/adjoint.hip "$HERE"/csrc/k_mg.hip "$HERE"/csrc/io.hip
 mv "$HERE/_lib/libcfd_hip.so.tmp" "$HERE/_lib/libcfd_hip.so"

@@ -37,6 +37,9 @@ CFD_MIXED_VALUES32 = 2
 CFD_MIXED_PC32 = 3
 CFD_PRECOND_JACOBI = 0
 CFD_PRECOND_MG = 1
+CFD_CDCL_REFERENCE = 0
+CFD_CDCL_BUFFERED = 1
+CFD_CDCL_EVERY_STEP = 2
 
 # field ids
 T0_U, T0_V, T0_W, T1_U, T1_V, T1_W, T3_U, T3_V, T3_W = range(9)
@@ -253,6 +256,13 @@ def _declare(L: C.CDLL) -> None:
         "cfd_set_forcing_gain": ([vp, C.c_double], C.c_int),
         "cfd_adjoint_forward": ([vp, C.c_int, C.c_int, C.c_int, dp], C.c_int),
         "cfd_adjoint_backward": ([vp, dp, C.c_size_t, dp, C.POINTER(CfdAdjointStats)], C.c_int),
+        "cfd_cdcl_open": ([C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.POINTER(vp)], C.c_int),
+        "cfd_cdcl_record": ([vp, C.c_int, C.c_double, C.c_double, C.c_double], C.c_int),
+        "cfd_cdcl_close": ([vp], C.c_int),
+        "cfd_plot3d_write_x": ([C.POINTER(CfdGridDesc), C.c_char_p], C.c_int),
+        "cfd_plot3d_write_q": ([vp, C.c_char_p, C.c_int, C.c_double], C.c_int),
+        "cfd_plot3d_read_q": ([C.c_char_p, C.POINTER(CfdGridDesc), ip, C.POINTER(C.c_float), dp, dp, dp, dp, dp],
+                              C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -217,6 +217,9 @@ int solve_configured(Ctx &c, int *iters, double *relres, int *inner) {
 }
 
 }  // namespace
+
+int sync_pressure(Ctx &c) { return sync_p(c); }
+
 }  // namespace cfd
 
 using namespace cfd;

@@ -192,6 +192,9 @@ hipError_t launch_ell_assemble(Ctx &c);
 hipError_t launch_ell_permute(Ctx &c);
 hipError_t launch_spe_assemble(Ctx &c);
 hipError_t launch_csr_assemble(Ctx &c);
+// ghosted p refreshed from X_result (with its BC ghosts) when the last
+// timestep left the pressure there
+int sync_pressure(Ctx &c);
 // CFD_OK when operator `op` is assembled, else the CFD_ERR_STATE failure
 int op_ready(const Ctx &c, int op);
 // the FP64 operator of the configured path: prm.op when it is assembled,

@@ -11,6 +11,7 @@ arrive as status codes and are raised as :class:`CfdError` (or
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -251,6 +252,12 @@ class Simulation:
         cd, cl = C.c_double(), C.c_double()
         check(self.lib.cfd_drag(self.h, C.byref(cd), C.byref(cl)), "cfd_drag")
         return cd.value, cl.value
+
+    def write_plot3d_q(self, path, time: float, vel_field: int = capi.T3_U):
+        """OutputPLOT3D_Qfile (inc/io/Plot3D/qfile.hpp:7-104) from the device
+        fields: p, the cell-centred velocity of vel_field's triple, eta."""
+        check(self.lib.cfd_plot3d_write_q(self.h, os.fsencode(path), int(vel_field), float(time)),
+              "cfd_plot3d_write_q")
 
     def CheckSteadyStatebyMaxVal(self):
         out = (C.c_double * 3)()

@@ -287,6 +287,36 @@ int cfd_adjoint_forward(cfd_ctx *ctx, int nsteps, int avg_from, int ckpt_every, 
    leaves the state where the forward run ended */
 int cfd_adjoint_backward(cfd_ctx *ctx, double *grad_uin, size_t n, double *grad_gain, cfd_adjoint_stats *stats);
 
+/* ---- output and restart files (host I/O) -------------------------------- */
+/* Time_cDcL.dat writer modes */
+#define CFD_CDCL_REFERENCE 0  /* Delay_IO_CD_CL as written: every flush writes delay_io copies of the current line */
+#define CFD_CDCL_BUFFERED 1   /* Delay_IO_CD_CL as intended: every flush writes the delay_io buffered steps */
+#define CFD_CDCL_EVERY_STEP 2 /* noDelay_IO_CD_CL: one line per step */
+typedef struct cfd_cdcl_log cfd_cdcl_log;
+/* the Tecplot drag / lift history of inc/dfib/virtualForceIntergrator.hpp:44-177
+   (simuClass::delayIO and its cd / cl ring buffers), appended to path; zone
+   is simuClass::ZONE(), e.g. "ELL (BICG)" */
+int cfd_cdcl_open(const char *path, int delay_io, const char *zone, int mode, cfd_cdcl_log **out);
+/* one step: simuClass::loop (1-based) and simuClass::time of the step (before
+   finishloop), its cD and cL (cfd_drag): header at loop 1, flush when
+   loop % delay_io == 0 (every call in CFD_CDCL_EVERY_STEP) */
+int cfd_cdcl_record(cfd_cdcl_log *log, int loop, double time, double cd, double cl);
+int cfd_cdcl_close(cfd_cdcl_log *log);
+/* OutputPlot3D_Xfile (inc/io/Plot3D/xfileWrite.hpp:6-61): one block, cell
+   centres Xc, Yc, Zc as float, i fastest */
+int cfd_plot3d_write_x(const cfd_grid_desc *grid, const char *path);
+/* OutputPLOT3D_Qfile (inc/io/Plot3D/qfile.hpp:7-104): header (1 block,
+   nxCal, nyCal, nzCal, mach 0, alpha 0, Re, time as float) then p, the
+   cell-centred velocity of the triple vel_field (CFD_T3_U in the reference's
+   loop) and eta as float, i fastest; assembled on the device */
+int cfd_plot3d_write_q(cfd_ctx *ctx, const char *path, int vel_field, double time);
+/* QfileRead (inc/io/Plot3D/qfileRead.hpp:6-100): header[4] = mach, alpha,
+   Re, time; p, u, v, w, eta ghosted (G entries each, zero outside the
+   interior), face velocities averaged back from the cell-centred values.
+   The file's grid must be the given one (CFD_ERR_ARG otherwise). */
+int cfd_plot3d_read_q(const char *path, const cfd_grid_desc *grid, int *nblock, float header[4], double *p,
+                      double *u, double *v, double *w, double *eta);
+
 /* ---- measurement ------------------------------------------------------ */
 /* average device time (ms) of one launch of the kernel named by `what`, over
    nrep launches bracketed by HIP events on the context stream:

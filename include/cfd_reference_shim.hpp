@@ -7,8 +7,10 @@
 // DfibArray, MxClass, calDomain, ConvectionDifussion, createBMatrix_omp,
 // createPressureMatrix, update_UandF_omp, Pressure_transform_X_result,
 // BC_staggered_main, BC_staggered_copy, DFIB_CylinderZ, virtualF_Int,
-// CheckSteadyStatebyMaxVal_omp, grid::createPressureMatrixCSR, mat::ELL_matrix,
-// mat::CSR_matrix, solver::bicgstabRe2.  Fields stay in
+// CheckSteadyStatebyMaxVal_omp, Delay_IO_CD_CL, noDelay_IO_CD_CL,
+// OutputPlot3D_Xfile, OutputPLOT3D_Qfile, QfileRead,
+// grid::createPressureMatrixCSR, mat::ELL_matrix, mat::CSR_matrix,
+// solver::bicgstabRe2.  Fields stay in
 // caller-owned std::vector<double> in the reference layout; each call
 // uploads its inputs, runs on the GPU and downloads its outputs, so it is a
 // migration path, not the fast path.  The fast path keeps the state on the
@@ -20,6 +22,8 @@
 
 #include <cmath>
 #include <cstring>
+#include <iostream>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -37,10 +41,14 @@ inline void check(int status, const char *where) {
   throw std::runtime_error(msg);
 }
 
-// inc/simuClass.hpp:7-296 (the members the hot path reads or writes)
+// inc/simuClass.hpp:7-296 (the members the hot path and its output read or write)
 struct simuClass {
   double Re = 0.0, nu = 0.0, dt = 0.0, time = 0.0, time_max = 0.0;
   size_t loop = 1, loop_max = 0;
+  int delayIO = 100;        // Delay_IO_CD_CL ring-buffer length
+  size_t currentFile = 0;   // Plot3D solution file number
+  std::string zone = "ELL (BICG)";  // ZONE(): the pressure solver's name
+  std::shared_ptr<cfd_cdcl_log> cdcl;  // the cd / cl ring buffers and their file
   int iters = 0;
   double error = 0.0;
   double p_criteria = 1.e-3;
@@ -52,7 +60,25 @@ struct simuClass {
   bool set_dt(double t) { dt = t; loop_max = (size_t)(time_max / dt); return true; }
   bool get_finishloop() const { return loop < loop_max || time < time_max; }
   bool finishloop() { time += dt; loop++; return true; }
+  std::string ZONE() const { return zone; }
+  // the Time_cDcL.dat log of this run, opened on first use
+  void cdcl_record(const char *path, int mode, std::pair<double, double> cdl) {
+    if (!cdcl) {
+      cfd_cdcl_log *h = nullptr;
+      check(cfd_cdcl_open(path, delayIO, zone.c_str(), mode, &h), "cfd_cdcl_open");
+      cdcl.reset(h, [](cfd_cdcl_log *q) { cfd_cdcl_close(q); });
+    }
+    check(cfd_cdcl_record(cdcl.get(), (int)loop, time, cdl.first, cdl.second), "cfd_cdcl_record");
+  }
 };
+
+// simuClass::ZONE (simuClass.hpp:203-236) for a parameter set
+inline std::string ZONE(const cfd_params &p) {
+  if (p.solver == CFD_SOLVER_RBSOR) return "SOR";
+  if (p.op == CFD_OP_CSR) return "CSR (BICG)";
+  if (p.op == CFD_OP_SPE) return "SPE (BICG)";
+  return "ELL (BICG)";
+}
 
 // inc/physicalVariables.hpp:7-169
 struct velocity {
@@ -114,6 +140,11 @@ class grid {
     return ((size_t)(i - gC) * nyCal + (j - gC)) * nzCal + (k - gC);
   }
   cfd_ctx *ctx() const { return ctx_; }
+  // the C-ABI view of the metrics (host pointers into this grid)
+  cfd_grid_desc desc() const {
+    return cfd_grid_desc{nxCal,     nyCal,     nzCal,     X.data(), Y.data(), Z.data(), Dx.data(), Dy.data(),
+                         Dz.data(), Dxs.data(), Dys.data(), Dzs.data(), lx,     ly,       lz};
+  }
   calDomain domain() const {
     calDomain d;
     d.i_endof = nx - gC; d.j_endof = ny - gC; d.k_endof = nz - gC;
@@ -172,8 +203,7 @@ class grid {
     Dxs.assign(nx, 0.0); Dys.assign(ny, 0.0); Dzs.assign(nz, 0.0);
   }
   void create(int device) {
-    cfd_grid_desc d{nxCal, nyCal, nzCal, X.data(), Y.data(), Z.data(), Dx.data(), Dy.data(), Dz.data(),
-                    Dxs.data(), Dys.data(), Dzs.data(), lx, ly, lz};
+    const cfd_grid_desc d = desc();
     check(cfd_create(&d, &prm, device, &ctx_), "cfd_create");
   }
   void release() {
@@ -276,6 +306,55 @@ inline std::tuple<double, double, double> CheckSteadyStatebyMaxVal_omp(simuClass
   double m[3];
   check(cfd_maxdiff(g.ctx(), m), "cfd_maxdiff");
   return {m[0], m[1], m[2]};
+}
+
+// inc/dfib/virtualForceIntergrator.hpp:105-177: cD, cL of f into simuClass's
+// delayIO ring buffer, flushed to Information/Time_cDcL.dat every delayIO steps
+// (as written: each flush repeats the current line delayIO times)
+inline std::pair<double, double> Delay_IO_CD_CL(simuClass &simu, calDomain &Lo, DfibArray &Dfib, grid &g) {
+  auto cdl = virtualF_Int(Dfib, Lo, g);
+  simu.cdcl_record("Information/Time_cDcL.dat", CFD_CDCL_REFERENCE, cdl);
+  return cdl;
+}
+
+// inc/dfib/virtualForceIntergrator.hpp:44-99: one line per step
+inline std::pair<double, double> noDelay_IO_CD_CL(simuClass &simu, calDomain &Lo, DfibArray &Dfib, grid &g) {
+  auto cdl = virtualF_Int(Dfib, Lo, g);
+  std::cout << "[cD, cL] : " << cdl.first << ", " << cdl.second << std::endl;
+  simu.cdcl_record("Information/Time_cDcL.dat", CFD_CDCL_EVERY_STEP, cdl);
+  return cdl;
+}
+
+// inc/io/Plot3D/xfileWrite.hpp:6-61
+inline int OutputPlot3D_Xfile(simuClass &, grid &g) {
+  const cfd_grid_desc d = g.desc();
+  check(cfd_plot3d_write_x(&d, "mx_out/P3D.x"), "cfd_plot3d_write_x");
+  return 0;
+}
+
+// inc/io/Plot3D/qfile.hpp:7-104: mx_out/P3D<currentFile, two digits at least>.q
+inline int OutputPLOT3D_Qfile(DfibArray &Dfib, simuClass &simu, pressure &pres, velocity &currt, grid &g) {
+  const size_t io = simu.currentFile;
+  const std::string name = std::string("mx_out/P3D") + (io < 10 ? "0" : "") + std::to_string(io) + ".q";
+  std::cout << "(Output Plaot3D):" << io << std::endl;
+  g.put(CFD_P, pres.p);
+  g.put(CFD_T3_U, currt);
+  g.put(CFD_ETA, Dfib.eta);
+  check(cfd_plot3d_write_q(g.ctx(), name.c_str(), CFD_T3_U, simu.time), "cfd_plot3d_write_q");
+  return 0;
+}
+
+// inc/io/Plot3D/qfileRead.hpp:6-100: (Nblock, mach, alpha, reyn, time, p, u, v, w, eta)
+inline std::tuple<int, float, float, float, float, std::vector<double>, std::vector<double>, std::vector<double>,
+                  std::vector<double>, std::vector<double>>
+QfileRead(grid &g, const std::string &filename) {
+  std::vector<double> p(g.iceltot), u(g.iceltot), v(g.iceltot), w(g.iceltot), eta(g.iceltot);
+  int nb = 0;
+  float h[4] = {0, 0, 0, 0};
+  const cfd_grid_desc d = g.desc();
+  check(cfd_plot3d_read_q(filename.c_str(), &d, &nb, h, p.data(), u.data(), v.data(), w.data(), eta.data()),
+        "cfd_plot3d_read_q");
+  return {nb, h[0], h[1], h[2], h[3], std::move(p), std::move(u), std::move(v), std::move(w), std::move(eta)};
 }
 
 namespace mat {

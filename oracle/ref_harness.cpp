@@ -211,11 +211,86 @@ static int synth_main() {
   return 0;
 }
 
+// usage: ref_harness OUTDIR io -- the output writers and the Plot3D reader on
+// seeded fields of the native grid, run inside OUTDIR (the writers use the
+// relative paths mx_out/ and Information/):
+//   mx_out/P3D.x, mx_out/P3D01.q   OutputPlot3D_Xfile, OutputPLOT3D_Qfile
+//                                  (p tag 20, T3 tags 21-23, eta of the cylinder,
+//                                  Re 40, time 0.25, currentFile 1)
+//   read_{p,u,v,w,eta}.bin, read_header.txt   QfileRead of that .q file
+//   cdcl_delay.dat      Delay_IO_CD_CL, delayIO 3, loops 1..7
+//   cdcl_nodelay.dat    noDelay_IO_CD_CL, loops 1..4
+//   (f of loop L: tag 30 + L, amplitude 1; time advanced by dt = 0.002 per loop)
+static int io_main() {
+  simuClass simu;
+  grid gA;
+  simu.set_time_max(40);
+  simu.set_dt(0.002);
+  simu.set_Re(40.0);
+  simu.Locality = 1;
+  simu.DfibMethod = "DFIB_Cylinder-Z";
+  DfibArray Dfib;
+  pressure t1;
+  velocity T0, T1, T3;
+  shareMenory ShareM;
+  calDomain dom;
+  std::vector<int> grid_size{gA.nx, gA.ny, gA.nz};
+  std::vector<int> dims{1, 1, 1};
+  dom.initTable(grid_size, dims);
+  dom.initLocal(0, 0, 0);
+  resize_variable(gA, t1, T0, T1, T3, Dfib);
+  generateGride(simu, ShareM, dom, gA);
+  std::fill(Dfib.eta.begin(), Dfib.eta.end(), 0.0);
+  DFIB_CylinderZ(Dfib, dom, gA);
+  const uint64_t seed = 20240611ULL;
+  fill_noise(t1.p, seed, 20, 0.0, 1.0);
+  fill_noise(T3.u, seed, 21, 1.0, 0.3);
+  fill_noise(T3.v, seed, 22, 0.0, 0.3);
+  fill_noise(T3.w, seed, 23, 0.0, 0.3);
+  if (chdir(g_out.c_str()) != 0) return 1;
+  mkdir("mx_out", 0755);
+  mkdir("Information", 0755);
+  OutputPlot3D_Xfile(simu, gA);
+  simu.currentFile = 1;
+  simu.set_time(0.25);
+  OutputPLOT3D_Qfile(Dfib, simu, t1, T3, gA);
+  {
+    auto [nb, mach, alpha, reyn, time, p, u, v, w, eta] = QfileRead(gA, "mx_out/P3D01.q");
+    g_out = ".";
+    dump("read_p.bin", p);
+    dump("read_u.bin", u);
+    dump("read_v.bin", v);
+    dump("read_w.bin", w);
+    dump("read_eta.bin", eta);
+    std::ofstream h("read_header.txt");
+    h << std::setprecision(9) << nb << " " << mach << " " << alpha << " " << reyn << " " << time << "\n";
+  }
+  simu.delayIO = 3;
+  simu.set_time(0.0);
+  for (simu.loop = 1; simu.loop <= 7; simu.finishloop()) {
+    fill_noise(Dfib.f, seed, 30 + (int)simu.loop, 0.0, 1.0);
+    Delay_IO_CD_CL(simu, dom, Dfib, gA);
+  }
+  if (rename("Information/Time_cDcL.dat", "cdcl_delay.dat") != 0) return 1;
+  simu.set_time(0.0);
+  for (simu.loop = 1; simu.loop <= 4; simu.finishloop()) {
+    fill_noise(Dfib.f, seed, 30 + (int)simu.loop, 0.0, 1.0);
+    noDelay_IO_CD_CL(simu, dom, Dfib, gA);
+  }
+  if (rename("Information/Time_cDcL.dat", "cdcl_nodelay.dat") != 0) return 1;
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc >= 3 && std::string(argv[2]) == "synth") {
     g_out = argv[1];
     mkdir(g_out.c_str(), 0755);
     return synth_main();
+  }
+  if (argc >= 3 && std::string(argv[2]) == "io") {
+    g_out = argv[1];
+    mkdir(g_out.c_str(), 0755);
+    return io_main();
   }
   if (argc < 6) {
     std::cerr << "usage: ref_harness OUTDIR NSTEPS TOL OP(ell|spe) FN_STEP\n";

@@ -2,8 +2,14 @@
 // :287-627) written against cfd_reference_shim.hpp -- host vectors, one
 // reference function per call -- must reproduce the device-resident fused
 // timestep (grid::step, cfd_step) on the same inputs.
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <string>
 #include <vector>
 
 #include "cfd_reference_shim.hpp"
@@ -138,6 +144,33 @@ int main() {
   }
   std::printf("CSR: nnz %zu product %s, solver on another operator %s\n", cidx.size(), csr_ok ? "exact" : "DIFFERS",
               op_threw ? "refused" : "NOT refused");
+  // output through the reference-shaped calls, in a scratch directory: the
+  // grid and solution files, QfileRead of the latter, and the drag history
+  char tmpl[] = "/tmp/cfd_shim_io_XXXXXX";
+  bool io_ok = mkdtemp(tmpl) != nullptr && chdir(tmpl) == 0 && mkdir("mx_out", 0755) == 0 &&
+               mkdir("Information", 0755) == 0;
+  if (io_ok) {
+    OutputPlot3D_Xfile(simu, gA);
+    simu.currentFile = 3;
+    OutputPLOT3D_Qfile(Dfib, simu, t1, T3, gA);
+    auto [nb, mach, alpha, reyn, qtime, qp, qu, qv, qw, qeta] = QfileRead(gA, "mx_out/P3D03.q");
+    (void)mach; (void)alpha; (void)qu; (void)qv; (void)qw;
+    io_ok = nb == 1 && reyn == (float)prm.Re && qtime == (float)simu.time;
+    for (int i = dom.i_begin; i < dom.i_endof && io_ok; ++i)
+      for (int j = dom.j_begin; j < dom.j_endof; ++j)
+        for (int k = dom.k_begin; k < dom.k_endof; ++k) {
+          const size_t m = gA.icel(i, j, k);
+          io_ok = io_ok && qp[m] == (double)(float)t1.p[m] && qeta[m] == (double)(float)Dfib.eta[m];
+        }
+    simu.delayIO = 2;
+    simu.zone = ZONE(prm);
+    for (simu.loop = 1; simu.loop <= 4; ++simu.loop) Delay_IO_CD_CL(simu, dom, Dfib, gA);
+    std::ifstream h("Information/Time_cDcL.dat");
+    int lines = 0;
+    for (std::string l; std::getline(h, l);) ++lines;
+    io_ok = io_ok && lines == 3 + 4;  // header, then two flushes of two lines
+  }
+  std::printf("output files: %s\n", io_ok ? "ok" : "FAILED");
   // an out-of-range argument surfaces as the reference's exception type
   bool threw = false;
   try {
@@ -145,7 +178,7 @@ int main() {
   } catch (const std::invalid_argument &) {
     threw = true;
   }
-  const bool ok = eu < 1e-11 && ev < 1e-11 && ew < 1e-11 && ex < 1e-8 && threw && csr_ok && op_threw;
+  const bool ok = eu < 1e-11 && ev < 1e-11 && ew < 1e-11 && ex < 1e-8 && threw && csr_ok && op_threw && io_ok;
   std::printf("%s\n", ok ? "PASS" : "FAIL");
   return ok ? 0 : 1;
 }

@@ -6,6 +6,8 @@ case (190x210x40 SML grid, cavity table, Re 40, dt 0.002) and records:
   * SHA-256 of the grid metrics, the IB mask, the ELL operator (columns,
     values), gA.jp, the SPE Jacobi scaling and the CSR operator (row
     pointers, columns, values and its Jacobi scaling);
+  * the output writers (Plot3D grid and solution files, Time_cDcL.dat in
+    both of its modes) and the Plot3D reader on seeded fields;
   * for every hot-path function applied once to seeded inputs (splitmix64
     noise, reproduced bit for bit by the oracle's orc_fill_noise): SHA-256 of
     its output, the L2 norm, and 256 seeded sample values;
@@ -58,7 +60,8 @@ def main():
     subprocess.run([h, str(DUMPS / "native"), "3", "1e-3", "ell", "2"], check=True, env=env,
                    stdout=subprocess.DEVNULL)
     subprocess.run([h, str(DUMPS / "synth"), "synth"], check=True, env=env, stdout=subprocess.DEVNULL)
-    nat, syn = DUMPS / "native", DUMPS / "synth"
+    subprocess.run([h, str(DUMPS / "io"), "io"], check=True, env=env, stdout=subprocess.DEVNULL)
+    nat, syn, io = DUMPS / "native", DUMPS / "synth", DUMPS / "io"
     fx = {"case": "native 190x210x40 SML, BC_Cavity_A, Re 40, dt 0.002, cylinder (6.5, 6) R 0.5",
           "seed": 20240611, "setup": {}, "synth": {}, "scalars": {}, "steps": []}
     for name in ("grid.bin", "eta.bin", "ell_col.bin", "ell_val.bin", "jp.bin", "spe_jp.bin", "csr_ptr.bin",
@@ -80,6 +83,14 @@ def main():
                                 "cd": float(v[4]), "cl": float(v[5])})
         elif k == "maxdiff":
             fx["steps"].append({"step": int(v[0]), "maxdiff": [float(x) for x in v[1:]]})
+    # output files: SHA-256 of the Plot3D files and of what QfileRead returns,
+    # the reader's header, and the two Time_cDcL.dat texts verbatim
+    fx["io"] = {"P3D.x": sha(io / "mx_out" / "P3D.x"), "P3D01.q": sha(io / "mx_out" / "P3D01.q"),
+                "read_header": (io / "read_header.txt").read_text().split(),
+                "cdcl_delay": (io / "cdcl_delay.dat").read_text(),
+                "cdcl_nodelay": (io / "cdcl_nodelay.dat").read_text()}
+    for q in "p u v w eta".split():
+        fx["io"][f"read_{q}"] = sha(io / f"read_{q}.bin")
     OUT.write_text(json.dumps(fx, indent=0))
     print(f"wrote {OUT} ({OUT.stat().st_size // 1024} KiB)")
 
