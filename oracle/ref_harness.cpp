@@ -20,7 +20,7 @@
 // runSeqOmp_predictionMethod (inc/run/RUN_CPU_PC_METHOD.hpp:287-627); it
 // restates only the sequencing, never the arithmetic.
 //
-// usage: ref_harness OUTDIR NSTEPS TOL OP(ell|spe) FN_STEP
+// usage: ref_harness OUTDIR NSTEPS TOL OP(ell|spe) FN_STEP [DUMP_EVERY]
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -293,9 +293,11 @@ int main(int argc, char **argv) {
     return io_main();
   }
   if (argc < 6) {
-    std::cerr << "usage: ref_harness OUTDIR NSTEPS TOL OP(ell|spe) FN_STEP\n";
+    std::cerr << "usage: ref_harness OUTDIR NSTEPS TOL OP(ell|spe) FN_STEP [DUMP_EVERY]\n";
     return 2;
   }
+  // state dumps every DUMP_EVERY steps (default 1; 0: statistics only)
+  const int dump_every = argc >= 7 ? std::atoi(argv[6]) : 1;
   g_out = argv[1];
   const int nsteps = std::atoi(argv[2]);
   const double tol = std::atof(argv[3]);
@@ -455,12 +457,14 @@ int main(int argc, char **argv) {
     BC_staggered_main(dom, T0, t1, gA);
     BC_staggered_copy(dom, T0, T1, gA);
     if (fn) { dump_vel("fn_bc_out_T0", T0); dump("fn_bc_out_p.bin", t1.p); dump_vel("fn_bccopy_out_T1", T1); }
-    std::snprintf(tag, sizeof(tag), "step_%03d", (int)simu.loop);
-    dump_vel(std::string(tag) + "_T0", T0);
-    dump_vel(std::string(tag) + "_T1", T1);
-    dump(std::string(tag) + "_p.bin", t1.p);
-    dump(std::string(tag) + "_X.bin", Mx.X_result);
-    dump(std::string(tag) + "_f.bin", Dfib.f);
+    if (dump_every > 0 && (int)simu.loop % dump_every == 0) {
+      std::snprintf(tag, sizeof(tag), "step_%03d", (int)simu.loop);
+      dump_vel(std::string(tag) + "_T0", T0);
+      dump_vel(std::string(tag) + "_T1", T1);
+      dump(std::string(tag) + "_p.bin", t1.p);
+      dump(std::string(tag) + "_X.bin", Mx.X_result);
+      dump(std::string(tag) + "_f.bin", Dfib.f);
+    }
     stats.flush();
   }
   return 0;
