@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(256) dfib_adj_kernel(DevGrid g, double dt, dou
   block_partials<1>(acc, part);
 }
 
-__global__ void __launch_bounds__(256) gain_finalize(const double *part, int nblk, double *gacc) {
+__global__ void __launch_bounds__(FIN_THREADS) gain_finalize(const double *part, int nblk, double *gacc) {
   double tot[1];
   combine_partials<1>(part, nblk, tot);
   if (threadIdx.x == 0) gacc[0] += tot[0];
@@ -614,7 +614,7 @@ int adjoint_step(Ctx &c, AdjState &A, const double *x, const double *t2, bool so
                                               t2 + (size_t)q * 3 * N, last ? A.lx : nullptr,
                                               last ? A.lx + G : nullptr, last ? A.lx + 2 * G : nullptr, A.Lam,
                                               A.lt2, part);
-    gain_finalize<<<1, 256, 0, c.stream>>>(part, nb, A.gacc);
+    gain_finalize<<<1, FIN_THREADS, 0, c.stream>>>(part, nb, A.gacc);
     ADJ_HIP(hipGetLastError());
     if (solved) {
       lx_gather_kernel<<<blocks(N), 256, 0, c.stream>>>(g, P.dt, A.lt2, A.lX);

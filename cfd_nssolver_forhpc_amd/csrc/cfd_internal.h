@@ -16,6 +16,9 @@ namespace cfd {
 constexpr int GC = 2;           // ghost layers per side (grid::gC)
 constexpr int KSLOTS = 7;       // ELL slots per row (createPressureMatrix resize(N, 7))
 constexpr int RED_THREADS = 256;
+// threads of the one-block kernels that combine the partials: 16 waves keep
+// enough loads in flight to read 32768-65536 partials in a few microseconds
+constexpr int FIN_THREADS = 1024;
 // upper bound of partial sums per reduction: one row per thread up to 8.4M
 // rows (256-thread blocks), which streams fastest on gfx950; larger vectors
 // fall back to a grid-stride loop over this many blocks

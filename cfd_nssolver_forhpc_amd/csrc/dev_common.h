@@ -50,11 +50,11 @@ __device__ inline void block_partials(double (&v)[NV], double *part) {
 }
 
 // Combine nblk partials per value in a fixed order; called by all
-// RED_THREADS threads of a single block; the totals are returned to every
+// FIN_THREADS threads of a single block; the totals are returned to every
 // thread.
 template <int NV, bool MAX = false>
 __device__ inline void combine_partials(const double *part, int nblk, double (&tot)[NV]) {
-  __shared__ double lds[NV][RED_THREADS / 64];
+  __shared__ double lds[NV][FIN_THREADS / 64];
   double v[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
@@ -62,7 +62,7 @@ __device__ inline void combine_partials(const double *part, int nblk, double (&t
     if (MAX) {
       double s = 0.0;
       bool first = true;
-      for (int b = threadIdx.x; b < nblk; b += RED_THREADS) {
+      for (int b = threadIdx.x; b < nblk; b += FIN_THREADS) {
         const double x = pq[b];
         s = first ? x : fmax(s, x);
         first = false;
@@ -73,11 +73,11 @@ __device__ inline void combine_partials(const double *part, int nblk, double (&t
       // thread; the combination order is fixed, so the sum is deterministic
       double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
       int b = threadIdx.x;
-      for (; b + 7 * RED_THREADS < nblk; b += 8 * RED_THREADS) {
+      for (; b + 7 * FIN_THREADS < nblk; b += 8 * FIN_THREADS) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) s[u] += pq[b + u * RED_THREADS];
+        for (int u = 0; u < 8; ++u) s[u] += pq[b + u * FIN_THREADS];
       }
-      for (; b < nblk; b += RED_THREADS) s[0] += pq[b];
+      for (; b < nblk; b += FIN_THREADS) s[0] += pq[b];
       v[q] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
     }
   }
@@ -92,7 +92,7 @@ __device__ inline void combine_partials(const double *part, int nblk, double (&t
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
     double s = lds[q][0];
-    for (int w = 1; w < RED_THREADS / 64; ++w) s = MAX ? fmax(s, lds[q][w]) : s + lds[q][w];
+    for (int w = 1; w < FIN_THREADS / 64; ++w) s = MAX ? fmax(s, lds[q][w]) : s + lds[q][w];
     tot[q] = s;
   }
   __syncthreads();

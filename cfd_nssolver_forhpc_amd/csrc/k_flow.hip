@@ -552,7 +552,7 @@ __global__ void __launch_bounds__(256) maxdiff_kernel(DevGrid g, const double *_
 }
 
 template <int NV, bool MAX>
-__global__ void __launch_bounds__(256) finalize_kernel(const double *part, int nblk, double *out) {
+__global__ void __launch_bounds__(FIN_THREADS) finalize_kernel(const double *part, int nblk, double *out) {
   double tot[NV];
   combine_partials<NV, MAX>(part, nblk, tot);
   if (threadIdx.x == 0)
@@ -651,11 +651,11 @@ hipError_t launch_dfib_update_into(Ctx &c, int mode, const double *p_or_x, doubl
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (mode & UPD_DRAG) {
-    finalize_kernel<2, false><<<1, 256, 0, c.stream>>>(red_part_host(c, RS_DRAG), nblk, drag_out);
+    finalize_kernel<2, false><<<1, FIN_THREADS, 0, c.stream>>>(red_part_host(c, RS_DRAG), nblk, drag_out);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if (mode & UPD_MAXDIFF) {
-    finalize_kernel<3, true><<<1, 256, 0, c.stream>>>(red_part_host(c, RS_MAX), nblk, max_out);
+    finalize_kernel<3, true><<<1, FIN_THREADS, 0, c.stream>>>(red_part_host(c, RS_MAX), nblk, max_out);
     e = hipGetLastError();
   }
   return e;
@@ -691,7 +691,7 @@ hipError_t launch_drag(Ctx &c) {
   drag_kernel<<<nblk, 256, 0, c.stream>>>(c.g, c.f[CFD_F], red_part_host(c, RS_DRAG));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  finalize_kernel<2, false><<<1, 256, 0, c.stream>>>(red_part_host(c, RS_DRAG), nblk, c.red.out + RS_DRAG * 4);
+  finalize_kernel<2, false><<<1, FIN_THREADS, 0, c.stream>>>(red_part_host(c, RS_DRAG), nblk, c.red.out + RS_DRAG * 4);
   return hipGetLastError();
 }
 
@@ -701,7 +701,7 @@ hipError_t launch_maxdiff(Ctx &c) {
                                              c.f[CFD_T3_V], c.f[CFD_T3_W], red_part_host(c, RS_MAX));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  finalize_kernel<3, true><<<1, 256, 0, c.stream>>>(red_part_host(c, RS_MAX), nblk, c.red.out + RS_MAX * 4);
+  finalize_kernel<3, true><<<1, FIN_THREADS, 0, c.stream>>>(red_part_host(c, RS_MAX), nblk, c.red.out + RS_MAX * 4);
   return hipGetLastError();
 }
 

@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(256) init_kernel(OpArgs a, const T *__restrict
   block_partials<2>(acc, part);
 }
 
-__global__ void __launch_bounds__(256) init_finalize(const double *part, int nblk, KrylovScalars *ks) {
+__global__ void __launch_bounds__(FIN_THREADS) init_finalize(const double *part, int nblk, KrylovScalars *ks) {
   double tot[2];
   combine_partials<2>(part, nblk, tot);
   if (threadIdx.x == 0) {
@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(256) spmv_dot_kernel(OpArgs a, const KrylovSca
   block_partials<1>(acc, part);
 }
 
-__global__ void __launch_bounds__(256) alpha_finalize(const double *part, int nblk, KrylovScalars *ks) {
+__global__ void __launch_bounds__(FIN_THREADS) alpha_finalize(const double *part, int nblk, KrylovScalars *ks) {
   if (ks->done) return;
   double tot[1];
   combine_partials<1>(part, nblk, tot);
@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(256) spmv_dot2_kernel(OpArgs a, const KrylovSc
   block_partials<2>(acc, part);
 }
 
-__global__ void __launch_bounds__(256) omega_finalize(const double *part, int nblk, KrylovScalars *ks) {
+__global__ void __launch_bounds__(FIN_THREADS) omega_finalize(const double *part, int nblk, KrylovScalars *ks) {
   if (ks->done) return;
   double tot[2];
   combine_partials<2>(part, nblk, tot);
@@ -181,7 +181,7 @@ __global__ void __launch_bounds__(256) xr_update_kernel(int64_t N, const KrylovS
 }
 
 // stop test, beta and the restart rule (bicgstabRe2.hpp:137-183)
-__global__ void __launch_bounds__(256) conv_finalize(const double *part, int nblk, KrylovScalars *ks) {
+__global__ void __launch_bounds__(FIN_THREADS) conv_finalize(const double *part, int nblk, KrylovScalars *ks) {
   if (ks->done) return;
   double tot[2];
   combine_partials<2>(part, nblk, tot);
@@ -276,7 +276,7 @@ __global__ void __launch_bounds__(256) nan_reset_kernel(int64_t N, double *__res
 }
 
 template <int NV>
-__global__ void __launch_bounds__(256) sum_finalize(const double *part, int nblk, double *out) {
+__global__ void __launch_bounds__(FIN_THREADS) sum_finalize(const double *part, int nblk, double *out) {
   double tot[NV];
   combine_partials<NV>(part, nblk, tot);
   if (threadIdx.x == 0)
@@ -329,12 +329,12 @@ hipError_t launch_iters(Ctx &c, const OpArgs &a, const Vecs<T> &v, KrylovScalars
   double *p1 = red_part_host(c, RS_SPMV_DOT), *p2 = red_part_host(c, RS_SPMV_DOT2), *p3 = red_part_host(c, RS_XR);
   for (int it = 0; it < nit; ++it) {
     spmv_dot_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, ks, v.p1, v.ap, v.r0, p1);
-    alpha_finalize<<<1, 256, 0, c.stream>>>(p1, nb, ks);
+    alpha_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p1, nb, ks);
     s_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.r1, v.ap, v.s1);
     spmv_dot2_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, ks, v.s1, v.as, p2);
-    omega_finalize<<<1, 256, 0, c.stream>>>(p2, nb, ks);
+    omega_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p2, nb, ks);
     xr_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.x, v.p1, v.s1, v.as, v.r1, v.r0, p3);
-    conv_finalize<<<1, 256, 0, c.stream>>>(p3, nb, ks);
+    conv_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p3, nb, ks);
     p_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.r1, v.p1, v.ap, v.r0);
   }
   return hipGetLastError();
@@ -352,7 +352,7 @@ int solve_bicg(Ctx &c, const OpArgs &a, const Vecs<T> &v, KrylovScalars *ks, int
   ks_setup_kernel<<<1, 1, 0, c.stream>>>(ks, tol, 1e-7 * tol, maxit, timestep);
   init_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, v.b, v.x, v.r0, v.r1, v.p1, v.px1, v.px2,
                                                       red_part_host(c, RS_INIT));
-  init_finalize<<<1, 256, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, ks);
+  init_finalize<<<1, FIN_THREADS, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, ks);
   CFD_HIP(hipGetLastError());
   int batch = 2;
   double prev_rel = -1.0;
@@ -380,7 +380,7 @@ int solve_bicg(Ctx &c, const OpArgs &a, const Vecs<T> &v, KrylovScalars *ks, int
   if (h.hit_maxit && v.px1) {
     double *part = red_part_host(c, RS_MISC);
     nan_reset_kernel<<<nb, 256, 0, c.stream>>>(N, (double *)v.x, v.px1, part);
-    sum_finalize<1><<<1, 256, 0, c.stream>>>(part, nb, c.red.out + RS_MISC * 4);
+    sum_finalize<1><<<1, FIN_THREADS, 0, c.stream>>>(part, nb, c.red.out + RS_MISC * 4);
     CFD_HIP(hipMemcpyAsync(c.hred, c.red.out + RS_MISC * 4, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     CFD_HIP(hipStreamSynchronize(c.stream));
     if (c.hred[0] > (double)N * 0.1)
@@ -433,13 +433,13 @@ int launch_pc_iters(Ctx &c, const OpArgs &a, const PVecs<T> &v, KrylovScalars *k
     int rc = precondition<T>(c, v.p1, v.ph, ks, pc32);
     if (rc) return rc;
     spmv_dot_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, ks, v.ph, v.v, v.r0, p1);
-    alpha_finalize<<<1, 256, 0, c.stream>>>(p1, nb, ks);
+    alpha_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p1, nb, ks);
     s_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.r1, v.v, v.s1);
     if ((rc = precondition<T>(c, v.s1, v.sh, ks, pc32))) return rc;
     spmv_dot2_pc_kernel<T><<<nb, 256, 0, c.stream>>>(a, ks, v.sh, v.s1, v.t, p2);
-    omega_finalize<<<1, 256, 0, c.stream>>>(p2, nb, ks);
+    omega_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p2, nb, ks);
     xr_update_pc_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.x, v.ph, v.sh, v.s1, v.t, v.r1, v.r0, p3);
-    conv_finalize<<<1, 256, 0, c.stream>>>(p3, nb, ks);
+    conv_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p3, nb, ks);
     p_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.r1, v.p1, v.v, v.r0);
     CFD_HIP(hipGetLastError());
   }
@@ -454,7 +454,7 @@ int solve_pbicg(Ctx &c, const OpArgs &a, const PVecs<T> &v, KrylovScalars *ks, d
   ks_setup_kernel<<<1, 1, 0, c.stream>>>(ks, tol, 0.0, maxit, 0);
   init_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, v.b, v.x, v.r0, v.r1, v.p1, nullptr, nullptr,
                                                       red_part_host(c, RS_INIT));
-  init_finalize<<<1, 256, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, ks);
+  init_finalize<<<1, FIN_THREADS, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, ks);
   CFD_HIP(hipGetLastError());
   CFD_HIP(hipMemcpyAsync(c.hks, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c.stream));
   CFD_HIP(hipStreamSynchronize(c.stream));
@@ -543,7 +543,7 @@ int defect_correction(Ctx &c, const double *b, double *x, double tol, double tol
   for (;;) {
     double *part = red_part_host(c, RS_NORM);
     dc_residual_kernel<<<nb, 256, op_lds_bytes(a64), c.stream>>>(a64, b, x, r32, part);
-    sum_finalize<2><<<1, 256, 0, c.stream>>>(part, nb, c.red.out + RS_NORM * 4);
+    sum_finalize<2><<<1, FIN_THREADS, 0, c.stream>>>(part, nb, c.red.out + RS_NORM * 4);
     CFD_HIP(hipGetLastError());
     CFD_HIP(hipMemcpyAsync(c.hred, c.red.out + RS_NORM * 4, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     CFD_HIP(hipStreamSynchronize(c.stream));
@@ -593,7 +593,7 @@ hipError_t launch_bicg_timing_setup(Ctx &c) {
   const OpArgs a = op_args(c, c.prm.op);
   init_kernel<double><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, c.f[CFD_RHS], c.f[CFD_VEC_B], c.r0, c.r1, c.p1,
                                                               nullptr, nullptr, red_part_host(c, RS_INIT));
-  init_finalize<<<1, 256, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, c.ks32);
+  init_finalize<<<1, FIN_THREADS, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, c.ks32);
   return hipGetLastError();
 }
 
@@ -640,7 +640,7 @@ __global__ void __launch_bounds__(256) mr_init_kernel(OpArgs a64, const double *
   block_partials<3>(acc, part);
 }
 
-__global__ void __launch_bounds__(256) mr_init_finalize(const double *part, int nblk, KrylovScalars *ks) {
+__global__ void __launch_bounds__(FIN_THREADS) mr_init_finalize(const double *part, int nblk, KrylovScalars *ks) {
   double tot[3];
   combine_partials<3>(part, nblk, tot);
   if (threadIdx.x == 0) {
@@ -655,7 +655,7 @@ __global__ void __launch_bounds__(256) mr_init_finalize(const double *part, int 
   }
 }
 
-__global__ void __launch_bounds__(256) mr_alpha_finalize(const double *part, int nblk, KrylovScalars *ks) {
+__global__ void __launch_bounds__(FIN_THREADS) mr_alpha_finalize(const double *part, int nblk, KrylovScalars *ks) {
   if (ks->done) return;
   double tot[1];
   combine_partials<1>(part, nblk, tot);
@@ -671,7 +671,7 @@ __global__ void __launch_bounds__(256) mr_alpha_finalize(const double *part, int
   }
 }
 
-__global__ void __launch_bounds__(256) mr_omega_finalize(const double *part, int nblk, KrylovScalars *ks) {
+__global__ void __launch_bounds__(FIN_THREADS) mr_omega_finalize(const double *part, int nblk, KrylovScalars *ks) {
   if (ks->done || ks->replace) return;
   double tot[2];
   combine_partials<2>(part, nblk, tot);
@@ -687,7 +687,7 @@ __global__ void __launch_bounds__(256) mr_omega_finalize(const double *part, int
 
 // recurrence residual test: trigger a replacement on a `delta` drop, on the
 // stop criterion, or at the iteration cap; otherwise beta and the restart rule
-__global__ void __launch_bounds__(256) mr_conv_finalize(const double *part, int nblk, KrylovScalars *ks) {
+__global__ void __launch_bounds__(FIN_THREADS) mr_conv_finalize(const double *part, int nblk, KrylovScalars *ks) {
   if (ks->done || ks->replace) return;
   double tot[2];
   combine_partials<2>(part, nblk, tot);
@@ -741,7 +741,7 @@ __global__ void __launch_bounds__(256) mr_replace_kernel(OpArgs a64, const Krylo
   block_partials<2>(acc, part);
 }
 
-__global__ void __launch_bounds__(256) mr_replace_finalize(const double *part, int nblk, KrylovScalars *ks) {
+__global__ void __launch_bounds__(FIN_THREADS) mr_replace_finalize(const double *part, int nblk, KrylovScalars *ks) {
   if (ks->done || !ks->replace) return;
   double tot[2];
   combine_partials<2>(part, nblk, tot);
@@ -858,7 +858,7 @@ int reliable_solve(Ctx &c, const double *b, double *x, double tol, double delta,
   set_delta_kernel<<<1, 1, 0, c.stream>>>(ks, delta);
   mr_init_kernel<T><<<nb, 256, op_lds_bytes(a64), c.stream>>>(a64, b, x, r1, r0, p1, (T *)xl,
                                                                red_part_host(c, RS_INIT));
-  mr_init_finalize<<<1, 256, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, ks);
+  mr_init_finalize<<<1, FIN_THREADS, 0, c.stream>>>(red_part_host(c, RS_INIT), nb, ks);
   CFD_HIP(hipGetLastError());
   int batch = 4;
   double prev_rel = -1.0;
@@ -866,18 +866,18 @@ int reliable_solve(Ctx &c, const double *b, double *x, double tol, double delta,
   for (;;) {
     for (int it = 0; it < batch; ++it) {
       spmv_dot_kernel<T><<<nb, 256, op_lds_bytes(ain), c.stream>>>(ain, ks, p1, ap, r0, P1);
-      mr_alpha_finalize<<<1, 256, 0, c.stream>>>(P1, nb, ks);
+      mr_alpha_finalize<<<1, FIN_THREADS, 0, c.stream>>>(P1, nb, ks);
       mr_s_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, r1, ap, s1);
       mr_spmv_dot2_kernel<T><<<nb, 256, 0, c.stream>>>(ain, ks, s1, as, P2);
-      mr_omega_finalize<<<1, 256, 0, c.stream>>>(P2, nb, ks);
+      mr_omega_finalize<<<1, FIN_THREADS, 0, c.stream>>>(P2, nb, ks);
       if constexpr (std::is_same<T, float>::value)
         mr_xr_update_kernel<T, float><<<nb, 256, 0, c.stream>>>(N, ks, xl, p1, s1, as, r1, r0, P3);
       else
         mr_xr_update_kernel<T, double><<<nb, 256, 0, c.stream>>>(N, ks, x, p1, s1, as, r1, r0, P3);
-      mr_conv_finalize<<<1, 256, 0, c.stream>>>(P3, nb, ks);
+      mr_conv_finalize<<<1, FIN_THREADS, 0, c.stream>>>(P3, nb, ks);
       if constexpr (std::is_same<T, float>::value) mr_fold_kernel<<<nb, 256, 0, c.stream>>>(N, ks, x, xl);
       mr_replace_kernel<T><<<nb, 256, op_lds_bytes(a64), c.stream>>>(a64, ks, b, x, r1, r0, P4);
-      mr_replace_finalize<<<1, 256, 0, c.stream>>>(P4, nb, ks);
+      mr_replace_finalize<<<1, FIN_THREADS, 0, c.stream>>>(P4, nb, ks);
       mr_p_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, r1, p1, ap, r0);
     }
     CFD_HIP(hipGetLastError());

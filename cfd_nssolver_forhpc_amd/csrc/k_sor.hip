@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(256) permute_out_kernel(int64_t N, const int32
   if (r < N) a[r] = aP[perm[r]];
 }
 
-__global__ void __launch_bounds__(256) finalize2(const double *part, int nblk, double *out) {
+__global__ void __launch_bounds__(FIN_THREADS) finalize2(const double *part, int nblk, double *out) {
   double tot[2];
   combine_partials<2>(part, nblk, tot);
   if (threadIdx.x == 0) {
@@ -97,7 +97,7 @@ int rbsor_generic(Ctx &c, const TV *valP, const TX *bP, TX *xP, double omega, do
     it += chunk;
     double *part = red_part_host(c, RS_NORM);
     resid_kernel<TV, TX><<<nbr, 256, 0, c.stream>>>(N, c.colP, valP, bP, xP, part);
-    finalize2<<<1, 256, 0, c.stream>>>(part, nbr, c.red.out + RS_NORM * 4);
+    finalize2<<<1, FIN_THREADS, 0, c.stream>>>(part, nbr, c.red.out + RS_NORM * 4);
     CFD_HIP(hipGetLastError());
     CFD_HIP(hipMemcpyAsync(c.hred, c.red.out + RS_NORM * 4, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     CFD_HIP(hipStreamSynchronize(c.stream));
