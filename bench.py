@@ -192,12 +192,13 @@ def kernel_table(sim, reps):
     return rows
 
 
-def adjoint_benchmark(steps, avg_from, every, fd_dirs, precond, tol=1e-10, h=1e-6):
+def adjoint_benchmark(steps, avg_from, every, fd_dirs, precond, tol=1e-10, h=1e-6, gain=1.0):
     """Discrete adjoint on the 256x128x32 Re-100 cylinder (dt 1e-3): J = mean
     corrector cD over steps [avg_from, steps), gradient w.r.t. the seeded
     smoothed inflow profile (128 values) and the DFIB forcing gain; timed
     objective run (with checkpoints) and reverse sweep (with recomputation);
-    optionally central differences (h = 1e-6) in fd_dirs random directions."""
+    optionally central differences (h = 1e-6) in fd_dirs random directions,
+    about the base forcing gain `gain`."""
     from cfd_nssolver_forhpc_amd.adjoint import DragProblem, smoothed_inflow
     w = configs.config_d()
     w.params.precond = precond
@@ -210,7 +211,7 @@ def adjoint_benchmark(steps, avg_from, every, fd_dirs, precond, tol=1e-10, h=1e-
     uin = smoothed_inflow(w.grid.ncal[1])
     log(f"[bench] adjoint {w.name}: {steps} steps, J over [{avg_from}, {steps}), checkpoint every {every}")
     t0 = time.perf_counter()
-    J = pr.run(uin, 1.0)
+    J = pr.run(uin, gain)
     t_fwd = time.perf_counter() - t0
     t0 = time.perf_counter()
     g_uin, g_gain, st = sim.adjoint_backward()
@@ -221,8 +222,8 @@ def adjoint_benchmark(steps, avg_from, every, fd_dirs, precond, tol=1e-10, h=1e-
     for q in range(fd_dirs):
         d = rng.standard_normal(uin.size + 1)
         d /= np.linalg.norm(d)
-        Jp = pr.run(uin + h * d[:-1], 1.0 + h * d[-1])
-        Jm = pr.run(uin - h * d[:-1], 1.0 - h * d[-1])
+        Jp = pr.run(uin + h * d[:-1], gain + h * d[-1])
+        Jm = pr.run(uin - h * d[:-1], gain - h * d[-1])
         fd = (Jp - Jm) / (2 * h)
         ad = float(np.dot(g_uin, d[:-1]) + g_gain * d[-1])
         checks.append({"fd": fd, "adjoint": ad, "rel_diff": abs(fd - ad) / max(abs(fd), abs(ad), 1e-300)})
@@ -234,7 +235,7 @@ def adjoint_benchmark(steps, avg_from, every, fd_dirs, precond, tol=1e-10, h=1e-
         "J_mean_cd": J, "grad_uin_norm": float(np.linalg.norm(g_uin)), "grad_gain": g_gain,
         "forward_s": t_fwd, "reverse_sweep_s": t_bwd, "reverse_over_forward": t_bwd / t_fwd,
         "forward_ms_per_step": 1e3 * t_fwd / steps, "reverse_ms_per_step": 1e3 * t_bwd / steps,
-        "stats": st, "fd_step": h, "fd_checks": checks,
+        "stats": st, "fd_step": h, "fd_base_gain": gain, "fd_checks": checks,
     }
 
 
@@ -261,6 +262,8 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg and print its JSON")
     ap.add_argument("--adjoint-tol", type=float, default=1e-10, help="pressure tolerance of the adjoint run")
     ap.add_argument("--adjoint-h", type=float, default=1e-6, help="central-difference step of the gradient check")
+    ap.add_argument("--adjoint-gain", type=float, default=1.0,
+                    help="base DFIB forcing gain of the adjoint run and its central differences")
     ap.add_argument("--adjoint-jacobi", action="store_true",
                     help="solve the adjoint run's pressure systems with the reference's Jacobi scaling (default: MG)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -276,7 +279,8 @@ def main():
         steps = max(args.adjoint_steps, 2)
         pc = capi.CFD_PRECOND_JACOBI if args.adjoint_jacobi else capi.CFD_PRECOND_MG
         print(json.dumps({"adjoint": adjoint_benchmark(steps, steps // 2, args.adjoint_every, args.adjoint_fd, pc,
-                                                       args.adjoint_tol, args.adjoint_h)}), flush=True)
+                                                       args.adjoint_tol, args.adjoint_h, args.adjoint_gain)}),
+              flush=True)
         return
 
     w64 = configs.config_b(capi.CFD_FP64)
