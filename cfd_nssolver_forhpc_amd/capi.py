@@ -37,6 +37,9 @@ CFD_MIXED_VALUES32 = 2
 CFD_MIXED_PC32 = 3
 CFD_PRECOND_JACOBI = 0
 CFD_PRECOND_MG = 1
+CFD_CONV_QUICK = 0
+CFD_CONV_LUD = 1
+CFD_CONV_UD = 2
 CFD_CDCL_REFERENCE = 0
 CFD_CDCL_BUFFERED = 1
 CFD_CDCL_EVERY_STEP = 2
@@ -75,6 +78,7 @@ class CfdParams(C.Structure):
         ("ib_nsub", C.c_int),
         ("u_init", C.c_double * 3),
         ("precond", C.c_int), ("mg_nu", C.c_int),
+        ("convection", C.c_int),
     ]
 
     def set_bc(self, dir_table, num_table):

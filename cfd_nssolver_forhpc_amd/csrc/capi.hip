@@ -73,6 +73,8 @@ int check_params(const cfd_params *p) {
     return fail(CFD_ERR_ARG, "mixed_delta must be in (0, 1)");
   if (p->ib_on && (p->ib_nsub < 1 || !(p->ib_r > 0.0))) return fail(CFD_ERR_ARG, "invalid immersed body");
   if (p->precond != CFD_PRECOND_JACOBI && p->precond != CFD_PRECOND_MG) return fail(CFD_ERR_ARG, "unknown preconditioner");
+  if (p->convection != CFD_CONV_QUICK && p->convection != CFD_CONV_LUD && p->convection != CFD_CONV_UD)
+    return fail(CFD_ERR_ARG, "unknown convection scheme");
   if (p->precond == CFD_PRECOND_MG && (p->mg_nu < 1 || p->mg_nu > 8)) return fail(CFD_ERR_ARG, "mg_nu must be in [1, 8]");
   return CFD_OK;
 }
@@ -269,6 +271,7 @@ int cfd_params_default(cfd_params *out) {
   out->u_init[0] = 1.0;   // RUN_CPU_PC_METHOD.hpp:77
   out->precond = CFD_PRECOND_JACOBI;
   out->mg_nu = 2;
+  out->convection = CFD_CONV_QUICK;  // controlPanel.hpp:125
   return CFD_OK;
 }
 

@@ -35,17 +35,44 @@ __device__ inline TX ell_row(const int32_t *__restrict__ col, const TV *__restri
   return acc;
 }
 
+// n / d for a 32-bit unsigned n by an invariant divisor d >= 1: one
+// multiply-high, a subtract, an add and two shifts (Granlund & Montgomery,
+// "Division by invariant integers using multiplication", PLDI 1994, sec. 4).
+// m = floor(2^32 (2^l - d) / d) + 1 with l = ceil(log2 d).
+struct FastDiv {
+  uint32_t m;
+  int s1, s2;
+};
+
+inline FastDiv fastdiv_make(uint32_t d) {
+  int l = 0;
+  while ((1ull << l) < d) ++l;
+  FastDiv f;
+  f.m = (uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+  f.s1 = l > 0 ? 1 : 0;
+  f.s2 = l > 0 ? l - 1 : 0;
+  return f;
+}
+
+__device__ inline uint32_t fastdiv(uint32_t n, const FastDiv &f) {
+  const uint32_t t = __umulhi(f.m, n);
+  return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
 // SPE_matrix1::spmv_ompJP (inc/matrix/SPE_sparseMatrix1.hpp:126-144): the
 // Jacobi-scaled operator applied matrix-free; coef = xm[nxc] xp[nxc]
 // ym[nyc] yp[nyc] zm[nzc] zp[nzc]; neighbour terms in the order xm, ym, zm,
 // diagonal, zp, yp, xp, each contracted to an FMA like the reference build.
+// The row's (i, j, k) comes from two invariant divisions (N < 2^31).
 __device__ inline double spe_row(const double *__restrict__ coef, const double *__restrict__ jp, int nxc, int nyc,
-                                 int nzc, int64_t ic, const double *__restrict__ x) {
+                                 int nzc, const FastDiv &div_nz, const FastDiv &div_ny, int64_t ic,
+                                 const double *__restrict__ x) {
   const int64_t nynz = (int64_t)nyc * nzc;
-  const int i = (int)(ic / nynz);
-  const int64_t rem = ic - (int64_t)i * nynz;
-  const int j = (int)(rem / nzc);
-  const int k = (int)(rem - (int64_t)j * nzc);
+  const uint32_t q = fastdiv((uint32_t)ic, div_nz);  // i nyc + j
+  const int k = (int)((uint32_t)ic - q * (uint32_t)nzc);
+  const uint32_t i_ = fastdiv(q, div_ny);
+  const int j = (int)(q - i_ * (uint32_t)nyc);
+  const int i = (int)i_;
   const double *xm = coef, *xp = xm + nxc, *ym = xp + nxc, *yp = ym + nyc, *zm = yp + nyc, *zp = zm + nzc;
   const double J = jp[ic];
   double sum = 0.0;
@@ -79,6 +106,7 @@ struct OpArgs {
   const float *val32;
   const double *coef, *jp;   // SPE
   int nxc, nyc, nzc;
+  FastDiv div_nz, div_ny;    // SPE: row -> (i, j, k)
   const int32_t *cptr, *ccol;  // CSR
   const double *cval;
 };
@@ -88,7 +116,7 @@ struct OpArgs {
 constexpr int OP_ELL_VAL32 = 16;
 
 __device__ inline double op_row(const OpArgs &a, int64_t r, const double *__restrict__ x) {
-  if (a.op == CFD_OP_SPE) return spe_row(a.coef, a.jp, a.nxc, a.nyc, a.nzc, r, x);
+  if (a.op == CFD_OP_SPE) return spe_row(a.coef, a.jp, a.nxc, a.nyc, a.nzc, a.div_nz, a.div_ny, r, x);
   if (a.op == CFD_OP_CSR) return csr_row(a.cptr, a.ccol, a.cval, r, x);
   if (a.op == OP_ELL_VAL32) return ell_row<float, double>(a.col, a.val32, a.N, r, x);
   return ell_row<double, double>(a.col, a.val, a.N, r, x);
@@ -160,6 +188,8 @@ inline OpArgs op_args(const Ctx &c, int op) {
   a.nxc = c.g.nxc;
   a.nyc = c.g.nyc;
   a.nzc = c.g.nzc;
+  a.div_nz = fastdiv_make((uint32_t)c.g.nzc);
+  a.div_ny = fastdiv_make((uint32_t)c.g.nyc);
   a.cptr = c.csr_ptr;
   a.ccol = c.csr_col;
   a.cval = c.csr_val;

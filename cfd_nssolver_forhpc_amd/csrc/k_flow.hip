@@ -1,6 +1,7 @@
 // k_flow.hip -- gfx950 kernels of the velocity side of the timestep:
-// QUICK convection-diffusion predictor, source term, pressure RHS, DFIB
-// forcing/correction, boundary conditions, IB mask, initial state.
+// convection-diffusion predictor (QUICK, LUD or UD face values), source term,
+// pressure RHS, DFIB forcing/correction, boundary conditions, IB mask,
+// initial state.
 //
 // All fields use the reference's ghosted layout icel = (i*ny + j)*nz + k
 // (k fastest), so consecutive lanes of a wave64 touch consecutive k and every
@@ -62,8 +63,41 @@ __device__ inline void quickB(double Up, double Un, int64_t pp, int64_t p, int64
     phiN = 0.5 * (fn + fc) - 0.125 * Ds[I - 1] * Ds[I - 1] / Di[I] * ((fp - fc) / Ds[I] - (fc - fn) / Ds[I - 1]);
 }
 
+// LudUnitA / LudUnitB (inc/source/convection_and_difussion.hpp:90-144; the
+// two bodies are the same): the upwind cell plus half its width ratio times
+// its one-sided difference
+__device__ inline void lud(double Up, double Un, int64_t pp, int64_t p, int64_t ic, int64_t n, int64_t nn,
+                           const double *__restrict__ Di, const double *__restrict__ Ds, int I,
+                           const double *__restrict__ phi, double &phiP, double &phiN) {
+  const double fc = phi[ic], fp = phi[p], fn = phi[n];
+  if (Up > 0.0) phiP = fc + 0.5 * Ds[I] * (fc - fn) / Di[I];
+  else phiP = fp + 0.5 * Ds[I + 1] * (fp - phi[pp]) / Di[I + 1];
+  if (Un > 0.0) phiN = fn + 0.5 * Ds[I - 1] * (fn - phi[nn]) / Di[I - 1];
+  else phiN = fc + 0.5 * Ds[I] * (fc - fp) / Di[I + 1];
+}
+
+// UdUnit (:148-165): the upwind cell's value
+__device__ inline void ud(double Up, double Un, int64_t p, int64_t ic, int64_t n, const double *__restrict__ phi,
+                          double &phiP, double &phiN) {
+  phiP = Up > 0.0 ? phi[ic] : phi[p];
+  phiN = Un > 0.0 ? phi[n] : phi[ic];
+}
+
+// the face-value unit of scheme S; OWN: the component's own direction
+// (QuickUnitA), else across it (QuickUnitB); LUD and UD use one unit for both
+template <int S, bool OWN>
+__device__ inline void face_unit(double Up, double Un, int64_t pp, int64_t p, int64_t ic, int64_t n, int64_t nn,
+                                 const double *__restrict__ Di, const double *__restrict__ Ds, int I,
+                                 const double *__restrict__ phi, double &phiP, double &phiN) {
+  if constexpr (S == CFD_CONV_LUD) lud(Up, Un, pp, p, ic, n, nn, Di, Ds, I, phi, phiP, phiN);
+  else if constexpr (S == CFD_CONV_UD) ud(Up, Un, p, ic, n, phi, phiP, phiN);
+  else if constexpr (OWN) quickA(Up, Un, pp, p, ic, n, nn, Di, Ds, I, phi, phiP, phiN);
+  else quickB(Up, Un, pp, p, ic, n, nn, Di, Ds, I, phi, phiP, phiN);
+}
+
 // src_mode: 0 none, 1 add src[icelCal] at the cell, 2 reference index
-// mapping (src[m] added at ghosted m = icel when m < N)
+// mapping (src[m] added at ghosted m = icel when m < N); S: CFD_CONV_*
+template <int S>
 __global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, double dt, const double *__restrict__ u,
                                                         const double *__restrict__ v, const double *__restrict__ w,
                                                         double *__restrict__ u1, double *__restrict__ v1,
@@ -93,9 +127,9 @@ __global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, do
     const double Wp = w[ic] + (w[xp] - w[ic]) * IT;
     const double Wn = w[zm] + (w[zm + sx] - w[zm]) * IT;
     double uPx, uNx, uPy, uNy, uPz, uNz;
-    quickA(Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, u, uPx, uNx);
-    quickB(Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, u, uPy, uNy);
-    quickB(Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, u, uPz, uNz);
+    face_unit<S, true>(Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, u, uPx, uNx);
+    face_unit<S, false>(Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, u, uPy, uNy);
+    face_unit<S, false>(Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, u, uPz, uNz);
     const double conv =
         -dt * ((uPx * uPx - uNx * uNx) / Dxs[i] + (uPy * Vp - uNy * Vn) / Dy[j] + (uPz * Wp - uNz * Wn) / Dz[k]);
     const double diff = nudt * (((u[xp] - uc) / Dx[i + 1] - (uc - u[xm]) / Dx[i]) / Dxs[i] +
@@ -117,9 +151,9 @@ __global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, do
     const double Wp = w[ic] + (w[yp] - w[ic]) * IT;
     const double Wn = w[zm] + (w[zm + sy] - w[zm]) * IT;
     double vPx, vNx, vPy, vNy, vPz, vNz;
-    quickB(Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, v, vPx, vNx);
-    quickA(Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, v, vPy, vNy);
-    quickB(Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, v, vPz, vNz);
+    face_unit<S, false>(Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, v, vPx, vNx);
+    face_unit<S, true>(Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, v, vPy, vNy);
+    face_unit<S, false>(Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, v, vPz, vNz);
     const double conv =
         -dt * ((Up * vPx - Un * vNx) / Dx[i] + (vPy * vPy - vNy * vNy) / Dys[j] + (Wp * vPz - Wn * vNz) / Dz[k]);
     const double diff = nudt * (((v[xp] - vc) / Dxs[i] - (vc - v[xm]) / Dxs[i - 1]) / Dx[i] +
@@ -141,9 +175,9 @@ __global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, do
     const double Vp = v[ic] + (v[zp] - v[ic]) * IT;
     const double Vn = v[ym] + (v[ym + 1] - v[ym]) * IT;
     double wPx, wNx, wPy, wNy, wPz, wNz;
-    quickB(Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, w, wPx, wNx);
-    quickB(Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, w, wPy, wNy);
-    quickA(Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, w, wPz, wNz);
+    face_unit<S, false>(Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, w, wPx, wNx);
+    face_unit<S, false>(Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, w, wPy, wNy);
+    face_unit<S, true>(Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, w, wPz, wNz);
     const double conv =
         -dt * ((Up * wPx - Un * wNx) / Dx[i] + (Vp * wPy - Vn * wNy) / Dy[j] + (wPz * wPz - wNz * wNz) / Dzs[k]);
     const double diff = nudt * (((w[xp] - wc) / Dxs[i] - (wc - w[xm]) / Dxs[i - 1]) / Dx[i] +
@@ -606,9 +640,18 @@ hipError_t launch_eta_cylinder_z(Ctx &c, double inv_n2) {
 
 hipError_t launch_predictor(Ctx &c, const double *u0, const double *v0, const double *w0, double *u1, double *v1,
                             double *w1, const double *src, int src_mode, double *const keep[3]) {
-  predictor_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(
-      c.g, 1.0 / c.prm.Re, c.prm.dt, u0, v0, w0, u1, v1, w1, src, src_mode, keep ? keep[0] : nullptr,
-      keep ? keep[1] : nullptr, keep ? keep[2] : nullptr);
+  const int nb = blocks_for(c.g.N, 256, 1 << 30);
+  double *k0 = keep ? keep[0] : nullptr, *k1 = keep ? keep[1] : nullptr, *k2 = keep ? keep[2] : nullptr;
+  const double nu = 1.0 / c.prm.Re;
+  if (c.prm.convection == CFD_CONV_LUD)
+    predictor_kernel<CFD_CONV_LUD><<<nb, 256, 0, c.stream>>>(c.g, nu, c.prm.dt, u0, v0, w0, u1, v1, w1, src,
+                                                             src_mode, k0, k1, k2);
+  else if (c.prm.convection == CFD_CONV_UD)
+    predictor_kernel<CFD_CONV_UD><<<nb, 256, 0, c.stream>>>(c.g, nu, c.prm.dt, u0, v0, w0, u1, v1, w1, src,
+                                                            src_mode, k0, k1, k2);
+  else
+    predictor_kernel<CFD_CONV_QUICK><<<nb, 256, 0, c.stream>>>(c.g, nu, c.prm.dt, u0, v0, w0, u1, v1, w1, src,
+                                                               src_mode, k0, k1, k2);
   return hipGetLastError();
 }
 

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define CFD_ABI_VERSION 2
+#define CFD_ABI_VERSION 3
 
 /* status codes */
 #define CFD_OK 0
@@ -60,6 +60,11 @@ extern "C" {
 #define CFD_OP_ELL 0 /* mat::ELL_matrix (P_SOLVER_BICG_ELL) */
 #define CFD_OP_SPE 1 /* mat::SPE_matrix1, matrix-free (P_SOLVER_BICG_SPE, the reference default) */
 #define CFD_OP_CSR 2 /* mat::CSR_matrix from grid::createPressureMatrixCSR (P_SOLVER_BICG_CSR) */
+
+/* convection schemes of ConvectionDifussion (inc/controlPanel.hpp:125-129) */
+#define CFD_CONV_QUICK 0 /* QuickUnitA / QuickUnitB (convection_and_difussion.hpp:20-84), the reference's default */
+#define CFD_CONV_LUD 1   /* LudUnitA / LudUnitB, linear upwind (:90-144) */
+#define CFD_CONV_UD 2    /* UdUnit, first-order upwind (:148-165) */
 
 /* arithmetic of the pressure solve */
 #define CFD_FP64 0
@@ -132,6 +137,7 @@ typedef struct cfd_params {
   int precond;           /* CFD_PRECOND_JACOBI (default) or CFD_PRECOND_MG */
   int mg_nu;             /* CFD_PRECOND_MG: red-black Gauss-Seidel sweeps before and after each coarse
                             correction (default 2) */
+  int convection;        /* face values of the predictor's convection term: CFD_CONV_* */
 } cfd_params;
 
 /* per-timestep report; pass index 0..inner_loops-1 are predictor passes, the last is the corrector */

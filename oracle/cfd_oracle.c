@@ -296,9 +296,41 @@ static inline void quickB(double Up, double Un, size_t pp, size_t p, size_t ic, 
                                            ((phi[p] - phi[ic]) / Ds[I] - (phi[ic] - phi[n]) / Ds[I - 1]);
 }
 
+/* LudUnitA / LudUnitB (:90-144): one body serves both */
+static inline void lud(double Up, double Un, size_t pp, size_t p, size_t ic, size_t n, size_t nn,
+                       const double *Di, const double *Ds, int I, const double *phi, double *phiP, double *phiN) {
+  if (Up > 0.0) *phiP = phi[ic] + 0.5 * Ds[I] * (phi[ic] - phi[n]) / Di[I];
+  else *phiP = phi[p] + 0.5 * Ds[I + 1] * (phi[p] - phi[pp]) / Di[I + 1];
+  if (Un > 0.0) *phiN = phi[n] + 0.5 * Ds[I - 1] * (phi[n] - phi[nn]) / Di[I - 1];
+  else *phiN = phi[ic] + 0.5 * Ds[I] * (phi[ic] - phi[p]) / Di[I + 1];
+}
+
+/* UdUnit (:148-165) */
+static inline void ud(double Up, double Un, size_t p, size_t ic, size_t n, const double *phi, double *phiP,
+                      double *phiN) {
+  *phiP = Up > 0.0 ? phi[ic] : phi[p];
+  *phiN = Un > 0.0 ? phi[n] : phi[ic];
+}
+
+/* the face-value unit of a scheme; own: the component's own direction */
+static inline void face(int scheme, int own, double Up, double Un, size_t pp, size_t p, size_t ic, size_t n,
+                        size_t nn, const double *Di, const double *Ds, int I, const double *phi, double *phiP,
+                        double *phiN) {
+  if (scheme == CFD_CONV_LUD) lud(Up, Un, pp, p, ic, n, nn, Di, Ds, I, phi, phiP, phiN);
+  else if (scheme == CFD_CONV_UD) ud(Up, Un, p, ic, n, phi, phiP, phiN);
+  else if (own) quickA(Up, Un, pp, p, ic, n, nn, Di, Ds, I, phi, phiP, phiN);
+  else quickB(Up, Un, pp, p, ic, n, nn, Di, Ds, I, phi, phiP, phiN);
+}
+
 void orc_predictor(const cfd_grid_desc *g, double nu, double dt,
                    const double *u, const double *v, const double *w,
                    double *u1, double *v1, double *w1) {
+  orc_predictor_ex(g, CFD_CONV_QUICK, nu, dt, u, v, w, u1, v1, w1);
+}
+
+void orc_predictor_ex(const cfd_grid_desc *g, int scheme, double nu, double dt,
+                      const double *u, const double *v, const double *w,
+                      double *u1, double *v1, double *w1) {
   const dims_t d = dims_of(g);
   const size_t sx = d.nzny, sy = (size_t)d.nz, sz = 1;
   const double *Dx = g->Dx, *Dy = g->Dy, *Dz = g->Dz, *Dxs = g->Dxs, *Dys = g->Dys, *Dzs = g->Dzs;
@@ -321,9 +353,9 @@ void orc_predictor(const cfd_grid_desc *g, double nu, double dt,
         const double Wp = w[ic] + (w[xp] - w[ic]) * IT;
         const double Wn = w[zm] + (w[zm + xp - ic] - w[zm]) * IT;
         double uPx, uNx, uPy, uNy, uPz, uNz;
-        quickA(Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, u, &uPx, &uNx);
-        quickB(Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, u, &uPy, &uNy);
-        quickB(Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, u, &uPz, &uNz);
+        face(scheme, 1, Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, u, &uPx, &uNx);
+        face(scheme, 0, Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, u, &uPy, &uNy);
+        face(scheme, 0, Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, u, &uPz, &uNz);
         const double convection = -dt * ((uPx * uPx - uNx * uNx) / Dxs[i] + (uPy * Vp - uNy * Vn) / Dy[j] +
                                          (uPz * Wp - uNz * Wn) / Dz[k]);
         const double difussion =
@@ -351,9 +383,9 @@ void orc_predictor(const cfd_grid_desc *g, double nu, double dt,
         const double Wp = w[ic] + (w[yp] - w[ic]) * IT;
         const double Wn = w[zm] + (w[zm + yp - ic] - w[zm]) * IT;
         double vPx, vNx, vPy, vNy, vPz, vNz;
-        quickB(Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, v, &vPx, &vNx);
-        quickA(Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, v, &vPy, &vNy);
-        quickB(Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, v, &vPz, &vNz);
+        face(scheme, 0, Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, v, &vPx, &vNx);
+        face(scheme, 1, Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, v, &vPy, &vNy);
+        face(scheme, 0, Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, v, &vPz, &vNz);
         const double convection = -dt * ((Up * vPx - Un * vNx) / Dx[i] + (vPy * vPy - vNy * vNy) / Dys[j] +
                                          (Wp * vPz - Wn * vNz) / Dz[k]);
         const double difussion =
@@ -381,9 +413,9 @@ void orc_predictor(const cfd_grid_desc *g, double nu, double dt,
         const double Vp = v[ic] + (v[zp] - v[ic]) * IT;
         const double Vn = v[ym] + (v[ym + zp - ic] - v[ym]) * IT;
         double wPx, wNx, wPy, wNy, wPz, wNz;
-        quickB(Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, w, &wPx, &wNx);
-        quickB(Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, w, &wPy, &wNy);
-        quickA(Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, w, &wPz, &wNz);
+        face(scheme, 0, Up, Un, xpp, xp, ic, xm, xmm, Dx, Dxs, i, w, &wPx, &wNx);
+        face(scheme, 0, Vp, Vn, ypp, yp, ic, ym, ymm, Dy, Dys, j, w, &wPy, &wNy);
+        face(scheme, 1, Wp, Wn, zpp, zp, ic, zm, zmm, Dz, Dzs, k, w, &wPz, &wNz);
         const double convection = -dt * ((Up * wPx - Un * wNx) / Dx[i] + (Vp * wPy - Vn * wNy) / Dy[j] +
                                          (wPz * wPz - wNz * wNz) / Dzs[k]);
         const double difussion =
@@ -1482,7 +1514,8 @@ int orc_sim_setup(orc_sim *s) {
 static void sim_pass(orc_sim *s, int pass, int accumulate, cfd_step_stats *st) {
   const cfd_params *P = &s->prm;
   const cfd_grid_desc *g = &s->g;
-  orc_predictor(g, 1.0 / P->Re, P->dt, s->T0[0], s->T0[1], s->T0[2], s->T1[0], s->T1[1], s->T1[2]);
+  orc_predictor_ex(g, P->convection, 1.0 / P->Re, P->dt, s->T0[0], s->T0[1], s->T0[2], s->T1[0], s->T1[1],
+                   s->T1[2]);
   orc_source_add(g, P->source_index_quirk, s->src, s->T1[0], s->T1[1], s->T1[2]);
   int it = -1;
   double rel = 0.0;
@@ -1591,7 +1624,7 @@ int orc_sim_gradient(orc_sim *s, double *g_uin, double *g_gain, int *adj_iters) 
   const cfd_grid_desc *g = &s->g;
   const dims_t d = s->d;
   const size_t G = d.G, N = d.N;
-  if (!s->adj_steps || P->source_index_quirk || P->op != CFD_OP_ELL) return -1;
+  if (!s->adj_steps || P->source_index_quirk || P->op != CFD_OP_ELL || P->convection != CFD_CONV_QUICK) return -1;
   const int passes = s->adj_passes, L = passes - 1;
   const double dt = P->dt, nu = 1.0 / P->Re;
   const double wJ = 1.0 / (double)(s->adj_steps - s->adj_avg_from);

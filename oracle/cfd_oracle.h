@@ -51,6 +51,10 @@ void orc_eta_cylinder_z(const cfd_grid_desc *g, double cx, double cy, double R, 
 void orc_predictor(const cfd_grid_desc *g, double nu, double dt,
                    const double *u0, const double *v0, const double *w0,
                    double *u1, double *v1, double *w1);
+/* the same with the scheme CFD_CONV_QUICK / _LUD / _UD (convection_and_difussion.hpp:20-165) */
+void orc_predictor_ex(const cfd_grid_desc *g, int scheme, double nu, double dt,
+                      const double *u0, const double *v0, const double *w0,
+                      double *u1, double *v1, double *w1);
 void orc_source_add(const cfd_grid_desc *g, int quirk, const double *src,
                     double *u1, double *v1, double *w1);
 void orc_source_accumulate(const cfd_grid_desc *g, double dt, const double *f, double *src);

@@ -20,7 +20,8 @@
 // runSeqOmp_predictionMethod (inc/run/RUN_CPU_PC_METHOD.hpp:287-627); it
 // restates only the sequencing, never the arithmetic.
 //
-// usage: ref_harness OUTDIR NSTEPS TOL OP(ell|spe) FN_STEP [DUMP_EVERY]
+// usage: ref_harness OUTDIR NSTEPS TOL OP(ell|spe) FN_STEP [DUMP_EVERY [SRC_AT_CELL]]
+// (ref_harness_lud / ref_harness_ud: the same with the LUD / UD convection scheme)
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -52,6 +53,17 @@
 // slot layout without touching the reference sources.
 #define private public
 #include "controlPanel.hpp"
+#if defined(HARNESS_CONV_LUD) || defined(HARNESS_CONV_UD)
+// controlPanel.hpp:125 fixes the convection scheme to QUICK; these builds
+// select another of the reference's own schemes (:127, :129) before
+// run/general.hpp compiles ConvectionDifussion
+#undef CONVECTION_DIFUSSION_QUICK
+#if defined(HARNESS_CONV_LUD)
+#define CONVECTION_DIFUSSION_LUD
+#else
+#define CONVECTION_DIFUSSION_UD
+#endif
+#endif
 #include "run/general.hpp"
 #undef private
 
@@ -293,11 +305,15 @@ int main(int argc, char **argv) {
     return io_main();
   }
   if (argc < 6) {
-    std::cerr << "usage: ref_harness OUTDIR NSTEPS TOL OP(ell|spe) FN_STEP [DUMP_EVERY]\n";
+    std::cerr << "usage: ref_harness OUTDIR NSTEPS TOL OP(ell|spe) FN_STEP [DUMP_EVERY [SRC_AT_CELL]]\n";
     return 2;
   }
   // state dumps every DUMP_EVERY steps (default 1; 0: statistics only)
   const int dump_every = argc >= 7 ? std::atoi(argv[6]) : 1;
+  // source term index: 0 (default) adds source_termF[icelCal] at
+  // T1[icelCal] as the reference's driver does (RUN_CPU_PC_METHOD.hpp:309-316);
+  // 1 adds it at the cell it belongs to, T1[icel]
+  const int src_at_cell = argc >= 8 ? std::atoi(argv[7]) : 0;
   g_out = argv[1];
   const int nsteps = std::atoi(argv[2]);
   const double tol = std::atof(argv[3]);
@@ -418,7 +434,8 @@ int main(int argc, char **argv) {
         for (auto i = dom.i_begin; i < dom.i_endof; ++i)
           for (auto j = dom.j_begin; j < dom.j_endof; ++j)
             for (auto k = dom.k_begin; k < dom.k_endof; ++k)
-              T1.U(c)[gA.icelCal(i, j, k)] += source_termF[gA.icelCal(i, j, k) + gA.iceltotCal * c];
+              T1.U(c)[src_at_cell ? gA.icel(i, j, k) : gA.icelCal(i, j, k)] +=
+                  source_termF[gA.icelCal(i, j, k) + gA.iceltotCal * c];
       if (fnp) dump_vel("fn_srcadd_out_T1", T1);
       int iters = -1;
       double err = 0.0;

@@ -48,6 +48,7 @@ def _declare(L):
     L.orc_axis_tanh.restype = C.c_int
     L.orc_eta_cylinder_z.argtypes = [G, C.c_double, C.c_double, C.c_double, C.c_int, dp]
     L.orc_predictor.argtypes = [G, C.c_double, C.c_double, dp, dp, dp, dp, dp, dp]
+    L.orc_predictor_ex.argtypes = [G, C.c_int, C.c_double, C.c_double, dp, dp, dp, dp, dp, dp]
     L.orc_source_add.argtypes = [G, C.c_int, dp, dp, dp, dp]
     L.orc_source_accumulate.argtypes = [G, C.c_double, dp, dp]
     L.orc_rhs.argtypes = [G, C.c_double, dp, dp, dp, dp, dp]

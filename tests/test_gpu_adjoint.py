@@ -157,6 +157,14 @@ def test_bad_adjoint_arguments_are_errors(require_gpu):
     sim.createPressureMatrix()
     with pytest.raises(capi.CfdInvalidArgument):
         sim.adjoint_forward(5, 5, 1)
+    # the transposed predictor is QUICK's
+    p2 = w.params.copy()
+    p2.convection = capi.CFD_CONV_LUD
+    sim2 = Simulation(w.grid, p2)
+    sim2.init_state(1e-3, 1)
+    sim2.createPressureMatrix()
+    with pytest.raises(capi.CfdInvalidArgument):
+        sim2.adjoint_forward(5, 2, 1)
 
 
 @pytest.mark.parametrize("mode", ["fp64", "pc32"])

@@ -193,6 +193,49 @@ def test_velocity_phases(require_gpu, gname):
     assert np.allclose(md, list(om), rtol=0, atol=0)
 
 
+@pytest.mark.parametrize("gname", list(GRIDS))
+@pytest.mark.parametrize("scheme", [capi.CFD_CONV_LUD, capi.CFD_CONV_UD])
+def test_predictor_schemes_match_oracle(require_gpu, gname, scheme):
+    """The LUD / UD convection schemes on the device against the oracle's,
+    at the predictor's bar (FMA contraction of a different compiler)."""
+    grid = GRIDS[gname]()
+    sim, p = make_sim(grid, quirk=1)
+    p.convection = scheme
+    sim = Simulation(grid, p)
+    sim.init_state()
+    g = O.Grid(grid)
+    T0 = random_state(grid, 21)
+    T1 = random_state(grid, 22)
+    for c in range(3):
+        sim.upload(capi.T0_U + c, T0[c])
+        sim.upload(capi.T1_U + c, T1[c])
+    sim.ConvectionDifussion()
+    o1 = [t.copy() for t in T1]
+    O.lib().orc_predictor_ex(g.ref, scheme, 1.0 / p.Re, p.dt, *T0, *o1)
+    for c in range(3):
+        assert rel_l2(sim.download(capi.T1_U + c), o1[c]) < 1e-13
+
+
+@pytest.mark.parametrize("scheme", [capi.CFD_CONV_LUD, capi.CFD_CONV_UD])
+def test_steps_with_schemes_match_oracle(require_gpu, scheme):
+    grid = GRIDS["sml"]()
+    sim, p = make_sim(grid, quirk=0, pd=(0, 1, 0, 0, 0, 0), tol=1e-13, inner=3)
+    p.convection = scheme
+    sim = Simulation(grid, p)
+    sim.setup(noise=1e-3, seed=3)
+    osim = O.Sim(O.Grid(grid), p)
+    osim.init_state(1e-3, 3)
+    osim.setup()
+    for k in range(3):
+        st = sim.step()
+        ost = osim.step()
+        for c in range(3):
+            e = rel_l2(sim.download(capi.T0_U + c), osim.field(capi.T0_U + c))
+            assert e < 1e-10, (k, c, e)
+        for q in range(st.n_passes):
+            assert abs(st.cd[q] - ost.cd[q]) <= 1e-9 * max(1.0, abs(ost.cd[q]))
+
+
 def oracle_csr(g):
     ptr, col, val, jp = np.zeros(g.N + 1, np.int32), np.zeros(7 * g.N, np.int32), np.zeros(7 * g.N), np.zeros(g.N)
     nnz = O.lib().orc_csr_assemble(g.ref, ptr, col, val, jp)

@@ -109,6 +109,41 @@ def test_quick_is_the_textbook_scheme():
     assert np.allclose(got, expect, rtol=1e-13, atol=1e-14)
 
 
+def test_lud_and_ud_are_the_textbook_schemes():
+    """Uniform grid, u(x) along x only, both flow directions: linear upwind
+    (LUD) extrapolates the upwind cell by half its one-sided difference,
+    phi_U + (phi_U - phi_UU) / 2; first-order upwind (UD) takes phi_U.  The
+    convective update is -dt (uP^2 - uN^2) / h with those face values."""
+    g = ugrid((16, 6, 6), (4.0, 1.5, 1.5))
+    L = O.lib()
+    h = 4.0 / 16
+    dt = 0.01
+    zero = np.zeros(g.G)
+    i = np.arange(2, g.nx - 3)
+    for sign in (1.0, -1.0):
+        ux = lambda i: sign * (1.0 + 0.3 * np.sin(0.7 * i) + 0.05 * i)
+        u = field(g, lambda i, j, k: ux(i) + 0 * j)
+        if sign > 0:  # face i+1/2 upwinds from i, face i-1/2 from i-1
+            lud_P, lud_N = 1.5 * ux(i) - 0.5 * ux(i - 1), 1.5 * ux(i - 1) - 0.5 * ux(i - 2)
+            ud_P, ud_N = ux(i), ux(i - 1)
+        else:         # upwind from i+1 and i
+            lud_P, lud_N = 1.5 * ux(i + 1) - 0.5 * ux(i + 2), 1.5 * ux(i) - 0.5 * ux(i + 1)
+            ud_P, ud_N = ux(i + 1), ux(i)
+        for scheme, (uP, uN) in ((capi.CFD_CONV_LUD, (lud_P, lud_N)), (capi.CFD_CONV_UD, (ud_P, ud_N))):
+            u1, v1, w1 = np.zeros(g.G), np.zeros(g.G), np.zeros(g.G)
+            L.orc_predictor_ex(g.ref, scheme, 1e-300, dt, u, zero, zero, u1, v1, w1)
+            expect = ux(i) - dt * (uP ** 2 - uN ** 2) / h
+            got = u1.reshape(g.nx, g.ny, g.nz)[2:g.nx - 3, 3, 3]
+            assert np.allclose(got, expect, rtol=1e-13, atol=1e-14), (scheme, sign)
+    # QUICK through the same entry point is orc_predictor itself
+    u = field(g, lambda i, j, k: 1.0 + 0.1 * i + 0 * j)
+    a = [np.zeros(g.G) for _ in range(3)]
+    b = [np.zeros(g.G) for _ in range(3)]
+    L.orc_predictor(g.ref, 0.01, dt, u, zero, zero, *a)
+    L.orc_predictor_ex(g.ref, capi.CFD_CONV_QUICK, 0.01, dt, u, zero, zero, *b)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
 def test_ell_neumann_eigenpairs_and_row_sums():
     """All-Neumann uniform grid: A 1 = 0 and cos(pi m (i+1/2)/n) are
     eigenvectors with eigenvalue jp * sum(4/h^2 sin^2(pi m/(2n)))."""

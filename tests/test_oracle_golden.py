@@ -148,6 +148,24 @@ def test_functions_on_seeded_inputs(native):
     assert np.allclose(list(md), FX["scalars"]["maxdiff"], rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("scheme,key", [(capi.CFD_CONV_LUD, "synth_lud"), (capi.CFD_CONV_UD, "synth_ud")])
+def test_predictor_lud_and_ud_against_reference(native, scheme, key):
+    """ConvectionDifussion compiled with the reference's LUD / UD scheme
+    (ref_harness_lud / _ud) on the seeded velocities of the QUICK check, at
+    the same bar (the reference build's FMA contraction differs per site)."""
+    g = native["g"]
+    G = g.G
+    T0 = [O.noise(G, SEED, 1, 1.0, 0.3), O.noise(G, SEED, 2, 0.0, 0.3), O.noise(G, SEED, 3, 0.0, 0.3)]
+    T1 = [np.full(G, 7.0) for _ in range(3)]
+    O.lib().orc_predictor_ex(g.ref, scheme, 1.0 / 40.0, 0.002, *T0, *T1)
+    for c, n in enumerate("uvw"):
+        f = FX[key][f"syn_pred_T1_{n}.bin"]
+        idx = np.array(f["idx"])
+        ref = np.array(f["val"])
+        assert np.all(np.abs(T1[c][idx] - ref) <= 1e-13 * np.maximum(np.abs(ref), np.abs(ref).max() * 1e-3)), n
+        assert abs(np.linalg.norm(T1[c]) - f["l2"]) <= 1e-13 * f["l2"], n
+
+
 def test_update_and_bc_bit_exact(native):
     """Stages whose inputs are pure seeded noise are bit-exact end to end."""
     g = native["g"]
