@@ -78,7 +78,7 @@ class CfdParams(C.Structure):
         ("ib_nsub", C.c_int),
         ("u_init", C.c_double * 3),
         ("precond", C.c_int), ("mg_nu", C.c_int),
-        ("convection", C.c_int),
+        ("convection", C.c_int), ("les", C.c_int),
     ]
 
     def set_bc(self, dir_table, num_table):

@@ -650,6 +650,7 @@ int adjoint_checks(const Ctx &c) {
   if (c.prm.solver != CFD_SOLVER_BICGSTAB) return fail(CFD_ERR_ARG, "adjoint: needs the BiCGSTAB pressure solve");
   if (c.prm.source_index_quirk) return fail(CFD_ERR_ARG, "adjoint: needs source_index_quirk = 0");
   if (c.prm.convection != CFD_CONV_QUICK) return fail(CFD_ERR_ARG, "adjoint: needs the QUICK convection scheme");
+  if (c.prm.les) return fail(CFD_ERR_ARG, "adjoint: needs les = 0");
   if (c.g.nxc < 2 || c.g.nyc < 2 || c.g.nzc < 2) return fail(CFD_ERR_ARG, "adjoint: needs >= 2 cells per axis");
   if (!c.ell_ready) return fail(CFD_ERR_STATE, "adjoint: ELL operator not assembled");
   return CFD_OK;

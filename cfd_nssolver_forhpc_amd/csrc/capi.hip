@@ -75,6 +75,7 @@ int check_params(const cfd_params *p) {
   if (p->precond != CFD_PRECOND_JACOBI && p->precond != CFD_PRECOND_MG) return fail(CFD_ERR_ARG, "unknown preconditioner");
   if (p->convection != CFD_CONV_QUICK && p->convection != CFD_CONV_LUD && p->convection != CFD_CONV_UD)
     return fail(CFD_ERR_ARG, "unknown convection scheme");
+  if (p->les != 0 && p->les != 1) return fail(CFD_ERR_ARG, "les must be 0 or 1");
   if (p->precond == CFD_PRECOND_MG && (p->mg_nu < 1 || p->mg_nu > 8)) return fail(CFD_ERR_ARG, "mg_nu must be in [1, 8]");
   return CFD_OK;
 }
@@ -93,6 +94,7 @@ void free_ctx(Ctx *c) {
   adjoint_release(*c);
   mg_release(*c);
   dfree(c->uin);
+  dfree(c->viseff);
   for (auto &q : c->pred) dfree(q);
   for (auto &p : c->f) dfree(p);
   dfree(c->col); dfree(c->val); dfree(c->val32); dfree(c->colP); dfree(c->valP); dfree(c->valP32); dfree(c->perm);
@@ -272,6 +274,7 @@ int cfd_params_default(cfd_params *out) {
   out->precond = CFD_PRECOND_JACOBI;
   out->mg_nu = 2;
   out->convection = CFD_CONV_QUICK;  // controlPanel.hpp:125
+  out->les = 0;                      // controlPanel.hpp:135 (TERBULENCE_SMAGORINSKY off)
   return CFD_OK;
 }
 

@@ -123,6 +123,7 @@ struct Ctx {
   float *ph32 = nullptr, *sh32 = nullptr;
   // the step's prediction (u, v, w; ghosted), kept by its first pass
   double *pred[3] = {nullptr, nullptr, nullptr};
+  double *viseff = nullptr;  // Smagorinsky eddy viscosity (N), when prm.les
 };
 
 // k_mg.hip

@@ -95,15 +95,43 @@ __device__ inline void face_unit(double Up, double Un, int64_t pp, int64_t p, in
   else quickB(Up, Un, pp, p, ic, n, nn, Di, Ds, I, phi, phiP, phiN);
 }
 
+// SmagorinskyModel (inc/source/smagorinskyModel.hpp:30-79) with
+// GradientGauss (:6-27): central differences phi[-] - phi[+] over the
+// metrics (Dxs, Dy, Dz) for u, (Dx, Dys, Dz) for v, (Dx, Dy, Dzs) for w,
+// |2 ux^2 + vy^2 + wz^2 + (uy + vx)^2 + (uz + wx)^2 + (vz + wy)^2| (the
+// reference's grouping), nu_t = (C_s (Dx Dy Dz)^(1/3))^2 sqrt(.), C_s 0.18
+__global__ void __launch_bounds__(256) smagorinsky_kernel(DevGrid g, const double *__restrict__ u,
+                                                          const double *__restrict__ v,
+                                                          const double *__restrict__ w, double *__restrict__ vis) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  const Cell c = decode_interior(g, t);
+  const int i = c.i, j = c.j, k = c.k;
+  const int64_t ic = icel(g, i, j, k);
+  const int64_t sx = (int64_t)g.ny * g.nz, sy = g.nz;
+  const double ux = 0.5 / g.Dxs[i] * (u[ic - sx] - u[ic + sx]), uy = 0.5 / g.Dy[j] * (u[ic - sy] - u[ic + sy]),
+               uz = 0.5 / g.Dz[k] * (u[ic - 1] - u[ic + 1]);
+  const double vx = 0.5 / g.Dx[i] * (v[ic - sx] - v[ic + sx]), vy = 0.5 / g.Dys[j] * (v[ic - sy] - v[ic + sy]),
+               vz = 0.5 / g.Dz[k] * (v[ic - 1] - v[ic + 1]);
+  const double wx = 0.5 / g.Dx[i] * (w[ic - sx] - w[ic + sx]), wy = 0.5 / g.Dy[j] * (w[ic - sy] - w[ic + sy]),
+               wz = 0.5 / g.Dzs[k] * (w[ic - 1] - w[ic + 1]);
+  const double s = fabs(2.0 * (ux * ux) + vy * vy + wz * wz + (uy + vx) * (uy + vx) + (uz + wx) * (uz + wx) +
+                        (vz + wy) * (vz + wy));
+  const double cd = 0.18 * pow(g.Dx[i] * g.Dy[j] * g.Dz[k], 1.0 / 3.0);
+  vis[t] = (cd * cd) * sqrt(s);
+}
+
 // src_mode: 0 none, 1 add src[icelCal] at the cell, 2 reference index
-// mapping (src[m] added at ghosted m = icel when m < N); S: CFD_CONV_*
+// mapping (src[m] added at ghosted m = icel when m < N); S: CFD_CONV_*;
+// vis: the eddy viscosity added to nu (convection_and_difussion.hpp:246-250), or null
 template <int S>
 __global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, double dt, const double *__restrict__ u,
                                                         const double *__restrict__ v, const double *__restrict__ w,
                                                         double *__restrict__ u1, double *__restrict__ v1,
                                                         double *__restrict__ w1, const double *__restrict__ src,
                                                         int src_mode, double *__restrict__ pu,
-                                                        double *__restrict__ pv, double *__restrict__ pw) {
+                                                        double *__restrict__ pv, double *__restrict__ pw,
+                                                        const double *__restrict__ vis) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   const Cell c = decode_interior(g, t);
@@ -114,7 +142,7 @@ __global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, do
   const int64_t xmm = ic - 2 * sx, xpp = ic + 2 * sx, ymm = ic - 2 * sy, ypp = ic + 2 * sy, zmm = ic - 2, zpp = ic + 2;
   const double *__restrict__ Dx = g.Dx, *__restrict__ Dy = g.Dy, *__restrict__ Dz = g.Dz;
   const double *__restrict__ Dxs = g.Dxs, *__restrict__ Dys = g.Dys, *__restrict__ Dzs = g.Dzs;
-  const double nudt = nu * dt;
+  const double nudt = vis ? (nu + vis[t]) * dt : nu * dt;
   const int64_t N = g.N;
 
   if (i < g.nx - GC - 1) {  // u (:196-269)
@@ -643,15 +671,27 @@ hipError_t launch_predictor(Ctx &c, const double *u0, const double *v0, const do
   const int nb = blocks_for(c.g.N, 256, 1 << 30);
   double *k0 = keep ? keep[0] : nullptr, *k1 = keep ? keep[1] : nullptr, *k2 = keep ? keep[2] : nullptr;
   const double nu = 1.0 / c.prm.Re;
+  const double *vis = nullptr;
+  if (c.prm.les) {  // the eddy viscosity of this T0 (RUN_CPU_PC_METHOD.hpp:298-300, 449-451)
+    if (!c.viseff) {
+      hipError_t e = hipMalloc((void **)&c.viseff, (size_t)c.g.N * sizeof(double));
+      if (e != hipSuccess) {
+        c.viseff = nullptr;
+        return e;
+      }
+    }
+    smagorinsky_kernel<<<nb, 256, 0, c.stream>>>(c.g, u0, v0, w0, c.viseff);
+    vis = c.viseff;
+  }
   if (c.prm.convection == CFD_CONV_LUD)
     predictor_kernel<CFD_CONV_LUD><<<nb, 256, 0, c.stream>>>(c.g, nu, c.prm.dt, u0, v0, w0, u1, v1, w1, src,
-                                                             src_mode, k0, k1, k2);
+                                                             src_mode, k0, k1, k2, vis);
   else if (c.prm.convection == CFD_CONV_UD)
     predictor_kernel<CFD_CONV_UD><<<nb, 256, 0, c.stream>>>(c.g, nu, c.prm.dt, u0, v0, w0, u1, v1, w1, src,
-                                                            src_mode, k0, k1, k2);
+                                                            src_mode, k0, k1, k2, vis);
   else
     predictor_kernel<CFD_CONV_QUICK><<<nb, 256, 0, c.stream>>>(c.g, nu, c.prm.dt, u0, v0, w0, u1, v1, w1, src,
-                                                               src_mode, k0, k1, k2);
+                                                               src_mode, k0, k1, k2, vis);
   return hipGetLastError();
 }
 

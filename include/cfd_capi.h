@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define CFD_ABI_VERSION 3
+#define CFD_ABI_VERSION 4
 
 /* status codes */
 #define CFD_OK 0
@@ -138,6 +138,8 @@ typedef struct cfd_params {
   int mg_nu;             /* CFD_PRECOND_MG: red-black Gauss-Seidel sweeps before and after each coarse
                             correction (default 2) */
   int convection;        /* face values of the predictor's convection term: CFD_CONV_* */
+  int les;               /* 1: Smagorinsky eddy viscosity (C_s 0.18) added to nu in the predictor's diffusion
+                            (smagorinskyModel.hpp:30-79, TERBULENCE_SMAGORINSKY); 0: off, the reference's default */
 } cfd_params;
 
 /* per-timestep report; pass index 0..inner_loops-1 are predictor passes, the last is the corrector */

@@ -331,6 +331,38 @@ void orc_predictor(const cfd_grid_desc *g, double nu, double dt,
 void orc_predictor_ex(const cfd_grid_desc *g, int scheme, double nu, double dt,
                       const double *u, const double *v, const double *w,
                       double *u1, double *v1, double *w1) {
+  orc_predictor_les(g, scheme, nu, dt, NULL, u, v, w, u1, v1, w1);
+}
+
+/* SmagorinskyModel + GradientGauss (inc/source/smagorinskyModel.hpp:6-79):
+   gradients phi[-] - phi[+] over (Dxs,Dy,Dz) for u, (Dx,Dys,Dz) for v,
+   (Dx,Dy,Dzs) for w; |2 ux^2 + vy^2 + wz^2 + (uy+vx)^2 + (uz+wx)^2 +
+   (vz+wy)^2|; nu_t = (0.18 (Dx Dy Dz)^(1/3))^2 sqrt(.) at icelCal */
+void orc_smagorinsky(const cfd_grid_desc *g, const double *u, const double *v, const double *w, double *vis) {
+  const dims_t d = dims_of(g);
+  const size_t sx = d.nzny, sy = (size_t)d.nz;
+  const double *Dx = g->Dx, *Dy = g->Dy, *Dz = g->Dz, *Dxs = g->Dxs, *Dys = g->Dys, *Dzs = g->Dzs;
+#pragma omp parallel for schedule(static)
+  for (int i = GC; i < d.nx - GC; ++i)
+    for (int j = GC; j < d.ny - GC; ++j)
+      for (int k = GC; k < d.nz - GC; ++k) {
+        const size_t ic = IC(d, i, j, k);
+        const double ux = 0.5 / Dxs[i] * (u[ic - sx] - u[ic + sx]), uy = 0.5 / Dy[j] * (u[ic - sy] - u[ic + sy]),
+                     uz = 0.5 / Dz[k] * (u[ic - 1] - u[ic + 1]);
+        const double vx = 0.5 / Dx[i] * (v[ic - sx] - v[ic + sx]), vy = 0.5 / Dys[j] * (v[ic - sy] - v[ic + sy]),
+                     vz = 0.5 / Dz[k] * (v[ic - 1] - v[ic + 1]);
+        const double wx = 0.5 / Dx[i] * (w[ic - sx] - w[ic + sx]), wy = 0.5 / Dy[j] * (w[ic - sy] - w[ic + sy]),
+                     wz = 0.5 / Dzs[k] * (w[ic - 1] - w[ic + 1]);
+        const double sq = fabs(2.0 * (ux * ux) + vy * vy + wz * wz + (uy + vx) * (uy + vx) + (uz + wx) * (uz + wx) +
+                               (vz + wy) * (vz + wy));
+        const double cd = 0.18 * pow(Dx[i] * Dy[j] * Dz[k], 1.0 / 3.0);
+        vis[ICAL(d, i, j, k)] = (cd * cd) * sqrt(sq);
+      }
+}
+
+void orc_predictor_les(const cfd_grid_desc *g, int scheme, double nu, double dt, const double *vis,
+                       const double *u, const double *v, const double *w,
+                       double *u1, double *v1, double *w1) {
   const dims_t d = dims_of(g);
   const size_t sx = d.nzny, sy = (size_t)d.nz, sz = 1;
   const double *Dx = g->Dx, *Dy = g->Dy, *Dz = g->Dz, *Dxs = g->Dxs, *Dys = g->Dys, *Dzs = g->Dzs;
@@ -359,7 +391,7 @@ void orc_predictor_ex(const cfd_grid_desc *g, int scheme, double nu, double dt,
         const double convection = -dt * ((uPx * uPx - uNx * uNx) / Dxs[i] + (uPy * Vp - uNy * Vn) / Dy[j] +
                                          (uPz * Wp - uNz * Wn) / Dz[k]);
         const double difussion =
-            nu * dt *
+            (vis ? nu + vis[ICAL(d, i, j, k)] : nu) * dt *
             (((u[xp] - u[ic]) / Dx[i + 1] - (u[ic] - u[xm]) / Dx[i]) / Dxs[i] +
              ((u[yp] - u[ic]) / Dys[j] - (u[ic] - u[ym]) / Dys[j - 1]) / Dy[j] +
              ((u[zp] - u[ic]) / Dzs[k] - (u[ic] - u[zm]) / Dzs[k - 1]) / Dz[k]);
@@ -389,7 +421,7 @@ void orc_predictor_ex(const cfd_grid_desc *g, int scheme, double nu, double dt,
         const double convection = -dt * ((Up * vPx - Un * vNx) / Dx[i] + (vPy * vPy - vNy * vNy) / Dys[j] +
                                          (Wp * vPz - Wn * vNz) / Dz[k]);
         const double difussion =
-            nu * dt *
+            (vis ? nu + vis[ICAL(d, i, j, k)] : nu) * dt *
             (((v[xp] - v[ic]) / Dxs[i] - (v[ic] - v[xm]) / Dxs[i - 1]) / Dx[i] +
              ((v[yp] - v[ic]) / Dy[j + 1] - (v[ic] - v[ym]) / Dy[j]) / Dys[j] +
              ((v[zp] - v[ic]) / Dzs[k] - (v[ic] - v[zm]) / Dzs[k - 1]) / Dz[k]);
@@ -419,7 +451,7 @@ void orc_predictor_ex(const cfd_grid_desc *g, int scheme, double nu, double dt,
         const double convection = -dt * ((Up * wPx - Un * wNx) / Dx[i] + (Vp * wPy - Vn * wNy) / Dy[j] +
                                          (wPz * wPz - wNz * wNz) / Dzs[k]);
         const double difussion =
-            nu * dt *
+            (vis ? nu + vis[ICAL(d, i, j, k)] : nu) * dt *
             (((w[xp] - w[ic]) / Dxs[i] - (w[ic] - w[xm]) / Dxs[i - 1]) / Dx[i] +
              ((w[yp] - w[ic]) / Dys[j] - (w[ic] - w[ym]) / Dys[j - 1]) / Dy[j] +
              ((w[zp] - w[ic]) / Dz[k + 1] - (w[ic] - w[zm]) / Dz[k]) / Dzs[k]);
@@ -1354,6 +1386,7 @@ struct orc_sim {
   double *spe_coef, *spe_jp;
   int32_t *csr_ptr, *csr_col;
   double *csr_val, *csr_jp;
+  double *vis;      /* Smagorinsky eddy viscosity (N) */
   orc_bicg *bicg;
   int loop;
   double time;
@@ -1396,6 +1429,7 @@ orc_sim *orc_sim_create(const cfd_grid_desc *g, const cfd_params *prm) {
   s->csr_col = (int32_t *)calloc(7 * N, sizeof(int32_t));
   s->csr_val = (double *)calloc(7 * N, sizeof(double));
   s->csr_jp = (double *)calloc(N, sizeof(double));
+  s->vis = (double *)calloc(N, sizeof(double));
   s->loop = 1;
   s->time = 0.0;
   s->gain = 1.0;
@@ -1414,7 +1448,7 @@ void orc_sim_destroy(orc_sim *s) {
   for (int c = 0; c < 3; ++c) { free(s->T0[c]); free(s->T1[c]); free(s->T3[c]); }
   free(s->p); free(s->eta); free(s->f); free(s->src); free(s->b); free(s->X); free(s->jp);
   free(s->va); free(s->vb); free(s->col); free(s->val); free(s->spe_coef); free(s->spe_jp);
-  free(s->csr_ptr); free(s->csr_col); free(s->csr_val); free(s->csr_jp);
+  free(s->csr_ptr); free(s->csr_col); free(s->csr_val); free(s->csr_jp); free(s->vis);
   orc_bicg_destroy(s->bicg);
   free(s->uin);
   adj_free(s);
@@ -1514,8 +1548,9 @@ int orc_sim_setup(orc_sim *s) {
 static void sim_pass(orc_sim *s, int pass, int accumulate, cfd_step_stats *st) {
   const cfd_params *P = &s->prm;
   const cfd_grid_desc *g = &s->g;
-  orc_predictor_ex(g, P->convection, 1.0 / P->Re, P->dt, s->T0[0], s->T0[1], s->T0[2], s->T1[0], s->T1[1],
-                   s->T1[2]);
+  if (P->les) orc_smagorinsky(g, s->T0[0], s->T0[1], s->T0[2], s->vis);  /* RUN_CPU_PC_METHOD.hpp:298-300 */
+  orc_predictor_les(g, P->convection, 1.0 / P->Re, P->dt, P->les ? s->vis : NULL, s->T0[0], s->T0[1], s->T0[2],
+                    s->T1[0], s->T1[1], s->T1[2]);
   orc_source_add(g, P->source_index_quirk, s->src, s->T1[0], s->T1[1], s->T1[2]);
   int it = -1;
   double rel = 0.0;
@@ -1624,7 +1659,8 @@ int orc_sim_gradient(orc_sim *s, double *g_uin, double *g_gain, int *adj_iters) 
   const cfd_grid_desc *g = &s->g;
   const dims_t d = s->d;
   const size_t G = d.G, N = d.N;
-  if (!s->adj_steps || P->source_index_quirk || P->op != CFD_OP_ELL || P->convection != CFD_CONV_QUICK) return -1;
+  if (!s->adj_steps || P->source_index_quirk || P->op != CFD_OP_ELL || P->convection != CFD_CONV_QUICK || P->les)
+    return -1;
   const int passes = s->adj_passes, L = passes - 1;
   const double dt = P->dt, nu = 1.0 / P->Re;
   const double wJ = 1.0 / (double)(s->adj_steps - s->adj_avg_from);

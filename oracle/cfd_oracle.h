@@ -55,6 +55,12 @@ void orc_predictor(const cfd_grid_desc *g, double nu, double dt,
 void orc_predictor_ex(const cfd_grid_desc *g, int scheme, double nu, double dt,
                       const double *u0, const double *v0, const double *w0,
                       double *u1, double *v1, double *w1);
+/* SmagorinskyModel (smagorinskyModel.hpp:6-79): eddy viscosity vis (N, icelCal) of (u, v, w) */
+void orc_smagorinsky(const cfd_grid_desc *g, const double *u, const double *v, const double *w, double *vis);
+/* the predictor with nu + vis[icelCal] in the diffusion (vis may be NULL) */
+void orc_predictor_les(const cfd_grid_desc *g, int scheme, double nu, double dt, const double *vis,
+                       const double *u0, const double *v0, const double *w0,
+                       double *u1, double *v1, double *w1);
 void orc_source_add(const cfd_grid_desc *g, int quirk, const double *src,
                     double *u1, double *v1, double *w1);
 void orc_source_accumulate(const cfd_grid_desc *g, double dt, const double *f, double *src);

@@ -21,7 +21,8 @@
 // restates only the sequencing, never the arithmetic.
 //
 // usage: ref_harness OUTDIR NSTEPS TOL OP(ell|spe) FN_STEP [DUMP_EVERY [SRC_AT_CELL]]
-// (ref_harness_lud / ref_harness_ud: the same with the LUD / UD convection scheme)
+// (ref_harness_lud / ref_harness_ud: the same with the LUD / UD convection scheme;
+//  ref_harness_les: with the Smagorinsky model, TERBULENCE_SMAGORINSKY)
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -131,6 +132,11 @@ static int synth_main() {
   T1.u.assign(T1.u.size(), 7.0);
   T1.v.assign(T1.v.size(), 7.0);
   T1.w.assign(T1.w.size(), 7.0);
+#ifdef TERBULENCE_SMAGORINSKY
+  // the eddy viscosity of the seeded T0 (RUN_CPU_PC_METHOD.hpp:298-300)
+  SmagorinskyModel(ShareM, simu, T0, t1, dom, gA);
+  dump("syn_les_viseff.bin", T0.Viseff);
+#endif
   ConvectionDifussion(simu, T0, T1, dom, gA);
   dump_vel("syn_pred_T1", T1);
   // source add with the driver's indexing (RUN_CPU_PC_METHOD.hpp:309-316)
@@ -428,6 +434,9 @@ int main(int argc, char **argv) {
     for (int pass = 0; pass < 4; ++pass) {
       const bool fnp = fn && pass == 1;
       if (fnp) { dump_vel("fn_pred_in_T0", T0); dump_vel("fn_pred_in_T1", T1); }
+#ifdef TERBULENCE_SMAGORINSKY
+      SmagorinskyModel(ShareM, simu, T0, t1, dom, gA);  // RUN_CPU_PC_METHOD.hpp:298-300, 449-451
+#endif
       ConvectionDifussion(simu, T0, T1, dom, gA);
       if (fnp) { dump_vel("fn_pred_out_T1", T1); dump("fn_src.bin", source_termF); }
       for (int c = 0; c < 3; c++)

@@ -6,8 +6,9 @@ case (190x210x40 SML grid, cavity table, Re 40, dt 0.002) and records:
   * SHA-256 of the grid metrics, the IB mask, the ELL operator (columns,
     values), gA.jp, the SPE Jacobi scaling and the CSR operator (row
     pointers, columns, values and its Jacobi scaling);
-  * the predictor with the LUD and UD convection schemes on the same seeded
-    velocities (ref_harness_lud / ref_harness_ud);
+  * the predictor with the LUD and UD convection schemes, and with the
+    Smagorinsky eddy viscosity (which it also records), on the same seeded
+    velocities (ref_harness_lud / _ud / _les);
   * the output writers (Plot3D grid and solution files, Time_cDcL.dat in
     both of its modes) and the Plot3D reader on seeded fields;
   * for every hot-path function applied once to seeded inputs (splitmix64
@@ -73,7 +74,9 @@ def main():
                    stdout=subprocess.DEVNULL)
     subprocess.run([h, str(DUMPS / "synth"), "synth"], check=True, env=env, stdout=subprocess.DEVNULL)
     subprocess.run([h, str(DUMPS / "io"), "io"], check=True, env=env, stdout=subprocess.DEVNULL)
-    for scheme in ("lud", "ud"):  # the predictor with the reference's other convection schemes
+    # the predictor with the reference's other convection schemes and with its
+    # Smagorinsky model
+    for scheme in ("lud", "ud", "les"):
         subprocess.run([h + "_" + scheme, str(DUMPS / f"synth_{scheme}"), "synth"], check=True, env=env,
                        stdout=subprocess.DEVNULL)
     reuse = "--reuse-dumps" in sys.argv
@@ -108,10 +111,10 @@ def main():
                                 "cd": float(v[4]), "cl": float(v[5])})
         elif k == "maxdiff":
             fx["steps"].append({"step": int(v[0]), "maxdiff": [float(x) for x in v[1:]]})
-    for scheme in ("lud", "ud"):
+    for scheme in ("lud", "ud", "les"):
         fx[f"synth_{scheme}"] = {}
-        for n in "uvw":
-            f = DUMPS / f"synth_{scheme}" / f"syn_pred_T1_{n}.bin"
+        for name in [f"syn_pred_T1_{n}.bin" for n in "uvw"] + (["syn_les_viseff.bin"] if scheme == "les" else []):
+            f = DUMPS / f"synth_{scheme}" / name
             a = np.fromfile(f)
             fx[f"synth_{scheme}"][f.name] = {"sha256": sha(f), "l2": float(np.linalg.norm(a)), **samples(a)}
     # output files: SHA-256 of the Plot3D files and of what QfileRead returns,

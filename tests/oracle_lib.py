@@ -49,6 +49,8 @@ def _declare(L):
     L.orc_eta_cylinder_z.argtypes = [G, C.c_double, C.c_double, C.c_double, C.c_int, dp]
     L.orc_predictor.argtypes = [G, C.c_double, C.c_double, dp, dp, dp, dp, dp, dp]
     L.orc_predictor_ex.argtypes = [G, C.c_int, C.c_double, C.c_double, dp, dp, dp, dp, dp, dp]
+    L.orc_smagorinsky.argtypes = [G, dp, dp, dp, dp]
+    L.orc_predictor_les.argtypes = [G, C.c_int, C.c_double, C.c_double, C.c_void_p, dp, dp, dp, dp, dp, dp]
     L.orc_source_add.argtypes = [G, C.c_int, dp, dp, dp, dp]
     L.orc_source_accumulate.argtypes = [G, C.c_double, dp, dp]
     L.orc_rhs.argtypes = [G, C.c_double, dp, dp, dp, dp, dp]

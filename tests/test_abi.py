@@ -33,11 +33,11 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_defaults():
     L = capi.load()
-    assert L.cfd_abi_version() == 3
+    assert L.cfd_abi_version() == 4
     p = default_params()
     # the reference's Jacobi-scaled solve unless multigrid is asked for
     assert p.precond == capi.CFD_PRECOND_JACOBI and p.mg_nu == 2
-    assert p.convection == capi.CFD_CONV_QUICK  # controlPanel.hpp:125
+    assert p.convection == capi.CFD_CONV_QUICK and p.les == 0  # controlPanel.hpp:125, 135
     # src/CFD_MX.cpp:35-57, simuClass.hpp:74-80, bicgstabRe2.hpp:57, setEta.hpp:145-152
     assert (p.Re, p.dt, p.tol, p.maxit, p.sor_omega) == (40.0, 0.002, 1e-3, 5000, 1.8)
     assert (p.ib_cx, p.ib_cy, p.ib_r, p.ib_nsub) == (6.5, 6.0, 0.5, 100)

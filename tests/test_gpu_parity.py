@@ -236,6 +236,48 @@ def test_steps_with_schemes_match_oracle(require_gpu, scheme):
             assert abs(st.cd[q] - ost.cd[q]) <= 1e-9 * max(1.0, abs(ost.cd[q]))
 
 
+@pytest.mark.parametrize("gname", list(GRIDS))
+def test_predictor_with_smagorinsky_matches_oracle(require_gpu, gname):
+    grid = GRIDS[gname]()
+    sim, p = make_sim(grid, quirk=1)
+    p.les = 1
+    sim = Simulation(grid, p)
+    sim.init_state()
+    g = O.Grid(grid)
+    T0 = random_state(grid, 31)
+    T1 = random_state(grid, 32)
+    for c in range(3):
+        sim.upload(capi.T0_U + c, T0[c])
+        sim.upload(capi.T1_U + c, T1[c])
+    sim.ConvectionDifussion()
+    vis = np.zeros(g.N)
+    O.lib().orc_smagorinsky(g.ref, *T0, vis)
+    assert vis.max() > 1e-3 * (1.0 / p.Re)  # the model is active on these fields
+    o1 = [t.copy() for t in T1]
+    O.lib().orc_predictor_les(g.ref, capi.CFD_CONV_QUICK, 1.0 / p.Re, p.dt, vis.ctypes.data, *T0, *o1)
+    for c in range(3):
+        assert rel_l2(sim.download(capi.T1_U + c), o1[c]) < 1e-13
+
+
+def test_steps_with_smagorinsky_match_oracle(require_gpu):
+    grid = GRIDS["sml"]()
+    sim, p = make_sim(grid, quirk=0, pd=(0, 1, 0, 0, 0, 0), tol=1e-13, inner=3)
+    p.les = 1
+    sim = Simulation(grid, p)
+    sim.setup(noise=1e-3, seed=3)
+    osim = O.Sim(O.Grid(grid), p)
+    osim.init_state(1e-3, 3)
+    osim.setup()
+    for k in range(3):
+        st = sim.step()
+        ost = osim.step()
+        for c in range(3):
+            e = rel_l2(sim.download(capi.T0_U + c), osim.field(capi.T0_U + c))
+            assert e < 1e-10, (k, c, e)
+        for q in range(st.n_passes):
+            assert abs(st.cd[q] - ost.cd[q]) <= 1e-9 * max(1.0, abs(ost.cd[q]))
+
+
 def oracle_csr(g):
     ptr, col, val, jp = np.zeros(g.N + 1, np.int32), np.zeros(7 * g.N, np.int32), np.zeros(7 * g.N), np.zeros(g.N)
     nnz = O.lib().orc_csr_assemble(g.ref, ptr, col, val, jp)

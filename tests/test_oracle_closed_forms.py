@@ -144,6 +144,34 @@ def test_lud_and_ud_are_the_textbook_schemes():
     assert all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
+def test_smagorinsky_pure_shear():
+    """u = s y on a uniform grid: the only strain is du/dy = s, so
+    nu_t = (0.18 h)^2 |s|; a field that is uniform has nu_t = 0, and with
+    nu_t the predictor's diffusion is that of nu + nu_t."""
+    g = ugrid((8, 10, 6), (2.0, 2.5, 1.5))
+    h = 0.25
+    L = O.lib()
+    for s_ in (0.7, -1.3):
+        u = field(g, lambda i, j, k: s_ * j * h + 0 * i)
+        zero = np.zeros(g.G)
+        vis = np.zeros(g.N)
+        L.orc_smagorinsky(g.ref, u, zero, zero, vis)
+        v3 = vis.reshape(8, 10, 6)[1:-1, 1:-1, 1:-1]  # away from the unfilled ghosts
+        assert np.allclose(v3, (0.18 * h) ** 2 * abs(s_), rtol=1e-12, atol=0)
+    vis = np.zeros(g.N)
+    L.orc_smagorinsky(g.ref, np.ones(g.G), np.zeros(g.G), np.zeros(g.G), vis)
+    assert np.all(vis == 0.0)
+    # constant nu_t: the predictor equals the plain one run with nu + nu_t
+    u = field(g, lambda i, j, k: 1.0 + 0.2 * np.sin(i + 0.5 * j) + 0.1 * k)
+    zero = np.zeros(g.G)
+    nut = np.full(g.N, 0.03)
+    a = [np.zeros(g.G) for _ in range(3)]
+    b = [np.zeros(g.G) for _ in range(3)]
+    L.orc_predictor_les(g.ref, capi.CFD_CONV_QUICK, 0.01, 0.002, nut.ctypes.data, u, zero, zero, *a)
+    L.orc_predictor_ex(g.ref, capi.CFD_CONV_QUICK, 0.01 + 0.03, 0.002, u, zero, zero, *b)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
 def test_ell_neumann_eigenpairs_and_row_sums():
     """All-Neumann uniform grid: A 1 = 0 and cos(pi m (i+1/2)/n) are
     eigenvectors with eigenvalue jp * sum(4/h^2 sin^2(pi m/(2n)))."""

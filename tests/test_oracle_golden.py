@@ -166,6 +166,28 @@ def test_predictor_lud_and_ud_against_reference(native, scheme, key):
         assert abs(np.linalg.norm(T1[c]) - f["l2"]) <= 1e-13 * f["l2"], n
 
 
+def test_smagorinsky_against_reference(native):
+    """SmagorinskyModel and the predictor with nu + nu_t, compiled from the
+    reference with TERBULENCE_SMAGORINSKY (ref_harness_les), on the seeded
+    velocities: nu_t to 1e-14 relative (GCC contracts the strain sum), the
+    predictor at its usual 1e-13."""
+    g = native["g"]
+    G, N = g.G, g.N
+    T0 = [O.noise(G, SEED, 1, 1.0, 0.3), O.noise(G, SEED, 2, 0.0, 0.3), O.noise(G, SEED, 3, 0.0, 0.3)]
+    vis = np.zeros(N)
+    O.lib().orc_smagorinsky(g.ref, *T0, vis)
+    f = FX["synth_les"]["syn_les_viseff.bin"]
+    idx, ref = np.array(f["idx"]), np.array(f["val"])
+    assert np.all(np.abs(vis[idx] - ref) <= 1e-14 * np.abs(ref))
+    assert abs(np.linalg.norm(vis) - f["l2"]) <= 1e-14 * f["l2"]
+    T1 = [np.full(G, 7.0) for _ in range(3)]
+    O.lib().orc_predictor_les(g.ref, capi.CFD_CONV_QUICK, 1.0 / 40.0, 0.002, vis.ctypes.data, *T0, *T1)
+    for c, n in enumerate("uvw"):
+        f = FX["synth_les"][f"syn_pred_T1_{n}.bin"]
+        idx, ref = np.array(f["idx"]), np.array(f["val"])
+        assert np.all(np.abs(T1[c][idx] - ref) <= 1e-13 * np.maximum(np.abs(ref), np.abs(ref).max() * 1e-3)), n
+
+
 def test_update_and_bc_bit_exact(native):
     """Stages whose inputs are pure seeded noise are bit-exact end to end."""
     g = native["g"]
