@@ -297,7 +297,7 @@ def main():
         print(json.dumps({"kernels": kernel_table(sim, args.reps)}), flush=True)
         return
     if args.mg_only:
-        sim64, tps64, iters64, kt, spmv = None, args.ref_tps, float("nan"), {}, {}
+        sim64, tps64, iters64, kt, spmv, st64 = None, args.ref_tps, float("nan"), {}, {}, []
     else:
         log(f"[bench] {w64.name}: warmup {warmup}, timed {args.steps}")
         sim64, t64, st64 = run_steps(w64, warmup, args.steps, "fp64")
@@ -396,6 +396,14 @@ def main():
         cpu = cpu_baseline(w64, iters64, threads)
         cpu["gpu_speedup"] = tps64 / cpu["value"]
 
+    # what the timed FP64 solves reached: bicgstabRe2 stops at iters_max
+    # (5000) as the reference does, whatever the residual then is
+    fp64_solves = None
+    if st64:
+        passes = [(it, rel) for s in st64 for it, rel in zip(s["iters"], s["relres"]) if it >= 0]
+        fp64_solves = {"passes": len(passes), "max_final_relres": max(rel for _, rel in passes),
+                       "passes_at_iteration_cap": sum(1 for it, _ in passes if it >= w64.params.maxit),
+                       "iteration_cap": w64.params.maxit}
     out = {
         "metric": "timesteps/s @512x256x64 (mixed vs FP64, same residual); Poisson SpMV HBM GB/s",
         "value": tps64,
@@ -415,6 +423,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": spmv["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv["gbs"] / HBM_PEAK_GBS, "traffic": profiled_traffic(),
                      "kernel": "ELL SpMV FP64 (100 B/row x 8388608 rows per launch)", "avg_us": spmv["avg_us"]},
+        "fp64_solves": fp64_solves,
         "kernels": kt,
         "mixed": mixed,
         "multigrid": mg,
