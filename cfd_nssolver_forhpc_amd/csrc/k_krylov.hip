@@ -500,7 +500,9 @@ int pbicgstab_solve_aux(Ctx &c, const double *b, double *x, double tol, int maxi
   if (!rc && pc32) rc = mg_ensure(c, false);  // the FP64 preconditioned directions
   if (rc) return rc;
   PVecs<double> v{x, c.r0, c.r1, c.p1, c.s1, c.ap, c.as, c.ph64, c.sh64, b};
-  return solve_pbicg<double>(c, op_args(c, CFD_OP_ELL), v, c.ks, tol, maxit, iters, relres, pc32);
+  // the configured operator (ELL, matrix-free SPE or CSR); the V-cycle
+  // approximates its inverse from the grid metrics
+  return solve_pbicg<double>(c, op_args(c, fp64_op(c)), v, c.ks, tol, maxit, iters, relres, pc32);
 }
 
 int pbicgstab_solve64(Ctx &c, const double *b, double *x, double tol, int maxit, bool pc32, int *iters,

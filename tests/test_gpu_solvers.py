@@ -164,9 +164,14 @@ def test_mg_on_odd_and_stretched_grids(require_gpu):
         assert rel < 1e-10 and tr < 2e-10, (prec, it, rel, tr)
 
 
-def test_mg_step_matches_jacobi_step(require_gpu):
+@pytest.mark.parametrize("op", [capi.CFD_OP_ELL, capi.CFD_OP_SPE])
+def test_mg_step_matches_jacobi_step(require_gpu, op):
+    """The multigrid-preconditioned solve applies the configured operator
+    (assembled ELL or the matrix-free SPE form) and lands on the Jacobi
+    solve's steps."""
     w = configs.small_cylinder(ncal=(32, 16, 8), lengths=(8.0, 4.0, 2.0), quirk=0, tol=1e-12)
     w.params.maxit = 50000
+    w.params.op = op
     a = Simulation(w.grid, w.params)
     a.setup(noise=w.noise, seed=w.seed)
     pm = w.params.copy()
