@@ -17,9 +17,18 @@
 // Dirichlet face keeps its mirror-ghost coefficient).  Restriction is the
 // volume-weighted sum of the children's residuals, prolongation adds the
 // parent's correction.  Smoother: red-black Gauss-Seidel, nu sweeps before
-// and after.  Levels of at most MG_SMALL cells run inside one 1024-thread
+// and after.  Levels of at most MG_SMALL cells run inside one 256-thread
 // workgroup (one launch for the whole bottom of the cycle); the coarsest is
-// smoothed MG_COARSE_SWEEPS times.
+// solved by its dense inverse or smoothed MG_COARSE_SWEEPS times.
+//
+// The finest level, two thirds of a cycle's bytes, keeps x and b
+// colour-split (ColourSplit) and has kernels of its own: colour sweeps whose
+// threads walk MG_PLANES planes and load each other-colour entry once for
+// its three uses, the post-smoothing's first sweep with the prolongation
+// folded in and its last one writing the result, and a restriction that
+// reads the children as colour pairs.  Each does the arithmetic of the
+// generic per-cell functions term for term, so the cycle's result is the
+// same bit for bit in either layout.
 //
 // Every kernel returns at once when the Krylov solver's `done` flag is set,
 // so preconditioned iterations can be launched in batches like the plain
@@ -97,22 +106,83 @@ __device__ inline Cell3 dec3(const MGView &L, int64_t t) {
 
 __device__ inline int64_t idx3(const MGView &L, int i, int j, int k) { return ((int64_t)i * L.ny + j) * L.nz + k; }
 
+// a cell's own index and its neighbours' (meaningful where the neighbour exists)
+struct Nb {
+  int64_t c, xm, xp, ym, yp, zm, zp;
+};
+
+// Layouts of a level's x and b.  Natural: (i ny + j) nz + k.
+struct Natural {
+  __device__ static int64_t at(const MGView &L, int i, int j, int k) { return idx3(L, i, j, k); }
+  __device__ static Nb nbrs(const MGView &L, int i, int j, int k) {
+    const int64_t c = idx3(L, i, j, k), sx = (int64_t)L.ny * L.nz, sy = L.nz;
+    return Nb{c, c - sx, c + sx, c - sy, c + sy, c - 1, c + 1};
+  }
+};
+
+// ColourSplit (the finest level of a multi-level cycle): the red cells
+// (i + j + k even) first, then the black ones, each colour at
+// (i ny + j) hz + k/2 with hz = ceil(nz / 2).  A colour sweep streams its
+// own half and reads the other at unit stride instead of touching every line
+// of both at stride 2.  For odd nz one colour has an unused last slot per
+// row.  The six neighbours of a cell are the other colour's entries at
+// h -+ ny hz, h -+ hz and, along k, h or h -+ 1 by the parity of k.
+struct ColourSplit {
+  __device__ static int64_t half(const MGView &L) { return (int64_t)L.nx * L.ny * ((L.nz + 1) >> 1); }
+  __device__ static int64_t at(const MGView &L, int i, int j, int k) {
+    const int hz = (L.nz + 1) >> 1;
+    const int64_t h = ((int64_t)i * L.ny + j) * hz + (k >> 1);
+    return ((i + j + k) & 1) ? half(L) + h : h;
+  }
+  __device__ static Nb nbrs(const MGView &L, int i, int j, int k) {
+    const int hz = (L.nz + 1) >> 1;
+    const int64_t h = ((int64_t)i * L.ny + j) * hz + (k >> 1), ns = half(L);
+    const bool black = (i + j + k) & 1;
+    const int64_t own = black ? ns + h : h, oth = black ? h : ns + h, sx = (int64_t)L.ny * hz;
+    return Nb{own, oth - sx, oth + sx, oth - hz, oth + hz, oth - ((k & 1) ? 0 : 1), oth + (k & 1)};
+  }
+};
+
 template <typename T>
 __device__ inline T mg_diag(const MGView &L, int i, int j, int k) {
   return (T)(L.ax[2 * L.nx + i] + L.ay[2 * L.ny + j] + L.az[2 * L.nz + k]);
 }
 
-// sum over the neighbours of coefficient * x
+// which of the six neighbours exist
+struct Has {
+  bool xm, xp, ym, yp, zm, zp;
+};
+__device__ inline Has has6(const MGView &L, int i, int j, int k) {
+  return Has{i > 0, i < L.nx - 1, j > 0, j < L.ny - 1, k > 0, k < L.nz - 1};
+}
+
+// the six neighbour coefficients (an absent neighbour's entry is 0 and in range)
 template <typename T>
-__device__ inline T mg_off(const MGView &L, const T *__restrict__ x, int i, int j, int k, int64_t c) {
-  const int64_t sx = (int64_t)L.ny * L.nz, sy = L.nz;
+struct Coef6 {
+  T xm, xp, ym, yp, zm, zp;
+};
+template <typename T>
+__device__ inline Coef6<T> coef6(const MGView &L, int i, int j, int k) {
+  return Coef6<T>{(T)L.ax[i], (T)L.ax[L.nx + i], (T)L.ay[j], (T)L.ay[L.ny + j], (T)L.az[k], (T)L.az[L.nz + k]};
+}
+
+// sum over the neighbours of coefficient * x, in the order -x +x -y +y -z
+// +z.  Every load is issued before the sum (an absent neighbour reads the
+// cell itself and is left out by a select), so a wave has all of them in
+// flight at once instead of one exec-masked load and wait per neighbour.
+template <typename T>
+__device__ inline T mg_off(const MGView &L, const T *__restrict__ x, int i, int j, int k, const Nb &n) {
+  const Has h = has6(L, i, j, k);
+  const T vxm = x[h.xm ? n.xm : n.c], vxp = x[h.xp ? n.xp : n.c], vym = x[h.ym ? n.ym : n.c];
+  const T vyp = x[h.yp ? n.yp : n.c], vzm = x[h.zm ? n.zm : n.c], vzp = x[h.zp ? n.zp : n.c];
+  const Coef6<T> a = coef6<T>(L, i, j, k);
   T s = (T)0;
-  if (i > 0) s += (T)L.ax[i] * x[c - sx];
-  if (i < L.nx - 1) s += (T)L.ax[L.nx + i] * x[c + sx];
-  if (j > 0) s += (T)L.ay[j] * x[c - sy];
-  if (j < L.ny - 1) s += (T)L.ay[L.ny + j] * x[c + sy];
-  if (k > 0) s += (T)L.az[k] * x[c - 1];
-  if (k < L.nz - 1) s += (T)L.az[L.nz + k] * x[c + 1];
+  s = h.xm ? s + a.xm * vxm : s;
+  s = h.xp ? s + a.xp * vxp : s;
+  s = h.ym ? s + a.ym * vym : s;
+  s = h.yp ? s + a.yp * vyp : s;
+  s = h.zm ? s + a.zm * vzm : s;
+  s = h.zp ? s + a.zp * vzp : s;
   return s;
 }
 
@@ -123,13 +193,13 @@ __device__ inline double mg_vol(const MGView &L, int i, int j, int k) {
 
 // one colour of a red-black Gauss-Seidel sweep at (i, j, k2), k = 2 k2 +
 // parity (colour 0 = red: i + j + k even)
-template <typename T>
+template <typename T, typename Lay>
 __device__ inline void rb_at(const MGView &L, int i, int j, int k2, int colour, const T *__restrict__ b,
                              T *__restrict__ x) {
   const int k = 2 * k2 + ((i + j + colour) & 1);
   if (k >= L.nz) return;
-  const int64_t c = idx3(L, i, j, k);
-  x[c] = (b[c] + mg_off<T>(L, x, i, j, k, c)) / mg_diag<T>(L, i, j, k);
+  const Nb n = Lay::nbrs(L, i, j, k);
+  x[n.c] = (b[n.c] + mg_off<T>(L, x, i, j, k, n)) / mg_diag<T>(L, i, j, k);
 }
 
 // parent of fine cell (i, j, k) on the next coarser level
@@ -141,65 +211,76 @@ __device__ inline int64_t parent(const MGView &F, const MGView &C, int i, int j,
 // folded in: the black neighbours are read as x + xc[parent].  Red cells are
 // overwritten here and black cells by the black sweep that follows, so the
 // corrected values are only ever needed as neighbours.
-template <typename T>
+template <typename T, typename Lay>
 __device__ inline void rb_prolong_at(const MGView &F, const MGView &C, int i, int j, int k2, const T *__restrict__ b,
                                      T *__restrict__ x, const T *__restrict__ xc) {
   const int k = 2 * k2 + ((i + j) & 1);
   if (k >= F.nz) return;
-  const int64_t c = idx3(F, i, j, k);
-  const int64_t sx = (int64_t)F.ny * F.nz, sy = F.nz;
+  const Nb n = Lay::nbrs(F, i, j, k);
+  const Has h = has6(F, i, j, k);
+  // all loads first (absent neighbours read the cell and its parent), as in mg_off
+  const T vxm = x[h.xm ? n.xm : n.c] + xc[parent(F, C, h.xm ? i - 1 : i, j, k)];
+  const T vxp = x[h.xp ? n.xp : n.c] + xc[parent(F, C, h.xp ? i + 1 : i, j, k)];
+  const T vym = x[h.ym ? n.ym : n.c] + xc[parent(F, C, i, h.ym ? j - 1 : j, k)];
+  const T vyp = x[h.yp ? n.yp : n.c] + xc[parent(F, C, i, h.yp ? j + 1 : j, k)];
+  const T vzm = x[h.zm ? n.zm : n.c] + xc[parent(F, C, i, j, h.zm ? k - 1 : k)];
+  const T vzp = x[h.zp ? n.zp : n.c] + xc[parent(F, C, i, j, h.zp ? k + 1 : k)];
+  const Coef6<T> a = coef6<T>(F, i, j, k);
   T s = (T)0;
-  if (i > 0) s += (T)F.ax[i] * (x[c - sx] + xc[parent(F, C, i - 1, j, k)]);
-  if (i < F.nx - 1) s += (T)F.ax[F.nx + i] * (x[c + sx] + xc[parent(F, C, i + 1, j, k)]);
-  if (j > 0) s += (T)F.ay[j] * (x[c - sy] + xc[parent(F, C, i, j - 1, k)]);
-  if (j < F.ny - 1) s += (T)F.ay[F.ny + j] * (x[c + sy] + xc[parent(F, C, i, j + 1, k)]);
-  if (k > 0) s += (T)F.az[k] * (x[c - 1] + xc[parent(F, C, i, j, k - 1)]);
-  if (k < F.nz - 1) s += (T)F.az[F.nz + k] * (x[c + 1] + xc[parent(F, C, i, j, k + 1)]);
-  x[c] = (b[c] + s) / mg_diag<T>(F, i, j, k);
+  s = h.xm ? s + a.xm * vxm : s;
+  s = h.xp ? s + a.xp * vxp : s;
+  s = h.ym ? s + a.ym * vym : s;
+  s = h.yp ? s + a.yp * vyp : s;
+  s = h.zm ? s + a.zm * vzm : s;
+  s = h.zp ? s + a.zp * vzp : s;
+  x[n.c] = (b[n.c] + s) / mg_diag<T>(F, i, j, k);
 }
 
-// a new right-hand side at cell c and the first red half-sweep from x = 0
-template <typename T>
+// a new right-hand side at cell (i, j, k) and the first red half-sweep from x = 0
+template <typename T, typename Lay>
 __device__ inline void set_rhs(const MGView &L, int i, int j, int k, T bv, T *__restrict__ b, T *__restrict__ x) {
-  const int64_t c = idx3(L, i, j, k);
+  const int64_t c = Lay::at(L, i, j, k);
   b[c] = bv;
   x[c] = ((i + j + k) & 1) == 0 ? bv / mg_diag<T>(L, i, j, k) : (T)0;
 }
 
 // residual of the children of coarse cell (I, J, K), volume-weighted into
-// the coarse right-hand side (which starts its own first red half-sweep)
-template <typename T>
+// the coarse right-hand side (which starts its own first red half-sweep);
+// the coarse level is always in natural layout
+template <typename T, typename Lay>
 __device__ inline double child_res(const MGView &F, int i, int j, int k, const T *__restrict__ bf,
                                    const T *__restrict__ xf) {
-  const int64_t c = idx3(F, i, j, k);
-  const T r = bf[c] - (mg_diag<T>(F, i, j, k) * xf[c] - mg_off<T>(F, xf, i, j, k, c));
+  const Nb n = Lay::nbrs(F, i, j, k);
+  const T r = bf[n.c] - (mg_diag<T>(F, i, j, k) * xf[n.c] - mg_off<T>(F, xf, i, j, k, n));
   return mg_vol<T>(F, i, j, k) * (double)r;
 }
 
-template <typename T>
+template <typename T, typename Lay>
 __device__ inline void restrict_at(const MGView &F, const MGView &C, int I, int J, int K, const T *__restrict__ bf,
                                    const T *__restrict__ xf, T *__restrict__ bc, T *__restrict__ xc) {
   double acc = 0.0;
   if (F.fx == 2 && F.fy == 2 && F.fz == 2) {
 #pragma unroll
-    for (int d = 0; d < 8; ++d) acc += child_res<T>(F, 2 * I + (d >> 2), 2 * J + ((d >> 1) & 1), 2 * K + (d & 1), bf, xf);
+    for (int d = 0; d < 8; ++d)
+      acc += child_res<T, Lay>(F, 2 * I + (d >> 2), 2 * J + ((d >> 1) & 1), 2 * K + (d & 1), bf, xf);
   } else {
     for (int di = 0; di < F.fx; ++di)
       for (int dj = 0; dj < F.fy; ++dj)
         for (int dk = 0; dk < F.fz; ++dk)
-          acc += child_res<T>(F, I * F.fx + di, J * F.fy + dj, K * F.fz + dk, bf, xf);
+          acc += child_res<T, Lay>(F, I * F.fx + di, J * F.fy + dj, K * F.fz + dk, bf, xf);
   }
-  set_rhs<T>(C, I, J, K, (T)(acc / mg_vol<T>(C, I, J, K)), bc, xc);
+  set_rhs<T, Natural>(C, I, J, K, (T)(acc / mg_vol<T>(C, I, J, K)), bc, xc);
 }
 
-template <typename T, typename V>
+// b = r / jp on the finest level (r and jp in natural layout, b and x in Lay)
+template <typename T, typename V, typename Lay>
 __global__ void __launch_bounds__(256) mg_scale_kernel(MGView L, const V *__restrict__ r, const double *__restrict__ jp,
                                                        T *__restrict__ b, T *__restrict__ x, const KrylovScalars *ks) {
   if (ks && ks->done) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
   if (k >= L.nz || j >= L.ny) return;
   const int64_t c = idx3(L, i, j, k);
-  set_rhs<T>(L, i, j, k, (T)((double)r[c] / jp[c]), b, x);
+  set_rhs<T, Lay>(L, i, j, k, (T)((double)r[c] / jp[c]), b, x);
 }
 
 template <typename T, typename V>
@@ -211,33 +292,204 @@ __global__ void __launch_bounds__(256) mg_widen_kernel(int64_t n, const T *__res
 }
 
 // 3-D launches (x: k or k2, y: j, z: i): no 64-bit index division per thread
-template <typename T>
+template <typename T, typename Lay>
 __global__ void __launch_bounds__(256) mg_rb_kernel(MGView L, int colour, const T *__restrict__ b, T *__restrict__ x,
                                                     const KrylovScalars *ks) {
   if (ks && ks->done) return;
   const int k2 = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
   if (k2 >= (L.nz + 1) / 2 || j >= L.ny) return;
-  rb_at<T>(L, i, j, k2, colour, b, x);
+  rb_at<T, Lay>(L, i, j, k2, colour, b, x);
 }
 
+// planes of the colour-split finest level that one thread of a sweep walks
+constexpr int MG_PLANES = 8;
+
+enum SplitSweep { SWEEP_PLAIN, SWEEP_PROLONG, SWEEP_OUT };
+
+// One colour of a red-black sweep of the colour-split finest level.  Thread
+// (k2, j) walks the planes i0 .. i0 + MG_PLANES - 1 (i0 = MG_PLANES
+// blockIdx.z) and updates the cell of the colour at (i, j, k2) of each.  The
+// other colour's entry at (i', j, k2), cen(i'), is loaded once: it is the -x
+// neighbour of plane i' + 1, the +x neighbour of plane i' - 1 and one of the
+// two k neighbours of plane i' (the other is the entry at k2 -+ 1).  The y
+// and z coefficients are loaded once per thread (the z ones for both
+// parities of k, which alternate from plane to plane), the x ones are
+// plane-uniform.  The arithmetic is rb_at's, term for term.
+//   SWEEP_PROLONG: the first red half-sweep after the coarse correction: the
+//     neighbours are read as x + xc[parent], as in rb_prolong_at.
+//   SWEEP_OUT: the cycle's last black half-sweep; it writes the new black
+//     value and the red one (its cen) to out in natural layout, precision V.
+template <typename T, typename V, int MODE>
+__global__ void __launch_bounds__(256) mg_split_sweep_kernel(MGView L, MGView C, int colour, const T *__restrict__ b,
+                                                             T *__restrict__ x, const T *__restrict__ xc,
+                                                             V *__restrict__ out, const KrylovScalars *ks) {
+  if (ks && ks->done) return;
+  const int k2 = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int hz = (L.nz + 1) >> 1;
+  if (k2 >= hz || j >= L.ny) return;
+  const int i0 = blockIdx.z * MG_PLANES;
+  const int64_t ns = (int64_t)L.nx * L.ny * hz, sx = (int64_t)L.ny * hz;
+  T *__restrict__ own = x + (colour ? ns : 0);
+  const T *__restrict__ oth = x + (colour ? 0 : ns);
+  const T *__restrict__ bown = b + (colour ? ns : 0);
+  const int64_t col = (int64_t)j * hz + k2;  // position of (0, j, k2) in a colour
+  const bool hym = j > 0, hyp = j < L.ny - 1;
+  const T cym = (T)L.ay[j], cyp = (T)L.ay[L.ny + j];
+  const double dy = L.ay[2 * L.ny + j];
+  T czm[2], czp[2];
+  double dz[2];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int k = min(2 * k2 + e, L.nz - 1);  // the clamped entry of an absent cell is never used
+    czm[e] = (T)L.az[k];
+    czp[e] = (T)L.az[L.nz + k];
+    dz[e] = L.az[2 * L.nz + k];
+  }
+  // the parent's indices along y and z of a neighbour at (j', k'); x is per plane
+  auto par = [&](int ip, int jp, int kp) -> int64_t {
+    return idx3(C, L.fx == 2 ? ip >> 1 : ip, L.fy == 2 ? jp >> 1 : jp, L.fz == 2 ? kp >> 1 : kp);
+  };
+  T cen[MG_PLANES + 2];
+#pragma unroll
+  for (int p = 0; p < MG_PLANES + 2; ++p) {
+    const int ip = min(max(i0 - 1 + p, 0), L.nx - 1);  // clamped planes are never used
+    cen[p] = oth[ip * sx + col];
+    if (MODE == SWEEP_PROLONG) {
+      // the other colour's cell at (ip, j, k2)
+      const int kb = 2 * k2 + ((ip + j + colour + 1) & 1);
+      cen[p] += xc[par(ip, j, min(kb, L.nz - 1))];
+    }
+  }
+  // every plane's loads are issued before any result is stored (planes past
+  // the grid and the padding slot of odd nz load clamped entries and store
+  // nothing), so the wave has them all in flight at once
+  T vym[MG_PLANES], vyp[MG_PLANES], side[MG_PLANES], bv[MG_PLANES];
+#pragma unroll
+  for (int p = 0; p < MG_PLANES; ++p) {
+    const int i = min(i0 + p, L.nx - 1);
+    const int e = (i + j + colour) & 1;
+    const int k = min(2 * k2 + e, L.nz - 1);
+    const int64_t h = i * sx + col;
+    // the k neighbour that is not cen: k2 - 1 below an even k, k2 + 1 above an odd one
+    const bool hside = e ? k < L.nz - 1 : k > 0;
+    vym[p] = oth[hym ? h - hz : h];
+    vyp[p] = oth[hyp ? h + hz : h];
+    side[p] = oth[hside ? (e ? h + 1 : h - 1) : h];
+    bv[p] = bown[h];
+    if (MODE == SWEEP_PROLONG) {
+      vym[p] += xc[par(i, hym ? j - 1 : j, k)];
+      vyp[p] += xc[par(i, hyp ? j + 1 : j, k)];
+      side[p] += xc[par(i, j, hside ? (e ? k + 1 : k - 1) : k)];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < MG_PLANES; ++p) {
+    const int i = i0 + p;
+    const int e = (i + j + colour) & 1;
+    const int k = 2 * k2 + e;
+    if (MODE == SWEEP_OUT && i < L.nx && 2 * k2 + 1 - e < L.nz) out[idx3(L, i, j, 2 * k2 + 1 - e)] = (V)cen[p + 1];
+    if (i >= L.nx || k >= L.nz) continue;
+    const bool hxm = i > 0, hxp = i < L.nx - 1, hzm = k > 0, hzp = k < L.nz - 1;
+    const T vzm = e ? cen[p + 1] : side[p], vzp = e ? side[p] : cen[p + 1];
+    const T cxm = (T)L.ax[i], cxp = (T)L.ax[L.nx + i];
+    T s = (T)0;
+    s = hxm ? s + cxm * cen[p] : s;
+    s = hxp ? s + cxp * cen[p + 2] : s;
+    s = hym ? s + cym * vym[p] : s;
+    s = hyp ? s + cyp * vyp[p] : s;
+    s = hzm ? s + czm[e] * vzm : s;
+    s = hzp ? s + czp[e] * vzp : s;
+    const T v = (bv[p] + s) / (T)(L.ax[2 * L.nx + i] + dy + dz[e]);
+    if (MODE == SWEEP_OUT)
+      out[idx3(L, i, j, k)] = (V)v;
+    else
+      own[i * sx + col] = v;
+  }
+}
+
+// Restriction from the colour-split finest level when every axis is halved.
+// The children of (I, J, K) are the entries at position K of the four (i, j)
+// columns 2I..2I+1 x 2J..2J+1, one of each colour per column (k = 2K and
+// 2K + 1).  The kernel loads those pairs, the pairs of the eight columns
+// beside them and the single entries below (2K - 1) and above (2K + 2) each
+// column (40 loads for 8 children, with b), and sums the children's
+// residuals in restrict_at's order with child_res's arithmetic.
 template <typename T>
+__global__ void __launch_bounds__(256) mg_restrict_split_kernel(MGView F, MGView C, const T *__restrict__ bf,
+                                                                const T *__restrict__ xf, T *__restrict__ bc,
+                                                                T *__restrict__ xc, const KrylovScalars *ks) {
+  if (ks && ks->done) return;
+  const int K = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y * blockDim.y + threadIdx.y, I = blockIdx.z;
+  if (K >= C.nz || J >= C.ny) return;
+  const int hz = F.nz >> 1;  // nz is even where z is halved
+  const int64_t ns = (int64_t)F.nx * F.ny * hz;
+  // entry at k = 2 KK + m (m = 0, 1) of column (i, j), both clamped into the grid
+  auto ld = [&](const T *a, int i, int j, int KK, int m) -> T {
+    i = min(max(i, 0), F.nx - 1);
+    j = min(max(j, 0), F.ny - 1);
+    KK = min(max(KK, 0), hz - 1);
+    return a[(((i + j + m) & 1) ? ns : 0) + ((int64_t)i * F.ny + j) * hz + KK];
+  };
+  // X[a][c] = the pair of column (2I - 1 + a, 2J - 1 + c): the four children's
+  // columns (a, c in 1..2) and the eight beside them (no corners)
+  T X[4][4][2], B[2][2][2], xkm[2][2], xkp[2][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+        X[a][c][m] = ((a == 0 || a == 3) && (c == 0 || c == 3)) ? (T)0 : ld(xf, 2 * I - 1 + a, 2 * J - 1 + c, K, m);
+#pragma unroll
+  for (int di = 0; di < 2; ++di)
+#pragma unroll
+    for (int dj = 0; dj < 2; ++dj) {
+      const int i = 2 * I + di, j = 2 * J + dj;
+      B[di][dj][0] = ld(bf, i, j, K, 0);
+      B[di][dj][1] = ld(bf, i, j, K, 1);
+      xkm[di][dj] = ld(xf, i, j, K - 1, 1);  // k = 2K - 1
+      xkp[di][dj] = ld(xf, i, j, K + 1, 0);  // k = 2K + 2
+    }
+  double acc = 0.0;
+#pragma unroll
+  for (int d = 0; d < 8; ++d) {
+    const int di = d >> 2, dj = (d >> 1) & 1, m = d & 1;
+    const int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + m;
+    const Has h = has6(F, i, j, k);
+    const Coef6<T> a = coef6<T>(F, i, j, k);
+    const T xv = X[di + 1][dj + 1][m];
+    const T vzm = m ? X[di + 1][dj + 1][0] : xkm[di][dj], vzp = m ? xkp[di][dj] : X[di + 1][dj + 1][1];
+    T s = (T)0;
+    s = h.xm ? s + a.xm * X[di][dj + 1][m] : s;
+    s = h.xp ? s + a.xp * X[di + 2][dj + 1][m] : s;
+    s = h.ym ? s + a.ym * X[di + 1][dj][m] : s;
+    s = h.yp ? s + a.yp * X[di + 1][dj + 2][m] : s;
+    s = h.zm ? s + a.zm * vzm : s;
+    s = h.zp ? s + a.zp * vzp : s;
+    const T r = B[di][dj][m] - (mg_diag<T>(F, i, j, k) * xv - s);
+    acc += mg_vol<T>(F, i, j, k) * (double)r;
+  }
+  set_rhs<T, Natural>(C, I, J, K, (T)(acc / mg_vol<T>(C, I, J, K)), bc, xc);
+}
+
+template <typename T, typename Lay>
 __global__ void __launch_bounds__(256) mg_rb_prolong_kernel(MGView F, MGView C, const T *__restrict__ b,
                                                             T *__restrict__ x, const T *__restrict__ xc,
                                                             const KrylovScalars *ks) {
   if (ks && ks->done) return;
   const int k2 = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
   if (k2 >= (F.nz + 1) / 2 || j >= F.ny) return;
-  rb_prolong_at<T>(F, C, i, j, k2, b, x, xc);
+  rb_prolong_at<T, Lay>(F, C, i, j, k2, b, x, xc);
 }
 
-template <typename T>
+template <typename T, typename Lay>
 __global__ void __launch_bounds__(256) mg_restrict_kernel(MGView F, MGView C, const T *__restrict__ bf,
                                                           const T *__restrict__ xf, T *__restrict__ bc,
                                                           T *__restrict__ xc, const KrylovScalars *ks) {
   if (ks && ks->done) return;
   const int K = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y * blockDim.y + threadIdx.y, I = blockIdx.z;
   if (K >= C.nz || J >= C.ny) return;
-  restrict_at<T>(F, C, I, J, K, bf, xf, bc, xc);
+  restrict_at<T, Lay>(F, C, I, J, K, bf, xf, bc, xc);
 }
 
 constexpr int MG_SMALL_THREADS = 256;
@@ -274,9 +526,9 @@ __global__ void __launch_bounds__(MG_SMALL_THREADS) mg_small_kernel(SmallArgs<T>
         for (int t = threadIdx.x; t < cnt; t += nt) {
           const int k2 = t % half, r = t / half;
           if (fold)
-            rb_prolong_at<T>(L, S.v[prolong_from], r / L.ny, r % L.ny, k2, S.b[l], S.x[l], S.x[prolong_from]);
+            rb_prolong_at<T, Natural>(L, S.v[prolong_from], r / L.ny, r % L.ny, k2, S.b[l], S.x[l], S.x[prolong_from]);
           else
-            rb_at<T>(L, r / L.ny, r % L.ny, k2, colour, S.b[l], S.x[l]);
+            rb_at<T, Natural>(L, r / L.ny, r % L.ny, k2, colour, S.b[l], S.x[l]);
         }
         __syncthreads();
       }
@@ -287,7 +539,7 @@ __global__ void __launch_bounds__(MG_SMALL_THREADS) mg_small_kernel(SmallArgs<T>
     const MGView &C = S.v[l + 1];
     for (int t = threadIdx.x; t < (int)C.n; t += nt) {
       const int K = t % C.nz, r = t / C.nz;
-      restrict_at<T>(S.v[l], C, r / C.ny, r % C.ny, K, S.b[l], S.x[l], S.b[l + 1], S.x[l + 1]);
+      restrict_at<T, Natural>(S.v[l], C, r / C.ny, r % C.ny, K, S.b[l], S.x[l], S.b[l + 1], S.x[l + 1]);
     }
     __syncthreads();
   }
@@ -336,46 +588,96 @@ double *lb<double>(MGLevel &L) { return L.b64; }
 template <>
 float *lb<float>(MGLevel &L) { return L.b32; }
 
+// level l's kernels in layout Lay
+template <typename T, typename Lay>
+struct LevelOps {
+  template <typename V>
+  static void scale_from(const MGView &v, const V *r, const double *jp, T *b, T *x, const KrylovScalars *ks,
+                         hipStream_t st) {
+    const Shape3 sh = shape3(v.nz, v.ny, v.nx);
+    mg_scale_kernel<T, V, Lay><<<sh.grid, sh.block, 0, st>>>(v, r, jp, b, x, ks);
+  }
+  static void rb(const MGView &v, int colour, const T *b, T *x, const KrylovScalars *ks, hipStream_t st) {
+    const Shape3 sh = shape3((v.nz + 1) / 2, v.ny, v.nx);
+    mg_rb_kernel<T, Lay><<<sh.grid, sh.block, 0, st>>>(v, colour, b, x, ks);
+  }
+  static void rb_prolong(const MGView &v, const MGView &C, const T *b, T *x, const T *xc, const KrylovScalars *ks,
+                         hipStream_t st) {
+    const Shape3 sh = shape3((v.nz + 1) / 2, v.ny, v.nx);
+    mg_rb_prolong_kernel<T, Lay><<<sh.grid, sh.block, 0, st>>>(v, C, b, x, xc, ks);
+  }
+  static void restrict_to(const MGView &v, const MGView &C, const T *b, const T *x, T *bc, T *xc,
+                          const KrylovScalars *ks, hipStream_t st) {
+    const Shape3 sh = shape3(C.nz, C.ny, C.nx);
+    mg_restrict_kernel<T, Lay><<<sh.grid, sh.block, 0, st>>>(v, C, b, x, bc, xc, ks);
+  }
+};
+
 // out = V-cycle(L, r / jp): the cycle runs in T, the vectors r and out are V
-// (an FP32 cycle inside an FP64 Krylov solve widens its result at the end)
+// (an FP32 cycle inside an FP64 Krylov solve widens its result at the end).
+// When the finest level is smoothed by the level kernels (the cycle has more
+// than the one-workgroup bottom), its x and b are colour-split and its last
+// black half-sweep writes out; otherwise the bottom kernel works on out
+// directly (or on x, widened into out).
 template <typename T, typename V>
 int mg_apply(Ctx &c, const V *r, V *out, const KrylovScalars *ks) {
   MGState &M = *c.mg;
   const int nl = (int)M.lv.size();
+  const int ls = M.small_from;
+  const bool split = ls > 0;
   T *b0 = lb<T>(M.lv[0]);
   constexpr bool same = std::is_same<T, V>::value;
-  T *x0 = same ? reinterpret_cast<T *>(out) : lx<T>(M.lv[0]);
+  T *x0 = (same && !split) ? reinterpret_cast<T *>(out) : lx<T>(M.lv[0]);
   if (!b0 || !x0) return fail(CFD_ERR_STATE, "multigrid buffers of this precision are not allocated");
   auto X = [&](int l) { return l == 0 ? x0 : lx<T>(M.lv[l]); };
-  {
-    const MGView &v = M.lv[0].v;
-    const Shape3 sh = shape3(v.nz, v.ny, v.nx);
-    mg_scale_kernel<T, V><<<sh.grid, sh.block, 0, c.stream>>>(v, r, c.f[CFD_JP], b0, x0, ks);
-  }
+  if (split)
+    LevelOps<T, ColourSplit>::scale_from(M.lv[0].v, r, c.f[CFD_JP], b0, x0, ks, c.stream);
+  else
+    LevelOps<T, Natural>::scale_from(M.lv[0].v, r, c.f[CFD_JP], b0, x0, ks, c.stream);
   // nu sweeps of level l: pre-smoothing skips the red half-sweep its
   // right-hand side's producer did; post-smoothing folds the prolongation
-  // into its first red half-sweep
+  // into its first red half-sweep, and on the finest level ends in the
+  // half-sweep that writes out
   auto smooth = [&](int l, bool pre) {
     const MGView &v = M.lv[l].v;
-    const Shape3 sh = shape3((v.nz + 1) / 2, v.ny, v.nx);
     for (int s = 0; s < M.nu; ++s)
       for (int colour = 0; colour < 2; ++colour) {
-        if (s == 0 && colour == 0) {
-          if (pre) continue;
-          mg_rb_prolong_kernel<T><<<sh.grid, sh.block, 0, c.stream>>>(v, M.lv[l + 1].v, lb<T>(M.lv[l]), X(l),
-                                                                      X(l + 1), ks);
-          continue;
+        const bool first = s == 0 && colour == 0, last = s == M.nu - 1 && colour == 1;
+        if (first && pre) continue;
+        if (l == 0) {
+          Shape3 sh = shape3((v.nz + 1) / 2, v.ny, v.nx);
+          sh.grid.z = (v.nx + MG_PLANES - 1) / MG_PLANES;
+          const MGView &C = M.lv[1].v;
+          if (first)
+            mg_split_sweep_kernel<T, T, SWEEP_PROLONG><<<sh.grid, sh.block, 0, c.stream>>>(v, C, 0, b0, x0, X(1),
+                                                                                          nullptr, ks);
+          else if (last && !pre)
+            mg_split_sweep_kernel<T, V, SWEEP_OUT><<<sh.grid, sh.block, 0, c.stream>>>(v, C, 1, b0, x0, nullptr, out,
+                                                                                      ks);
+          else
+            mg_split_sweep_kernel<T, T, SWEEP_PLAIN><<<sh.grid, sh.block, 0, c.stream>>>(v, C, colour, b0, x0,
+                                                                                        nullptr, nullptr, ks);
+        } else {
+          using Ops = LevelOps<T, Natural>;
+          if (first)
+            Ops::rb_prolong(v, M.lv[l + 1].v, lb<T>(M.lv[l]), X(l), X(l + 1), ks, c.stream);
+          else
+            Ops::rb(v, colour, lb<T>(M.lv[l]), X(l), ks, c.stream);
         }
-        mg_rb_kernel<T><<<sh.grid, sh.block, 0, c.stream>>>(v, colour, lb<T>(M.lv[l]), X(l), ks);
       }
   };
-  const int ls = M.small_from;
   for (int l = 0; l < ls; ++l) {
     smooth(l, true);
-    const MGView &C = M.lv[l + 1].v;
-    const Shape3 sh = shape3(C.nz, C.ny, C.nx);
-    mg_restrict_kernel<T><<<sh.grid, sh.block, 0, c.stream>>>(M.lv[l].v, C, lb<T>(M.lv[l]), X(l),
-                                                               lb<T>(M.lv[l + 1]), X(l + 1), ks);
+    const MGView &F = M.lv[l].v;
+    if (l == 0 && F.fx == 2 && F.fy == 2 && F.fz == 2) {
+      const MGView &C = M.lv[1].v;
+      const Shape3 sh = shape3(C.nz, C.ny, C.nx);
+      mg_restrict_split_kernel<T><<<sh.grid, sh.block, 0, c.stream>>>(F, C, b0, x0, lb<T>(M.lv[1]), X(1), ks);
+    } else if (l == 0)
+      LevelOps<T, ColourSplit>::restrict_to(F, M.lv[1].v, b0, x0, lb<T>(M.lv[1]), X(1), ks, c.stream);
+    else
+      LevelOps<T, Natural>::restrict_to(M.lv[l].v, M.lv[l + 1].v, lb<T>(M.lv[l]), X(l), lb<T>(M.lv[l + 1]),
+                                        X(l + 1), ks, c.stream);
   }
   SmallArgs<T> S;
   std::memset(&S, 0, sizeof(S));
@@ -391,7 +693,7 @@ int mg_apply(Ctx &c, const V *r, V *out, const KrylovScalars *ks) {
   }
   mg_small_kernel<T><<<1, MG_SMALL_THREADS, 0, c.stream>>>(S, ks);
   for (int l = ls - 1; l >= 0; --l) smooth(l, false);
-  if (!same) mg_widen_kernel<T, V><<<nblk(c.g.N), 256, 0, c.stream>>>(c.g.N, x0, out, ks);
+  if (!same && !split) mg_widen_kernel<T, V><<<nblk(c.g.N), 256, 0, c.stream>>>(c.g.N, x0, out, ks);
   CFD_HIP(hipGetLastError());
   return CFD_OK;
 }
@@ -558,13 +860,15 @@ int mg_ensure(Ctx &c, bool fp32) {
   M.nu = c.prm.mg_nu > 0 ? c.prm.mg_nu : 2;
   for (size_t l = 0; l < M.lv.size(); ++l) {
     MGLevel &L = M.lv[l];
+    // the finest level's colour-split vectors hold nx ny ceil(nz / 2) cells per colour
+    const size_t n = l == 0 ? (size_t)L.v.nx * L.v.ny * (2 * ((L.v.nz + 1) / 2)) : (size_t)L.v.n;
     int rc = CFD_OK;
     if (fp32) {
-      if (!L.b32 && (rc = mg_alloc(&L.b32, L.v.n))) return rc;
-      if (!L.x32 && (rc = mg_alloc(&L.x32, L.v.n))) return rc;  // level 0 too: the FP64 solve's FP32 cycle
+      if (!L.b32 && (rc = mg_alloc(&L.b32, n))) return rc;
+      if (!L.x32 && (rc = mg_alloc(&L.x32, n))) return rc;
     } else {
-      if (!L.b64 && (rc = mg_alloc(&L.b64, L.v.n))) return rc;
-      if (l > 0 && !L.x64 && (rc = mg_alloc(&L.x64, L.v.n))) return rc;
+      if (!L.b64 && (rc = mg_alloc(&L.b64, n))) return rc;
+      if (!L.x64 && (rc = mg_alloc(&L.x64, n))) return rc;
     }
   }
   const size_t N = (size_t)c.g.N;
