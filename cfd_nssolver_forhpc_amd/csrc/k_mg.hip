@@ -497,28 +497,64 @@ constexpr int MG_SMALL_THREADS = 256;
 template <typename T>
 struct SmallArgs {
   int nlev, nu, coarse_sweeps, dense;
+  int lds_off[MG_MAX_LEVELS];  // with the levels in LDS: element offset of level l's x (its b follows)
   MGView v[MG_MAX_LEVELS];
   T *x[MG_MAX_LEVELS];
   T *b[MG_MAX_LEVELS];
-  const T *cinv;  // dense inverse of the coarsest operator (row-major), when dense
+  const T *cinv;  // dense inverse of the coarsest operator, column-major, when dense
 };
 
-// the bottom of the V-cycle (levels of at most MG_SMALL cells) in one
-// workgroup; __syncthreads separates the phases, whose data stays in this
-// workgroup (L2-resident).  The first level's right-hand side and first red
-// half-sweep were written by the launch before; the coarsest level is solved
-// by its dense inverse when it is nonsingular, else smoothed coarse_sweeps
-// times.
+// fills lds_off; the LDS bytes to launch with (0: the levels do not fit in
+// a workgroup's default 64 KiB and stay in global memory)
 template <typename T>
+size_t small_lds_layout(SmallArgs<T> &S) {
+  size_t elems = 0;
+  for (int l = 0; l < S.nlev; ++l) {
+    S.lds_off[l] = (int)elems;
+    elems += 2 * (size_t)S.v[l].n;
+  }
+  const size_t bytes = elems * sizeof(T);
+  return bytes <= 65536 ? bytes : 0;
+}
+
+// the bottom of the V-cycle (levels of at most MG_SMALL cells) in one
+// workgroup; __syncthreads separates the phases.  When they fit, every
+// level's x and b live in LDS for the whole launch: the first level's
+// right-hand side and first red half-sweep (written by the launch before)
+// are read in at the start and its x is written back at the end, the only
+// values the rest of the cycle uses.  Otherwise they stay in global memory
+// (L2-resident).  The coarsest level is solved by its dense inverse when it
+// is nonsingular, else smoothed coarse_sweeps times.
+template <typename T, bool LDS>
 __global__ void __launch_bounds__(MG_SMALL_THREADS) mg_small_kernel(SmallArgs<T> S, const KrylovScalars *ks) {
   if (ks && ks->done) return;
   const int nt = MG_SMALL_THREADS;
+  extern __shared__ __align__(16) unsigned char small_lds[];
+  T *const lds = reinterpret_cast<T *>(small_lds);
+  // compile-time choice, so that the LDS accesses compile to LDS instructions
+  auto X = [&](int l) -> T * {
+    if constexpr (LDS) return lds + S.lds_off[l];
+    else return S.x[l];
+  };
+  auto B = [&](int l) -> T * {
+    if constexpr (LDS) return lds + S.lds_off[l] + S.v[l].n;
+    else return S.b[l];
+  };
+  if (LDS) {
+    for (int t = threadIdx.x; t < (int)S.v[0].n; t += nt) {
+      X(0)[t] = S.x[0][t];
+      B(0)[t] = S.b[0][t];
+    }
+    __syncthreads();
+  }
   // nsw sweeps of level l; skip_red0: the first red half-sweep is done;
   // prolong_from >= 0: the first red half-sweep folds in that level's correction
   auto sweep = [&](int l, int nsw, bool skip_red0, int prolong_from) {
     const MGView &L = S.v[l];
     const int half = (L.nz + 1) / 2;
     const int cnt = L.nx * L.ny * half;
+    T *xl = X(l);
+    const T *bl = B(l);
     for (int s = 0; s < nsw; ++s)
       for (int colour = 0; colour < 2; ++colour) {
         if (s == 0 && colour == 0 && skip_red0) continue;
@@ -526,9 +562,9 @@ __global__ void __launch_bounds__(MG_SMALL_THREADS) mg_small_kernel(SmallArgs<T>
         for (int t = threadIdx.x; t < cnt; t += nt) {
           const int k2 = t % half, r = t / half;
           if (fold)
-            rb_prolong_at<T, Natural>(L, S.v[prolong_from], r / L.ny, r % L.ny, k2, S.b[l], S.x[l], S.x[prolong_from]);
+            rb_prolong_at<T, Natural>(L, S.v[prolong_from], r / L.ny, r % L.ny, k2, bl, xl, X(prolong_from));
           else
-            rb_at<T, Natural>(L, r / L.ny, r % L.ny, k2, colour, S.b[l], S.x[l]);
+            rb_at<T, Natural>(L, r / L.ny, r % L.ny, k2, colour, bl, xl);
         }
         __syncthreads();
       }
@@ -539,24 +575,27 @@ __global__ void __launch_bounds__(MG_SMALL_THREADS) mg_small_kernel(SmallArgs<T>
     const MGView &C = S.v[l + 1];
     for (int t = threadIdx.x; t < (int)C.n; t += nt) {
       const int K = t % C.nz, r = t / C.nz;
-      restrict_at<T, Natural>(S.v[l], C, r / C.ny, r % C.ny, K, S.b[l], S.x[l], S.b[l + 1], S.x[l + 1]);
+      restrict_at<T, Natural>(S.v[l], C, r / C.ny, r % C.ny, K, B(l), X(l), B(l + 1), X(l + 1));
     }
     __syncthreads();
   }
   if (S.dense) {
+    // consecutive threads read consecutive entries of a column
     const int n = (int)S.v[last].n;
-    const T *bl = S.b[last];
+    const T *bl = B(last);
+    T *xl = X(last);
     for (int t = threadIdx.x; t < n; t += nt) {
-      const T *row = S.cinv + (int64_t)t * n;
       double acc = 0.0;
-      for (int q = 0; q < n; ++q) acc += (double)row[q] * (double)bl[q];
-      S.x[last][t] = (T)acc;
+      for (int q = 0; q < n; ++q) acc += (double)S.cinv[(int64_t)q * n + t] * (double)bl[q];
+      xl[t] = (T)acc;
     }
     __syncthreads();
   } else {
     sweep(last, S.coarse_sweeps, true, -1);
   }
   for (int l = last - 1; l >= 0; --l) sweep(l, S.nu, false, l + 1);
+  if (LDS)
+    for (int t = threadIdx.x; t < (int)S.v[0].n; t += nt) S.x[0][t] = X(0)[t];
 }
 
 // launch shape of a 3-D kernel over (nk, ny, nx) with 256-thread blocks
@@ -691,7 +730,11 @@ int mg_apply(Ctx &c, const V *r, V *out, const KrylovScalars *ks) {
     S.x[l - ls] = X(l);
     S.b[l - ls] = lb<T>(M.lv[l]);
   }
-  mg_small_kernel<T><<<1, MG_SMALL_THREADS, 0, c.stream>>>(S, ks);
+  const size_t lds_bytes = small_lds_layout(S);
+  if (lds_bytes)
+    mg_small_kernel<T, true><<<1, MG_SMALL_THREADS, lds_bytes, c.stream>>>(S, ks);
+  else
+    mg_small_kernel<T, false><<<1, MG_SMALL_THREADS, 0, c.stream>>>(S, ks);
   for (int l = ls - 1; l >= 0; --l) smooth(l, false);
   if (!same && !split) mg_widen_kernel<T, V><<<nblk(c.g.N), 256, 0, c.stream>>>(c.g.N, x0, out, ks);
   CFD_HIP(hipGetLastError());
@@ -843,6 +886,11 @@ int mg_ensure(Ctx &c, bool fp32) {
         }
       }
       if (ok) {
+        // column-major: the kernel's threads (one per row) read a column together
+        std::vector<double> InvT((size_t)n * n);
+        for (int r = 0; r < n; ++r)
+          for (int q = 0; q < n; ++q) InvT[(size_t)q * n + r] = Inv[(size_t)r * n + q];
+        Inv.swap(InvT);
         std::vector<float> inv32(Inv.begin(), Inv.end());
         int rc = mg_alloc(&M->cinv64, Inv.size());
         if (!rc) rc = mg_alloc(&M->cinv32, inv32.size());

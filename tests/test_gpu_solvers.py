@@ -164,6 +164,28 @@ def test_mg_on_odd_and_stretched_grids(require_gpu):
         assert rel < 1e-10 and tr < 2e-10, (prec, it, rel, tr)
 
 
+@pytest.mark.parametrize("ncal", [(40, 24, 11), (33, 24, 12), (36, 20, 14), (16, 12, 8), (16, 16, 16)])
+def test_mg_finest_level_shapes(require_gpu, ncal):
+    """The finest level's colour-split storage: odd nz (one colour has a
+    spare slot per row, z is not halved), odd nx (x is not halved; both take
+    the restriction's general path), every axis halved (the pair-wise
+    restriction); and grids small enough that the whole cycle is the
+    one-workgroup bottom in natural layout, with its levels in LDS (16x12x8)
+    or, for the FP64 cycle of 16x16x16 (75 KB), in global memory."""
+    ref, b = poisson_sim(ncal=ncal)
+    ref.bicgstab(tol=1e-13, maxit=50000)
+    x64 = ref.download(capi.X)
+    for prec, mode in ((capi.CFD_FP64, capi.CFD_MIXED_RESTARTED), (capi.CFD_MIXED, capi.CFD_MIXED_PC32)):
+        sim, _ = mg_sim(ncal=ncal, precision=prec)
+        p = sim.params.copy()
+        p.mixed_mode = mode
+        sim.set_params(p)
+        it, rel = sim.solve()
+        tr, x = true_relres(sim, b)
+        assert rel < 1e-10 and tr < 2e-10 and it <= 40, (ncal, prec, it, rel, tr)
+        assert rel_l2(x, x64) < 1e-7
+
+
 @pytest.mark.parametrize("op", [capi.CFD_OP_ELL, capi.CFD_OP_SPE])
 def test_mg_step_matches_jacobi_step(require_gpu, op):
     """The multigrid-preconditioned solve applies the configured operator
