@@ -63,7 +63,10 @@ __device__ inline uint32_t fastdiv(uint32_t n, const FastDiv &f) {
 // Jacobi-scaled operator applied matrix-free; coef = xm[nxc] xp[nxc]
 // ym[nyc] yp[nyc] zm[nzc] zp[nzc]; neighbour terms in the order xm, ym, zm,
 // diagonal, zp, yp, xp, each contracted to an FMA like the reference build.
-// The row's (i, j, k) comes from two invariant divisions (N < 2^31).
+// The row's (i, j, k) comes from two invariant divisions (N < 2^31).  Every
+// load is issued before the sum (an absent neighbour reads the row itself,
+// and its coefficient entry, which is in range, and is left out by a
+// select), so a wave waits once per row instead of once per neighbour.
 __device__ inline double spe_row(const double *__restrict__ coef, const double *__restrict__ jp, int nxc, int nyc,
                                  int nzc, const FastDiv &div_nz, const FastDiv &div_ny, int64_t ic,
                                  const double *__restrict__ x) {
@@ -74,15 +77,20 @@ __device__ inline double spe_row(const double *__restrict__ coef, const double *
   const int j = (int)(q - i_ * (uint32_t)nyc);
   const int i = (int)i_;
   const double *xm = coef, *xp = xm + nxc, *ym = xp + nxc, *yp = ym + nyc, *zm = yp + nyc, *zp = zm + nzc;
+  const bool hxm = i != 0, hym = j != 0, hzm = k != 0, hzp = k != nzc - 1, hyp = j != nyc - 1, hxp = i != nxc - 1;
+  const double vxm = x[hxm ? ic - nynz : ic], vym = x[hym ? ic - nzc : ic], vzm = x[hzm ? ic - 1 : ic];
+  const double vc = x[ic];
+  const double vzp = x[hzp ? ic + 1 : ic], vyp = x[hyp ? ic + nzc : ic], vxp = x[hxp ? ic + nynz : ic];
+  const double cxm = xm[i], cym = ym[j], czm = zm[k], czp = zp[k], cyp = yp[j], cxp = xp[i];
   const double J = jp[ic];
   double sum = 0.0;
-  if (i != 0) sum = fma(-xm[i] * J, x[ic - nynz], sum);
-  if (j != 0) sum = fma(-ym[j] * J, x[ic - nzc], sum);
-  if (k != 0) sum = fma(-zm[k] * J, x[ic - 1], sum);
-  sum += x[ic];
-  if (k != nzc - 1) sum = fma(-zp[k] * J, x[ic + 1], sum);
-  if (j != nyc - 1) sum = fma(-yp[j] * J, x[ic + nzc], sum);
-  if (i != nxc - 1) sum = fma(-xp[i] * J, x[ic + nynz], sum);
+  sum = hxm ? fma(-cxm * J, vxm, sum) : sum;
+  sum = hym ? fma(-cym * J, vym, sum) : sum;
+  sum = hzm ? fma(-czm * J, vzm, sum) : sum;
+  sum += vc;
+  sum = hzp ? fma(-czp * J, vzp, sum) : sum;
+  sum = hyp ? fma(-cyp * J, vyp, sum) : sum;
+  sum = hxp ? fma(-cxp * J, vxp, sum) : sum;
   return sum;
 }
 
