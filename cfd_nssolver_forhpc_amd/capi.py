@@ -267,6 +267,8 @@ def _declare(L: C.CDLL) -> None:
         "cfd_plot3d_write_q": ([vp, C.c_char_p, C.c_int, C.c_double], C.c_int),
         "cfd_plot3d_read_q": ([C.c_char_p, C.POINTER(CfdGridDesc), ip, C.POINTER(C.c_float), dp, dp, dp, dp, dp],
                               C.c_int),
+        "cfd_checkpoint_save": ([vp, C.c_char_p], C.c_int),
+        "cfd_checkpoint_load": ([vp, C.c_char_p], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

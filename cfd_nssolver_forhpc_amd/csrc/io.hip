@@ -10,7 +10,14 @@
 // cell-centred velocity and eta of every interior cell into the file's
 // i-fastest order as float, so the host only copies 5 N floats back and
 // writes them.  The grid file, the reader and the text log are host code.
+//
+// The exact FP64 checkpoint (cfd_checkpoint_save / _load) has no reference
+// counterpart: the reference's only restart reads the float Plot3D file
+// back, and that block is commented out (RUN_CPU_PC_METHOD.hpp:98-111).
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <memory>
 #include <string>
@@ -59,6 +66,65 @@ void cdcl_header(std::ofstream &f, const std::string &zone) {
 bool grid_ok(const cfd_grid_desc *g) {
   return g && g->nx_cal >= 1 && g->ny_cal >= 1 && g->nz_cal >= 1 && g->X && g->Y && g->Z;
 }
+
+// ---- checkpoint file -------------------------------------------------------
+// Layout (little-endian): CkptHeader, then the arrays of ckpt_arrays() in
+// order, then a 64-bit checksum of the arrays' bytes (ckpt_mix over 8-byte
+// words, FNV-1a constants).
+constexpr char CKPT_MAGIC[8] = {'C', 'F', 'D', 'C', 'K', 'P', 'T', '1'};
+constexpr uint32_t CKPT_VERSION = 1;
+
+struct CkptHeader {
+  char magic[8];
+  uint32_t version, abi;
+  int32_t nx_cal, ny_cal, nz_cal, loop;
+  int32_t solves, solves32, p_from_x, narrays;
+  double time;
+  uint64_t grid_hash;  // of the extents and the host metric arrays
+  uint64_t G, N;
+};
+
+constexpr uint64_t FNV_OFFSET = 1469598103934665603ull, FNV_PRIME = 1099511628211ull;
+
+inline uint64_t ckpt_mix(uint64_t h, const void *data, size_t bytes) {
+  const unsigned char *b = static_cast<const unsigned char *>(data);
+  size_t q = 0;
+  for (; q + 8 <= bytes; q += 8) {
+    uint64_t w;
+    std::memcpy(&w, b + q, 8);
+    h = (h ^ w) * FNV_PRIME;
+  }
+  for (; q < bytes; ++q) h = (h ^ b[q]) * FNV_PRIME;
+  return h;
+}
+
+uint64_t grid_hash(const Ctx &c) {
+  uint64_t h = FNV_OFFSET;
+  const int32_t ext[3] = {c.g.nxc, c.g.nyc, c.g.nzc};
+  h = ckpt_mix(h, ext, sizeof(ext));
+  for (const std::vector<double> *a : {&c.hX, &c.hY, &c.hZ, &c.hDx, &c.hDy, &c.hDz, &c.hDxs, &c.hDys, &c.hDzs})
+    h = ckpt_mix(h, a->data(), a->size() * sizeof(double));
+  return h;
+}
+
+struct CkptArray {
+  double *dev;
+  size_t len;
+};
+
+// what a timestep carries into the next
+std::vector<CkptArray> ckpt_arrays(Ctx &c) {
+  const size_t G = (size_t)c.g.G, N = (size_t)c.g.N;
+  std::vector<CkptArray> a;
+  for (int fid = CFD_T0_U; fid <= CFD_ETA; ++fid) a.push_back({c.f[fid], G});  // T0, T1, T3, p, eta
+  a.push_back({c.f[CFD_SRC], 3 * N});
+  a.push_back({c.f[CFD_X], N});
+  a.push_back({c.px1, N});
+  a.push_back({c.px2, N});
+  return a;
+}
+
+constexpr size_t CKPT_CHUNK = (size_t)1 << 21;  // doubles per transfer (16 MiB)
 
 }  // namespace
 
@@ -245,6 +311,104 @@ int cfd_plot3d_read_q(const char *path, const cfd_grid_desc *g, int *nblock, flo
         v[m] = j == nyc - 1 ? (double)d2[cc] : (d2[cc] + d2[cc + nzc]) / 2.0;
         w[m] = k == nzc - 1 ? (double)d3[cc] : (d3[cc] + d3[cc + 1]) / 2.0;
       }
+  return CFD_OK;
+}
+
+int cfd_checkpoint_save(cfd_ctx *ctx, const char *path) {
+  if (!ctx) return fail(CFD_ERR_ARG, "context is NULL");
+  Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  if (hipSetDevice(c.device) != hipSuccess) return fail(CFD_ERR_HIP, "hipSetDevice failed");
+  if (!path) return fail(CFD_ERR_ARG, "path is NULL");
+  const std::vector<CkptArray> arrays = ckpt_arrays(c);
+  CkptHeader h;
+  std::memset(&h, 0, sizeof(h));
+  std::memcpy(h.magic, CKPT_MAGIC, 8);
+  h.version = CKPT_VERSION;
+  h.abi = CFD_ABI_VERSION;
+  h.nx_cal = c.g.nxc;
+  h.ny_cal = c.g.nyc;
+  h.nz_cal = c.g.nzc;
+  h.loop = c.loop;
+  h.solves = c.solves;
+  h.solves32 = c.solves32;
+  h.p_from_x = c.p_from_x ? 1 : 0;
+  h.narrays = (int32_t)arrays.size();
+  h.time = c.time;
+  h.grid_hash = grid_hash(c);
+  h.G = (uint64_t)c.g.G;
+  h.N = (uint64_t)c.g.N;
+  CFD_HIP(hipStreamSynchronize(c.stream));  // the state of every queued step
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return fail(CFD_ERR_RUNTIME, std::string("cannot open ") + path);
+  std::unique_ptr<FILE, int (*)(FILE *)> hold(f, std::fclose);
+  bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
+  uint64_t sum = FNV_OFFSET;
+  std::vector<double> buf(CKPT_CHUNK);
+  for (const CkptArray &a : arrays) {
+    for (size_t off = 0; ok && off < a.len; off += CKPT_CHUNK) {
+      const size_t n = std::min(CKPT_CHUNK, a.len - off);
+      CFD_HIP(hipMemcpyAsync(buf.data(), a.dev + off, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      CFD_HIP(hipStreamSynchronize(c.stream));
+      sum = ckpt_mix(sum, buf.data(), n * sizeof(double));
+      ok = std::fwrite(buf.data(), sizeof(double), n, f) == n;
+    }
+  }
+  ok = ok && std::fwrite(&sum, sizeof(sum), 1, f) == 1;
+  const bool closed = std::fclose(hold.release()) == 0;
+  return ok && closed ? CFD_OK : fail(CFD_ERR_RUNTIME, std::string("write to ") + path + " failed");
+}
+
+int cfd_checkpoint_load(cfd_ctx *ctx, const char *path) {
+  if (!ctx) return fail(CFD_ERR_ARG, "context is NULL");
+  Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  if (hipSetDevice(c.device) != hipSuccess) return fail(CFD_ERR_HIP, "hipSetDevice failed");
+  if (!path) return fail(CFD_ERR_ARG, "path is NULL");
+  const std::vector<CkptArray> arrays = ckpt_arrays(c);
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return fail(CFD_ERR_RUNTIME, std::string("cannot open ") + path);
+  std::unique_ptr<FILE, int (*)(FILE *)> hold(f, std::fclose);
+  CkptHeader h;
+  if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, CKPT_MAGIC, 8) != 0)
+    return fail(CFD_ERR_RUNTIME, std::string(path) + " is not a checkpoint");
+  if (h.version != CKPT_VERSION || h.narrays != (int32_t)arrays.size())
+    return fail(CFD_ERR_RUNTIME, std::string(path) + ": unsupported checkpoint version");
+  if (h.nx_cal != c.g.nxc || h.ny_cal != c.g.nyc || h.nz_cal != c.g.nzc || h.G != (uint64_t)c.g.G ||
+      h.N != (uint64_t)c.g.N || h.grid_hash != grid_hash(c))
+    return fail(CFD_ERR_ARG, std::string(path) + " was written for another grid");
+  if (h.loop < 1 || h.solves < 0 || h.solves32 < 0 || !std::isfinite(h.time))
+    return fail(CFD_ERR_RUNTIME, std::string(path) + ": invalid step counters");
+  // first pass: the whole file is read and its checksum verified before any
+  // of the context's state is touched
+  const long data_at = std::ftell(f);
+  std::vector<double> buf(CKPT_CHUNK);
+  uint64_t sum = FNV_OFFSET;
+  for (const CkptArray &a : arrays)
+    for (size_t off = 0; off < a.len; off += CKPT_CHUNK) {
+      const size_t n = std::min(CKPT_CHUNK, a.len - off);
+      if (std::fread(buf.data(), sizeof(double), n, f) != n)
+        return fail(CFD_ERR_RUNTIME, std::string(path) + " is truncated");
+      sum = ckpt_mix(sum, buf.data(), n * sizeof(double));
+    }
+  uint64_t stored = 0;
+  if (std::fread(&stored, sizeof(stored), 1, f) != 1 || stored != sum)
+    return fail(CFD_ERR_RUNTIME, std::string(path) + ": checksum mismatch");
+  if (std::fgetc(f) != EOF) return fail(CFD_ERR_RUNTIME, std::string(path) + ": trailing bytes");
+  // second pass: the arrays to the device
+  if (std::fseek(f, data_at, SEEK_SET) != 0) return fail(CFD_ERR_RUNTIME, std::string("cannot seek in ") + path);
+  CFD_HIP(hipStreamSynchronize(c.stream));  // nothing queued still reads or writes these
+  for (const CkptArray &a : arrays)
+    for (size_t off = 0; off < a.len; off += CKPT_CHUNK) {
+      const size_t n = std::min(CKPT_CHUNK, a.len - off);
+      if (std::fread(buf.data(), sizeof(double), n, f) != n)
+        return fail(CFD_ERR_RUNTIME, std::string(path) + " changed while it was read");
+      CFD_HIP(hipMemcpyAsync(a.dev + off, buf.data(), n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+      CFD_HIP(hipStreamSynchronize(c.stream));  // buf is refilled next
+    }
+  c.loop = h.loop;
+  c.time = h.time;
+  c.solves = h.solves;
+  c.solves32 = h.solves32;
+  c.p_from_x = h.p_from_x != 0;
   return CFD_OK;
 }
 

@@ -259,6 +259,15 @@ class Simulation:
         check(self.lib.cfd_plot3d_write_q(self.h, os.fsencode(path), int(vel_field), float(time)),
               "cfd_plot3d_write_q")
 
+    def save_checkpoint(self, path):
+        """Exact FP64 snapshot of the step state (cfd_checkpoint_save)."""
+        check(self.lib.cfd_checkpoint_save(self.h, os.fsencode(path)), "cfd_checkpoint_save")
+
+    def load_checkpoint(self, path):
+        """Resume from a snapshot written for this grid (cfd_checkpoint_load);
+        the context must be set up with the same parameters and operator."""
+        check(self.lib.cfd_checkpoint_load(self.h, os.fsencode(path)), "cfd_checkpoint_load")
+
     def CheckSteadyStatebyMaxVal(self):
         out = (C.c_double * 3)()
         check(self.lib.cfd_maxdiff(self.h, out), "cfd_maxdiff")

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define CFD_ABI_VERSION 4
+#define CFD_ABI_VERSION 5
 
 /* status codes */
 #define CFD_OK 0
@@ -324,6 +324,22 @@ int cfd_plot3d_write_q(cfd_ctx *ctx, const char *path, int vel_field, double tim
    The file's grid must be the given one (CFD_ERR_ARG otherwise). */
 int cfd_plot3d_read_q(const char *path, const cfd_grid_desc *grid, int *nblock, float header[4], double *p,
                       double *u, double *v, double *w, double *eta);
+
+/* ---- checkpoint / resume ---------------------------------------------- */
+/* An exact FP64 snapshot of what a timestep carries into the next: T0, T1,
+   T3, p and eta (ghosted), the source term (3 N), X_result and the solver's
+   NaN-reset history (N), and the loop, time, solve count (the restart rule's
+   timestep_) and pressure-location flag.  No reference counterpart: its only
+   restart reads the float Plot3D file back, and that block is commented out
+   (inc/run/RUN_CPU_PC_METHOD.hpp:98-111).  The operator, cfd_params, the
+   forcing gain and the inflow profile are not in the file: a run resumes in a
+   context created with the same grid and parameters and assembled the same
+   way, where cfd_step continues bit for bit.  The file ends in a checksum.
+   cfd_checkpoint_load reads and verifies the whole file before it changes
+   the context; it rejects a file of another grid (CFD_ERR_ARG) and a damaged,
+   truncated or foreign one (CFD_ERR_RUNTIME). */
+int cfd_checkpoint_save(cfd_ctx *ctx, const char *path);
+int cfd_checkpoint_load(cfd_ctx *ctx, const char *path);
 
 /* ---- measurement ------------------------------------------------------ */
 /* average device time (ms) of one launch of the kernel named by `what`, over
