@@ -115,6 +115,11 @@ class CfdStepStats(C.Structure):
         }
 
 
+class CfdSolverCounters(C.Structure):
+    _fields_ = [("solves", C.c_int), ("restarts", C.c_int), ("resets", C.c_int), ("pad", C.c_int),
+                ("iters", C.c_longlong)]
+
+
 class CfdAdjointStats(C.Structure):
     _fields_ = [
         ("steps", C.c_int), ("checkpoints", C.c_int), ("recomputed_steps", C.c_int),
@@ -268,6 +273,7 @@ def _declare(L: C.CDLL) -> None:
         "cfd_plot3d_read_q": ([C.c_char_p, C.POINTER(CfdGridDesc), ip, C.POINTER(C.c_float), dp, dp, dp, dp, dp],
                               C.c_int),
         "cfd_checkpoint_save": ([vp, C.c_char_p], C.c_int),
+        "cfd_get_solver_counters": ([vp, C.POINTER(CfdSolverCounters)], C.c_int),
         "cfd_checkpoint_load": ([vp, C.c_char_p], C.c_int),
     }
     for name, (args, res) in sig.items():

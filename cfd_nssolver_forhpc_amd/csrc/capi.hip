@@ -956,6 +956,21 @@ int cfd_set_loop(cfd_ctx *ctx, int loop, double time) {
   return CFD_OK;
 }
 
+int cfd_get_solver_counters(cfd_ctx *ctx, cfd_solver_counters *out) {
+  CTX_CHECK(ctx);
+  Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  if (!out) return fail(CFD_ERR_ARG, "out is NULL");
+  KrylovScalars h;
+  CFD_HIP(hipMemcpyAsync(&h, c.ks, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+  CFD_HIP(hipStreamSynchronize(c.stream));
+  out->solves = c.solves;
+  out->restarts = h.restarts;
+  out->resets = c.nan_resets;
+  out->pad = 0;
+  out->iters = c.total_iters;
+  return CFD_OK;
+}
+
 // ---- measurement -----------------------------------------------------------------
 int cfd_time_kernel(cfd_ctx *ctx, int what, int nrep, double *ms_avg) {
   CTX_CHECK(ctx);

@@ -105,6 +105,8 @@ struct Ctx {
   double *hred = nullptr;        // pinned host mirror of red.out
   double *stepbuf = nullptr;     // per-pass drag sums and max differences of one step
   int solves = 0;                // bicgstabRe2::timestep_ (solve count)
+  int nan_resets = 0;            // bicgstabRe2::pResets_ of the FP64 solves
+  long long total_iters = 0;     // bicgstabRe2::totalNumIter_ of the FP64 solves
   int solves32 = 0;
   int loop = 1;
   double time = 0.0;

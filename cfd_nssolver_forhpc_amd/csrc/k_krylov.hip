@@ -385,7 +385,9 @@ int solve_bicg(Ctx &c, const OpArgs &a, const Vecs<T> &v, KrylovScalars *ks, int
     CFD_HIP(hipStreamSynchronize(c.stream));
     if (c.hred[0] > (double)N * 0.1)
       CFD_HIP(hipMemcpyAsync(v.x, v.px2, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+    if (c.hred[0] > 0.0 && ks == c.ks) c.nan_resets++;
   }
+  if (ks == c.ks) c.total_iters += h.iters;
   *iters = h.iters;
   *relres = h.relres;
   return CFD_OK;

@@ -259,6 +259,13 @@ class Simulation:
         check(self.lib.cfd_plot3d_write_q(self.h, os.fsencode(path), int(vel_field), float(time)),
               "cfd_plot3d_write_q")
 
+    def solver_counters(self) -> dict:
+        """bicgstabRe2's timestep_, numRestarts_, pResets_ and totalNumIter_
+        over this context's FP64 BiCGSTAB solves."""
+        out = capi.CfdSolverCounters()
+        check(self.lib.cfd_get_solver_counters(self.h, C.byref(out)), "cfd_get_solver_counters")
+        return {"solves": out.solves, "restarts": out.restarts, "resets": out.resets, "iters": out.iters}
+
     def save_checkpoint(self, path):
         """Exact FP64 snapshot of the step state (cfd_checkpoint_save)."""
         check(self.lib.cfd_checkpoint_save(self.h, os.fsencode(path)), "cfd_checkpoint_save")

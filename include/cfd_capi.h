@@ -268,6 +268,17 @@ int cfd_step(cfd_ctx *ctx, cfd_step_stats *stats);
 int cfd_loop(const cfd_ctx *ctx, int *loop, double *time);
 int cfd_set_loop(cfd_ctx *ctx, int loop, double time);
 
+/* bicgstabRe2's counters over the context's FP64 BiCGSTAB solves
+   (inc/solver/bicgstabRe2.hpp:80-84): solves is timestep_ (the restart
+   rule's clock, :98, 171), restarts numRestarts_ (shadow-residual restarts,
+   :177-182), resets pResets_ (solves ended by the NaN / Inf reset,
+   :143-163), iters totalNumIter_ */
+typedef struct cfd_solver_counters {
+  int solves, restarts, resets, pad;
+  long long iters;
+} cfd_solver_counters;
+int cfd_get_solver_counters(cfd_ctx *ctx, cfd_solver_counters *out);
+
 /* ---- discrete adjoint (no reference counterpart) ---------------------- */
 /* Objective J = mean of the corrector pass's cD (virtualF_Int,
    virtualForceIntergrator.hpp:5-37) over steps [avg_from, nsteps) of a run

@@ -100,6 +100,75 @@ static void fill_noise(std::vector<double> &v, uint64_t seed, int tag, double ba
   }
 }
 
+// usage: ref_harness OUTDIR rules -- bicgstabRe2's failure handling
+// (bicgstabRe2.hpp:91-189) driven through the reference's own ELL_matrix and
+// solver on a 200-row tridiagonal system (diagonal 2 + 0.01 (i mod 7), -1 off
+// the diagonal, entries set in the order i-1, i, i+1):
+//   nan:     solve b1 from x = 0, solve b2, then b1 with a NaN at row 7: every
+//            entry of x diverges, and after iters_max the reset rule leaves
+//            px2, the initial guess of the solve before (x1);
+//   restart: ten solves at tol 1e-10, then from x = 0 a right-hand side of
+//            1e-12 noise, whose |r . r0| is below 1e-7 tol from the start, so
+//            the shadow residual restarts every iteration; the same solve in
+//            a fresh solver (timestep_ 1) does not restart.
+static int rules_main() {
+  const int n = 200;
+  const uint64_t seed = 20240611;
+  ELL_type A;
+  A.resize(n, 3);
+  for (int i = 0; i < n; ++i) {
+    if (i > 0) A.set(i, i - 1, -1.0);
+    A.set(i, i, 2.0 + 0.01 * (i % 7));
+    if (i < n - 1) A.set(i, i + 1, -1.0);
+  }
+  std::ofstream s(g_out + "/rules.txt");
+  s << std::setprecision(17);
+  auto line = [&](const char *key, const std::vector<double> &v) {
+    s << key;
+    for (double a : v) s << " " << a;
+    s << "\n";
+  };
+  {
+    solver::bicgstabRe2<ELL_type> sol(A);
+    sol.setTolerance(1e-10);
+    std::vector<double> b1(n), b2(n), x(n, 0.0);
+    fill_noise(b1, seed, 40, 0.0, 1.0);
+    fill_noise(b2, seed, 41, 0.0, 1.0);
+    auto [it1, e1] = sol(b1, x);
+    const std::vector<double> x1 = x;
+    auto [it2, e2] = sol(b2, x);
+    std::vector<double> b3 = b1;
+    b3[7] = std::nan("");
+    auto [it3, e3] = sol(b3, x);
+    s << "nan " << it1 << " " << e1 << " " << it2 << " " << e2 << " " << it3 << " " << e3 << " "
+      << (x == x1 ? 1 : 0) << "\n";
+    line("nan_x1", x1);
+    line("nan_x3", x);
+  }
+  {
+    solver::bicgstabRe2<ELL_type> sol(A);
+    sol.setTolerance(1e-10);
+    std::vector<double> b(n), x(n, 0.0);
+    for (int k = 0; k < 10; ++k) {
+      fill_noise(b, seed, 50 + k, 0.0, 1.0);
+      sol(b, x);
+    }
+    fill_noise(b, seed, 61, 0.0, 1e-12);
+    std::fill(x.begin(), x.end(), 0.0);
+    auto [itr, er] = sol(b, x);
+    const std::vector<double> xr = x;
+    solver::bicgstabRe2<ELL_type> fresh(A);
+    fresh.setTolerance(1e-10);
+    std::fill(x.begin(), x.end(), 0.0);
+    auto [itn, en] = fresh(b, x);
+    s << "restart " << itr << " " << er << " " << itn << " " << en << " " << sol.numRestarts_ << " "
+      << fresh.numRestarts_ << "\n";
+    line("restart_x", xr);
+    line("norestart_x", x);
+  }
+  return 0;
+}
+
 // usage: ref_harness OUTDIR synth -- every hot-path function once on seeded inputs
 static int synth_main() {
   simuClass simu;
@@ -309,6 +378,11 @@ int main(int argc, char **argv) {
     g_out = argv[1];
     mkdir(g_out.c_str(), 0755);
     return io_main();
+  }
+  if (argc >= 3 && std::string(argv[2]) == "rules") {
+    g_out = argv[1];
+    mkdir(g_out.c_str(), 0755);
+    return rules_main();
   }
   if (argc < 6) {
     std::cerr << "usage: ref_harness OUTDIR NSTEPS TOL OP(ell|spe) FN_STEP [DUMP_EVERY [SRC_AT_CELL]]\n";

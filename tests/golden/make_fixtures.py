@@ -16,6 +16,9 @@ case (190x210x40 SML grid, cavity table, Re 40, dt 0.002) and records:
     its output, the L2 norm, and 256 seeded sample values;
   * drag/lift, max-difference, BiCGSTAB (iterations, residual) and
     SorPipeLine_seq scalars;
+  * bicgstabRe2's failure handling on a 200-row tridiagonal system in the
+    reference's own ELL_matrix: the NaN / Inf reset after iters_max (which
+    leaves px2) and the shadow-residual restart after the tenth solve;
   * the per-pass statistics of three PC-method timesteps at the reference
     tolerance 1e-3;
   * whole trajectories of the native case with the reference's default
@@ -79,6 +82,8 @@ def main():
     for scheme in ("lud", "ud", "les"):
         subprocess.run([h + "_" + scheme, str(DUMPS / f"synth_{scheme}"), "synth"], check=True, env=env,
                        stdout=subprocess.DEVNULL)
+    # bicgstabRe2's NaN reset and shadow-residual restart on a small system
+    subprocess.run([h, str(DUMPS / "rules"), "rules"], check=True, env=env, stdout=subprocess.DEVNULL)
     reuse = "--reuse-dumps" in sys.argv
     # (steps, tolerance, OpenMP threads, source term at its own cell)
     trajs = {"spe_1e-3": ("50", "1e-3", "1", "0"), "spe_1e-8_cell": ("20", "1e-8", "8", "1")}
@@ -125,6 +130,18 @@ def main():
                 "cdcl_nodelay": (io / "cdcl_nodelay.dat").read_text()}
     for q in "p u v w eta".split():
         fx["io"][f"read_{q}"] = sha(io / f"read_{q}.bin")
+    fx["rules"] = {}
+    for line in (DUMPS / "rules" / "rules.txt").read_text().splitlines():
+        k, *v = line.split()
+        if k == "nan":
+            fx["rules"]["nan"] = {"iters": [int(v[0]), int(v[2]), int(v[4])], "relres": [float(v[1]), float(v[3])],
+                                  "relres3_is_nan": v[5] == "nan", "x3_equals_x1": v[6] == "1"}
+        elif k == "restart":
+            fx["rules"]["restart"] = {"iters": int(v[0]), "relres": float(v[1]), "iters_fresh": int(v[2]),
+                                      "relres_fresh": float(v[3]), "restarts": int(v[4]),
+                                      "restarts_fresh": int(v[5])}
+        else:
+            fx["rules"][k] = [float(x) for x in v]
     fx["trajectories"] = {}
     for key, (nsteps, tol, threads, at_cell) in trajs.items():
         t = {"op": "spe", "tol": float(tol), "steps": int(nsteps), "omp_threads": int(threads),

@@ -48,3 +48,29 @@ def random_state(grid: GridArrays, seed: int, amp=0.3):
     G = grid.G
     base = [1.0, 0.0, 0.0]
     return [base[c] + amp * rng.standard_normal(G) for c in range(3)]
+
+
+# ---- bicgstabRe2's failure-handling case (the reference harness's `rules` mode)
+RULES_N = 200
+RULES_SEED = 20240611
+
+
+def rules_system():
+    """The 200-row tridiagonal system of the harness's rules mode in the
+    7-slot slot-major ELL layout: each row's entries in the order the
+    harness sets them (i-1, i, i+1), the rest padded like ELL_matrix does,
+    (column N-1, 0.0) (ELL_sparseMatrix.hpp:157-158)."""
+    n = RULES_N
+    col = np.full((7, n), n - 1, dtype=np.int32)
+    val = np.zeros((7, n))
+    for i in range(n):
+        entries = ([(i - 1, -1.0)] if i > 0 else []) + [(i, 2.0 + 0.01 * (i % 7))] + \
+                  ([(i + 1, -1.0)] if i < n - 1 else [])
+        for s, (c, v) in enumerate(entries):
+            col[s, i] = c
+            val[s, i] = v
+    return col, val
+
+
+def rules_rhs(tag, amp=1.0):
+    return O.noise(RULES_N, RULES_SEED, tag, 0.0, amp)

@@ -223,3 +223,82 @@ def test_mixed_pc32_reaches_fp64_residual(require_gpu):
     print("PC32 iterations", it, "true relres", tr)
     assert rel < 1e-10 and tr < 2e-10 and it <= 40
     assert rel_l2(x, x64) < 1e-7
+
+
+# ---- bicgstabRe2's failure handling (bicgstabRe2.hpp:143-183) on the GPU
+def rules_sim():
+    from helpers import rules_system
+    sim = Simulation(uniform_grid((10, 5, 4), (1.0, 1.0, 1.0)), configs.cylinder_params(100.0, 0.002, 0.5, 0.5,
+                                                                                         0.1, 1e-10))
+    sim.set_ell(*rules_system())
+    sim.upload(capi.X, np.zeros(sim.N))
+    return sim
+
+
+def test_nan_reset_after_iters_max(require_gpu):
+    """A NaN in the right-hand side makes every entry of x diverge; after
+    iters_max (5000) the reset rule leaves px2, the initial guess of the
+    solve before, bit for bit, as the reference does on the same system
+    (tests/golden rules: 84 / 85 / 5000 iterations, x3 == x1)."""
+    import json
+    from pathlib import Path
+    from helpers import rules_rhs
+    R = json.loads((Path(__file__).parent / "golden" / "native_fixtures.json").read_text())["rules"]
+    sim = rules_sim()
+    sim.upload(capi.RHS, rules_rhs(40))
+    it1, rel1 = sim.bicgstab(tol=1e-10, maxit=5000)
+    x1 = sim.download(capi.X)
+    sim.upload(capi.RHS, rules_rhs(41))
+    it2, rel2 = sim.bicgstab(tol=1e-10, maxit=5000)
+    b3 = rules_rhs(40)
+    b3[7] = np.nan
+    sim.upload(capi.RHS, b3)
+    it3, rel3 = sim.bicgstab(tol=1e-10, maxit=5000)
+    x3 = sim.download(capi.X)
+    print("GPU", it1, it2, it3, "reference", R["nan"]["iters"])
+    ref = R["nan"]["iters"]
+    assert abs(it1 - ref[0]) <= ref[0] // 10 and abs(it2 - ref[1]) <= ref[1] // 10
+    assert rel1 < 1e-10 and rel2 < 1e-10
+    assert rel_l2(x1, np.array(R["nan_x1"])) < 1e-7
+    assert it3 == 5000 and np.isnan(rel3)
+    assert np.array_equal(x3, x1)
+    cnt = sim.solver_counters()
+    assert cnt["solves"] == 3 and cnt["resets"] == 1 and cnt["iters"] == it1 + it2 + it3
+
+
+def test_shadow_residual_restart(require_gpu):
+    """After the tenth solve a right-hand side of 1e-12 noise has |r . r0|
+    below 1e-7 tol from the start, so the shadow residual restarts in every
+    iteration (numRestarts_ == iterations, as in the reference: 113 of 113);
+    the same solve in a fresh solver, before its tenth solve, restarts never.
+    The iteration counts themselves follow the rounding of the dot products
+    (reference 113 / 84, oracle 104 / 89) and are only bounded loosely."""
+    import json
+    from pathlib import Path
+    from helpers import rules_rhs
+    R = json.loads((Path(__file__).parent / "golden" / "native_fixtures.json").read_text())
+    rr = R["rules"]["restart"]
+    sim = rules_sim()
+    for k in range(10):
+        sim.upload(capi.RHS, rules_rhs(50 + k))
+        sim.bicgstab(tol=1e-10, maxit=5000)
+    before = sim.solver_counters()
+    b = rules_rhs(61, 1e-12)
+    sim.upload(capi.RHS, b)
+    sim.upload(capi.X, np.zeros(sim.N))
+    itr, relr = sim.bicgstab(tol=1e-10, maxit=5000)
+    xr = sim.download(capi.X)
+    after = sim.solver_counters()
+    fresh = rules_sim()
+    fresh.upload(capi.RHS, b)
+    itn, reln = fresh.bicgstab(tol=1e-10, maxit=5000)
+    xn = fresh.download(capi.X)
+    print("GPU restarted", itr, "fresh", itn, "reference", rr["iters"], rr["iters_fresh"], after)
+    assert relr < 1e-10 and reln < 1e-10
+    assert before["solves"] == 10 and before["restarts"] == 0
+    assert after["restarts"] - before["restarts"] == itr == after["iters"] - before["iters"]
+    assert fresh.solver_counters()["restarts"] == 0
+    assert rr["restarts"] == rr["iters"] and rr["restarts_fresh"] == 0
+    assert abs(itr - rr["iters"]) <= rr["iters"] // 4 and abs(itn - rr["iters_fresh"]) <= rr["iters_fresh"] // 4
+    assert rel_l2(xr, np.array(R["rules"]["restart_x"])) < 1e-7
+    assert rel_l2(xn, np.array(R["rules"]["norestart_x"])) < 1e-7

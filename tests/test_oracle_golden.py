@@ -284,3 +284,70 @@ def test_pc_loop_statistics(native):
                 assert st.relres[q] < 1e-3 and r["err"] < 1e-3
                 assert abs(st.iters[q] - r["iters"]) <= max(10, 0.25 * r["iters"])
                 assert abs(st.cd[q] - r["cd"]) <= 2e-2 * abs(r["cd"])
+
+
+def test_bicgstab_failure_handling_against_reference():
+    """bicgstabRe2's NaN / Inf reset and its shadow-residual restart
+    (bicgstabRe2.hpp:143-183), forced on the harness's 200-row system:
+    a right-hand side with a NaN makes every entry of x diverge, and after
+    iters_max the reset leaves px2, the previous solve's initial guess; after
+    the tenth solve a 1e-12 right-hand side has |r . r0| below 1e-7 tol, so
+    the shadow residual restarts every iteration (113 iterations against 84
+    for the same solve in a fresh solver, whose timestep_ is below 10)."""
+    from helpers import rules_rhs, rules_system
+    R = FX["rules"]
+    col, val = rules_system()
+    L = O.lib()
+    n = col.shape[1]
+    s = L.orc_bicg_create(n, 0, col, val, None, None, None)
+    L.orc_bicg_set_tolerance(s, 1e-10)
+    x = np.zeros(n)
+    it = C.c_int()
+    rel1 = L.orc_bicg_solve(s, rules_rhs(40), x, C.byref(it))
+    it1 = it.value
+    x1 = x.copy()
+    rel2 = L.orc_bicg_solve(s, rules_rhs(41), x, C.byref(it))
+    it2 = it.value
+    b3 = rules_rhs(40)
+    b3[7] = np.nan
+    rel3 = L.orc_bicg_solve(s, b3, x, C.byref(it))
+    it3 = it.value
+    resets = L.orc_bicg_resets(s)
+    L.orc_bicg_destroy(s)
+    # the normal solves: the oracle sums its dot products in fixed blocks, the
+    # reference build sequentially with FMA contraction, so the counts differ
+    # by a few iterations (81 / 89 against 84 / 85) and the solutions by the
+    # 1e-10 stopping residual times the conditioning
+    ref_it = R["nan"]["iters"]
+    assert abs(it1 - ref_it[0]) <= ref_it[0] // 10 and abs(it2 - ref_it[1]) <= ref_it[1] // 10
+    assert rel1 < 1e-10 and rel2 < 1e-10
+    assert np.linalg.norm(x1 - R["nan_x1"]) <= 1e-7 * np.linalg.norm(R["nan_x1"])
+    # the failure: iters_max reached, NaN residual, x reset to px2 = x1 bit for bit
+    assert it3 == ref_it[2] == 5000 and np.isnan(rel3) and R["nan"]["relres3_is_nan"]
+    assert R["nan"]["x3_equals_x1"] and np.array_equal(x, x1) and resets == 1
+
+    s = L.orc_bicg_create(n, 0, col, val, None, None, None)
+    L.orc_bicg_set_tolerance(s, 1e-10)
+    x = np.zeros(n)
+    for k in range(10):
+        L.orc_bicg_solve(s, rules_rhs(50 + k), x, C.byref(it))
+    b = rules_rhs(61, 1e-12)
+    x[:] = 0.0
+    L.orc_bicg_solve(s, b, x, C.byref(it))
+    itr, restarts = it.value, L.orc_bicg_restarts(s)
+    xr = x.copy()
+    L.orc_bicg_destroy(s)
+    s = L.orc_bicg_create(n, 0, col, val, None, None, None)
+    L.orc_bicg_set_tolerance(s, 1e-10)
+    x[:] = 0.0
+    L.orc_bicg_solve(s, b, x, C.byref(it))
+    itn, restarts_fresh = it.value, L.orc_bicg_restarts(s)
+    L.orc_bicg_destroy(s)
+    rr = R["restart"]
+    print("restart", itr, restarts, "fresh", itn, restarts_fresh, "reference", rr)
+    # every iteration restarts after the tenth solve, none in a fresh solver
+    assert restarts == itr and rr["restarts"] == rr["iters"]
+    assert restarts_fresh == 0 == rr["restarts_fresh"]
+    assert abs(itr - rr["iters"]) <= rr["iters"] // 10 and abs(itn - rr["iters_fresh"]) <= rr["iters_fresh"] // 10
+    assert np.linalg.norm(xr - R["restart_x"]) <= 1e-7 * np.linalg.norm(R["restart_x"])
+    assert np.linalg.norm(x - R["norestart_x"]) <= 1e-7 * np.linalg.norm(R["norestart_x"])
