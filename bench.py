@@ -64,6 +64,7 @@ def run_steps(w, warmup, steps, tag):
         st = sim.step()
         log(f"  [{tag}] warmup step {k + 1}: iters {list(st.iters[:st.n_passes])}")
     stats = []
+    sim.phase_timing(True)  # HIP events at the step's phase boundaries
     t0 = time.perf_counter()
     for _ in range(steps):
         stats.append(sim.step().as_dict())
@@ -72,6 +73,18 @@ def run_steps(w, warmup, steps, tag):
         log(f"  [{tag}] step {warmup + k + 1}: iters {s['iters']} inner {s['inner_iters']} "
             f"relres {max(s['relres']):.2e} cD {s['cd'][-1]:.4f}")
     return sim, dt, stats
+
+
+def phase_report(sim):
+    """Device milliseconds per timed step in each phase of the PC loop (the
+    reference's clockstruct stopwatches: convectionDifussion, pressure,
+    updateT1toT3 with the fused checkL2norm / updateT3toT0 / getCdCl, BC)."""
+    ph = sim.phase_times()
+    n = max(ph["steps"], 1)
+    ms = {k: v / n for k, v in ph["ms"].items()}
+    total = sum(ms.values())
+    return {"unit": "ms per step", "steps": ph["steps"], **ms,
+            "pressure_share": ms["pressure"] / total if total > 0 else None}
 
 
 def per_step(stats, key):
@@ -298,10 +311,12 @@ def main():
         return
     if args.mg_only:
         sim64, tps64, iters64, kt, spmv, st64 = None, args.ref_tps, float("nan"), {}, {}, []
+        phases64 = None
     else:
         log(f"[bench] {w64.name}: warmup {warmup}, timed {args.steps}")
         sim64, t64, st64 = run_steps(w64, warmup, args.steps, "fp64")
         tps64 = args.steps / t64
+        phases64 = phase_report(sim64)
         iters64 = per_step(st64, "iters")
         # kernel roofline table; the dominant kernel is the ELL SpMV (FP64)
         sim64.setupPressure()
@@ -345,7 +360,7 @@ def main():
             mg[tag] = {"timesteps_per_s": args.mg_steps / tg, "speedup_vs_fp64": (args.mg_steps / tg) / tps64,
                        "solver_iters_per_step": per_step(stg, "iters"),
                        "inner_iters_per_step": per_step(stg, "inner_iters"),
-                       "max_final_relres": max(max(s["relres"]) for s in stg)}
+                       "max_final_relres": max(max(s["relres"]) for s in stg), "phases": phase_report(simg)}
             sims[tag] = simg
         # mixed runs against the FP64 run with the same preconditioner, after the same steps
         ref_mg = sims["fp64_mg"]
@@ -424,6 +439,7 @@ def main():
                      "frac": spmv["gbs"] / HBM_PEAK_GBS, "traffic": profiled_traffic(),
                      "kernel": "ELL SpMV FP64 (100 B/row x 8388608 rows per launch)", "avg_us": spmv["avg_us"]},
         "fp64_solves": fp64_solves,
+        "phases": phases64,
         "kernels": kt,
         "mixed": mixed,
         "multigrid": mg,

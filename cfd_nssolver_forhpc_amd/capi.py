@@ -274,6 +274,8 @@ def _declare(L: C.CDLL) -> None:
                               C.c_int),
         "cfd_checkpoint_save": ([vp, C.c_char_p], C.c_int),
         "cfd_get_solver_counters": ([vp, C.POINTER(CfdSolverCounters)], C.c_int),
+        "cfd_phase_timing": ([vp, C.c_int], C.c_int),
+        "cfd_phase_times": ([vp, dp, ip], C.c_int),
         "cfd_checkpoint_load": ([vp, C.c_char_p], C.c_int),
     }
     for name, (args, res) in sig.items():

@@ -111,6 +111,7 @@ void free_ctx(Ctx *c) {
   for (auto p : metr) if (p) (void)hipFree(p);
   if (c->hks) (void)hipHostFree(c->hks);
   if (c->hred) (void)hipHostFree(c->hred);
+  for (hipEvent_t e : c->phase_ev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -880,6 +881,15 @@ int step_impl(Ctx &c, cfd_step_stats *stats) {
   st.n_passes = P.inner_loops + 1;
   double *u0 = c.f[CFD_T0_U], *v0 = c.f[CFD_T0_V], *w0 = c.f[CFD_T0_W];
   double *u1 = c.f[CFD_T1_U], *v1 = c.f[CFD_T1_V], *w1 = c.f[CFD_T1_W];
+  // phase boundaries (cfd_phase_timing): event k closes phase ev_phase[k]
+  int nev = 0;
+  int ev_phase[2 + 3 * CFD_MAX_PASSES];
+  auto mark = [&](int phase) -> hipError_t {
+    if (!c.phase_timing) return hipSuccess;
+    ev_phase[nev] = phase;
+    return hipEventRecord(c.phase_ev[nev++], c.stream);
+  };
+  CFD_HIP(mark(-1));
   for (int pass = 0; pass <= P.inner_loops; ++pass) {
     const bool corrector = pass == P.inner_loops;
     const bool with_src = pass > 0 && P.inner_loops > 0;
@@ -900,6 +910,7 @@ int step_impl(Ctx &c, cfd_step_stats *stats) {
     } else {
       RET_LAUNCH(launch_pred_add(c, c.pred, c.f[CFD_SRC], u1, v1, w1));
     }
+    CFD_HIP(mark(CFD_PHASE_CONVECTION));
     st.iters[pass] = -1;
     if (solve) {
       RET_LAUNCH(launch_rhs(c, u1, v1, w1, c.f[CFD_JP], c.f[CFD_RHS]));
@@ -912,6 +923,7 @@ int step_impl(Ctx &c, cfd_step_stats *stats) {
       st.inner_iters[pass] = inner;
       c.p_from_x = true;
     }
+    CFD_HIP(mark(CFD_PHASE_PRESSURE));
     int mode = UPD_DRAG;
     if (c.p_from_x) mode |= UPD_P_FROM_X;
     if (corrector) mode |= UPD_WRITE_T3 | UPD_WRITE_T0 | UPD_MAXDIFF;
@@ -920,10 +932,20 @@ int step_impl(Ctx &c, cfd_step_stats *stats) {
     double *t2 = c.t2_rec ? c.t2_rec + (size_t)3 * c.g.N * pass : nullptr;
     RET_LAUNCH(
         launch_dfib_update_into(c, mode, pp, c.stepbuf + 2 * pass, corrector ? c.stepbuf + 32 : nullptr, t2));
+    CFD_HIP(mark(CFD_PHASE_UPDATE));
   }
   RET_LAUNCH(launch_bc(c, true, true, !c.p_from_x));
+  CFD_HIP(mark(CFD_PHASE_BC));
   CFD_HIP(hipMemcpyAsync(c.hred, c.stepbuf, 40 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   CFD_HIP(hipStreamSynchronize(c.stream));
+  if (c.phase_timing) {
+    for (int k = 1; k < nev; ++k) {
+      float ms = 0.0f;
+      CFD_HIP(hipEventElapsedTime(&ms, c.phase_ev[k - 1], c.phase_ev[k]));
+      c.phase_ms[ev_phase[k]] += ms;
+    }
+    c.phase_steps += 1;
+  }
   for (int pass = 0; pass <= P.inner_loops; ++pass) {
     st.cd[pass] = -2.0 * c.hred[2 * pass] / c.g.lz;
     st.cl[pass] = -2.0 * c.hred[2 * pass + 1] / c.g.lz;
@@ -953,6 +975,31 @@ int cfd_set_loop(cfd_ctx *ctx, int loop, double time) {
   Ctx &c = *reinterpret_cast<Ctx *>(ctx);
   c.loop = loop;
   c.time = time;
+  return CFD_OK;
+}
+
+int cfd_phase_timing(cfd_ctx *ctx, int on) {
+  CTX_CHECK(ctx);
+  Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  // one event per boundary: the step's start, three per pass, the BC's end
+  const size_t need = 2 + 3 * CFD_MAX_PASSES;
+  while (on && c.phase_ev.size() < need) {
+    hipEvent_t e = nullptr;
+    CFD_HIP(hipEventCreate(&e));
+    c.phase_ev.push_back(e);
+  }
+  c.phase_timing = on != 0;
+  for (double &m : c.phase_ms) m = 0.0;
+  c.phase_steps = 0;
+  return CFD_OK;
+}
+
+int cfd_phase_times(cfd_ctx *ctx, double ms[CFD_NPHASES], int *steps) {
+  CTX_CHECK(ctx);
+  Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  if (!ms) return fail(CFD_ERR_ARG, "ms is NULL");
+  for (int q = 0; q < CFD_NPHASES; ++q) ms[q] = c.phase_ms[q];
+  if (steps) *steps = c.phase_steps;
   return CFD_OK;
 }
 

@@ -112,6 +112,12 @@ struct Ctx {
   double time = 0.0;
   bool p_from_x = false;         // ghosted p interior lags X (step mode)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // per-phase device time of cfd_step (the reference's clockstruct
+  // stopwatches): events recorded at the phase boundaries when enabled
+  bool phase_timing = false;
+  std::vector<hipEvent_t> phase_ev;
+  double phase_ms[CFD_NPHASES] = {};
+  int phase_steps = 0;
   bool debug = false;  // CFD_DEBUG=1: solver progress on stderr
   // parameters of the discrete adjoint and its forward record
   double gain = 1.0;             // DFIB forcing gain

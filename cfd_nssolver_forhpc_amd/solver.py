@@ -259,6 +259,20 @@ class Simulation:
         check(self.lib.cfd_plot3d_write_q(self.h, os.fsencode(path), int(vel_field), float(time)),
               "cfd_plot3d_write_q")
 
+    PHASES = ("convection", "pressure", "update", "bc")
+
+    def phase_timing(self, on: bool = True):
+        """Per-phase device timers of step() (the reference's clockstruct
+        stopwatches); switching clears the totals."""
+        check(self.lib.cfd_phase_timing(self.h, 1 if on else 0), "cfd_phase_timing")
+
+    def phase_times(self) -> dict:
+        """Milliseconds per phase summed over the timed steps, and their number."""
+        ms = np.zeros(len(self.PHASES))
+        n = C.c_int()
+        check(self.lib.cfd_phase_times(self.h, dptr(ms), C.byref(n)), "cfd_phase_times")
+        return {"steps": n.value, "ms": dict(zip(self.PHASES, ms.tolist()))}
+
     def solver_counters(self) -> dict:
         """bicgstabRe2's timestep_, numRestarts_, pResets_ and totalNumIter_
         over this context's FP64 BiCGSTAB solves."""

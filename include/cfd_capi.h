@@ -273,6 +273,21 @@ int cfd_set_loop(cfd_ctx *ctx, int loop, double time);
    rule's clock, :98, 171), restarts numRestarts_ (shadow-residual restarts,
    :177-182), resets pResets_ (solves ended by the NaN / Inf reset,
    :143-163), iters totalNumIter_ */
+/* per-phase device time of cfd_step, the reference's clockstruct stopwatches
+   (inc/profiling/STL_clock.hpp:35-47) around the same parts of the loop
+   (inc/run/RUN_CPU_PC_METHOD.hpp:296-607), as HIP events on the context
+   stream.  checkL2norm, updateT3toT0 and getCdCl are fused into the update
+   kernel, so their time is in CFD_PHASE_UPDATE.  Off by default;
+   cfd_phase_timing switches it and clears the totals. */
+#define CFD_PHASE_CONVECTION 0 /* convectionDifussion: predictor and source add */
+#define CFD_PHASE_PRESSURE 1   /* pressure: right-hand side, solve, scatter */
+#define CFD_PHASE_UPDATE 2     /* updateT1toT3 (+ checkL2norm, updateT3toT0, getCdCl) */
+#define CFD_PHASE_BC 3         /* BC */
+#define CFD_NPHASES 4
+int cfd_phase_timing(cfd_ctx *ctx, int on);
+/* milliseconds per phase summed over the timed steps, and their number */
+int cfd_phase_times(cfd_ctx *ctx, double ms[CFD_NPHASES], int *steps);
+
 typedef struct cfd_solver_counters {
   int solves, restarts, resets, pad;
   long long iters;
