@@ -14,7 +14,9 @@ from pathlib import Path
 import numpy as np
 
 PKG = Path(__file__).resolve().parent
-LIB_PATH = PKG / "_lib" / "libcfd_hip.so"
+# CFD_HIP_LIB names another build of the same library (e.g. one whose host
+# code is built with sanitizers)
+LIB_PATH = Path(os.environ["CFD_HIP_LIB"]) if os.environ.get("CFD_HIP_LIB") else PKG / "_lib" / "libcfd_hip.so"
 
 # status codes
 CFD_OK = 0

@@ -29,6 +29,8 @@ def lib(native: bool = False):
         return _LIB
     name = "liboracle_native.so" if native else "liboracle.so"
     path = ORACLE_DIR / "_build" / name
+    if not native and os.environ.get("CFD_ORACLE_LIB"):
+        path = Path(os.environ["CFD_ORACLE_LIB"])  # e.g. the sanitizer build
     if not path.exists():
         subprocess.run(["make", "-C", str(ORACLE_DIR), "native" if native else "all"], check=True,
                        stdout=subprocess.DEVNULL)
