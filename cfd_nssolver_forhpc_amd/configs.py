@@ -98,3 +98,102 @@ def small_cylinder(ncal=(32, 16, 8), lengths=(8.0, 4.0, 2.0), quirk=0, tol=1e-10
     p.source_index_quirk = quirk
     p.inner_loops = inner
     return Workload("small", g, p, 10, 1e-3, 7)
+
+
+# ---------------------------------------------------------------------------
+# runtime configuration under the reference's option names
+# ---------------------------------------------------------------------------
+# The reference fixes its configuration at compile time (inc/controlPanel.hpp
+# macros, the BC table #included in Boundary_Condition.hpp:9-11) and in main
+# (simuClass setters, src/CFD_MX.cpp:35-57).  Here the same names select the
+# same things in cfd_params at run time.
+
+_P_SOLVER = {  # controlPanel.hpp:17-29
+    "P_SOLVER_BICG_ELL": (capi.CFD_SOLVER_BICGSTAB, capi.CFD_OP_ELL),
+    "P_SOLVER_BICG_SPE": (capi.CFD_SOLVER_BICGSTAB, capi.CFD_OP_SPE),
+    "P_SOLVER_BICG_CSR": (capi.CFD_SOLVER_BICGSTAB, capi.CFD_OP_CSR),
+    # the reference's SOR is lexicographic (SorPipeLine); this build's is
+    # red-black on the ELL operator, which converges to the same solution
+    "P_SOLVER_SOR": (capi.CFD_SOLVER_RBSOR, capi.CFD_OP_ELL),
+}
+_CONVECTION = {  # controlPanel.hpp:125-129
+    "CONVECTION_DIFUSSION_QUICK": capi.CFD_CONV_QUICK,
+    "CONVECTION_DIFUSSION_LUD": capi.CFD_CONV_LUD,
+    "CONVECTION_DIFUSSION_UD": capi.CFD_CONV_UD,
+}
+_BC_TABLES = {  # the table headers Boundary_Condition.hpp includes
+    "BC_Cavity_A": (capi.CAVITY_A_DIR, capi.CAVITY_A_NUM),
+    "FLOWpassing": (capi.FLOWPASSING_DIR, capi.FLOWPASSING_NUM),
+}
+_METHOD = {"PC_METHOD": 3, "SINGLE_PASS": 0}  # runSeqOmp_predictionMethod / runSeqOmp
+
+
+def params_from_options(options: dict, base: CfdParams | None = None) -> CfdParams:
+    """cfd_params from the reference's option names:
+
+    * simuClass: Re, dt (set_Re / set_dt), p_criteria, p_sor_omega,
+      DfibMethod ("DFIB_Cylinder-Z" or "OFF"), with the cylinder as
+      Dfib centre ib_cx, ib_cy and radius ib_r;
+    * controlPanel.hpp: P_SOLVER (P_SOLVER_BICG_ELL / _SPE / _CSR,
+      P_SOLVER_SOR), CONVECTION_DIFUSSION (CONVECTION_DIFUSSION_QUICK / _LUD
+      / _UD), TERBULENCE_SMAGORINSKY (true / false);
+    * the BC table, BC ("BC_Cavity_A" or "FLOWpassing"), and the driver,
+      method ("PC_METHOD": 3 predictor passes and a corrector, or
+      "SINGLE_PASS");
+    * the additions: precision ("FP64" / "MIXED"), precond ("JACOBI" / "MG"),
+      mg_nu, tol_inner, maxit.
+
+    Unknown names and values are errors, as are options this build does not
+    have (other IB shapes, the AMGCL and Eigen solvers)."""
+    p = (base or default_params()).copy()
+    for key, val in options.items():
+        if key == "Re":
+            p.Re = float(val)
+        elif key == "dt":
+            p.dt = float(val)
+        elif key == "p_criteria":
+            p.tol = float(val)
+        elif key == "p_sor_omega":
+            p.sor_omega = float(val)
+        elif key == "DfibMethod":
+            if val not in ("DFIB_Cylinder-Z", "OFF"):
+                raise ValueError(f"DfibMethod {val!r}: only DFIB_Cylinder-Z and OFF are built")
+            p.ib_on = 1 if val == "DFIB_Cylinder-Z" else 0
+        elif key in ("ib_cx", "ib_cy", "ib_r"):
+            setattr(p, key, float(val))
+        elif key == "P_SOLVER":
+            if val not in _P_SOLVER:
+                raise ValueError(f"P_SOLVER {val!r}: one of {sorted(_P_SOLVER)}")
+            p.solver, p.op = _P_SOLVER[val]
+        elif key == "CONVECTION_DIFUSSION":
+            if val not in _CONVECTION:
+                raise ValueError(f"CONVECTION_DIFUSSION {val!r}: one of {sorted(_CONVECTION)}")
+            p.convection = _CONVECTION[val]
+        elif key == "TERBULENCE_SMAGORINSKY":
+            p.les = 1 if bool(val) else 0
+        elif key == "BC":
+            if val not in _BC_TABLES:
+                raise ValueError(f"BC {val!r}: one of {sorted(_BC_TABLES)}")
+            p.set_bc(*_BC_TABLES[val])
+        elif key == "method":
+            if val not in _METHOD:
+                raise ValueError(f"method {val!r}: one of {sorted(_METHOD)}")
+            p.inner_loops = _METHOD[val]
+        elif key == "precision":
+            p.precision = {"FP64": capi.CFD_FP64, "MIXED": capi.CFD_MIXED}[val]
+        elif key == "precond":
+            p.precond = {"JACOBI": capi.CFD_PRECOND_JACOBI, "MG": capi.CFD_PRECOND_MG}[val]
+        elif key in ("mg_nu", "maxit"):
+            setattr(p, key, int(val))
+        elif key == "tol_inner":
+            p.tol_inner = float(val)
+        else:
+            raise ValueError(f"unknown option {key!r}")
+    return p
+
+
+def params_from_json(path) -> CfdParams:
+    """params_from_options of a JSON object read from path."""
+    import json
+    from pathlib import Path
+    return params_from_options(json.loads(Path(path).read_text()))
