@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(256) spmv_kernel(OpArgs a, const double *__res
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.N; base += stride) {
     const double v = op_row_block(a, base, x);
-    if (base + threadIdx.x < a.N) y[base + threadIdx.x] = v;
+    if (base + threadIdx.x < a.N) ntstore(v, y + base + threadIdx.x);
   }
 }
 

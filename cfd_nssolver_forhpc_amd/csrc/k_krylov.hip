@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(256) spmv_dot_kernel(OpArgs a, const KrylovSca
     const T y = op_apply_block<T>(a, base, p1);
     const int64_t r = base + threadIdx.x;
     if (r >= a.N) continue;
-    ap[r] = y;
+    ntstore(y, ap + r);
     acc[0] += (double)y * (double)r0[r];
   }
   block_partials<1>(acc, part);
@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(256) spmv_dot2_kernel(OpArgs a, const KrylovSc
     const T y = op_apply_block<T>(a, base, s1);
     const int64_t r = base + threadIdx.x;
     if (r >= a.N) continue;
-    as[r] = y;
+    ntstore(y, as + r);
     acc[0] += (double)y * (double)s1[r];
     acc[1] += (double)y * (double)y;
   }
@@ -171,9 +171,9 @@ __global__ void __launch_bounds__(256) xr_update_kernel(int64_t N, const KrylovS
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += stride) {
     const double s = (double)s1[r];
-    x[r] = (T)((double)x[r] + (alpha * (double)p1[r] + omega * s));
+    ntstore((T)((double)x[r] + (alpha * (double)p1[r] + omega * s)), x + r);
     const T rn = (T)(s - omega * (double)as[r]);
-    r1[r] = rn;
+    ntstore(rn, r1 + r);
     acc[0] += (double)rn * (double)rn;
     acc[1] += (double)rn * (double)r0[r];
   }
@@ -251,9 +251,9 @@ __global__ void __launch_bounds__(256) xr_update_pc_kernel(int64_t N, const Kryl
   double acc[2] = {0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += stride) {
-    x[r] = (T)((double)x[r] + (alpha * (double)ph[r] + omega * (double)sh[r]));
+    ntstore((T)((double)x[r] + (alpha * (double)ph[r] + omega * (double)sh[r])), x + r);
     const T rn = (T)((double)s1[r] - omega * (double)tv[r]);
-    r1[r] = rn;
+    ntstore(rn, r1 + r);
     acc[0] += (double)rn * (double)rn;
     acc[1] += (double)rn * (double)r0[r];
   }
@@ -809,7 +809,7 @@ __global__ void __launch_bounds__(256) mr_spmv_dot2_kernel(OpArgs a, const Krylo
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) {
     const T y = op_apply<T>(a, r, s1);
-    as[r] = y;
+    ntstore(y, as + r);
     acc[0] += (double)y * (double)s1[r];
     acc[1] += (double)y * (double)y;
   }
