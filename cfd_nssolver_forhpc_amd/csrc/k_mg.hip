@@ -91,19 +91,6 @@ void mg_free(T *&p) {
   p = nullptr;
 }
 
-struct Cell3 {
-  int i, j, k;
-};
-
-__device__ inline Cell3 dec3(const MGView &L, int64_t t) {
-  Cell3 c;
-  c.k = (int)(t % L.nz);
-  const int64_t r = t / L.nz;
-  c.j = (int)(r % L.ny);
-  c.i = (int)(r / L.ny);
-  return c;
-}
-
 __device__ inline int64_t idx3(const MGView &L, int i, int j, int k) { return ((int64_t)i * L.ny + j) * L.nz + k; }
 
 // a cell's own index and its neighbours' (meaningful where the neighbour exists)
