@@ -128,6 +128,95 @@ struct OpArgs {
   const double *cval;
 };
 
+// The SPE operator over plane columns.  A 1-D grid of 256-thread blocks:
+// block b covers a tile of kb x jb (k, j) columns (kb a power of two, at most
+// 64, jb = 256 / kb) over SPE_PLANES consecutive i-planes; thread t owns the
+// column (k, j) of the tile and walks the planes.  The entry x at (i', j, k)
+// is loaded once and serves plane i' as its diagonal and planes i' -+ 1 as
+// their x-neighbours; the y and z coefficients are loaded once per thread and
+// the x ones are plane-uniform.  Each row's terms are spe_row's, in its order,
+// so every output is spe_row's bit for bit.
+constexpr int SPE_PLANES = 8;
+
+struct SpeTiles {
+  int kb, jb, nkt, njt, nig;
+  int64_t blocks;
+};
+
+inline SpeTiles spe_tiles(int nxc, int nyc, int nzc) {
+  SpeTiles t;
+  t.kb = 1;
+  while (t.kb < nzc && t.kb < 64) t.kb <<= 1;
+  t.jb = 256 / t.kb;
+  t.nkt = (nzc + t.kb - 1) / t.kb;
+  t.njt = (nyc + t.jb - 1) / t.jb;
+  t.nig = (nxc + SPE_PLANES - 1) / SPE_PLANES;
+  t.blocks = (int64_t)t.nkt * t.njt * t.nig;
+  return t;
+}
+
+// this thread's column and first plane; false outside the grid
+struct SpeColumn {
+  int i0, j, k;
+  bool ok;
+};
+
+__device__ inline SpeColumn spe_column(const OpArgs &a, const SpeTiles &T) {
+  const int b = blockIdx.x;
+  const int tk = b % T.nkt, rest = b / T.nkt;
+  SpeColumn c;
+  c.k = tk * T.kb + (int)(threadIdx.x % T.kb);
+  c.j = (rest % T.njt) * T.jb + (int)(threadIdx.x / T.kb);
+  c.i0 = (rest / T.njt) * SPE_PLANES;
+  c.ok = c.k < a.nzc && c.j < a.nyc;
+  return c;
+}
+
+// y[p] = (A x)[row of plane i0 + p] and xc[p] = x at that row, for the
+// planes inside the grid (np of them)
+__device__ inline int spe_column_rows(const OpArgs &a, const SpeColumn &c, const double *__restrict__ x,
+                                      double (&y)[SPE_PLANES], double (&xc)[SPE_PLANES]) {
+  const int nxc = a.nxc, nyc = a.nyc, nzc = a.nzc;
+  const int64_t nynz = (int64_t)nyc * nzc, col = (int64_t)c.j * nzc + c.k;
+  const double *xm = a.coef, *xp = xm + nxc, *ym = xp + nxc, *yp = ym + nyc, *zm = yp + nyc, *zp = zm + nzc;
+  const bool hym = c.j != 0, hzm = c.k != 0, hzp = c.k != nzc - 1, hyp = c.j != nyc - 1;
+  const double cym = ym[c.j], cyp = yp[c.j], czm = zm[c.k], czp = zp[c.k];
+  double cen[SPE_PLANES + 2];
+#pragma unroll
+  for (int p = 0; p < SPE_PLANES + 2; ++p) {
+    const int ip = min(max(c.i0 - 1 + p, 0), nxc - 1);  // clamped planes are never used
+    cen[p] = x[ip * nynz + col];
+  }
+  const int np = min(SPE_PLANES, nxc - c.i0);
+  double vym[SPE_PLANES], vyp[SPE_PLANES], vzm[SPE_PLANES], vzp[SPE_PLANES], J[SPE_PLANES];
+#pragma unroll
+  for (int p = 0; p < SPE_PLANES; ++p) {
+    const int64_t ic = (int64_t)min(c.i0 + p, nxc - 1) * nynz + col;
+    vym[p] = x[hym ? ic - nzc : ic];
+    vyp[p] = x[hyp ? ic + nzc : ic];
+    vzm[p] = x[hzm ? ic - 1 : ic];
+    vzp[p] = x[hzp ? ic + 1 : ic];
+    J[p] = a.jp[ic];
+  }
+#pragma unroll
+  for (int p = 0; p < SPE_PLANES; ++p) {
+    const int i = c.i0 + p;
+    const bool hxm = i != 0, hxp = i != nxc - 1;
+    const double cxm = xm[min(i, nxc - 1)], cxp = xp[min(i, nxc - 1)];
+    double sum = 0.0;
+    sum = hxm ? fma(-cxm * J[p], cen[p], sum) : sum;
+    sum = hym ? fma(-cym * J[p], vym[p], sum) : sum;
+    sum = hzm ? fma(-czm * J[p], vzm[p], sum) : sum;
+    sum += cen[p + 1];
+    sum = hzp ? fma(-czp * J[p], vzp[p], sum) : sum;
+    sum = hyp ? fma(-cyp * J[p], vyp[p], sum) : sum;
+    sum = hxp ? fma(-cxp * J[p], cen[p + 2], sum) : sum;
+    y[p] = sum;
+    xc[p] = cen[p + 1];
+  }
+  return np;
+}
+
 // internal operator id: the ELL operator with FP32 values applied to an FP64
 // vector (FP64 products and sum)
 constexpr int OP_ELL_VAL32 = 16;

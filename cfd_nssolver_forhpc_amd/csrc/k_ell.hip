@@ -161,6 +161,20 @@ __global__ void __launch_bounds__(256) spmv_kernel(OpArgs a, const double *__res
   }
 }
 
+// the SPE operator over plane columns (spe_column_rows): the rows of
+// spmv_kernel's SPE path bit for bit, with a third of its loads
+__global__ void __launch_bounds__(256) spe_spmv_kernel(OpArgs a, SpeTiles T, const double *__restrict__ x,
+                                                       double *__restrict__ y) {
+  const SpeColumn c = spe_column(a, T);
+  if (!c.ok) return;
+  double v[SPE_PLANES], xc[SPE_PLANES];
+  const int np = spe_column_rows(a, c, x, v, xc);
+  const int64_t nynz = (int64_t)a.nyc * a.nzc, col = (int64_t)c.j * a.nzc + c.k;
+#pragma unroll
+  for (int p = 0; p < SPE_PLANES; ++p)
+    if (p < np) ntstore(v[p], y + (int64_t)(c.i0 + p) * nynz + col);
+}
+
 // FP32 values and vectors, FP32 accumulation (the mixed-precision inner operator)
 __global__ void __launch_bounds__(256) spmv32_kernel(OpArgs a, const float *__restrict__ x, float *__restrict__ y) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -231,8 +245,11 @@ hipError_t launch_spe_assemble(Ctx &c) {
 
 hipError_t launch_spmv(Ctx &c, int op, int precision, const double *x, double *y) {
   const OpArgs a = op_args(c, op);
+  const SpeTiles T = spe_tiles(c.g.nxc, c.g.nyc, c.g.nzc);
   if (precision == CFD_MIXED && op == CFD_OP_ELL)
     spmv32_from64_kernel<<<stream_blocks(c.g.N), 256, 0, c.stream>>>(a, x, y);
+  else if (op == CFD_OP_SPE && T.blocks <= (1 << 30))
+    spe_spmv_kernel<<<(unsigned)T.blocks, 256, 0, c.stream>>>(a, T, x, y);
   else
     spmv_kernel<<<stream_blocks(c.g.N), 256, op_lds_bytes(a), c.stream>>>(a, x, y);
   return hipGetLastError();

@@ -118,6 +118,52 @@ __global__ void __launch_bounds__(256) spmv_dot_kernel(OpArgs a, const KrylovSca
   block_partials<1>(acc, part);
 }
 
+// spmv_dot and spmv_dot2 for the SPE operator over plane columns
+// (spe_column_rows): the same rows, a third of the loads; the partials of a
+// block sum its columns' planes, so the dot products are ordered differently
+// from the row-wise kernels (deterministically)
+__global__ void __launch_bounds__(256) spe_spmv_dot_kernel(OpArgs a, SpeTiles T, const KrylovScalars *__restrict__ ks,
+                                                           const double *__restrict__ p1, double *__restrict__ ap,
+                                                           const double *__restrict__ r0, double *part) {
+  if (ks->done) return;
+  double acc[1] = {0.0};
+  const SpeColumn c = spe_column(a, T);
+  if (c.ok) {
+    double y[SPE_PLANES], xc[SPE_PLANES];
+    const int np = spe_column_rows(a, c, p1, y, xc);
+    const int64_t nynz = (int64_t)a.nyc * a.nzc, col = (int64_t)c.j * a.nzc + c.k;
+#pragma unroll
+    for (int p = 0; p < SPE_PLANES; ++p)
+      if (p < np) {
+        const int64_t r = (int64_t)(c.i0 + p) * nynz + col;
+        ntstore(y[p], ap + r);
+        acc[0] += y[p] * r0[r];
+      }
+  }
+  block_partials<1>(acc, part);
+}
+
+__global__ void __launch_bounds__(256) spe_spmv_dot2_kernel(OpArgs a, SpeTiles T, const KrylovScalars *__restrict__ ks,
+                                                            const double *__restrict__ s1, double *__restrict__ as,
+                                                            double *part) {
+  if (ks->done) return;
+  double acc[2] = {0.0, 0.0};
+  const SpeColumn c = spe_column(a, T);
+  if (c.ok) {
+    double y[SPE_PLANES], xc[SPE_PLANES];
+    const int np = spe_column_rows(a, c, s1, y, xc);
+    const int64_t nynz = (int64_t)a.nyc * a.nzc, col = (int64_t)c.j * a.nzc + c.k;
+#pragma unroll
+    for (int p = 0; p < SPE_PLANES; ++p)
+      if (p < np) {
+        ntstore(y[p], as + (int64_t)(c.i0 + p) * nynz + col);
+        acc[0] += y[p] * xc[p];
+        acc[1] += y[p] * y[p];
+      }
+  }
+  block_partials<2>(acc, part);
+}
+
 __global__ void __launch_bounds__(FIN_THREADS) alpha_finalize(const double *part, int nblk, KrylovScalars *ks) {
   if (ks->done) return;
   double tot[1];
@@ -327,12 +373,25 @@ hipError_t launch_iters(Ctx &c, const OpArgs &a, const Vecs<T> &v, KrylovScalars
   const int64_t N = c.g.N;
   const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
   double *p1 = red_part_host(c, RS_SPMV_DOT), *p2 = red_part_host(c, RS_SPMV_DOT2), *p3 = red_part_host(c, RS_XR);
+  // the SPE operator's SpMV-dot kernels over plane columns (FP64)
+  const SpeTiles tiles = spe_tiles(c.g.nxc, c.g.nyc, c.g.nzc);
+  const bool spe = std::is_same<T, double>::value && a.op == CFD_OP_SPE && tiles.blocks <= RED_MAX_BLOCKS;
+  const int nbs = spe ? (int)tiles.blocks : nb;
   for (int it = 0; it < nit; ++it) {
-    spmv_dot_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, ks, v.p1, v.ap, v.r0, p1);
-    alpha_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p1, nb, ks);
+    if (spe)
+      spe_spmv_dot_kernel<<<nbs, 256, 0, c.stream>>>(a, tiles, ks, reinterpret_cast<const double *>(v.p1),
+                                                     reinterpret_cast<double *>(v.ap),
+                                                     reinterpret_cast<const double *>(v.r0), p1);
+    else
+      spmv_dot_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, ks, v.p1, v.ap, v.r0, p1);
+    alpha_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p1, nbs, ks);
     s_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.r1, v.ap, v.s1);
-    spmv_dot2_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, ks, v.s1, v.as, p2);
-    omega_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p2, nb, ks);
+    if (spe)
+      spe_spmv_dot2_kernel<<<nbs, 256, 0, c.stream>>>(a, tiles, ks, reinterpret_cast<const double *>(v.s1),
+                                                      reinterpret_cast<double *>(v.as), p2);
+    else
+      spmv_dot2_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, ks, v.s1, v.as, p2);
+    omega_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p2, nbs, ks);
     xr_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.x, v.p1, v.s1, v.as, v.r1, v.r0, p3);
     conv_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p3, nb, ks);
     p_update_kernel<T><<<nb, 256, 0, c.stream>>>(N, ks, v.r1, v.p1, v.ap, v.r0);

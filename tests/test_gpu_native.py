@@ -9,10 +9,14 @@ whole trajectories against the compiled reference's own (tests/golden):
   at a 1e-3 relative residual legitimately differ by up to a few percent in
   cD; the oracle, which repeats the reference's functions bit for bit,
   differs from it by up to 3.0% (step 8) and by 0.3% at step 50.  The bars
-  (6%, 1.5% over the last 25 steps) are twice those.  Iteration counts per
-  pass scatter by up to 70% between the two (a loose stop lands anywhere in a
-  slow stretch of the convergence), while their 50-step totals agree to
-  0.3%; the total is held to 5%.
+  (6%, 1.5% over the last 25 steps) are twice those.  A loose stop lands
+  anywhere in a slow stretch of the convergence, so the iteration count of a
+  single pass moves with the summation order alone: the reference run with
+  8 OpenMP threads differs from its serial run by up to 59% of a pass's
+  iterations (44 against 70) and by up to 44 on a pass below 100
+  (tests/golden, spe_1e-3_omp8).  A pass is held to twice that scatter;
+  the 50-step totals, which agree to 0.3% (1.5% for the reference against
+  itself), are held to 5%.
 * 20 steps at 1e-8, where the comparison is tight: cD, cL and the
   max-difference of every pass and step.  This run adds the source term at
   the cell it belongs to: with the driver's icelCal index the right-hand
@@ -54,6 +58,11 @@ def by_pass(t):
 def test_native_trajectory_at_reference_tolerance(require_gpu):
     t = TRAJ["spe_1e-3"]
     ref, _ = by_pass(t)
+    ref8, _ = by_pass(TRAJ["spe_1e-3_omp8"])
+    # the reference's own per-pass scatter with another summation order
+    own = [(abs(ref8[key][0] - v[0]), v[0]) for key, v in ref.items() if key[0] > 1]
+    rel_bar = 2 * max(d / it for d, it in own)
+    abs_floor = 2 * max(d for d, it in own if it < 100)
     stats = run(t["tol"], t["steps"])
     late = []
     total = ref_total = 0
@@ -66,7 +75,7 @@ def test_native_trajectory_at_reference_tolerance(require_gpu):
                 assert abs(st.cd[q] - cd) <= 1e-12 * abs(cd), (k, q)
                 continue
             assert st.relres[q] < t["tol"] and err < t["tol"]
-            assert abs(st.iters[q] - it) <= max(40, 0.8 * it), (k, q, st.iters[q], it)
+            assert abs(st.iters[q] - it) <= max(abs_floor, rel_bar * it), (k, q, st.iters[q], it)
             total += st.iters[q]
             ref_total += it
         e = abs(st.cd[3] - ref[(k, 3)][2]) / abs(ref[(k, 3)][2])
