@@ -69,16 +69,31 @@ __device__ inline void combine_partials(const double *part, int nblk, double (&t
       }
       v[q] = s;
     } else {
-      // eight independent accumulators keep eight loads in flight per
-      // thread; the combination order is fixed, so the sum is deterministic
-      double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      int b = threadIdx.x;
-      for (; b + 7 * FIN_THREADS < nblk; b += 8 * FIN_THREADS) {
+      // the slots a thread owns are loaded sixteen at a time (clamped
+      // index, slots past nblk select zero) and added in a fixed pairwise
+      // order, so the sum is deterministic and the finalize waits for one
+      // round trip to memory per sixteen slots instead of one per slot
+      constexpr int PER = RED_MAX_BLOCKS / FIN_THREADS, CH = 16;
+      static_assert(PER % CH == 0 && (CH & (CH - 1)) == 0, "slots per thread");
+      const int last = nblk > 0 ? nblk - 1 : 0;
+      double acc = 0.0;
+#pragma unroll 1
+      for (int c0 = 0; c0 < PER && c0 * FIN_THREADS < nblk; c0 += CH) {
+        double s[CH];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) s[u] += pq[b + u * FIN_THREADS];
+        for (int u = 0; u < CH; ++u) {
+          const int b = threadIdx.x + (c0 + u) * FIN_THREADS;
+          const double x = pq[b < nblk ? b : last];
+          s[u] = b < nblk ? x : 0.0;
+        }
+#pragma unroll
+        for (int w = 1; w < CH; w *= 2) {
+#pragma unroll
+          for (int u = 0; u < CH; u += 2 * w) s[u] += s[u + w];
+        }
+        acc += s[0];
       }
-      for (; b < nblk; b += FIN_THREADS) s[0] += pq[b];
-      v[q] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+      v[q] = acc;
     }
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;

@@ -71,7 +71,15 @@ def test_gradient_matches_central_differences(require_gpu):
     """About gain 0.9: at gain 1 exactly the fully immersed cells hold exact
     zeros, whose QUICK upwind test flips with the sign of (gain - 1), so the
     discrete objective has a small jump there that central differences
-    straddle."""
+    straddle.
+
+    The bar is the 1e-5 relative agreement the adjoint is specified to.  The
+    central difference itself carries noise: the forward runs' pressure
+    solves stop at a relative residual of 1e-13, which leaves ~1e-12 in J,
+    and (J+ - J-) / 2h with h = 1e-6 turns that into ~1e-6 in the
+    difference quotient (0.64 here), so a tighter bar would test the solver's
+    stopping point, not the gradient.  The gradient itself is pinned to the
+    oracle's adjoint at 1e-8 above."""
     w, uin = case()
     pr = gpu_problem(w, every=4)
     base = 0.9
@@ -86,8 +94,8 @@ def test_gradient_matches_central_differences(require_gpu):
         Jm = pr.run(uin - h * d[:-1], base - h * d[-1])
         fd = (Jp - Jm) / (2 * h)
         ad = float(np.dot(g_uin, d[:-1]) + g_gain * d[-1])
-        print("fd", fd, "adjoint", ad)
-        assert rel(fd, ad) < 1e-6
+        print("fd", fd, "adjoint", ad, "rel", rel(fd, ad))
+        assert rel(fd, ad) < 1e-5
 
 
 def test_checkpoint_interval_does_not_change_the_gradient(require_gpu):
