@@ -255,16 +255,46 @@ __device__ inline double op_row_block(const OpArgs &a, int64_t base, const doubl
   const int32_t cnt = a.cptr[rend] - e0;
   if (cnt > CSR_STAGE)  // more than 7 entries a row on average: not staged (block-uniform)
     return r < a.N ? csr_row(a.cptr, a.ccol, a.cval, r, x) : 0.0;
+  // every slot a thread stages is loaded before the first LDS store (slots
+  // past cnt load the block's last entry and are not stored), so staging
+  // costs one round trip to memory instead of one per slot
+  double sv[KSLOTS];
+  int32_t sc[KSLOTS];
+  if (cnt > 0) {  // block-uniform; every row holds its diagonal, so always taken
+#pragma unroll
+    for (int u = 0; u < KSLOTS; ++u) {
+      const int32_t q = threadIdx.x + u * OP_BLOCK, qc = q < cnt ? q : cnt - 1;
+      sv[u] = ntload(a.cval + e0 + qc);
+      sc[u] = ntload(a.ccol + e0 + qc);
+    }
+  }
   __syncthreads();  // the previous block's rows have been read
-  for (int32_t q = threadIdx.x; q < cnt; q += OP_BLOCK) {
-    sval[q] = ntload(a.cval + e0 + q);
-    scol[q] = ntload(a.ccol + e0 + q);
+#pragma unroll
+  for (int u = 0; u < KSLOTS; ++u) {
+    const int32_t q = threadIdx.x + u * OP_BLOCK;
+    if (q < cnt) {
+      sval[q] = sv[u];
+      scol[q] = sc[u];
+    }
   }
   __syncthreads();
   if (r >= a.N) return 0.0;
-  const int32_t e1 = a.cptr[r + 1] - e0;
+  const int32_t eb = a.cptr[r] - e0, e1 = a.cptr[r + 1] - e0;
+  // the row's first KSLOTS entries: every LDS read and x gather is issued
+  // before the first FMA (entries past the row read slot 0 and are not
+  // added); the FMAs keep csr_row's order
+  double vv[KSLOTS], xv[KSLOTS];
+#pragma unroll
+  for (int u = 0; u < KSLOTS; ++u) {
+    const int32_t e = eb + u < e1 ? eb + u : 0;
+    vv[u] = sval[e];
+    xv[u] = x[scol[e]];
+  }
   double acc = 0.0;
-  for (int32_t e = a.cptr[r] - e0; e < e1; ++e) acc = fma(sval[e], x[scol[e]], acc);
+#pragma unroll
+  for (int u = 0; u < KSLOTS; ++u)
+    if (eb + u < e1) acc = fma(vv[u], xv[u], acc);
+  for (int32_t e = eb + KSLOTS; e < e1; ++e) acc = fma(sval[e], x[scol[e]], acc);
   return acc;
 }
 
