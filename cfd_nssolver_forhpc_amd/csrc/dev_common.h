@@ -13,10 +13,32 @@
 
 namespace cfd {
 
+// One DPP lane move of a double (its two 32-bit halves); CTRL is the DPP
+// control word.
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes of a wave, returned to every lane; all 64 lanes
+// must be active.  Lane moves within and across the rows of 16 lanes (the
+// DPP sequence of AMD's GCN cross-lane reduction: quad permutes, row
+// rotates by 4 and 8, row broadcasts of lanes 15 and 31) leave the total in
+// lane 63, which is read back with readlane; the adds run on the VALU with
+// no LDS round trip per step (a shuffle is a ds_bpermute), in a fixed order.
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
+  v += dpp_move<0xb1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4e>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x124>(v);  // row_ror:4
+  v += dpp_move<0x128>(v);  // row_ror:8
+  v += dpp_move<0x142>(v);  // row_bcast:15
+  v += dpp_move<0x143>(v);  // row_bcast:31
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 __device__ inline double wave_max(double v) {
