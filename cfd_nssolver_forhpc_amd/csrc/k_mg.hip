@@ -479,7 +479,7 @@ __global__ void __launch_bounds__(256) mg_restrict_kernel(MGView F, MGView C, co
   restrict_at<T, Lay>(F, C, I, J, K, bf, xf, bc, xc);
 }
 
-constexpr int MG_SMALL_THREADS = 256;
+constexpr int MG_SMALL_THREADS = 512;
 
 template <typename T>
 struct SmallArgs {
