@@ -17,7 +17,7 @@
 // Dirichlet face keeps its mirror-ghost coefficient).  Restriction is the
 // volume-weighted sum of the children's residuals, prolongation adds the
 // parent's correction.  Smoother: red-black Gauss-Seidel, nu sweeps before
-// and after.  Levels of at most MG_SMALL cells run inside one 256-thread
+// and after.  Levels of at most MG_SMALL cells run inside one 512-thread
 // workgroup (one launch for the whole bottom of the cycle); the coarsest is
 // solved by its dense inverse or smoothed MG_COARSE_SWEEPS times.
 //
