@@ -6,25 +6,43 @@
 #                          UndefinedBehaviorSanitizer (device code unchanged;
 #                          loaded with clang's ASan runtime preloaded, for the
 #                          CPU tests of the host-side entry points)
+# Each translation unit compiles in its own hipcc process (in parallel, at
+# most BUILD_JOBS at once), then one link step makes the shared library.
 set -e
 HERE="$(cd "$(dirname "$0")" && pwd)"
-mkdir -p "$HERE/_lib"
+mkdir -p "$HERE/_lib" "$HERE/_lib/obj"
 HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
-SRC="$HERE/csrc/capi.hip $HERE/csrc/k_flow.hip $HERE/csrc/k_ell.hip $HERE/csrc/k_krylov.hip $HERE/csrc/k_sor.hip $HERE/csrc/adjoint.hip $HERE/csrc/k_mg.hip $HERE/csrc/io.hip"
+UNITS="capi k_flow k_ell k_krylov k_sor adjoint k_mg io"
+COMMON="--offload-arch=gfx950 -std=c++17 -fPIC -Xclang -target-feature -Xclang -packed-fp32-ops -I $HERE/../include"
 if [ "$1" = "sanitize" ]; then
   OUT="$HERE/_lib/libcfd_hip_san.so"
-  # shellcheck disable=SC2086
-  "$HIPCC" --offload-arch=gfx950 -O1 -g -std=c++17 -fPIC -shared -fno-omit-frame-pointer \
-    -Xclang -target-feature -Xclang -packed-fp32-ops \
-    -Xarch_host -fsanitize=address -Xarch_host -fsanitize=undefined -Xarch_host -shared-libasan \
-    -I "$HERE/../include" -o "$OUT.tmp" $SRC
+  TAG=san
+  FLAGS="$COMMON -O1 -g -fno-omit-frame-pointer -Xarch_host -fsanitize=address -Xarch_host -fsanitize=undefined"
+  LINK="-Xarch_host -fsanitize=address -Xarch_host -fsanitize=undefined -Xarch_host -shared-libasan"
 else
   OUT="$HERE/_lib/libcfd_hip.so"
-  # shellcheck disable=SC2086
-  "$HIPCC" --offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared \
-    -Xclang -target-feature -Xclang -packed-fp32-ops \
-    -Wall -Wno-unused-function -Wno-unused-variable \
-    -I "$HERE/../include" \
-    -o "$OUT.tmp" $SRC
+  TAG=opt
+  FLAGS="$COMMON -O3 -Wall -Wno-unused-function -Wno-unused-variable"
+  LINK=""
 fi
+JOBS="${BUILD_JOBS:-8}"
+pids=""
+n=0
+for u in $UNITS; do
+  # shellcheck disable=SC2086
+  "$HIPCC" $FLAGS -c "$HERE/csrc/$u.hip" -o "$HERE/_lib/obj/$u.$TAG.o" 2>"$HERE/_lib/obj/$u.$TAG.err" &
+  pids="$pids $!"
+  n=$((n + 1))
+  if [ "$n" -ge "$JOBS" ]; then
+    for p in $pids; do wait "$p" || { cat "$HERE"/_lib/obj/*."$TAG".err >&2; exit 1; }; done
+    pids=""
+    n=0
+  fi
+done
+for p in $pids; do wait "$p" || { cat "$HERE"/_lib/obj/*."$TAG".err >&2; exit 1; }; done
+grep -h -v "packed-fp32-ops" "$HERE"/_lib/obj/*."$TAG".err >&2 || true
+OBJS=""
+for u in $UNITS; do OBJS="$OBJS $HERE/_lib/obj/$u.$TAG.o"; done
+# shellcheck disable=SC2086
+"$HIPCC" --offload-arch=gfx950 -shared -fPIC $LINK -o "$OUT.tmp" $OBJS
 mv "$OUT.tmp" "$OUT"

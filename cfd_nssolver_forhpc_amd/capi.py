@@ -28,6 +28,7 @@ CFD_ERR_RUNTIME = -5
 
 CFD_SOLVER_BICGSTAB = 0
 CFD_SOLVER_RBSOR = 1
+CFD_SOLVER_SOR_PIPELINE = 2
 CFD_OP_ELL = 0
 CFD_OP_SPE = 1
 CFD_OP_CSR = 2
@@ -102,6 +103,8 @@ class CfdStepStats(C.Structure):
         ("cd", C.c_double * MAX_PASSES), ("cl", C.c_double * MAX_PASSES),
         ("maxdiff", C.c_double * 3),
         ("inner_iters", C.c_int * MAX_PASSES),
+        ("div_max", C.c_double), ("div_l2", C.c_double), ("div_fluid_max", C.c_double),
+        ("div_fluid_l2", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -114,6 +117,8 @@ class CfdStepStats(C.Structure):
             "cl": list(self.cl[:n]),
             "maxdiff": list(self.maxdiff),
             "inner_iters": list(self.inner_iters[:n]),
+            "div": {"max": self.div_max, "l2": self.div_l2, "fluid_max": self.div_fluid_max,
+                    "fluid_l2": self.div_fluid_l2},
         }
 
 
@@ -246,6 +251,7 @@ def _declare(L: C.CDLL) -> None:
         "cfd_spmv": ([vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
         "cfd_bicgstab": ([vp, C.c_double, C.c_int, ip, dp], C.c_int),
         "cfd_rbsor": ([vp, C.c_double, C.c_double, C.c_int, C.c_int, ip, dp], C.c_int),
+        "cfd_sor_pipeline": ([vp, C.c_double, C.c_double, C.c_int, ip, dp], C.c_int),
         "cfd_defect_correction": ([vp, C.c_double, C.c_double, C.c_int, ip, ip, dp], C.c_int),
         "cfd_solve_pressure": ([vp, ip, dp], C.c_int),
         "cfd_predictor": ([vp], C.c_int),
@@ -257,8 +263,10 @@ def _declare(L: C.CDLL) -> None:
         "cfd_bc_main": ([vp], C.c_int),
         "cfd_bc_copy": ([vp], C.c_int),
         "cfd_drag": ([vp, dp, dp], C.c_int),
+        "cfd_force_sum": ([vp, C.c_int, dp], C.c_int),
         "cfd_maxdiff": ([vp, dp], C.c_int),
         "cfd_copy_t3_to_t0": ([vp], C.c_int),
+        "cfd_divergence": ([vp, C.c_int, dp], C.c_int),
         "cfd_step": ([vp, C.POINTER(CfdStepStats)], C.c_int),
         "cfd_loop": ([vp, ip, dp], C.c_int),
         "cfd_set_loop": ([vp, C.c_int, C.c_double], C.c_int),

@@ -112,9 +112,14 @@ _P_SOLVER = {  # controlPanel.hpp:17-29
     "P_SOLVER_BICG_ELL": (capi.CFD_SOLVER_BICGSTAB, capi.CFD_OP_ELL),
     "P_SOLVER_BICG_SPE": (capi.CFD_SOLVER_BICGSTAB, capi.CFD_OP_SPE),
     "P_SOLVER_BICG_CSR": (capi.CFD_SOLVER_BICGSTAB, capi.CFD_OP_CSR),
-    # the reference's SOR is lexicographic (SorPipeLine); this build's is
-    # red-black on the ELL operator, which converges to the same solution
-    "P_SOLVER_SOR": (capi.CFD_SOLVER_RBSOR, capi.CFD_OP_ELL),
+    # SorPipeLine's volume-weighted update on the ghosted pressure with its
+    # max |dp| stop; the sweep order is red-black instead of lexicographic
+    # (same fixed point, race-free on the device)
+    "P_SOLVER_SOR": (capi.CFD_SOLVER_SOR_PIPELINE, capi.CFD_OP_ELL),
+    # this build's red-black SOR on the Jacobi-scaled ELL / CSR operator
+    # with a relative-residual stop (no reference counterpart)
+    "RBSOR_ELL": (capi.CFD_SOLVER_RBSOR, capi.CFD_OP_ELL),
+    "RBSOR_CSR": (capi.CFD_SOLVER_RBSOR, capi.CFD_OP_CSR),
 }
 _CONVECTION = {  # controlPanel.hpp:125-129
     "CONVECTION_DIFUSSION_QUICK": capi.CFD_CONV_QUICK,

@@ -57,11 +57,18 @@ int check_params(const cfd_params *p) {
   if (!(p->Re > 0.0) || !std::isfinite(p->Re)) return fail(CFD_ERR_ARG, "Re must be positive and finite");
   if (p->inner_loops < 0 || p->inner_loops > CFD_MAX_PASSES - 1)
     return fail(CFD_ERR_ARG, "inner_loops must be in [0, 7]");
-  if (p->solver != CFD_SOLVER_BICGSTAB && p->solver != CFD_SOLVER_RBSOR) return fail(CFD_ERR_ARG, "unknown solver");
+  if (p->solver != CFD_SOLVER_BICGSTAB && p->solver != CFD_SOLVER_RBSOR && p->solver != CFD_SOLVER_SOR_PIPELINE)
+    return fail(CFD_ERR_ARG, "unknown solver");
+  if (p->solver == CFD_SOLVER_SOR_PIPELINE && p->precision != CFD_FP64)
+    return fail(CFD_ERR_ARG, "SorPipeLine runs in FP64");
+  if (p->solver == CFD_SOLVER_SOR_PIPELINE && !(p->sor_omega > 0.0 && p->sor_omega < 2.0))
+    return fail(CFD_ERR_ARG, "p_sor_omega must be in (0, 2)");
   if (p->op != CFD_OP_ELL && p->op != CFD_OP_SPE && p->op != CFD_OP_CSR) return fail(CFD_ERR_ARG, "unknown operator");
   if (p->precision != CFD_FP64 && p->precision != CFD_MIXED) return fail(CFD_ERR_ARG, "unknown precision");
-  if (p->solver == CFD_SOLVER_RBSOR && p->op != CFD_OP_ELL)
-    return fail(CFD_ERR_ARG, "red-black SOR runs on the ELL operator");
+  if (p->solver == CFD_SOLVER_RBSOR && p->op != CFD_OP_ELL && p->op != CFD_OP_CSR)
+    return fail(CFD_ERR_ARG, "red-black SOR runs on the ELL or the CSR operator");
+  if (p->solver == CFD_SOLVER_RBSOR && p->op == CFD_OP_CSR && p->precision != CFD_FP64)
+    return fail(CFD_ERR_ARG, "red-black SOR on the CSR operator runs in FP64");
   if (p->maxit < 0) return fail(CFD_ERR_ARG, "maxit must be >= 0");
   if (p->precision == CFD_MIXED && p->mixed_mode != CFD_MIXED_RELIABLE && p->mixed_mode != CFD_MIXED_RESTARTED &&
       p->mixed_mode != CFD_MIXED_VALUES32 && p->mixed_mode != CFD_MIXED_PC32)
@@ -193,6 +200,10 @@ int sync_p(Ctx &c) {
 int solve_configured(Ctx &c, int *iters, double *relres, int *inner) {
   const cfd_params &P = c.prm;
   *inner = 0;
+  if (P.solver == CFD_SOLVER_SOR_PIPELINE) {  // T1 and the ghosted p, in place
+    if (int rc = sync_p(c)) return rc;
+    return sor_pipeline_solve(c, P.sor_omega, P.tol, P.maxit, iters, relres);
+  }
   if (P.precond == CFD_PRECOND_MG && P.solver == CFD_SOLVER_BICGSTAB &&
       (P.precision == CFD_FP64 || P.mixed_mode == CFD_MIXED_PC32)) {
     // FP64 BiCGSTAB preconditioned by the V-cycle, the cycle in FP32 for CFD_MIXED_PC32
@@ -213,6 +224,8 @@ int solve_configured(Ctx &c, int *iters, double *relres, int *inner) {
     *iters = outer;
     return rc;
   }
+  if (P.solver == CFD_SOLVER_RBSOR && P.op == CFD_OP_CSR)
+    return rbsor_csr_solve(c, c.f[CFD_RHS], c.f[CFD_X], P.sor_omega, P.tol, P.maxit, P.sor_check, iters, relres);
   if (P.solver == CFD_SOLVER_RBSOR) {
     int rc = ensure_perm(c);
     if (rc) return rc;
@@ -709,9 +722,21 @@ int cfd_rbsor(cfd_ctx *ctx, double omega, double tol, int maxit, int check_every
   Ctx &c = *reinterpret_cast<Ctx *>(ctx);
   if (!iters || !relres || maxit < 0 || !(omega > 0.0 && omega < 2.0) || !(tol >= 0.0))
     return fail(CFD_ERR_ARG, "bad rbsor arguments");
+  if (c.prm.op == CFD_OP_CSR)
+    return rbsor_csr_solve(c, c.f[CFD_RHS], c.f[CFD_X], omega, tol, maxit, check_every, iters, relres);
   int rc = ensure_perm(c);
   if (rc) return rc;
   return rbsor_solve(c, c.f[CFD_RHS], c.f[CFD_X], omega, tol, maxit, check_every, iters, relres);
+}
+
+int cfd_sor_pipeline(cfd_ctx *ctx, double omega, double crit, int itmax, int *iters, double *error) {
+  CTX_CHECK(ctx);
+  Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  if (!iters || !error || itmax < 1 || !(omega > 0.0 && omega < 2.0) || !(crit >= 0.0))
+    return fail(CFD_ERR_ARG, "bad SorPipeLine arguments");
+  int rc = sync_p(c);
+  if (rc) return rc;
+  return sor_pipeline_solve(c, omega, crit, itmax, iters, error);
 }
 
 int cfd_defect_correction(cfd_ctx *ctx, double tol, double tol_inner, int max_outer, int *outer, int *inner_total,
@@ -822,6 +847,18 @@ int cfd_drag(cfd_ctx *ctx, double *cd, double *cl) {
   return CFD_OK;
 }
 
+int cfd_force_sum(cfd_ctx *ctx, int which_direction, double *sum) {
+  CTX_CHECK(ctx);
+  Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  if (!sum) return fail(CFD_ERR_ARG, "NULL output");
+  if (which_direction < 0 || which_direction > 2) return fail(CFD_ERR_ARG, "whichDirection must be 0, 1 or 2");
+  RET_LAUNCH(launch_force_sum(c, which_direction, c.red.out + RS_DRAG * 4 + 2));
+  CFD_HIP(hipMemcpyAsync(c.hred, c.red.out + RS_DRAG * 4 + 2, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  CFD_HIP(hipStreamSynchronize(c.stream));
+  *sum = c.hred[0];
+  return CFD_OK;
+}
+
 int cfd_maxdiff(cfd_ctx *ctx, double out[3]) {
   CTX_CHECK(ctx);
   Ctx &c = *reinterpret_cast<Ctx *>(ctx);
@@ -830,6 +867,23 @@ int cfd_maxdiff(cfd_ctx *ctx, double out[3]) {
   CFD_HIP(hipMemcpyAsync(c.hred, c.red.out + RS_MAX * 4, 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   CFD_HIP(hipStreamSynchronize(c.stream));
   for (int q = 0; q < 3; ++q) out[q] = c.hred[q];
+  return CFD_OK;
+}
+
+int cfd_divergence(cfd_ctx *ctx, int vel_field, double out[4]) {
+  CTX_CHECK(ctx);
+  Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  if (!out) return fail(CFD_ERR_ARG, "NULL output");
+  if (vel_field != CFD_T0_U && vel_field != CFD_T1_U && vel_field != CFD_T3_U)
+    return fail(CFD_ERR_ARG, "vel_field must be CFD_T0_U, CFD_T1_U or CFD_T3_U");
+  const double *vel[3] = {c.f[vel_field], c.f[vel_field + 1], c.f[vel_field + 2]};
+  RET_LAUNCH(launch_divergence(c, vel, vel, c.red.out + RS_MISC * 4));
+  CFD_HIP(hipMemcpyAsync(c.hred, c.red.out + RS_MISC * 4, 4 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  CFD_HIP(hipStreamSynchronize(c.stream));
+  out[0] = c.hred[0];
+  out[1] = std::sqrt(c.hred[2]);
+  out[2] = c.hred[1];
+  out[3] = std::sqrt(c.hred[3]);
   return CFD_OK;
 }
 
@@ -872,7 +926,7 @@ int op_ready(const Ctx &c, int op) {
 int step_impl(Ctx &c, cfd_step_stats *stats) {
   const cfd_params &P = c.prm;
   const bool solve = c.loop > 1;
-  if (solve) {
+  if (solve && P.solver != CFD_SOLVER_SOR_PIPELINE) {
     if (int rc = op_ready(c, P.op)) return rc;
   }
   cfd_step_stats st;
@@ -913,7 +967,8 @@ int step_impl(Ctx &c, cfd_step_stats *stats) {
     CFD_HIP(mark(CFD_PHASE_CONVECTION));
     st.iters[pass] = -1;
     if (solve) {
-      RET_LAUNCH(launch_rhs(c, u1, v1, w1, c.f[CFD_JP], c.f[CFD_RHS]));
+      const bool sor_p = P.solver == CFD_SOLVER_SOR_PIPELINE;  // in place on the ghosted p, its own RHS
+      if (!sor_p) RET_LAUNCH(launch_rhs(c, u1, v1, w1, c.f[CFD_JP], c.f[CFD_RHS]));
       int it = 0, inner = 0;
       double rel = 0.0;
       int rc = solve_configured(c, &it, &rel, &inner);
@@ -921,7 +976,7 @@ int step_impl(Ctx &c, cfd_step_stats *stats) {
       st.iters[pass] = it;
       st.relres[pass] = rel;
       st.inner_iters[pass] = inner;
-      c.p_from_x = true;
+      c.p_from_x = !sor_p;
     }
     CFD_HIP(mark(CFD_PHASE_PRESSURE));
     int mode = UPD_DRAG;
@@ -932,6 +987,12 @@ int step_impl(Ctx &c, cfd_step_stats *stats) {
     double *t2 = c.t2_rec ? c.t2_rec + (size_t)3 * c.g.N * pass : nullptr;
     RET_LAUNCH(
         launch_dfib_update_into(c, mode, pp, c.stepbuf + 2 * pass, corrector ? c.stepbuf + 32 : nullptr, t2));
+    if (corrector) {
+      // FP64 post-projection divergence check: the corrected velocity (now
+      // in T0) on the boundary faces the projection saw (T1's)
+      const double *vel[3] = {u0, v0, w0}, *bnd[3] = {u1, v1, w1};
+      RET_LAUNCH(launch_divergence(c, vel, bnd, c.stepbuf + 36));
+    }
     CFD_HIP(mark(CFD_PHASE_UPDATE));
   }
   RET_LAUNCH(launch_bc(c, true, true, !c.p_from_x));
@@ -951,6 +1012,10 @@ int step_impl(Ctx &c, cfd_step_stats *stats) {
     st.cl[pass] = -2.0 * c.hred[2 * pass + 1] / c.g.lz;
   }
   for (int q = 0; q < 3; ++q) st.maxdiff[q] = c.hred[32 + q];
+  st.div_max = c.hred[36];
+  st.div_fluid_max = c.hred[37];
+  st.div_l2 = std::sqrt(c.hred[38]);
+  st.div_fluid_l2 = std::sqrt(c.hred[39]);
   c.time += P.dt;  // simuClass::finishloop (simuClass.hpp:152-169)
   c.loop += 1;
   if (stats) *stats = st;

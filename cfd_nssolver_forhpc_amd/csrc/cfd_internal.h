@@ -197,6 +197,12 @@ hipError_t launch_source_accumulate(Ctx &c);
 hipError_t launch_bc(Ctx &c, bool main, bool copy, bool pressure);
 hipError_t launch_drag(Ctx &c);
 hipError_t launch_maxdiff(Ctx &c);
+// sum over the interior of f[dir] times the cell volume into out[0]
+hipError_t launch_force_sum(Ctx &c, int dir, double *out);
+// divergence check of a velocity triple whose boundary faces come from bnd:
+// out[0] max |div| over every interior cell, out[1] over the fluid cells,
+// out[2] / out[3] the sums of div^2
+hipError_t launch_divergence(Ctx &c, const double *const vel[3], const double *const bnd[3], double *out);
 hipError_t launch_copy(Ctx &c, const double *src, double *dst, int64_t n);
 hipError_t launch_fill(Ctx &c, double *dst, double v, int64_t n);
 // k_ell.hip
@@ -232,6 +238,13 @@ hipError_t launch_bicg_timing_setup(Ctx &c);
 int rbsor_solve(Ctx &c, const double *b, double *x, double omega, double tol, int maxit, int check_every,
                 int *iters, double *relres);
 hipError_t launch_rbsor_sweep(Ctx &c, double omega);
+// SorPipeLine: the reference's volume-weighted SOR on the ghosted pressure
+// (in place, cf6 from T1 into CFD_RHS), red-black order, max |dp| stop;
+// iters = sweeps - 1 and error = the last sweep's max |dp|, as the reference
+int sor_pipeline_solve(Ctx &c, double omega, double crit, int itmax, int *iters, double *error);
+// red-black SOR on the CSR operator (natural row order, colour by (i+j+k) parity)
+int rbsor_csr_solve(Ctx &c, const double *b, double *x, double omega, double tol, int maxit, int check_every,
+                    int *iters, double *relres);
 int rbsor_solve_f32(Ctx &c, const float *b, float *x, double omega, double tol, int maxit, int check_every,
                     int *iters, double *relres);
 hipError_t launch_copy_kernel_bench(Ctx &c, const double *src, double *dst, int64_t n);

@@ -597,6 +597,18 @@ __global__ void __launch_bounds__(256) drag_kernel(DevGrid g, const double *__re
   block_partials<2>(v, part);
 }
 
+// sum of one force component times the cell volume (virtualF_Int's sum for
+// one direction, virtualForceIntergrator.hpp:5-37)
+__global__ void __launch_bounds__(256) force_sum_kernel(DevGrid g, const double *__restrict__ fc, double *part) {
+  double v[1] = {0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < g.N; t += stride) {
+    const Cell c = decode_interior(g, t);
+    v[0] += fc[t] * (g.Dx[c.i] * g.Dy[c.j] * g.Dz[c.k]);
+  }
+  block_partials<1>(v, part);
+}
+
 __global__ void __launch_bounds__(256) maxdiff_kernel(DevGrid g, const double *__restrict__ u0,
                                                       const double *__restrict__ v0, const double *__restrict__ w0,
                                                       const double *__restrict__ u3, const double *__restrict__ v3,
@@ -611,6 +623,55 @@ __global__ void __launch_bounds__(256) maxdiff_kernel(DevGrid g, const double *_
     v[2] = fmax(v[2], w0[ic] - w3[ic]);
   }
   block_partials<3, true>(v, part);
+}
+
+// FP64 divergence of a staggered velocity over the interior cells, in the
+// finite-volume form of the pressure right-hand side (createBMatrix_omp,
+// inc/matrix/b_matrix.hpp:68-80, without its -jp/dt scaling):
+//   d = (u[ic] - u[i-1]) / Dx + (v[ic] - v[j-1]) / Dy + (w[ic] - w[k-1]) / Dz.
+// The boundary faces (index 1 and n-3 along each component's own axis) are
+// read from (ub, vb, wb), the others from (u, v, w): after the corrector's
+// update that is the projected velocity on the boundary faces the projection
+// saw, so d is what the pressure solve left.  Partials: max |d| and sum d^2
+// over every cell, and the same over the fluid cells -- eta zero at the
+// cell and its six neighbours, so that no face of the cell carries
+// immersed-boundary forcing, and no face on a Dirichlet pressure boundary,
+// whose velocity update_UandF leaves uncorrected (its index range stops
+// before the last face, inc/dfib/updateUandF.hpp:107-111).
+struct DivMask {
+  int pd[6];
+};
+
+__global__ void __launch_bounds__(256) divergence_kernel(DevGrid g, DivMask dm, const double *__restrict__ u,
+                                                         const double *__restrict__ v, const double *__restrict__ w,
+                                                         const double *__restrict__ ub, const double *__restrict__ vb,
+                                                         const double *__restrict__ wb,
+                                                         const double *__restrict__ eta, double *part_max,
+                                                         double *part_sum) {
+  double mx[2] = {0.0, 0.0}, sm[2] = {0.0, 0.0};
+  const int64_t sx = (int64_t)g.ny * g.nz, sy = g.nz;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < g.N; t += stride) {
+    const Cell c = decode_interior(g, t);
+    const int64_t ic = icel(g, c.i, c.j, c.k);
+    const double uP = (c.i == g.nx - 3 ? ub : u)[ic], uM = (c.i == GC ? ub : u)[ic - sx];
+    const double vP = (c.j == g.ny - 3 ? vb : v)[ic], vM = (c.j == GC ? vb : v)[ic - sy];
+    const double wP = (c.k == g.nz - 3 ? wb : w)[ic], wM = (c.k == GC ? wb : w)[ic - 1];
+    const double d = (uP - uM) / g.Dx[c.i] + (vP - vM) / g.Dy[c.j] + (wP - wM) / g.Dz[c.k];
+    const double a = fabs(d);
+    mx[0] = fmax(mx[0], a);
+    sm[0] += d * d;
+    const bool dir_face = (dm.pd[0] && c.i == GC) || (dm.pd[1] && c.i == g.nx - 3) || (dm.pd[2] && c.j == GC) ||
+                          (dm.pd[3] && c.j == g.ny - 3) || (dm.pd[4] && c.k == GC) || (dm.pd[5] && c.k == g.nz - 3);
+    const bool fluid = !dir_face && eta[ic] == 0.0 && eta[ic - sx] == 0.0 && eta[ic + sx] == 0.0 &&
+                       eta[ic - sy] == 0.0 && eta[ic + sy] == 0.0 && eta[ic - 1] == 0.0 && eta[ic + 1] == 0.0;
+    if (fluid) {
+      mx[1] = fmax(mx[1], a);
+      sm[1] += d * d;
+    }
+  }
+  block_partials<2, true>(mx, part_max);
+  block_partials<2>(sm, part_sum);
 }
 
 template <int NV, bool MAX>
@@ -778,6 +839,15 @@ hipError_t launch_drag(Ctx &c) {
   return hipGetLastError();
 }
 
+hipError_t launch_force_sum(Ctx &c, int dir, double *out) {
+  const int nblk = blocks_for(c.g.N, 256, RED_MAX_BLOCKS);
+  force_sum_kernel<<<nblk, 256, 0, c.stream>>>(c.g, c.f[CFD_F] + (int64_t)dir * c.g.N, red_part_host(c, RS_DRAG));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  finalize_kernel<1, false><<<1, FIN_THREADS, 0, c.stream>>>(red_part_host(c, RS_DRAG), nblk, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_maxdiff(Ctx &c) {
   const int nblk = blocks_for(c.g.N, 256, RED_MAX_BLOCKS);
   maxdiff_kernel<<<nblk, 256, 0, c.stream>>>(c.g, c.f[CFD_T0_U], c.f[CFD_T0_V], c.f[CFD_T0_W], c.f[CFD_T3_U],
@@ -785,6 +855,22 @@ hipError_t launch_maxdiff(Ctx &c) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   finalize_kernel<3, true><<<1, FIN_THREADS, 0, c.stream>>>(red_part_host(c, RS_MAX), nblk, c.red.out + RS_MAX * 4);
+  return hipGetLastError();
+}
+
+hipError_t launch_divergence(Ctx &c, const double *const vel[3], const double *const bnd[3], double *out) {
+  const int nblk = blocks_for(c.g.N, 256, RED_MAX_BLOCKS);
+  double *pmax = red_part_host(c, RS_MISC), *psum = red_part_host(c, RS_NORM);
+  DivMask dm;
+  for (int f = 0; f < 6; ++f) dm.pd[f] = c.prm.p_dirichlet[f];
+  divergence_kernel<<<nblk, 256, 0, c.stream>>>(c.g, dm, vel[0], vel[1], vel[2], bnd[0], bnd[1], bnd[2],
+                                                c.f[CFD_ETA], pmax, psum);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // out: max |d| (all), max |d| (fluid), sum d^2 (all), sum d^2 (fluid)
+  finalize_kernel<2, true><<<1, FIN_THREADS, 0, c.stream>>>(pmax, nblk, out);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  finalize_kernel<2, false><<<1, FIN_THREADS, 0, c.stream>>>(psum, nblk, out + 2);
   return hipGetLastError();
 }
 

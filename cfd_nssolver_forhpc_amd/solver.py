@@ -200,6 +200,16 @@ class Simulation:
         check(self.lib.cfd_bicgstab(self.h, tol, maxit, C.byref(it), C.byref(rel)), "cfd_bicgstab")
         return it.value, rel.value
 
+    def SorPipeLine(self, omega=None, crit=None, itmax=None):
+        """SorPipeLine_seq on the device's T1 and ghosted p (in place):
+        (simu.iters, simu.error) = (sweeps - 1, last sweep's max |dp|)."""
+        it, err = C.c_int(), C.c_double()
+        omega = self.params.sor_omega if omega is None else omega
+        crit = self.params.tol if crit is None else crit
+        itmax = self.params.maxit if itmax is None else itmax
+        check(self.lib.cfd_sor_pipeline(self.h, omega, crit, itmax, C.byref(it), C.byref(err)), "cfd_sor_pipeline")
+        return it.value, err.value
+
     def rbsor(self, omega=None, tol=None, maxit=None, check_every=None):
         it, rel = C.c_int(), C.c_double()
         omega = self.params.sor_omega if omega is None else omega
@@ -252,6 +262,14 @@ class Simulation:
         cd, cl = C.c_double(), C.c_double()
         check(self.lib.cfd_drag(self.h, C.byref(cd), C.byref(cl)), "cfd_drag")
         return cd.value, cl.value
+
+    def divergence(self, vel_field: int = capi.T0_U) -> dict:
+        """FP64 divergence check of a velocity triple (the pressure RHS's
+        finite-volume divergence): max and L2 over every interior cell and
+        over the fluid cells."""
+        out = (C.c_double * 4)()
+        check(self.lib.cfd_divergence(self.h, int(vel_field), out), "cfd_divergence")
+        return {"max": out[0], "l2": out[1], "fluid_max": out[2], "fluid_l2": out[3]}
 
     def write_plot3d_q(self, path, time: float, vel_field: int = capi.T3_U):
         """OutputPLOT3D_Qfile (inc/io/Plot3D/qfile.hpp:7-104) from the device

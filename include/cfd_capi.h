@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define CFD_ABI_VERSION 5
+#define CFD_ABI_VERSION 6
 
 /* status codes */
 #define CFD_OK 0
@@ -54,7 +54,10 @@ extern "C" {
 
 /* pressure solvers (inc/controlPanel.hpp:17-29) */
 #define CFD_SOLVER_BICGSTAB 0 /* solver::bicgstabRe2 */
-#define CFD_SOLVER_RBSOR 1    /* red-black SOR on the ELL operator */
+#define CFD_SOLVER_RBSOR 1    /* red-black SOR on the ELL (or CSR) operator, ||b - Ax|| / ||b|| stop */
+#define CFD_SOLVER_SOR_PIPELINE 2 /* SorPipeLine (P_SOLVER_SOR, SorPipeLine.seq.hpp:4-122): the volume-weighted
+                                     form on the ghosted pressure in place, stop on max |dp| <= tol
+                                     (SOR_ERROR_MAXVAL); red-black order; FP64 */
 
 /* pressure operators */
 #define CFD_OP_ELL 0 /* mat::ELL_matrix (P_SOLVER_BICG_ELL) */
@@ -153,6 +156,17 @@ typedef struct cfd_step_stats {
   double cl[CFD_MAX_PASSES];
   double maxdiff[3];              /* CheckSteadyStatebyMaxVal (CheckSteadystate.hpp:165-219) */
   int inner_iters[CFD_MAX_PASSES];/* CFD_MIXED: total FP32 inner iterations per pass */
+  /* FP64 post-projection divergence check of the corrector pass, in the
+     finite-volume form of the pressure right-hand side (b_matrix.hpp:68-80):
+     the corrected velocity on the interior faces and the predicted velocity
+     T1 on the boundary faces (index 1 and n-3 along each component's axis,
+     the faces the projection saw).  max |div u| and sqrt(sum (div u)^2)
+     over every interior cell, and the same over the fluid cells (eta zero
+     at the cell and its six neighbours: no face carries immersed-boundary
+     forcing, so div u is what the pressure solve left).  0 in a step
+     without a pressure solve is not implied: loop 1 reports the predicted
+     field's divergence after the forcing. */
+  double div_max, div_l2, div_fluid_max, div_fluid_l2;
 } cfd_step_stats;
 
 typedef struct cfd_ctx cfd_ctx;
@@ -227,9 +241,19 @@ int cfd_spmv(cfd_ctx *ctx, int op, int precision, int x_field, int y_field);
 /* ---- pressure solves (matB -> X_result, warm start from X_result) ------ */
 /* solver::bicgstabRe2::solve (inc/solver/bicgstabRe2.hpp:91-189), FP64 */
 int cfd_bicgstab(cfd_ctx *ctx, double tol, int maxit, int *iters, double *relres);
-/* red-black SOR on the ELL operator to ||b-Ax||/||b|| < tol, checked every
-   check_every sweeps; iters = sweeps taken */
+/* red-black SOR on the ELL operator (on the CSR operator when params.op is
+   CFD_OP_CSR) to ||b-Ax||/||b|| < tol, checked every check_every sweeps;
+   iters = sweeps taken */
 int cfd_rbsor(cfd_ctx *ctx, double omega, double tol, int maxit, int check_every, int *iters, double *relres);
+/* SorPipeLine_seq (inc/pressure/solver/SorPipeLine.seq.hpp:4-122) on the
+   context's predicted velocity T1 and ghosted pressure p (in place):
+   cf6 = mChange / dt of T1 (written to CFD_RHS), sweeps of the
+   volume-weighted update p += omega (pNEW - p) in red-black order until
+   max |pNEW - p| over a sweep is at most crit or itmax sweeps ran; iters =
+   sweeps - 1 (simu.iters), error = the last sweep's max |pNEW - p|
+   (simu.error).  Boundary neighbours read the ghost pressure as the last
+   BC left it. */
+int cfd_sor_pipeline(cfd_ctx *ctx, double omega, double crit, int itmax, int *iters, double *error);
 /* FP64 defect correction around FP32 BiCGSTAB (or RB-SOR) inner solves */
 int cfd_defect_correction(cfd_ctx *ctx, double tol, double tol_inner, int max_outer,
                           int *outer, int *inner_total, double *relres);
@@ -255,10 +279,19 @@ int cfd_bc_main(cfd_ctx *ctx);
 int cfd_bc_copy(cfd_ctx *ctx);
 /* virtualF_Int x and y (virtualForceIntergrator.hpp:5-37): cD, cL of f */
 int cfd_drag(cfd_ctx *ctx, double *cd, double *cl);
+/* virtualF_Int (virtualForceIntergrator.hpp:5-37) for one direction: the sum
+   over the interior of f[which_direction] times the cell volume (the value
+   the reference stores in DfibArray::ValSum[whichDirection]) */
+int cfd_force_sum(cfd_ctx *ctx, int which_direction, double *sum);
 /* CheckSteadyStatebyMaxVal_omp (CheckSteadystate.hpp:165-219): max(T0 - T3) per component */
 int cfd_maxdiff(cfd_ctx *ctx, double out[3]);
 /* velocity::copy_omp T0 <- T3 (physicalVariables.hpp:84-102) */
 int cfd_copy_t3_to_t0(cfd_ctx *ctx);
+/* FP64 divergence of the velocity triple starting at vel_field (CFD_T0_U,
+   CFD_T1_U or CFD_T3_U), every face from that triple, in the form of
+   cfd_step_stats: out[0] max |div u|, out[1] sqrt(sum (div u)^2) over every
+   interior cell, out[2] and out[3] the same over the fluid cells */
+int cfd_divergence(cfd_ctx *ctx, int vel_field, double out[4]);
 
 /* ---- whole timestep --------------------------------------------------- */
 /* one iteration of runSeqOmp_predictionMethod's main loop

@@ -1046,6 +1046,105 @@ double orc_rbsor(size_t N, const cfd_grid_desc *g, const int32_t *col, const dou
   return rel;
 }
 
+/* red-black SOR on the CSR operator (no reference counterpart): the
+   colouring, update and residual check of orc_rbsor, with each row's
+   products summed in stored order (CSR_matrix::multiply_omp,
+   CSR_sparseMatrix.hpp:278-292) and the diagonal the entry whose column is
+   the row */
+double orc_rbsor_csr(size_t N, const cfd_grid_desc *g, const int32_t *ptr, const int32_t *col, const double *val,
+                     const double *b, double *x, double omega, double tol, int maxit, int check_every, int *iters) {
+  const dims_t d = dims_of(g);
+  double *r = (double *)malloc(sizeof(double) * (N ? N : 1));
+  const double nb = sqrt(orc_dot(N, b, b));
+  double rel = INFINITY;
+  int it = 0;
+  if (check_every < 1) check_every = 1;
+  while (it < maxit) {
+    for (int colour = 0; colour < 2; ++colour) {
+#pragma omp parallel for schedule(static)
+      for (int ic = 0; ic < d.nxc; ++ic)
+        for (int jc = 0; jc < d.nyc; ++jc)
+          for (int kc = ((ic + jc + colour) & 1); kc < d.nzc; kc += 2) {
+            const size_t row = ((size_t)ic * d.nyc + jc) * d.nzc + kc;
+            double acc = 0.0, diag = 0.0;
+            for (int32_t e = ptr[row]; e < ptr[row + 1]; ++e) {
+              acc = fma(val[e], x[col[e]], acc);
+              if ((size_t)col[e] == row) diag = val[e];
+            }
+            x[row] = x[row] + omega * (b[row] - acc) / diag;
+          }
+    }
+    ++it;
+    if (it % check_every == 0 || it == maxit) {
+      orc_spmv_csr(N, ptr, col, val, x, r);
+#pragma omp parallel for schedule(static)
+      for (long long q = 0; q < (long long)N; ++q) r[q] = b[q] - r[q];
+      rel = sqrt(orc_dot(N, r, r)) / nb;
+      if (rel < tol) break;
+    }
+  }
+  free(r);
+  *iters = it;
+  return rel;
+}
+
+/* SorPipeLine_seq's volume-weighted update (SorPipeLine.seq.hpp:26-112) in
+   red-black order: the cells with (i + j + k) even, then odd (interior
+   indices), each reading the other colour; cf0..cf7 formed per cell in the
+   reference's operation order; the same max |pNEW - p| stop and
+   iters = sweeps - 1.  Converges to the lexicographic sweep's fixed point. */
+double orc_sor_rb_vw(const cfd_grid_desc *g, double dt, double omega, double crit, int itmax,
+                     const double *u1, const double *v1, const double *w1, double *p, int *iters) {
+  const dims_t d = dims_of(g);
+  const size_t N = d.N;
+  double *cf6 = (double *)malloc(sizeof(double) * (N ? N : 1));
+  const double rdt = 1.0 / dt;
+  const double *Dx = g->Dx, *Dy = g->Dy, *Dz = g->Dz, *Dxs = g->Dxs, *Dys = g->Dys, *Dzs = g->Dzs;
+  for (int i = GC; i < d.nx - GC; ++i)
+    for (int j = GC; j < d.ny - GC; ++j)
+      for (int k = GC; k < d.nz - GC; ++k) {
+        const size_t icel = IC(d, i, j, k);
+        const double mChange = (u1[icel] - u1[IC(d, i - 1, j, k)]) * Dy[j] * Dz[k] +
+                               (v1[icel] - v1[IC(d, i, j - 1, k)]) * Dx[i] * Dz[k] +
+                               (w1[icel] - w1[IC(d, i, j, k - 1)]) * Dx[i] * Dy[j];
+        cf6[ICAL(d, i, j, k)] = mChange * rdt;
+      }
+  int it = 0;
+  double residual;
+  do {
+    residual = 0.0;
+    for (int colour = 0; colour < 2; ++colour) {
+      double rmax = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : rmax)
+      for (int ic = 0; ic < d.nxc; ++ic)
+        for (int jc = 0; jc < d.nyc; ++jc)
+          for (int kc = ((ic + jc + colour) & 1); kc < d.nzc; kc += 2) {
+            const int i = ic + GC, j = jc + GC, k = kc + GC;
+            const size_t icel = IC(d, i, j, k);
+            const double cf0 = Dy[j] * Dz[k] / Dxs[i], cf1 = Dy[j] * Dz[k] / Dxs[i - 1];
+            const double cf2 = Dx[i] * Dz[k] / Dys[j], cf3 = Dx[i] * Dz[k] / Dys[j - 1];
+            const double cf4 = Dx[i] * Dy[j] / Dzs[k], cf5 = Dx[i] * Dy[j] / Dzs[k - 1];
+            const double coef = Dy[j] * Dz[k] / Dxs[i] + Dy[j] * Dz[k] / Dxs[i - 1] + Dx[i] * Dz[k] / Dys[j] +
+                                Dx[i] * Dz[k] / Dys[j - 1] + Dx[i] * Dy[j] / Dzs[k] + Dx[i] * Dy[j] / Dzs[k - 1];
+            const double cf7 = -1.0 / coef;
+            const double pc = p[icel];
+            const double pNEW = (-p[IC(d, i + 1, j, k)] * cf0 - p[IC(d, i - 1, j, k)] * cf1 -
+                                 p[IC(d, i, j + 1, k)] * cf2 - p[IC(d, i, j - 1, k)] * cf3 -
+                                 p[IC(d, i, j, k + 1)] * cf4 - p[IC(d, i, j, k - 1)] * cf5 +
+                                 cf6[((size_t)ic * d.nyc + jc) * d.nzc + kc]) * cf7;
+            const double ch = fabs(pNEW - pc);
+            if (ch > rmax) rmax = ch;
+            p[icel] = pc + omega * (pNEW - pc);
+          }
+      if (rmax > residual) residual = rmax;
+    }
+    ++it;
+  } while (residual > crit && it < itmax);
+  free(cf6);
+  *iters = it - 1;
+  return residual;
+}
+
 /* SorPipeLine_seq (inc/pressure/solver/SorPipeLine.seq.hpp:4-122) with its
    coefficients computed on every call (the reference computes them only when
    simuClass::loop == 1, a loop in which the driver never calls it) */
@@ -1555,13 +1654,18 @@ static void sim_pass(orc_sim *s, int pass, int accumulate, cfd_step_stats *st) {
   int it = -1;
   double rel = 0.0;
   if (s->loop > 1) {
-    orc_rhs(g, P->dt, s->jp, s->T1[0], s->T1[1], s->T1[2], s->b);
-    if (P->solver == CFD_SOLVER_RBSOR)
+    if (P->solver != CFD_SOLVER_SOR_PIPELINE) orc_rhs(g, P->dt, s->jp, s->T1[0], s->T1[1], s->T1[2], s->b);
+    if (P->solver == CFD_SOLVER_SOR_PIPELINE) /* SorPipeLine: in place on the ghosted p, no X */
+      rel = orc_sor_rb_vw(g, P->dt, P->sor_omega, P->tol, P->maxit, s->T1[0], s->T1[1], s->T1[2], s->p, &it);
+    else if (P->solver == CFD_SOLVER_RBSOR && P->op == CFD_OP_CSR)
+      rel = orc_rbsor_csr(s->d.N, g, s->csr_ptr, s->csr_col, s->csr_val, s->b, s->X, P->sor_omega, P->tol,
+                          P->maxit, P->sor_check, &it);
+    else if (P->solver == CFD_SOLVER_RBSOR)
       rel = orc_rbsor(s->d.N, g, s->col, s->val, s->b, s->X, P->sor_omega, P->tol, P->maxit,
                       P->sor_check, &it);
     else
       rel = orc_bicg_solve(s->bicg, s->b, s->X, &it);
-    orc_pressure_scatter(g, s->X, s->p);
+    if (P->solver != CFD_SOLVER_SOR_PIPELINE) orc_pressure_scatter(g, s->X, s->p);
   }
   orc_dfib_update_ex(g, P->dt, s->gain, s->eta, s->p, s->T1[0], s->T1[1], s->T1[2], s->T3[0], s->T3[1], s->T3[2],
                      s->f, s->t2_rec ? s->t2_rec + (size_t)pass * 3 * s->d.N : NULL);
@@ -1573,6 +1677,44 @@ static void sim_pass(orc_sim *s, int pass, int accumulate, cfd_step_stats *st) {
   }
 }
 
+/* FP64 divergence check (no reference counterpart; the divergence is the
+   finite-volume form of createBMatrix_omp, inc/matrix/b_matrix.hpp:68-80,
+   without its -jp/dt scaling).  Boundary faces (index 1 and n-3 along each
+   component's own axis) come from (ub, vb, wb), the others from (u, v, w).
+   out[0] max |d| and out[1] sqrt(sum d^2) over the interior cells, out[2] /
+   out[3] the same over the fluid cells: eta zero at the cell and its six
+   neighbours, and no face on a Dirichlet pressure boundary (pd), whose
+   velocity update_UandF leaves uncorrected (inc/dfib/updateUandF.hpp:107-111) */
+void orc_divergence(const cfd_grid_desc *g, const int pd[6], const double *u, const double *v, const double *w,
+                    const double *ub, const double *vb, const double *wb, const double *eta, double out[4]) {
+  const dims_t d = dims_of(g);
+  const size_t sx = (size_t)d.ny * d.nz, sy = (size_t)d.nz;
+  double mx[2] = {0.0, 0.0}, sm[2] = {0.0, 0.0};
+  for (int i = GC; i < d.nx - GC; ++i)
+    for (int j = GC; j < d.ny - GC; ++j)
+      for (int k = GC; k < d.nz - GC; ++k) {
+        const size_t ic = IC(d, i, j, k);
+        const double uP = (i == d.nx - 3 ? ub : u)[ic], uM = (i == GC ? ub : u)[ic - sx];
+        const double vP = (j == d.ny - 3 ? vb : v)[ic], vM = (j == GC ? vb : v)[ic - sy];
+        const double wP = (k == d.nz - 3 ? wb : w)[ic], wM = (k == GC ? wb : w)[ic - 1];
+        const double dv = (uP - uM) / g->Dx[i] + (vP - vM) / g->Dy[j] + (wP - wM) / g->Dz[k];
+        const double a = fabs(dv);
+        if (a > mx[0]) mx[0] = a;
+        sm[0] += dv * dv;
+        const int dir_face = (pd[0] && i == GC) || (pd[1] && i == d.nx - 3) || (pd[2] && j == GC) ||
+                             (pd[3] && j == d.ny - 3) || (pd[4] && k == GC) || (pd[5] && k == d.nz - 3);
+        if (!dir_face && eta[ic] == 0.0 && eta[ic - sx] == 0.0 && eta[ic + sx] == 0.0 && eta[ic - sy] == 0.0 &&
+            eta[ic + sy] == 0.0 && eta[ic - 1] == 0.0 && eta[ic + 1] == 0.0) {
+          if (a > mx[1]) mx[1] = a;
+          sm[1] += dv * dv;
+        }
+      }
+  out[0] = mx[0];
+  out[1] = sqrt(sm[0]);
+  out[2] = mx[1];
+  out[3] = sqrt(sm[1]);
+}
+
 int orc_sim_step(orc_sim *s, cfd_step_stats *st) {
   const cfd_params *P = &s->prm;
   const cfd_grid_desc *g = &s->g;
@@ -1581,6 +1723,14 @@ int orc_sim_step(orc_sim *s, cfd_step_stats *st) {
   memset(s->src, 0, sizeof(double) * 3 * d.N);
   for (int pass = 0; pass < P->inner_loops; ++pass) sim_pass(s, pass, 1, st);
   sim_pass(s, P->inner_loops, 0, st);
+  if (st) {  /* the corrected velocity on the boundary faces the projection saw */
+    double dv[4];
+    orc_divergence(g, P->p_dirichlet, s->T3[0], s->T3[1], s->T3[2], s->T1[0], s->T1[1], s->T1[2], s->eta, dv);
+    st->div_max = dv[0];
+    st->div_l2 = dv[1];
+    st->div_fluid_max = dv[2];
+    st->div_fluid_l2 = dv[3];
+  }
   double md[3];
   orc_maxdiff(g, s->T0[0], s->T0[1], s->T0[2], s->T3[0], s->T3[1], s->T3[2], md);
   if (st) memcpy(st->maxdiff, md, sizeof(md));

@@ -118,8 +118,22 @@ double orc_rbsor(size_t N, const cfd_grid_desc *g, const int32_t *col, const dou
 /* SorPipeLine_seq (inc/pressure/solver/SorPipeLine.seq.hpp:4-122): lexicographic
    SOR on the ghosted p with the volume-weighted coefficients, stops on
    max|dp| < crit; returns the final max|dp| and the reference's iteration count */
+double orc_rbsor_csr(size_t N, const cfd_grid_desc *g, const int32_t *ptr, const int32_t *col, const double *val,
+                     const double *b, double *x, double omega, double tol, int maxit, int check_every, int *iters);
+/* SorPipeLine_seq's volume-weighted update in red-black order (the GPU's
+   CFD_SOLVER_SOR_PIPELINE): in place on the ghosted p, max |dp| stop,
+   iters = sweeps - 1; returns the last sweep's max |dp| */
+double orc_sor_rb_vw(const cfd_grid_desc *g, double dt, double omega, double crit, int itmax,
+                     const double *u1, const double *v1, const double *w1, double *p, int *iters);
 double orc_sor_lex(const cfd_grid_desc *g, double dt, double omega, double crit, int itmax,
                    const double *u1, const double *v1, const double *w1, double *p, int *iters);
+
+/* FP64 divergence check of a velocity triple, boundary faces from (ub, vb,
+   wb): out[0] max |div|, out[1] sqrt(sum div^2) over the interior cells,
+   out[2..3] over the fluid cells (no IB forcing on a face, no face on a
+   Dirichlet pressure boundary pd) */
+void orc_divergence(const cfd_grid_desc *g, const int pd[6], const double *u, const double *v, const double *w,
+                    const double *ub, const double *vb, const double *wb, const double *eta, double out[4]);
 
 /* ---- the timestep loop (RUN_CPU_PC_METHOD.hpp:287-627 / RUN_CPU.hpp:290-460) */
 typedef struct orc_sim orc_sim;

@@ -41,6 +41,7 @@ int main() {
   prm.ib_cy = 2.0;
   prm.ib_r = 0.5;
 
+  bool vf_ok = true;
   // ---- reference-shaped driver, host vectors ----
   grid gA(32, 16, 8, 8.0, 4.0, 2.0, prm);
   simuClass simu;
@@ -59,10 +60,10 @@ int main() {
   T3 = T0;
   BC_staggered_main(dom, T0, t1, gA);
   BC_staggered_copy(dom, T0, T1, gA);
+  shareMenory ShareM;
   Mx.X_result.assign(gA.iceltotCal, 0.0);
   createPressureMatrix(Mx, simu, dom, gA);
-  mat::ELL_matrix A(gA);
-  solver::bicgstabRe2<mat::ELL_matrix> pSolver(A);
+  solver::bicgstabRe2<ELL_type> pSolver(Mx.matA_ell);  // RUN_CPU_PC_METHOD.hpp:137
   pSolver.setTolerance(simu.p_criteria);
   DFIB_CylinderZ(Dfib, dom, gA);
   auto source_termF = Dfib.f;
@@ -86,12 +87,18 @@ int main() {
       if (pass < 3)
         for (size_t m = 0; m < 3 * N; ++m) source_termF[m] += simu.dt * Dfib.f[m];
     }
-    CheckSteadyStatebyMaxVal_omp(simu, T0, T3, dom, gA);
+    auto [uDif, vDif, wDif] = CheckSteadyStatebyMaxVal_omp(simu, ShareM, T0, T3, dom, gA);  // :581
+    (void)vDif; (void)wDif;
     T0 = T3;
     BC_staggered_main(dom, T0, t1, gA);
     BC_staggered_copy(dom, T0, T1, gA);
-    auto cdl = virtualF_Int(Dfib, dom, gA);
-    std::printf("loop %zu iters %d err %.3e cD %.6f\n", simu.loop, simu.iters, simu.error, cdl.first);
+    virtualF_Int(Dfib, dom, gA, 0);
+    virtualF_Int(Dfib, dom, gA, 1);
+    const double cD = -2.0 * Dfib.ValSum[0] / gA.lz;
+    const auto cdl = drag_lift(Dfib, gA);
+    vf_ok = vf_ok && std::fabs(cD - cdl.first) <= 1e-12 * std::fabs(cdl.first) &&
+            std::fabs(-2.0 * Dfib.ValSum[1] / gA.lz - cdl.second) <= 1e-12 * std::fabs(cdl.second) + 1e-300;
+    std::printf("loop %zu iters %d err %.3e cD %.6f maxdiff u %.3e\n", simu.loop, simu.iters, simu.error, cD, uDif);
   }
 
   // ---- device-resident fused timestep ----
@@ -171,6 +178,155 @@ int main() {
     io_ok = io_ok && lines == 3 + 4;  // header, then two flushes of two lines
   }
   std::printf("output files: %s\n", io_ok ? "ok" : "FAILED");
+  // ---- P_SOLVER_SOR: SorPipeLine_omp in the loop (RUN_CPU_PC_METHOD.hpp:333)
+  //      against the fused device step with the same solver ----
+  bool sor_ok = true;
+  {
+    cfd_params ps = prm;
+    ps.solver = CFD_SOLVER_SOR_PIPELINE;
+    ps.tol = 1e-11;
+    ps.maxit = 200000;
+    grid gS(32, 16, 8, 8.0, 4.0, 2.0, ps);
+    simuClass su;
+    su.set_Re(ps.Re);
+    su.set_time_max(3 * ps.dt);
+    su.set_dt(ps.dt);
+    su.p_criteria = ps.tol;
+    su.p_sor_iter_max = 200000;
+    SORcoefficient Sor;
+    DfibArray Df;
+    pressure ts;
+    velocity S0, S1, S3;
+    resize_variable(gS, ts, S0, S1, S3, Df);
+    std::fill(S0.u.begin(), S0.u.end(), 1.0);
+    S1 = S0;
+    S3 = S0;
+    calDomain ds = gS.domain();
+    BC_staggered_main(ds, S0, ts, gS);
+    BC_staggered_copy(ds, S0, S1, gS);
+    DFIB_CylinderZ(Df, ds, gS);
+    auto src = Df.f;
+    std::vector<int> sweeps;
+    for (su.loop = 1; su.get_finishloop(); su.finishloop()) {
+      std::fill(src.begin(), src.end(), 0.0);
+      for (int pass = 0; pass < 4; ++pass) {
+        ConvectionDifussion(su, S0, S1, ds, gS);
+        if (pass > 0)
+          for (int c = 0; c < 3; ++c)
+            for (int i = ds.i_begin; i < ds.i_endof; ++i)
+              for (int j = ds.j_begin; j < ds.j_endof; ++j)
+                for (int k = ds.k_begin; k < ds.k_endof; ++k)
+                  S1.U(c)[gS.icel(i, j, k)] += src[gS.icelCal(i, j, k) + N * c];
+        if (su.loop > 1) {
+          SorPipeLine_omp(Sor, su, S1, ts, ds, gS);
+          sweeps.push_back(su.iters);
+          sor_ok = sor_ok && su.error <= su.p_criteria;
+        }
+        update_UandF_omp(Df, su, ts, S1, S3, ds, gS);
+        if (pass < 3)
+          for (size_t m = 0; m < 3 * N; ++m) src[m] += su.dt * Df.f[m];
+      }
+      S0 = S3;
+      BC_staggered_main(ds, S0, ts, gS);
+      BC_staggered_copy(ds, S0, S1, gS);
+    }
+    ps.source_index_quirk = 0;
+    grid gF(32, 16, 8, 8.0, 4.0, 2.0, ps);
+    check(cfd_init_state(gF.ctx(), 0.0, 0), "cfd_init_state");
+    check(cfd_set_eta_cylinder_z(gF.ctx()), "cfd_set_eta");
+    std::vector<int> fused;
+    for (int s = 0; s < 3; ++s) {
+      cfd_step_stats st = gF.step();
+      for (int q = 0; q < st.n_passes; ++q)
+        if (st.iters[q] >= 0) fused.push_back(st.iters[q]);
+    }
+    velocity F;
+    gF.get(CFD_T0_U, F);
+    pressure fp;
+    gF.get(CFD_P, fp.p);
+    const double e_u = rel_l2(F.u, S0.u), e_p = rel_l2(fp.p, ts.p);
+    // the host-side source accumulation may round differently from the
+    // device's (contracted) one, so a pass may stop one sweep apart
+    bool close = sweeps.size() == fused.size() && !sweeps.empty();
+    for (size_t q = 0; close && q < sweeps.size(); ++q) close = std::abs(sweeps[q] - fused[q]) <= 1;
+    sor_ok = sor_ok && e_u < 1e-10 && e_p < 1e-8 && close;
+    std::printf("SorPipeLine loop vs fused SOR step: u %.2e p %.2e, %zu passes, sweeps %d..%d %s\n", e_u, e_p,
+                sweeps.size(), sweeps.empty() ? 0 : sweeps.front(), sweeps.empty() ? 0 : sweeps.back(),
+                close ? "within 1" : "DIFFER");
+  }
+
+  // ---- mat::ELL_matrix<double>: the reference container on the device ----
+  bool ell_ok = true;
+  {
+    cfd_params pe = prm;
+    grid gE(10, 5, 4, 1.0, 1.0, 1.0, pe);  // 200 interior cells
+    const int n = 200;
+    ELL_type E(n, 3);
+    for (int r = 0; r < n; ++r) {
+      if (r > 0) E.set(r, r - 1, -1.0);
+      E.set(r, r, 1.0);  // overwritten below
+      if (r < n - 1) E.set(r, r + 1, -1.0);
+      E.set(r, r, 2.0 + 0.01 * (r % 7));
+    }
+    E.set(5, 4, 0.0);  // removes the entry and shifts the row
+    bool threw = false;
+    try {
+      E.set(7, 9, 1.0);  // a fourth entry in a row of three
+    } catch (const std::invalid_argument &) {
+      threw = true;
+    }
+    E.bind(gE);
+    std::vector<double> xe(n), ye, yh(n);
+    for (int r = 0; r < n; ++r) xe[r] = std::cos(0.11 * r);
+    E.multiply_omp(xe, ye);
+    auto csr = E.get_CSR();
+    const auto &cp = std::get<0>(csr);
+    const auto &ci = std::get<1>(csr);
+    const auto &cv = std::get<2>(csr);
+    // host product in each row's insertion order: the device's FMA chain
+    for (int r = 0; r < n; ++r) {
+      double acc = 0.0;
+      std::vector<std::pair<int, double>> row;
+      if (r > 0 && r != 5) row.push_back({r - 1, -1.0});
+      row.push_back({r, 2.0 + 0.01 * (r % 7)});
+      if (r < n - 1) row.push_back({r + 1, -1.0});
+      for (auto &q : row) acc = std::fma(q.second, xe[q.first], acc);
+      for (size_t s = row.size(); s < 3; ++s) acc = std::fma(0.0, xe[n - 1], acc);
+      yh[r] = acc;
+    }
+    ell_ok = threw && ye == yh && cp.back() == 3 * n - 3 && ci[cp[5]] == 5 && cv[cp[5]] == 2.0 + 0.01 * 5;
+    // the solver keeps its own copy: after the device received another
+    // operator, its solve uploads this one again
+    solver::bicgstabRe2<ELL_type> sE(E);
+    sE.setTolerance(1e-12);
+    gE.prm.op = CFD_OP_ELL;
+    check(cfd_ell_assemble(gE.ctx()), "cfd_ell_assemble");  // the device now holds the grid's operator
+    gE.set_ell_token(0);
+    std::vector<double> bE = yh, xE(n, 0.0), rE;
+    auto [itE, relE] = sE(bE, xE);
+    E.multiply_omp(xE, rE);
+    double err = 0.0, nb = 0.0;
+    for (int r = 0; r < n; ++r) {
+      err += (rE[r] - bE[r]) * (rE[r] - bE[r]);
+      nb += bE[r] * bE[r];
+    }
+    ell_ok = ell_ok && relE < 1e-12 && std::sqrt(err / nb) < 1e-11 && itE > 0;
+    // the device-assembled pressure operator through the host container
+    auto pc = Mx.matA_ell.get_CSR();
+    std::vector<int32_t> dc(7 * N);
+    std::vector<double> dv(7 * N);
+    check(cfd_ell_download(gA.ctx(), dc.data(), dv.data(), N, 7), "cfd_ell_download");
+    size_t nnz = 0;
+    for (size_t q = 0; q < 7 * N; ++q) nnz += !(dc[q] == (int32_t)N - 1 && dv[q] == 0.0);
+    bool sorted = true;
+    for (size_t r = 0; r < N; ++r)
+      for (int e = std::get<0>(pc)[r] + 1; e < std::get<0>(pc)[r + 1]; ++e)
+        sorted = sorted && std::get<1>(pc)[e - 1] < std::get<1>(pc)[e];
+    ell_ok = ell_ok && (size_t)std::get<0>(pc).back() == nnz && sorted && Mx.matA_ell.at(0, 0) == dv[0];
+    std::printf("ELL_matrix<double>: set / remove / overflow %s, product %s, solve relres %.1e, get_CSR nnz %zu\n",
+                threw ? "ok" : "FAILED", ye == yh ? "exact" : "DIFFERS", relE, nnz);
+  }
+
   // an out-of-range argument surfaces as the reference's exception type
   bool threw = false;
   try {
@@ -178,7 +334,9 @@ int main() {
   } catch (const std::invalid_argument &) {
     threw = true;
   }
-  const bool ok = eu < 1e-11 && ev < 1e-11 && ew < 1e-11 && ex < 1e-8 && threw && csr_ok && op_threw && io_ok;
+  std::printf("virtualF_Int per direction vs cfd_drag: %s\n", vf_ok ? "ok" : "FAILED");
+  const bool ok = eu < 1e-11 && ev < 1e-11 && ew < 1e-11 && ex < 1e-8 && threw && csr_ok && op_threw && io_ok &&
+                  vf_ok && sor_ok && ell_ok;
   std::printf("%s\n", ok ? "PASS" : "FAIL");
   return ok ? 0 : 1;
 }

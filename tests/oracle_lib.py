@@ -56,6 +56,7 @@ def _declare(L):
     L.orc_source_add.argtypes = [G, C.c_int, dp, dp, dp, dp]
     L.orc_source_accumulate.argtypes = [G, C.c_double, dp, dp]
     L.orc_rhs.argtypes = [G, C.c_double, dp, dp, dp, dp, dp]
+    L.orc_divergence.argtypes = [G, C.POINTER(C.c_int), dp, dp, dp, dp, dp, dp, dp, dp]
     L.orc_ell_assemble.argtypes = [G, C.POINTER(C.c_int), ip, dp, dp]
     L.orc_ell_assemble.restype = C.c_int
     L.orc_spmv_ell.argtypes = [C.c_size_t, C.c_int, ip, dp, dp, dp]
@@ -90,6 +91,12 @@ def _declare(L):
     L.orc_rbsor.argtypes = [C.c_size_t, G, ip, dp, dp, dp, C.c_double, C.c_double, C.c_int, C.c_int,
                             C.POINTER(C.c_int)]
     L.orc_rbsor.restype = C.c_double
+    L.orc_rbsor_csr.argtypes = [C.c_size_t, G, ip, ip, dp, dp, dp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                C.POINTER(C.c_int)]
+    L.orc_rbsor_csr.restype = C.c_double
+    L.orc_sor_rb_vw.argtypes = [G, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp,
+                                C.POINTER(C.c_int)]
+    L.orc_sor_rb_vw.restype = C.c_double
     L.orc_sor_lex.argtypes = [G, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp,
                               C.POINTER(C.c_int)]
     L.orc_sor_lex.restype = C.c_double

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_defaults():
     L = capi.load()
-    assert L.cfd_abi_version() == 5
+    assert L.cfd_abi_version() == 6
     p = default_params()
     # the reference's Jacobi-scaled solve unless multigrid is asked for
     assert p.precond == capi.CFD_PRECOND_JACOBI and p.mg_nu == 2

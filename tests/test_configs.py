@@ -38,7 +38,8 @@ def test_reference_main_settings():
 @pytest.mark.parametrize("opts, field, want", [
     ({"P_SOLVER": "P_SOLVER_BICG_ELL"}, "op", capi.CFD_OP_ELL),
     ({"P_SOLVER": "P_SOLVER_BICG_CSR"}, "op", capi.CFD_OP_CSR),
-    ({"P_SOLVER": "P_SOLVER_SOR"}, "solver", capi.CFD_SOLVER_RBSOR),
+    ({"P_SOLVER": "P_SOLVER_SOR"}, "solver", capi.CFD_SOLVER_SOR_PIPELINE),
+    ({"P_SOLVER": "RBSOR_CSR"}, "op", capi.CFD_OP_CSR),
     ({"CONVECTION_DIFUSSION": "CONVECTION_DIFUSSION_LUD"}, "convection", capi.CFD_CONV_LUD),
     ({"CONVECTION_DIFUSSION": "CONVECTION_DIFUSSION_UD"}, "convection", capi.CFD_CONV_UD),
     ({"TERBULENCE_SMAGORINSKY": True}, "les", 1),
