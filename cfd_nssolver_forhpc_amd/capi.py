@@ -46,6 +46,7 @@ CFD_CONV_UD = 2
 CFD_CDCL_REFERENCE = 0
 CFD_CDCL_BUFFERED = 1
 CFD_CDCL_EVERY_STEP = 2
+CFD_NOALLOC_MAX_ROWS = 1048576
 
 # field ids
 T0_U, T0_V, T0_W, T1_U, T1_V, T1_W, T3_U, T3_V, T3_W = range(9)
@@ -287,6 +288,7 @@ def _declare(L: C.CDLL) -> None:
         "cfd_phase_timing": ([vp, C.c_int], C.c_int),
         "cfd_phase_times": ([vp, dp, ip], C.c_int),
         "cfd_checkpoint_load": ([vp, C.c_char_p], C.c_int),
+        "cfd_bicgstab_ell_noalloc": ([vp, vp, vp, vp, C.c_size_t, C.c_double, C.c_int, vp, ip, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

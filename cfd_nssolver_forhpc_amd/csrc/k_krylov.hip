@@ -18,6 +18,8 @@
 // scalar, also when the vectors are FP32.
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
+#include <string>
 #include <type_traits>
 #include <cmath>
 
@@ -81,8 +83,8 @@ __global__ void __launch_bounds__(256) init_kernel(OpArgs a, const T *__restrict
     r0[r] = rr;
     r1[r] = rr;
     p1[r] = rr;
-    if (px1) {
-      px2[r] = px1[r];
+    if (px1) {  // px2: NULL for a solver without history (its px2 is all zero)
+      if (px2) px2[r] = px1[r];
       px1[r] = (double)x[r];
     }
     acc[0] += (double)br * (double)br;
@@ -445,8 +447,10 @@ int solve_bicg(Ctx &c, const OpArgs &a, const Vecs<T> &v, KrylovScalars *ks, int
     sum_finalize<1><<<1, FIN_THREADS, 0, c.stream>>>(part, nb, c.red.out + RS_MISC * 4);
     CFD_HIP(hipMemcpyAsync(c.hred, c.red.out + RS_MISC * 4, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     CFD_HIP(hipStreamSynchronize(c.stream));
-    if (c.hred[0] > (double)N * 0.1)
-      CFD_HIP(hipMemcpyAsync(v.x, v.px2, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+    if (c.hred[0] > (double)N * 0.1) {
+      if (v.px2) CFD_HIP(hipMemcpyAsync(v.x, v.px2, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+      else CFD_HIP(launch_fill(c, (double *)v.x, 0.0, N));
+    }
     if (c.hred[0] > 0.0 && ks == c.ks) c.nan_resets++;
   }
   if (ks == c.ks) c.total_iters += h.iters;
@@ -983,3 +987,100 @@ int mixed_reliable_solve(Ctx &c, const double *b, double *x, double tol, double 
 }
 
 }  // namespace cfd
+
+// ---------------------------------------------------------------------------
+// cfd_bicgstab_ell_noalloc: the BiCGSTAB solve of cfd_bicgstab on a
+// caller-owned ELL operator and caller-owned vectors, with every buffer the
+// solve needs beyond them taken from one static device array.  It calls no
+// hipMalloc / hipFree and no allocating library routine, so it can run where
+// allocation is not allowed (code under stream capture, tight serving loops).
+//
+// The array holds, per device, the solver's seven work vectors (r0, r1, p1,
+// s1, ap, as and px1, the start value kept for the NaN reset) of at most
+// CFD_NOALLOC_MAX_ROWS rows, the reduction partials and results, and the
+// device scalars.  The host refers to it only through the device address
+// hipGetSymbolAddress returns (taken once per device, with that device
+// current); the array's own name is a host-side stand-in, not an address.
+// Every call on a device shares that device's array and leaves its contents
+// behind: calls that overlap on one device race.  Nothing in it needs to
+// start at zero: each solve writes every vector and partial before it reads
+// it and sets the scalars itself.
+// ---------------------------------------------------------------------------
+namespace cfd {
+namespace {
+
+constexpr int64_t NOALLOC_ROWS = CFD_NOALLOC_MAX_ROWS;
+constexpr int NOALLOC_VECS = 7;  // r0 r1 p1 s1 ap as px1
+constexpr int64_t NOALLOC_PARTIALS = (int64_t)RED_SITES * 3 * RED_MAX_BLOCKS;
+constexpr int64_t NOALLOC_OUT = (int64_t)RED_SITES * 4;
+constexpr int64_t NOALLOC_KS = (sizeof(KrylovScalars) + 255) / 256 * 32;  // doubles, 256-byte multiple
+constexpr int64_t NOALLOC_DOUBLES = NOALLOC_VECS * NOALLOC_ROWS + NOALLOC_PARTIALS + NOALLOC_OUT + NOALLOC_KS;
+static_assert(NOALLOC_PARTIALS % 32 == 0 && NOALLOC_OUT % 32 == 0, "256-byte sections");
+
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// the device address of this device's `scratch`, looked up on the first call
+int scratch_address(int dev, double **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(CFD_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double *>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return CFD_OK;
+}
+
+}  // namespace
+}  // namespace cfd
+
+extern "C" int cfd_bicgstab_ell_noalloc(const int32_t *col, const double *val, const double *b, double *x,
+                                        size_t n_rows, double tol, int maxit, void *stream, int *iters,
+                                        double *relres) {
+  using namespace cfd;
+  if (!col || !val || !b || !x || !iters || !relres) return fail(CFD_ERR_ARG, "cfd_bicgstab_ell_noalloc: NULL argument");
+  if (n_rows < 1) return fail(CFD_ERR_ARG, "cfd_bicgstab_ell_noalloc: n_rows must be >= 1");
+  if ((int64_t)n_rows > NOALLOC_ROWS)
+    return fail(CFD_ERR_ARG, "cfd_bicgstab_ell_noalloc: n_rows " + std::to_string(n_rows) + " exceeds the maximum " +
+                                 std::to_string(NOALLOC_ROWS) + " (CFD_NOALLOC_MAX_ROWS)");
+  if (maxit < 0 || !(tol >= 0.0)) return fail(CFD_ERR_ARG, "cfd_bicgstab_ell_noalloc: bad tol / maxit");
+  int dev = 0;
+  CFD_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(CFD_ERR_ARG, "cfd_bicgstab_ell_noalloc: device index too large");
+  double *base = nullptr;
+  if (int rc = scratch_address(dev, &base)) return rc;
+  const int64_t N = (int64_t)n_rows;
+  // a context on the stack that borrows the caller's operator and the
+  // scratch sections; its host mirrors are stack variables (pageable)
+  Ctx c;
+  c.device = dev;
+  c.stream = static_cast<hipStream_t>(stream);
+  c.g.N = N;
+  c.g.nxc = (int)N;
+  c.g.nyc = c.g.nzc = 1;
+  c.col = const_cast<int32_t *>(col);
+  c.val = const_cast<double *>(val);
+  c.ell_ready = true;
+  double *vec[NOALLOC_VECS];
+  for (int q = 0; q < NOALLOC_VECS; ++q) vec[q] = base + (int64_t)q * NOALLOC_ROWS;
+  double *tail = base + (int64_t)NOALLOC_VECS * NOALLOC_ROWS;
+  c.red.partials = tail;
+  c.red.out = tail + NOALLOC_PARTIALS;
+  c.red.counters = nullptr;
+  c.ks = reinterpret_cast<KrylovScalars *>(tail + NOALLOC_PARTIALS + NOALLOC_OUT);
+  KrylovScalars hks{};
+  double hred[64] = {};
+  c.hks = &hks;
+  c.hred = hred;
+  // a fresh bicgstabRe2's first solve: timestep_ 1 (no restart rule), px2 zero
+  Vecs<double> v{x, vec[0], vec[1], vec[2], vec[3], vec[4], vec[5], b, vec[6], nullptr};
+  const int rc = solve_bicg<double>(c, op_args(c, CFD_OP_ELL), v, c.ks, 1, tol, maxit, iters, relres);
+  c.stream = nullptr;  // borrowed: the stack context must not release anything
+  return rc;
+}

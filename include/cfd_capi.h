@@ -400,6 +400,27 @@ int cfd_plot3d_read_q(const char *path, const cfd_grid_desc *grid, int *nblock, 
 int cfd_checkpoint_save(cfd_ctx *ctx, const char *path);
 int cfd_checkpoint_load(cfd_ctx *ctx, const char *path);
 
+/* ---- allocation-free solve ---------------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory (code under stream capture, tight serving loops, harnesses that
+   forbid hipMalloc): the BiCGSTAB solve of cfd_bicgstab -- a fresh
+   bicgstabRe2's first solve (bicgstabRe2.hpp:91-189: no restart rule, NaN
+   reset to the start value, px2 zero), the same kernels and reduction order,
+   so the result is bit for bit cfd_bicgstab's on a new context -- on a
+   caller-owned slot-major ELL operator (col, val: 7 * n_rows entries, the
+   layout of cfd_ell_upload) with caller-owned right-hand side b and solution
+   x (n_rows entries, x is the warm start and is overwritten).  All four are
+   DEVICE pointers on the current HIP device; stream is a hipStream_t (NULL:
+   the null stream) the solve is ordered on; the call returns once the solve
+   has finished.  It allocates nothing: its work vectors, reduction partials
+   and scalars live in a static device array of the library (about 65 MB per
+   device, held until the process ends), which limits n_rows to
+   CFD_NOALLOC_MAX_ROWS (CFD_ERR_ARG beyond).  Every call on one device
+   shares that array, so calls that overlap on one device race. */
+#define CFD_NOALLOC_MAX_ROWS 1048576
+int cfd_bicgstab_ell_noalloc(const int32_t *col, const double *val, const double *b, double *x, size_t n_rows,
+                             double tol, int maxit, void *stream, int *iters, double *relres);
+
 /* ---- measurement ------------------------------------------------------ */
 /* average device time (ms) of one launch of the kernel named by `what`, over
    nrep launches bracketed by HIP events on the context stream:
