@@ -50,11 +50,15 @@ class DragObjective(torch.autograd.Function):
             raise ValueError("u_in must be a 1-D tensor of ny_cal values")
         J = problem.run(u_in.detach().to("cpu", torch.float64).numpy(), float(gain.detach().cpu()))
         ctx.problem = problem
+        ctx.generation = problem.sim.adjoint_generation()  # the run this graph node differentiates
         ctx.device = u_in.device
         return torch.tensor(J, dtype=torch.float64, device=u_in.device)
 
     @staticmethod
     def backward(ctx, grad_J: torch.Tensor):
+        if ctx.problem.sim.adjoint_generation() != ctx.generation:
+            raise RuntimeError("DragObjective.backward: the simulation has run another objective (or changed its "
+                               "state or parameters) since this forward; evaluate J again before backward")
         g_uin, g_gain, stats = ctx.problem.sim.adjoint_backward()
         ctx.problem.last_stats = stats
         gJ = grad_J.to(torch.float64)

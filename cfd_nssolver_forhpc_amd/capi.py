@@ -83,6 +83,7 @@ class CfdParams(C.Structure):
         ("u_init", C.c_double * 3),
         ("precond", C.c_int), ("mg_nu", C.c_int),
         ("convection", C.c_int), ("les", C.c_int),
+        ("periodic_z", C.c_int), ("outflow_convective", C.c_int), ("outflow_speed", C.c_double),
     ]
 
     def set_bc(self, dir_table, num_table):
@@ -276,6 +277,7 @@ def _declare(L: C.CDLL) -> None:
         "cfd_set_forcing_gain": ([vp, C.c_double], C.c_int),
         "cfd_adjoint_forward": ([vp, C.c_int, C.c_int, C.c_int, dp], C.c_int),
         "cfd_adjoint_backward": ([vp, dp, C.c_size_t, dp, C.POINTER(CfdAdjointStats)], C.c_int),
+        "cfd_adjoint_generation": ([vp, C.POINTER(C.c_ulonglong)], C.c_int),
         "cfd_cdcl_open": ([C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.POINTER(vp)], C.c_int),
         "cfd_cdcl_record": ([vp, C.c_int, C.c_double, C.c_double, C.c_double], C.c_int),
         "cfd_cdcl_close": ([vp], C.c_int),

@@ -59,21 +59,36 @@ def cylinder_params(Re: float, dt: float, cx: float, cy: float, D: float, tol: f
     return p
 
 
+def baseline_bc(p: CfdParams, speed: float = 1.0) -> CfdParams:
+    """The cylinder workloads' boundaries: convective outflow on x+ at the
+    inflow speed (pressure p = 0 there) and periodic z; uniform inflow and
+    slip y-walls stay in the table."""
+    p.periodic_z = 1
+    p.outflow_convective = 1
+    p.outflow_speed = speed
+    return p
+
+
 def config_a() -> Workload:
-    """128x64x4 over 20D x 10D x 0.5D, D = 1 at (5, 5), Re 100, dt 0.002, 100 steps, FP64, tol 1e-10."""
+    """128x64x4 over 20D x 10D x 0.5D, D = 1 at (5, 5), Re 100, dt 0.002, 100 steps, FP64, tol 1e-10;
+    convective outflow, slip y-walls, periodic z."""
     g = uniform_grid((128, 64, 4), (20.0, 10.0, 0.5))
-    return Workload("cyl-128x64x4-Re100", g, cylinder_params(100.0, 0.002, 5.0, 5.0, 1.0, 1e-10), 100, 1e-3, 1)
+    return Workload("cyl-128x64x4-Re100", g, baseline_bc(cylinder_params(100.0, 0.002, 5.0, 5.0, 1.0, 1e-10)), 100,
+                    1e-3, 1)
 
 
 def config_b(precision: int = capi.CFD_FP64) -> Workload:
-    """512x256x64 over 20 x 10 x 2.5 (D = 25 cells), Re 200, dt 5e-4, 200 steps, tol 1e-10."""
+    """512x256x64 over 20 x 10 x 2.5 (D = 25 cells), Re 200, dt 5e-4, 200 steps, tol 1e-10;
+    convective outflow, slip y-walls, periodic z."""
     g = uniform_grid((512, 256, 64), (20.0, 10.0, 2.5))
     name = "cyl-512x256x64-Re200" + ("-mixed" if precision == capi.CFD_MIXED else "-fp64")
-    return Workload(name, g, cylinder_params(200.0, 5e-4, 5.0, 5.0, 1.0, 1e-10, precision), 200, 1e-3, 1)
+    return Workload(name, g, baseline_bc(cylinder_params(200.0, 5e-4, 5.0, 5.0, 1.0, 1e-10, precision)), 200, 1e-3,
+                    1)
 
 
 def config_d() -> Workload:
-    """256x128x32 over 20 x 10 x 2.5, Re 100, dt 1e-3, 400 steps (adjoint case)."""
+    """256x128x32 over 20 x 10 x 2.5, Re 100, dt 1e-3, 400 steps (adjoint case).  The discrete adjoint
+    transposes the table boundary conditions, so this case keeps zero-gradient outflow and slip z walls."""
     g = uniform_grid((256, 128, 32), (20.0, 10.0, 2.5))
     return Workload("cyl-256x128x32-Re100", g, cylinder_params(100.0, 1e-3, 5.0, 5.0, 1.0, 1e-10), 400, 1e-3, 1)
 
@@ -87,8 +102,8 @@ def config_e(precision: int = capi.CFD_MIXED) -> Workload:
     from .solver import axis_uniform
     az = axis_uniform(128, 128 * h0)
     g = _GA((1024, 512, 128), ax[0], ay[0], az[0], ax[1], ay[1], az[1], ax[2], ay[2], az[2], 40.0, 20.0, 128 * h0)
-    return Workload("cyl-1024x512x128-Re300", g, cylinder_params(300.0, 2.5e-4, 10.0, 10.0, D, 1e-10, precision),
-                    100, 1e-3, 1)
+    return Workload("cyl-1024x512x128-Re300", g,
+                    baseline_bc(cylinder_params(300.0, 2.5e-4, 10.0, 10.0, D, 1e-10, precision)), 100, 1e-3, 1)
 
 
 def small_cylinder(ncal=(32, 16, 8), lengths=(8.0, 4.0, 2.0), quirk=0, tol=1e-10, inner=3) -> Workload:

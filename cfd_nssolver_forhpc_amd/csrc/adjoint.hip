@@ -48,6 +48,7 @@ struct AdjState {
   std::vector<AdjCkpt> ck;
   AdjCkpt fin;
   bool ready = false;
+  unsigned long long generation = 0;  // counts objective runs; the last one's number while ready
   // reverse-sweep workspace
   int ws_every = 0, ws_passes = 0;
   double *lx = nullptr, *lsum = nullptr;          // 3G each: adjoint of T0 / summed adjoint of T1
@@ -651,12 +652,20 @@ int adjoint_checks(const Ctx &c) {
   if (c.prm.source_index_quirk) return fail(CFD_ERR_ARG, "adjoint: needs source_index_quirk = 0");
   if (c.prm.convection != CFD_CONV_QUICK) return fail(CFD_ERR_ARG, "adjoint: needs the QUICK convection scheme");
   if (c.prm.les) return fail(CFD_ERR_ARG, "adjoint: needs les = 0");
+  if (c.prm.periodic_z || c.prm.outflow_convective)
+    return fail(CFD_ERR_ARG, "adjoint: needs the table boundary conditions (periodic_z = outflow_convective = 0)");
   if (c.g.nxc < 2 || c.g.nyc < 2 || c.g.nzc < 2) return fail(CFD_ERR_ARG, "adjoint: needs >= 2 cells per axis");
   if (!c.ell_ready) return fail(CFD_ERR_STATE, "adjoint: ELL operator not assembled");
   return CFD_OK;
 }
 
 }  // namespace
+
+// the recorded objective run no longer describes the context: its state,
+// parameters, forcing gain or inflow profile changed after it
+void adjoint_invalidate(Ctx &c) {
+  if (c.adj) c.adj->ready = false;
+}
 
 void adjoint_release(Ctx &c) {
   AdjState *A = c.adj;
@@ -687,6 +696,7 @@ int cfd_set_inflow_profile(cfd_ctx *ctx, const double *u_in, size_t n) {
   Ctx &c = *reinterpret_cast<Ctx *>(ctx);
   if (!u_in) {
     afree(c.uin);
+    adjoint_invalidate(c);
     return CFD_OK;
   }
   if (n != (size_t)c.g.nyc) return fail(CFD_ERR_ARG, "inflow profile needs ny_cal values");
@@ -698,6 +708,7 @@ int cfd_set_inflow_profile(cfd_ctx *ctx, const double *u_in, size_t n) {
   }
   ADJ_HIP(hipMemcpyAsync(c.uin, u_in, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
   ADJ_HIP(hipStreamSynchronize(c.stream));
+  adjoint_invalidate(c);
   return CFD_OK;
 }
 
@@ -705,6 +716,15 @@ int cfd_set_forcing_gain(cfd_ctx *ctx, double gain) {
   ADJ_CTX(ctx);
   if (!std::isfinite(gain)) return fail(CFD_ERR_ARG, "gain is not finite");
   reinterpret_cast<Ctx *>(ctx)->gain = gain;
+  adjoint_invalidate(*reinterpret_cast<Ctx *>(ctx));
+  return CFD_OK;
+}
+
+int cfd_adjoint_generation(cfd_ctx *ctx, unsigned long long *generation) {
+  ADJ_CTX(ctx);
+  Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  if (!generation) return fail(CFD_ERR_ARG, "generation is NULL");
+  *generation = (c.adj && c.adj->ready) ? c.adj->generation : 0ull;
   return CFD_OK;
 }
 
@@ -747,6 +767,7 @@ int cfd_adjoint_forward(cfd_ctx *ctx, int nsteps, int avg_from, int ckpt_every, 
   for (int q = avg_from; q < nsteps; ++q) s += A.cd[q];
   *J = s / (double)(nsteps - avg_from);
   A.ready = true;
+  A.generation += 1;
   return CFD_OK;
 }
 

@@ -84,6 +84,25 @@ int check_params(const cfd_params *p) {
     return fail(CFD_ERR_ARG, "unknown convection scheme");
   if (p->les != 0 && p->les != 1) return fail(CFD_ERR_ARG, "les must be 0 or 1");
   if (p->precond == CFD_PRECOND_MG && (p->mg_nu < 1 || p->mg_nu > 8)) return fail(CFD_ERR_ARG, "mg_nu must be in [1, 8]");
+  if (p->periodic_z != 0 && p->periodic_z != 1) return fail(CFD_ERR_ARG, "periodic_z must be 0 or 1");
+  if (p->outflow_convective != 0 && p->outflow_convective != 1)
+    return fail(CFD_ERR_ARG, "outflow_convective must be 0 or 1");
+  if (p->outflow_convective && !(p->outflow_speed > 0.0 && std::isfinite(p->outflow_speed)))
+    return fail(CFD_ERR_ARG, "outflow_speed must be positive and finite");
+  if (p->outflow_convective && p->p_dirichlet[1] != 1)
+    return fail(CFD_ERR_ARG, "a convective outflow needs the Dirichlet pressure p = 0 on face 1");
+  if (p->periodic_z && p->op == CFD_OP_CSR)
+    return fail(CFD_ERR_ARG, "the CSR builder closes every face (createPressureMatrixCSR): no periodic z");
+  if ((p->periodic_z || p->outflow_convective) && p->source_index_quirk)
+    return fail(CFD_ERR_ARG, "periodic / convective boundaries add the source term at its own cell "
+                             "(source_index_quirk 0)");
+  return CFD_OK;
+}
+
+// the checks of cfd_params that need the grid
+int check_params_grid(const cfd_params *p, int nz_cal) {
+  if (p->periodic_z && (nz_cal < 4 || nz_cal % 2 != 0))
+    return fail(CFD_ERR_ARG, "periodic_z needs an even nz_cal of at least 4 (red-black colouring across the seam)");
   return CFD_OK;
 }
 
@@ -289,6 +308,9 @@ int cfd_params_default(cfd_params *out) {
   out->mg_nu = 2;
   out->convection = CFD_CONV_QUICK;  // controlPanel.hpp:125
   out->les = 0;                      // controlPanel.hpp:135 (TERBULENCE_SMAGORINSKY off)
+  out->periodic_z = 0;
+  out->outflow_convective = 0;
+  out->outflow_speed = 1.0;
   return CFD_OK;
 }
 
@@ -345,57 +367,83 @@ int cfd_axis_sml(int ncal, double L, double lS, int nS, int off_nS, double *X, d
   return CFD_OK;
 }
 
-// Stretched axis of the large configuration (no reference counterpart):
-// uniform width h0 over a band around xc, widths growing geometrically by
-// `ratio` per cell away from it, capped so the ncal widths sum to L.
-static double stretch_sum(int ncal, int ic0, int ic1, double h0, double ratio, double hcap, double *w) {
-  double s = 0.0;
-  for (int c = 0; c < ncal; ++c) {
-    const int dist = c < ic0 ? ic0 - c : (c > ic1 ? c - ic1 : 0);
-    double h = h0 * std::pow(ratio, (double)dist);
-    if (h > hcap) h = hcap;
-    if (w) w[c] = h;
-    s += h;
-  }
-  return s;
+// Stretched axis of the large configuration (no reference counterpart): tanh
+// clustering around xc.  The cell width, as a function of the node index
+// offset xi from the centre, blends from h0 inside a band of +-a cells to H
+// outside through tanh steps of width w cells,
+//   dx/dxi = H - (H - h0)/2 [tanh((xi + a)/w) - tanh((xi - a)/w)],
+// so the node positions have the closed form
+//   x(xi) = xc + H xi - (H - h0)/2 w [lncosh((xi + a)/w) - lncosh((xi - a)/w)]
+// (lncosh(t) = |t| + log1p(exp(-2|t|)) - log 2).  Node i sits at xi0 + i:
+// xi0 puts node 0 at 0, H puts node ncal at L (both by bisection; x is
+// increasing in xi, and x(xi0 + ncal) in H), and w is the narrowest step
+// whose largest ratio of neighbouring widths is at most `ratio`.
+namespace {
+double lncosh(double t) {
+  const double a = std::fabs(t);
+  return a + std::log1p(std::exp(-2.0 * a)) - std::log(2.0);
 }
+struct TanhAxis {
+  double h0, a, w, H, xc;
+  double f(double xi) const {
+    return H * xi - 0.5 * (H - h0) * w * (lncosh((xi + a) / w) - lncosh((xi - a) / w));
+  }
+};
+// xi0 with xc + f(xi0) = 0
+double tanh_xi0(const TanhAxis &t, int ncal) {
+  double lo = -(double)ncal - 1.0, hi = 0.0;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (t.xc + t.f(mid) > 0.0) hi = mid;
+    else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+// H (and xi0) with node ncal at L; returns the largest neighbouring-width ratio
+double tanh_fit(TanhAxis &t, int ncal, double L, double *xi0_out) {
+  double lo = t.h0, hi = L;
+  double xi0 = 0.0;
+  for (int it = 0; it < 200; ++it) {
+    t.H = 0.5 * (lo + hi);
+    xi0 = tanh_xi0(t, ncal);
+    if (t.xc + t.f(xi0 + ncal) > L) hi = t.H;
+    else lo = t.H;
+  }
+  t.H = 0.5 * (lo + hi);
+  xi0 = tanh_xi0(t, ncal);
+  double rmax = 1.0, prev = t.f(xi0 + 1.0) - t.f(xi0);
+  for (int c = 1; c < ncal; ++c) {
+    const double h = t.f(xi0 + c + 1.0) - t.f(xi0 + c);
+    rmax = std::max(rmax, std::max(h / prev, prev / h));
+    prev = h;
+  }
+  *xi0_out = xi0;
+  return rmax;
+}
+}  // namespace
 
 int cfd_axis_tanh(int ncal, double L, double xc, double h0, double band, double ratio, double *X, double *D,
                   double *Ds) {
-  if (ncal < 4 || !(L > 0) || !(h0 > 0) || !(ratio >= 1.0) || h0 * ncal > L || !X || !D || !Ds)
+  if (ncal < 4 || !(L > 0) || !(h0 > 0) || !(ratio > 1.0) || h0 * ncal >= L || !(xc > 0 && xc < L) ||
+      !(band > 0) || !X || !D || !Ds)
     return fail(CFD_ERR_ARG, "cfd_axis_tanh: bad arguments");
   const int n = ncal + 2 * GC;
-  int nb = (int)std::floor(band / h0 + 0.5);
-  nb = std::max(1, std::min(nb, ncal));
-  std::vector<double> w((size_t)ncal);
-  int best_ic0 = 0;
-  double lo = h0, hi = L;
-  for (int pass = 0; pass < 2; ++pass) {
-    lo = h0;
-    hi = L;
-    for (int it = 0; it < 200; ++it) {
-      const double mid = 0.5 * (lo + hi);
-      const double s = stretch_sum(ncal, best_ic0, best_ic0 + nb - 1, h0, ratio, mid, nullptr);
-      if (s > L) hi = mid;
-      else lo = mid;
-    }
-    if (pass == 0) {
-      double best = 1e300;
-      for (int ic0 = 0; ic0 + nb <= ncal; ++ic0) {
-        stretch_sum(ncal, ic0, ic0 + nb - 1, h0, ratio, lo, w.data());
-        double x = 0.0;
-        for (int c = 0; c < ic0; ++c) x += w[c];
-        const double centre = x + 0.5 * nb * h0;
-        if (std::fabs(centre - xc) < best) {
-          best = std::fabs(centre - xc);
-          best_ic0 = ic0;
-        }
-      }
-    }
+  TanhAxis t{h0, 0.5 * band / h0, 1.0, h0, xc};
+  double xi0 = 0.0;
+  // the narrowest tanh step that keeps the width ratio within `ratio`
+  double wlo = 0.25, whi = (double)ncal;
+  t.w = whi;
+  if (tanh_fit(t, ncal, L, &xi0) > ratio) return fail(CFD_ERR_ARG, "cfd_axis_tanh: ratio not reachable");
+  for (int it = 0; it < 60; ++it) {
+    t.w = 0.5 * (wlo + whi);
+    if (tanh_fit(t, ncal, L, &xi0) > ratio) wlo = t.w;
+    else whi = t.w;
   }
-  stretch_sum(ncal, best_ic0, best_ic0 + nb - 1, h0, ratio, lo, w.data());
+  t.w = whi;
+  tanh_fit(t, ncal, L, &xi0);
   std::fill(X, X + n + 1, 0.0);
-  for (int c = 0; c < ncal; ++c) X[GC + c + 1] = X[GC + c] + w[c];
+  for (int c = 1; c < ncal; ++c) X[GC + c] = xc + t.f(xi0 + c);
+  X[GC] = 0.0;
   X[GC + ncal] = L;
   gc_dx(n, X, D, Ds);
   return CFD_OK;
@@ -418,6 +466,7 @@ int cfd_create(const cfd_grid_desc *grid, const cfd_params *params, int device, 
   if (rc) return rc;
   if (grid->nx_cal < 2 || grid->ny_cal < 2 || grid->nz_cal < 2)
     return fail(CFD_ERR_ARG, "every axis needs at least 2 interior cells");
+  if ((rc = check_params_grid(params, grid->nz_cal))) return rc;
   const int64_t N = (int64_t)grid->nx_cal * grid->ny_cal * grid->nz_cal;
   if (N >= (int64_t)INT32_MAX) return fail(CFD_ERR_ARG, "grid too large for int32 ELL columns");
   const double *m[9] = {grid->X, grid->Y, grid->Z, grid->Dx, grid->Dy, grid->Dz, grid->Dxs, grid->Dys, grid->Dzs};
@@ -501,6 +550,8 @@ int cfd_set_params(cfd_ctx *ctx, const cfd_params *params) {
   int rc = check_params(params);
   if (rc) return rc;
   Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  if ((rc = check_params_grid(params, c.g.nzc))) return rc;
+  adjoint_invalidate(c);
   c.prm = *params;
   return CFD_OK;
 }
@@ -520,6 +571,7 @@ int cfd_upload(cfd_ctx *ctx, int field, const double *host, size_t len) {
   int rc = field_len(c, field, &n);
   if (rc) return rc;
   if (!host || len != n) return fail(CFD_ERR_ARG, "upload length mismatch: expected " + std::to_string(n));
+  adjoint_invalidate(c);
   CFD_HIP(hipMemcpyAsync(c.f[field], host, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
   CFD_HIP(hipStreamSynchronize(c.stream));
   if (field == CFD_P) c.p_from_x = false;
@@ -544,6 +596,7 @@ int cfd_init_state(cfd_ctx *ctx, double noise_amp, uint64_t seed) {
   CTX_CHECK(ctx);
   Ctx &c = *reinterpret_cast<Ctx *>(ctx);
   if (!std::isfinite(noise_amp) || noise_amp < 0.0) return fail(CFD_ERR_ARG, "noise amplitude must be >= 0");
+  adjoint_invalidate(c);
   RET_LAUNCH(launch_init_fields(c, noise_amp, seed));
   RET_LAUNCH(launch_fill(c, c.px1, 0.0, c.g.N));
   RET_LAUNCH(launch_fill(c, c.px2, 0.0, c.g.N));
@@ -736,6 +789,7 @@ int cfd_sor_pipeline(cfd_ctx *ctx, double omega, double crit, int itmax, int *it
     return fail(CFD_ERR_ARG, "bad SorPipeLine arguments");
   int rc = sync_p(c);
   if (rc) return rc;
+  RET_LAUNCH(launch_periodic_refresh(c, c.f[CFD_T1_U], c.f[CFD_T1_V], c.f[CFD_T1_W]));
   return sor_pipeline_solve(c, omega, crit, itmax, iters, error);
 }
 
@@ -784,6 +838,7 @@ int cfd_source_add(cfd_ctx *ctx) {
 int cfd_build_rhs(cfd_ctx *ctx) {
   CTX_CHECK(ctx);
   Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  RET_LAUNCH(launch_periodic_refresh(c, c.f[CFD_T1_U], c.f[CFD_T1_V], c.f[CFD_T1_W]));
   RET_LAUNCH(launch_rhs(c, c.f[CFD_T1_U], c.f[CFD_T1_V], c.f[CFD_T1_W], c.f[CFD_JP], c.f[CFD_RHS]));
   CFD_HIP(hipStreamSynchronize(c.stream));
   return CFD_OK;
@@ -795,6 +850,7 @@ int cfd_pressure_scatter(cfd_ctx *ctx) {
   int rc = sync_p(c);
   if (rc) return rc;
   RET_LAUNCH(launch_scatter_p(c, c.f[CFD_X], c.f[CFD_P]));
+  RET_LAUNCH(launch_periodic_refresh_p(c, c.f[CFD_P]));  // periodic z: the seam's ghosts follow the interior
   CFD_HIP(hipStreamSynchronize(c.stream));
   return CFD_OK;
 }
@@ -909,6 +965,7 @@ int cfd_copy_t3_to_t0(cfd_ctx *ctx) {
 // change inside a step, so every evaluation is bit-identical.
 int cfd_step(cfd_ctx *ctx, cfd_step_stats *stats) {
   CTX_CHECK(ctx);
+  adjoint_invalidate(*reinterpret_cast<Ctx *>(ctx));
   return step_impl(*reinterpret_cast<Ctx *>(ctx), stats);
 }
 
@@ -964,6 +1021,9 @@ int step_impl(Ctx &c, cfd_step_stats *stats) {
     } else {
       RET_LAUNCH(launch_pred_add(c, c.pred, c.f[CFD_SRC], u1, v1, w1));
     }
+    // periodic z: T1's ghost layers take the new interior's images before
+    // the right-hand side reads them
+    RET_LAUNCH(launch_periodic_refresh(c, u1, v1, w1));
     CFD_HIP(mark(CFD_PHASE_CONVECTION));
     st.iters[pass] = -1;
     if (solve) {
@@ -977,6 +1037,9 @@ int step_impl(Ctx &c, cfd_step_stats *stats) {
       st.relres[pass] = rel;
       st.inner_iters[pass] = inner;
       c.p_from_x = !sor_p;
+      // SorPipeLine leaves the ghosted p: periodic z refreshes the seam's
+      // ghosts the update reads (the X-reading update wraps by itself)
+      if (sor_p) RET_LAUNCH(launch_periodic_refresh_p(c, c.f[CFD_P]));
     }
     CFD_HIP(mark(CFD_PHASE_PRESSURE));
     int mode = UPD_DRAG;
@@ -1038,6 +1101,7 @@ int cfd_set_loop(cfd_ctx *ctx, int loop, double time) {
   if (!ctx) return fail(CFD_ERR_ARG, "context is NULL");
   if (loop < 1) return fail(CFD_ERR_ARG, "loop starts at 1");
   Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+  adjoint_invalidate(c);
   c.loop = loop;
   c.time = time;
   return CFD_OK;
@@ -1089,13 +1153,13 @@ int cfd_time_kernel(cfd_ctx *ctx, int what, int nrep, double *ms_avg) {
   Ctx &c = *reinterpret_cast<Ctx *>(ctx);
   if (!ms_avg || nrep < 1) return fail(CFD_ERR_ARG, "bad timing arguments");
   int rc = CFD_OK;
-  if ((what == 0 || what == 1 || what == 3 || what == 4) && !c.ell_ready && c.prm.op == CFD_OP_ELL)
+  if ((what == 0 || what == 1 || what == 3 || what == 4 || what == 8) && !c.ell_ready && c.prm.op == CFD_OP_ELL)
     return fail(CFD_ERR_STATE, "ELL operator not assembled");
   if (what == 1 && (rc = ensure_mixed(c))) return rc;
   if (what == 2 && !c.spe_ready) return fail(CFD_ERR_STATE, "SPE operator not assembled");
   if (what == 7 && !c.csr_ready) return fail(CFD_ERR_STATE, "CSR operator not assembled");
   if (what == 3 && (rc = ensure_perm(c))) return rc;
-  if (what < 0 || what > 7) return fail(CFD_ERR_ARG, "unknown kernel id");
+  if (what < 0 || what > 8) return fail(CFD_ERR_ARG, "unknown kernel id");
   auto body = [&]() -> hipError_t {
     switch (what) {
       case 0: return launch_spmv(c, CFD_OP_ELL, CFD_FP64, c.f[CFD_X], c.f[CFD_VEC_A]);
@@ -1104,13 +1168,14 @@ int cfd_time_kernel(cfd_ctx *ctx, int what, int nrep, double *ms_avg) {
       case 3: return launch_rbsor_sweep(c, c.prm.sor_omega);
       case 4: return launch_bicg_iteration(c);
       case 7: return launch_spmv(c, CFD_OP_CSR, CFD_FP64, c.f[CFD_X], c.f[CFD_VEC_A]);
+      case 8: return launch_spmv_dot(c);
       case 5:
         return launch_predictor(c, c.f[CFD_T0_U], c.f[CFD_T0_V], c.f[CFD_T0_W], c.f[CFD_T1_U], c.f[CFD_T1_V],
                                 c.f[CFD_T1_W], nullptr, 0);
       default: return launch_copy_kernel_bench(c, c.f[CFD_VEC_A], c.f[CFD_VEC_B], c.g.N);
     }
   };
-  if (what == 4) RET_LAUNCH(launch_bicg_timing_setup(c));  // r0 = b - A*0 on scratch vectors, no stop test
+  if (what == 4 || what == 8) RET_LAUNCH(launch_bicg_timing_setup(c));  // r0 = b - A*0 on scratch vectors, no stop test
   RET_LAUNCH(body());  // warm-up
   CFD_HIP(hipEventRecord(c.ev0, c.stream));
   for (int r = 0; r < nrep; ++r) RET_LAUNCH(body());

@@ -151,6 +151,8 @@ int pbicgstab_solve_aux(Ctx &c, const double *b, double *x, double tol, int maxi
 // one timestep (cfd_step's body); adjoint.hip's recomputation calls it too
 int step_impl(Ctx &c, cfd_step_stats *stats);
 void adjoint_release(Ctx &c);
+// the last objective run stops being differentiable (state or parameters changed)
+void adjoint_invalidate(Ctx &c);
 // FP64 BiCGSTAB on the ELL operator that leaves the pressure solver's state
 // (solve count, NaN-reset history) untouched
 int bicgstab_solve_aux(Ctx &c, const double *b, double *x, double tol, int maxit, int timestep, int *iters,
@@ -195,6 +197,10 @@ hipError_t launch_dfib_update_into(Ctx &c, int mode, const double *p_or_x, doubl
                                    double *t2out = nullptr);
 hipError_t launch_source_accumulate(Ctx &c);
 hipError_t launch_bc(Ctx &c, bool main, bool copy, bool pressure);
+// periodic z: the ghost layers of (u, v, w) from their interior images (no-op otherwise)
+hipError_t launch_periodic_refresh(Ctx &c, double *u, double *v, double *w);
+// periodic z: the pressure's z ghost layers from their interior images (no-op otherwise)
+hipError_t launch_periodic_refresh_p(Ctx &c, double *p);
 hipError_t launch_drag(Ctx &c);
 hipError_t launch_maxdiff(Ctx &c);
 // sum over the interior of f[dir] times the cell volume into out[0]
@@ -234,6 +240,7 @@ int mixed_reliable_solve(Ctx &c, const double *b, double *x, double tol, double 
                          int *iters, double *relres);
 hipError_t launch_bicg_iteration(Ctx &c);
 hipError_t launch_bicg_timing_setup(Ctx &c);
+hipError_t launch_spmv_dot(Ctx &c);
 // k_sor.hip
 int rbsor_solve(Ctx &c, const double *b, double *x, double omega, double tol, int maxit, int check_every,
                 int *iters, double *relres);

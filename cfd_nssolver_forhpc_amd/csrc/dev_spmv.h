@@ -78,7 +78,7 @@ __device__ inline uint32_t fastdiv(uint32_t n, const FastDiv &f) {
 // select), so a wave waits once per row instead of once per neighbour.
 __device__ inline double spe_row(const double *__restrict__ coef, const double *__restrict__ jp, int nxc, int nyc,
                                  int nzc, const FastDiv &div_nz, const FastDiv &div_ny, int64_t ic,
-                                 const double *__restrict__ x) {
+                                 const double *__restrict__ x, bool wrapz = false) {
   const int64_t nynz = (int64_t)nyc * nzc;
   const uint32_t q = fastdiv((uint32_t)ic, div_nz);  // i nyc + j
   const int k = (int)((uint32_t)ic - q * (uint32_t)nzc);
@@ -86,10 +86,13 @@ __device__ inline double spe_row(const double *__restrict__ coef, const double *
   const int j = (int)(q - i_ * (uint32_t)nyc);
   const int i = (int)i_;
   const double *xm = coef, *xp = xm + nxc, *ym = xp + nxc, *yp = ym + nyc, *zm = yp + nyc, *zp = zm + nzc;
-  const bool hxm = i != 0, hym = j != 0, hzm = k != 0, hzp = k != nzc - 1, hyp = j != nyc - 1, hxp = i != nxc - 1;
-  const double vxm = x[hxm ? ic - nynz : ic], vym = x[hym ? ic - nzc : ic], vzm = x[hzm ? ic - 1 : ic];
+  // periodic z: the z neighbours always exist, across the seam at +-(nzc - 1)
+  const bool hxm = i != 0, hym = j != 0, hzm = wrapz || k != 0, hzp = wrapz || k != nzc - 1, hyp = j != nyc - 1,
+             hxp = i != nxc - 1;
+  const int64_t izm = k != 0 ? ic - 1 : ic + (nzc - 1), izp = k != nzc - 1 ? ic + 1 : ic - (nzc - 1);
+  const double vxm = x[hxm ? ic - nynz : ic], vym = x[hym ? ic - nzc : ic], vzm = x[hzm ? izm : ic];
   const double vc = x[ic];
-  const double vzp = x[hzp ? ic + 1 : ic], vyp = x[hyp ? ic + nzc : ic], vxp = x[hxp ? ic + nynz : ic];
+  const double vzp = x[hzp ? izp : ic], vyp = x[hyp ? ic + nzc : ic], vxp = x[hxp ? ic + nynz : ic];
   const double cxm = xm[i], cym = ym[j], czm = zm[k], czp = zp[k], cyp = yp[j], cxp = xp[i];
   const double J = jp[ic];
   double sum = 0.0;
@@ -124,6 +127,7 @@ struct OpArgs {
   const double *coef, *jp;   // SPE
   int nxc, nyc, nzc;
   FastDiv div_nz, div_ny;    // SPE: row -> (i, j, k)
+  bool wrapz;                // SPE: periodic z (the z neighbours wrap across the seam)
   const int32_t *cptr, *ccol;  // CSR
   const double *cval;
 };
@@ -179,7 +183,8 @@ __device__ inline int spe_column_rows(const OpArgs &a, const SpeColumn &c, const
   const int nxc = a.nxc, nyc = a.nyc, nzc = a.nzc;
   const int64_t nynz = (int64_t)nyc * nzc, col = (int64_t)c.j * nzc + c.k;
   const double *xm = a.coef, *xp = xm + nxc, *ym = xp + nxc, *yp = ym + nyc, *zm = yp + nyc, *zp = zm + nzc;
-  const bool hym = c.j != 0, hzm = c.k != 0, hzp = c.k != nzc - 1, hyp = c.j != nyc - 1;
+  const bool hym = c.j != 0, hzm = a.wrapz || c.k != 0, hzp = a.wrapz || c.k != nzc - 1, hyp = c.j != nyc - 1;
+  const int dzm = c.k != 0 ? -1 : nzc - 1, dzp = c.k != nzc - 1 ? 1 : -(nzc - 1);  // z steps, across the seam
   const double cym = ym[c.j], cyp = yp[c.j], czm = zm[c.k], czp = zp[c.k];
   double cen[SPE_PLANES + 2];
 #pragma unroll
@@ -194,8 +199,8 @@ __device__ inline int spe_column_rows(const OpArgs &a, const SpeColumn &c, const
     const int64_t ic = (int64_t)min(c.i0 + p, nxc - 1) * nynz + col;
     vym[p] = x[hym ? ic - nzc : ic];
     vyp[p] = x[hyp ? ic + nzc : ic];
-    vzm[p] = x[hzm ? ic - 1 : ic];
-    vzp[p] = x[hzp ? ic + 1 : ic];
+    vzm[p] = x[hzm ? ic + dzm : ic];
+    vzp[p] = x[hzp ? ic + dzp : ic];
     J[p] = a.jp[ic];
   }
 #pragma unroll
@@ -222,7 +227,7 @@ __device__ inline int spe_column_rows(const OpArgs &a, const SpeColumn &c, const
 constexpr int OP_ELL_VAL32 = 16;
 
 __device__ inline double op_row(const OpArgs &a, int64_t r, const double *__restrict__ x) {
-  if (a.op == CFD_OP_SPE) return spe_row(a.coef, a.jp, a.nxc, a.nyc, a.nzc, a.div_nz, a.div_ny, r, x);
+  if (a.op == CFD_OP_SPE) return spe_row(a.coef, a.jp, a.nxc, a.nyc, a.nzc, a.div_nz, a.div_ny, r, x, a.wrapz);
   if (a.op == CFD_OP_CSR) return csr_row(a.cptr, a.ccol, a.cval, r, x);
   if (a.op == OP_ELL_VAL32) return ell_row<float, double>(a.col, a.val32, a.N, r, x);
   return ell_row<double, double>(a.col, a.val, a.N, r, x);
@@ -326,6 +331,7 @@ inline OpArgs op_args(const Ctx &c, int op) {
   a.nzc = c.g.nzc;
   a.div_nz = fastdiv_make((uint32_t)c.g.nzc);
   a.div_ny = fastdiv_make((uint32_t)c.g.nyc);
+  a.wrapz = c.prm.periodic_z != 0;
   a.cptr = c.csr_ptr;
   a.ccol = c.csr_col;
   a.cval = c.csr_val;

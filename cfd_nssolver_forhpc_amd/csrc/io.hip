@@ -72,7 +72,7 @@ bool grid_ok(const cfd_grid_desc *g) {
 // order, then a 64-bit checksum of the arrays' bytes (ckpt_mix over 8-byte
 // words, FNV-1a constants).
 constexpr char CKPT_MAGIC[8] = {'C', 'F', 'D', 'C', 'K', 'P', 'T', '1'};
-constexpr uint32_t CKPT_VERSION = 1;
+constexpr uint32_t CKPT_VERSION = 2;
 
 struct CkptHeader {
   char magic[8];
@@ -82,6 +82,9 @@ struct CkptHeader {
   double time;
   uint64_t grid_hash;  // of the extents and the host metric arrays
   uint64_t G, N;
+  // bicgstabRe2's counters (bicgstabRe2.hpp:80-84): pResets_, numRestarts_, totalNumIter_
+  int32_t nan_resets, restarts;
+  int64_t total_iters;
 };
 
 constexpr uint64_t FNV_OFFSET = 1469598103934665603ull, FNV_PRIME = 1099511628211ull;
@@ -338,6 +341,11 @@ int cfd_checkpoint_save(cfd_ctx *ctx, const char *path) {
   h.G = (uint64_t)c.g.G;
   h.N = (uint64_t)c.g.N;
   CFD_HIP(hipStreamSynchronize(c.stream));  // the state of every queued step
+  KrylovScalars ks;
+  CFD_HIP(hipMemcpy(&ks, c.ks, sizeof(ks), hipMemcpyDeviceToHost));
+  h.nan_resets = c.nan_resets;
+  h.restarts = ks.restarts;
+  h.total_iters = c.total_iters;
   FILE *f = std::fopen(path, "wb");
   if (!f) return fail(CFD_ERR_RUNTIME, std::string("cannot open ") + path);
   std::unique_ptr<FILE, int (*)(FILE *)> hold(f, std::fclose);
@@ -375,7 +383,8 @@ int cfd_checkpoint_load(cfd_ctx *ctx, const char *path) {
   if (h.nx_cal != c.g.nxc || h.ny_cal != c.g.nyc || h.nz_cal != c.g.nzc || h.G != (uint64_t)c.g.G ||
       h.N != (uint64_t)c.g.N || h.grid_hash != grid_hash(c))
     return fail(CFD_ERR_ARG, std::string(path) + " was written for another grid");
-  if (h.loop < 1 || h.solves < 0 || h.solves32 < 0 || !std::isfinite(h.time))
+  if (h.loop < 1 || h.solves < 0 || h.solves32 < 0 || !std::isfinite(h.time) || h.nan_resets < 0 ||
+      h.restarts < 0 || h.total_iters < 0)
     return fail(CFD_ERR_RUNTIME, std::string(path) + ": invalid step counters");
   // first pass: the whole file is read and its checksum verified before any
   // of the context's state is touched
@@ -409,6 +418,13 @@ int cfd_checkpoint_load(cfd_ctx *ctx, const char *path) {
   c.solves = h.solves;
   c.solves32 = h.solves32;
   c.p_from_x = h.p_from_x != 0;
+  c.nan_resets = h.nan_resets;
+  c.total_iters = h.total_iters;
+  KrylovScalars ks;
+  CFD_HIP(hipMemcpy(&ks, c.ks, sizeof(ks), hipMemcpyDeviceToHost));
+  ks.restarts = h.restarts;
+  CFD_HIP(hipMemcpy(c.ks, &ks, sizeof(ks), hipMemcpyHostToDevice));
+  adjoint_invalidate(c);
   return CFD_OK;
 }
 

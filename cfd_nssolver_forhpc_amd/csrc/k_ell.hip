@@ -15,6 +15,7 @@ namespace {
 // order, so every stored value is bit-identical to the reference's.
 struct PBC {
   int pd[6];
+  int periodic_z;  // z faces wrap: no closure term, the wrapped neighbour appended after zp
 };
 
 __global__ void __launch_bounds__(256) ell_assemble_kernel(DevGrid g, PBC b, int32_t *__restrict__ col,
@@ -36,8 +37,8 @@ __global__ void __launch_bounds__(256) ell_assemble_kernel(DevGrid g, PBC b, int
   if (i == xEnd && !b.pd[1]) aa = -xp;
   if (j == yBeg && !b.pd[2]) bb = -ym;
   if (j == yEnd && !b.pd[3]) bb = -yp;
-  if (k == zBeg && !b.pd[4]) cc = -zm;
-  if (k == zEnd && !b.pd[5]) cc = -zp;
+  if (k == zBeg && !b.pd[4] && !b.periodic_z) cc = -zm;
+  if (k == zEnd && !b.pd[5] && !b.periodic_z) cc = -zp;
   double vd = xm + xp;
   vd = vd + ym;
   vd = vd + yp;
@@ -58,6 +59,11 @@ __global__ void __launch_bounds__(256) ell_assemble_kernel(DevGrid g, PBC b, int
   if (j != yEnd) { cs[s] = (int32_t)(r + g.nzc); vs[s] = -yp * jr; ++s; }
   if (k != zBeg) { cs[s] = (int32_t)(r - 1); vs[s] = -zm * jr; ++s; }
   if (k != zEnd) { cs[s] = (int32_t)(r + 1); vs[s] = -zp * jr; ++s; }
+  // periodic z: the neighbour across the seam, after the regular entries
+  // (where the reference's periodic branch sets it, PressureMatrix.hpp:101-111,
+  // with the wrapped cell's own index)
+  if (b.periodic_z && k == zBeg) { cs[s] = (int32_t)(r + (g.nzc - 1)); vs[s] = -zm * jr; ++s; }
+  if (b.periodic_z && k == zEnd) { cs[s] = (int32_t)(r - (g.nzc - 1)); vs[s] = -zp * jr; ++s; }
   for (; s < KSLOTS; ++s) { cs[s] = (int32_t)(N - 1); vs[s] = 0.0; }
 #pragma unroll
   for (int q = 0; q < KSLOTS; ++q) {
@@ -148,8 +154,8 @@ __global__ void __launch_bounds__(256) spe_jp_kernel(int nxc, int nyc, int nzc, 
   if (i == nxc - 1 && !b.pd[1]) ijk -= xp[i];
   if (j == 0 && !b.pd[2]) ijk -= ym[j];
   if (j == nyc - 1 && !b.pd[3]) ijk -= yp[j];
-  if (k == 0 && !b.pd[4]) ijk -= zm[k];
-  if (k == nzc - 1 && !b.pd[5]) ijk -= zp[k];
+  if (k == 0 && !b.pd[4] && !b.periodic_z) ijk -= zm[k];
+  if (k == nzc - 1 && !b.pd[5] && !b.periodic_z) ijk -= zp[k];
   jp[ic] = 1.0 / ijk;
 }
 
@@ -204,6 +210,7 @@ __global__ void __launch_bounds__(256) cast32_kernel(const double *__restrict__ 
 PBC pbc_of(const Ctx &c) {
   PBC b;
   for (int q = 0; q < 6; ++q) b.pd[q] = c.prm.p_dirichlet[q];
+  b.periodic_z = c.prm.periodic_z;
   return b;
 }
 

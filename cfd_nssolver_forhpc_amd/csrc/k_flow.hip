@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, do
                                                         double *__restrict__ w1, const double *__restrict__ src,
                                                         int src_mode, double *__restrict__ pu,
                                                         double *__restrict__ pv, double *__restrict__ pw,
-                                                        const double *__restrict__ vis) {
+                                                        const double *__restrict__ vis, int kw_end) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   const Cell c = decode_interior(g, t);
@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, do
     else if (src_mode == 2 && ic < N) r += src[N + ic];
     v1[ic] = r;
   }
-  if (k < g.nz - GC - 1) {  // w (:360-434)
+  if (k < kw_end) {  // w (:360-434); with periodic z its last face too
     const double IT = (Dzs[k] * 0.5) / Dz[k];
     const double wc = w[ic];
     const double Wp = 0.5 * (w[zp] + wc);
@@ -225,7 +225,8 @@ __global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, do
 __global__ void __launch_bounds__(256) pred_add_kernel(DevGrid g, const double *__restrict__ pu,
                                                        const double *__restrict__ pv, const double *__restrict__ pw,
                                                        const double *__restrict__ src, double *__restrict__ u1,
-                                                       double *__restrict__ v1, double *__restrict__ w1) {
+                                                       double *__restrict__ v1, double *__restrict__ w1, int kw_end) {
+#pragma clang fp contract(off)  // the oracle's operation order, no fused multiply-adds
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   const Cell c = decode_interior(g, t);
@@ -233,7 +234,7 @@ __global__ void __launch_bounds__(256) pred_add_kernel(DevGrid g, const double *
   const int64_t N = g.N;
   if (c.i < g.nx - GC - 1) u1[ic] = pu[ic] + src[t];
   if (c.j < g.ny - GC - 1) v1[ic] = pv[ic] + src[N + t];
-  if (c.k < g.nz - GC - 1) w1[ic] = pw[ic] + src[2 * N + t];
+  if (c.k < kw_end) w1[ic] = pw[ic] + src[2 * N + t];
 }
 
 // T1 += source_termF at positions the predictor did not write (reference
@@ -260,6 +261,7 @@ __global__ void __launch_bounds__(256) source_quirk_fixup_kernel(DevGrid g, doub
 __global__ void __launch_bounds__(256) source_add_kernel(DevGrid g, double *__restrict__ u1, double *__restrict__ v1,
                                                          double *__restrict__ w1, const double *__restrict__ src,
                                                          int quirk) {
+#pragma clang fp contract(off)  // the oracle's operation order, no fused multiply-adds
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   int64_t m = t;
@@ -276,6 +278,7 @@ __global__ void __launch_bounds__(256) source_add_kernel(DevGrid g, double *__re
 __global__ void __launch_bounds__(256) rhs_kernel(DevGrid g, double ddt, const double *__restrict__ u1,
                                                   const double *__restrict__ v1, const double *__restrict__ w1,
                                                   const double *__restrict__ jp, double *__restrict__ b) {
+#pragma clang fp contract(off)  // the oracle's operation order, no fused multiply-adds
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   const Cell c = decode_interior(g, t);
@@ -301,8 +304,17 @@ __global__ void __launch_bounds__(256) scatter_kernel(DevGrid g, const double *_
 // velocity straight into T0 (the T0 <- T3 copy, physicalVariables.hpp:84-102).
 // gain scales the forcing (u_new = (1 - gain*eta_f) T2; 1 is the reference,
 // bit for bit); t2out, when set, records T2 for the discrete adjoint.
-__global__ void __launch_bounds__(256) dfib_update_kernel(DevGrid g, double dt, double gain, double *__restrict__ t2out,
-                                                          int mode,
+// Faces the update covers beyond the reference's ranges: the outflow face
+// of u (convective outflow, projected with p = 0 outside and free of
+// immersed-boundary forcing) and the last z face of w (periodic z, whose
+// +z pressure neighbour wraps to the first plane).
+struct UpdFaces {
+  int iu_end, kw_end;  // exclusive ends of the u / w ranges
+  int conv, wrapz;
+};
+
+__global__ void __launch_bounds__(256) dfib_update_kernel(DevGrid g, UpdFaces uf, double dt, double gain,
+                                                          double *__restrict__ t2out, int mode,
                                                           const double *__restrict__ eta,
                                                           const double *__restrict__ p,
                                                           const double *__restrict__ u1,
@@ -313,6 +325,7 @@ __global__ void __launch_bounds__(256) dfib_update_kernel(DevGrid g, double dt, 
                                                           double *__restrict__ w0, double *__restrict__ f,
                                                           double *__restrict__ src, double *part_drag,
                                                           double *part_max) {
+#pragma clang fp contract(off)  // the oracle's operation order, no fused multiply-adds
   double drag[2] = {0.0, 0.0};
   double mx[3] = {0.0, 0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -329,7 +342,7 @@ __global__ void __launch_bounds__(256) dfib_update_kernel(DevGrid g, double dt, 
     const double comp_in[3] = {u1[ic], v1[ic], w1[ic]};
     double *const outs3[3] = {u3, v3, w3};
     double *const outs0[3] = {u0, v0, w0};
-    const bool active[3] = {i < g.nx - GC - 1, j < g.ny - GC - 1, k < g.nz - GC - 1};
+    const bool active[3] = {i < uf.iu_end, j < g.ny - GC - 1, k < uf.kw_end};
     const int64_t step_ghost[3] = {sx, sy, 1};
     const int64_t step_int[3] = {(int64_t)g.nyc * g.nzc, g.nzc, 1};
     const double *Ds[3] = {g.Dxs, g.Dys, g.Dzs};
@@ -338,10 +351,18 @@ __global__ void __launch_bounds__(256) dfib_update_kernel(DevGrid g, double dt, 
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       if (active[q]) {
-        const double pP = from_x ? p[t + step_int[q]] : p[ic + step_ghost[q]];
+        // the +q pressure neighbour: across the outflow face p = 0 (Dirichlet),
+        // across the periodic z seam the first plane
+        const bool outflow = q == 0 && uf.conv && i == g.nx - 3;
+        const bool seam = q == 2 && uf.wrapz && k == g.nz - 3;
+        double pP;
+        if (from_x)
+          pP = outflow ? 0.0 : p[seam ? t - (g.nzc - 1) : t + step_int[q]];
+        else
+          pP = p[ic + step_ghost[q]];
         const double ds = Ds[q][idx[q]];
         const double T2 = comp_in[q] - dt * (pP - pc) / ds;
-        const double e = eC + (eta[ic + step_ghost[q]] - eC) * ((ds * 0.5) / Dd[q][idx[q]]);
+        const double e = outflow ? 0.0 : eC + (eta[ic + step_ghost[q]] - eC) * ((ds * 0.5) / Dd[q][idx[q]]);
         const double nv = eC * 0.0 + (1.0 - gain * e) * T2;
         const double fv = (nv - T2) / dt;
         f[q * N + t] = fv;
@@ -369,6 +390,7 @@ __global__ void __launch_bounds__(256) dfib_update_kernel(DevGrid g, double dt, 
 // source_termF += dt * f (RUN_CPU_PC_METHOD.hpp:393-401)
 __global__ void __launch_bounds__(256) source_accumulate_kernel(int64_t n3, double dt, const double *__restrict__ f,
                                                                 double *__restrict__ src) {
+#pragma clang fp contract(off)  // the oracle's operation order, no fused multiply-adds
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n3) src[t] += dt * f[t];
 }
@@ -380,6 +402,9 @@ struct BCArgs {
   double dir[6][3], num[6][3];
   int pd[6];
   const double *uin;  // face-0 u Dirichlet value per interior row j (NULL: dir[0][0])
+  int conv;           // face 1 is a convective outflow
+  double conv_udt;    // U dt of the convective outflow
+  int periodic_z;     // faces 4 and 5 are periodic
 };
 
 __device__ inline double nrm(const BCArgs &a, int b, int c, double x) { return a.dir[b][c] + a.num[b][c] * x; }
@@ -392,6 +417,7 @@ __global__ void __launch_bounds__(256) bc_x_kernel(DevGrid g, BCArgs a, int do_m
                                                    double *__restrict__ w, double *__restrict__ p,
                                                    double *__restrict__ u1, double *__restrict__ v1,
                                                    double *__restrict__ w1) {
+#pragma clang fp contract(off)  // the oracle's operation order, no fused multiply-adds
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nj = g.nyc, nk = g.nzc;
   if (t >= nj * nk) return;
@@ -405,9 +431,20 @@ __global__ void __launch_bounds__(256) bc_x_kernel(DevGrid g, BCArgs a, int do_m
     x = (a.uin ? a.uin[j - GC] : a.dir[0][0]) + a.num[0][0] * u[m2]; u[m1] = x; u[m0] = x;
     x = tan_(a, 0, 1, v[m2]); v[m1] = x; v[m0] = x;
     x = tan_(a, 0, 2, w[m2]); w[m1] = x; w[m0] = x;
-    x = nrm(a, 1, 0, u[n4]); u[n3] = x; u[n2] = x; u[n1] = x;
-    x = tan_(a, 1, 1, v[n3]); v[n2] = x; v[n1] = x;
-    x = tan_(a, 1, 2, w[n3]); w[n2] = x; w[n1] = x;
+    if (a.conv) {
+      // convective outflow, one implicit upwind step from the boundary's
+      // previous value: the outflow face of u from the last interior face
+      // (one cell width away), the ghosts of v and w from the last cell
+      // (one centre distance away); both ghost layers take the new value
+      const double su = a.conv_udt / g.Dx[nx - 3], st = a.conv_udt / g.Dxs[nx - 3];
+      x = (u[n3] + su * u[n4]) / (1.0 + su); u[n3] = x; u[n2] = x; u[n1] = x;
+      x = (v[n2] + st * v[n3]) / (1.0 + st); v[n2] = x; v[n1] = x;
+      x = (w[n2] + st * w[n3]) / (1.0 + st); w[n2] = x; w[n1] = x;
+    } else {
+      x = nrm(a, 1, 0, u[n4]); u[n3] = x; u[n2] = x; u[n1] = x;
+      x = tan_(a, 1, 1, v[n3]); v[n2] = x; v[n1] = x;
+      x = tan_(a, 1, 2, w[n3]); w[n2] = x; w[n1] = x;
+    }
   }
   if (do_p) {
     const double lo = a.pd[0] ? 0.0 : p[m2], hi = a.pd[1] ? 0.0 : p[n3];
@@ -452,6 +489,33 @@ __global__ void __launch_bounds__(256) bc_y_kernel(DevGrid g, BCArgs a, int do_m
 #pragma unroll
     for (int q = 0; q < 4; ++q) { u1[ms[q]] = u[ms[q]]; v1[ms[q]] = v[ms[q]]; w1[ms[q]] = w[ms[q]]; }
     v1[n3] = v[n3];
+  }
+}
+
+// periodic z: the two ghost layers on each side are the images of the
+// interior planes one period away (k = 0, 1 <- nz-4, nz-3; nz-2, nz-1 <- 2,
+// 3) for every (i, j) of the ghosted plane, after the x and y faces, so the
+// edge and corner ghosts wrap too; the same for u, v, w and p
+__global__ void __launch_bounds__(256) bc_z_periodic_kernel(DevGrid g, int do_main, int do_copy, int do_p,
+                                                            double *__restrict__ u, double *__restrict__ v,
+                                                            double *__restrict__ w, double *__restrict__ p,
+                                                            double *__restrict__ u1, double *__restrict__ v1,
+                                                            double *__restrict__ w1) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)g.nx * g.ny) return;
+  const int i = (int)(t / g.ny), j = (int)(t % g.ny);
+  const int nz = g.nz;
+  const int64_t base = icel(g, i, j, 0);
+  const int dst[4] = {0, 1, nz - 2, nz - 1}, src[4] = {nz - 4, nz - 3, 2, 3};
+  double *const f[3] = {u, v, w};
+  double *const f1[3] = {u1, v1, w1};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (do_main)
+      for (int c = 0; c < 3; ++c) f[c][base + dst[q]] = f[c][base + src[q]];
+    if (do_p) p[base + dst[q]] = p[base + src[q]];
+    if (do_copy)
+      for (int c = 0; c < 3; ++c) f1[c][base + dst[q]] = f[c][base + dst[q]];
   }
 }
 
@@ -640,6 +704,7 @@ __global__ void __launch_bounds__(256) maxdiff_kernel(DevGrid g, const double *_
 // before the last face, inc/dfib/updateUandF.hpp:107-111).
 struct DivMask {
   int pd[6];
+  int conv, wrapz;  // the outflow face and the z seam are projected faces
 };
 
 __global__ void __launch_bounds__(256) divergence_kernel(DevGrid g, DivMask dm, const double *__restrict__ u,
@@ -654,15 +719,18 @@ __global__ void __launch_bounds__(256) divergence_kernel(DevGrid g, DivMask dm, 
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < g.N; t += stride) {
     const Cell c = decode_interior(g, t);
     const int64_t ic = icel(g, c.i, c.j, c.k);
-    const double uP = (c.i == g.nx - 3 ? ub : u)[ic], uM = (c.i == GC ? ub : u)[ic - sx];
+    // periodic z: the -z face of the first plane is the last plane's +z face
+    const double uP = (c.i == g.nx - 3 && !dm.conv ? ub : u)[ic], uM = (c.i == GC ? ub : u)[ic - sx];
     const double vP = (c.j == g.ny - 3 ? vb : v)[ic], vM = (c.j == GC ? vb : v)[ic - sy];
-    const double wP = (c.k == g.nz - 3 ? wb : w)[ic], wM = (c.k == GC ? wb : w)[ic - 1];
+    const double wP = (c.k == g.nz - 3 && !dm.wrapz ? wb : w)[ic];
+    const double wM = dm.wrapz ? w[c.k == GC ? ic + (g.nzc - 1) : ic - 1] : (c.k == GC ? wb : w)[ic - 1];
     const double d = (uP - uM) / g.Dx[c.i] + (vP - vM) / g.Dy[c.j] + (wP - wM) / g.Dz[c.k];
     const double a = fabs(d);
     mx[0] = fmax(mx[0], a);
     sm[0] += d * d;
-    const bool dir_face = (dm.pd[0] && c.i == GC) || (dm.pd[1] && c.i == g.nx - 3) || (dm.pd[2] && c.j == GC) ||
-                          (dm.pd[3] && c.j == g.ny - 3) || (dm.pd[4] && c.k == GC) || (dm.pd[5] && c.k == g.nz - 3);
+    const bool dir_face = (dm.pd[0] && c.i == GC) || (dm.pd[1] && !dm.conv && c.i == g.nx - 3) ||
+                          (dm.pd[2] && c.j == GC) || (dm.pd[3] && c.j == g.ny - 3) ||
+                          (dm.pd[4] && !dm.wrapz && c.k == GC) || (dm.pd[5] && !dm.wrapz && c.k == g.nz - 3);
     const bool fluid = !dir_face && eta[ic] == 0.0 && eta[ic - sx] == 0.0 && eta[ic + sx] == 0.0 &&
                        eta[ic - sy] == 0.0 && eta[ic + sy] == 0.0 && eta[ic - 1] == 0.0 && eta[ic + 1] == 0.0;
     if (fluid) {
@@ -692,7 +760,19 @@ BCArgs bc_args(const Ctx &c) {
     a.pd[b] = c.prm.p_dirichlet[b];
   }
   a.uin = c.uin;
+  a.conv = c.prm.outflow_convective;
+  a.conv_udt = c.prm.outflow_speed * c.prm.dt;
+  a.periodic_z = c.prm.periodic_z;
   return a;
+}
+
+UpdFaces upd_faces(const Ctx &c) {
+  UpdFaces f;
+  f.conv = c.prm.outflow_convective;
+  f.wrapz = c.prm.periodic_z;
+  f.iu_end = c.g.nx - GC - (f.conv ? 0 : 1);
+  f.kw_end = c.g.nz - GC - (f.wrapz ? 0 : 1);
+  return f;
 }
 
 }  // namespace
@@ -732,6 +812,7 @@ hipError_t launch_predictor(Ctx &c, const double *u0, const double *v0, const do
   const int nb = blocks_for(c.g.N, 256, 1 << 30);
   double *k0 = keep ? keep[0] : nullptr, *k1 = keep ? keep[1] : nullptr, *k2 = keep ? keep[2] : nullptr;
   const double nu = 1.0 / c.prm.Re;
+  const int kw_end = upd_faces(c).kw_end;
   const double *vis = nullptr;
   if (c.prm.les) {  // the eddy viscosity of this T0 (RUN_CPU_PC_METHOD.hpp:298-300, 449-451)
     if (!c.viseff) {
@@ -746,19 +827,19 @@ hipError_t launch_predictor(Ctx &c, const double *u0, const double *v0, const do
   }
   if (c.prm.convection == CFD_CONV_LUD)
     predictor_kernel<CFD_CONV_LUD><<<nb, 256, 0, c.stream>>>(c.g, nu, c.prm.dt, u0, v0, w0, u1, v1, w1, src,
-                                                             src_mode, k0, k1, k2, vis);
+                                                             src_mode, k0, k1, k2, vis, kw_end);
   else if (c.prm.convection == CFD_CONV_UD)
     predictor_kernel<CFD_CONV_UD><<<nb, 256, 0, c.stream>>>(c.g, nu, c.prm.dt, u0, v0, w0, u1, v1, w1, src,
-                                                            src_mode, k0, k1, k2, vis);
+                                                            src_mode, k0, k1, k2, vis, kw_end);
   else
     predictor_kernel<CFD_CONV_QUICK><<<nb, 256, 0, c.stream>>>(c.g, nu, c.prm.dt, u0, v0, w0, u1, v1, w1, src,
-                                                               src_mode, k0, k1, k2, vis);
+                                                               src_mode, k0, k1, k2, vis, kw_end);
   return hipGetLastError();
 }
 
 hipError_t launch_pred_add(Ctx &c, double *const keep[3], const double *src, double *u1, double *v1, double *w1) {
   pred_add_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g, keep[0], keep[1], keep[2], src, u1,
-                                                                         v1, w1);
+                                                                         v1, w1, upd_faces(c).kw_end);
   return hipGetLastError();
 }
 
@@ -789,7 +870,7 @@ hipError_t launch_dfib_update_into(Ctx &c, int mode, const double *p_or_x, doubl
   const int nblk = (mode & (UPD_DRAG | UPD_MAXDIFF)) ? blocks_for(c.g.N, 256, RED_MAX_BLOCKS)
                                                      : blocks_for(c.g.N, 256, 1 << 30);
   dfib_update_kernel<<<nblk, 256, 0, c.stream>>>(
-      c.g, c.prm.dt, c.gain, t2out, mode, c.f[CFD_ETA], p_or_x, c.f[CFD_T1_U], c.f[CFD_T1_V], c.f[CFD_T1_W], c.f[CFD_T3_U],
+      c.g, upd_faces(c), c.prm.dt, c.gain, t2out, mode, c.f[CFD_ETA], p_or_x, c.f[CFD_T1_U], c.f[CFD_T1_V], c.f[CFD_T1_W], c.f[CFD_T3_U],
       c.f[CFD_T3_V], c.f[CFD_T3_W], c.f[CFD_T0_U], c.f[CFD_T0_V], c.f[CFD_T0_W], c.f[CFD_F], c.f[CFD_SRC],
       red_part_host(c, RS_DRAG), red_part_host(c, RS_MAX));
   hipError_t e = hipGetLastError();
@@ -825,8 +906,28 @@ hipError_t launch_bc(Ctx &c, bool main, bool copy, bool pressure) {
   bc_y_kernel<<<blocks_for((int64_t)g.nxc * g.nzc, 256, 1 << 30), 256, 0, c.stream>>>(g, a, m, cp, pp, u, v, w, p,
                                                                                        u1, v1, w1);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  bc_z_kernel<<<blocks_for((int64_t)g.nxc * g.nyc, 256, 1 << 30), 256, 0, c.stream>>>(g, a, m, cp, pp, u, v, w, p,
-                                                                                       u1, v1, w1);
+  if (c.prm.periodic_z)
+    bc_z_periodic_kernel<<<blocks_for((int64_t)g.nx * g.ny, 256, 1 << 30), 256, 0, c.stream>>>(g, m, cp, pp, u, v, w,
+                                                                                              p, u1, v1, w1);
+  else
+    bc_z_kernel<<<blocks_for((int64_t)g.nxc * g.nyc, 256, 1 << 30), 256, 0, c.stream>>>(g, a, m, cp, pp, u, v, w, p,
+                                                                                         u1, v1, w1);
+  return hipGetLastError();
+}
+
+hipError_t launch_periodic_refresh_p(Ctx &c, double *p) {
+  if (!c.prm.periodic_z) return hipSuccess;
+  const DevGrid &g = c.g;
+  bc_z_periodic_kernel<<<blocks_for((int64_t)g.nx * g.ny, 256, 1 << 30), 256, 0, c.stream>>>(
+      g, 0, 0, 1, nullptr, nullptr, nullptr, p, nullptr, nullptr, nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_periodic_refresh(Ctx &c, double *u, double *v, double *w) {
+  if (!c.prm.periodic_z) return hipSuccess;
+  const DevGrid &g = c.g;
+  bc_z_periodic_kernel<<<blocks_for((int64_t)g.nx * g.ny, 256, 1 << 30), 256, 0, c.stream>>>(
+      g, 1, 0, 0, u, v, w, nullptr, nullptr, nullptr, nullptr);
   return hipGetLastError();
 }
 
@@ -863,6 +964,8 @@ hipError_t launch_divergence(Ctx &c, const double *const vel[3], const double *c
   double *pmax = red_part_host(c, RS_MISC), *psum = red_part_host(c, RS_NORM);
   DivMask dm;
   for (int f = 0; f < 6; ++f) dm.pd[f] = c.prm.p_dirichlet[f];
+  dm.conv = c.prm.outflow_convective;
+  dm.wrapz = c.prm.periodic_z;
   divergence_kernel<<<nblk, 256, 0, c.stream>>>(c.g, dm, vel[0], vel[1], vel[2], bnd[0], bnd[1], bnd[2],
                                                 c.f[CFD_ETA], pmax, psum);
   hipError_t e = hipGetLastError();

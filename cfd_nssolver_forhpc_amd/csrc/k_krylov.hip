@@ -278,11 +278,38 @@ __global__ void __launch_bounds__(256) spmv_dot2_pc_kernel(OpArgs a, const Krylo
   if (ks->done) return;
   double acc[2] = {0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) {
-    const T y = op_apply<T>(a, r, sh);
-    tv[r] = y;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.N; base += stride) {
+    const T y = op_apply_block<T>(a, base, sh);
+    const int64_t r = base + threadIdx.x;
+    if (r >= a.N) continue;
+    ntstore(y, tv + r);
     acc[0] += (double)y * (double)s1[r];
     acc[1] += (double)y * (double)y;
+  }
+  block_partials<2>(acc, part);
+}
+
+// the same over SPE plane columns (FP64)
+__global__ void __launch_bounds__(256) spe_spmv_dot2_pc_kernel(OpArgs a, SpeTiles T,
+                                                               const KrylovScalars *__restrict__ ks,
+                                                               const double *__restrict__ sh,
+                                                               const double *__restrict__ s1, double *__restrict__ tv,
+                                                               double *part) {
+  if (ks->done) return;
+  double acc[2] = {0.0, 0.0};
+  const SpeColumn c = spe_column(a, T);
+  if (c.ok) {
+    double y[SPE_PLANES], xc[SPE_PLANES];
+    const int np = spe_column_rows(a, c, sh, y, xc);
+    const int64_t nynz = (int64_t)a.nyc * a.nzc, col = (int64_t)c.j * a.nzc + c.k;
+#pragma unroll
+    for (int p = 0; p < SPE_PLANES; ++p)
+      if (p < np) {
+        const int64_t r = (int64_t)(c.i0 + p) * nynz + col;
+        ntstore(y[p], tv + r);
+        acc[0] += y[p] * s1[r];
+        acc[1] += y[p] * y[p];
+      }
   }
   block_partials<2>(acc, part);
 }
@@ -498,15 +525,29 @@ int launch_pc_iters(Ctx &c, const OpArgs &a, const PVecs<T> &v, KrylovScalars *k
   const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
   double *p1 = red_part_host(c, RS_SPMV_DOT), *p2 = red_part_host(c, RS_SPMV_DOT2), *p3 = red_part_host(c, RS_XR);
   const int nbu = blocks_for(N, 2 * 256, RED_MAX_BLOCKS);  // two rows a thread, as in launch_iters
+  // the SPE operator's kernels over plane columns (FP64), as in launch_iters
+  const SpeTiles tiles = spe_tiles(c.g.nxc, c.g.nyc, c.g.nzc);
+  const bool spe = std::is_same<T, double>::value && a.op == CFD_OP_SPE && tiles.blocks <= RED_MAX_BLOCKS;
+  const int nbs = spe ? (int)tiles.blocks : nb;
   for (int it = 0; it < nit; ++it) {
     int rc = precondition<T>(c, v.p1, v.ph, ks, pc32);
     if (rc) return rc;
-    spmv_dot_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, ks, v.ph, v.v, v.r0, p1);
-    alpha_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p1, nb, ks);
+    if (spe)
+      spe_spmv_dot_kernel<<<nbs, 256, 0, c.stream>>>(a, tiles, ks, reinterpret_cast<const double *>(v.ph),
+                                                     reinterpret_cast<double *>(v.v),
+                                                     reinterpret_cast<const double *>(v.r0), p1);
+    else
+      spmv_dot_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, ks, v.ph, v.v, v.r0, p1);
+    alpha_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p1, nbs, ks);
     s_update_kernel<T><<<nbu, 256, 0, c.stream>>>(N, ks, v.r1, v.v, v.s1);
     if ((rc = precondition<T>(c, v.s1, v.sh, ks, pc32))) return rc;
-    spmv_dot2_pc_kernel<T><<<nb, 256, 0, c.stream>>>(a, ks, v.sh, v.s1, v.t, p2);
-    omega_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p2, nb, ks);
+    if (spe)
+      spe_spmv_dot2_pc_kernel<<<nbs, 256, 0, c.stream>>>(a, tiles, ks, reinterpret_cast<const double *>(v.sh),
+                                                         reinterpret_cast<const double *>(v.s1),
+                                                         reinterpret_cast<double *>(v.t), p2);
+    else
+      spmv_dot2_pc_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, ks, v.sh, v.s1, v.t, p2);
+    omega_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p2, nbs, ks);
     xr_update_pc_kernel<T><<<nbu, 256, 0, c.stream>>>(N, ks, v.x, v.ph, v.sh, v.s1, v.t, v.r1, v.r0, p3);
     conv_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p3, nbu, ks);
     p_update_kernel<T><<<nbu, 256, 0, c.stream>>>(N, ks, v.r1, v.p1, v.v, v.r0);
@@ -653,6 +694,22 @@ int defect_correction(Ctx &c, const double *b, double *x, double tol, double tol
 hipError_t launch_bicg_iteration(Ctx &c) {
   Vecs<double> v{c.f[CFD_VEC_B], c.r0, c.r1, c.p1, c.s1, c.ap, c.as, c.f[CFD_RHS], nullptr, nullptr};
   return launch_iters<double>(c, op_args(c, c.prm.op), v, c.ks32, 1);
+}
+
+// Timing of the iteration's first kernel alone (K1: ap = A p with the
+// partial dot ap . r0), the iteration's largest share, on the same scratch
+// vectors and scalars as launch_bicg_iteration
+hipError_t launch_spmv_dot(Ctx &c) {
+  const OpArgs a = op_args(c, c.prm.op);
+  const int nb = blocks_for(c.g.N, 256, RED_MAX_BLOCKS);
+  const SpeTiles tiles = spe_tiles(c.g.nxc, c.g.nyc, c.g.nzc);
+  if (a.op == CFD_OP_SPE && tiles.blocks <= RED_MAX_BLOCKS)
+    spe_spmv_dot_kernel<<<(int)tiles.blocks, 256, 0, c.stream>>>(a, tiles, c.ks32, c.p1, c.ap, c.r0,
+                                                                 red_part_host(c, RS_SPMV_DOT));
+  else
+    spmv_dot_kernel<double><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, c.ks32, c.p1, c.ap, c.r0,
+                                                                    red_part_host(c, RS_SPMV_DOT));
+  return hipGetLastError();
 }
 
 hipError_t launch_bicg_timing_setup(Ctx &c) {

@@ -781,8 +781,11 @@ int mg_ensure(Ctx &c, bool fp32) {
         dhi[a].push_back((*Dsh[a])[I]);
       }
     }
-    const bool dl[3] = {c.prm.p_dirichlet[0] != 0, c.prm.p_dirichlet[2] != 0, c.prm.p_dirichlet[4] != 0};
-    const bool dh[3] = {c.prm.p_dirichlet[1] != 0, c.prm.p_dirichlet[3] != 0, c.prm.p_dirichlet[5] != 0};
+    // periodic z: the cycle closes the z faces (Neumann) and preconditions
+    // the wrapped operator the Krylov method applies
+    const bool pz = c.prm.periodic_z != 0;
+    const bool dl[3] = {c.prm.p_dirichlet[0] != 0, c.prm.p_dirichlet[2] != 0, !pz && c.prm.p_dirichlet[4] != 0};
+    const bool dh[3] = {c.prm.p_dirichlet[1] != 0, c.prm.p_dirichlet[3] != 0, !pz && c.prm.p_dirichlet[5] != 0};
     std::vector<double> last_arr[3];
     for (;;) {
       MGLevel L;

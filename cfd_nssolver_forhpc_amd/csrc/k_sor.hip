@@ -202,7 +202,10 @@ __global__ void sorp_setup_kernel(KrylovScalars *ks, double crit, int itmax) {
 }
 
 // one colour of a sweep; partial max |pNEW - p| per block
-__global__ void __launch_bounds__(256) sorp_sweep_kernel(DevGrid g, int colour, double omega,
+// wrapz: periodic z, the z neighbours of the first and last planes are the
+// live pressure across the seam (not ghosts), which keeps the red-black
+// colouring valid for an even nz
+__global__ void __launch_bounds__(256) sorp_sweep_kernel(DevGrid g, int colour, double omega, int wrapz,
                                                          const double *__restrict__ cf6, double *__restrict__ p,
                                                          const KrylovScalars *__restrict__ ks, double *part) {
 #pragma clang fp contract(off)
@@ -229,8 +232,10 @@ __global__ void __launch_bounds__(256) sorp_sweep_kernel(DevGrid g, int colour, 
     const double coef = Dy[j] * Dz[k] / Dxs[i] + Dy[j] * Dz[k] / Dxs[i - 1] + Dx[i] * Dz[k] / Dys[j] +
                         Dx[i] * Dz[k] / Dys[j - 1] + Dx[i] * Dy[j] / Dzs[k] + Dx[i] * Dy[j] / Dzs[k - 1];
     const double cf7 = -1.0 / coef;
-    const double pxp = p[ic + sx], pxm = p[ic - sx], pyp = p[ic + sy], pym = p[ic - sy], pzp = p[ic + 1],
-                 pzm = p[ic - 1], pc = p[ic];
+    const int64_t izp = (wrapz && k == g.nz - 3) ? ic - (g.nzc - 1) : ic + 1;
+    const int64_t izm = (wrapz && k == GC) ? ic + (g.nzc - 1) : ic - 1;
+    const double pxp = p[ic + sx], pxm = p[ic - sx], pyp = p[ic + sy], pym = p[ic - sy], pzp = p[izp],
+                 pzm = p[izm], pc = p[ic];
     const int64_t row = ((int64_t)ic_ * g.nyc + jc) * g.nzc + kc;
     const double pNEW = (-pxp * cf0 - pxm * cf1 - pyp * cf2 - pym * cf3 - pzp * cf4 - pzm * cf5 + cf6[row]) * cf7;
     mx[0] = fmax(mx[0], fabs(pNEW - pc));
@@ -272,8 +277,8 @@ int sor_pipeline_solve(Ctx &c, double omega, double crit, int itmax, int *iters,
   int prev_it = 0;
   for (;;) {
     for (int s = 0; s < batch; ++s) {
-      sorp_sweep_kernel<<<nb, 256, 0, c.stream>>>(g, 0, omega, cf6, c.f[CFD_P], c.ks, part0);
-      sorp_sweep_kernel<<<nb, 256, 0, c.stream>>>(g, 1, omega, cf6, c.f[CFD_P], c.ks, part1);
+      sorp_sweep_kernel<<<nb, 256, 0, c.stream>>>(g, 0, omega, c.prm.periodic_z, cf6, c.f[CFD_P], c.ks, part0);
+      sorp_sweep_kernel<<<nb, 256, 0, c.stream>>>(g, 1, omega, c.prm.periodic_z, cf6, c.f[CFD_P], c.ks, part1);
       sorp_finalize_kernel<<<1, FIN_THREADS, 0, c.stream>>>(part0, part1, nb, c.ks);
     }
     CFD_HIP(hipGetLastError());

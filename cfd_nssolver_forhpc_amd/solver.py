@@ -348,6 +348,12 @@ class Simulation:
         check(self.lib.cfd_adjoint_forward(self.h, nsteps, avg_from, ckpt_every, C.byref(J)), "cfd_adjoint_forward")
         return J.value
 
+    def adjoint_generation(self) -> int:
+        """Number of the objective run adjoint_backward would differentiate (0: none)."""
+        g = C.c_ulonglong()
+        check(self.lib.cfd_adjoint_generation(self.h, C.byref(g)), "cfd_adjoint_generation")
+        return g.value
+
     def adjoint_backward(self):
         """(dJ/du_in, dJ/dgain, stats) of the last adjoint_forward run."""
         g = np.zeros(self.grid.ncal[1])

@@ -143,6 +143,24 @@ typedef struct cfd_params {
   int convection;        /* face values of the predictor's convection term: CFD_CONV_* */
   int les;               /* 1: Smagorinsky eddy viscosity (C_s 0.18) added to nu in the predictor's diffusion
                             (smagorinskyModel.hpp:30-79, TERBULENCE_SMAGORINSKY); 0: off, the reference's default */
+  /* boundary conditions the reference's Dirichlet/Neumann tables cannot
+     express (no reference counterpart; 0 keeps the table semantics):
+     periodic_z      1: velocity and pressure periodic across the z faces
+                     (faces 4 and 5 of the tables are ignored; nz_cal even,
+                     at least 4).  The last z face of w is a real face,
+                     predicted and projected like the others; the pressure
+                     operator wraps in z (ELL and matrix-free; not CSR).
+     outflow_convective 1: face 1 (x+) is a convective outflow
+                     d(phi)/dt + U d(phi)/dx = 0 for u, v and w, applied by
+                     BC_staggered_main after each step as an implicit upwind
+                     step phi_b = (phi_b + s phi_in) / (1 + s), s = U dt / h;
+                     the outflow face of u is projected with the Dirichlet
+                     pressure p = 0 there (p_dirichlet[1] must be 1) and
+                     carries no immersed-boundary forcing.
+     outflow_speed   U of the convective outflow (> 0) */
+  int periodic_z;
+  int outflow_convective;
+  double outflow_speed;
 } cfd_params;
 
 /* per-timestep report; pass index 0..inner_loops-1 are predictor passes, the last is the corrector */
@@ -353,6 +371,13 @@ int cfd_adjoint_forward(cfd_ctx *ctx, int nsteps, int avg_from, int ckpt_every, 
    run, by the transposed timestep with recomputation from the checkpoints;
    leaves the state where the forward run ended */
 int cfd_adjoint_backward(cfd_ctx *ctx, double *grad_uin, size_t n, double *grad_gain, cfd_adjoint_stats *stats);
+/* the number of the objective run cfd_adjoint_backward would differentiate
+   (each cfd_adjoint_forward counts one), 0 when there is none: changing the
+   state or the parameters after the run (cfd_step, cfd_init_state,
+   cfd_upload, cfd_set_params, cfd_set_loop, cfd_checkpoint_load, the gain or
+   the inflow profile) discards it, and cfd_adjoint_backward then returns
+   CFD_ERR_STATE instead of mixing the old trajectory with the new values */
+int cfd_adjoint_generation(cfd_ctx *ctx, unsigned long long *generation);
 
 /* ---- output and restart files (host I/O) -------------------------------- */
 /* Time_cDcL.dat writer modes */
@@ -388,7 +413,8 @@ int cfd_plot3d_read_q(const char *path, const cfd_grid_desc *grid, int *nblock, 
 /* An exact FP64 snapshot of what a timestep carries into the next: T0, T1,
    T3, p and eta (ghosted), the source term (3 N), X_result and the solver's
    NaN-reset history (N), and the loop, time, solve count (the restart rule's
-   timestep_) and pressure-location flag.  No reference counterpart: its only
+   timestep_), pressure-location flag and bicgstabRe2's counters (pResets_,
+   numRestarts_, totalNumIter_), so cfd_get_solver_counters continues too.  No reference counterpart: its only
    restart reads the float Plot3D file back, and that block is commented out
    (inc/run/RUN_CPU_PC_METHOD.hpp:98-111).  The operator, cfd_params, the
    forcing gain and the inflow profile are not in the file: a run resumes in a
@@ -426,7 +452,7 @@ int cfd_bicgstab_ell_noalloc(const int32_t *col, const double *val, const double
    nrep launches bracketed by HIP events on the context stream:
    what = 0 ELL SpMV FP64, 1 ELL SpMV FP32, 2 SPE SpMV FP64, 3 RB-SOR sweep,
    4 BiCGSTAB iteration (FP64), 5 predictor, 6 copy kernel (2*N doubles moved),
-   7 CSR SpMV FP64 */
+   7 CSR SpMV FP64, 8 the iteration's SpMV-dot kernel (ap = A p, partial ap . r0) */
 int cfd_time_kernel(cfd_ctx *ctx, int what, int nrep, double *ms_avg);
 
 #ifdef __cplusplus

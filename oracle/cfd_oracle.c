@@ -106,60 +106,70 @@ int orc_axis_sml(int ncal, double L, double lS, int nS, int off_nS, double *X, d
 }
 
 /* Stretched axis for the large configuration (no reference counterpart):
-   uniform spacing h0 inside a band of width `band` centred on xc, widths
-   growing by `ratio` per cell outside it, capped at hcap chosen by bisection
-   so that the ncal widths sum to L; the last width absorbs the rounding. */
-static double stretch_sum(int ncal, int ic0, int ic1, double h0, double ratio, double hcap, double *w) {
-  double s = 0.0;
-  for (int c = 0; c < ncal; ++c) {
-    int dist = c < ic0 ? ic0 - c : (c > ic1 ? c - ic1 : 0);
-    double h = h0 * pow(ratio, (double)dist);
-    if (h > hcap) h = hcap;
-    if (w) w[c] = h;
-    s += h;
+   tanh clustering around xc, restated from the library.  Cell width over the
+   node index offset xi: dx/dxi = H - (H - h0)/2 [tanh((xi + a)/w) -
+   tanh((xi - a)/w)], a = band / (2 h0); node positions
+   x(xi) = xc + H xi - (H - h0)/2 w [lncosh((xi + a)/w) - lncosh((xi - a)/w)];
+   node i at xi0 + i with xi0 (node 0 at 0) and H (node ncal at L) by
+   bisection, w the narrowest step with neighbouring-width ratio <= ratio. */
+static double lncosh_(double t) {
+  const double a = fabs(t);
+  return a + log1p(exp(-2.0 * a)) - log(2.0);
+}
+typedef struct { double h0, a, w, H, xc; } tanh_axis;
+static double tanh_f(const tanh_axis *t, double xi) {
+  return t->H * xi - 0.5 * (t->H - t->h0) * t->w * (lncosh_((xi + t->a) / t->w) - lncosh_((xi - t->a) / t->w));
+}
+static double tanh_xi0(const tanh_axis *t, int ncal) {
+  double lo = -(double)ncal - 1.0, hi = 0.0;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (t->xc + tanh_f(t, mid) > 0.0) hi = mid;
+    else lo = mid;
   }
-  return s;
+  return 0.5 * (lo + hi);
+}
+static double tanh_fit(tanh_axis *t, int ncal, double L, double *xi0_out) {
+  double lo = t->h0, hi = L, xi0 = 0.0;
+  for (int it = 0; it < 200; ++it) {
+    t->H = 0.5 * (lo + hi);
+    xi0 = tanh_xi0(t, ncal);
+    if (t->xc + tanh_f(t, xi0 + ncal) > L) hi = t->H;
+    else lo = t->H;
+  }
+  t->H = 0.5 * (lo + hi);
+  xi0 = tanh_xi0(t, ncal);
+  double rmax = 1.0, prev = tanh_f(t, xi0 + 1.0) - tanh_f(t, xi0);
+  for (int c = 1; c < ncal; ++c) {
+    const double h = tanh_f(t, xi0 + c + 1.0) - tanh_f(t, xi0 + c);
+    const double r = h / prev > prev / h ? h / prev : prev / h;
+    if (r > rmax) rmax = r;
+    prev = h;
+  }
+  *xi0_out = xi0;
+  return rmax;
 }
 
 int orc_axis_tanh(int ncal, double L, double xc, double h0, double band, double ratio,
                   double *X, double *D, double *Ds) {
+  if (ncal < 4 || !(L > 0) || !(h0 > 0) || !(ratio > 1.0) || h0 * ncal >= L || !(xc > 0 && xc < L) || !(band > 0))
+    return -1;
   const int n = ncal + 2 * GC;
-  if (ncal < 4 || !(L > 0) || !(h0 > 0) || !(ratio >= 1.0)) return -1;
-  if (h0 * ncal > L) return -1; /* cannot reach L with widths >= h0 */
-  int nb = (int)floor(band / h0 + 0.5);
-  if (nb < 1) nb = 1;
-  if (nb > ncal) nb = ncal;
-  /* place the band so that its centre is at xc when the far field is uniform */
-  double *w = (double *)malloc(sizeof(double) * (size_t)ncal);
-  if (!w) return -1;
-  int best_ic0 = 0;
-  double lo = h0, hi = L; /* hcap */
-  for (int pass = 0; pass < 2; ++pass) {
-    lo = h0; hi = L;
-    for (int it = 0; it < 200; ++it) {
-      double mid = 0.5 * (lo + hi);
-      double s = stretch_sum(ncal, best_ic0, best_ic0 + nb - 1, h0, ratio, mid, NULL);
-      if (s > L) hi = mid; else lo = mid;
-    }
-    if (pass == 0) {
-      /* choose ic0 so the band centre lands closest to xc */
-      stretch_sum(ncal, 0, nb - 1, h0, ratio, lo, w);
-      double best = 1e300;
-      for (int ic0 = 0; ic0 + nb <= ncal; ++ic0) {
-        stretch_sum(ncal, ic0, ic0 + nb - 1, h0, ratio, lo, w);
-        double x = 0.0;
-        for (int c = 0; c < ic0; ++c) x += w[c];
-        double centre = x + 0.5 * nb * h0;
-        if (fabs(centre - xc) < best) { best = fabs(centre - xc); best_ic0 = ic0; }
-      }
-    }
+  tanh_axis t = {h0, 0.5 * band / h0, 1.0, h0, xc};
+  double xi0 = 0.0, wlo = 0.25, whi = (double)ncal;
+  t.w = whi;
+  if (tanh_fit(&t, ncal, L, &xi0) > ratio) return -1;
+  for (int it = 0; it < 60; ++it) {
+    t.w = 0.5 * (wlo + whi);
+    if (tanh_fit(&t, ncal, L, &xi0) > ratio) wlo = t.w;
+    else whi = t.w;
   }
-  stretch_sum(ncal, best_ic0, best_ic0 + nb - 1, h0, ratio, lo, w);
-  memset(X, 0, sizeof(double) * (size_t)(n + 1));
+  t.w = whi;
+  tanh_fit(&t, ncal, L, &xi0);
+  for (int q = 0; q <= n; ++q) X[q] = 0.0;
+  for (int c = 1; c < ncal; ++c) X[GC + c] = xc + tanh_f(&t, xi0 + c);
   X[GC] = 0.0;
-  for (int c = 0; c < ncal; ++c) X[GC + c + 1] = X[GC + c] + w[c];
   X[GC + ncal] = L;
-  free(w);
   orc_axis_gc_dx(n, X, D, Ds);
   return 0;
 }
@@ -360,9 +370,27 @@ void orc_smagorinsky(const cfd_grid_desc *g, const double *u, const double *v, c
       }
 }
 
+static void predictor_impl(const cfd_grid_desc *g, int scheme, double nu, double dt, const double *vis, int pz,
+                           const double *u, const double *v, const double *w,
+                           double *u1, double *v1, double *w1);
+
 void orc_predictor_les(const cfd_grid_desc *g, int scheme, double nu, double dt, const double *vis,
                        const double *u, const double *v, const double *w,
                        double *u1, double *v1, double *w1) {
+  predictor_impl(g, scheme, nu, dt, vis, 0, u, v, w, u1, v1, w1);
+}
+
+/* the predictor with the boundary mode of P (periodic z: the last z face of
+   w is a real face and is predicted too) */
+void orc_predictor_bl(const cfd_grid_desc *g, const cfd_params *P, const double *vis,
+                      const double *u, const double *v, const double *w,
+                      double *u1, double *v1, double *w1) {
+  predictor_impl(g, P->convection, 1.0 / P->Re, P->dt, vis, P->periodic_z, u, v, w, u1, v1, w1);
+}
+
+static void predictor_impl(const cfd_grid_desc *g, int scheme, double nu, double dt, const double *vis, int pz,
+                           const double *u, const double *v, const double *w,
+                           double *u1, double *v1, double *w1) {
   const dims_t d = dims_of(g);
   const size_t sx = d.nzny, sy = (size_t)d.nz, sz = 1;
   const double *Dx = g->Dx, *Dy = g->Dy, *Dz = g->Dz, *Dxs = g->Dxs, *Dys = g->Dys, *Dzs = g->Dzs;
@@ -428,11 +456,11 @@ void orc_predictor_les(const cfd_grid_desc *g, int scheme, double nu, double dt,
         v1[ic] = v[ic] + convection + difussion;
       }
 
-  /* w: k in [2, nz-3) (:357-434) */
+  /* w: k in [2, nz-3) (:357-434); periodic z: [2, nz-2) */
 #pragma omp parallel for schedule(static)
   for (int i = ib; i < ie; ++i)
     for (int j = jb; j < je; ++j)
-      for (int k = kb; k < ke - 1; ++k) {
+      for (int k = kb; k < ke - 1 + (pz ? 1 : 0); ++k) {
         const size_t ic = IC(d, i, j, k);
         const size_t xm = ic - sx, xp = ic + sx, ym = ic - sy, yp = ic + sy, zm = ic - sz, zp = ic + sz;
         const size_t xmm = ic - 2 * sx, xpp = ic + 2 * sx, ymm = ic - 2 * sy, ypp = ic + 2 * sy;
@@ -514,6 +542,14 @@ void orc_rhs(const cfd_grid_desc *g, double dt, const double *jp,
    neighbours, padding (N-1, 0.0).  Pressure faces marked Dirichlet keep the
    neighbour coefficient in the diagonal (p = 0 in the ghost). */
 int orc_ell_assemble(const cfd_grid_desc *g, const int pd[6], int32_t *col, double *val, double *jp) {
+  return orc_ell_assemble_bl(g, pd, 0, col, val, jp);
+}
+
+/* periodic z (pz): no closure term on the z faces; the neighbour across the
+   seam is set after the regular entries, where the reference's periodic
+   branch sets it (PressureMatrix.hpp:101-111), with the wrapped cell's own
+   index */
+int orc_ell_assemble_bl(const cfd_grid_desc *g, const int pd[6], int pz, int32_t *col, double *val, double *jp) {
   const dims_t d = dims_of(g);
   const size_t N = d.N;
   if (N == 0 || N > 0x7fffffffULL) return -1;
@@ -530,8 +566,8 @@ int orc_ell_assemble(const cfd_grid_desc *g, const int pd[6], int32_t *col, doub
         if (i == xEnd && !pd[1]) aa = -xp;
         if (j == yBeg && !pd[2]) bb = -ym;
         if (j == yEnd && !pd[3]) bb = -yp;
-        if (k == zBeg && !pd[4]) cc = -zm;
-        if (k == zEnd && !pd[5]) cc = -zp;
+        if (k == zBeg && !pd[4] && !pz) cc = -zm;
+        if (k == zEnd && !pd[5] && !pz) cc = -zp;
         const size_t row = ICAL(d, i, j, k);
         const double vdiag = xm + xp + ym + yp + zm + zp + aa + bb + cc;
         const double jr = 1. / vdiag;
@@ -544,6 +580,8 @@ int orc_ell_assemble(const cfd_grid_desc *g, const int pd[6], int32_t *col, doub
         if (j != yEnd) { col[(size_t)s * N + row] = (int32_t)ICAL(d, i, j + 1, k); val[(size_t)s * N + row] = -yp * jr; ++s; }
         if (k != zBeg) { col[(size_t)s * N + row] = (int32_t)ICAL(d, i, j, k - 1); val[(size_t)s * N + row] = -zm * jr; ++s; }
         if (k != zEnd) { col[(size_t)s * N + row] = (int32_t)ICAL(d, i, j, k + 1); val[(size_t)s * N + row] = -zp * jr; ++s; }
+        if (pz && k == zBeg) { col[(size_t)s * N + row] = (int32_t)ICAL(d, i, j, zEnd); val[(size_t)s * N + row] = -zm * jr; ++s; }
+        if (pz && k == zEnd) { col[(size_t)s * N + row] = (int32_t)ICAL(d, i, j, zBeg); val[(size_t)s * N + row] = -zp * jr; ++s; }
         for (; s < 7; ++s) { col[(size_t)s * N + row] = (int32_t)(N - 1); val[(size_t)s * N + row] = 0.0; }
       }
   return 0;
@@ -615,6 +653,10 @@ void orc_spmv_csr(size_t N, const int32_t *ptr, const int32_t *col, const double
 }
 
 void orc_spe_setup(const cfd_grid_desc *g, const int pd[6], double *coef, double *jp) {
+  orc_spe_setup_bl(g, pd, 0, coef, jp);
+}
+
+void orc_spe_setup_bl(const cfd_grid_desc *g, const int pd[6], int pz, double *coef, double *jp) {
   const dims_t d = dims_of(g);
   double *xm = coef, *xp = xm + d.nxc, *ym = xp + d.nxc, *yp = ym + d.nyc, *zm = yp + d.nyc, *zp = zm + d.nzc;
   for (int i = 0; i < d.nxc; ++i) {
@@ -639,8 +681,8 @@ void orc_spe_setup(const cfd_grid_desc *g, const int pd[6], double *coef, double
         if (i == d.nxc - 1 && !pd[1]) ijk -= xp[i];
         if (j == 0 && !pd[2]) ijk -= ym[j];
         if (j == d.nyc - 1 && !pd[3]) ijk -= yp[j];
-        if (k == 0 && !pd[4]) ijk -= zm[k];
-        if (k == d.nzc - 1 && !pd[5]) ijk -= zp[k];
+        if (k == 0 && !pd[4] && !pz) ijk -= zm[k];
+        if (k == d.nzc - 1 && !pd[5] && !pz) ijk -= zp[k];
         jp[ic] = 1.0 / ijk;
       }
 }
@@ -648,6 +690,12 @@ void orc_spe_setup(const cfd_grid_desc *g, const int pd[6], double *coef, double
 /* SPE_matrix1::spmv_ompJP (SPE_sparseMatrix1.hpp:126-144); each
    sum += (-c*jp) * x contracts to an FMA in the reference build */
 void orc_spmv_spe(const cfd_grid_desc *g, const double *coef, const double *jp, const double *x, double *y) {
+  orc_spmv_spe_bl(g, 0, coef, jp, x, y);
+}
+
+/* periodic z (pz): the z neighbours of the first and last planes wrap */
+void orc_spmv_spe_bl(const cfd_grid_desc *g, int pz, const double *coef, const double *jp, const double *x,
+                     double *y) {
   const dims_t d = dims_of(g);
   const double *xm = coef, *xp = xm + d.nxc, *ym = xp + d.nxc, *yp = ym + d.nyc, *zm = yp + d.nyc, *zp = zm + d.nzc;
   const size_t nz_ = (size_t)d.nzc, nynz_ = (size_t)d.nyc * d.nzc;
@@ -661,8 +709,10 @@ void orc_spmv_spe(const cfd_grid_desc *g, const double *coef, const double *jp, 
         if (i != 0) sum = fma(-xm[i] * J, x[ic - nynz_], sum);
         if (j != 0) sum = fma(-ym[j] * J, x[ic - nz_], sum);
         if (k != 0) sum = fma(-zm[k] * J, x[ic - 1], sum);
+        else if (pz) sum = fma(-zm[k] * J, x[ic + (nz_ - 1)], sum);
         sum += x[ic];
         if (k != d.nzc - 1) sum = fma(-zp[k] * J, x[ic + 1], sum);
+        else if (pz) sum = fma(-zp[k] * J, x[ic - (nz_ - 1)], sum);
         if (j != d.nyc - 1) sum = fma(-yp[j] * J, x[ic + nz_], sum);
         if (i != d.nxc - 1) sum = fma(-xp[i] * J, x[ic + nynz_], sum);
         y[ic] = sum;
@@ -682,19 +732,40 @@ void orc_pressure_scatter(const cfd_grid_desc *g, const double *X, double *p) {
    gain scales the forcing, u_new = (1 - gain*eta_f) T2 (gain = 1 is the
    reference, bit for bit); t2, when not NULL, receives the intermediate
    velocity T2 = u* - dt dp/dx (3*N, interior order) for the adjoint. */
+static void dfib_update_impl(const cfd_grid_desc *g, double dt, double gain, int conv, int pz, const double *eta,
+                             const double *p, const double *u1, const double *v1, const double *w1,
+                             double *u3, double *v3, double *w3, double *f, double *t2);
+
 void orc_dfib_update_ex(const cfd_grid_desc *g, double dt, double gain, const double *eta, const double *p,
                         const double *u1, const double *v1, const double *w1,
                         double *u3, double *v3, double *w3, double *f, double *t2) {
+  dfib_update_impl(g, dt, gain, 0, 0, eta, p, u1, v1, w1, u3, v3, w3, f, t2);
+}
+
+/* the update with the boundary mode of P: a convective outflow projects the
+   outflow face of u too (with the ghost pressure, 0 there) and gives it no
+   immersed-boundary forcing; periodic z projects the last z face of w with
+   the ghost pressure of the plane across the seam */
+void orc_dfib_update_bl(const cfd_grid_desc *g, const cfd_params *P, double gain, const double *eta, const double *p,
+                        const double *u1, const double *v1, const double *w1,
+                        double *u3, double *v3, double *w3, double *f, double *t2) {
+  dfib_update_impl(g, P->dt, gain, P->outflow_convective, P->periodic_z, eta, p, u1, v1, w1, u3, v3, w3, f, t2);
+}
+
+static void dfib_update_impl(const cfd_grid_desc *g, double dt, double gain, int conv, int pz, const double *eta,
+                             const double *p, const double *u1, const double *v1, const double *w1,
+                             double *u3, double *v3, double *w3, double *f, double *t2) {
   const dims_t d = dims_of(g);
   const size_t N = d.N;
   const double uS = 0.0, vS = 0.0, wS = 0.0;
 #pragma omp parallel for schedule(static)
-  for (int i = GC; i < d.nx - GC - 1; ++i)
+  for (int i = GC; i < d.nx - GC - 1 + (conv ? 1 : 0); ++i)
     for (int j = GC; j < d.ny - GC; ++j)
       for (int k = GC; k < d.nz - GC; ++k) {
         const size_t ic = IC(d, i, j, k), icP = IC(d, i + 1, j, k);
         const double T2u = u1[ic] - dt * (p[icP] - p[ic]) / g->Dxs[i];
-        const double e = eta[ic] + (eta[icP] - eta[ic]) * ((g->Dxs[i] * 0.5) / g->Dx[i]);
+        const double e = (conv && i == d.nx - 3) ? 0.0
+                                                 : eta[ic] + (eta[icP] - eta[ic]) * ((g->Dxs[i] * 0.5) / g->Dx[i]);
         u3[ic] = eta[ic] * uS + (1.0 - gain * e) * T2u;
         f[ICAL(d, i, j, k)] = (u3[ic] - T2u) / dt;
         if (t2) t2[ICAL(d, i, j, k)] = T2u;
@@ -713,7 +784,7 @@ void orc_dfib_update_ex(const cfd_grid_desc *g, double dt, double gain, const do
 #pragma omp parallel for schedule(static)
   for (int i = GC; i < d.nx - GC; ++i)
     for (int j = GC; j < d.ny - GC; ++j)
-      for (int k = GC; k < d.nz - GC - 1; ++k) {
+      for (int k = GC; k < d.nz - GC - 1 + (pz ? 1 : 0); ++k) {
         const size_t ic = IC(d, i, j, k), icP = IC(d, i, j, k + 1);
         const double T2w = w1[ic] - dt * (p[icP] - p[ic]) / g->Dzs[k];
         const double e = eta[ic] + (eta[icP] - eta[ic]) * ((g->Dzs[k] * 0.5) / g->Dz[k]);
@@ -740,8 +811,54 @@ void orc_dfib_update(const cfd_grid_desc *g, double dt, const double *eta, const
    on faces marked Dirichlet. */
 /* uin (NULL: the table value) replaces Dir[0][0] by a profile over y,
    uin[j - 2] for interior row j: the inflow the adjoint differentiates. */
+static void bc_main_impl(const cfd_grid_desc *g, const double Dir[6][3], const double Num[6][3], const int pd[6],
+                         const double *uin, int conv, double conv_udt, int pz, double *u, double *v, double *w,
+                         double *p);
+
 void orc_bc_main_ex(const cfd_grid_desc *g, const double Dir[6][3], const double Num[6][3],
                     const int pd[6], const double *uin, double *u, double *v, double *w, double *p) {
+  bc_main_impl(g, Dir, Num, pd, uin, 0, 0.0, 0, u, v, w, p);
+}
+
+/* BC_staggered_main with the boundary mode of P (no reference counterpart
+   for these two): a convective outflow on face 1 -- one implicit upwind step
+   phi_b = (phi_b + s phi_in) / (1 + s), s = U dt / h, for the outflow face
+   of u (h = Dx of the last cell, phi_in the last interior face) and the
+   ghosts of v and w (h = Dxs of the last cell, phi_in the last cell); both
+   ghost layers take the new value -- and periodic z: the two ghost layers on
+   each z side are the images one period away, over whole planes, after the
+   x and y faces, for u, v, w and p */
+void orc_bc_main_bl(const cfd_grid_desc *g, const cfd_params *P, const double *uin, double *u, double *v, double *w,
+                    double *p) {
+  bc_main_impl(g, P->bc_dir, P->bc_num, P->p_dirichlet, uin, P->outflow_convective, P->outflow_speed * P->dt,
+               P->periodic_z, u, v, w, p);
+}
+
+/* the z ghost layers of a field from their periodic images, whole planes */
+static void periodic_z_fill(const dims_t d, double *a) {
+  const int nz = d.nz;
+  for (int i = 0; i < d.nx; ++i)
+    for (int j = 0; j < d.ny; ++j) {
+      const size_t b = IC(d, i, j, 0);
+      a[b + 0] = a[b + nz - 4];
+      a[b + 1] = a[b + nz - 3];
+      a[b + nz - 2] = a[b + 2];
+      a[b + nz - 1] = a[b + 3];
+    }
+}
+
+void orc_periodic_refresh(const cfd_grid_desc *g, double *u, double *v, double *w) {
+  const dims_t d = dims_of(g);
+  periodic_z_fill(d, u);
+  periodic_z_fill(d, v);
+  periodic_z_fill(d, w);
+}
+
+void orc_periodic_refresh_p(const cfd_grid_desc *g, double *p) { periodic_z_fill(dims_of(g), p); }
+
+static void bc_main_impl(const cfd_grid_desc *g, const double Dir[6][3], const double Num[6][3], const int pd[6],
+                         const double *uin, int conv, double conv_udt, int pz, double *u, double *v, double *w,
+                         double *p) {
   const dims_t d = dims_of(g);
   const int nx = d.nx, ny = d.ny, nz = d.nz;
 #define NRM(b, c, x) (Dir[b][c] + Num[b][c] * (x))
@@ -755,9 +872,19 @@ void orc_bc_main_ex(const cfd_grid_desc *g, const double Dir[6][3], const double
     }
   for (int j = GC; j < ny - GC; ++j)
     for (int k = GC; k < nz - GC; ++k) { /* BC_1 */
-      u[IC(d, nx - 1, j, k)] = u[IC(d, nx - 2, j, k)] = u[IC(d, nx - 3, j, k)] = NRM(1, 0, u[IC(d, nx - 4, j, k)]);
-      v[IC(d, nx - 1, j, k)] = v[IC(d, nx - 2, j, k)] = TAN(1, 1, v[IC(d, nx - 3, j, k)]);
-      w[IC(d, nx - 1, j, k)] = w[IC(d, nx - 2, j, k)] = TAN(1, 2, w[IC(d, nx - 3, j, k)]);
+      if (conv) {
+        const double su = conv_udt / g->Dx[nx - 3], st = conv_udt / g->Dxs[nx - 3];
+        const double xu = (u[IC(d, nx - 3, j, k)] + su * u[IC(d, nx - 4, j, k)]) / (1.0 + su);
+        u[IC(d, nx - 1, j, k)] = u[IC(d, nx - 2, j, k)] = u[IC(d, nx - 3, j, k)] = xu;
+        const double xv = (v[IC(d, nx - 2, j, k)] + st * v[IC(d, nx - 3, j, k)]) / (1.0 + st);
+        v[IC(d, nx - 1, j, k)] = v[IC(d, nx - 2, j, k)] = xv;
+        const double xw = (w[IC(d, nx - 2, j, k)] + st * w[IC(d, nx - 3, j, k)]) / (1.0 + st);
+        w[IC(d, nx - 1, j, k)] = w[IC(d, nx - 2, j, k)] = xw;
+      } else {
+        u[IC(d, nx - 1, j, k)] = u[IC(d, nx - 2, j, k)] = u[IC(d, nx - 3, j, k)] = NRM(1, 0, u[IC(d, nx - 4, j, k)]);
+        v[IC(d, nx - 1, j, k)] = v[IC(d, nx - 2, j, k)] = TAN(1, 1, v[IC(d, nx - 3, j, k)]);
+        w[IC(d, nx - 1, j, k)] = w[IC(d, nx - 2, j, k)] = TAN(1, 2, w[IC(d, nx - 3, j, k)]);
+      }
     }
   for (int i = GC; i < nx - GC; ++i)
     for (int k = GC; k < nz - GC; ++k) { /* BC_2 */
@@ -771,18 +898,24 @@ void orc_bc_main_ex(const cfd_grid_desc *g, const double Dir[6][3], const double
       u[IC(d, i, ny - 1, k)] = u[IC(d, i, ny - 2, k)] = TAN(3, 0, u[IC(d, i, ny - 3, k)]);
       w[IC(d, i, ny - 1, k)] = w[IC(d, i, ny - 2, k)] = TAN(3, 2, w[IC(d, i, ny - 3, k)]);
     }
-  for (int i = GC; i < nx - GC; ++i)
-    for (int j = GC; j < ny - GC; ++j) { /* BC_4 */
-      u[IC(d, i, j, 0)] = u[IC(d, i, j, 1)] = TAN(4, 0, u[IC(d, i, j, 2)]);
-      v[IC(d, i, j, 0)] = v[IC(d, i, j, 1)] = TAN(4, 1, v[IC(d, i, j, 2)]);
-      w[IC(d, i, j, 0)] = w[IC(d, i, j, 1)] = NRM(4, 2, w[IC(d, i, j, 2)]);
-    }
-  for (int i = GC; i < nx - GC; ++i)
-    for (int j = GC; j < ny - GC; ++j) { /* BC_5 */
-      w[IC(d, i, j, nz - 1)] = w[IC(d, i, j, nz - 2)] = w[IC(d, i, j, nz - 3)] = NRM(5, 2, w[IC(d, i, j, nz - 4)]);
-      u[IC(d, i, j, nz - 1)] = u[IC(d, i, j, nz - 2)] = TAN(5, 0, u[IC(d, i, j, nz - 3)]);
-      v[IC(d, i, j, nz - 1)] = v[IC(d, i, j, nz - 2)] = TAN(5, 1, v[IC(d, i, j, nz - 3)]);
-    }
+  if (pz) {
+    periodic_z_fill(d, u);
+    periodic_z_fill(d, v);
+    periodic_z_fill(d, w);
+  } else {
+    for (int i = GC; i < nx - GC; ++i)
+      for (int j = GC; j < ny - GC; ++j) { /* BC_4 */
+        u[IC(d, i, j, 0)] = u[IC(d, i, j, 1)] = TAN(4, 0, u[IC(d, i, j, 2)]);
+        v[IC(d, i, j, 0)] = v[IC(d, i, j, 1)] = TAN(4, 1, v[IC(d, i, j, 2)]);
+        w[IC(d, i, j, 0)] = w[IC(d, i, j, 1)] = NRM(4, 2, w[IC(d, i, j, 2)]);
+      }
+    for (int i = GC; i < nx - GC; ++i)
+      for (int j = GC; j < ny - GC; ++j) { /* BC_5 */
+        w[IC(d, i, j, nz - 1)] = w[IC(d, i, j, nz - 2)] = w[IC(d, i, j, nz - 3)] = NRM(5, 2, w[IC(d, i, j, nz - 4)]);
+        u[IC(d, i, j, nz - 1)] = u[IC(d, i, j, nz - 2)] = TAN(5, 0, u[IC(d, i, j, nz - 3)]);
+        v[IC(d, i, j, nz - 1)] = v[IC(d, i, j, nz - 2)] = TAN(5, 1, v[IC(d, i, j, nz - 3)]);
+      }
+  }
 #undef NRM
 #undef TAN
   if (!p) return;
@@ -796,6 +929,10 @@ void orc_bc_main_ex(const cfd_grid_desc *g, const double Dir[6][3], const double
       p[IC(d, i, 0, k)] = p[IC(d, i, 1, k)] = pd[2] ? 0.0 : p[IC(d, i, 2, k)];
       p[IC(d, i, ny - 1, k)] = p[IC(d, i, ny - 2, k)] = pd[3] ? 0.0 : p[IC(d, i, ny - 3, k)];
     }
+  if (pz) {
+    periodic_z_fill(d, p);
+    return;
+  }
   for (int i = GC; i < nx - GC; ++i)
     for (int j = GC; j < ny - GC; ++j) {
       p[IC(d, i, j, 0)] = p[IC(d, i, j, 1)] = pd[4] ? 0.0 : p[IC(d, i, j, 2)];
@@ -809,8 +946,23 @@ void orc_bc_main(const cfd_grid_desc *g, const double Dir[6][3], const double Nu
 }
 
 /* BC_staggered_copy (updateBC.hpp:6-109): boundary faces old -> current */
+static void bc_copy_impl(const cfd_grid_desc *g, int pz, const double *u0, const double *v0, const double *w0,
+                         double *u1, double *v1, double *w1);
+
 void orc_bc_copy(const cfd_grid_desc *g, const double *u0, const double *v0, const double *w0,
                  double *u1, double *v1, double *w1) {
+  bc_copy_impl(g, 0, u0, v0, w0, u1, v1, w1);
+}
+
+/* periodic z: the z ghost layers are copied over whole planes, and the last
+   z face of w, a real face, is not */
+void orc_bc_copy_bl(const cfd_grid_desc *g, const cfd_params *P, const double *u0, const double *v0,
+                    const double *w0, double *u1, double *v1, double *w1) {
+  bc_copy_impl(g, P->periodic_z, u0, v0, w0, u1, v1, w1);
+}
+
+static void bc_copy_impl(const cfd_grid_desc *g, int pz, const double *u0, const double *v0, const double *w0,
+                         double *u1, double *v1, double *w1) {
   const dims_t d = dims_of(g);
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   for (int j = GC; j < ny - GC; ++j)
@@ -833,6 +985,17 @@ void orc_bc_copy(const cfd_grid_desc *g, const double *u0, const double *v0, con
       const size_t c = IC(d, i, ny - 3, k);
       v1[c] = v0[c];
     }
+  if (pz) {
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j) {
+        const int gk[4] = {0, 1, nz - 2, nz - 1};
+        for (int q = 0; q < 4; ++q) {
+          const size_t m = IC(d, i, j, gk[q]);
+          u1[m] = u0[m]; v1[m] = v0[m]; w1[m] = w0[m];
+        }
+      }
+    return;
+  }
   for (int i = GC; i < nx - GC; ++i)
     for (int j = GC; j < ny - GC; ++j) {
       const int gk[4] = {0, 1, nz - 2, nz - 1};
@@ -902,6 +1065,7 @@ struct orc_bicg {
   const double *val;
   const cfd_grid_desc *g;
   const double *coef, *jp;
+  int pz; /* SPE: periodic z */
   double *r0, *r1, *p1, *s1, *ap, *as, *px1, *px2;
   int iters_max;
   double tol, restart_factor, restart_tol;
@@ -909,7 +1073,7 @@ struct orc_bicg {
 };
 
 static void bicg_matvec(orc_bicg *s, const double *x, double *y) {
-  if (s->op == CFD_OP_SPE) orc_spmv_spe(s->g, s->coef, s->jp, x, y);
+  if (s->op == CFD_OP_SPE) orc_spmv_spe_bl(s->g, s->pz, s->coef, s->jp, x, y);
   else if (s->op == CFD_OP_CSR) orc_spmv_csr(s->N, s->ptr, s->col, s->val, x, y);
   else orc_spmv_ell(s->N, 7, s->col, s->val, x, y);
 }
@@ -945,6 +1109,7 @@ void orc_bicg_destroy(orc_bicg *s) {
 
 void orc_bicg_set_tolerance(orc_bicg *s, double tol) { s->tol = tol; s->restart_tol = s->restart_factor * tol; }
 void orc_bicg_set_maxit(orc_bicg *s, int maxit) { s->iters_max = maxit; }
+void orc_bicg_set_periodic_z(orc_bicg *s, int pz) { s->pz = pz; }
 int orc_bicg_restarts(const orc_bicg *s) { return s->restarts; }
 int orc_bicg_resets(const orc_bicg *s) { return s->resets; }
 
@@ -1095,6 +1260,13 @@ double orc_rbsor_csr(size_t N, const cfd_grid_desc *g, const int32_t *ptr, const
    iters = sweeps - 1.  Converges to the lexicographic sweep's fixed point. */
 double orc_sor_rb_vw(const cfd_grid_desc *g, double dt, double omega, double crit, int itmax,
                      const double *u1, const double *v1, const double *w1, double *p, int *iters) {
+  return orc_sor_rb_vw_bl(g, 0, dt, omega, crit, itmax, u1, v1, w1, p, iters);
+}
+
+/* periodic z (pz): the z neighbours of the first and last planes are the
+   live pressure across the seam, not ghosts */
+double orc_sor_rb_vw_bl(const cfd_grid_desc *g, int pz, double dt, double omega, double crit, int itmax,
+                        const double *u1, const double *v1, const double *w1, double *p, int *iters) {
   const dims_t d = dims_of(g);
   const size_t N = d.N;
   double *cf6 = (double *)malloc(sizeof(double) * (N ? N : 1));
@@ -1128,9 +1300,10 @@ double orc_sor_rb_vw(const cfd_grid_desc *g, double dt, double omega, double cri
                                 Dx[i] * Dz[k] / Dys[j - 1] + Dx[i] * Dy[j] / Dzs[k] + Dx[i] * Dy[j] / Dzs[k - 1];
             const double cf7 = -1.0 / coef;
             const double pc = p[icel];
+            const int kp = (pz && k == d.nz - 3) ? GC : k + 1, km = (pz && k == GC) ? d.nz - 3 : k - 1;
             const double pNEW = (-p[IC(d, i + 1, j, k)] * cf0 - p[IC(d, i - 1, j, k)] * cf1 -
                                  p[IC(d, i, j + 1, k)] * cf2 - p[IC(d, i, j - 1, k)] * cf3 -
-                                 p[IC(d, i, j, k + 1)] * cf4 - p[IC(d, i, j, k - 1)] * cf5 +
+                                 p[IC(d, i, j, kp)] * cf4 - p[IC(d, i, j, km)] * cf5 +
                                  cf6[((size_t)ic * d.nyc + jc) * d.nzc + kc]) * cf7;
             const double ch = fabs(pNEW - pc);
             if (ch > rmax) rmax = ch;
@@ -1150,6 +1323,11 @@ double orc_sor_rb_vw(const cfd_grid_desc *g, double dt, double omega, double cri
    simuClass::loop == 1, a loop in which the driver never calls it) */
 double orc_sor_lex(const cfd_grid_desc *g, double dt, double omega, double crit, int itmax,
                    const double *u1, const double *v1, const double *w1, double *p, int *iters) {
+  return orc_sor_lex_bl(g, 0, dt, omega, crit, itmax, u1, v1, w1, p, iters);
+}
+
+double orc_sor_lex_bl(const cfd_grid_desc *g, int pz, double dt, double omega, double crit, int itmax,
+                      const double *u1, const double *v1, const double *w1, double *p, int *iters) {
   const dims_t d = dims_of(g);
   const size_t N = d.N;
   double *cf = (double *)malloc(sizeof(double) * 8 * (N ? N : 1));
@@ -1183,9 +1361,10 @@ double orc_sor_lex(const cfd_grid_desc *g, double dt, double omega, double crit,
         for (int k = GC; k < d.nz - GC; ++k) {
           const size_t icel = IC(d, i, j, k);
           const size_t ii8 = ICAL(d, i, j, k) * 8;
+          const int kp = (pz && k == d.nz - 3) ? GC : k + 1, km = (pz && k == GC) ? d.nz - 3 : k - 1;
           const double pNEW = (-p[IC(d, i + 1, j, k)] * cf[ii8] - p[IC(d, i - 1, j, k)] * cf[ii8 + 1] -
                                p[IC(d, i, j + 1, k)] * cf[ii8 + 2] - p[IC(d, i, j - 1, k)] * cf[ii8 + 3] -
-                               p[IC(d, i, j, k + 1)] * cf[ii8 + 4] - p[IC(d, i, j, k - 1)] * cf[ii8 + 5] +
+                               p[IC(d, i, j, kp)] * cf[ii8 + 4] - p[IC(d, i, j, km)] * cf[ii8 + 5] +
                                cf[ii8 + 6]) * cf[ii8 + 7];
           const double pChange = fabs(pNEW - p[icel]);
           p[icel] += omega * (pNEW - p[icel]);
@@ -1614,9 +1793,8 @@ int orc_sim_init_state(orc_sim *s, double noise_amp, uint64_t seed) {
   memset(s->X, 0, sizeof(double) * d.N);
   memset(s->f, 0, sizeof(double) * 3 * d.N);
   memset(s->src, 0, sizeof(double) * 3 * d.N);
-  orc_bc_main_ex(&s->g, s->prm.bc_dir, s->prm.bc_num, s->prm.p_dirichlet, s->uin, s->T0[0], s->T0[1], s->T0[2],
-                 s->p);
-  orc_bc_copy(&s->g, s->T0[0], s->T0[1], s->T0[2], s->T1[0], s->T1[1], s->T1[2]);
+  orc_bc_main_bl(&s->g, &s->prm, s->uin, s->T0[0], s->T0[1], s->T0[2], s->p);
+  orc_bc_copy_bl(&s->g, &s->prm, s->T0[0], s->T0[1], s->T0[2], s->T1[0], s->T1[1], s->T1[2]);
   s->loop = 1;
   s->time = 0.0;
   return 0;
@@ -1624,8 +1802,8 @@ int orc_sim_init_state(orc_sim *s, double noise_amp, uint64_t seed) {
 
 int orc_sim_setup(orc_sim *s) {
   const cfd_params *P = &s->prm;
-  if (orc_ell_assemble(&s->g, P->p_dirichlet, s->col, s->val, s->jp) != 0) return -1;
-  orc_spe_setup(&s->g, P->p_dirichlet, s->spe_coef, s->spe_jp);
+  if (orc_ell_assemble_bl(&s->g, P->p_dirichlet, P->periodic_z, s->col, s->val, s->jp) != 0) return -1;
+  orc_spe_setup_bl(&s->g, P->p_dirichlet, P->periodic_z, s->spe_coef, s->spe_jp);
   if (P->op == CFD_OP_SPE) memcpy(s->jp, s->spe_jp, sizeof(double) * s->d.N); /* gA.jp = set_Jp_pc() */
   orc_bicg_destroy(s->bicg);
   if (P->op == CFD_OP_CSR) {
@@ -1637,6 +1815,7 @@ int orc_sim_setup(orc_sim *s) {
     s->bicg = orc_bicg_create(s->d.N, P->op, s->col, s->val, &s->g, s->spe_coef, s->spe_jp);
   }
   if (!s->bicg) return -1;
+  orc_bicg_set_periodic_z(s->bicg, P->periodic_z);
   orc_bicg_set_tolerance(s->bicg, P->tol);
   orc_bicg_set_maxit(s->bicg, P->maxit);
   if (P->ib_on) orc_eta_cylinder_z(&s->g, P->ib_cx, P->ib_cy, P->ib_r, P->ib_nsub, s->eta);
@@ -1648,15 +1827,16 @@ static void sim_pass(orc_sim *s, int pass, int accumulate, cfd_step_stats *st) {
   const cfd_params *P = &s->prm;
   const cfd_grid_desc *g = &s->g;
   if (P->les) orc_smagorinsky(g, s->T0[0], s->T0[1], s->T0[2], s->vis);  /* RUN_CPU_PC_METHOD.hpp:298-300 */
-  orc_predictor_les(g, P->convection, 1.0 / P->Re, P->dt, P->les ? s->vis : NULL, s->T0[0], s->T0[1], s->T0[2],
-                    s->T1[0], s->T1[1], s->T1[2]);
+  orc_predictor_bl(g, P, P->les ? s->vis : NULL, s->T0[0], s->T0[1], s->T0[2], s->T1[0], s->T1[1], s->T1[2]);
   orc_source_add(g, P->source_index_quirk, s->src, s->T1[0], s->T1[1], s->T1[2]);
+  if (P->periodic_z) orc_periodic_refresh(g, s->T1[0], s->T1[1], s->T1[2]); /* before the RHS reads the ghosts */
   int it = -1;
   double rel = 0.0;
   if (s->loop > 1) {
     if (P->solver != CFD_SOLVER_SOR_PIPELINE) orc_rhs(g, P->dt, s->jp, s->T1[0], s->T1[1], s->T1[2], s->b);
     if (P->solver == CFD_SOLVER_SOR_PIPELINE) /* SorPipeLine: in place on the ghosted p, no X */
-      rel = orc_sor_rb_vw(g, P->dt, P->sor_omega, P->tol, P->maxit, s->T1[0], s->T1[1], s->T1[2], s->p, &it);
+      rel = orc_sor_rb_vw_bl(g, P->periodic_z, P->dt, P->sor_omega, P->tol, P->maxit, s->T1[0], s->T1[1],
+                             s->T1[2], s->p, &it);
     else if (P->solver == CFD_SOLVER_RBSOR && P->op == CFD_OP_CSR)
       rel = orc_rbsor_csr(s->d.N, g, s->csr_ptr, s->csr_col, s->csr_val, s->b, s->X, P->sor_omega, P->tol,
                           P->maxit, P->sor_check, &it);
@@ -1666,8 +1846,9 @@ static void sim_pass(orc_sim *s, int pass, int accumulate, cfd_step_stats *st) {
     else
       rel = orc_bicg_solve(s->bicg, s->b, s->X, &it);
     if (P->solver != CFD_SOLVER_SOR_PIPELINE) orc_pressure_scatter(g, s->X, s->p);
+    if (P->periodic_z) orc_periodic_refresh_p(g, s->p); /* the update's +z neighbour across the seam */
   }
-  orc_dfib_update_ex(g, P->dt, s->gain, s->eta, s->p, s->T1[0], s->T1[1], s->T1[2], s->T3[0], s->T3[1], s->T3[2],
+  orc_dfib_update_bl(g, P, s->gain, s->eta, s->p, s->T1[0], s->T1[1], s->T1[2], s->T3[0], s->T3[1], s->T3[2],
                      s->f, s->t2_rec ? s->t2_rec + (size_t)pass * 3 * s->d.N : NULL);
   if (accumulate) orc_source_accumulate(g, P->dt, s->f, s->src);
   if (st && pass < CFD_MAX_PASSES) {
@@ -1685,8 +1866,27 @@ static void sim_pass(orc_sim *s, int pass, int accumulate, cfd_step_stats *st) {
    out[3] the same over the fluid cells: eta zero at the cell and its six
    neighbours, and no face on a Dirichlet pressure boundary (pd), whose
    velocity update_UandF leaves uncorrected (inc/dfib/updateUandF.hpp:107-111) */
+static void divergence_impl(const cfd_grid_desc *g, const int pd[6], int conv, int pz, const double *u,
+                            const double *v, const double *w, const double *ub, const double *vb, const double *wb,
+                            const double *eta, double out[4]);
+
 void orc_divergence(const cfd_grid_desc *g, const int pd[6], const double *u, const double *v, const double *w,
                     const double *ub, const double *vb, const double *wb, const double *eta, double out[4]) {
+  divergence_impl(g, pd, 0, 0, u, v, w, ub, vb, wb, eta, out);
+}
+
+/* with the boundary mode of P: the outflow face (convective outflow) and
+   the last z face (periodic z, also the -z face of the first plane) are
+   projected faces, read from (u, v, w) */
+void orc_divergence_bl(const cfd_grid_desc *g, const cfd_params *P, const double *u, const double *v,
+                       const double *w, const double *ub, const double *vb, const double *wb, const double *eta,
+                       double out[4]) {
+  divergence_impl(g, P->p_dirichlet, P->outflow_convective, P->periodic_z, u, v, w, ub, vb, wb, eta, out);
+}
+
+static void divergence_impl(const cfd_grid_desc *g, const int pd[6], int conv, int pz, const double *u,
+                            const double *v, const double *w, const double *ub, const double *vb, const double *wb,
+                            const double *eta, double out[4]) {
   const dims_t d = dims_of(g);
   const size_t sx = (size_t)d.ny * d.nz, sy = (size_t)d.nz;
   double mx[2] = {0.0, 0.0}, sm[2] = {0.0, 0.0};
@@ -1694,15 +1894,16 @@ void orc_divergence(const cfd_grid_desc *g, const int pd[6], const double *u, co
     for (int j = GC; j < d.ny - GC; ++j)
       for (int k = GC; k < d.nz - GC; ++k) {
         const size_t ic = IC(d, i, j, k);
-        const double uP = (i == d.nx - 3 ? ub : u)[ic], uM = (i == GC ? ub : u)[ic - sx];
+        const double uP = (i == d.nx - 3 && !conv ? ub : u)[ic], uM = (i == GC ? ub : u)[ic - sx];
         const double vP = (j == d.ny - 3 ? vb : v)[ic], vM = (j == GC ? vb : v)[ic - sy];
-        const double wP = (k == d.nz - 3 ? wb : w)[ic], wM = (k == GC ? wb : w)[ic - 1];
+        const double wP = (k == d.nz - 3 && !pz ? wb : w)[ic];
+        const double wM = pz ? w[k == GC ? IC(d, i, j, d.nz - 3) : ic - 1] : (k == GC ? wb : w)[ic - 1];
         const double dv = (uP - uM) / g->Dx[i] + (vP - vM) / g->Dy[j] + (wP - wM) / g->Dz[k];
         const double a = fabs(dv);
         if (a > mx[0]) mx[0] = a;
         sm[0] += dv * dv;
-        const int dir_face = (pd[0] && i == GC) || (pd[1] && i == d.nx - 3) || (pd[2] && j == GC) ||
-                             (pd[3] && j == d.ny - 3) || (pd[4] && k == GC) || (pd[5] && k == d.nz - 3);
+        const int dir_face = (pd[0] && i == GC) || (pd[1] && !conv && i == d.nx - 3) || (pd[2] && j == GC) ||
+                             (pd[3] && j == d.ny - 3) || (pd[4] && !pz && k == GC) || (pd[5] && !pz && k == d.nz - 3);
         if (!dir_face && eta[ic] == 0.0 && eta[ic - sx] == 0.0 && eta[ic + sx] == 0.0 && eta[ic - sy] == 0.0 &&
             eta[ic + sy] == 0.0 && eta[ic - 1] == 0.0 && eta[ic + 1] == 0.0) {
           if (a > mx[1]) mx[1] = a;
@@ -1725,7 +1926,7 @@ int orc_sim_step(orc_sim *s, cfd_step_stats *st) {
   sim_pass(s, P->inner_loops, 0, st);
   if (st) {  /* the corrected velocity on the boundary faces the projection saw */
     double dv[4];
-    orc_divergence(g, P->p_dirichlet, s->T3[0], s->T3[1], s->T3[2], s->T1[0], s->T1[1], s->T1[2], s->eta, dv);
+    orc_divergence_bl(g, P, s->T3[0], s->T3[1], s->T3[2], s->T1[0], s->T1[1], s->T1[2], s->eta, dv);
     st->div_max = dv[0];
     st->div_l2 = dv[1];
     st->div_fluid_max = dv[2];
@@ -1734,9 +1935,25 @@ int orc_sim_step(orc_sim *s, cfd_step_stats *st) {
   double md[3];
   orc_maxdiff(g, s->T0[0], s->T0[1], s->T0[2], s->T3[0], s->T3[1], s->T3[2], md);
   if (st) memcpy(st->maxdiff, md, sizeof(md));
+  /* a convective outflow steps from the ghost values the last BC left in
+     T0: the whole-array T0 <- T3 copy must not replace them by T3's */
+  const size_t plane = (size_t)d.ny * d.nz;
+  double *keep = NULL;
+  if (P->outflow_convective) {
+    keep = (double *)malloc(sizeof(double) * 4 * plane);
+    for (int c = 1; c < 3; ++c)
+      for (int q = 0; q < 2; ++q) memcpy(keep + ((c - 1) * 2 + q) * plane, s->T0[c] + (size_t)(d.nx - 2 + q) * plane,
+                                         sizeof(double) * plane);
+  }
   for (int c = 0; c < 3; ++c) vcopy(d.G, s->T3[c], s->T0[c]);
-  orc_bc_main_ex(g, P->bc_dir, P->bc_num, P->p_dirichlet, s->uin, s->T0[0], s->T0[1], s->T0[2], s->p);
-  orc_bc_copy(g, s->T0[0], s->T0[1], s->T0[2], s->T1[0], s->T1[1], s->T1[2]);
+  if (keep) {
+    for (int c = 1; c < 3; ++c)
+      for (int q = 0; q < 2; ++q) memcpy(s->T0[c] + (size_t)(d.nx - 2 + q) * plane, keep + ((c - 1) * 2 + q) * plane,
+                                         sizeof(double) * plane);
+    free(keep);
+  }
+  orc_bc_main_bl(g, P, s->uin, s->T0[0], s->T0[1], s->T0[2], s->p);
+  orc_bc_copy_bl(g, P, s->T0[0], s->T0[1], s->T0[2], s->T1[0], s->T1[1], s->T1[2]);
   /* simuClass::finishloop (simuClass.hpp:152-169) */
   s->time += P->dt;
   s->loop++;

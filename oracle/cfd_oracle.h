@@ -106,6 +106,7 @@ orc_bicg *orc_bicg_create_csr(size_t N, const int32_t *ptr, const int32_t *col, 
 void orc_bicg_destroy(orc_bicg *s);
 void orc_bicg_set_tolerance(orc_bicg *s, double tol);
 void orc_bicg_set_maxit(orc_bicg *s, int maxit);
+void orc_bicg_set_periodic_z(orc_bicg *s, int pz);
 /* bicgstabRe2::solve (inc/solver/bicgstabRe2.hpp:91-189); returns relres, *iters */
 double orc_bicg_solve(orc_bicg *s, const double *b, double *x, int *iters);
 int orc_bicg_restarts(const orc_bicg *s);
@@ -184,6 +185,36 @@ double orc_sim_objective(orc_sim *s, int nsteps, int avg_from);
    there is none or the configuration is not differentiable here (source
    index quirk, matrix-free operator) */
 int orc_sim_gradient(orc_sim *s, double *g_uin, double *g_gain, int *adj_iters);
+
+
+/* ---- boundary conditions the reference's tables cannot express (no
+   reference counterpart): periodic z and a convective outflow on face 1,
+   selected by cfd_params.periodic_z / outflow_convective / outflow_speed.
+   The _bl functions take the boundary mode from P; the others keep the
+   table semantics. */
+void orc_predictor_bl(const cfd_grid_desc *g, const cfd_params *P, const double *vis,
+                      const double *u, const double *v, const double *w,
+                      double *u1, double *v1, double *w1);
+int orc_ell_assemble_bl(const cfd_grid_desc *g, const int pd[6], int pz, int32_t *col, double *val, double *jp);
+void orc_spe_setup_bl(const cfd_grid_desc *g, const int pd[6], int pz, double *coef, double *jp);
+void orc_spmv_spe_bl(const cfd_grid_desc *g, int pz, const double *coef, const double *jp, const double *x,
+                     double *y);
+void orc_dfib_update_bl(const cfd_grid_desc *g, const cfd_params *P, double gain, const double *eta, const double *p,
+                        const double *u1, const double *v1, const double *w1,
+                        double *u3, double *v3, double *w3, double *f, double *t2);
+void orc_bc_main_bl(const cfd_grid_desc *g, const cfd_params *P, const double *uin, double *u, double *v, double *w,
+                    double *p);
+void orc_bc_copy_bl(const cfd_grid_desc *g, const cfd_params *P, const double *u0, const double *v0,
+                    const double *w0, double *u1, double *v1, double *w1);
+void orc_periodic_refresh(const cfd_grid_desc *g, double *u, double *v, double *w);
+void orc_periodic_refresh_p(const cfd_grid_desc *g, double *p);
+double orc_sor_rb_vw_bl(const cfd_grid_desc *g, int pz, double dt, double omega, double crit, int itmax,
+                        const double *u1, const double *v1, const double *w1, double *p, int *iters);
+double orc_sor_lex_bl(const cfd_grid_desc *g, int pz, double dt, double omega, double crit, int itmax,
+                      const double *u1, const double *v1, const double *w1, double *p, int *iters);
+void orc_divergence_bl(const cfd_grid_desc *g, const cfd_params *P, const double *u, const double *v,
+                       const double *w, const double *ub, const double *vb, const double *wb, const double *eta,
+                       double out[4]);
 
 #ifdef __cplusplus
 }

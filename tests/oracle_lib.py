@@ -57,6 +57,25 @@ def _declare(L):
     L.orc_source_accumulate.argtypes = [G, C.c_double, dp, dp]
     L.orc_rhs.argtypes = [G, C.c_double, dp, dp, dp, dp, dp]
     L.orc_divergence.argtypes = [G, C.POINTER(C.c_int), dp, dp, dp, dp, dp, dp, dp, dp]
+    P = C.POINTER(CfdParams)
+    vpp = C.c_void_p
+    L.orc_ell_assemble_bl.argtypes = [G, C.POINTER(C.c_int), C.c_int, ip, dp, dp]
+    L.orc_ell_assemble_bl.restype = C.c_int
+    L.orc_spe_setup_bl.argtypes = [G, C.POINTER(C.c_int), C.c_int, dp, dp]
+    L.orc_spmv_spe_bl.argtypes = [G, C.c_int, dp, dp, dp, dp]
+    L.orc_bc_main_bl.argtypes = [G, P, vpp, dp, dp, dp, dp]
+    L.orc_bc_copy_bl.argtypes = [G, P, dp, dp, dp, dp, dp, dp]
+    L.orc_periodic_refresh.argtypes = [G, dp, dp, dp]
+    L.orc_periodic_refresh_p.argtypes = [G, dp]
+    L.orc_sor_rb_vw_bl.argtypes = [G, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp,
+                                   C.POINTER(C.c_int)]
+    L.orc_sor_rb_vw_bl.restype = C.c_double
+    L.orc_sor_lex_bl.argtypes = [G, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp,
+                                 C.POINTER(C.c_int)]
+    L.orc_sor_lex_bl.restype = C.c_double
+    L.orc_divergence_bl.argtypes = [G, P, dp, dp, dp, dp, dp, dp, dp, dp]
+    L.orc_predictor_bl.argtypes = [G, P, vpp, dp, dp, dp, dp, dp, dp]
+    L.orc_dfib_update_bl.argtypes = [G, P, C.c_double, dp, dp, dp, dp, dp, dp, dp, dp, dp, vpp]
     L.orc_ell_assemble.argtypes = [G, C.POINTER(C.c_int), ip, dp, dp]
     L.orc_ell_assemble.restype = C.c_int
     L.orc_spmv_ell.argtypes = [C.c_size_t, C.c_int, ip, dp, dp, dp]

@@ -56,12 +56,51 @@ def test_grid_generators_match_oracle():
     assert all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
-def test_stretched_axis_invariants():
-    X, D, Ds = axis_tanh(1024, 40.0, 10.0, 0.025, 4.0, 1.05)
-    w = D[2:-2]
-    assert abs(X[1026] - 40.0) < 1e-9 and X[2] == 0.0
-    assert np.all(w > 0) and np.max(w[1:] / w[:-1]) <= 1.05 + 1e-9 and np.max(w[:-1] / w[1:]) <= 1.05 + 1e-9
-    assert abs(w.min() - 0.025) < 1e-12
+def tanh_axis_numpy(ncal, L, xc, h0, band, ratio):
+    """The tanh clustering from its definition, solved independently with
+    scipy's brentq: widths dx/dxi = H - (H - h0)/2 [tanh((xi+a)/w) -
+    tanh((xi-a)/w)], a = band/(2 h0), node 0 at 0, node ncal at L, the
+    narrowest w whose neighbouring-width ratio is <= ratio."""
+    from scipy.optimize import brentq
+    a = 0.5 * band / h0
+    lncosh = lambda t: np.abs(t) + np.log1p(np.exp(-2 * np.abs(t))) - np.log(2.0)
+
+    def nodes(H, w):
+        f = lambda xi: H * xi - 0.5 * (H - h0) * w * (lncosh((xi + a) / w) - lncosh((xi - a) / w))
+        xi0 = brentq(lambda z: xc + f(z), -ncal - 1.0, 0.0, xtol=1e-14)
+        return xc + f(xi0 + np.arange(ncal + 1))
+
+    def fit(w):
+        H = brentq(lambda H: nodes(H, w)[-1] - L, h0, L, xtol=1e-15)
+        x = nodes(H, w)
+        d = np.diff(x)
+        return x, max(np.max(d[1:] / d[:-1]), np.max(d[:-1] / d[1:]))
+
+    w = brentq(lambda w: fit(w)[1] - ratio, 0.25, ncal, xtol=1e-12)
+    return fit(w)[0]
+
+
+def test_stretched_axis_is_tanh_clustering():
+    for ncal, L, xc in ((1024, 40.0, 10.0), (512, 20.0, 10.0)):
+        X, D, Ds = axis_tanh(ncal, L, xc, 0.025, 4.0, 1.05)
+        w = D[2:-2]
+        nodes = X[2:ncal + 3]
+        assert nodes[0] == 0.0 and nodes[-1] == L
+        # the ratio bound and the fine band: D = 1 resolved by 40 cells of ~h0
+        assert np.all(w > 0) and np.max(w[1:] / w[:-1]) <= 1.05 + 1e-9 and np.max(w[:-1] / w[1:]) <= 1.05 + 1e-9
+        assert abs(w.min() - 0.025) < 0.01 * 0.025
+        near = np.abs(0.5 * (nodes[1:] + nodes[:-1]) - xc) < 0.5
+        assert near.sum() >= 39 and np.all(np.abs(w[near] - 0.025) < 0.01 * 0.025)
+        # widths grow monotonically away from the centre
+        c = int(np.argmin(w))
+        assert np.all(np.diff(w[c:]) >= -1e-12) and np.all(np.diff(w[:c + 1]) <= 1e-12)  # up to node rounding
+        # the closed form, solved independently
+        ref = tanh_axis_numpy(ncal, L, xc, 0.025, 4.0, 1.05)
+        assert np.max(np.abs(nodes - ref)) < 1e-6 * L
+    # centred: symmetric about xc
+    X, _, _ = axis_tanh(512, 20.0, 10.0, 0.025, 4.0, 1.05)
+    nodes = X[2:515]
+    assert np.max(np.abs(nodes + nodes[::-1] - 20.0)) < 1e-9
 
 
 def test_native_grid_closed_form():

@@ -142,6 +142,35 @@ def test_autograd_function(require_gpu):
     assert pr.last_stats["steps"] == STEPS
 
 
+def test_stale_objective_run_is_refused(require_gpu):
+    """backward differentiates the run its forward recorded: after a second
+    forward, or a changed gain / state, the first graph node raises and the
+    C-ABI refuses the sweep instead of mixing trajectories."""
+    w, uin = case(tol=1e-10)
+    pr = gpu_problem(w, every=5)
+    u1 = torch.tensor(uin, dtype=torch.float64, requires_grad=True)
+    g1 = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    J1 = DragObjective.apply(u1, g1, pr)
+    u2 = torch.tensor(uin * 1.01, dtype=torch.float64, requires_grad=True)
+    g2 = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    J2 = DragObjective.apply(u2, g2, pr)
+    with pytest.raises(RuntimeError, match="another objective"):
+        J1.backward()
+    J2.backward()  # the latest run is still differentiable
+    assert u2.grad is not None and np.all(np.isfinite(u2.grad.numpy()))
+    pr.run(uin, 1.0)
+    assert pr.sim.adjoint_generation() > 0
+    pr.sim.set_forcing_gain(1.001)
+    assert pr.sim.adjoint_generation() == 0
+    with pytest.raises(capi.CfdError) as e:
+        pr.sim.adjoint_backward()
+    assert e.value.status == capi.CFD_ERR_STATE
+    pr.run(uin, 1.0)
+    pr.sim.step()  # the state moved on
+    with pytest.raises(capi.CfdError):
+        pr.sim.adjoint_backward()
+
+
 def test_reference_parameter_values_keep_the_step_bit_identical(require_gpu):
     w, _ = case(tol=1e-10)
     runs = []

@@ -4,7 +4,8 @@ file back, and that block is commented out at RUN_CPU_PC_METHOD.hpp:98-111).
 
 A run that is saved after a few steps and resumed in a fresh context
 continues bit for bit like the run that never stopped: the same iteration
-counts, residuals, drag and max-differences per pass, the same fields.  The
+counts, residuals, drag and max-differences per pass, the same fields and
+solver counters.  The
 resumed part crosses the tenth solve, where the reference's restart rule
 starts to apply, so the solve count travels too.  A file of another grid, a
 truncated, corrupted or foreign file is rejected and leaves the context as
@@ -54,6 +55,7 @@ def test_resume_continues_bit_for_bit(require_gpu, tmp_path, kind):
     a.save_checkpoint(path)
     ref = [stats_tuple(a.step()) for _ in range(AFTER)]
     ref_fields = [a.download(f) for f in FIELDS]
+    ref_counters = a.solver_counters()
     loop_a = a.loop
     a.close()
 
@@ -66,6 +68,8 @@ def test_resume_continues_bit_for_bit(require_gpu, tmp_path, kind):
     for fid, x in zip(FIELDS, ref_fields):
         assert np.array_equal(b.download(fid), x), fid
     assert b.loop == loop_a
+    # bicgstabRe2's counters continue too (timestep_, numRestarts_, pResets_, totalNumIter_)
+    assert b.solver_counters() == ref_counters, (b.solver_counters(), ref_counters)
     b.close()
 
 

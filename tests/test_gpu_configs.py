@@ -1,0 +1,212 @@
+"""The benchmark workloads at their own sizes on the GPU.
+
+- 512x256x64 cylinder, Re 200, dt 5e-4, FP64 (configs.config_b; convective
+  outflow, periodic z):
+  * the reference's algorithm (Jacobi-scaled BiCGSTAB, iters_max 5000):
+    every pass keeps bicgstabRe2's semantics -- at most 5000 iterations, a
+    finite solution, no NaN reset -- and its residual is recorded;
+  * multigrid-preconditioned BiCGSTAB: every pass reaches 1e-10, and the
+    corrector pass's residual recomputed on the host from the downloaded
+    ELL operator is below 1e-10 in FP64;
+  * the standalone Poisson problem (Neumann walls, p = 0 on the outflow,
+    seeded zero-mean N(0,1) RHS): BiCGSTAB to 1e-10, true residual checked.
+- its mixed-precision twin: FP64 Krylov around an FP32 V-cycle, and FP64
+  defect correction around FP32 inner solves to 1e-5: every pass at the
+  FP64 residual 1e-10 (host-checked on the corrector pass), fields against
+  the FP64 run of the same preconditioner, divergence check in the fluid.
+- 256x128x32 adjoint case (configs.config_d), Re 100, dt 1e-3, 400 steps,
+  checkpoints every 20: the gradient of the mean drag over the last 200
+  steps against central differences (h = 1e-6), one direction in u_in only
+  at the reference gain 1 and two random directions in (u_in, gain) about
+  gain 1.001 (the forcing gain 1 makes the immersed cells' velocity exactly
+  zero, where QUICK's upwind test sits on a kink).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from cfd_nssolver_forhpc_amd import capi, configs
+from cfd_nssolver_forhpc_amd.adjoint import DragProblem, smoothed_inflow
+from cfd_nssolver_forhpc_amd.solver import Simulation
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = (("u", capi.T0_U), ("v", capi.T0_V), ("w", capi.T0_W), ("p", capi.X))
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def true_relres(sim, col, val):
+    """||b - A x|| / ||b|| of the context's last solve, from host copies."""
+    b, x = sim.download(capi.RHS), sim.download(capi.X)
+    y = np.zeros(sim.N)
+    O.lib().orc_spmv_ell(sim.N, 7, col, val, x, y)
+    return float(np.linalg.norm(b - y) / np.linalg.norm(b))
+
+
+def mg_workload(precision, mode):
+    w = configs.config_b(precision)
+    w.params.precond = capi.CFD_PRECOND_MG
+    w.params.mixed_mode = mode
+    w.params.tol_inner = 1e-5
+    w.params.maxit = 200
+    return w
+
+
+@pytest.mark.timeout(600)
+def test_b_reference_algorithm_semantics(require_gpu):
+    w = configs.config_b(capi.CFD_FP64)
+    assert tuple(w.grid.ncal) == (512, 256, 64) and w.params.maxit == 5000 and w.params.tol == 1e-10
+    sim = Simulation(w.grid, w.params)
+    sim.setup(noise=w.noise, seed=w.seed)
+    for k in range(3):
+        t0 = time.perf_counter()
+        st = sim.step()
+        its, rel = list(st.iters[:st.n_passes]), list(st.relres[:st.n_passes])
+        say(f"[B jacobi] step {k + 1}: iters {its} relres {['%.2e' % r for r in rel]} "
+            f"{time.perf_counter() - t0:.1f} s")
+        if k == 0:
+            assert its == [-1] * 4
+            continue
+        for it, r in zip(its, rel):
+            assert 0 < it <= 5000 and np.isfinite(r)
+            assert r < 1e-10 or it == 5000  # a pass stops at the tolerance or at iters_max
+    cnt = sim.solver_counters()
+    x = sim.download(capi.X)
+    assert np.all(np.isfinite(x)) and cnt["resets"] == 0, cnt
+    say(f"[B jacobi] counters {cnt}")
+    sim.close()
+
+
+@pytest.mark.timeout(600)
+def test_b_multigrid_fp64_reaches_1e10(require_gpu):
+    w = mg_workload(capi.CFD_FP64, capi.CFD_MIXED_RESTARTED)
+    sim = Simulation(w.grid, w.params)
+    sim.setup(noise=w.noise, seed=w.seed)
+    col, val = sim.ell()
+    for k in range(4):
+        st = sim.step()
+        its, rel = list(st.iters[:st.n_passes]), list(st.relres[:st.n_passes])
+        if k == 0:
+            continue
+        tr = true_relres(sim, col, val)
+        say(f"[B mg fp64] step {k + 1}: iters {its} relres max {max(rel):.2e} corrector true {tr:.2e} "
+            f"div fluid max {st.div_fluid_max:.2e}")
+        assert all(0 < it < w.params.maxit for it in its) and max(rel) < 1e-10
+        assert tr < 1e-10
+    sim.close()
+
+
+@pytest.mark.timeout(300)
+def test_b_standalone_poisson(require_gpu):
+    w = configs.config_b(capi.CFD_FP64)
+    w.params.maxit = 20000
+    w.params.periodic_z = w.params.outflow_convective = 0  # Neumann walls, Dirichlet outflow face
+    sim = Simulation(w.grid, w.params)
+    sim.init_state()
+    sim.createPressureMatrix()
+    rng = np.random.default_rng(2)
+    b = rng.standard_normal(sim.N)
+    b -= b.mean()
+    sim.upload(capi.RHS, b)
+    sim.upload(capi.X, np.zeros(sim.N))
+    it, rel = sim.bicgstab(tol=1e-10, maxit=20000)
+    col, val = sim.ell()
+    tr = true_relres(sim, col, val)
+    say(f"[B poisson] BiCGSTAB {it} iterations, relres {rel:.2e}, true {tr:.2e}")
+    assert rel < 1e-10 and tr < 2e-10
+    sim.close()
+
+
+@pytest.mark.timeout(900)
+def test_c_mixed_twin_against_fp64(require_gpu):
+    fields, col, val = {}, None, None
+    for tag, prec, mode in (("fp64_mg", capi.CFD_FP64, capi.CFD_MIXED_RESTARTED),
+                            ("mixed_mg", capi.CFD_MIXED, capi.CFD_MIXED_PC32),
+                            ("mixed_mg_dc", capi.CFD_MIXED, capi.CFD_MIXED_RESTARTED)):
+        w = mg_workload(prec, mode)
+        sim = Simulation(w.grid, w.params)
+        sim.setup(noise=w.noise, seed=w.seed)
+        if col is None:
+            col, val = sim.ell()
+        for k in range(4):
+            st = sim.step()
+            if k == 0:
+                continue
+            its, rel = list(st.iters[:st.n_passes]), list(st.relres[:st.n_passes])
+            tr = true_relres(sim, col, val)
+            say(f"[C {tag}] step {k + 1}: iters {its} inner {list(st.inner_iters[:st.n_passes])} "
+                f"relres max {max(rel):.2e} corrector true {tr:.2e} div fluid max {st.div_fluid_max:.2e}")
+            assert max(rel) < 1e-10 and tr < 1e-10
+            # the projection's divergence in the fluid, against the predicted field's
+            pred = sim.divergence(capi.T1_U)["fluid_l2"]
+            assert st.div_fluid_l2 < 1e-6 * pred, (st.div_fluid_l2, pred)
+        fields[tag] = {name: sim.download(fid) for name, fid in FIELDS}
+        sim.close()
+    for tag in ("mixed_mg", "mixed_mg_dc"):
+        for name, _ in FIELDS:
+            x, y = fields[tag][name], fields["fp64_mg"][name]
+            err = float(np.linalg.norm(x - y) / np.linalg.norm(y))
+            say(f"[C {tag}] vs fp64_mg after 4 steps: {name} rel L2 {err:.2e}")
+            assert err < (1e-7 if name == "p" else 1e-9), (tag, name, err)
+
+
+@pytest.mark.timeout(900)
+def test_d_adjoint_against_central_differences(require_gpu):
+    w = configs.config_d()
+    assert tuple(w.grid.ncal) == (256, 128, 32)
+    w.params.precond = capi.CFD_PRECOND_MG
+    w.params.maxit = 200
+    sim = Simulation(w.grid, w.params)
+    sim.init_state(w.noise, w.seed)
+    sim.DFIB_CylinderZ()
+    sim.createPressureMatrix()
+    steps, every, h = 400, 20, 1e-6
+    pr = DragProblem(sim, steps, steps // 2, every, noise=w.noise, seed=w.seed)
+    uin = smoothed_inflow(w.grid.ncal[1])
+    rng = np.random.default_rng(8)
+    # u_in only at gain 1: a random profile change with a nonzero mean (the
+    # flux through the inlet, which the projection carries to the cylinder at
+    # once) and a zero-mean one (carried only by advection and diffusion, so
+    # J's derivative along it is ~1e-5: a central difference at h = 1e-6 then
+    # resolves it only to ~1e-16 / h, the rounding of J, and the check is
+    # against the gradient's scale instead); then two random directions in
+    # (u_in, gain) about gain 1.001, off the kink of QUICK's upwind test at
+    # gain 1
+    zero_mean = rng.standard_normal(uin.size)
+    zero_mean -= zero_mean.mean()
+    cases = [(1.0, np.concatenate([1.0 + 0.5 * rng.standard_normal(uin.size), [0.0]]), "rel"),
+             (1.0, np.concatenate([zero_mean, [0.0]]), "scale")]
+    cases += [(1.001, rng.standard_normal(uin.size + 1), "rel") for _ in range(2)]
+    for gain, d, check in cases:
+        d = d / np.linalg.norm(d)
+        t0 = time.perf_counter()
+        J = pr.run(uin, gain)
+        t1 = time.perf_counter()
+        g_uin, g_gain, st = sim.adjoint_backward()
+        t2 = time.perf_counter()
+        ad = float(np.dot(g_uin, d[:-1]) + g_gain * d[-1])
+        Jp = pr.run(uin + h * d[:-1], gain + h * d[-1])
+        Jm = pr.run(uin - h * d[:-1], gain - h * d[-1])
+        fd = (Jp - Jm) / (2 * h)
+        rel = abs(fd - ad) / max(abs(fd), abs(ad))
+        say(f"[D adjoint] gain {gain}: J {J:.8f} fd {fd:.9e} adjoint {ad:.9e} rel {rel:.2e}; forward {t1 - t0:.1f} s "
+            f"reverse {t2 - t1:.1f} s ({(t2 - t1) / (t1 - t0):.2f}x), checkpoints {st['checkpoints']}")
+        assert st["steps"] == steps and st["checkpoints"] >= steps // every
+        if check == "rel":
+            assert rel < 1e-5, (gain, fd, ad, rel)
+        else:
+            scale = float(np.hypot(np.linalg.norm(g_uin), g_gain))
+            say(f"[D adjoint] zero-mean u_in direction: |fd - adjoint| {abs(fd - ad):.2e}, gradient norm {scale:.3e}")
+            assert abs(fd - ad) < 1e-5 * scale, (fd, ad, scale)
+        assert t2 - t1 < 3.0 * (t1 - t0), (t1 - t0, t2 - t1)
+    sim.close()

@@ -127,6 +127,36 @@ def test_mg_bicgstab_reaches_the_jacobi_solution(require_gpu, nu):
     assert rel_l2(x, x64) < 1e-7
 
 
+@pytest.mark.parametrize("op", [capi.CFD_OP_SPE, capi.CFD_OP_CSR])
+@pytest.mark.parametrize("pc32", [False, True])
+def test_mg_bicgstab_on_the_spe_and_csr_operators(require_gpu, op, pc32):
+    """The preconditioned iteration applies the configured operator (SPE over
+    plane columns, CSR staged through LDS): the same solution and true FP64
+    residual as on the ELL operator, in about as many iterations."""
+    ref, b = poisson_sim(ncal=(64, 32, 16))
+    ref.bicgstab(tol=1e-13, maxit=50000)
+    x64 = ref.download(capi.X)
+    runs = {}
+    for o in (capi.CFD_OP_ELL, op):
+        sim, _ = mg_sim(precision=capi.CFD_MIXED if pc32 else capi.CFD_FP64)
+        p = sim.params.copy()
+        p.op = o
+        if pc32:
+            p.mixed_mode = capi.CFD_MIXED_PC32
+        sim.set_params(p)
+        if o == capi.CFD_OP_SPE:
+            sim.setupPressure()
+        elif o == capi.CFD_OP_CSR:
+            sim.createPressureMatrixCSR()
+        it, rel = sim.solve()
+        tr, x = true_relres(sim, b)
+        print("op", o, "pc32", pc32, "iterations", it, "relres", rel, "true relres", tr)
+        assert rel < 1e-10 and tr < 2e-10
+        assert rel_l2(x, x64) < 1e-7
+        runs[o] = it
+    assert abs(runs[op] - runs[capi.CFD_OP_ELL]) <= 1
+
+
 def test_mixed_mg_defect_correction(require_gpu):
     ref, b = poisson_sim(ncal=(64, 32, 16))
     ref.bicgstab(tol=1e-13, maxit=50000)

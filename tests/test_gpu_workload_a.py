@@ -1,19 +1,18 @@
 """The CPU-runnable workload on its own grid: 128x64x4 cells over
 20D x 10D x 0.5D, a D = 1 cylinder at (5D, 5D), Re 100, dt 0.002, seeded
 +-1e-3 noise (configs.config_a), stepped on the GPU and by the oracle from
-the same state, with every field compared after every step.
+the same state for all of its 100 steps, with every field compared after
+every step.
 
 The pressure passes solve to a relative residual of 1e-13 here instead of
 the workload's 1e-10: two solves that each stop below tol may differ by up
 to kappa * tol, and at 1e-10 that floor alone is ~1e-9 in velocity and up to
 ~1e-6 in pressure (measured over the full 100 steps: velocity <= 1.3e-9,
 pressure <= 9e-7, cD <= 3.3e-8, with no growth along the run).  At 1e-13
-the velocity agrees to better than 1e-12 on every step.  The pressure
-correction keeps its own floor, kappa * 1e-13 ~ 1e-10 (1.3e-10 at step 6):
-its error sits in the smooth modes the residual weighs least, and a tighter
-tolerance is out of reach (the oracle's solves meet the 5000-iteration cap
-at 1e-13 from step 8 on), so its bar is 1e-9.  Steps 1-7 keep every pass
-below that cap, where the two sides could stop at different points.
+the solves need more than the reference's 5000-iteration cap on some passes
+(up to ~7000 on the oracle), so both sides run with a cap of 100000 and every
+pass reaches the tolerance.  The pressure correction keeps its own floor,
+kappa * 1e-13: its error sits in the smooth modes the residual weighs least.
 """
 from __future__ import annotations
 
@@ -26,17 +25,20 @@ from cfd_nssolver_forhpc_amd.solver import Simulation
 
 pytestmark = pytest.mark.gpu
 
-STEPS = 7
+STEPS = 100
+MAXIT = 100000
 
 
 def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+@pytest.mark.timeout(900)
 def test_cylinder_128x64x4_steps_match_oracle(require_gpu):
     w = configs.config_a()
     assert tuple(w.grid.ncal) == (128, 64, 4)
     w.params.tol = 1e-13
+    w.params.maxit = MAXIT
     sim = Simulation(w.grid, w.params)
     sim.setup(noise=w.noise, seed=w.seed)
     osim = O.Sim(O.Grid(w.grid), w.params)
@@ -44,17 +46,24 @@ def test_cylinder_128x64x4_steps_match_oracle(require_gpu):
     osim.setup()
     for c in range(3):
         assert np.array_equal(sim.download(capi.T0_U + c), osim.field(capi.T0_U + c))
+    worst = {"u": 0.0, "v": 0.0, "w": 0.0, "p": 0.0}
     for k in range(STEPS):
         st = sim.step()
         ost = osim.step()
         errs = {name: rel_l2(sim.download(fid), osim.field(fid))
                 for name, fid in (("u", capi.T0_U), ("v", capi.T0_V), ("w", capi.T0_W), ("p", capi.X))}
-        print(f"step {k + 1}: iters {list(st.iters[:st.n_passes])} oracle {list(ost.iters[:ost.n_passes])} "
-              + " ".join(f"{n} {e:.2e}" for n, e in errs.items()), flush=True)
+        for n, e in errs.items():
+            worst[n] = max(worst[n], e)
+        if k < 3 or (k + 1) % 10 == 0:
+            print(f"step {k + 1}: iters {list(st.iters[:st.n_passes])} oracle {list(ost.iters[:ost.n_passes])} "
+                  + " ".join(f"{n} {e:.2e}" for n, e in errs.items()), flush=True)
         for q in range(st.n_passes):
             assert (st.iters[q] < 0) == (ost.iters[q] < 0)
-            assert st.iters[q] < w.params.maxit and ost.iters[q] < w.params.maxit
+            if st.iters[q] >= 0:  # every pass reaches the tolerance, none stops at the cap
+                assert st.relres[q] < w.params.tol and ost.relres[q] < w.params.tol, (k + 1, q)
+                assert st.iters[q] < MAXIT and ost.iters[q] < MAXIT
             assert abs(st.cd[q] - ost.cd[q]) <= 1e-9 * max(1.0, abs(ost.cd[q]))
         for name, e in errs.items():
             assert e < (1e-9 if name == "p" else 1e-10), (k + 1, name, e)
+    print("worst over 100 steps: " + " ".join(f"{n} {e:.2e}" for n, e in worst.items()), flush=True)
     sim.close()

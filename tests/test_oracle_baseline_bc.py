@@ -1,0 +1,193 @@
+"""The boundary conditions the reference's tables cannot express, in the
+oracle: periodic z and a convective outflow on face 1 (cfd_params
+periodic_z / outflow_convective / outflow_speed; no reference counterpart).
+
+Pinned by what the mathematics fixes:
+- the convective outflow is one implicit upwind step of
+  d(phi)/dt + U d(phi)/dx = 0: its output satisfies that discrete equation
+  to rounding, keeps a uniform field, and leaves the other faces to the table;
+- periodic z: the ghost layers are the images one period away; the wrapped
+  operator annihilates constants with Neumann x/y faces, has the discrete
+  Fourier modes cos(2 pi m k / nz) as eigenvectors on a uniform grid, is
+  symmetric in its volume-weighted form, and its ELL and matrix-free (SPE)
+  forms agree; the red-black and lexicographic SOR share its fixed point;
+- a whole step commutes with a circular shift in z (the cylinder is
+  z-invariant), and the corrected velocity is divergence-free in the fluid,
+  the outflow column included.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+import oracle_lib as O
+from cfd_nssolver_forhpc_amd import capi, configs
+from test_oracle_closed_forms import dense, sgrid, ugrid
+
+
+def bl_params(tol=1e-13, conv=1, pz=1):
+    p = configs.cylinder_params(100.0, 0.002, 2.0, 2.0, 1.0, tol)
+    p.periodic_z, p.outflow_convective, p.outflow_speed = pz, conv, 1.3
+    p.maxit = 100000
+    return p
+
+
+def ell_bl(g, pd, pz):
+    col = np.zeros(7 * g.N, np.int32)
+    val = np.zeros(7 * g.N)
+    jp = np.zeros(g.N)
+    assert O.lib().orc_ell_assemble_bl(g.ref, O.ints6(pd), pz, col, val, jp) == 0
+    return col, val, jp
+
+
+def test_convective_outflow_is_an_implicit_upwind_step():
+    g = sgrid()
+    p = bl_params()
+    rng = np.random.default_rng(1)
+    u, v, w = (rng.standard_normal(g.G) for _ in range(3))
+    pr = np.zeros(g.G)
+    u0, v0, w0 = u.copy(), v.copy(), w.copy()
+    O.lib().orc_bc_main_bl(g.ref, C.byref(p), None, u, v, w, pr)
+    sh = (g.nx, g.ny, g.nz)
+    U, V, W = u.reshape(sh), v.reshape(sh), w.reshape(sh)
+    U0, V0, W0 = u0.reshape(sh), v0.reshape(sh), w0.reshape(sh)
+    nx, dt, Us = g.nx, p.dt, p.outflow_speed
+    J, K = slice(2, g.ny - 2), slice(2, g.nz - 2)
+    hu, ht = g.a.Dx[nx - 3], g.a.Dxs[nx - 3]
+    # (phi_new - phi_old)/dt + U (phi_new - phi_in)/h = 0, phi_in the interior
+    # value the x faces read (the y+ face runs after them and may reset it)
+    ru = (U[nx - 3, J, K] - U0[nx - 3, J, K]) / dt + Us * (U[nx - 3, J, K] - U0[nx - 4, J, K]) / hu
+    rv = (V[nx - 2, J, K] - V0[nx - 2, J, K]) / dt + Us * (V[nx - 2, J, K] - V0[nx - 3, J, K]) / ht
+    rw = (W[nx - 2, J, K] - W0[nx - 2, J, K]) / dt + Us * (W[nx - 2, J, K] - W0[nx - 3, J, K]) / ht
+    for r in (ru, rv, rw):
+        assert np.abs(r).max() < 1e-9 * (1.0 / dt)
+    # both ghost layers take the boundary value; the interior is untouched
+    assert np.array_equal(U[nx - 1, J, K], U[nx - 3, J, K]) and np.array_equal(U[nx - 2, J, K], U[nx - 3, J, K])
+    assert np.array_equal(V[nx - 1, J, K], V[nx - 2, J, K])
+    assert np.array_equal(U[2:nx - 3, J, K], U0[2:nx - 3, J, K])
+    # a uniform field is a fixed point
+    c = np.full(g.G, 0.7)
+    cu, cv, cw = c.copy(), c.copy(), c.copy()
+    O.lib().orc_bc_main_bl(g.ref, C.byref(p), None, cu, cv, cw, pr)
+    assert np.allclose(cu.reshape(sh)[nx - 3:, J, K], 0.7, rtol=0, atol=1e-15)
+
+
+def test_periodic_ghosts_are_the_images():
+    g = ugrid((10, 6, 8), (2.5, 1.5, 2.0))
+    p = bl_params(conv=0)
+    rng = np.random.default_rng(2)
+    u, v, w, pr = (rng.standard_normal(g.G) for _ in range(4))
+    O.lib().orc_bc_main_bl(g.ref, C.byref(p), None, u, v, w, pr)
+    nz = g.nz
+    for a in (u, v, w, pr):
+        A = a.reshape(g.nx, g.ny, g.nz)
+        for dst, src in ((0, nz - 4), (1, nz - 3), (nz - 2, 2), (nz - 1, 3)):
+            assert np.array_equal(A[:, :, dst], A[:, :, src])  # whole planes, edges included
+
+
+def test_periodic_operator_structure():
+    g = ugrid((8, 6, 8), (2.0, 1.5, 2.0))
+    col, val, jp = ell_bl(g, (0,) * 6, 1)
+    A = dense(g, col, val)
+    assert np.abs(A @ np.ones(g.N)).max() < 1e-12  # Neumann x/y, periodic z: constants in the null space
+    L = A / jp[:, None]
+    assert np.allclose(L, L.T, rtol=0, atol=1e-9 * np.abs(L).max())  # uniform cells: L symmetric
+    h = 2.0 / 8
+    kk = np.tile(np.arange(8), g.N // 8)
+    for m in (1, 2, 3):
+        vec = np.cos(2 * np.pi * m * kk / 8)
+        lam = (2 - 2 * np.cos(2 * np.pi * m / 8)) / h ** 2
+        assert np.allclose(L @ vec, lam * vec, rtol=0, atol=1e-9 * lam)
+    # volume-weighted form symmetric on a stretched grid with a Dirichlet face
+    gs = sgrid()
+    col, val, jp = ell_bl(gs, (0, 1, 0, 0, 0, 0), 1)
+    As = dense(gs, col, val)
+    vol = np.array([gs.a.Dx[i] * gs.a.Dy[j] * gs.a.Dz[k] for i in range(2, gs.nx - 2) for j in range(2, gs.ny - 2)
+                    for k in range(2, gs.nz - 2)])
+    S = (vol / jp)[:, None] * As
+    assert np.allclose(S, S.T, rtol=0, atol=1e-12 * np.abs(S).max())
+
+
+def test_periodic_ell_and_matrix_free_agree():
+    g = sgrid()
+    pd = (0, 1, 0, 0, 0, 0)
+    col, val, jp = ell_bl(g, pd, 1)
+    coef = np.zeros(2 * (g.nxc + g.nyc + g.nzc))
+    sjp = np.zeros(g.N)
+    O.lib().orc_spe_setup_bl(g.ref, O.ints6(pd), 1, coef, sjp)
+    x = np.random.default_rng(3).standard_normal(g.N)
+    y1, y2 = np.zeros(g.N), np.zeros(g.N)
+    O.lib().orc_spmv_ell(g.N, 7, col, val, x, y1)
+    O.lib().orc_spmv_spe_bl(g.ref, 1, coef, sjp, x, y2)
+    assert np.allclose(sjp, jp, rtol=1e-15)
+    assert np.allclose(y1, y2, rtol=1e-12, atol=1e-12)
+
+
+def test_periodic_sor_fixed_point():
+    """Red-black and lexicographic SorPipeLine with periodic z and the x/y
+    ghost pressure held at 0: one fixed point, the dense solution of the
+    wrapped ELL system with Dirichlet x/y faces."""
+    g = sgrid()
+    rng = np.random.default_rng(8)
+    u1 = [1.0 + 0.05 * rng.standard_normal(g.G), 0.05 * rng.standard_normal(g.G), 0.05 * rng.standard_normal(g.G)]
+    p1, p2 = np.zeros(g.G), np.zeros(g.G)
+    it = C.c_int()
+    assert O.lib().orc_sor_lex_bl(g.ref, 1, 0.002, 1.8, 1e-13, 200000, *u1, p1, C.byref(it)) <= 1e-13
+    assert O.lib().orc_sor_rb_vw_bl(g.ref, 1, 0.002, 1.8, 1e-13, 200000, *u1, p2, C.byref(it)) <= 1e-13
+    inner = lambda p: p.reshape(g.nx, g.ny, g.nz)[2:-2, 2:-2, 2:-2].ravel()
+    col, val, jp = ell_bl(g, (1, 1, 1, 1, 0, 0), 1)
+    # the right-hand side of createBMatrix; with periodic z the -z face of the
+    # first plane is the last plane's +z face
+    w1p = u1[2].copy()
+    O.lib().orc_periodic_refresh(g.ref, u1[0], u1[1], w1p)
+    b = np.zeros(g.N)
+    O.lib().orc_rhs(g.ref, 0.002, jp, u1[0], u1[1], w1p, b)
+    x = np.linalg.solve(dense(g, col, val), b)
+    # SorPipeLine reads the lagged ghost w[1] of its input; give it the image too
+    p3 = np.zeros(g.G)
+    O.lib().orc_sor_rb_vw_bl(g.ref, 1, 0.002, 1.8, 1e-13, 200000, u1[0], u1[1], w1p, p3, C.byref(it))
+    assert np.linalg.norm(inner(p1) - inner(p2)) / np.linalg.norm(inner(p1)) < 1e-10
+    assert np.linalg.norm(inner(p3) - x) / np.linalg.norm(x) < 1e-9
+
+
+def run_steps(g, p, T0, nsteps):
+    sim = O.Sim(g, p)
+    sim.init_state(0.0, 1)
+    sim.setup()
+    for c in range(3):
+        sim.field(capi.T0_U + c)[:] = T0[c]
+    pr = sim.field(capi.P)
+    t = [sim.field(capi.T0_U + c) for c in range(3)]
+    t1 = [sim.field(capi.T1_U + c) for c in range(3)]
+    O.lib().orc_bc_main_bl(g.ref, C.byref(p), None, *t, pr)
+    O.lib().orc_bc_copy_bl(g.ref, C.byref(p), *t, *t1)
+    st = None
+    for _ in range(nsteps):
+        st = sim.step()
+    return [sim.field(capi.T0_U + c).copy() for c in range(3)], st
+
+
+def test_step_commutes_with_a_z_shift_and_is_divergence_free():
+    w = configs.small_cylinder(ncal=(24, 12, 8), lengths=(6.0, 3.0, 2.0), quirk=0, tol=1e-13)
+    p = w.params
+    p.periodic_z, p.outflow_convective, p.outflow_speed, p.maxit = 1, 1, 1.0, 100000
+    g = O.Grid(w.grid)
+    rng = np.random.default_rng(5)
+    T0 = [np.full(g.G, 1.0) + 0.05 * rng.standard_normal(g.G), 0.05 * rng.standard_normal(g.G),
+          0.05 * rng.standard_normal(g.G)]
+    shift = lambda a: np.roll(a.reshape(g.nx, g.ny, g.nz)[:, :, 2:-2], 3, axis=2)
+
+    def shifted_field(a):
+        b = a.reshape(g.nx, g.ny, g.nz).copy()
+        b[:, :, 2:-2] = shift(a)
+        return b.ravel()
+
+    A, st = run_steps(g, p, T0, 3)
+    B, _ = run_steps(g, p, [shifted_field(a) for a in T0], 3)
+    for a, b in zip(A, B):
+        ia = shift(a)[2:-2, 2:-2]
+        ib = b.reshape(g.nx, g.ny, g.nz)[2:-2, 2:-2, 2:-2]
+        assert np.linalg.norm(ia - ib) / np.linalg.norm(ia) < 1e-11
+    # the projection leaves the fluid divergence-free, the outflow column included
+    assert st.div_fluid_max < 1e-10, st.div_fluid_max
