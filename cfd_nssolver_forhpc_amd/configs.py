@@ -87,10 +87,11 @@ def config_b(precision: int = capi.CFD_FP64) -> Workload:
 
 
 def config_d() -> Workload:
-    """256x128x32 over 20 x 10 x 2.5, Re 100, dt 1e-3, 400 steps (adjoint case).  The discrete adjoint
-    transposes the table boundary conditions, so this case keeps zero-gradient outflow and slip z walls."""
+    """256x128x32 over 20 x 10 x 2.5, Re 100, dt 1e-3, 400 steps (adjoint case); convective outflow,
+    slip y-walls, periodic z."""
     g = uniform_grid((256, 128, 32), (20.0, 10.0, 2.5))
-    return Workload("cyl-256x128x32-Re100", g, cylinder_params(100.0, 1e-3, 5.0, 5.0, 1.0, 1e-10), 400, 1e-3, 1)
+    return Workload("cyl-256x128x32-Re100", g, baseline_bc(cylinder_params(100.0, 1e-3, 5.0, 5.0, 1.0, 1e-10)),
+                    400, 1e-3, 1)
 
 
 def config_e(precision: int = capi.CFD_MIXED) -> Workload:

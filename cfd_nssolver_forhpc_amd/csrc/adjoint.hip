@@ -169,8 +169,19 @@ __global__ void __launch_bounds__(256) bc_y_adj_kernel(DevGrid g, BCNum b, doubl
   adj2(lu, at(b, 2, 0), m0, m1, m2);
 }
 
-// guin[(j-2)*nz_cal + (k-2)] accumulates the adjoint of the face-0 u value
-__global__ void __launch_bounds__(256) bc_x_adj_kernel(DevGrid g, BCNum b, double *__restrict__ lu,
+// convective outflow y[out] = (y[old] + s y[in]) / (1 + s) into the
+// positions outs (y[old] among them)  ->  l[old] = S / (1 + s),
+// l[in] += s S / (1 + s), the other outputs 0, with S the outputs' sum
+__device__ inline void adj_conv2(double *L, double s, int64_t n1, int64_t n2, int64_t in) {
+  const double S = L[n1] + L[n2];
+  L[n1] = 0.0;
+  L[n2] = S / (1.0 + s);
+  L[in] += s * S / (1.0 + s);
+}
+
+// guin[(j-2)*nz_cal + (k-2)] accumulates the adjoint of the face-0 u value;
+// conv_udt > 0: face 1 is the convective outflow of bc_x_kernel (U dt)
+__global__ void __launch_bounds__(256) bc_x_adj_kernel(DevGrid g, BCNum b, double conv_udt, double *__restrict__ lu,
                                                        double *__restrict__ lv, double *__restrict__ lw,
                                                        double *__restrict__ guin) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -180,9 +191,22 @@ __global__ void __launch_bounds__(256) bc_x_adj_kernel(DevGrid g, BCNum b, doubl
   const int64_t m0 = icel(g, 0, j, k), m1 = icel(g, 1, j, k), m2 = icel(g, 2, j, k);
   const int64_t n1 = icel(g, nx - 1, j, k), n2 = icel(g, nx - 2, j, k), n3 = icel(g, nx - 3, j, k),
                 n4 = icel(g, nx - 4, j, k);
-  adj2(lw, at(b, 1, 2), n1, n2, n3);  // BC_1
-  adj2(lv, at(b, 1, 1), n1, n2, n3);
-  adj3(lu, an(b, 1, 0), n1, n2, n3, n4);
+  if (conv_udt > 0.0) {
+    // u: the outflow face from its old value and the last interior face;
+    // v, w: the ghosts from their old value and the last cell
+    const double su = conv_udt / g.Dx[nx - 3], st = conv_udt / g.Dxs[nx - 3];
+    adj_conv2(lw, st, n1, n2, n3);
+    adj_conv2(lv, st, n1, n2, n3);
+    const double S = lu[n1] + lu[n2] + lu[n3];
+    lu[n1] = 0.0;
+    lu[n2] = 0.0;
+    lu[n3] = S / (1.0 + su);
+    lu[n4] += su * S / (1.0 + su);
+  } else {
+    adj2(lw, at(b, 1, 2), n1, n2, n3);  // BC_1
+    adj2(lv, at(b, 1, 1), n1, n2, n3);
+    adj3(lu, an(b, 1, 0), n1, n2, n3, n4);
+  }
   adj2(lw, at(b, 0, 2), m0, m1, m2);  // BC_0
   adj2(lv, at(b, 0, 1), m0, m1, m2);
   const double s = lu[m0] + lu[m1];
@@ -192,11 +216,40 @@ __global__ void __launch_bounds__(256) bc_x_adj_kernel(DevGrid g, BCNum b, doubl
   lu[m2] += an(b, 0, 0) * s;
 }
 
+// periodic z (bc_z_periodic_kernel): the ghost planes {0, 1, nz-2, nz-1} are
+// copies of {nz-4, nz-3, 2, 3} over the whole ghosted (i, j) plane, so their
+// adjoints fold into those images.  fold_into_images: the same for one array.
+__device__ inline void fold_z(double *L, int64_t base, int nz) {
+  const int dst[4] = {0, 1, nz - 2, nz - 1}, src[4] = {nz - 4, nz - 3, 2, 3};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    L[base + src[q]] += L[base + dst[q]];
+    L[base + dst[q]] = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(256) bc_z_periodic_adj_kernel(DevGrid g, double *__restrict__ lu,
+                                                                double *__restrict__ lv, double *__restrict__ lw) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)g.nx * g.ny) return;
+  const int64_t base = icel(g, (int)(t / g.ny), (int)(t % g.ny), 0);
+  fold_z(lu, base, g.nz);
+  fold_z(lv, base, g.nz);
+  fold_z(lw, base, g.nz);
+}
+
 // ---- update_UandF^T ------------------------------------------------------
 // u3 = (1 - gain e) T2, f = (u3 - T2)/dt with lf = dt*Lam (+ the drag weight
 // on f_x in the corrector): writes lt2 (3N, 0 where a component is not
 // updated) and the per-block partial of dJ/dgain.
-__global__ void __launch_bounds__(256) dfib_adj_kernel(DevGrid g, double dt, double gain, double drag_w,
+// Faces updated beyond the table ranges (BASELINE mode): the outflow face of
+// u (iu_end = nx - 2; no forcing there, e = 0) and the seam face of w
+// (kw_end = nz - 2).
+struct AdjFaces {
+  int iu_end, kw_end, conv, wrapz;
+};
+
+__global__ void __launch_bounds__(256) dfib_adj_kernel(DevGrid g, AdjFaces af, double dt, double gain, double drag_w,
                                                        const double *__restrict__ eta, const double *__restrict__ t2,
                                                        const double *__restrict__ lu3, const double *__restrict__ lv3,
                                                        const double *__restrict__ lw3, const double *__restrict__ Lam,
@@ -208,7 +261,7 @@ __global__ void __launch_bounds__(256) dfib_adj_kernel(DevGrid g, double dt, dou
     const ICell c = decode_cell(g, t);
     const int64_t ic = icel(g, c.i, c.j, c.k);
     const int idx[3] = {c.i, c.j, c.k};
-    const int lim[3] = {g.nx - GC - 1, g.ny - GC - 1, g.nz - GC - 1};
+    const int lim[3] = {af.iu_end, g.ny - GC - 1, af.kw_end};
     const int64_t sg[3] = {(int64_t)g.ny * g.nz, g.nz, 1};
     const double *Ds[3] = {g.Dxs, g.Dys, g.Dzs};
     const double *Dd[3] = {g.Dx, g.Dy, g.Dz};
@@ -219,7 +272,8 @@ __global__ void __launch_bounds__(256) dfib_adj_kernel(DevGrid g, double dt, dou
       if (idx[q] < lim[q]) {
         const int I = idx[q];
         const double ds = Ds[q][I];
-        const double e = eC + (eta[ic + sg[q]] - eC) * ((ds * 0.5) / Dd[q][I]);
+        const bool outflow = q == 0 && af.conv && I == g.nx - 3;
+        const double e = outflow ? 0.0 : eC + (eta[ic + sg[q]] - eC) * ((ds * 0.5) / Dd[q][I]);
         double lf = dt * Lam[q * N + t];
         if (q == 0 && drag_w != 0.0) lf += drag_w * (-2.0 / g.lz) * (g.Dx[c.i] * g.Dy[c.j] * g.Dz[c.k]);
         const double lu3t = (L3[q] ? L3[q][ic] : 0.0) + lf / dt;
@@ -240,8 +294,9 @@ __global__ void __launch_bounds__(FIN_THREADS) gain_finalize(const double *part,
 }
 
 // adjoint of the interior pressure: T2 = T1 - dt (p[+] - p) / ds per component
-__global__ void __launch_bounds__(256) lx_gather_kernel(DevGrid g, double dt, const double *__restrict__ lt2,
-                                                        double *__restrict__ lX) {
+// (with periodic z the seam face's +z neighbour is the first plane)
+__global__ void __launch_bounds__(256) lx_gather_kernel(DevGrid g, int wrapz, double dt,
+                                                        const double *__restrict__ lt2, double *__restrict__ lX) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   const ICell c = decode_cell(g, t);
@@ -255,6 +310,7 @@ __global__ void __launch_bounds__(256) lx_gather_kernel(DevGrid g, double dt, co
     const int I = idx[q];
     acc += dt / Ds[q][I] * lt2[q * N + t];
     if (I - 1 >= GC) acc -= dt / Ds[q][I - 1] * lt2[q * N + t - si[q]];
+    else if (q == 2 && wrapz) acc -= dt / Ds[2][g.nz - 3] * lt2[2 * N + t + (g.nzc - 1)];
   }
   lX[t] = acc;
 }
@@ -277,50 +333,68 @@ __global__ void __launch_bounds__(256) adj_rhs_back_kernel(DevGrid g, const doub
 }
 
 // Adjoint of T1 in one pass, over every ghosted position: the DFIB part
-// (lt2 where the component is updated) plus createBMatrix^T (b = -jp/dt
-// (du/dx + dv/dy + dw/dz) per interior cell), then added to the per-step sum
-// and, at interior cells, to the source-term adjoint (T1 = P + S there).
-__global__ void __launch_bounds__(256) t1_adj_kernel(DevGrid g, double ddt, int solved, const double *__restrict__ jp,
-                                                     const double *__restrict__ lb, const double *__restrict__ lt2,
-                                                     double *__restrict__ lsu, double *__restrict__ lsv,
-                                                     double *__restrict__ lsw, double *__restrict__ Lam) {
+// (lt2, zero where the component is not updated) plus createBMatrix^T
+// (b = -jp/dt (du/dx + dv/dy + dw/dz) per interior cell), then added to the
+// per-step sum and, at interior cells, to the source-term adjoint (T1 = P + S
+// there).  With periodic z the pass refreshes T1's z ghost planes from their
+// interior images before the right-hand side reads them, so a ghost plane's
+// adjoint is added to its image's and the ghost keeps none.
+__device__ inline double t1_val(const DevGrid &g, int ci, int cj, int ck, int q, double ddt, int solved,
+                                const double *__restrict__ jp, const double *__restrict__ lb,
+                                const double *__restrict__ lt2) {
+  const int idx[3] = {ci, cj, ck};
+  const int lo = GC, hi[3] = {g.nx - GC, g.ny - GC, g.nz - GC};
+  const bool in[3] = {ci >= lo && ci < hi[0], cj >= lo && cj < hi[1], ck >= lo && ck < hi[2]};
+  const bool interior = in[0] && in[1] && in[2];
+  const int64_t N = g.N;
+  const double *Dd[3] = {g.Dx, g.Dy, g.Dz};
+  const int64_t t = interior ? icelcal(g, ci, cj, ck) : 0;
+  double val = 0.0;
+  if (interior) val = lt2[q * N + t];
+  if (solved) {
+    if (interior) val += -jp[t] * ddt * lb[t] / Dd[q][idx[q]];
+    // the cell one step up along q reads this position as its lower face
+    const int up = idx[q] + 1;
+    bool cell_up = up >= lo && up < hi[q];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      if (r != q) cell_up = cell_up && in[r];
+    if (cell_up) {
+      int ii = ci, jj = cj, kk = ck;
+      if (q == 0) ii = up;
+      else if (q == 1) jj = up;
+      else kk = up;
+      const int64_t tu = icelcal(g, ii, jj, kk);
+      val -= -jp[tu] * ddt * lb[tu] / Dd[q][up];
+    }
+  }
+  return val;
+}
+
+__global__ void __launch_bounds__(256) t1_adj_kernel(DevGrid g, int wrapz, double ddt, int solved,
+                                                     const double *__restrict__ jp, const double *__restrict__ lb,
+                                                     const double *__restrict__ lt2, double *__restrict__ lsu,
+                                                     double *__restrict__ lsv, double *__restrict__ lsw,
+                                                     double *__restrict__ Lam) {
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= g.G) return;
   const ICell c = decode_ghosted(g, m);
-  const int idx[3] = {c.i, c.j, c.k};
-  const int lo = GC, hi[3] = {g.nx - GC, g.ny - GC, g.nz - GC};
-  const bool in[3] = {c.i >= lo && c.i < hi[0], c.j >= lo && c.j < hi[1], c.k >= lo && c.k < hi[2]};
-  const bool interior = in[0] && in[1] && in[2];
+  const int nz = g.nz;
+  if (wrapz && (c.k < GC || c.k >= nz - GC)) return;  // a refreshed ghost: its adjoint goes to its image
+  // with periodic z, the ghost plane that is this plane's image, if any
+  int img = -1;
+  if (wrapz) img = c.k == nz - 4 ? 0 : c.k == nz - 3 ? 1 : c.k == 2 ? nz - 2 : c.k == 3 ? nz - 1 : -1;
+  const bool interior = c.i >= GC && c.i < g.nx - GC && c.j >= GC && c.j < g.ny - GC && c.k >= GC && c.k < nz - GC;
   const int64_t N = g.N;
-  const int64_t si[3] = {(int64_t)g.nyc * g.nzc, g.nzc, 1};
-  const double *Dd[3] = {g.Dx, g.Dy, g.Dz};
   double *LS[3] = {lsu, lsv, lsw};
   const int64_t t = interior ? icelcal(g, c.i, c.j, c.k) : 0;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    double val = 0.0;
-    if (interior && idx[q] < hi[q] - 1) val = lt2[q * N + t];
-    if (solved) {
-      if (interior) val += -jp[t] * ddt * lb[t] / Dd[q][idx[q]];
-      // the cell one step up along q reads this position as its lower face
-      const int up = idx[q] + 1;
-      bool cell_up = up >= lo && up < hi[q];
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-        if (r != q) cell_up = cell_up && in[r];
-      if (cell_up) {
-        int ii = c.i, jj = c.j, kk = c.k;
-        if (q == 0) ii = up;
-        else if (q == 1) jj = up;
-        else kk = up;
-        const int64_t tu = icelcal(g, ii, jj, kk);
-        val -= -jp[tu] * ddt * lb[tu] / Dd[q][up];
-      }
-    }
+    double val = t1_val(g, c.i, c.j, c.k, q, ddt, solved, jp, lb, lt2);
+    if (img >= 0) val += t1_val(g, c.i, c.j, img, q, ddt, solved, jp, lb, lt2);
     LS[q][m] += val;
     if (interior) Lam[q * N + t] += val;
   }
-  (void)si;
 }
 
 // ---- QUICK predictor^T ----------------------------------------------------
@@ -366,7 +440,7 @@ __host__ __device__ inline int first_other(int c) { return c == 0 ? 1 : 0; }
 // 3+4a +2 s_a, 4+4a -2 s_a.  Convecting-velocity slots of velocity a in
 // output c (13.. for first_other(c), 17.. for the other): centre, +s_c,
 // -s_a, -s_a + s_c.
-__global__ void __launch_bounds__(256) pred_contrib_kernel(DevGrid g, double nu, double dt,
+__global__ void __launch_bounds__(256) pred_contrib_kernel(DevGrid g, int kw_end, double nu, double dt,
                                                            const double *__restrict__ u, const double *__restrict__ v,
                                                            const double *__restrict__ w, const double *__restrict__ lsu,
                                                            const double *__restrict__ lsv,
@@ -377,7 +451,7 @@ __global__ void __launch_bounds__(256) pred_contrib_kernel(DevGrid g, double nu,
   const int idx[3] = {cl.i, cl.j, cl.k};
   const int64_t ic = icel(g, cl.i, cl.j, cl.k);
   const int64_t sa[3] = {(int64_t)g.ny * g.nz, g.nz, 1};
-  const int lim[3] = {g.nx - GC - 1, g.ny - GC - 1, g.nz - GC - 1};
+  const int lim[3] = {g.nx - GC - 1, g.ny - GC - 1, kw_end};
   const double *Di[3] = {g.Dx, g.Dy, g.Dz}, *Ds[3] = {g.Dxs, g.Dys, g.Dzs};
   const double *F[3] = {u, v, w};
   const double *LY[3] = {lsu, lsv, lsw};
@@ -460,8 +534,9 @@ __global__ void __launch_bounds__(256) pred_contrib_kernel(DevGrid g, double nu,
   }
 }
 
-// positions BC_staggered_copy fills in T1 from T0 (updateBC.hpp:6-109)
-__device__ inline bool copied(const DevGrid &g, int comp, const ICell &c) {
+// positions BC_staggered_copy fills in T1 from T0 (updateBC.hpp:6-109); with
+// periodic z the predictor writes w's last face itself
+__device__ inline bool copied(const DevGrid &g, int wrapz, int comp, const ICell &c) {
   const bool ii = c.i >= GC && c.i < g.nx - GC, jj = c.j >= GC && c.j < g.ny - GC, kk = c.k >= GC && c.k < g.nz - GC;
   const bool gi = c.i < GC || c.i >= g.nx - GC, gj = c.j < GC || c.j >= g.ny - GC, gk = c.k < GC || c.k >= g.nz - GC;
   if (gi && jj && kk) return true;
@@ -469,11 +544,11 @@ __device__ inline bool copied(const DevGrid &g, int comp, const ICell &c) {
   if (gk && ii && jj) return true;
   if (comp == 0 && c.i == g.nx - GC - 1 && jj && kk) return true;
   if (comp == 1 && c.j == g.ny - GC - 1 && ii && kk) return true;
-  if (comp == 2 && c.k == g.nz - GC - 1 && ii && jj) return true;
+  if (comp == 2 && !wrapz && c.k == g.nz - GC - 1 && ii && jj) return true;
   return false;
 }
 
-__global__ void __launch_bounds__(256) pred_gather_kernel(DevGrid g, const double *__restrict__ contrib,
+__global__ void __launch_bounds__(256) pred_gather_kernel(DevGrid g, int wrapz, const double *__restrict__ contrib,
                                                           const double *__restrict__ lsu, const double *__restrict__ lsv,
                                                           const double *__restrict__ lsw, double *__restrict__ lu,
                                                           double *__restrict__ lv, double *__restrict__ lw) {
@@ -515,8 +590,12 @@ __global__ void __launch_bounds__(256) pred_gather_kernel(DevGrid g, const doubl
         if (o >= 0 && o < G) acc += cr[(int64_t)r * G + o];
       }
     }
-    if (copied(g, cp, c)) acc += LS[cp][m];
-    LX[cp][m] = acc;
+    if (copied(g, wrapz, cp, c)) acc += LS[cp][m];
+    // a ghost of T0 keeps its value through the step (the update writes the
+    // interior only), so the adjoint its end-of-step reads left there (the
+    // convective outflow's old ghost value) carries over
+    const bool ghost = c.i < GC || c.i >= g.nx - GC || c.j < GC || c.j >= g.ny - GC || c.k < GC || c.k >= g.nz - GC;
+    LX[cp][m] = ghost ? acc + LX[cp][m] : acc;
   }
 }
 
@@ -534,9 +613,14 @@ hipError_t launch_bc_adjoint(Ctx &c, AdjState &A) {
   const BCNum b = bc_num(c);
   const int64_t G = g.G;
   double *lu = A.lx, *lv = A.lx + G, *lw = A.lx + 2 * G;
-  bc_z_adj_kernel<<<blocks((int64_t)g.nxc * g.nyc), 256, 0, c.stream>>>(g, b, lu, lv, lw);
+  // reverse of launch_bc: z (periodic images, or the table), y, x
+  if (c.prm.periodic_z)
+    bc_z_periodic_adj_kernel<<<blocks((int64_t)g.nx * g.ny), 256, 0, c.stream>>>(g, lu, lv, lw);
+  else
+    bc_z_adj_kernel<<<blocks((int64_t)g.nxc * g.nyc), 256, 0, c.stream>>>(g, b, lu, lv, lw);
   bc_y_adj_kernel<<<blocks((int64_t)g.nxc * g.nzc), 256, 0, c.stream>>>(g, b, lu, lv, lw);
-  bc_x_adj_kernel<<<blocks((int64_t)g.nyc * g.nzc), 256, 0, c.stream>>>(g, b, lu, lv, lw, A.guin);
+  const double conv_udt = c.prm.outflow_convective ? c.prm.outflow_speed * c.prm.dt : 0.0;
+  bc_x_adj_kernel<<<blocks((int64_t)g.nyc * g.nzc), 256, 0, c.stream>>>(g, b, conv_udt, lu, lv, lw, A.guin);
   return hipGetLastError();
 }
 
@@ -609,16 +693,21 @@ int adjoint_step(Ctx &c, AdjState &A, const double *x, const double *t2, bool so
   ADJ_HIP(launch_fill(c, A.Lam, 0.0, 3 * N));
   const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
   double *part = red_part_host(c, RS_MISC);
+  AdjFaces af;
+  af.conv = P.outflow_convective;
+  af.wrapz = P.periodic_z;
+  af.iu_end = g.nx - GC - (af.conv ? 0 : 1);
+  af.kw_end = g.nz - GC - (af.wrapz ? 0 : 1);
   for (int q = L; q >= 0; --q) {
     const bool last = q == L;
-    dfib_adj_kernel<<<nb, 256, 0, c.stream>>>(g, P.dt, c.gain, last ? drag_w : 0.0, c.f[CFD_ETA],
+    dfib_adj_kernel<<<nb, 256, 0, c.stream>>>(g, af, P.dt, c.gain, last ? drag_w : 0.0, c.f[CFD_ETA],
                                               t2 + (size_t)q * 3 * N, last ? A.lx : nullptr,
                                               last ? A.lx + G : nullptr, last ? A.lx + 2 * G : nullptr, A.Lam,
                                               A.lt2, part);
     gain_finalize<<<1, FIN_THREADS, 0, c.stream>>>(part, nb, A.gacc);
     ADJ_HIP(hipGetLastError());
     if (solved) {
-      lx_gather_kernel<<<blocks(N), 256, 0, c.stream>>>(g, P.dt, A.lt2, A.lX);
+      lx_gather_kernel<<<blocks(N), 256, 0, c.stream>>>(g, af.wrapz, P.dt, A.lt2, A.lX);
       adj_rhs_form_kernel<<<blocks(N), 256, 0, c.stream>>>(g, c.f[CFD_JP], A.lX, A.rhs);
       ADJ_HIP(hipGetLastError());
       int it = 0;
@@ -634,14 +723,14 @@ int adjoint_step(Ctx &c, AdjState &A, const double *x, const double *t2, bool so
       A.adj_iters += it;
       adj_rhs_back_kernel<<<blocks(N), 256, 0, c.stream>>>(g, c.f[CFD_JP], A.z, A.lb);
     }
-    t1_adj_kernel<<<blocks(G), 256, 0, c.stream>>>(g, 1.0 / P.dt, solved ? 1 : 0, c.f[CFD_JP], A.lb, A.lt2, A.lsum,
-                                                   A.lsum + G, A.lsum + 2 * G, A.Lam);
+    t1_adj_kernel<<<blocks(G), 256, 0, c.stream>>>(g, af.wrapz, 1.0 / P.dt, solved ? 1 : 0, c.f[CFD_JP], A.lb,
+                                                   A.lt2, A.lsum, A.lsum + G, A.lsum + 2 * G, A.Lam);
     ADJ_HIP(hipGetLastError());
   }
-  pred_contrib_kernel<<<blocks(N), 256, 0, c.stream>>>(g, 1.0 / P.Re, P.dt, x, x + G, x + 2 * G, A.lsum, A.lsum + G,
-                                                       A.lsum + 2 * G, A.contrib);
-  pred_gather_kernel<<<blocks(G), 256, 0, c.stream>>>(g, A.contrib, A.lsum, A.lsum + G, A.lsum + 2 * G, A.lx,
-                                                      A.lx + G, A.lx + 2 * G);
+  pred_contrib_kernel<<<blocks(N), 256, 0, c.stream>>>(g, af.kw_end, 1.0 / P.Re, P.dt, x, x + G, x + 2 * G, A.lsum,
+                                                       A.lsum + G, A.lsum + 2 * G, A.contrib);
+  pred_gather_kernel<<<blocks(G), 256, 0, c.stream>>>(g, af.wrapz, A.contrib, A.lsum, A.lsum + G, A.lsum + 2 * G,
+                                                      A.lx, A.lx + G, A.lx + 2 * G);
   ADJ_HIP(hipGetLastError());
   return CFD_OK;
 }
@@ -652,8 +741,14 @@ int adjoint_checks(const Ctx &c) {
   if (c.prm.source_index_quirk) return fail(CFD_ERR_ARG, "adjoint: needs source_index_quirk = 0");
   if (c.prm.convection != CFD_CONV_QUICK) return fail(CFD_ERR_ARG, "adjoint: needs the QUICK convection scheme");
   if (c.prm.les) return fail(CFD_ERR_ARG, "adjoint: needs les = 0");
-  if (c.prm.periodic_z || c.prm.outflow_convective)
-    return fail(CFD_ERR_ARG, "adjoint: needs the table boundary conditions (periodic_z = outflow_convective = 0)");
+  if (c.prm.periodic_z) {
+    // the transposed solve relies on A = diag(jp) V^-1 S with S symmetric;
+    // the wrapped z coupling is symmetric on a uniform z axis
+    const double dz = c.hDz[0], dzs = c.hDzs[0];
+    for (size_t k = 0; k < c.hDz.size(); ++k)
+      if (std::fabs(c.hDz[k] - dz) > 1e-12 * dz || std::fabs(c.hDzs[k] - dzs) > 1e-12 * dzs)
+        return fail(CFD_ERR_ARG, "adjoint: periodic z needs a uniform z axis");
+  }
   if (c.g.nxc < 2 || c.g.nyc < 2 || c.g.nzc < 2) return fail(CFD_ERR_ARG, "adjoint: needs >= 2 cells per axis");
   if (!c.ell_ready) return fail(CFD_ERR_STATE, "adjoint: ELL operator not assembled");
   return CFD_OK;

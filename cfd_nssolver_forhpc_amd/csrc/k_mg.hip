@@ -14,7 +14,10 @@
 // diagonal share, cell widths); a smoothing sweep streams only x and b.
 // Coarse levels halve every axis that is even and at least 4 cells long;
 // their coefficients are rediscretised from the merged cell widths (a
-// Dirichlet face keeps its mirror-ghost coefficient).  Restriction is the
+// Dirichlet face keeps its mirror-ghost coefficient).  With periodic z every
+// level wraps in z like the operator (the first and last planes are
+// neighbours across the seam); z is halved only while its length stays even,
+// so the red-black colouring stays consistent across the seam.  Restriction is the
 // volume-weighted sum of the children's residuals, prolongation adds the
 // parent's correction.  Smoother: red-black Gauss-Seidel, nu sweeps before
 // and after.  Levels of at most MG_SMALL cells run inside one 512-thread
@@ -51,6 +54,7 @@ constexpr int MG_COARSE_SWEEPS = 48;
 // per-level device view: ax = cm[nx] cp[nx] dd[nx] w[nx] (and ay, az)
 struct MGView {
   int nx, ny, nz, fx, fy, fz;
+  int wz;  // periodic z: k = 0 and k = nz - 1 are neighbours (nz even)
   int64_t n;
   const double *ax, *ay, *az;
 };
@@ -93,6 +97,11 @@ void mg_free(T *&p) {
 
 __device__ inline int64_t idx3(const MGView &L, int i, int j, int k) { return ((int64_t)i * L.ny + j) * L.nz + k; }
 
+// the z index one step down / up, across the seam with periodic z (meaningful
+// where that neighbour exists)
+__device__ inline int kdn(const MGView &L, int k) { return k > 0 ? k - 1 : L.nz - 1; }
+__device__ inline int kup(const MGView &L, int k) { return k < L.nz - 1 ? k + 1 : 0; }
+
 // a cell's own index and its neighbours' (meaningful where the neighbour exists)
 struct Nb {
   int64_t c, xm, xp, ym, yp, zm, zp;
@@ -103,7 +112,7 @@ struct Natural {
   __device__ static int64_t at(const MGView &L, int i, int j, int k) { return idx3(L, i, j, k); }
   __device__ static Nb nbrs(const MGView &L, int i, int j, int k) {
     const int64_t c = idx3(L, i, j, k), sx = (int64_t)L.ny * L.nz, sy = L.nz;
-    return Nb{c, c - sx, c + sx, c - sy, c + sy, c - 1, c + 1};
+    return Nb{c, c - sx, c + sx, c - sy, c + sy, k > 0 ? c - 1 : c + (L.nz - 1), k < L.nz - 1 ? c + 1 : c - (L.nz - 1)};
   }
 };
 
@@ -126,7 +135,11 @@ struct ColourSplit {
     const int64_t h = ((int64_t)i * L.ny + j) * hz + (k >> 1), ns = half(L);
     const bool black = (i + j + k) & 1;
     const int64_t own = black ? ns + h : h, oth = black ? h : ns + h, sx = (int64_t)L.ny * hz;
-    return Nb{own, oth - sx, oth + sx, oth - hz, oth + hz, oth - ((k & 1) ? 0 : 1), oth + (k & 1)};
+    // across the periodic seam (nz even): k = 0's -z neighbour is k = nz - 1
+    // at k/2 = hz - 1, and k = nz - 1's +z neighbour k = 0 at k/2 = 0
+    const int64_t zm = k > 0 ? oth - ((k & 1) ? 0 : 1) : oth + (hz - 1);
+    const int64_t zp = k < L.nz - 1 ? oth + (k & 1) : oth - (hz - 1);
+    return Nb{own, oth - sx, oth + sx, oth - hz, oth + hz, zm, zp};
   }
 };
 
@@ -140,7 +153,7 @@ struct Has {
   bool xm, xp, ym, yp, zm, zp;
 };
 __device__ inline Has has6(const MGView &L, int i, int j, int k) {
-  return Has{i > 0, i < L.nx - 1, j > 0, j < L.ny - 1, k > 0, k < L.nz - 1};
+  return Has{i > 0, i < L.nx - 1, j > 0, j < L.ny - 1, L.wz || k > 0, L.wz || k < L.nz - 1};
 }
 
 // the six neighbour coefficients (an absent neighbour's entry is 0 and in range)
@@ -210,8 +223,8 @@ __device__ inline void rb_prolong_at(const MGView &F, const MGView &C, int i, in
   const T vxp = x[h.xp ? n.xp : n.c] + xc[parent(F, C, h.xp ? i + 1 : i, j, k)];
   const T vym = x[h.ym ? n.ym : n.c] + xc[parent(F, C, i, h.ym ? j - 1 : j, k)];
   const T vyp = x[h.yp ? n.yp : n.c] + xc[parent(F, C, i, h.yp ? j + 1 : j, k)];
-  const T vzm = x[h.zm ? n.zm : n.c] + xc[parent(F, C, i, j, h.zm ? k - 1 : k)];
-  const T vzp = x[h.zp ? n.zp : n.c] + xc[parent(F, C, i, j, h.zp ? k + 1 : k)];
+  const T vzm = x[h.zm ? n.zm : n.c] + xc[parent(F, C, i, j, h.zm ? kdn(F, k) : k)];
+  const T vzp = x[h.zp ? n.zp : n.c] + xc[parent(F, C, i, j, h.zp ? kup(F, k) : k)];
   const Coef6<T> a = coef6<T>(F, i, j, k);
   T s = (T)0;
   s = h.xm ? s + a.xm * vxm : s;
@@ -357,16 +370,19 @@ __global__ void __launch_bounds__(256) mg_split_sweep_kernel(MGView L, MGView C,
     const int e = (i + j + colour) & 1;
     const int k = min(2 * k2 + e, L.nz - 1);
     const int64_t h = i * sx + col;
-    // the k neighbour that is not cen: k2 - 1 below an even k, k2 + 1 above an odd one
-    const bool hside = e ? k < L.nz - 1 : k > 0;
+    // the k neighbour that is not cen: k2 - 1 below an even k, k2 + 1 above
+    // an odd one; across the periodic seam k2 = hz - 1 below k = 0 and
+    // k2 = 0 above k = nz - 1
+    const bool hside = L.wz || (e ? k < L.nz - 1 : k > 0);
+    const int64_t hs = e ? (k < L.nz - 1 ? h + 1 : h - (hz - 1)) : (k > 0 ? h - 1 : h + (hz - 1));
     vym[p] = oth[hym ? h - hz : h];
     vyp[p] = oth[hyp ? h + hz : h];
-    side[p] = oth[hside ? (e ? h + 1 : h - 1) : h];
+    side[p] = oth[hside ? hs : h];
     bv[p] = bown[h];
     if (MODE == SWEEP_PROLONG) {
       vym[p] += xc[par(i, hym ? j - 1 : j, k)];
       vyp[p] += xc[par(i, hyp ? j + 1 : j, k)];
-      side[p] += xc[par(i, j, hside ? (e ? k + 1 : k - 1) : k)];
+      side[p] += xc[par(i, j, hside ? (e ? kup(L, k) : kdn(L, k)) : k)];
     }
   }
 #pragma unroll
@@ -376,7 +392,7 @@ __global__ void __launch_bounds__(256) mg_split_sweep_kernel(MGView L, MGView C,
     const int k = 2 * k2 + e;
     if (MODE == SWEEP_OUT && i < L.nx && 2 * k2 + 1 - e < L.nz) out[idx3(L, i, j, 2 * k2 + 1 - e)] = (V)cen[p + 1];
     if (i >= L.nx || k >= L.nz) continue;
-    const bool hxm = i > 0, hxp = i < L.nx - 1, hzm = k > 0, hzp = k < L.nz - 1;
+    const bool hxm = i > 0, hxp = i < L.nx - 1, hzm = L.wz || k > 0, hzp = L.wz || k < L.nz - 1;
     const T vzm = e ? cen[p + 1] : side[p], vzp = e ? side[p] : cen[p + 1];
     const T cxm = (T)L.ax[i], cxp = (T)L.ax[L.nx + i];
     T s = (T)0;
@@ -410,10 +426,12 @@ __global__ void __launch_bounds__(256) mg_restrict_split_kernel(MGView F, MGView
   if (K >= C.nz || J >= C.ny) return;
   const int hz = F.nz >> 1;  // nz is even where z is halved
   const int64_t ns = (int64_t)F.nx * F.ny * hz;
-  // entry at k = 2 KK + m (m = 0, 1) of column (i, j), both clamped into the grid
+  // entry at k = 2 KK + m (m = 0, 1) of column (i, j), both clamped into the
+  // grid (KK wraps across the periodic seam)
   auto ld = [&](const T *a, int i, int j, int KK, int m) -> T {
     i = min(max(i, 0), F.nx - 1);
     j = min(max(j, 0), F.ny - 1);
+    if (F.wz) KK = KK < 0 ? hz - 1 : (KK >= hz ? 0 : KK);
     KK = min(max(KK, 0), hz - 1);
     return a[(((i + j + m) & 1) ? ns : 0) + ((int64_t)i * F.ny + j) * hz + KK];
   };
@@ -731,15 +749,17 @@ int mg_apply(Ctx &c, const V *r, V *out, const KrylovScalars *ks) {
 // per-axis level arrays from widths w (n) and the centre distances of the
 // finest level (ds_lo: centre of cell i-1 to i, for i = 0 the ghost
 // distance; ds_hi: centre of i to i+1, for i = n-1 the ghost distance)
+// (periodic: the end cells keep their coefficient to the neighbour across
+// the seam, whose distances dlo[0] and dhi[n-1] give)
 void axis_arrays(const std::vector<double> &w, const std::vector<double> &dlo, const std::vector<double> &dhi,
-                 bool dir_lo, bool dir_hi, std::vector<double> &out) {
+                 bool dir_lo, bool dir_hi, std::vector<double> &out, bool periodic = false) {
   const size_t n = w.size();
   out.assign(4 * n, 0.0);
   for (size_t i = 0; i < n; ++i) {
     const double xm = 1.0 / w[i] / dlo[i], xp = 1.0 / w[i] / dhi[i];
-    out[i] = i > 0 ? xm : 0.0;
-    out[n + i] = i + 1 < n ? xp : 0.0;
-    out[2 * n + i] = ((i > 0 || dir_lo) ? xm : 0.0) + ((i + 1 < n || dir_hi) ? xp : 0.0);
+    out[i] = (i > 0 || periodic) ? xm : 0.0;
+    out[n + i] = (i + 1 < n || periodic) ? xp : 0.0;
+    out[2 * n + i] = ((i > 0 || dir_lo || periodic) ? xm : 0.0) + ((i + 1 < n || dir_hi || periodic) ? xp : 0.0);
     out[3 * n + i] = w[i];
   }
 }
@@ -781,8 +801,8 @@ int mg_ensure(Ctx &c, bool fp32) {
         dhi[a].push_back((*Dsh[a])[I]);
       }
     }
-    // periodic z: the cycle closes the z faces (Neumann) and preconditions
-    // the wrapped operator the Krylov method applies
+    // periodic z: every level wraps in z (the fine level's seam coefficient
+    // is the operator's, from the ghost distances Dzs)
     const bool pz = c.prm.periodic_z != 0;
     const bool dl[3] = {c.prm.p_dirichlet[0] != 0, c.prm.p_dirichlet[2] != 0, !pz && c.prm.p_dirichlet[4] != 0};
     const bool dh[3] = {c.prm.p_dirichlet[1] != 0, c.prm.p_dirichlet[3] != 0, !pz && c.prm.p_dirichlet[5] != 0};
@@ -793,9 +813,10 @@ int mg_ensure(Ctx &c, bool fp32) {
       L.v.ny = (int)w[1].size();
       L.v.nz = (int)w[2].size();
       L.v.n = (int64_t)L.v.nx * L.v.ny * L.v.nz;
+      L.v.wz = pz ? 1 : 0;
       std::vector<double> arr[3], all;
       for (int a = 0; a < 3; ++a) {
-        axis_arrays(w[a], dlo[a], dhi[a], dl[a], dh[a], arr[a]);
+        axis_arrays(w[a], dlo[a], dhi[a], dl[a], dh[a], arr[a], a == 2 && pz);
         all.insert(all.end(), arr[a].begin(), arr[a].end());
         last_arr[a] = arr[a];
       }
@@ -807,6 +828,7 @@ int mg_ensure(Ctx &c, bool fp32) {
       L.v.az = L.coef + 4 * L.v.nx + 4 * L.v.ny;
       int f[3];
       for (int a = 0; a < 3; ++a) f[a] = (w[a].size() % 2 == 0 && w[a].size() >= 4) ? 2 : 1;
+      if (pz && (w[2].size() / 2) % 2 != 0) f[2] = 1;  // periodic z stays even
       const bool more = (f[0] > 1 || f[1] > 1 || f[2] > 1) && L.v.n > 64 && (int)M->lv.size() < MG_MAX_LEVELS - 1;
       L.v.fx = more ? f[0] : 1;
       L.v.fy = more ? f[1] : 1;
@@ -820,9 +842,10 @@ int mg_ensure(Ctx &c, bool fp32) {
         for (size_t q = 0; q < w[a].size(); q += 2) wc.push_back(w[a][q] + w[a][q + 1]);
         const size_t n = wc.size();
         std::vector<double> lo(n), hi(n);
+        const double seam = 0.5 * (wc[n - 1] + wc[0]);  // centre distance across a periodic seam
         for (size_t q = 0; q < n; ++q) {
-          lo[q] = q > 0 ? 0.5 * (wc[q - 1] + wc[q]) : wc[q];
-          hi[q] = q + 1 < n ? 0.5 * (wc[q] + wc[q + 1]) : wc[q];
+          lo[q] = q > 0 ? 0.5 * (wc[q - 1] + wc[q]) : (a == 2 && pz ? seam : wc[q]);
+          hi[q] = q + 1 < n ? 0.5 * (wc[q] + wc[q + 1]) : (a == 2 && pz ? seam : wc[q]);
         }
         w[a] = wc;
         dlo[a] = lo;
@@ -844,12 +867,14 @@ int mg_ensure(Ctx &c, bool fp32) {
           for (int k = 0; k < nz; ++k) {
             const int r = (i * ny + j) * nz + k;
             A[(size_t)r * n + r] = ax[2 * nx + i] + ay[2 * ny + j] + az[2 * nz + k];
-            if (i > 0) A[(size_t)r * n + r - ny * nz] = -ax[i];
-            if (i < nx - 1) A[(size_t)r * n + r + ny * nz] = -ax[nx + i];
-            if (j > 0) A[(size_t)r * n + r - nz] = -ay[j];
-            if (j < ny - 1) A[(size_t)r * n + r + nz] = -ay[ny + j];
-            if (k > 0) A[(size_t)r * n + r - 1] = -az[k];
-            if (k < nz - 1) A[(size_t)r * n + r + 1] = -az[nz + k];
+            if (i > 0) A[(size_t)r * n + r - ny * nz] -= ax[i];
+            if (i < nx - 1) A[(size_t)r * n + r + ny * nz] -= ax[nx + i];
+            if (j > 0) A[(size_t)r * n + r - nz] -= ay[j];
+            if (j < ny - 1) A[(size_t)r * n + r + nz] -= ay[ny + j];
+            // z neighbours, across the seam with periodic z (with nz = 2 both
+            // couplings of a cell reach the same neighbour and add)
+            if (k > 0 || Lc.wz) A[(size_t)r * n + r - k + (k > 0 ? k - 1 : nz - 1)] -= az[k];
+            if (k < nz - 1 || Lc.wz) A[(size_t)r * n + r - k + (k < nz - 1 ? k + 1 : 0)] -= az[nz + k];
           }
       for (int r = 0; r < n; ++r) Inv[(size_t)r * n + r] = 1.0;
       bool ok = true;

@@ -2026,7 +2026,8 @@ int orc_sim_gradient(orc_sim *s, double *g_uin, double *g_gain, int *adj_iters) 
   const cfd_grid_desc *g = &s->g;
   const dims_t d = s->d;
   const size_t G = d.G, N = d.N;
-  if (!s->adj_steps || P->source_index_quirk || P->op != CFD_OP_ELL || P->convection != CFD_CONV_QUICK || P->les)
+  if (!s->adj_steps || P->source_index_quirk || P->op != CFD_OP_ELL || P->convection != CFD_CONV_QUICK || P->les ||
+      P->periodic_z || P->outflow_convective) /* the boundary-table conditions only */
     return -1;
   const int passes = s->adj_passes, L = passes - 1;
   const double dt = P->dt, nu = 1.0 / P->Re;

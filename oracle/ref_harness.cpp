@@ -169,6 +169,51 @@ static int rules_main() {
   return 0;
 }
 
+// usage: ref_harness OUTDIR bench COL VAL N NIT -- the CPU baseline's
+// iteration: the reference's own bicgstabRe2 (bicgstabRe2.hpp:91-189) on its
+// own ELL_matrix, loaded from a slot-major [7][N] operator (int32 columns,
+// FP64 values; the 512x256x64 workload's operator is written by the bench),
+// right-hand side splitmix64 noise in [-1, 1), x = 0, run for exactly NIT
+// iterations (iters_max_ = NIT, tolerance 1e-300).  Prints one line:
+//   ref_bench n N nit I seconds S ms_per_iteration M threads T relres R
+// The grid sizes of the reference's grid class are compile-time constants
+// (structureGrid.hpp:44-96), so the workload's operator cannot be assembled by
+// the reference itself at this size; its solver is size-generic.
+static int bench_main(const char *colf, const char *valf, long n, int nit) {
+  std::vector<int32_t> col((size_t)7 * n);
+  std::vector<double> val((size_t)7 * n);
+  {
+    std::ifstream fc(colf, std::ios::binary), fv(valf, std::ios::binary);
+    fc.read(reinterpret_cast<char *>(col.data()), sizeof(int32_t) * col.size());
+    fv.read(reinterpret_cast<char *>(val.data()), sizeof(double) * val.size());
+    if (!fc || !fv) {
+      std::cerr << "ref_bench: cannot read the operator files\n";
+      return 2;
+    }
+  }
+  ELL_type A;
+  A.resize(n, 7);
+  for (long r = 0; r < n; ++r)
+    for (int q = 0; q < 7; ++q) {
+      const double v = val[(size_t)q * n + r];
+      if (v != 0.0) A.set(r, col[(size_t)q * n + r], v);
+    }
+  std::vector<double>().swap(val);
+  std::vector<int32_t>().swap(col);
+  std::vector<double> b(n), x(n, 0.0);
+  fill_noise(b, 20240611, 70, 0.0, 1.0);
+  solver::bicgstabRe2<ELL_type> sol(A);
+  sol.setTolerance(1e-300);
+  sol.iters_max_ = nit;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto [it, err] = sol(b, x);
+  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::cout << std::setprecision(9) << "ref_bench n " << n << " nit " << it << " seconds " << sec
+            << " ms_per_iteration " << 1e3 * sec / std::max(it, 1) << " threads " << omp_get_max_threads()
+            << " relres " << err << "\n";
+  return 0;
+}
+
 // usage: ref_harness OUTDIR synth -- every hot-path function once on seeded inputs
 static int synth_main() {
   simuClass simu;
@@ -379,6 +424,8 @@ int main(int argc, char **argv) {
     mkdir(g_out.c_str(), 0755);
     return io_main();
   }
+  if (argc >= 7 && std::string(argv[2]) == "bench")
+    return bench_main(argv[3], argv[4], std::atol(argv[5]), std::atoi(argv[6]));
   if (argc >= 3 && std::string(argv[2]) == "rules") {
     g_out = argv[1];
     mkdir(g_out.c_str(), 0755);

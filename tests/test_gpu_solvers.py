@@ -127,12 +127,15 @@ def test_mg_bicgstab_reaches_the_jacobi_solution(require_gpu, nu):
     assert rel_l2(x, x64) < 1e-7
 
 
-@pytest.mark.parametrize("op", [capi.CFD_OP_SPE, capi.CFD_OP_CSR])
+@pytest.mark.parametrize("op", [capi.CFD_OP_SPE])
 @pytest.mark.parametrize("pc32", [False, True])
-def test_mg_bicgstab_on_the_spe_and_csr_operators(require_gpu, op, pc32):
+def test_mg_bicgstab_on_the_spe_operator(require_gpu, op, pc32):
     """The preconditioned iteration applies the configured operator (SPE over
-    plane columns, CSR staged through LDS): the same solution and true FP64
-    residual as on the ELL operator, in about as many iterations."""
+    plane columns): the same solution and true FP64 residual as on the ELL
+    operator, in about as many iterations.  (The reference's CSR builder
+    closes every face with the Neumann term whatever the pressure table
+    says, so with this case's Dirichlet outlet it is another, singular
+    operator.)"""
     ref, b = poisson_sim(ncal=(64, 32, 16))
     ref.bicgstab(tol=1e-13, maxit=50000)
     x64 = ref.download(capi.X)
