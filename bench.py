@@ -1,18 +1,26 @@
 """Benchmark of the prediction-correction timestep on one MI355X.
 
 Workload: 3-D flow past a z-cylinder, 512x256x64 uniform cells over
-20D x 10D x 2.5D (D = 25 cells), Re 200, dt 5e-4, pressure by BiCGSTAB on the
-Jacobi-scaled ELL 7-point operator to ||b - Ax|| / ||b|| < 1e-10 (FP64).
-A step is one iteration of the reference's main loop: 3 predictor passes and
-a corrector, each with its pressure solve.
+20D x 10D x 2.5D (D = 25 cells), Re 200, dt 5e-4, uniform inflow, convective
+outflow, slip y-walls, periodic z.  A step is one iteration of the
+reference's main loop: 3 predictor passes and a corrector, each with its
+pressure solve.
 
-Prints one JSON line.  `value` is FP64 timesteps per second.  Reported beside
-it: the same run with mixed-precision pressure solves and with the
-reference's default matrix-free (SPE) operator, the standalone Poisson
-benchmark on that grid (seeded zero-mean N(0,1) RHS; BiCGSTAB FP64 and mixed,
-red-black SOR omega 1.8), the ELL SpMV roofline measured with HIP events on
-the library's stream, and the CPU baseline (the FP64 oracle on the host's
-cores, a bounded sample of the same workload).
+Prints one JSON line.  `value` is FP64 timesteps per second with the
+reference's own pressure solve: bicgstabRe2 on the Jacobi-scaled ELL
+operator, tol 1e-10, iters_max 5000.  At this size many of its passes stop
+at iters_max above 1e-10; the line states how many and the worst residual.
+`same_residual` compares FP64 and mixed precision where every pass reaches
+the FP64 relative residual 1e-10 (multigrid-preconditioned BiCGSTAB: FP64
+throughout, FP64 Krylov around an FP32 V-cycle, FP64 defect correction
+around FP32 inner solves).  Reported beside them: the reference's default
+matrix-free (SPE) operator, the standalone Poisson problem on the grid, the
+256x128x32 discrete adjoint, the kernel table, the roofline of the kernel
+the timed step spends most in (the SpMV-dot of the BiCGSTAB iteration,
+timed with HIP events on the library's stream), and the CPU baseline on the
+host's cores, a bounded sample of the same workload: the reference's own
+bicgstabRe2 iterations (compiled from its sources) on the workload's operator,
+and the non-solver part of a step from the FP64 oracle.
 """
 from __future__ import annotations
 
@@ -34,27 +42,42 @@ from cfd_nssolver_forhpc_amd import capi, configs  # noqa: E402
 from cfd_nssolver_forhpc_amd.solver import Simulation  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+try:  # the host cores this process may use, before an OpenMP runtime binds its first thread to one of them
+    HOST_CPUS = os.sched_getaffinity(0)
+except (AttributeError, OSError):
+    HOST_CPUS = None
 # algorithmic bytes per row: ELL SpMV 7 x (val + col) + x + y; SPE x + jp + y;
 # BiCGSTAB iteration (5 fused kernels) 320; RB-SOR sweep 84 + x + b + x(w) + x(nbr)
+# SpMV-dot (K1 of the iteration): the SpMV's 100 plus r0 8
 BYTES = {"spmv_fp64": 100, "spmv_fp32": 64, "spe_fp64": 24, "bicg_iter": 320, "sor_sweep": 116,
-         "copy": 16, "predictor": 48}
+         "copy": 16, "predictor": 48, "spmv_dot": 108}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def profiled_traffic():
-    """HBM bytes per ELL SpMV launch from the committed rocprofv3 counter
+def profiled_traffic(key="spmv_dot_kernel<double> traffic_bytes_per_launch:"):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration)."""
     path = ROOT / "profiles" / "kernels_512x256x64.md"
     try:
         for line in path.read_text().splitlines():
-            if line.startswith("ELL SpMV FP64 traffic_bytes_per_launch:"):
+            if line.startswith(key):
                 return float(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+def host_cpu():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def run_steps(w, warmup, steps, tag):
@@ -91,9 +114,11 @@ def per_step(stats, key):
     return float(np.mean([sum(max(i, 0) for i in s[key]) for s in stats]))
 
 
-def poisson_benchmark(w):
+def poisson_benchmark(w, sor_full=False):
     """Standalone Poisson solve on the 512x256x64 grid: ELL operator with
-    Neumann walls and p = 0 on the outflow face, seeded zero-mean N(0,1) RHS."""
+    Neumann walls and p = 0 on the outflow face, seeded zero-mean N(0,1) RHS;
+    BiCGSTAB FP64 and mixed, red-black SOR (omega 1.8) capped at 2000 sweeps
+    or, with sor_full, to 1e-10."""
     out = {}
     rng = np.random.default_rng(2)
     b = rng.standard_normal(w.grid.N)
@@ -102,6 +127,7 @@ def poisson_benchmark(w):
                               ("rbsor_fp64", capi.CFD_FP64, 1)):
         p = w.params.copy()
         p.precision, p.solver, p.maxit = prec, solver, 20000
+        p.periodic_z = p.outflow_convective = 0  # Neumann walls, Dirichlet outflow face
         sim = Simulation(w.grid, p)
         sim.init_state()
         sim.createPressureMatrix()
@@ -109,13 +135,15 @@ def poisson_benchmark(w):
         sim.upload(capi.X, np.zeros(w.grid.N))
         t0 = time.perf_counter()
         if solver == 1:
-            it, rel = sim.rbsor(omega=1.8, tol=1e-10, maxit=2000, check_every=100)
+            cap = 3000000 if sor_full else 2000
+            it, rel = sim.rbsor(omega=1.8, tol=1e-10, maxit=cap, check_every=1000 if sor_full else 100)
         else:
             it, rel = sim.solve()
         dt = time.perf_counter() - t0
         rec = {"seconds": dt, "iters": it, "relres": rel}
         if solver == 1:
-            rec["note"] = "capped at 2000 sweeps (omega 1.8 needs ~1e6 sweeps for 1e-10 at this size)"
+            rec["note"] = ("to 1e-10" if sor_full else
+                           "capped at 2000 sweeps (omega 1.8 needs ~8e5 sweeps for 1e-10 at this size; --poisson-sor-full)")
             rec["us_per_sweep"] = dt / max(it, 1) * 1e6
         out[tag] = rec
         log(f"[poisson] {tag}: {it} iters, relres {rel:.2e}, {dt:.2f} s")
@@ -123,13 +151,68 @@ def poisson_benchmark(w):
     return out
 
 
-def cpu_baseline(w, iters_per_step, threads):
-    """FP64 oracle (OpenMP) on the host: one step without pressure solves
-    (the loop == 1 step: 4 passes of predictor, source, DFIB update, BC) plus
-    a fixed count of BiCGSTAB iterations on the same ELL operator; the step
-    time is t_nonsolver + iters_per_step * t_iteration."""
+# The restatement against the compiled reference in the build container (its
+# own ELL BiCGSTAB loop, 190x210x40, tol 1e-3, steps 2-6, 8 OpenMP threads,
+# Xeon @ 2.70 GHz): 33.7 s for 3373 iterations against the restatement's
+# 40.2 s for 3249, i.e. 10.0 against 12.4 ms per iteration with its step work.
+CALIBRATION = {"reference_ms_per_iteration": 10.0, "restatement_ms_per_iteration": 12.4, "threads": 8,
+               "case": "190x210x40 native grid, ELL BiCGSTAB tol 1e-3, steps 2-6"}
+
+
+def reference_iterations(col, val, n, nit, threads, timeout=600):
+    """Milliseconds per iteration of the reference's own bicgstabRe2 on its own
+    ELL_matrix (oracle/_ref/ref_bench, compiled from the reference's sources),
+    on the given slot-major operator; None where the binary is absent."""
+    exe = ROOT / "oracle" / "_ref" / "ref_bench"
+    if not exe.exists():
+        return None
+    import shutil
+    import tempfile
+    tmp = tempfile.mkdtemp(prefix="cfd_ref_bench_")
+    try:
+        cf, vf = os.path.join(tmp, "col.bin"), os.path.join(tmp, "val.bin")
+        col.astype(np.int32).tofile(cf)
+        val.astype(np.float64).tofile(vf)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        # the child inherits the calling thread's CPU set, which the oracle's
+        # OpenMP runtime (OMP_PROC_BIND) has narrowed to one core: widen it
+        # for the launch, so that the reference's threads get every core
+        prev = os.sched_getaffinity(0) if HOST_CPUS else None
+        try:
+            if HOST_CPUS:
+                os.sched_setaffinity(0, HOST_CPUS)
+            out = subprocess.run([str(exe), tmp, "bench", cf, vf, str(n), str(nit)], env=env, capture_output=True,
+                                 text=True, timeout=timeout)
+        finally:
+            if prev:
+                os.sched_setaffinity(0, prev)
+        for line in out.stdout.splitlines():
+            if line.startswith("ref_bench"):
+                f = line.split()
+                rec = {f[k]: float(f[k + 1]) for k in range(1, len(f) - 1, 2)}
+                return rec
+        log(f"[bench] ref_bench: rc {out.returncode}: {out.stderr.strip()[-300:]}")
+    except Exception as e:
+        log(f"[bench] ref_bench: {e}")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return None
+
+
+def cpu_baseline(w, iters_per_step, threads, nit=300, ref_nit=600):
+    """The workload on the host's cores at 512x256x64 with its boundary
+    conditions, FP64: one timestep without pressure solves (the loop == 1
+    step: 4 passes of predictor, source, DFIB update, BC) of the oracle, plus
+    ref_nit iterations of the reference's own bicgstabRe2 (oracle/_ref/ref_bench)
+    on the workload's ELL operator, and nit iterations of the oracle's
+    restatement of it for comparison on the same host.  The step time is
+    t_nonsolver + iters_per_step * t_iteration (iters_per_step: the GPU run's
+    count, the same algorithm); t_iteration is the reference's where its
+    binary runs, else the restatement's."""
     sys.path.insert(0, str(ROOT / "tests"))
     os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import oracle_lib as O
     flags = "-O3 -march=x86-64-v3 -fopenmp"
     lib = None
@@ -148,34 +231,55 @@ def cpu_baseline(w, iters_per_step, threads):
     t0 = time.perf_counter()
     sim.step()  # loop 1: no pressure solve
     t_nonsolver = time.perf_counter() - t0
+    del sim
     N = g.N
     col = np.zeros(7 * N, np.int32)
     val = np.zeros(7 * N)
     jp = np.zeros(N)
-    lib.orc_ell_assemble(g.ref, O.ints6([w.params.p_dirichlet[f] for f in range(6)]), col, val, jp)
+    lib.orc_ell_assemble_bl(g.ref, O.ints6([w.params.p_dirichlet[f] for f in range(6)]), w.params.periodic_z,
+                            col, val, jp)
     b = np.random.default_rng(2).standard_normal(N)
     x = np.zeros(N)
     s = lib.orc_bicg_create(N, 0, col, val, g.ref, None, None)
-    nit = 20
     lib.orc_bicg_set_tolerance(s, 1e-300)
-    lib.orc_bicg_set_maxit(s, nit - 1)
     it = C.c_int()
+    # a short untimed solve first: the solver's work vectors are allocated
+    # zeroed and their pages are touched by the first iterations (the
+    # reference's solver fills its vectors when it is built, untimed)
+    lib.orc_bicg_set_maxit(s, 9)
+    lib.orc_bicg_solve(s, b, x, C.byref(it))
+    x[:] = 0.0
+    lib.orc_bicg_set_maxit(s, nit - 1)
     t0 = time.perf_counter()
     lib.orc_bicg_solve(s, b, x, C.byref(it))
-    t_iter = (time.perf_counter() - t0) / nit
+    t_port = (time.perf_counter() - t0) / nit
     lib.orc_bicg_destroy(s)
+    del b, x, jp
+    ref = reference_iterations(col, val, N, ref_nit, threads)
+    kind = "reference" if ref else "port"
+    t_iter = ref["ms_per_iteration"] * 1e-3 if ref else t_port
     step_s = t_nonsolver + iters_per_step * t_iter
+    it_src = (f"{int(ref['nit'])} iterations of the reference's own bicgstabRe2 on its own ELL_matrix "
+              f"(oracle/_ref/ref_bench, g++ -O3 -march=x86-64-v3 -fopenmp from the reference's sources) on the "
+              f"workload's operator, timed ({t_iter * 1e3:.1f} ms each; the restatement's {t_port * 1e3:.1f} ms "
+              f"on the same host)") if ref else (
+              f"{nit} iterations of the restatement of bicgstabRe2 on the workload's operator, timed "
+              f"({t_port * 1e3:.1f} ms each)")
     return {
         "value": 1.0 / step_s,
         "unit": "timesteps/s",
         "cores": threads,
-        "kind": "port",
-        "sample": (f"FP64 oracle ({flags}, {threads} OpenMP threads) at 512x256x64: one PC-method step without "
-                   f"solves ({t_nonsolver:.2f} s) + {nit} BiCGSTAB iterations ({t_iter * 1e3:.1f} ms each), "
-                   f"step = non-solver + {iters_per_step:.0f} iterations (the GPU run's count per step)"),
+        "kind": kind,
+        "sample": (f"512x256x64 with the workload's boundaries, FP64, {threads} OpenMP threads on {host_cpu()}: one "
+                   f"PC-method step without solves of the oracle ({flags}), timed ({t_nonsolver:.2f} s), plus "
+                   f"{it_src}; step = non-solver + {iters_per_step:.0f} iterations (the GPU run's count per step, "
+                   f"the same algorithm)"),
         "t_nonsolver_s": t_nonsolver,
         "t_iteration_ms": t_iter * 1e3,
-        "host": os.uname().nodename,
+        "t_iteration_restatement_ms": t_port * 1e3,
+        "reference_bench": ref,
+        "host_cpu": host_cpu(),
+        "calibration_build_host": CALIBRATION,
     }
 
 
@@ -190,8 +294,8 @@ def kernel_table(sim, reps):
         nbytes["spmv_csr"] = (12 * nnz + 4 * (N + 1) + 16 * N) / N
     except Exception as e:  # CSR operator not assembled for this context
         log(f"[bench] kernel spmv_csr: {e}")
-    for what, name in ((0, "spmv_fp64"), (1, "spmv_fp32"), (2, "spe_fp64"), (7, "spmv_csr"), (6, "copy"),
-                       (5, "predictor"), (4, "bicg_iter"), (3, "sor_sweep")):
+    for what, name in ((8, "spmv_dot"), (0, "spmv_fp64"), (1, "spmv_fp32"), (2, "spe_fp64"), (7, "spmv_csr"),
+                       (6, "copy"), (5, "predictor"), (4, "bicg_iter"), (3, "sor_sweep")):
         if name not in nbytes:
             continue
         try:
@@ -252,21 +356,92 @@ def adjoint_benchmark(steps, avg_from, every, fd_dirs, precond, tol=1e-10, h=1e-
     }
 
 
+def passes_report(stats, maxit):
+    """What the timed solves reached: bicgstabRe2 stops at iters_max as the
+    reference does, whatever the residual then is."""
+    passes = [(it, rel) for s in stats for it, rel in zip(s["iters"], s["relres"]) if it >= 0]
+    if not passes:
+        return None
+    return {"passes": len(passes), "passes_at_iteration_cap": sum(1 for it, _ in passes if it >= maxit),
+            "iteration_cap": maxit, "max_final_relres": max(rel for _, rel in passes),
+            "median_final_relres": float(np.median([rel for _, rel in passes]))}
+
+
+def field_errors(a, b):
+    err = {}
+    for name, fid in (("u", capi.T0_U), ("v", capi.T0_V), ("w", capi.T0_W), ("p", capi.X)):
+        x, y = a.download(fid), b.download(fid)
+        err[name] = float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+    return err
+
+
+def same_residual_legs(steps, warmup, op=None, nu=2):
+    """FP64 and mixed precision at the same FP64 relative residual 1e-10 in
+    every pass: BiCGSTAB on the ELL operator right-preconditioned by the
+    multigrid V-cycle (FP64; FP64 Krylov around the FP32 cycle; FP64 defect
+    correction around FP32 preconditioned BiCGSTAB to 1e-5 on FP32 ELL values)."""
+    out, sims = {}, {}
+    for tag, prec, mode in (("fp64_mg", capi.CFD_FP64, capi.CFD_MIXED_RESTARTED),
+                            ("mixed_mg", capi.CFD_MIXED, capi.CFD_MIXED_PC32),
+                            ("mixed_mg_dc", capi.CFD_MIXED, capi.CFD_MIXED_RESTARTED)):
+        wg = configs.config_b(prec)
+        wg.params.precond = capi.CFD_PRECOND_MG
+        wg.params.mixed_mode = mode
+        wg.params.tol_inner = 1e-5
+        wg.params.maxit = 200
+        wg.params.mg_nu = nu
+        if op is not None:
+            wg.params.op = op
+        wg.name += "-mg"
+        simg, tg, stg = run_steps(wg, warmup, steps, tag)
+        rep = passes_report(stg, wg.params.maxit)
+        out[tag] = {"timesteps_per_s": steps / tg, "ms_per_step": 1e3 * tg / steps,
+                    "solver_iters_per_step": per_step(stg, "iters"),
+                    "inner_iters_per_step": per_step(stg, "inner_iters"),
+                    "max_final_relres": rep["max_final_relres"],
+                    "all_passes_at_1e-10": rep["max_final_relres"] < 1e-10 and rep["passes_at_iteration_cap"] == 0,
+                    "divergence_fluid_max": max(s["div"]["fluid_max"] for s in stg),
+                    "phases": phase_report(simg)}
+        sims[tag] = simg
+    for tag in ("mixed_mg", "mixed_mg_dc"):
+        out[tag]["rel_l2_vs_fp64_mg_after_timed_steps"] = field_errors(sims[tag], sims["fp64_mg"])
+        out[tag]["speedup_vs_fp64_mg"] = out[tag]["timesteps_per_s"] / out["fp64_mg"]["timesteps_per_s"]
+    for simg in sims.values():
+        simg.close()
+    out["residual"] = 1e-10
+    out["operator"] = "matrix-free SPE" if op == capi.CFD_OP_SPE else "ELL"
+    out["sweeps"] = nu
+    out["note"] = ("every pass of every leg stops at ||b - Ax||/||b|| < 1e-10 in FP64 (all_passes_at_1e-10); "
+                   "fp64_mg: FP64 throughout; mixed_mg: FP64 BiCGSTAB with the V-cycle in FP32; mixed_mg_dc: FP64 "
+                   "defect correction around FP32 preconditioned BiCGSTAB (FP32 ELL values) to 1e-5; "
+                   "speedup_vs_fp64_mg is at that common residual")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=6, help="timed steps of the headline (reference-algorithm) leg")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--mg-steps", type=int, default=20, help="timed steps of each same-residual leg")
+    ap.add_argument("--spe-steps", type=int, default=3, help="timed steps of the matrix-free (SPE) leg (0: skip)")
     ap.add_argument("--no-spe", action="store_true")
     ap.add_argument("--no-poisson", action="store_true")
+    ap.add_argument("--poisson-sor-full", action="store_true",
+                    help="run the Poisson problem's red-black SOR (omega 1.8) to 1e-10 instead of 2000 sweeps "
+                         "(about 8e5 sweeps, minutes)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=300,
+                    help="iterations of the restatement's BiCGSTAB the CPU baseline times")
+    ap.add_argument("--cpu-ref-iters", type=int, default=600,
+                    help="iterations of the reference's own BiCGSTAB (oracle/_ref/ref_bench) the CPU baseline times")
     ap.add_argument("--no-mg", action="store_true")
-    ap.add_argument("--mg-steps", type=int, default=20, help="timed steps of the multigrid-preconditioned runs")
-    ap.add_argument("--mg-only", action="store_true", help="run only the multigrid legs (FP64 reference speed from "
-                    "--ref-tps) and print their JSON")
-    ap.add_argument("--ref-tps", type=float, default=float("nan"),
-                    help="with --mg-only: the Jacobi FP64 timesteps/s to compare against")
+    ap.add_argument("--mg-only", action="store_true", help="run only the same-residual legs and print their JSON")
+    ap.add_argument("--mg-op", choices=("ell", "spe"), default="spe",
+                    help="operator of the same-residual legs' FP64 Krylov and residual (default: the matrix-free "
+                         "SPE form, the reference's default operator; the defect correction's FP32 inner solves "
+                         "use the FP32 ELL values either way)")
+    ap.add_argument("--mg-nu", type=int, default=2, help="red-black sweeps before and after the coarse correction")
     ap.add_argument("--reps", type=int, default=200)
     ap.add_argument("--adjoint-steps", type=int, default=40,
                     help="steps of the 256x128x32 adjoint run (0: skip); the drag is averaged over the second half")
@@ -287,6 +462,7 @@ def main():
               "(the hot path runs on a single MI355X; multi-GPU x-slab sharding is design-only)", file=sys.stderr)
         sys.exit(2)
     warmup = max(args.warmup, 2)  # step 1 of the reference loop has no pressure solve
+    mg_op = capi.CFD_OP_SPE if args.mg_op == "spe" else capi.CFD_OP_ELL
 
     if args.adjoint_only:
         steps = max(args.adjoint_steps, 2)
@@ -310,89 +486,37 @@ def main():
         print(json.dumps({"kernels": kernel_table(sim, args.reps)}), flush=True)
         return
     if args.mg_only:
-        sim64, tps64, iters64, kt, spmv, st64 = None, args.ref_tps, float("nan"), {}, {}, []
-        phases64 = None
-    else:
-        log(f"[bench] {w64.name}: warmup {warmup}, timed {args.steps}")
-        sim64, t64, st64 = run_steps(w64, warmup, args.steps, "fp64")
-        tps64 = args.steps / t64
-        phases64 = phase_report(sim64)
-        iters64 = per_step(st64, "iters")
-        # kernel roofline table; the dominant kernel is the ELL SpMV (FP64)
-        sim64.setupPressure()
-        sim64.createPressureMatrixCSR()
-        kt = kernel_table(sim64, args.reps)
-        spmv = kt["spmv_fp64"]
-
-    mixed = None
-    if not args.no_mixed and not args.mg_only:
-        wm = configs.config_b(capi.CFD_MIXED)
-        simm, tm, stm = run_steps(wm, warmup, args.steps, "mixed")
-        err = {}
-        for name, fid in (("u", capi.T0_U), ("v", capi.T0_V), ("w", capi.T0_W), ("p", capi.X)):
-            a, b = simm.download(fid), sim64.download(fid)
-            err[name] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
-        mixed = {
-            "scheme": "FP64 BiCGSTAB on FP32 ELL values (FP64 accumulation), FP64 residual replacement at 10x drops",
-            "timesteps_per_s": args.steps / tm,
-            "speedup_vs_fp64": (args.steps / tm) / tps64,
-            "replacements_per_step": per_step(stm, "iters"),
-            "iterations_per_step": per_step(stm, "inner_iters"),
-            "max_final_relres": max(max(s["relres"]) for s in stm),
-            "rel_l2_vs_fp64_after_timed_steps": err,
-        }
-        simm.close()
-
-    mg = None
-    if not args.no_mg:
-        mg = {}
-        sims = {}
-        variants = (("fp64_mg", capi.CFD_FP64, capi.CFD_MIXED_RESTARTED),
-                    ("mixed_mg", capi.CFD_MIXED, capi.CFD_MIXED_PC32),
-                    ("mixed_mg_dc", capi.CFD_MIXED, capi.CFD_MIXED_RESTARTED))
-        for tag, prec, mode in variants:
-            wg = configs.config_b(prec)
-            wg.params.precond = capi.CFD_PRECOND_MG
-            wg.params.mixed_mode = mode
-            wg.params.tol_inner = 1e-5
-            wg.name += "-mg"
-            simg, tg, stg = run_steps(wg, warmup, args.mg_steps, tag)
-            mg[tag] = {"timesteps_per_s": args.mg_steps / tg, "speedup_vs_fp64": (args.mg_steps / tg) / tps64,
-                       "solver_iters_per_step": per_step(stg, "iters"),
-                       "inner_iters_per_step": per_step(stg, "inner_iters"),
-                       "max_final_relres": max(max(s["relres"]) for s in stg), "phases": phase_report(simg)}
-            sims[tag] = simg
-        # mixed runs against the FP64 run with the same preconditioner, after the same steps
-        ref_mg = sims["fp64_mg"]
-        for tag in ("mixed_mg", "mixed_mg_dc"):
-            err = {}
-            for name, fid in (("u", capi.T0_U), ("v", capi.T0_V), ("w", capi.T0_W), ("p", capi.X)):
-                x, y = sims[tag].download(fid), ref_mg.download(fid)
-                err[name] = float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
-            mg[tag]["rel_l2_vs_fp64_mg_after_timed_steps"] = err
-            mg[tag]["speedup_vs_fp64_mg"] = mg[tag]["timesteps_per_s"] / mg["fp64_mg"]["timesteps_per_s"]
-        for simg in sims.values():
-            simg.close()
-        mg["note"] = ("BiCGSTAB right-preconditioned by a cell-centred multigrid V-cycle (red-black Gauss-Seidel 2+2, "
-                      "matrix-free levels), every solve to ||b - Ax||/||b|| < 1e-10 in FP64. fp64_mg: FP64 throughout; "
-                      "mixed_mg: FP64 BiCGSTAB with the V-cycle in FP32; mixed_mg_dc: FP64 defect correction around "
-                      "FP32 preconditioned BiCGSTAB (FP32 ELL values) to 1e-5. speedup_vs_fp64 is against the "
-                      "reference's Jacobi-scaled FP64 solve (the bench value)")
-
-    if args.mg_only:
-        print(json.dumps({"multigrid": mg}), flush=True)
+        print(json.dumps({"same_residual": same_residual_legs(args.mg_steps, warmup, mg_op, args.mg_nu)}), flush=True)
         return
 
+    # ---- headline: the reference's algorithm, FP64 ----
+    log(f"[bench] {w64.name}: warmup {warmup}, timed {args.steps}")
+    sim64, t64, st64 = run_steps(w64, warmup, args.steps, "fp64")
+    tps64 = args.steps / t64
+    phases64 = phase_report(sim64)
+    iters64 = per_step(st64, "iters")
+    solves = passes_report(st64, w64.params.maxit)
+    counters = sim64.solver_counters()
+    # kernel table; the timed step spends most in the iteration's SpMV-dot kernels
+    sim64.setupPressure()
+    sim64.createPressureMatrixCSR()
+    kt = kernel_table(sim64, args.reps)
+    sim64.close()
+
+    mg = None if args.no_mg else same_residual_legs(args.mg_steps, warmup, mg_op, args.mg_nu)
+
     spe = None
-    if not args.no_spe:
+    if not args.no_spe and args.spe_steps > 0:
         ws = configs.config_b(capi.CFD_FP64)
         ws.params.op = capi.CFD_OP_SPE
-        sims, ts, sts = run_steps(ws, warmup, args.steps, "spe")
-        spe = {"operator": "matrix-free SPE (the reference's default P_SOLVER_BICG_SPE)",
-               "timesteps_per_s": args.steps / ts, "bicgstab_iters_per_step": per_step(sts, "iters")}
+        sims, ts, sts = run_steps(ws, warmup, args.spe_steps, "spe")
+        spe = {"operator": "matrix-free SPE (the reference's default P_SOLVER_BICG_SPE), bicgstabRe2 tol 1e-10, "
+                           "iters_max 5000", "steps": args.spe_steps,
+               "timesteps_per_s": args.spe_steps / ts, "bicgstab_iters_per_step": per_step(sts, "iters"),
+               "solves": passes_report(sts, ws.params.maxit)}
         sims.close()
 
-    poisson = None if args.no_poisson else poisson_benchmark(w64)
+    poisson = None if args.no_poisson else poisson_benchmark(w64, args.poisson_sor_full)
 
     adjoint = None
     if args.adjoint_steps > 1:
@@ -402,23 +526,19 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        try:
-            threads = len(os.sched_getaffinity(0))
-        except Exception:
-            threads = os.cpu_count() or 1
-        threads = max(1, min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)), 16))
+        threads = len(HOST_CPUS) if HOST_CPUS else (os.cpu_count() or 1)
+        threads = max(1, min(threads, int(os.environ.get("OMP_NUM_THREADS", threads))))
         log(f"[bench] CPU baseline on {threads} threads")
-        cpu = cpu_baseline(w64, iters64, threads)
+        cpu = cpu_baseline(w64, iters64, threads, args.cpu_iters, args.cpu_ref_iters)
         cpu["gpu_speedup"] = tps64 / cpu["value"]
+        if mg:
+            cpu["gpu_speedup_fp64_mg"] = mg["fp64_mg"]["timesteps_per_s"] / cpu["value"]
 
-    # what the timed FP64 solves reached: bicgstabRe2 stops at iters_max
-    # (5000) as the reference does, whatever the residual then is
-    fp64_solves = None
-    if st64:
-        passes = [(it, rel) for s in st64 for it, rel in zip(s["iters"], s["relres"]) if it >= 0]
-        fp64_solves = {"passes": len(passes), "max_final_relres": max(rel for _, rel in passes),
-                       "passes_at_iteration_cap": sum(1 for it, _ in passes if it >= w64.params.maxit),
-                       "iteration_cap": w64.params.maxit}
+    kd = kt["spmv_dot"]
+    pressure = (f"bicgstabRe2 (the reference's BiCGSTAB) on the Jacobi-scaled ELL operator, FP64, tol 1e-10, iters_max "
+                f"{w64.params.maxit} as the reference: {solves['passes_at_iteration_cap']} of {solves['passes']} "
+                f"timed passes stopped at iters_max, worst final relative residual {solves['max_final_relres']:.1e} "
+                f"(median {solves['median_final_relres']:.1e}); every pass at 1e-10: see same_residual")
     out = {
         "metric": "timesteps/s @512x256x64 (mixed vs FP64, same residual); Poisson SpMV HBM GB/s",
         "value": tps64,
@@ -433,16 +553,18 @@ def main():
         "dtype": "f64",
         "data": "synthetic (uniform inflow, seeded +-1e-3 noise on the initial velocity; seeded N(0,1) Poisson RHS)",
         "config": {"workload": w64.name, "grid": [512, 256, 64], "Re": 200, "dt": 5e-4,
-                   "pressure": "BiCGSTAB (bicgstabRe2), Jacobi-scaled ELL, tol 1e-10, maxit 5000",
-                   "passes_per_step": 4, "bicgstab_iters_per_step": iters64},
-        "roofline": {"bound": "hbm", "achieved": spmv["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": spmv["gbs"] / HBM_PEAK_GBS, "traffic": profiled_traffic(),
-                     "kernel": "ELL SpMV FP64 (100 B/row x 8388608 rows per launch)", "avg_us": spmv["avg_us"]},
-        "fp64_solves": fp64_solves,
+                   "boundaries": "uniform inflow u = 1, convective outflow (U = 1, p = 0), slip y-walls, periodic z",
+                   "pressure": pressure, "passes_per_step": 4, "bicgstab_iters_per_step": iters64},
+        "roofline": {"bound": "hbm", "achieved": kd["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kd["gbs"] / HBM_PEAK_GBS, "traffic": profiled_traffic(),
+                     "kernel": "spmv_dot_kernel<double>: K1 of the BiCGSTAB iteration, ap = A p (ELL) with the partial "
+                               "ap . r0; 108 B/row x 8388608 rows per launch",
+                     "avg_us": kd["avg_us"]},
+        "fp64_solves": solves,
+        "solver_counters": counters,
         "phases": phases64,
         "kernels": kt,
-        "mixed": mixed,
-        "multigrid": mg,
+        "same_residual": mg,
         "spe": spe,
         "poisson": poisson,
         "adjoint": adjoint,

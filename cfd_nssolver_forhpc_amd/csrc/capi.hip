@@ -552,6 +552,13 @@ int cfd_set_params(cfd_ctx *ctx, const cfd_params *params) {
   Ctx &c = *reinterpret_cast<Ctx *>(ctx);
   if ((rc = check_params_grid(params, c.g.nzc))) return rc;
   adjoint_invalidate(c);
+  // the multigrid hierarchy follows the pressure boundary conditions
+  bool mg_stale = params->periodic_z != c.prm.periodic_z;
+  for (int f = 0; f < 6; ++f) mg_stale = mg_stale || params->p_dirichlet[f] != c.prm.p_dirichlet[f];
+  if (mg_stale && c.mg) {
+    CFD_HIP(hipStreamSynchronize(c.stream));
+    mg_release(c);
+  }
   c.prm = *params;
   return CFD_OK;
 }

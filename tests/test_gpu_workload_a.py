@@ -10,9 +10,14 @@ to kappa * tol, and at 1e-10 that floor alone is ~1e-9 in velocity and up to
 ~1e-6 in pressure (measured over the full 100 steps: velocity <= 1.3e-9,
 pressure <= 9e-7, cD <= 3.3e-8, with no growth along the run).  At 1e-13
 the solves need more than the reference's 5000-iteration cap on some passes
-(up to ~7000 on the oracle), so both sides run with a cap of 100000 and every
+(up to ~8000 on the oracle), so both sides run with a cap of 100000 and every
 pass reaches the tolerance.  The pressure correction keeps its own floor,
-kappa * 1e-13: its error sits in the smooth modes the residual weighs least.
+kappa * 1e-13 (up to ~1e-9 relative L2, in the smooth modes the residual
+weighs least), so it is pinned where the mathematics fixes it: each
+correction solves its side's system to 1e-13, so ||A (X_gpu - X_oracle)|| is
+at most ||b_gpu - b_oracle|| + 2e-13 ||b|| (checked with 2.5e-13, A the
+assembled ELL operator, b the last pass's right-hand sides); its relative L2
+difference is held to 5e-9.
 """
 from __future__ import annotations
 
@@ -46,12 +51,19 @@ def test_cylinder_128x64x4_steps_match_oracle(require_gpu):
     osim.setup()
     for c in range(3):
         assert np.array_equal(sim.download(capi.T0_U + c), osim.field(capi.T0_U + c))
-    worst = {"u": 0.0, "v": 0.0, "w": 0.0, "p": 0.0}
+    worst = {"u": 0.0, "v": 0.0, "w": 0.0, "p": 0.0, "A dp": 0.0}
+    col, val = sim.ell()
     for k in range(STEPS):
         st = sim.step()
         ost = osim.step()
         errs = {name: rel_l2(sim.download(fid), osim.field(fid))
                 for name, fid in (("u", capi.T0_U), ("v", capi.T0_V), ("w", capi.T0_W), ("p", capi.X))}
+        if k > 0:
+            dp = sim.download(capi.X) - osim.field(capi.X)
+            adp = np.zeros(dp.size)
+            O.lib().orc_spmv_ell(dp.size, 7, col, val, dp, adp)
+            b_gpu, b_orc = sim.download(capi.RHS), osim.field(capi.RHS)
+            errs["A dp"] = float((np.linalg.norm(adp) - np.linalg.norm(b_gpu - b_orc)) / np.linalg.norm(b_gpu))
         for n, e in errs.items():
             worst[n] = max(worst[n], e)
         if k < 3 or (k + 1) % 10 == 0:
@@ -64,6 +76,6 @@ def test_cylinder_128x64x4_steps_match_oracle(require_gpu):
                 assert st.iters[q] < MAXIT and ost.iters[q] < MAXIT
             assert abs(st.cd[q] - ost.cd[q]) <= 1e-9 * max(1.0, abs(ost.cd[q]))
         for name, e in errs.items():
-            assert e < (1e-9 if name == "p" else 1e-10), (k + 1, name, e)
+            assert e < {"p": 5e-9, "A dp": 2.5e-13}.get(name, 1e-10), (k + 1, name, e)
     print("worst over 100 steps: " + " ".join(f"{n} {e:.2e}" for n, e in worst.items()), flush=True)
     sim.close()
