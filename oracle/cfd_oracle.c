@@ -1393,8 +1393,40 @@ double orc_sor_lex_bl(const cfd_grid_desc *g, int pz, double dt, double omega, d
 /* transpose of BC_staggered_main's velocity part (orc_bc_main_ex), faces in
    the reverse of the forward order; the adjoint of the face-0 u Dirichlet
    value is added to g_uin[j - 2] (when g_uin is not NULL) */
+static void bc_main_adjoint_impl(const cfd_grid_desc *g, const double Num[6][3], double *g_uin, int conv,
+                                 double conv_udt, int pz, double *lu, double *lv, double *lw);
+
 void orc_bc_main_adjoint(const cfd_grid_desc *g, const double Num[6][3], double *g_uin,
                          double *lu, double *lv, double *lw) {
+  bc_main_adjoint_impl(g, Num, g_uin, 0, 0.0, 0, lu, lv, lw);
+}
+
+/* the same with the boundary mode of P (orc_bc_main_bl): the periodic z
+   images fold their ghosts' adjoints back (the z faces were written last),
+   and the convective outflow's y_b = (y_old + s y_in) / (1 + s) sends
+   S / (1 + s) to the old value and s S / (1 + s) to y_in, S the outputs'
+   summed adjoint */
+void orc_bc_main_adjoint_bl(const cfd_grid_desc *g, const cfd_params *P, double *g_uin, double *lu, double *lv,
+                            double *lw) {
+  bc_main_adjoint_impl(g, P->bc_num, g_uin, P->outflow_convective, P->outflow_speed * P->dt, P->periodic_z, lu,
+                       lv, lw);
+}
+
+/* the adjoint of periodic_z_fill: each ghost plane's into its image's */
+static void periodic_z_fold(const dims_t d, double *L) {
+  const int nz = d.nz;
+  for (int i = 0; i < d.nx; ++i)
+    for (int j = 0; j < d.ny; ++j) {
+      const size_t b = IC(d, i, j, 0);
+      L[b + nz - 4] += L[b + 0]; L[b + 0] = 0.0;
+      L[b + nz - 3] += L[b + 1]; L[b + 1] = 0.0;
+      L[b + 2] += L[b + nz - 2]; L[b + nz - 2] = 0.0;
+      L[b + 3] += L[b + nz - 1]; L[b + nz - 1] = 0.0;
+    }
+}
+
+static void bc_main_adjoint_impl(const cfd_grid_desc *g, const double Num[6][3], double *g_uin, int conv,
+                                 double conv_udt, int pz, double *lu, double *lv, double *lw) {
   const dims_t d = dims_of(g);
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   double *L[3] = {lu, lv, lw};
@@ -1413,13 +1445,18 @@ void orc_bc_main_adjoint(const cfd_grid_desc *g, const double Num[6][3], double 
   } while (0)
 #define AN(b, c) (Num[b][c])
 #define AT(b, c) (2.0 * Num[b][c] - 1.0)
-  for (int i = GC; i < nx - GC; ++i)
+  if (pz) {
+    periodic_z_fold(d, lu);
+    periodic_z_fold(d, lv);
+    periodic_z_fold(d, lw);
+  }
+  for (int i = GC; i < nx - GC && !pz; ++i)
     for (int j = GC; j < ny - GC; ++j) { /* BC_5, reverse order of its writes */
       ADJ2(1, AT(5, 1), IC(d, i, j, nz - 1), IC(d, i, j, nz - 2), IC(d, i, j, nz - 3));
       ADJ2(0, AT(5, 0), IC(d, i, j, nz - 1), IC(d, i, j, nz - 2), IC(d, i, j, nz - 3));
       ADJ3(2, AN(5, 2), IC(d, i, j, nz - 1), IC(d, i, j, nz - 2), IC(d, i, j, nz - 3), IC(d, i, j, nz - 4));
     }
-  for (int i = GC; i < nx - GC; ++i)
+  for (int i = GC; i < nx - GC && !pz; ++i)
     for (int j = GC; j < ny - GC; ++j) { /* BC_4 */
       ADJ2(2, AN(4, 2), IC(d, i, j, 0), IC(d, i, j, 1), IC(d, i, j, 2));
       ADJ2(1, AT(4, 1), IC(d, i, j, 0), IC(d, i, j, 1), IC(d, i, j, 2));
@@ -1439,9 +1476,25 @@ void orc_bc_main_adjoint(const cfd_grid_desc *g, const double Num[6][3], double 
     }
   for (int j = GC; j < ny - GC; ++j)
     for (int k = GC; k < nz - GC; ++k) { /* BC_1 */
-      ADJ2(2, AT(1, 2), IC(d, nx - 1, j, k), IC(d, nx - 2, j, k), IC(d, nx - 3, j, k));
-      ADJ2(1, AT(1, 1), IC(d, nx - 1, j, k), IC(d, nx - 2, j, k), IC(d, nx - 3, j, k));
-      ADJ3(0, AN(1, 0), IC(d, nx - 1, j, k), IC(d, nx - 2, j, k), IC(d, nx - 3, j, k), IC(d, nx - 4, j, k));
+      const size_t n1 = IC(d, nx - 1, j, k), n2 = IC(d, nx - 2, j, k), n3 = IC(d, nx - 3, j, k),
+                   n4 = IC(d, nx - 4, j, k);
+      if (conv) {
+        const double su = conv_udt / g->Dx[nx - 3], st = conv_udt / g->Dxs[nx - 3];
+        for (int c = 2; c >= 1; --c) { /* w, v: (y[n2] + st y[n3]) / (1 + st) -> n2, n1 */
+          const double S = L[c][n1] + L[c][n2];
+          L[c][n1] = 0.0;
+          L[c][n2] = S / (1.0 + st);
+          L[c][n3] += st * S / (1.0 + st);
+        }
+        const double S = lu[n1] + lu[n2] + lu[n3]; /* u: (u[n3] + su u[n4]) / (1 + su) -> n3, n2, n1 */
+        lu[n1] = 0.0; lu[n2] = 0.0;
+        lu[n3] = S / (1.0 + su);
+        lu[n4] += su * S / (1.0 + su);
+      } else {
+        ADJ2(2, AT(1, 2), n1, n2, n3);
+        ADJ2(1, AT(1, 1), n1, n2, n3);
+        ADJ3(0, AN(1, 0), n1, n2, n3, n4);
+      }
     }
   for (int j = GC; j < ny - GC; ++j)
     for (int k = GC; k < nz - GC; ++k) { /* BC_0 */
@@ -1464,9 +1517,29 @@ void orc_bc_main_adjoint(const cfd_grid_desc *g, const double Num[6][3], double 
    when the pass's velocity is overwritten later) and of f (lf, 3*N) to the
    adjoints of u* (added into lu1.. at the update positions), of the
    interior pressure (added into lX, N) and of the gain (added to *g_gain) */
+static void dfib_adjoint_impl(const cfd_grid_desc *g, double dt, double gain, int conv, int pz, const double *eta,
+                              const double *t2, const double *lu3, const double *lv3, const double *lw3,
+                              const double *lf, double *lu1, double *lv1, double *lw1, double *lX, double *g_gain);
+
 void orc_dfib_adjoint(const cfd_grid_desc *g, double dt, double gain, const double *eta, const double *t2,
                       const double *lu3, const double *lv3, const double *lw3, const double *lf,
                       double *lu1, double *lv1, double *lw1, double *lX, double *g_gain) {
+  dfib_adjoint_impl(g, dt, gain, 0, 0, eta, t2, lu3, lv3, lw3, lf, lu1, lv1, lw1, lX, g_gain);
+}
+
+/* the same with the boundary mode of P (orc_dfib_update_bl): the outflow
+   face of u (no forcing; its + pressure neighbour is the Dirichlet ghost,
+   no unknown) and the seam face of w (its + neighbour the first plane) */
+void orc_dfib_adjoint_bl(const cfd_grid_desc *g, const cfd_params *P, double gain, const double *eta,
+                         const double *t2, const double *lu3, const double *lv3, const double *lw3,
+                         const double *lf, double *lu1, double *lv1, double *lw1, double *lX, double *g_gain) {
+  dfib_adjoint_impl(g, P->dt, gain, P->outflow_convective, P->periodic_z, eta, t2, lu3, lv3, lw3, lf, lu1, lv1,
+                    lw1, lX, g_gain);
+}
+
+static void dfib_adjoint_impl(const cfd_grid_desc *g, double dt, double gain, int conv, int pz, const double *eta,
+                              const double *t2, const double *lu3, const double *lv3, const double *lw3,
+                              const double *lf, double *lu1, double *lv1, double *lw1, double *lX, double *g_gain) {
   const dims_t d = dims_of(g);
   const size_t N = d.N;
   const double *L3[3] = {lu3, lv3, lw3};
@@ -1476,14 +1549,15 @@ void orc_dfib_adjoint(const cfd_grid_desc *g, double dt, double gain, const doub
   const size_t si[3] = {(size_t)d.nyc * d.nzc, (size_t)d.nzc, 1};
   double gg = 0.0;
   for (int q = 0; q < 3; ++q) {
-    const int iend = d.nx - GC - (q == 0), jend = d.ny - GC - (q == 1), kend = d.nz - GC - (q == 2);
+    const int iend = d.nx - GC - (q == 0 && !conv), jend = d.ny - GC - (q == 1), kend = d.nz - GC - (q == 2 && !pz);
     for (int i = GC; i < iend; ++i)
       for (int j = GC; j < jend; ++j)
         for (int k = GC; k < kend; ++k) {
           const size_t ic = IC(d, i, j, k), t = ICAL(d, i, j, k);
           const int I = q == 0 ? i : (q == 1 ? j : k);
           const double ds = Ds[q][I];
-          const double e = eta[ic] + (eta[ic + sg[q]] - eta[ic]) * ((ds * 0.5) / Dd[q][I]);
+          const int outflow = q == 0 && conv && i == d.nx - 3, seam = q == 2 && pz && k == d.nz - 3;
+          const double e = outflow ? 0.0 : eta[ic] + (eta[ic + sg[q]] - eta[ic]) * ((ds * 0.5) / Dd[q][I]);
           const double T2 = t2[q * N + t];
           /* u3 = (1 - gain e) T2; f = (u3 - T2)/dt */
           const double lu3t = (L3[q] ? L3[q][ic] : 0.0) + lf[q * N + t] / dt;
@@ -1492,7 +1566,8 @@ void orc_dfib_adjoint(const cfd_grid_desc *g, double dt, double gain, const doub
           L1[q][ic] += lT2;
           /* T2 = u* - dt (p[+] - p) / ds */
           lX[t] += dt / ds * lT2;
-          lX[t + si[q]] -= dt / ds * lT2;
+          if (seam) lX[t - (size_t)(d.nzc - 1)] -= dt / ds * lT2;
+          else if (!outflow) lX[t + si[q]] -= dt / ds * lT2;
         }
   }
   *g_gain += gg;
@@ -1553,10 +1628,20 @@ static double dot5(const double c[5], const double *phi, const size_t m[5]) {
 /* transpose of the Jacobian of orc_predictor at (u, v, w): ly1.. hold the
    adjoints of the predicted velocity at the positions the predictor writes;
    the adjoints of its inputs are added into lu, lv, lw (ghosted) */
+static void predictor_adjoint_impl(const cfd_grid_desc *g, double nu, double dt, int pz, const double *u,
+                                   const double *v, const double *w, const double *lu1, const double *lv1,
+                                   const double *lw1, double *lu, double *lv, double *lw);
+
 void orc_predictor_adjoint(const cfd_grid_desc *g, double nu, double dt,
                            const double *u, const double *v, const double *w,
                            const double *lu1, const double *lv1, const double *lw1,
                            double *lu, double *lv, double *lw) {
+  predictor_adjoint_impl(g, nu, dt, 0, u, v, w, lu1, lv1, lw1, lu, lv, lw);
+}
+
+static void predictor_adjoint_impl(const cfd_grid_desc *g, double nu, double dt, int pz, const double *u,
+                                   const double *v, const double *w, const double *lu1, const double *lv1,
+                                   const double *lw1, double *lu, double *lv, double *lw) {
   const dims_t d = dims_of(g);
   const size_t sa[3] = {d.nzny, (size_t)d.nz, 1};
   const double *Di[3] = {g->Dx, g->Dy, g->Dz}, *Ds[3] = {g->Dxs, g->Dys, g->Dzs};
@@ -1565,7 +1650,7 @@ void orc_predictor_adjoint(const cfd_grid_desc *g, double nu, double dt,
   double *LX[3] = {lu, lv, lw};
   const double nudt = nu * dt;
   for (int c = 0; c < 3; ++c) {
-    const int iend = d.nx - GC - (c == 0), jend = d.ny - GC - (c == 1), kend = d.nz - GC - (c == 2);
+    const int iend = d.nx - GC - (c == 0), jend = d.ny - GC - (c == 1), kend = d.nz - GC - (c == 2 && !pz);
     for (int i = GC; i < iend; ++i)
       for (int j = GC; j < jend; ++j)
         for (int k = GC; k < kend; ++k) {
@@ -1997,7 +2082,7 @@ double orc_sim_objective(orc_sim *s, int nsteps, int avg_from) {
 
 /* transpose of BC_staggered_copy (orc_bc_copy): ly at the copied positions
    added into lx */
-static void bc_copy_adjoint(const dims_t d, double *const ly[3], double *const lx[3]) {
+static void bc_copy_adjoint(const dims_t d, int pz, double *const ly[3], double *const lx[3]) {
   for (int j = GC; j < d.ny - GC; ++j)
     for (int k = GC; k < d.nz - GC; ++k) {
       const int gi[4] = {0, 1, d.nx - 2, d.nx - 1};
@@ -2017,7 +2102,7 @@ static void bc_copy_adjoint(const dims_t d, double *const ly[3], double *const l
       const int gk[4] = {0, 1, d.nz - 2, d.nz - 1};
       for (int q = 0; q < 4; ++q)
         for (int c = 0; c < 3; ++c) lx[c][IC(d, i, j, gk[q])] += ly[c][IC(d, i, j, gk[q])];
-      lx[2][IC(d, i, j, d.nz - 3)] += ly[2][IC(d, i, j, d.nz - 3)];
+      if (!pz) lx[2][IC(d, i, j, d.nz - 3)] += ly[2][IC(d, i, j, d.nz - 3)];
     }
 }
 
@@ -2026,9 +2111,9 @@ int orc_sim_gradient(orc_sim *s, double *g_uin, double *g_gain, int *adj_iters) 
   const cfd_grid_desc *g = &s->g;
   const dims_t d = s->d;
   const size_t G = d.G, N = d.N;
-  if (!s->adj_steps || P->source_index_quirk || P->op != CFD_OP_ELL || P->convection != CFD_CONV_QUICK || P->les ||
-      P->periodic_z || P->outflow_convective) /* the boundary-table conditions only */
+  if (!s->adj_steps || P->source_index_quirk || P->op != CFD_OP_ELL || P->convection != CFD_CONV_QUICK || P->les)
     return -1;
+  const int pz = P->periodic_z, conv = P->outflow_convective;
   const int passes = s->adj_passes, L = passes - 1;
   const double dt = P->dt, nu = 1.0 / P->Re;
   const double wJ = 1.0 / (double)(s->adj_steps - s->adj_avg_from);
@@ -2052,7 +2137,7 @@ int orc_sim_gradient(orc_sim *s, double *g_uin, double *g_gain, int *adj_iters) 
     const double *x = s->adj_x + (size_t)st * 3 * G;
     const double *t2 = s->adj_t2 + (size_t)st * passes * 3 * N;
     /* T0' = BC(T3): lx <- adjoint of T3 */
-    orc_bc_main_adjoint(g, P->bc_num, g_uin, lx[0], lx[1], lx[2]);
+    orc_bc_main_adjoint_bl(g, P, g_uin, lx[0], lx[1], lx[2]);
     memset(lsum[0], 0, sizeof(double) * 3 * G);
     memset(Lam, 0, sizeof(double) * 3 * N);
     for (int q = L; q >= 0; --q) {
@@ -2064,14 +2149,17 @@ int orc_sim_gradient(orc_sim *s, double *g_uin, double *g_gain, int *adj_iters) 
           for (int j = GC; j < d.ny - GC; ++j)
             for (int k = GC; k < d.nz - GC; ++k)
               lf[ICAL(d, i, j, k)] += wJ * (-2.0 / g->lz) * (g->Dx[i] * g->Dy[j] * g->Dz[k]);
-      orc_dfib_adjoint(g, dt, s->gain, s->eta, t2 + (size_t)q * 3 * N, q == L ? lx[0] : NULL,
-                       q == L ? lx[1] : NULL, q == L ? lx[2] : NULL, lf, lT1[0], lT1[1], lT1[2], lX, &gg);
+      orc_dfib_adjoint_bl(g, P, s->gain, s->eta, t2 + (size_t)q * 3 * N, q == L ? lx[0] : NULL,
+                          q == L ? lx[1] : NULL, q == L ? lx[2] : NULL, lf, lT1[0], lT1[1], lT1[2], lX, &gg);
       if (s->adj_solved[st]) {
         int it = 0;
         orc_bicg_solve(bt, lX, zl, &it); /* A^T zl = lX, warm start from the last adjoint solve */
         total_it += it;
         orc_rhs_adjoint(g, dt, s->jp, zl, lT1[0], lT1[1], lT1[2]);
       }
+      /* periodic z: the pass refreshed T1's z ghosts from their images */
+      if (pz)
+        for (int c = 0; c < 3; ++c) periodic_z_fold(d, lT1[c]);
       /* T1 = P + S_q at every interior cell (source added at its own cell) */
       for (int i = GC; i < d.nx - GC; ++i)
         for (int j = GC; j < d.ny - GC; ++j)
@@ -2082,13 +2170,30 @@ int orc_sim_gradient(orc_sim *s, double *g_uin, double *g_gain, int *adj_iters) 
       for (int c = 0; c < 3; ++c)
         for (size_t m = 0; m < G; ++m) lsum[c][m] += lT1[c][m];
     }
-    /* T1 = predictor(T0) at the predictor's positions, T0 copied elsewhere */
+    /* T1 = predictor(T0) at the predictor's positions, T0 copied elsewhere;
+       the ghosts of v and w on the convective outflow keep their value
+       through the step (the T0 <- T3 copy leaves them), so the adjoint their
+       end-of-step update left there carries over */
+    const size_t plane = (size_t)d.ny * d.nz;
+    double *keep = NULL;
+    if (conv) {
+      keep = (double *)malloc(sizeof(double) * 4 * plane);
+      for (int c = 1; c < 3; ++c)
+        for (int q = 0; q < 2; ++q)
+          memcpy(keep + ((c - 1) * 2 + q) * plane, lx[c] + (size_t)(d.nx - 2 + q) * plane, sizeof(double) * plane);
+    }
     memset(lx[0], 0, sizeof(double) * 3 * G);
-    orc_predictor_adjoint(g, nu, dt, x, x + G, x + 2 * G, lsum[0], lsum[1], lsum[2], lx[0], lx[1], lx[2]);
-    bc_copy_adjoint(d, lsum, lx);
+    if (keep) {
+      for (int c = 1; c < 3; ++c)
+        for (int q = 0; q < 2; ++q)
+          memcpy(lx[c] + (size_t)(d.nx - 2 + q) * plane, keep + ((c - 1) * 2 + q) * plane, sizeof(double) * plane);
+      free(keep);
+    }
+    predictor_adjoint_impl(g, nu, dt, pz, x, x + G, x + 2 * G, lsum[0], lsum[1], lsum[2], lx[0], lx[1], lx[2]);
+    bc_copy_adjoint(d, pz, lsum, lx);
   }
   /* the initial state's ghosts: BC_staggered_main of the initial field */
-  orc_bc_main_adjoint(g, P->bc_num, g_uin, lx[0], lx[1], lx[2]);
+  orc_bc_main_adjoint_bl(g, P, g_uin, lx[0], lx[1], lx[2]);
   *g_gain = gg;
   if (adj_iters) *adj_iters = total_it;
   orc_bicg_destroy(bt);

@@ -168,9 +168,14 @@ void orc_sim_set_gain(orc_sim *s, double gain);
 /* ---- discrete adjoint (transposes of the functions above) --------------- */
 void orc_bc_main_adjoint(const cfd_grid_desc *g, const double num[6][3], double *g_uin,
                          double *lu, double *lv, double *lw);
+void orc_bc_main_adjoint_bl(const cfd_grid_desc *g, const cfd_params *P, double *g_uin, double *lu, double *lv,
+                            double *lw);
 void orc_dfib_adjoint(const cfd_grid_desc *g, double dt, double gain, const double *eta, const double *t2,
                       const double *lu3, const double *lv3, const double *lw3, const double *lf,
                       double *lu1, double *lv1, double *lw1, double *lX, double *g_gain);
+void orc_dfib_adjoint_bl(const cfd_grid_desc *g, const cfd_params *P, double gain, const double *eta,
+                         const double *t2, const double *lu3, const double *lv3, const double *lw3,
+                         const double *lf, double *lu1, double *lv1, double *lw1, double *lX, double *g_gain);
 void orc_rhs_adjoint(const cfd_grid_desc *g, double dt, const double *jp, const double *lb,
                      double *lu1, double *lv1, double *lw1);
 void orc_predictor_adjoint(const cfd_grid_desc *g, double nu, double dt,

@@ -141,6 +141,9 @@ def _declare(L):
     L.orc_bc_main_adjoint.argtypes = [G, T, C.c_void_p, dp, dp, dp]
     L.orc_dfib_adjoint.argtypes = [G, C.c_double, C.c_double, dp, dp, C.c_void_p, C.c_void_p, C.c_void_p, dp,
                                    dp, dp, dp, dp, C.POINTER(C.c_double)]
+    L.orc_bc_main_adjoint_bl.argtypes = [G, C.POINTER(CfdParams), C.c_void_p, dp, dp, dp]
+    L.orc_dfib_adjoint_bl.argtypes = [G, C.POINTER(CfdParams), C.c_double, dp, dp, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, dp, dp, dp, dp, dp, C.POINTER(C.c_double)]
     L.orc_rhs_adjoint.argtypes = [G, C.c_double, dp, dp, dp, dp, dp]
     L.orc_predictor_adjoint.argtypes = [G, C.c_double, C.c_double, dp, dp, dp, dp, dp, dp, dp, dp, dp]
     L.orc_ell_transpose.argtypes = [C.c_size_t, ip, dp, ip, dp]

@@ -3,7 +3,8 @@
 Pins: the objective and the gradient against the oracle's adjoint (which
 solves the transposed pressure system on the explicit transpose of the
 assembled ELL operator and records every step instead of recomputing from
-checkpoints), the gradient against central finite differences of the GPU
+checkpoints), with the boundary table and with the workloads' convective
+outflow and periodic z, the gradient against central finite differences of the GPU
 forward run, independence from the checkpoint interval, the state left
 after the sweep, the torch.autograd.Function, and bit-identity of the
 forward step at the parameters' reference values.
@@ -106,19 +107,21 @@ def baseline_case(tol=1e-13):
     return w, uin
 
 
-def test_baseline_bc_objective_matches_the_oracle(require_gpu):
+def test_baseline_bc_objective_and_gradient_match_the_oracle(require_gpu):
+    """With the workloads' boundaries the oracle's gradient (explicit
+    transposes, the transposed solve on the explicit transpose of the
+    wrapped ELL operator, every step recorded) pins the GPU's (the symmetric
+    reformulation of the transposed solve, checkpoints and recomputation)."""
     w, uin = baseline_case()
     pr = gpu_problem(w, every=3)
     J = pr.run(uin, 0.9)
-    g = O.Grid(w.grid)
-    s = O.Sim(g, w.params)
-    s.set_inflow(uin)
-    s.set_gain(0.9)
-    s.init_state(1e-3, 1)
-    s.setup()
-    Jo = s.objective(STEPS, AVG_FROM)
-    print("J", J, "oracle", Jo)
+    g_uin, g_gain, st = pr.sim.adjoint_backward()
+    Jo, go_uin, go_gain = oracle_gradient(w, uin, 0.9)
+    print("J", J, Jo, "gain grad", g_gain, go_gain, "|g_uin - oracle| / |oracle|",
+          np.linalg.norm(g_uin - go_uin) / np.linalg.norm(go_uin))
     assert rel(J, Jo) < 1e-10
+    assert np.linalg.norm(g_uin - go_uin) <= 1e-8 * np.linalg.norm(go_uin)
+    assert rel(g_gain, go_gain) < 1e-8
 
 
 def test_baseline_bc_gradient_matches_central_differences(require_gpu):

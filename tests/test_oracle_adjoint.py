@@ -72,6 +72,84 @@ def test_bc_adjoint_dot_product(seed):
     assert rel(sum(np.dot(a, b) for a, b in zip(ly, y)), np.dot(g_uin, du)) < 1e-13
 
 
+def _bl_params(g, seed):
+    """The workloads' boundary mode (convective outflow, periodic z) with
+    random Neumann factors and no Dirichlet constants, so that BC is linear."""
+    p = configs.baseline_bc(configs.cylinder_params(100.0, 0.002, 1.5, 1.0, 0.45, 1e-10))
+    p.outflow_speed = 0.7
+    rng = np.random.default_rng(seed)
+    for f in range(6):
+        for c in range(3):
+            p.bc_dir[f][c] = 0.0
+            p.bc_num[f][c] = float(rng.choice([0.0, 1.0, 0.5, -0.25]))
+    return p
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_bc_adjoint_dot_product_convective_periodic(seed):
+    """The transposed boundary update of the workloads' boundaries: the
+    convective outflow (linear in the interior face and the old ghost) and
+    the periodic z images, after the table's x and y faces."""
+    g = sgrid()
+    p = _bl_params(g, seed)
+    rng = np.random.default_rng(10 + seed)
+    dx = [rng.standard_normal(g.G) for _ in range(3)]
+    Mdx = [a.copy() for a in dx]
+    O.lib().orc_bc_main_bl(g.ref, C.byref(p), None, *Mdx, np.zeros(g.G))
+    ly = [rng.standard_normal(g.G) for _ in range(3)]
+    lx = [a.copy() for a in ly]
+    g_uin = np.zeros(g.nyc)
+    O.lib().orc_bc_main_adjoint_bl(g.ref, C.byref(p), g_uin.ctypes.data_as(C.c_void_p), *lx)
+    lhs = sum(np.dot(a, b) for a, b in zip(ly, Mdx))
+    rhs = sum(np.dot(a, b) for a, b in zip(lx, dx))
+    assert rel(lhs, rhs) < 1e-13
+    du = rng.standard_normal(g.nyc)
+    y = [np.zeros(g.G) for _ in range(3)]
+    O.lib().orc_bc_main_bl(g.ref, C.byref(p), du.ctypes.data_as(C.c_void_p), *y, np.zeros(g.G))
+    assert rel(sum(np.dot(a, b) for a, b in zip(ly, y)), np.dot(g_uin, du)) < 1e-13
+
+
+def test_dfib_adjoint_dot_product_convective_periodic():
+    """The transposed update with the outflow face (no forcing, p = 0
+    outside) and the seam face of w (its + neighbour the first plane, through
+    the periodic pressure ghosts)."""
+    g = sgrid()
+    p = _bl_params(g, 2)
+    rng = np.random.default_rng(13)
+    eta = eta_of(g, 1.5, 1.0, 0.45)
+    gain = 0.8
+    dT1 = [rng.standard_normal(g.G) for _ in range(3)]
+    dX = rng.standard_normal(g.N)
+    dp_ = np.zeros(g.G)
+    O.lib().orc_pressure_scatter(g.ref, dX, dp_)
+    O.lib().orc_periodic_refresh_p(g.ref, dp_)
+
+    def fwd(T1, pr, gn):
+        u3 = [np.zeros(g.G) for _ in range(3)]
+        f = np.zeros(3 * g.N)
+        t2 = np.zeros(3 * g.N)
+        O.lib().orc_dfib_update_bl(g.ref, C.byref(p), gn, eta, pr, *T1, *u3, f, t2.ctypes.data_as(C.c_void_p))
+        return u3, f, t2
+
+    u3, f, t2 = fwd(dT1, dp_, gain)
+    lu3 = [rng.standard_normal(g.G) for _ in range(3)]
+    lf = rng.standard_normal(3 * g.N)
+    lT1 = [np.zeros(g.G) for _ in range(3)]
+    lX = np.zeros(g.N)
+    gg = C.c_double(0.0)
+    O.lib().orc_dfib_adjoint_bl(g.ref, C.byref(p), gain, eta, t2, *[a.ctypes.data_as(C.c_void_p) for a in lu3], lf,
+                                *lT1, lX, C.byref(gg))
+    # (u3 stays 0 where the update does not write, so lu3 meets it only at
+    # the updated faces, the outflow and seam faces among them)
+    lhs = sum(np.dot(a, b) for a, b in zip(lu3, u3)) + np.dot(lf, f)
+    rhs = sum(np.dot(a, b) for a, b in zip(lT1, dT1)) + np.dot(lX, dX)
+    assert rel(lhs, rhs) < 1e-12
+    u3a, fa, _ = fwd(dT1, dp_, 1.0)
+    u3b, fb, _ = fwd(dT1, dp_, 0.0)
+    dg = sum(np.dot(a, b - c_) for a, b, c_ in zip(lu3, u3a, u3b)) + np.dot(lf, fa - fb)
+    assert rel(dg, gg.value) < 1e-12
+
+
 def test_dfib_adjoint_dot_product():
     g = sgrid()
     rng = np.random.default_rng(3)
@@ -184,6 +262,27 @@ def test_gradient_matches_central_differences():
         d /= np.linalg.norm(d)
         Jp = _objective(w, g, base + h * d[:-1], 1.0 + h * d[-1], steps, avg_from)
         Jm = _objective(w, g, base - h * d[:-1], 1.0 - h * d[-1], steps, avg_from)
+        fd = (Jp - Jm) / (2 * h)
+        ad = np.dot(g_uin, d[:-1]) + g_gain * d[-1]
+        assert rel(fd, ad) < 1e-5, (fd, ad)
+
+
+def test_gradient_matches_central_differences_convective_periodic():
+    """The workloads' boundaries (convective outflow, periodic z) through the
+    whole objective: the oracle's gradient against its own forward run."""
+    w, g, base = _case()
+    configs.baseline_bc(w.params)
+    steps, avg_from = 10, 5
+    J, (g_uin, g_gain, it) = _objective(w, g, base, 0.9, steps, avg_from, grad=True)
+    assert np.isfinite(J) and it > 0
+    rng = np.random.default_rng(12)
+    h = 1e-6
+    dirs = [np.concatenate([1.0 + 0.5 * rng.standard_normal(g.nyc), [0.0]])]
+    dirs += [rng.standard_normal(g.nyc + 1) for _ in range(2)]
+    for d in dirs:
+        d /= np.linalg.norm(d)
+        Jp = _objective(w, g, base + h * d[:-1], 0.9 + h * d[-1], steps, avg_from)
+        Jm = _objective(w, g, base - h * d[:-1], 0.9 - h * d[-1], steps, avg_from)
         fd = (Jp - Jm) / (2 * h)
         ad = np.dot(g_uin, d[:-1]) + g_gain * d[-1]
         assert rel(fd, ad) < 1e-5, (fd, ad)
