@@ -10,6 +10,7 @@
 // they are written for one pass over memory per phase, with the phases of a
 // pass fused where the data dependence allows.
 #include "dev_common.h"
+#include "pred_tiled.h"
 
 namespace cfd {
 
@@ -28,39 +29,6 @@ __device__ inline Cell decode_interior(const DevGrid &g, int64_t t) {
   c.j = (int)(jk / nzc) + GC;
   c.k = (int)(jk % nzc) + GC;
   return c;
-}
-
-// QuickUnitA (inc/source/convection_and_difussion.hpp:20-49)
-__device__ inline void quickA(double Up, double Un, int64_t pp, int64_t p, int64_t ic, int64_t n, int64_t nn,
-                              const double *__restrict__ Di, const double *__restrict__ Ds, int I,
-                              const double *__restrict__ phi, double &phiP, double &phiN) {
-  const double fc = phi[ic], fp = phi[p], fn = phi[n];
-  if (Up > 0.0)
-    phiP = 0.5 * (fc + fp) - 0.125 * Di[I + 1] * Di[I + 1] / Ds[I] * ((fp - fc) / Di[I + 1] - (fc - fn) / Di[I]);
-  else
-    phiP = 0.5 * (fc + fp) -
-           0.125 * Di[I + 1] * Di[I + 1] / Ds[I + 1] * ((phi[pp] - fp) / Di[I + 2] - (fp - fc) / Di[I + 1]);
-  if (Un > 0.0)
-    phiN = 0.5 * (fn + fc) - 0.125 * Di[I] * Di[I] / Ds[I - 1] * ((fc - fn) / Di[I] - (fn - phi[nn]) / Di[I - 1]);
-  else
-    phiN = 0.5 * (fn + fc) - 0.125 * Di[I] * Di[I] / Ds[I] * ((fp - fc) / Di[I + 1] - (fc - fn) / Di[I]);
-}
-
-// QuickUnitB (inc/source/convection_and_difussion.hpp:53-84)
-__device__ inline void quickB(double Up, double Un, int64_t pp, int64_t p, int64_t ic, int64_t n, int64_t nn,
-                              const double *__restrict__ Di, const double *__restrict__ Ds, int I,
-                              const double *__restrict__ phi, double &phiP, double &phiN) {
-  const double fc = phi[ic], fp = phi[p], fn = phi[n];
-  if (Up > 0.0)
-    phiP = 0.5 * (fc + fp) - 0.125 * Ds[I] * Ds[I] / Di[I] * ((fp - fc) / Ds[I] - (fc - fn) / Ds[I - 1]);
-  else
-    phiP = 0.5 * (fc + fp) -
-           0.125 * Ds[I] * Ds[I] / Di[I + 1] * ((phi[pp] - fp) / Ds[I + 1] - (fp - fc) / Ds[I]);
-  if (Un > 0.0)
-    phiN = 0.5 * (fn + fc) -
-           0.125 * Ds[I - 1] * Ds[I - 1] / Di[I - 1] * ((fc - fn) / Ds[I - 1] - (fn - phi[nn]) / Ds[I - 2]);
-  else
-    phiN = 0.5 * (fn + fc) - 0.125 * Ds[I - 1] * Ds[I - 1] / Di[I] * ((fp - fc) / Ds[I] - (fc - fn) / Ds[I - 1]);
 }
 
 // LudUnitA / LudUnitB (inc/source/convection_and_difussion.hpp:90-144; the
@@ -83,16 +51,15 @@ __device__ inline void ud(double Up, double Un, int64_t p, int64_t ic, int64_t n
   phiN = Un > 0.0 ? phi[n] : phi[ic];
 }
 
-// the face-value unit of scheme S; OWN: the component's own direction
-// (QuickUnitA), else across it (QuickUnitB); LUD and UD use one unit for both
+// the face-value unit of scheme S (LUD or UD: one unit for the component's
+// own direction and across it; QUICK runs in quick_predictor_kernel)
 template <int S, bool OWN>
 __device__ inline void face_unit(double Up, double Un, int64_t pp, int64_t p, int64_t ic, int64_t n, int64_t nn,
                                  const double *__restrict__ Di, const double *__restrict__ Ds, int I,
                                  const double *__restrict__ phi, double &phiP, double &phiN) {
+  static_assert(S == CFD_CONV_LUD || S == CFD_CONV_UD, "QUICK has its own kernel");
   if constexpr (S == CFD_CONV_LUD) lud(Up, Un, pp, p, ic, n, nn, Di, Ds, I, phi, phiP, phiN);
-  else if constexpr (S == CFD_CONV_UD) ud(Up, Un, p, ic, n, phi, phiP, phiN);
-  else if constexpr (OWN) quickA(Up, Un, pp, p, ic, n, nn, Di, Ds, I, phi, phiP, phiN);
-  else quickB(Up, Un, pp, p, ic, n, nn, Di, Ds, I, phi, phiP, phiN);
+  else ud(Up, Un, p, ic, n, phi, phiP, phiN);
 }
 
 // SmagorinskyModel (inc/source/smagorinskyModel.hpp:30-79) with
@@ -121,9 +88,11 @@ __global__ void __launch_bounds__(256) smagorinsky_kernel(DevGrid g, const doubl
   vis[t] = (cd * cd) * sqrt(s);
 }
 
+// The LUD / UD predictor, one cell a thread.
 // src_mode: 0 none, 1 add src[icelCal] at the cell, 2 reference index
-// mapping (src[m] added at ghosted m = icel when m < N); S: CFD_CONV_*;
-// vis: the eddy viscosity added to nu (convection_and_difussion.hpp:246-250), or null
+// mapping (src[m] added at ghosted m = icel when m < N); S: CFD_CONV_LUD or
+// CFD_CONV_UD; vis: the eddy viscosity added to nu
+// (convection_and_difussion.hpp:246-250), or null
 template <int S>
 __global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, double dt, const double *__restrict__ u,
                                                         const double *__restrict__ v, const double *__restrict__ w,
@@ -216,6 +185,41 @@ __global__ void __launch_bounds__(256) predictor_kernel(DevGrid g, double nu, do
     if (src_mode == 1) r += src[2 * N + t];
     else if (src_mode == 2 && ic < N) r += src[2 * N + ic];
     w1[ic] = r;
+  }
+}
+
+// The QUICK predictor, LDS-tiled (pred_tiled.h): a workgroup marches a
+// tj x TK patch of (j, k) columns over PT_TI i-planes.  Two barriers a
+// plane: the plane's face gradients (with the next plane's slabs fetched
+// before and stored after them), then the plane's cells (with the column's
+// next values fetched before them).  Same arguments and outputs as
+// predictor_kernel.
+template <int TK>
+__global__ void __launch_bounds__(PT_THREADS) quick_predictor_kernel(DevGrid g, PtShape s, PtArgs a) {
+  extern __shared__ __align__(16) double pt_mem[];
+  const PtLds L = pt_lds<TK>(pt_mem);
+  const PtBlock B = pt_block(g, s, (int)blockIdx.x);
+  const int tid = (int)threadIdx.x;
+  const bool act = pt_active<TK>(g, B, tid);
+  const int j = B.j0 + tid / TK, k = B.k0 + tid % TK;
+  double r[3][PtGeom<TK>::NPT];
+  pt_fetch<TK>(g, B, a.u, a.v, a.w, B.ib, tid, r);
+  PtCol c;
+  if (act) pt_col_init(g, a.u, a.v, a.w, B.ib, j, k, c);
+  pt_tables<TK>(g, B, L, tid);
+  pt_store<TK>(L, B.ib, tid, r);
+  __syncthreads();
+  for (int i = B.ib; i < B.ie; ++i) {
+    const bool more = i + 1 < B.ie;
+    if (more) pt_fetch<TK>(g, B, a.u, a.v, a.w, i + 1, tid, r);
+    pt_grads<TK>(g, B, L, i, tid);
+    if (more) pt_store<TK>(L, i + 1, tid, r);
+    __syncthreads();
+    double nxt[3];
+    if (act && more) pt_col_fetch(g, a.u, a.v, a.w, i, j, k, nxt);
+    if (act) pt_cell<TK>(g, B, L, a, c, i, tid);
+    if (act && more) pt_col_shift(B, L, i, nxt, c);
+    __syncthreads();
   }
 }
 
@@ -831,9 +835,20 @@ hipError_t launch_predictor(Ctx &c, const double *u0, const double *v0, const do
   else if (c.prm.convection == CFD_CONV_UD)
     predictor_kernel<CFD_CONV_UD><<<nb, 256, 0, c.stream>>>(c.g, nu, c.prm.dt, u0, v0, w0, u1, v1, w1, src,
                                                             src_mode, k0, k1, k2, vis, kw_end);
-  else
-    predictor_kernel<CFD_CONV_QUICK><<<nb, 256, 0, c.stream>>>(c.g, nu, c.prm.dt, u0, v0, w0, u1, v1, w1, src,
-                                                               src_mode, k0, k1, k2, vis, kw_end);
+  else {
+    const PtShape s = pt_shape(c.g);
+    const PtArgs a{nu, c.prm.dt, u0, v0, w0, u1, v1, w1, src, src_mode, k0, k1, k2, vis, kw_end};
+    const int64_t nblk = (int64_t)s.njt * s.nkt * s.nit;
+    if (nblk > INT32_MAX) return hipErrorInvalidValue;
+    const size_t lds = pt_lds_doubles(s) * sizeof(double);
+    switch (s.tk) {
+      case 4: quick_predictor_kernel<4><<<(unsigned)nblk, PT_THREADS, lds, c.stream>>>(c.g, s, a); break;
+      case 8: quick_predictor_kernel<8><<<(unsigned)nblk, PT_THREADS, lds, c.stream>>>(c.g, s, a); break;
+      case 16: quick_predictor_kernel<16><<<(unsigned)nblk, PT_THREADS, lds, c.stream>>>(c.g, s, a); break;
+      case 32: quick_predictor_kernel<32><<<(unsigned)nblk, PT_THREADS, lds, c.stream>>>(c.g, s, a); break;
+      default: quick_predictor_kernel<64><<<(unsigned)nblk, PT_THREADS, lds, c.stream>>>(c.g, s, a); break;
+    }
+  }
   return hipGetLastError();
 }
 

@@ -17,7 +17,7 @@ import pytest
 import oracle_lib as O
 from helpers import bc_tables, pd6, random_state, rel_l2
 from cfd_nssolver_forhpc_amd import capi, configs
-from cfd_nssolver_forhpc_amd.solver import Simulation, native_grid, uniform_grid, axis_sml
+from cfd_nssolver_forhpc_amd.solver import Simulation, native_grid, uniform_grid, axis_sml, axis_tanh
 from cfd_nssolver_forhpc_amd.capi import GridArrays
 
 pytestmark = pytest.mark.gpu
@@ -259,6 +259,50 @@ def test_predictor_with_smagorinsky_matches_oracle(require_gpu, gname):
     O.lib().orc_predictor_les(g.ref, capi.CFD_CONV_QUICK, 1.0 / p.Re, p.dt, vis.ctypes.data, *T0, *o1)
     for c in range(3):
         assert rel_l2(sim.download(capi.T1_U + c), o1[c]) < 1e-13
+
+
+def stretched_grid(ncal):
+    """Stretched x / y axes (uniform band, growth 1.05 outside), uniform z."""
+    nx, ny, nz = ncal
+    ax = axis_tanh(nx, 20.0, 6.0, 20.0 / nx * 0.5, 2.0, 1.05)
+    ay = axis_tanh(ny, 10.0, 5.0, 10.0 / ny * 0.5, 2.0, 1.05)
+    az = uniform_grid((nz, nz, nz), (2.5, 2.5, 2.5))
+    return GridArrays(tuple(ncal), ax[0], ay[0], az.Z, ax[1], ay[1], az.Dz, ax[2], ay[2], az.Dzs, 20.0, 10.0, 2.5)
+
+
+@pytest.mark.parametrize("ncal", [(20, 70, 4), (17, 33, 6), (37, 19, 16), (23, 11, 32), (35, 9, 64), (19, 13, 70)])
+def test_tiled_quick_predictor_every_patch_shape(require_gpu, ncal):
+    """The QUICK predictor runs on LDS patches of tj x tk (j, k) columns
+    (tk = 4 ... 64 from the grid's depth: the extents here select each, 70
+    cells take five 16-deep patches) marching over 16 i-planes. None of these
+    extents fills its patches or plane ranges, the x / y axes are stretched,
+    z is closed or periodic, the eddy viscosity is off or on: against the
+    oracle at the predictor's bar."""
+    grid = stretched_grid(ncal)
+    g = O.Grid(grid)
+    T0 = random_state(grid, 41)
+    for periodic in (0, 1):
+        if periodic and ncal[2] % 2:
+            continue
+        for les in (0, 1):
+            sim, p = make_sim(grid, quirk=0)  # periodic z needs the source at its own cell
+            p.periodic_z, p.les = periodic, les
+            sim = Simulation(grid, p)
+            sim.init_state()
+            T1 = random_state(grid, 42)
+            for c in range(3):
+                sim.upload(capi.T0_U + c, T0[c])
+                sim.upload(capi.T1_U + c, T1[c])
+            sim.ConvectionDifussion()
+            vis = np.zeros(g.N)
+            if les:
+                O.lib().orc_smagorinsky(g.ref, *T0, vis)
+            o1 = [t.copy() for t in T1]
+            O.lib().orc_predictor_bl(g.ref, C.byref(p), vis.ctypes.data if les else None, *T0, *o1)
+            for c in range(3):
+                out = sim.download(capi.T1_U + c)
+                assert rel_l2(out, o1[c]) < 1e-13, (ncal, periodic, les, c)
+                assert np.array_equal(out == T1[c], o1[c] == T1[c]), (ncal, periodic, les, c)  # the same cells written
 
 
 def test_steps_with_smagorinsky_match_oracle(require_gpu):
