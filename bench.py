@@ -382,7 +382,7 @@ def field_errors(a, b):
     return err
 
 
-def same_residual_legs(steps, warmup, op=None, nu=2):
+def same_residual_legs(steps, warmup, op=None, nu=4):
     """FP64 and mixed precision at the same FP64 relative residual 1e-10 in
     every pass: BiCGSTAB on the ELL operator right-preconditioned by the
     multigrid V-cycle (FP64; FP64 Krylov around the FP32 cycle; FP64 defect
@@ -449,7 +449,7 @@ def main():
                     help="operator of the same-residual legs' FP64 Krylov and residual (default: the matrix-free "
                          "SPE form, the reference's default operator; the defect correction's FP32 inner solves "
                          "use the FP32 ELL values either way)")
-    ap.add_argument("--mg-nu", type=int, default=2, help="red-black sweeps before and after the coarse correction")
+    ap.add_argument("--mg-nu", type=int, default=4, help="red-black sweeps before and after the coarse correction")
     ap.add_argument("--reps", type=int, default=200)
     ap.add_argument("--adjoint-steps", type=int, default=40,
                     help="steps of the 256x128x32 adjoint run (0: skip); the drag is averaged over the second half")

@@ -305,7 +305,7 @@ int cfd_params_default(cfd_params *out) {
   out->ib_nsub = 100;
   out->u_init[0] = 1.0;   // RUN_CPU_PC_METHOD.hpp:77
   out->precond = CFD_PRECOND_JACOBI;
-  out->mg_nu = 2;
+  out->mg_nu = 4;  // fastest at 512x256x64: 2 -> 4 halves the iterations for ~1.4x the cycle
   out->convection = CFD_CONV_QUICK;  // controlPanel.hpp:125
   out->les = 0;                      // controlPanel.hpp:135 (TERBULENCE_SMAGORINSKY off)
   out->periodic_z = 0;

@@ -915,12 +915,12 @@ int mg_ensure(Ctx &c, bool fp32) {
         M->dense_n = n;
       }
     }
-    M->nu = c.prm.mg_nu > 0 ? c.prm.mg_nu : 2;
+    M->nu = c.prm.mg_nu > 0 ? c.prm.mg_nu : 4;
     for (auto &L : M->lv) M->shape += std::to_string(L.v.nx) + "x" + std::to_string(L.v.ny) + "x" +
                                       std::to_string(L.v.nz) + " ";
   }
   MGState &M = *c.mg;
-  M.nu = c.prm.mg_nu > 0 ? c.prm.mg_nu : 2;
+  M.nu = c.prm.mg_nu > 0 ? c.prm.mg_nu : 4;
   for (size_t l = 0; l < M.lv.size(); ++l) {
     MGLevel &L = M.lv[l];
     // the finest level's colour-split vectors hold nx ny ceil(nz / 2) cells per colour

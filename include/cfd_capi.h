@@ -139,7 +139,7 @@ typedef struct cfd_params {
   double u_init[3];      /* initial velocity (RUN_CPU_PC_METHOD.hpp:77) */
   int precond;           /* CFD_PRECOND_JACOBI (default) or CFD_PRECOND_MG */
   int mg_nu;             /* CFD_PRECOND_MG: red-black Gauss-Seidel sweeps before and after each coarse
-                            correction (default 2) */
+                            correction (default 4) */
   int convection;        /* face values of the predictor's convection term: CFD_CONV_* */
   int les;               /* 1: Smagorinsky eddy viscosity (C_s 0.18) added to nu in the predictor's diffusion
                             (smagorinskyModel.hpp:30-79, TERBULENCE_SMAGORINSKY); 0: off, the reference's default */

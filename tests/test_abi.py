@@ -36,7 +36,7 @@ def test_abi_version_and_defaults():
     assert L.cfd_abi_version() == 6
     p = default_params()
     # the reference's Jacobi-scaled solve unless multigrid is asked for
-    assert p.precond == capi.CFD_PRECOND_JACOBI and p.mg_nu == 2
+    assert p.precond == capi.CFD_PRECOND_JACOBI and p.mg_nu == 4
     assert p.convection == capi.CFD_CONV_QUICK and p.les == 0  # controlPanel.hpp:125, 135
     # src/CFD_MX.cpp:35-57, simuClass.hpp:74-80, bicgstabRe2.hpp:57, setEta.hpp:145-152
     assert (p.Re, p.dt, p.tol, p.maxit, p.sor_omega) == (40.0, 0.002, 1e-3, 5000, 1.8)
