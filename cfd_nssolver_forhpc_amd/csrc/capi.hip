@@ -149,9 +149,9 @@ int ensure_mixed(Ctx &c) {
   if (!c.ell_ready) return fail(CFD_ERR_STATE, "ELL operator not assembled (cfd_ell_assemble)");
   const int64_t N = c.g.N;
   if (!c.val32) {
-    int rc = dmalloc(&c.val32, (size_t)KSLOTS * N);
+    int rc = dmalloc(&c.val32, ell_len(N));
     if (rc) return rc;
-    CFD_HIP(launch_cast32(c, c.val, c.val32, KSLOTS * N));
+    CFD_HIP(launch_cast32(c, c.val, c.val32, (int64_t)ell_len(N)));
   }
   for (int q = 0; q < 8; ++q)
     if (!c.q32[q]) {
@@ -159,9 +159,9 @@ int ensure_mixed(Ctx &c) {
       if (rc) return rc;
     }
   if (c.valP && !c.valP32) {
-    int rc = dmalloc(&c.valP32, (size_t)KSLOTS * N);
+    int rc = dmalloc(&c.valP32, ell_len(N));
     if (rc) return rc;
-    CFD_HIP(launch_cast32(c, c.valP, c.valP32, KSLOTS * N));
+    CFD_HIP(launch_cast32(c, c.valP, c.valP32, (int64_t)ell_len(N)));
   }
   if (c.valP && !c.xP32) {
     int rc = dmalloc(&c.xP32, (size_t)N);
@@ -190,11 +190,11 @@ int ensure_perm(Ctx &c) {
   c.ncolour[0] = n0;
   c.ncolour[1] = N - n0;
   int rc = dmalloc(&c.perm, (size_t)N);
-  if (!rc) rc = dmalloc(&c.colP, (size_t)KSLOTS * N);
-  if (!rc) rc = dmalloc(&c.valP, (size_t)KSLOTS * N);
+  if (!rc) rc = dmalloc(&c.colP, ell_len(N));
+  if (!rc) rc = dmalloc(&c.valP, ell_len(N));
   if (!rc) rc = dmalloc(&c.xP, (size_t)N);
   if (!rc) rc = dmalloc(&c.bP, (size_t)N);
-  if (!rc && c.val32) rc = dmalloc(&c.valP32, (size_t)KSLOTS * N);
+  if (!rc && c.val32) rc = dmalloc(&c.valP32, ell_len(N));
   if (rc) return rc;
   CFD_HIP(hipMemcpyAsync(c.perm, perm.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, c.stream));
   CFD_HIP(launch_ell_permute(c));
@@ -512,7 +512,7 @@ int cfd_create(const cfd_grid_desc *grid, const cfd_params *params, int device, 
     if ((rc = dmalloc(&c->f[fid], len))) return bail(rc);
     if (hipMemset(c->f[fid], 0, len * sizeof(double)) != hipSuccess) return bail(fail(CFD_ERR_HIP, "memset"));
   }
-  if ((rc = dmalloc(&c->col, (size_t)KSLOTS * N)) || (rc = dmalloc(&c->val, (size_t)KSLOTS * N))) return bail(rc);
+  if ((rc = dmalloc(&c->col, ell_len(N))) || (rc = dmalloc(&c->val, ell_len(N)))) return bail(rc);
   double **kv[8] = {&c->r0, &c->r1, &c->p1, &c->s1, &c->ap, &c->as, &c->px1, &c->px2};
   for (auto p : kv) {
     if ((rc = dmalloc(p, (size_t)N))) return bail(rc);
@@ -631,7 +631,7 @@ int cfd_ell_assemble(cfd_ctx *ctx) {
   RET_LAUNCH(launch_ell_assemble(c));
   c.ell_ready = true;
   dfree(c.colP); dfree(c.valP); dfree(c.valP32); dfree(c.perm);
-  if (c.val32) RET_LAUNCH(launch_cast32(c, c.val, c.val32, KSLOTS * c.g.N));
+  if (c.val32) RET_LAUNCH(launch_cast32(c, c.val, c.val32, (int64_t)ell_len(c.g.N)));
   int rc = CFD_OK;
   if (c.prm.precision == CFD_MIXED) rc = ensure_mixed(c);
   if (!rc && c.prm.solver == CFD_SOLVER_RBSOR) rc = ensure_perm(c);
@@ -729,9 +729,21 @@ int cfd_ell_download(cfd_ctx *ctx, int32_t *col, double *val, size_t n_rows, int
   Ctx &c = *reinterpret_cast<Ctx *>(ctx);
   if (!c.ell_ready) return fail(CFD_ERR_STATE, "ELL operator not assembled");
   if (!col || !val || n_rows != (size_t)c.g.N || k != KSLOTS) return fail(CFD_ERR_ARG, "ELL shape mismatch");
-  CFD_HIP(hipMemcpyAsync(col, c.col, sizeof(int32_t) * KSLOTS * n_rows, hipMemcpyDeviceToHost, c.stream));
-  CFD_HIP(hipMemcpyAsync(val, c.val, sizeof(double) * KSLOTS * n_rows, hipMemcpyDeviceToHost, c.stream));
+  // the device arrays are sliced; the caller gets the reference's slot-major layout
+  const int64_t N = c.g.N;
+  std::vector<int32_t> hc(ell_len(N));
+  std::vector<double> hv(ell_len(N));
+  CFD_HIP(hipMemcpyAsync(hc.data(), c.col, sizeof(int32_t) * hc.size(), hipMemcpyDeviceToHost, c.stream));
+  CFD_HIP(hipMemcpyAsync(hv.data(), c.val, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, c.stream));
   CFD_HIP(hipStreamSynchronize(c.stream));
+  const int64_t st = ell_stride(N, c.ell_sliced);
+  for (int64_t r = 0; r < N; ++r) {
+    const int64_t b = ell_base(r, c.ell_sliced);
+    for (int s = 0; s < KSLOTS; ++s) {
+      col[(size_t)s * N + r] = hc[(size_t)(b + s * st)];
+      val[(size_t)s * N + r] = hv[(size_t)(b + s * st)];
+    }
+  }
   return CFD_OK;
 }
 
@@ -741,11 +753,25 @@ int cfd_ell_upload(cfd_ctx *ctx, const int32_t *col, const double *val, size_t n
   if (!col || !val || n_rows != (size_t)c.g.N || k != KSLOTS) return fail(CFD_ERR_ARG, "ELL shape mismatch");
   for (size_t q = 0; q < (size_t)KSLOTS * n_rows; ++q)
     if (col[q] < 0 || (size_t)col[q] >= n_rows) return fail(CFD_ERR_ARG, "ELL column index out of range");
-  CFD_HIP(hipMemcpyAsync(c.col, col, sizeof(int32_t) * KSLOTS * n_rows, hipMemcpyHostToDevice, c.stream));
-  CFD_HIP(hipMemcpyAsync(c.val, val, sizeof(double) * KSLOTS * n_rows, hipMemcpyHostToDevice, c.stream));
+  // slot-major from the caller into the library's sliced arrays (the last
+  // slice's padding rows: column 0, value 0, never read)
+  const int64_t N = c.g.N;
+  std::vector<int32_t> hc(ell_len(N), 0);
+  std::vector<double> hv(ell_len(N), 0.0);
+  const int64_t st = ell_stride(N, c.ell_sliced);
+  for (int64_t r = 0; r < N; ++r) {
+    const int64_t b = ell_base(r, c.ell_sliced);
+    for (int s = 0; s < KSLOTS; ++s) {
+      hc[(size_t)(b + s * st)] = col[(size_t)s * N + r];
+      hv[(size_t)(b + s * st)] = val[(size_t)s * N + r];
+    }
+  }
+  CFD_HIP(hipMemcpyAsync(c.col, hc.data(), sizeof(int32_t) * hc.size(), hipMemcpyHostToDevice, c.stream));
+  CFD_HIP(hipMemcpyAsync(c.val, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice, c.stream));
+  CFD_HIP(hipStreamSynchronize(c.stream));  // the staging vectors go out of scope
   c.ell_ready = true;
   dfree(c.colP); dfree(c.valP); dfree(c.valP32); dfree(c.perm);
-  if (c.val32) RET_LAUNCH(launch_cast32(c, c.val, c.val32, KSLOTS * c.g.N));
+  if (c.val32) RET_LAUNCH(launch_cast32(c, c.val, c.val32, (int64_t)ell_len(c.g.N)));
   CFD_HIP(hipStreamSynchronize(c.stream));
   return CFD_OK;
 }

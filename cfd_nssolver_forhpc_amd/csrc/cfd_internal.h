@@ -34,6 +34,23 @@ struct DevGrid {
   double lx, ly, lz;
 };
 
+// Position of slot s of row r in a device ELL array.  Slot-major (sliced
+// 0) is the reference's layout, slot s of every row in a slab of N
+// (ELL_sparseMatrix.hpp:146-160), as callers hand arrays over.  The library's
+// own arrays are sliced: rows in slices of ELL_SLICE, each slice's KSLOTS
+// slots contiguous, so a wave's slot loads stay inside one 3.5 KB (FP64
+// values) region instead of spreading over seven slabs 67 MB apart (SpMV-dot
+// at 512x256x64: 134.0 against 140.4 us).  Row r's slots are then
+// base + s * stride.
+constexpr int ELL_SLICE = 64;
+__host__ __device__ inline int64_t ell_base(int64_t r, int sliced) {
+  const uint64_t u = (uint64_t)r;
+  return sliced ? (int64_t)((u / ELL_SLICE) * ((uint64_t)KSLOTS * ELL_SLICE) + u % ELL_SLICE) : r;
+}
+__host__ __device__ inline int64_t ell_stride(int64_t N, int sliced) { return sliced ? ELL_SLICE : N; }
+// entries of a sliced array (the last slice padded to ELL_SLICE rows)
+inline size_t ell_len(int64_t N) { return (size_t)KSLOTS * (size_t)((N + ELL_SLICE - 1) / ELL_SLICE) * ELL_SLICE; }
+
 __host__ __device__ inline int64_t icel(const DevGrid &g, int i, int j, int k) {
   return ((int64_t)i * g.ny + j) * g.nz + k;
 }
@@ -70,7 +87,9 @@ struct Ctx {
   std::vector<double> hX, hY, hZ, hDx, hDy, hDz, hDxs, hDys, hDzs;
   // fields
   double *f[CFD_NFIELDS] = {};
-  // ELL (slot-major), FP32 copy, colour-permuted copy for RB-SOR
+  // ELL (sliced, ell_base; slot-major where ell_sliced is 0: arrays a
+  // caller hands over), FP32 copy, colour-permuted copy for RB-SOR
+  int ell_sliced = 1;
   int32_t *col = nullptr;
   double *val = nullptr;
   float *val32 = nullptr;

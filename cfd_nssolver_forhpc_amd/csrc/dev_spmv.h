@@ -1,9 +1,10 @@
 // dev_spmv.h -- row kernels of the pressure operator, inlined into the
 // SpMV, Krylov and SOR kernels.
 //
-// ELL rows are read slot-major (val[s*N + row]): lane r of a wave reads
-// element r of each slot slab, so every slot load of a wave64 is one
-// contiguous 512 B (FP64) / 256 B (FP32) segment.  The matrix streams once
+// ELL rows are read at ell_base(row) + s * stride (the library's arrays in
+// 64-row slices, a caller's slot-major): lane r of a wave reads element r of
+// each slot, so every slot load of a wave64 is one contiguous 512 B (FP64) /
+// 256 B (FP32) segment.  The matrix streams once
 // per SpMV and is loaded non-temporally so that it does not evict the
 // x vector, whose seven gathers per row (row, row +- 1, +- nz, +- ny*nz)
 // are served from L2 / Infinity Cache.  The accumulation is an FMA chain in
@@ -32,13 +33,14 @@ __device__ inline void ntstore(T v, T *p) {
 }
 
 template <typename TV, typename TX>
-__device__ inline TX ell_row(const int32_t *__restrict__ col, const TV *__restrict__ val, int64_t N, int64_t r,
-                             const TX *__restrict__ x) {
+__device__ inline TX ell_row(const int32_t *__restrict__ col, const TV *__restrict__ val, int64_t N, int sliced,
+                             int64_t r, const TX *__restrict__ x) {
+  const int64_t b = ell_base(r, sliced), st = ell_stride(N, sliced);
   TX acc = TX(0);
 #pragma unroll
   for (int s = 0; s < KSLOTS; ++s) {
-    const int32_t c = ntload(col + (int64_t)s * N + r);
-    const TX v = (TX)ntload(val + (int64_t)s * N + r);
+    const int32_t c = ntload(col + b + s * st);
+    const TX v = (TX)ntload(val + b + s * st);
     acc = fma(v, x[c], acc);
   }
   return acc;
@@ -121,6 +123,7 @@ __device__ inline double csr_row(const int32_t *__restrict__ ptr, const int32_t 
 struct OpArgs {
   int op;                    // CFD_OP_ELL / CFD_OP_SPE / CFD_OP_CSR
   int64_t N;
+  int sliced;                // ELL layout (ell_base)
   const int32_t *col;
   const double *val;
   const float *val32;
@@ -229,8 +232,8 @@ constexpr int OP_ELL_VAL32 = 16;
 __device__ inline double op_row(const OpArgs &a, int64_t r, const double *__restrict__ x) {
   if (a.op == CFD_OP_SPE) return spe_row(a.coef, a.jp, a.nxc, a.nyc, a.nzc, a.div_nz, a.div_ny, r, x, a.wrapz);
   if (a.op == CFD_OP_CSR) return csr_row(a.cptr, a.ccol, a.cval, r, x);
-  if (a.op == OP_ELL_VAL32) return ell_row<float, double>(a.col, a.val32, a.N, r, x);
-  return ell_row<double, double>(a.col, a.val, a.N, r, x);
+  if (a.op == OP_ELL_VAL32) return ell_row<float, double>(a.col, a.val32, a.N, a.sliced, r, x);
+  return ell_row<double, double>(a.col, a.val, a.N, a.sliced, r, x);
 }
 
 // CSR rows of one 256-row block staged through LDS.  The block's entries are
@@ -308,11 +311,12 @@ __device__ inline double op_row_block(const OpArgs &a, int64_t base, const doubl
 // digits (a pure FP32 accumulation loses them for smooth x and stalls the
 // Krylov iteration at 1e5+ unknowns).
 __device__ inline float op_row32(const OpArgs &a, int64_t r, const float *__restrict__ x) {
+  const int64_t b = ell_base(r, a.sliced), st = ell_stride(a.N, a.sliced);
   double acc = 0.0;
 #pragma unroll
   for (int s = 0; s < KSLOTS; ++s) {
-    const int32_t c = ntload(a.col + (int64_t)s * a.N + r);
-    acc = fma((double)ntload(a.val32 + (int64_t)s * a.N + r), (double)x[c], acc);
+    const int32_t c = ntload(a.col + b + s * st);
+    acc = fma((double)ntload(a.val32 + b + s * st), (double)x[c], acc);
   }
   return (float)acc;
 }
@@ -321,6 +325,7 @@ inline OpArgs op_args(const Ctx &c, int op) {
   OpArgs a;
   a.op = op;
   a.N = c.g.N;
+  a.sliced = c.ell_sliced;
   a.col = c.col;
   a.val = c.val;
   a.val32 = c.val32;

@@ -18,7 +18,7 @@ struct PBC {
   int periodic_z;  // z faces wrap: no closure term, the wrapped neighbour appended after zp
 };
 
-__global__ void __launch_bounds__(256) ell_assemble_kernel(DevGrid g, PBC b, int32_t *__restrict__ col,
+__global__ void __launch_bounds__(256) ell_assemble_kernel(DevGrid g, PBC b, int sliced, int32_t *__restrict__ col,
                                                            double *__restrict__ val, float *__restrict__ val32,
                                                            double *__restrict__ jp) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -65,11 +65,12 @@ __global__ void __launch_bounds__(256) ell_assemble_kernel(DevGrid g, PBC b, int
   if (b.periodic_z && k == zBeg) { cs[s] = (int32_t)(r + (g.nzc - 1)); vs[s] = -zm * jr; ++s; }
   if (b.periodic_z && k == zEnd) { cs[s] = (int32_t)(r - (g.nzc - 1)); vs[s] = -zp * jr; ++s; }
   for (; s < KSLOTS; ++s) { cs[s] = (int32_t)(N - 1); vs[s] = 0.0; }
+  const int64_t eb = ell_base(r, sliced), est = ell_stride(N, sliced);
 #pragma unroll
   for (int q = 0; q < KSLOTS; ++q) {
-    col[(int64_t)q * N + r] = cs[q];
-    val[(int64_t)q * N + r] = vs[q];
-    if (val32) val32[(int64_t)q * N + r] = (float)vs[q];
+    col[eb + q * est] = cs[q];
+    val[eb + q * est] = vs[q];
+    if (val32) val32[eb + q * est] = (float)vs[q];
   }
 }
 
@@ -77,19 +78,20 @@ __global__ void __launch_bounds__(256) ell_assemble_kernel(DevGrid g, PBC b, int
 // perm[r] (colour-0 rows first, natural order within a colour) and its
 // columns are renumbered the same way, so each colour sweep streams a
 // contiguous half of the matrix.
-__global__ void __launch_bounds__(256) ell_permute_kernel(int64_t N, const int32_t *__restrict__ perm,
+__global__ void __launch_bounds__(256) ell_permute_kernel(int64_t N, int sliced, const int32_t *__restrict__ perm,
                                                           const int32_t *__restrict__ col,
                                                           const double *__restrict__ val, int32_t *__restrict__ colP,
                                                           double *__restrict__ valP, float *__restrict__ valP32) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= N) return;
   const int64_t q = perm[r];
+  const int64_t br = ell_base(r, sliced), bq = ell_base(q, sliced), st = ell_stride(N, sliced);
 #pragma unroll
   for (int s = 0; s < KSLOTS; ++s) {
-    colP[(int64_t)s * N + q] = perm[col[(int64_t)s * N + r]];
-    const double v = val[(int64_t)s * N + r];
-    valP[(int64_t)s * N + q] = v;
-    if (valP32) valP32[(int64_t)s * N + q] = (float)v;
+    colP[bq + s * st] = perm[col[br + s * st]];
+    const double v = val[br + s * st];
+    valP[bq + s * st] = v;
+    if (valP32) valP32[bq + s * st] = (float)v;
   }
 }
 
@@ -192,11 +194,12 @@ __global__ void __launch_bounds__(256) spmv32_from64_kernel(OpArgs a, const doub
                                                             double *__restrict__ y) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.N; r += stride) {
+    const int64_t b = ell_base(r, a.sliced), st = ell_stride(a.N, a.sliced);
     double acc = 0.0;
 #pragma unroll
     for (int s = 0; s < KSLOTS; ++s) {
-      const int32_t c = ntload(a.col + (int64_t)s * a.N + r);
-      acc = fma((double)ntload(a.val32 + (int64_t)s * a.N + r), (double)(float)x[c], acc);
+      const int32_t c = ntload(a.col + b + s * st);
+      acc = fma((double)ntload(a.val32 + b + s * st), (double)(float)x[c], acc);
     }
     y[r] = acc;
   }
@@ -222,14 +225,14 @@ PBC pbc_of(const Ctx &c) {
 int stream_blocks(int64_t n) { return blocks_for(n, 256, 1 << 30); }
 
 hipError_t launch_ell_assemble(Ctx &c) {
-  ell_assemble_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g, pbc_of(c), c.col, c.val, c.val32,
-                                                                             c.f[CFD_JP]);
+  ell_assemble_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g, pbc_of(c), c.ell_sliced, c.col,
+                                                                             c.val, c.val32, c.f[CFD_JP]);
   return hipGetLastError();
 }
 
 hipError_t launch_ell_permute(Ctx &c) {
-  ell_permute_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g.N, c.perm, c.col, c.val, c.colP,
-                                                                            c.valP, c.valP32);
+  ell_permute_kernel<<<blocks_for(c.g.N, 256, 1 << 30), 256, 0, c.stream>>>(c.g.N, c.ell_sliced, c.perm, c.col,
+                                                                            c.val, c.colP, c.valP, c.valP32);
   return hipGetLastError();
 }
 

@@ -1121,6 +1121,7 @@ extern "C" int cfd_bicgstab_ell_noalloc(const int32_t *col, const double *val, c
   c.g.N = N;
   c.g.nxc = (int)N;
   c.g.nyc = c.g.nzc = 1;
+  c.ell_sliced = 0;  // the caller's arrays are in the reference's slot-major layout
   c.col = const_cast<int32_t *>(col);
   c.val = const_cast<double *>(val);
   c.ell_ready = true;

@@ -18,35 +18,38 @@ namespace cfd {
 namespace {
 
 template <typename TV, typename TX>
-__global__ void __launch_bounds__(256) sweep_kernel(int64_t N, int64_t off, int64_t cnt,
+__global__ void __launch_bounds__(256) sweep_kernel(int64_t N, int sliced, int64_t off, int64_t cnt,
                                                     const int32_t *__restrict__ colP, const TV *__restrict__ valP,
                                                     const TX *__restrict__ bP, TX *__restrict__ xP, double omega) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t st = ell_stride(N, sliced);
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += stride) {
-    const int64_t q = off + t;
+    const int64_t q = off + t, b = ell_base(q, sliced);
     double acc = 0.0;
 #pragma unroll
     for (int s = 0; s < KSLOTS; ++s) {
-      const int32_t cidx = ntload(colP + (int64_t)s * N + q);
-      acc = fma((double)ntload(valP + (int64_t)s * N + q), (double)xP[cidx], acc);
+      const int32_t cidx = ntload(colP + b + s * st);
+      acc = fma((double)ntload(valP + b + s * st), (double)xP[cidx], acc);
     }
-    const double d = (double)valP[q];
+    const double d = (double)valP[b];  // slot 0: the diagonal
     xP[q] = (TX)((double)xP[q] + omega * ((double)bP[q] - acc) / d);
   }
 }
 
 template <typename TV, typename TX>
-__global__ void __launch_bounds__(256) resid_kernel(int64_t N, const int32_t *__restrict__ colP,
+__global__ void __launch_bounds__(256) resid_kernel(int64_t N, int sliced, const int32_t *__restrict__ colP,
                                                     const TV *__restrict__ valP, const TX *__restrict__ bP,
                                                     const TX *__restrict__ xP, double *part) {
   double acc[2] = {0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t st = ell_stride(N, sliced);
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += stride) {
+    const int64_t b = ell_base(q, sliced);
     double ax = 0.0;
 #pragma unroll
     for (int s = 0; s < KSLOTS; ++s) {
-      const int32_t cidx = ntload(colP + (int64_t)s * N + q);
-      ax = fma((double)ntload(valP + (int64_t)s * N + q), (double)xP[cidx], ax);
+      const int32_t cidx = ntload(colP + b + s * st);
+      ax = fma((double)ntload(valP + b + s * st), (double)xP[cidx], ax);
     }
     const double r = (double)bP[q] - ax;
     acc[0] += r * r;
@@ -91,13 +94,13 @@ int rbsor_generic(Ctx &c, const TV *valP, const TX *bP, TX *xP, double omega, do
     for (int s = 0; s < chunk; ++s) {
       for (int colour = 0; colour < 2; ++colour) {
         const int64_t off = colour ? c.ncolour[0] : 0, cnt = c.ncolour[colour];
-        sweep_kernel<TV, TX><<<blocks_for(cnt, 256, 1 << 30), 256, 0, c.stream>>>(N, off, cnt, c.colP, valP, bP, xP,
-                                                                                omega);
+        sweep_kernel<TV, TX><<<blocks_for(cnt, 256, 1 << 30), 256, 0, c.stream>>>(N, c.ell_sliced, off, cnt, c.colP,
+                                                                                valP, bP, xP, omega);
       }
     }
     it += chunk;
     double *part = red_part_host(c, RS_NORM);
-    resid_kernel<TV, TX><<<nbr, 256, 0, c.stream>>>(N, c.colP, valP, bP, xP, part);
+    resid_kernel<TV, TX><<<nbr, 256, 0, c.stream>>>(N, c.ell_sliced, c.colP, valP, bP, xP, part);
     finalize2<<<1, FIN_THREADS, 0, c.stream>>>(part, nbr, c.red.out + RS_NORM * 4);
     CFD_HIP(hipGetLastError());
     CFD_HIP(hipMemcpyAsync(c.hred, c.red.out + RS_NORM * 4, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -146,8 +149,8 @@ hipError_t launch_rbsor_sweep(Ctx &c, double omega) {
   const int64_t N = c.g.N;
   for (int colour = 0; colour < 2; ++colour) {
     const int64_t off = colour ? c.ncolour[0] : 0, cnt = c.ncolour[colour];
-    sweep_kernel<double, double><<<blocks_for(cnt, 256, 1 << 30), 256, 0, c.stream>>>(N, off, cnt, c.colP, c.valP,
-                                                                                    c.bP, c.xP, omega);
+    sweep_kernel<double, double><<<blocks_for(cnt, 256, 1 << 30), 256, 0, c.stream>>>(N, c.ell_sliced, off, cnt,
+                                                                                    c.colP, c.valP, c.bP, c.xP, omega);
   }
   return hipGetLastError();
 }

@@ -25,7 +25,9 @@
  *     hold xm, xp, ym, yp, zm, zp with absent boundary neighbours skipped and
  *     padding (col = N-1, val = 0) at the end of each row, exactly the slot
  *     order mat::ELL_matrix<T>::set produces (inc/matrix/PressureMatrix.hpp:71-89,
- *     inc/matrix/ELL_sparseMatrix.hpp:146-160, 246-320).
+ *     inc/matrix/ELL_sparseMatrix.hpp:146-160, 246-320).  This is the layout
+ *     of every ELL array crossing the ABI; the library's own device copy is
+ *     stored in 64-row slices and cfd_ell_download / cfd_ell_upload convert.
  *
  * Threading: a context owns one HIP stream and its device buffers; calls on
  * one context are stream-ordered and must come from one thread at a time.
