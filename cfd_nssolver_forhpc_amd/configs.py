@@ -59,7 +59,7 @@ def cylinder_params(Re: float, dt: float, cx: float, cy: float, D: float, tol: f
     return p
 
 
-def baseline_bc(p: CfdParams, speed: float = 1.0) -> CfdParams:
+def outflow_periodic_bc(p: CfdParams, speed: float = 1.0) -> CfdParams:
     """The cylinder workloads' boundaries: convective outflow on x+ at the
     inflow speed (pressure p = 0 there) and periodic z; uniform inflow and
     slip y-walls stay in the table."""
@@ -73,7 +73,7 @@ def config_a() -> Workload:
     """128x64x4 over 20D x 10D x 0.5D, D = 1 at (5, 5), Re 100, dt 0.002, 100 steps, FP64, tol 1e-10;
     convective outflow, slip y-walls, periodic z."""
     g = uniform_grid((128, 64, 4), (20.0, 10.0, 0.5))
-    return Workload("cyl-128x64x4-Re100", g, baseline_bc(cylinder_params(100.0, 0.002, 5.0, 5.0, 1.0, 1e-10)), 100,
+    return Workload("cyl-128x64x4-Re100", g, outflow_periodic_bc(cylinder_params(100.0, 0.002, 5.0, 5.0, 1.0, 1e-10)), 100,
                     1e-3, 1)
 
 
@@ -82,7 +82,7 @@ def config_b(precision: int = capi.CFD_FP64) -> Workload:
     convective outflow, slip y-walls, periodic z."""
     g = uniform_grid((512, 256, 64), (20.0, 10.0, 2.5))
     name = "cyl-512x256x64-Re200" + ("-mixed" if precision == capi.CFD_MIXED else "-fp64")
-    return Workload(name, g, baseline_bc(cylinder_params(200.0, 5e-4, 5.0, 5.0, 1.0, 1e-10, precision)), 200, 1e-3,
+    return Workload(name, g, outflow_periodic_bc(cylinder_params(200.0, 5e-4, 5.0, 5.0, 1.0, 1e-10, precision)), 200, 1e-3,
                     1)
 
 
@@ -90,7 +90,7 @@ def config_d() -> Workload:
     """256x128x32 over 20 x 10 x 2.5, Re 100, dt 1e-3, 400 steps (adjoint case); convective outflow,
     slip y-walls, periodic z."""
     g = uniform_grid((256, 128, 32), (20.0, 10.0, 2.5))
-    return Workload("cyl-256x128x32-Re100", g, baseline_bc(cylinder_params(100.0, 1e-3, 5.0, 5.0, 1.0, 1e-10)),
+    return Workload("cyl-256x128x32-Re100", g, outflow_periodic_bc(cylinder_params(100.0, 1e-3, 5.0, 5.0, 1.0, 1e-10)),
                     400, 1e-3, 1)
 
 
@@ -104,7 +104,7 @@ def config_e(precision: int = capi.CFD_MIXED) -> Workload:
     az = axis_uniform(128, 128 * h0)
     g = _GA((1024, 512, 128), ax[0], ay[0], az[0], ax[1], ay[1], az[1], ax[2], ay[2], az[2], 40.0, 20.0, 128 * h0)
     return Workload("cyl-1024x512x128-Re300", g,
-                    baseline_bc(cylinder_params(300.0, 2.5e-4, 10.0, 10.0, D, 1e-10, precision)), 100, 1e-3, 1)
+                    outflow_periodic_bc(cylinder_params(300.0, 2.5e-4, 10.0, 10.0, D, 1e-10, precision)), 100, 1e-3, 1)
 
 
 def small_cylinder(ncal=(32, 16, 8), lengths=(8.0, 4.0, 2.0), quirk=0, tol=1e-10, inner=3) -> Workload:

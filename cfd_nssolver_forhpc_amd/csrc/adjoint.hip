@@ -242,9 +242,9 @@ __global__ void __launch_bounds__(256) bc_z_periodic_adj_kernel(DevGrid g, doubl
 // u3 = (1 - gain e) T2, f = (u3 - T2)/dt with lf = dt*Lam (+ the drag weight
 // on f_x in the corrector): writes lt2 (3N, 0 where a component is not
 // updated) and the per-block partial of dJ/dgain.
-// Faces updated beyond the table ranges (BASELINE mode): the outflow face of
-// u (iu_end = nx - 2; no forcing there, e = 0) and the seam face of w
-// (kw_end = nz - 2).
+// Faces updated beyond the table ranges (convective outflow, periodic z):
+// the outflow face of u (iu_end = nx - 2; no forcing there, e = 0) and the
+// seam face of w (kw_end = nz - 2).
 struct AdjFaces {
   int iu_end, kw_end, conv, wrapz;
 };

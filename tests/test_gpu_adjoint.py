@@ -99,20 +99,20 @@ def test_gradient_matches_central_differences(require_gpu):
         assert rel(fd, ad) < 1e-5
 
 
-def baseline_case(tol=1e-13):
+def outflow_periodic_case(tol=1e-13):
     """The small case with the workloads' boundaries: convective outflow and
     periodic z."""
     w, uin = case(tol)
-    configs.baseline_bc(w.params)
+    configs.outflow_periodic_bc(w.params)
     return w, uin
 
 
-def test_baseline_bc_objective_and_gradient_match_the_oracle(require_gpu):
+def test_outflow_periodic_bc_objective_and_gradient_match_the_oracle(require_gpu):
     """With the workloads' boundaries the oracle's gradient (explicit
     transposes, the transposed solve on the explicit transpose of the
     wrapped ELL operator, every step recorded) pins the GPU's (the symmetric
     reformulation of the transposed solve, checkpoints and recomputation)."""
-    w, uin = baseline_case()
+    w, uin = outflow_periodic_case()
     pr = gpu_problem(w, every=3)
     J = pr.run(uin, 0.9)
     g_uin, g_gain, st = pr.sim.adjoint_backward()
@@ -124,13 +124,13 @@ def test_baseline_bc_objective_and_gradient_match_the_oracle(require_gpu):
     assert rel(g_gain, go_gain) < 1e-8
 
 
-def test_baseline_bc_gradient_matches_central_differences(require_gpu):
+def test_outflow_periodic_bc_gradient_matches_central_differences(require_gpu):
     """The transposed step with the convective outflow (its implicit upwind
     update from the ghosts' old values, the projected outflow face) and
     periodic z (ghost planes folded into their images, the wrapped pressure
     coupling, the seam face of w), against central differences of the GPU
     forward run, as for the table conditions above."""
-    w, uin = baseline_case()
+    w, uin = outflow_periodic_case()
     pr = gpu_problem(w, every=4)
     base = 0.9
     pr.run(uin, base)
@@ -146,12 +146,12 @@ def test_baseline_bc_gradient_matches_central_differences(require_gpu):
         Jm = pr.run(uin - h * d[:-1], base - h * d[-1])
         fd = (Jp - Jm) / (2 * h)
         ad = float(np.dot(g_uin, d[:-1]) + g_gain * d[-1])
-        print("baseline bc: fd", fd, "adjoint", ad, "rel", rel(fd, ad))
+        print("outflow/periodic: fd", fd, "adjoint", ad, "rel", rel(fd, ad))
         assert rel(fd, ad) < 1e-5
 
 
-def test_baseline_bc_checkpoint_interval_does_not_change_the_gradient(require_gpu):
-    w, uin = baseline_case(tol=1e-10)
+def test_outflow_periodic_bc_checkpoint_interval_does_not_change_the_gradient(require_gpu):
+    w, uin = outflow_periodic_case(tol=1e-10)
     res = []
     for every in (1, 4, STEPS):
         pr = gpu_problem(w, every)
@@ -285,7 +285,7 @@ def test_bad_adjoint_arguments_are_errors(require_gpu):
     from cfd_nssolver_forhpc_amd.solver import axis_tanh, axis_uniform
     ax, ay, az = axis_uniform(24, 6.0), axis_uniform(12, 3.0), axis_tanh(8, 2.0, 1.0, 0.2, 0.4, 1.1)
     gz = GridArrays((24, 12, 8), ax[0], ay[0], az[0], ax[1], ay[1], az[1], ax[2], ay[2], az[2], 6.0, 3.0, 2.0)
-    p4 = configs.baseline_bc(w.params.copy())
+    p4 = configs.outflow_periodic_bc(w.params.copy())
     sim4 = Simulation(gz, p4)
     sim4.init_state(1e-3, 1)
     sim4.createPressureMatrix()

@@ -104,7 +104,7 @@ def test_steps_match_oracle(require_gpu, op, solver):
         st = sim.step()
         ost = osim.step()
         errs = [rel_l2(sim.download(capi.T0_U + c), osim.field(capi.T0_U + c)) for c in range(3)]
-        print(f"[baseline bc op {op} solver {solver}] step {k + 1}: iters {list(st.iters[:4])} "
+        print(f"[outflow/periodic op {op} solver {solver}] step {k + 1}: iters {list(st.iters[:4])} "
               f"oracle {list(ost.iters[:4])} err {['%.1e' % e for e in errs]} div fluid {st.div_fluid_max:.1e}",
               flush=True)
         for c, e in enumerate(errs):
@@ -156,7 +156,7 @@ def test_multigrid_levels_wrap_in_z(require_gpu, ncal, precision):
     from cfd_nssolver_forhpc_amd.solver import uniform_grid
     grid = uniform_grid(ncal, (10.0, 5.0, 2.5))
     p = configs.cylinder_params(100.0, 0.002, 2.5, 2.5, 1.0, 1e-10, precision)
-    configs.baseline_bc(p)
+    configs.outflow_periodic_bc(p)
     p.maxit = 200000
     rng = np.random.default_rng(31)
     ref = Simulation(grid, p)

@@ -75,7 +75,7 @@ def test_bc_adjoint_dot_product(seed):
 def _bl_params(g, seed):
     """The workloads' boundary mode (convective outflow, periodic z) with
     random Neumann factors and no Dirichlet constants, so that BC is linear."""
-    p = configs.baseline_bc(configs.cylinder_params(100.0, 0.002, 1.5, 1.0, 0.45, 1e-10))
+    p = configs.outflow_periodic_bc(configs.cylinder_params(100.0, 0.002, 1.5, 1.0, 0.45, 1e-10))
     p.outflow_speed = 0.7
     rng = np.random.default_rng(seed)
     for f in range(6):
@@ -271,7 +271,7 @@ def test_gradient_matches_central_differences_convective_periodic():
     """The workloads' boundaries (convective outflow, periodic z) through the
     whole objective: the oracle's gradient against its own forward run."""
     w, g, base = _case()
-    configs.baseline_bc(w.params)
+    configs.outflow_periodic_bc(w.params)
     steps, avg_from = 10, 5
     J, (g_uin, g_gain, it) = _objective(w, g, base, 0.9, steps, avg_from, grad=True)
     assert np.isfinite(J) and it > 0
