@@ -324,7 +324,7 @@ def adjoint_benchmark(steps, avg_from, every, fd_dirs, precond, tol=1e-10, h=1e-
     optionally central differences (h = 1e-6) in fd_dirs random directions,
     about the base forcing gain `gain`."""
     from cfd_nssolver_forhpc_amd.adjoint import DragProblem, smoothed_inflow
-    w = configs.config_d()
+    w = configs.cylinder_256x128x32()
     w.params.precond = precond
     w.params.tol = tol
     sim = Simulation(w.grid, w.params)
@@ -391,7 +391,7 @@ def same_residual_legs(steps, warmup, op=None, nu=4):
     for tag, prec, mode in (("fp64_mg", capi.CFD_FP64, capi.CFD_MIXED_RESTARTED),
                             ("mixed_mg", capi.CFD_MIXED, capi.CFD_MIXED_PC32),
                             ("mixed_mg_dc", capi.CFD_MIXED, capi.CFD_MIXED_RESTARTED)):
-        wg = configs.config_b(prec)
+        wg = configs.cylinder_512x256x64(prec)
         wg.params.precond = capi.CFD_PRECOND_MG
         wg.params.mixed_mode = mode
         wg.params.tol_inner = 1e-5
@@ -480,7 +480,7 @@ def main():
               flush=True)
         return
 
-    w64 = configs.config_b(capi.CFD_FP64)
+    w64 = configs.cylinder_512x256x64(capi.CFD_FP64)
     if args.poisson_only:
         print(json.dumps({"poisson": poisson_benchmark(w64, args.poisson_sor_full)}), flush=True)
         return
@@ -518,7 +518,7 @@ def main():
 
     spe = None
     if not args.no_spe and args.spe_steps > 0:
-        ws = configs.config_b(capi.CFD_FP64)
+        ws = configs.cylinder_512x256x64(capi.CFD_FP64)
         ws.params.op = capi.CFD_OP_SPE
         sims, ts, sts = run_steps(ws, warmup, args.spe_steps, "spe")
         spe = {"operator": "matrix-free SPE (the reference's default P_SOLVER_BICG_SPE), bicgstabRe2 tol 1e-10, "

@@ -69,24 +69,24 @@ def outflow_periodic_bc(p: CfdParams, speed: float = 1.0) -> CfdParams:
     return p
 
 
-def config_a() -> Workload:
+def cylinder_128x64x4() -> Workload:
     """128x64x4 over 20D x 10D x 0.5D, D = 1 at (5, 5), Re 100, dt 0.002, 100 steps, FP64, tol 1e-10;
     convective outflow, slip y-walls, periodic z."""
     g = uniform_grid((128, 64, 4), (20.0, 10.0, 0.5))
-    return Workload("cyl-128x64x4-Re100", g, outflow_periodic_bc(cylinder_params(100.0, 0.002, 5.0, 5.0, 1.0, 1e-10)), 100,
-                    1e-3, 1)
+    return Workload("cyl-128x64x4-Re100", g,
+                    outflow_periodic_bc(cylinder_params(100.0, 0.002, 5.0, 5.0, 1.0, 1e-10)), 100, 1e-3, 1)
 
 
-def config_b(precision: int = capi.CFD_FP64) -> Workload:
+def cylinder_512x256x64(precision: int = capi.CFD_FP64) -> Workload:
     """512x256x64 over 20 x 10 x 2.5 (D = 25 cells), Re 200, dt 5e-4, 200 steps, tol 1e-10;
     convective outflow, slip y-walls, periodic z."""
     g = uniform_grid((512, 256, 64), (20.0, 10.0, 2.5))
     name = "cyl-512x256x64-Re200" + ("-mixed" if precision == capi.CFD_MIXED else "-fp64")
-    return Workload(name, g, outflow_periodic_bc(cylinder_params(200.0, 5e-4, 5.0, 5.0, 1.0, 1e-10, precision)), 200, 1e-3,
-                    1)
+    return Workload(name, g, outflow_periodic_bc(cylinder_params(200.0, 5e-4, 5.0, 5.0, 1.0, 1e-10, precision)),
+                    200, 1e-3, 1)
 
 
-def config_d() -> Workload:
+def cylinder_256x128x32() -> Workload:
     """256x128x32 over 20 x 10 x 2.5, Re 100, dt 1e-3, 400 steps (adjoint case); convective outflow,
     slip y-walls, periodic z."""
     g = uniform_grid((256, 128, 32), (20.0, 10.0, 2.5))
@@ -94,7 +94,7 @@ def config_d() -> Workload:
                     400, 1e-3, 1)
 
 
-def config_e(precision: int = capi.CFD_MIXED) -> Workload:
+def cylinder_1024x512x128(precision: int = capi.CFD_MIXED) -> Workload:
     """1024x512x128, x/y stretched (growth 1.05) around a D = 40-cell cylinder, Re 300, dt 2.5e-4, 100 steps."""
     D = 1.0
     h0 = D / 40.0

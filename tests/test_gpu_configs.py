@@ -1,6 +1,6 @@
 """The benchmark workloads at their own sizes on the GPU.
 
-- 512x256x64 cylinder, Re 200, dt 5e-4, FP64 (configs.config_b; convective
+- 512x256x64 cylinder, Re 200, dt 5e-4, FP64 (configs.cylinder_512x256x64; convective
   outflow, periodic z):
   * the reference's algorithm (Jacobi-scaled BiCGSTAB, iters_max 5000):
     every pass keeps bicgstabRe2's semantics -- at most 5000 iterations, a
@@ -14,7 +14,7 @@
   defect correction around FP32 inner solves to 1e-5: every pass at the
   FP64 residual 1e-10 (host-checked on the corrector pass), fields against
   the FP64 run of the same preconditioner, divergence check in the fluid.
-- 256x128x32 adjoint case (configs.config_d), Re 100, dt 1e-3, 400 steps,
+- 256x128x32 adjoint case (configs.cylinder_256x128x32), Re 100, dt 1e-3, 400 steps,
   checkpoints every 20: the gradient of the mean drag over the last 200
   steps against central differences (h = 1e-6), one direction in u_in only
   at the reference gain 1 and two random directions in (u_in, gain) about
@@ -54,7 +54,7 @@ def true_relres(sim, col, val):
 
 
 def mg_workload(precision, mode):
-    w = configs.config_b(precision)
+    w = configs.cylinder_512x256x64(precision)
     w.params.precond = capi.CFD_PRECOND_MG
     w.params.mixed_mode = mode
     w.params.tol_inner = 1e-5
@@ -64,7 +64,7 @@ def mg_workload(precision, mode):
 
 @pytest.mark.timeout(600)
 def test_b_reference_algorithm_semantics(require_gpu):
-    w = configs.config_b(capi.CFD_FP64)
+    w = configs.cylinder_512x256x64(capi.CFD_FP64)
     assert tuple(w.grid.ncal) == (512, 256, 64) and w.params.maxit == 5000 and w.params.tol == 1e-10
     sim = Simulation(w.grid, w.params)
     sim.setup(noise=w.noise, seed=w.seed)
@@ -108,7 +108,7 @@ def test_b_multigrid_fp64_reaches_1e10(require_gpu):
 
 @pytest.mark.timeout(300)
 def test_b_standalone_poisson(require_gpu):
-    w = configs.config_b(capi.CFD_FP64)
+    w = configs.cylinder_512x256x64(capi.CFD_FP64)
     w.params.maxit = 20000
     w.params.periodic_z = w.params.outflow_convective = 0  # Neumann walls, Dirichlet outflow face
     sim = Simulation(w.grid, w.params)
@@ -162,7 +162,7 @@ def test_c_mixed_twin_against_fp64(require_gpu):
 
 @pytest.mark.timeout(900)
 def test_d_adjoint_against_central_differences(require_gpu):
-    w = configs.config_d()
+    w = configs.cylinder_256x128x32()
     assert tuple(w.grid.ncal) == (256, 128, 32)
     w.params.precond = capi.CFD_PRECOND_MG
     w.params.maxit = 200

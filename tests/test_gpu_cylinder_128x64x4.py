@@ -1,6 +1,6 @@
 """The CPU-runnable workload on its own grid: 128x64x4 cells over
 20D x 10D x 0.5D, a D = 1 cylinder at (5D, 5D), Re 100, dt 0.002, seeded
-+-1e-3 noise (configs.config_a), stepped on the GPU and by the oracle from
++-1e-3 noise (configs.cylinder_128x64x4), stepped on the GPU and by the oracle from
 the same state for all of its 100 steps, with every field compared after
 every step.
 
@@ -40,7 +40,7 @@ def rel_l2(a, b):
 
 @pytest.mark.timeout(900)
 def test_cylinder_128x64x4_steps_match_oracle(require_gpu):
-    w = configs.config_a()
+    w = configs.cylinder_128x64x4()
     assert tuple(w.grid.ncal) == (128, 64, 4)
     w.params.tol = 1e-13
     w.params.maxit = MAXIT

@@ -1,6 +1,6 @@
 """The largest workload at its full size: 1024x512x128 cells (67.1M), x/y
 stretched by tanh clustering (growth 1.05) around a D = 40-cell cylinder,
-Re 300, dt 2.5e-4 (configs.config_e).  The oracle does not finish at this
+Re 300, dt 2.5e-4 (configs.cylinder_1024x512x128).  The oracle does not finish at this
 size, so the checks are size-independent properties:
 
 - every multigrid-preconditioned pressure pass reaches the FP64 relative
@@ -33,7 +33,7 @@ def say(msg):
 
 
 def large(precision, mode):
-    w = configs.config_e(precision)
+    w = configs.cylinder_1024x512x128(precision)
     w.params.precond = capi.CFD_PRECOND_MG
     w.params.mixed_mode = mode
     w.params.tol_inner = 1e-5
