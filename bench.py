@@ -57,7 +57,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def profiled_traffic(key="spmv_dot_kernel<double> traffic_bytes_per_launch:"):
+def profiled_traffic(key="spmv_dot_kernel<double, 512> traffic_bytes_per_launch:"):
     """HBM bytes per launch of a kernel from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration)."""
     path = ROOT / "profiles" / "kernels_512x256x64.md"
@@ -566,7 +566,7 @@ def main():
                    "pressure": pressure, "passes_per_step": 4, "bicgstab_iters_per_step": iters64},
         "roofline": {"bound": "hbm", "achieved": kd["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kd["gbs"] / HBM_PEAK_GBS, "traffic": profiled_traffic(),
-                     "kernel": "spmv_dot_kernel<double>: K1 of the BiCGSTAB iteration, ap = A p (ELL) with the partial "
+                     "kernel": "spmv_dot_kernel<double, 512>: K1 of the BiCGSTAB iteration, ap = A p (ELL) with the partial "
                                "ap . r0; 108 B/row x 8388608 rows per launch",
                      "avg_us": kd["avg_us"]},
         "fp64_solves": solves,

@@ -49,10 +49,10 @@ __device__ inline double wave_max(double v) {
 
 // Sum (or max) NV per-thread values over the block; the block's result is
 // written by thread 0 to part[q * RED_MAX_BLOCKS + blockIdx.x].  Block size
-// must be RED_THREADS.
-template <int NV, bool MAX = false>
+// must be BT.
+template <int NV, bool MAX = false, int BT = RED_THREADS>
 __device__ inline void block_partials(double (&v)[NV], double *part) {
-  __shared__ double lds[NV][RED_THREADS / 64];
+  __shared__ double lds[NV][BT / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < NV; ++q) v[q] = MAX ? wave_max(v[q]) : wave_sum(v[q]);
@@ -65,7 +65,7 @@ __device__ inline void block_partials(double (&v)[NV], double *part) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
       double s = lds[q][0];
-      for (int w = 1; w < RED_THREADS / 64; ++w) s = MAX ? fmax(s, lds[q][w]) : s + lds[q][w];
+      for (int w = 1; w < BT / 64; ++w) s = MAX ? fmax(s, lds[q][w]) : s + lds[q][w];
       part[q * RED_MAX_BLOCKS + blockIdx.x] = s;
     }
   }

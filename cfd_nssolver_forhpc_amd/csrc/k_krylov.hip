@@ -103,10 +103,12 @@ __global__ void __launch_bounds__(FIN_THREADS) init_finalize(const double *part,
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) spmv_dot_kernel(OpArgs a, const KrylovScalars *__restrict__ ks,
-                                                       const T *__restrict__ p1, T *__restrict__ ap,
-                                                       const T *__restrict__ r0, double *part) {
+// BT threads a block: 256 for the CSR operator (op_apply_block stages 256
+// rows), SPMV_DOT_THREADS for the others
+template <typename T, int BT = 256>
+__global__ void __launch_bounds__(BT) spmv_dot_kernel(OpArgs a, const KrylovScalars *__restrict__ ks,
+                                                      const T *__restrict__ p1, T *__restrict__ ap,
+                                                      const T *__restrict__ r0, double *part) {
   if (ks->done) return;
   double acc[1] = {0.0};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -117,7 +119,7 @@ __global__ void __launch_bounds__(256) spmv_dot_kernel(OpArgs a, const KrylovSca
     ntstore(y, ap + r);
     acc[0] += (double)y * (double)r0[r];
   }
-  block_partials<1>(acc, part);
+  block_partials<1, false, BT>(acc, part);
 }
 
 // spmv_dot and spmv_dot2 for the SPE operator over plane columns
@@ -184,9 +186,9 @@ __global__ void __launch_bounds__(256) s_update_kernel(int64_t N, const KrylovSc
     s1[r] = (T)((double)r1[r] - alpha * (double)ap[r]);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) spmv_dot2_kernel(OpArgs a, const KrylovScalars *__restrict__ ks,
-                                                        const T *__restrict__ s1, T *__restrict__ as, double *part) {
+template <typename T, int BT = 256>
+__global__ void __launch_bounds__(BT) spmv_dot2_kernel(OpArgs a, const KrylovScalars *__restrict__ ks,
+                                                       const T *__restrict__ s1, T *__restrict__ as, double *part) {
   if (ks->done) return;
   double acc[2] = {0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -198,7 +200,7 @@ __global__ void __launch_bounds__(256) spmv_dot2_kernel(OpArgs a, const KrylovSc
     acc[0] += (double)y * (double)s1[r];
     acc[1] += (double)y * (double)y;
   }
-  block_partials<2>(acc, part);
+  block_partials<2, false, BT>(acc, part);
 }
 
 __global__ void __launch_bounds__(FIN_THREADS) omega_finalize(const double *part, int nblk, KrylovScalars *ks) {
@@ -397,13 +399,21 @@ struct Vecs {
   double *px1, *px2;
 };
 
+// threads a block of the iteration's SpMV-dot kernels (ELL and SPE-as-rows
+// operators): 512 halves the partials of 256-thread blocks at the same rows
+// a thread
+constexpr int SPMV_DOT_THREADS = 512;
+
 template <typename T>
 hipError_t launch_iters(Ctx &c, const OpArgs &a, const Vecs<T> &v, KrylovScalars *ks, int nit) {
   const int64_t N = c.g.N;
-  const int nb = blocks_for(N, 256, RED_MAX_BLOCKS);
-  // the vector updates take two rows a thread: the s / p updates stream 3%
-  // faster, and the x / r update leaves half the partials for conv_finalize
-  const int nbu = blocks_for(N, 2 * 256, RED_MAX_BLOCKS);
+  // The SpMV-dots run SPMV_DOT_THREADS-thread blocks (256 on the CSR
+  // operator) and the x / r update four rows a thread, which cuts the
+  // partials their one-block finalize kernels combine; the s / p updates
+  // take two rows a thread (3% faster streams).
+  const bool wide = a.op != CFD_OP_CSR;
+  const int nb = blocks_for(N, wide ? SPMV_DOT_THREADS : 256, RED_MAX_BLOCKS);
+  const int nbu = blocks_for(N, 2 * 256, RED_MAX_BLOCKS), nbx = blocks_for(N, 4 * 256, RED_MAX_BLOCKS);
   double *p1 = red_part_host(c, RS_SPMV_DOT), *p2 = red_part_host(c, RS_SPMV_DOT2), *p3 = red_part_host(c, RS_XR);
   // the SPE operator's SpMV-dot kernels over plane columns (FP64)
   const SpeTiles tiles = spe_tiles(c.g.nxc, c.g.nyc, c.g.nzc);
@@ -414,6 +424,8 @@ hipError_t launch_iters(Ctx &c, const OpArgs &a, const Vecs<T> &v, KrylovScalars
       spe_spmv_dot_kernel<<<nbs, 256, 0, c.stream>>>(a, tiles, ks, reinterpret_cast<const double *>(v.p1),
                                                      reinterpret_cast<double *>(v.ap),
                                                      reinterpret_cast<const double *>(v.r0), p1);
+    else if (wide)
+      spmv_dot_kernel<T, SPMV_DOT_THREADS><<<nb, SPMV_DOT_THREADS, 0, c.stream>>>(a, ks, v.p1, v.ap, v.r0, p1);
     else
       spmv_dot_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, ks, v.p1, v.ap, v.r0, p1);
     alpha_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p1, nbs, ks);
@@ -421,11 +433,13 @@ hipError_t launch_iters(Ctx &c, const OpArgs &a, const Vecs<T> &v, KrylovScalars
     if (spe)
       spe_spmv_dot2_kernel<<<nbs, 256, 0, c.stream>>>(a, tiles, ks, reinterpret_cast<const double *>(v.s1),
                                                       reinterpret_cast<double *>(v.as), p2);
+    else if (wide)
+      spmv_dot2_kernel<T, SPMV_DOT_THREADS><<<nb, SPMV_DOT_THREADS, 0, c.stream>>>(a, ks, v.s1, v.as, p2);
     else
       spmv_dot2_kernel<T><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, ks, v.s1, v.as, p2);
     omega_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p2, nbs, ks);
-    xr_update_kernel<T><<<nbu, 256, 0, c.stream>>>(N, ks, v.x, v.p1, v.s1, v.as, v.r1, v.r0, p3);
-    conv_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p3, nbu, ks);
+    xr_update_kernel<T><<<nbx, 256, 0, c.stream>>>(N, ks, v.x, v.p1, v.s1, v.as, v.r1, v.r0, p3);
+    conv_finalize<<<1, FIN_THREADS, 0, c.stream>>>(p3, nbx, ks);
     p_update_kernel<T><<<nbu, 256, 0, c.stream>>>(N, ks, v.r1, v.p1, v.ap, v.r0);
   }
   return hipGetLastError();
@@ -701,14 +715,17 @@ hipError_t launch_bicg_iteration(Ctx &c) {
 // vectors and scalars as launch_bicg_iteration
 hipError_t launch_spmv_dot(Ctx &c) {
   const OpArgs a = op_args(c, c.prm.op);
-  const int nb = blocks_for(c.g.N, 256, RED_MAX_BLOCKS);
   const SpeTiles tiles = spe_tiles(c.g.nxc, c.g.nyc, c.g.nzc);
+  double *part = red_part_host(c, RS_SPMV_DOT);
   if (a.op == CFD_OP_SPE && tiles.blocks <= RED_MAX_BLOCKS)
-    spe_spmv_dot_kernel<<<(int)tiles.blocks, 256, 0, c.stream>>>(a, tiles, c.ks32, c.p1, c.ap, c.r0,
-                                                                 red_part_host(c, RS_SPMV_DOT));
+    spe_spmv_dot_kernel<<<(int)tiles.blocks, 256, 0, c.stream>>>(a, tiles, c.ks32, c.p1, c.ap, c.r0, part);
+  else if (a.op != CFD_OP_CSR)  // launch_iters' launch
+    spmv_dot_kernel<double, SPMV_DOT_THREADS>
+        <<<blocks_for(c.g.N, SPMV_DOT_THREADS, RED_MAX_BLOCKS), SPMV_DOT_THREADS, 0, c.stream>>>(a, c.ks32, c.p1,
+                                                                                               c.ap, c.r0, part);
   else
-    spmv_dot_kernel<double><<<nb, 256, op_lds_bytes(a), c.stream>>>(a, c.ks32, c.p1, c.ap, c.r0,
-                                                                    red_part_host(c, RS_SPMV_DOT));
+    spmv_dot_kernel<double><<<blocks_for(c.g.N, 256, RED_MAX_BLOCKS), 256, op_lds_bytes(a), c.stream>>>(
+        a, c.ks32, c.p1, c.ap, c.r0, part);
   return hipGetLastError();
 }
 
