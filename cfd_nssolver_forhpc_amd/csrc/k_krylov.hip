@@ -168,11 +168,17 @@ __global__ void __launch_bounds__(256) spe_spmv_dot2_kernel(OpArgs a, SpeTiles T
   block_partials<2>(acc, part);
 }
 
+// The finalize kernels read the scalars they use before the partials, so
+// that those reads share the partials' round trip to memory instead of
+// preceding it; with `done` set the totals are combined and dropped.
 __global__ void __launch_bounds__(FIN_THREADS) alpha_finalize(const double *part, int nblk, KrylovScalars *ks) {
-  if (ks->done) return;
+  const int done = ks->done;
+  const double rho = ks->rho;
+  asm volatile("" ::: "memory");  // issue both loads here, not after the combine
   double tot[1];
   combine_partials<1>(part, nblk, tot);
-  if (threadIdx.x == 0) ks->alpha = ks->rho / tot[0];
+  if (done) return;
+  if (threadIdx.x == 0) ks->alpha = rho / tot[0];
 }
 
 template <typename T>
@@ -204,9 +210,11 @@ __global__ void __launch_bounds__(BT) spmv_dot2_kernel(OpArgs a, const KrylovSca
 }
 
 __global__ void __launch_bounds__(FIN_THREADS) omega_finalize(const double *part, int nblk, KrylovScalars *ks) {
-  if (ks->done) return;
+  const int done = ks->done;
+  asm volatile("" ::: "memory");
   double tot[2];
   combine_partials<2>(part, nblk, tot);
+  if (done) return;
   if (threadIdx.x == 0) ks->omega = tot[0] / tot[1];
 }
 
@@ -232,24 +240,29 @@ __global__ void __launch_bounds__(256) xr_update_kernel(int64_t N, const KrylovS
 
 // stop test, beta and the restart rule (bicgstabRe2.hpp:137-183)
 __global__ void __launch_bounds__(FIN_THREADS) conv_finalize(const double *part, int nblk, KrylovScalars *ks) {
-  if (ks->done) return;
+  const KrylovScalars k = *ks;
+  asm volatile("" ::: "memory");
   double tot[2];
   combine_partials<2>(part, nblk, tot);
+  if (k.done) return;
   if (threadIdx.x == 0) {
-    ks->rnorm = sqrt(tot[0]);
-    ks->relres = ks->rnorm / ks->norm0;
-    if (ks->relres < ks->tol) {
+    const double rnorm = sqrt(tot[0]), relres = rnorm / k.norm0;
+    ks->rnorm = rnorm;
+    ks->relres = relres;
+    if (relres < k.tol) {
       ks->done = 1;
-    } else if (ks->iters >= ks->maxit) {
+    } else if (k.iters >= k.maxit) {
       ks->done = 1;
       ks->hit_maxit = 1;
     } else {
-      ks->pre_rho = ks->rho;
-      ks->rho = tot[1];
-      ks->beta = (ks->alpha / ks->omega) * (ks->rho / ks->pre_rho);
-      ks->restart = (ks->timestep < 10 || fabs(ks->pre_rho) > ks->restart_tol) ? 0 : 1;
-      ks->restarts += ks->restart;
-      ks->iters += 1;
+      const double pre_rho = k.rho, rho = tot[1];
+      ks->pre_rho = pre_rho;
+      ks->rho = rho;
+      ks->beta = (k.alpha / k.omega) * (rho / pre_rho);
+      const int restart = (k.timestep < 10 || fabs(pre_rho) > k.restart_tol) ? 0 : 1;
+      ks->restart = restart;
+      ks->restarts = k.restarts + restart;
+      ks->iters = k.iters + 1;
     }
   }
 }
