@@ -301,8 +301,11 @@ __global__ void __launch_bounds__(256) mg_rb_kernel(MGView L, int colour, const 
   rb_at<T, Lay>(L, i, j, k2, colour, b, x);
 }
 
-// planes of the colour-split finest level that one thread of a sweep walks
-constexpr int MG_PLANES = 8;
+// planes of the colour-split finest level that one thread of a sweep walks:
+// at 4 the FP64 sweeps hold 72-112 VGPRs (4-7 waves a SIMD; 8 planes: 112-182,
+// 2-4 waves), and the 512x256x64 multigrid steps ran 1-1.5% faster than at 8
+// (2 planes: 5% slower)
+constexpr int MG_PLANES = 4;
 
 enum SplitSweep { SWEEP_PLAIN, SWEEP_PROLONG, SWEEP_OUT };
 
