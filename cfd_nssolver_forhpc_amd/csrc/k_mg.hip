@@ -502,9 +502,17 @@ __global__ void __launch_bounds__(256) mg_restrict_kernel(MGView F, MGView C, co
 
 constexpr int MG_SMALL_THREADS = 512;
 
+// dense-inverse entries a thread of the bottom kernel prefetches: with the
+// levels in LDS and an inverse of at most DENSE_PF * MG_SMALL_THREADS
+// entries (64 cells), the whole inverse is loaded with the first level's x and
+// b and staged through the LDS left over, instead of 16+ dependent round
+// trips to memory at the coarsest solve
+constexpr int DENSE_PF = 8;
+
 template <typename T>
 struct SmallArgs {
   int nlev, nu, coarse_sweeps, dense;
+  int stage_off, stage_cols;   // dense-inverse staging in LDS: element offset, columns a chunk (0: none)
   int lds_off[MG_MAX_LEVELS];  // with the levels in LDS: element offset of level l's x (its b follows)
   MGView v[MG_MAX_LEVELS];
   T *x[MG_MAX_LEVELS];
@@ -514,6 +522,9 @@ struct SmallArgs {
 
 // fills lds_off; the LDS bytes to launch with (0: the levels do not fit in
 // a workgroup's default 64 KiB and stay in global memory)
+// fills lds_off and, when the dense inverse is prefetched, stage_off /
+// stage_cols; the LDS bytes to launch with (0: the levels do not fit in a
+// workgroup's default 64 KiB and stay in global memory)
 template <typename T>
 size_t small_lds_layout(SmallArgs<T> &S) {
   size_t elems = 0;
@@ -521,8 +532,19 @@ size_t small_lds_layout(SmallArgs<T> &S) {
     S.lds_off[l] = (int)elems;
     elems += 2 * (size_t)S.v[l].n;
   }
-  const size_t bytes = elems * sizeof(T);
-  return bytes <= 65536 ? bytes : 0;
+  S.stage_off = 0;
+  S.stage_cols = 0;
+  if (elems * sizeof(T) > 65536) return 0;
+  const int64_t n = S.v[S.nlev - 1].n;
+  if (S.dense && n * n <= (int64_t)DENSE_PF * MG_SMALL_THREADS) {
+    const int64_t cols = std::min<int64_t>(n, (65536 / (int64_t)sizeof(T) - (int64_t)elems) / n);
+    if (cols >= 1) {
+      S.stage_off = (int)elems;
+      S.stage_cols = (int)cols;
+      elems += (size_t)(cols * n);
+    }
+  }
+  return elems * sizeof(T);
 }
 
 // the bottom of the V-cycle (levels of at most MG_SMALL cells) in one
@@ -548,8 +570,35 @@ __global__ void __launch_bounds__(MG_SMALL_THREADS) mg_small_kernel(SmallArgs<T>
     if constexpr (LDS) return lds + S.lds_off[l] + S.v[l].n;
     else return S.b[l];
   };
+  // with the levels in LDS: the first level's x and b (at most MG_SMALL
+  // cells, 8 a thread) and, with staging, the dense inverse are loaded
+  // before the first LDS store, one round trip to memory for all of them
+  T pf[DENSE_PF];
+  const int nn = S.stage_cols > 0 ? (int)(S.v[S.nlev - 1].n * S.v[S.nlev - 1].n) : 0;
   if (LDS) {
-    for (int t = threadIdx.x; t < (int)S.v[0].n; t += nt) {
+    constexpr int CP = (int)(MG_SMALL / MG_SMALL_THREADS);
+    const int n0 = (int)S.v[0].n;
+    T xv[CP], bv[CP];
+#pragma unroll
+    for (int u = 0; u < CP; ++u) {
+      const int t = threadIdx.x + u * nt;
+      xv[u] = t < n0 ? S.x[0][t] : (T)0;
+      bv[u] = t < n0 ? S.b[0][t] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < DENSE_PF; ++u) {
+      const int e = threadIdx.x + u * nt;
+      pf[u] = e < nn ? S.cinv[e] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < CP; ++u) {
+      const int t = threadIdx.x + u * nt;
+      if (t < n0) {
+        X(0)[t] = xv[u];
+        B(0)[t] = bv[u];
+      }
+    }
+    for (int t = threadIdx.x + CP * nt; t < n0; t += nt) {  // not taken: the levels hold at most MG_SMALL cells
       X(0)[t] = S.x[0][t];
       B(0)[t] = S.b[0][t];
     }
@@ -587,7 +636,30 @@ __global__ void __launch_bounds__(MG_SMALL_THREADS) mg_small_kernel(SmallArgs<T>
     }
     __syncthreads();
   }
-  if (S.dense) {
+  if (S.dense && LDS && S.stage_cols > 0) {
+    // the prefetched inverse (entry e = q n + t in thread e % nt) through
+    // the LDS stage, stage_cols columns at a time; thread t < n sums its row
+    // over q = 0 .. n - 1 in order, as below
+    const int n = (int)S.v[last].n, sc = S.stage_cols;
+    const T *bl = B(last);
+    T *xl = X(last);
+    T *stage = lds + S.stage_off;
+    double acc = 0.0;
+    for (int q0 = 0; q0 < n; q0 += sc) {
+      const int q1 = min(n, q0 + sc);
+#pragma unroll
+      for (int u = 0; u < DENSE_PF; ++u) {
+        const int e = threadIdx.x + u * nt;
+        if (e >= q0 * n && e < q1 * n) stage[e - q0 * n] = pf[u];
+      }
+      __syncthreads();
+      if ((int)threadIdx.x < n)
+        for (int q = q0; q < q1; ++q) acc += (double)stage[(q - q0) * n + threadIdx.x] * (double)bl[q];
+      __syncthreads();
+    }
+    if ((int)threadIdx.x < n) xl[threadIdx.x] = (T)acc;
+    __syncthreads();
+  } else if (S.dense) {
     // consecutive threads read consecutive entries of a column
     const int n = (int)S.v[last].n;
     const T *bl = B(last);
