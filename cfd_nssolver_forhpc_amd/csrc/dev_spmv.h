@@ -179,6 +179,17 @@ __device__ inline SpeColumn spe_column(const OpArgs &a, const SpeTiles &T) {
   return c;
 }
 
+// v at the column's row of each of its planes, loaded together (a plane
+// past the grid loads the last plane's row, which is never used), so that a
+// kernel that pairs the column's outputs with another vector waits for those
+// loads once instead of once per plane
+__device__ inline void spe_plane_values(const OpArgs &a, const SpeColumn &c, const double *__restrict__ v,
+                                        double (&w)[SPE_PLANES]) {
+  const int64_t nynz = (int64_t)a.nyc * a.nzc, col = (int64_t)c.j * a.nzc + c.k;
+#pragma unroll
+  for (int p = 0; p < SPE_PLANES; ++p) w[p] = v[(int64_t)min(c.i0 + p, a.nxc - 1) * nynz + col];
+}
+
 // y[p] = (A x)[row of plane i0 + p] and xc[p] = x at that row, for the
 // planes inside the grid (np of them)
 __device__ inline int spe_column_rows(const OpArgs &a, const SpeColumn &c, const double *__restrict__ x,

@@ -133,15 +133,17 @@ __global__ void __launch_bounds__(256) spe_spmv_dot_kernel(OpArgs a, SpeTiles T,
   double acc[1] = {0.0};
   const SpeColumn c = spe_column(a, T);
   if (c.ok) {
+    const int64_t nynz = (int64_t)a.nyc * a.nzc, col = (int64_t)c.j * a.nzc + c.k;
+    double w[SPE_PLANES];
+    spe_plane_values(a, c, r0, w);
     double y[SPE_PLANES], xc[SPE_PLANES];
     const int np = spe_column_rows(a, c, p1, y, xc);
-    const int64_t nynz = (int64_t)a.nyc * a.nzc, col = (int64_t)c.j * a.nzc + c.k;
 #pragma unroll
     for (int p = 0; p < SPE_PLANES; ++p)
       if (p < np) {
         const int64_t r = (int64_t)(c.i0 + p) * nynz + col;
         ntstore(y[p], ap + r);
-        acc[0] += y[p] * r0[r];
+        acc[0] += y[p] * w[p];
       }
   }
   block_partials<1>(acc, part);
@@ -314,15 +316,17 @@ __global__ void __launch_bounds__(256) spe_spmv_dot2_pc_kernel(OpArgs a, SpeTile
   double acc[2] = {0.0, 0.0};
   const SpeColumn c = spe_column(a, T);
   if (c.ok) {
+    const int64_t nynz = (int64_t)a.nyc * a.nzc, col = (int64_t)c.j * a.nzc + c.k;
+    double w[SPE_PLANES];
+    spe_plane_values(a, c, s1, w);
     double y[SPE_PLANES], xc[SPE_PLANES];
     const int np = spe_column_rows(a, c, sh, y, xc);
-    const int64_t nynz = (int64_t)a.nyc * a.nzc, col = (int64_t)c.j * a.nzc + c.k;
 #pragma unroll
     for (int p = 0; p < SPE_PLANES; ++p)
       if (p < np) {
         const int64_t r = (int64_t)(c.i0 + p) * nynz + col;
         ntstore(y[p], tv + r);
-        acc[0] += y[p] * s1[r];
+        acc[0] += y[p] * w[p];
         acc[1] += y[p] * y[p];
       }
   }
