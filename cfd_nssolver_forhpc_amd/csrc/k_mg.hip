@@ -520,8 +520,6 @@ struct SmallArgs {
   const T *cinv;  // dense inverse of the coarsest operator, column-major, when dense
 };
 
-// fills lds_off; the LDS bytes to launch with (0: the levels do not fit in
-// a workgroup's default 64 KiB and stay in global memory)
 // fills lds_off and, when the dense inverse is prefetched, stage_off /
 // stage_cols; the LDS bytes to launch with (0: the levels do not fit in a
 // workgroup's default 64 KiB and stay in global memory)
