@@ -41,6 +41,36 @@ def large(precision, mode):
     return w
 
 
+def test_large_jacobi_bicgstab_residual_is_the_true_one(require_gpu):
+    """The reference's Jacobi-scaled BiCGSTAB on the 67M-row ELL operator,
+    where every kernel of the iteration runs its grid-stride loop (more rows
+    than its grid's threads): after 30 iterations from x = 0 the solver's
+    recursive relative residual must equal ||b - A x|| / ||b|| recomputed
+    from the returned x (cfd_spmv), which holds only if every row's SpMV and
+    update ran.  A skipped row block leaves a gap of order its share of
+    ||b||; rounding leaves ~1e-13."""
+    w = large(capi.CFD_FP64, capi.CFD_MIXED_RESTARTED)
+    w.params.precond = capi.CFD_PRECOND_JACOBI
+    w.params.op = capi.CFD_OP_ELL
+    say(f"[large] jacobi bicgstab: setup {w.name}")
+    sim = Simulation(w.grid, w.params)
+    sim.init_state()
+    sim.createPressureMatrix()
+    assert sim.N == 1024 * 512 * 128
+    b = np.random.default_rng(5).standard_normal(sim.N)
+    sim.upload(capi.RHS, b)
+    sim.upload(capi.X, np.zeros(sim.N))
+    it, rel = sim.bicgstab(tol=1e-300, maxit=30)
+    sim.spmv(capi.X, capi.VEC_A)
+    ax = sim.download(capi.VEC_A)
+    true = float(np.linalg.norm(b - ax) / np.linalg.norm(b))
+    say(f"[large] jacobi bicgstab: {it} iterations, relres {rel:.6e}, true {true:.6e}")
+    sim.close()
+    assert 25 <= it <= 31
+    assert 0.0 < rel < 0.9
+    assert abs(rel - true) <= 1e-9 * true
+
+
 def test_large_stretched_mixed_against_fp64(require_gpu):
     """Three steps of the 67M-cell case in FP64 and in the two mixed schemes
     (the first step has no pressure solve, as in the reference).  Each pass stops at a
