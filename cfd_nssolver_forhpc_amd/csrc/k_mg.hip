@@ -29,9 +29,11 @@
 // threads walk MG_PLANES planes and load each other-colour entry once for
 // its three uses, the post-smoothing's first sweep with the prolongation
 // folded in and its last one writing the result, and a restriction that
-// reads the children as colour pairs.  Each does the arithmetic of the
-// generic per-cell functions term for term, so the cycle's result is the
-// same bit for bit in either layout.
+// reads the children as colour pairs.  Level 1 runs the same kernels in the
+// same layout when the finest level halves every axis and level 1 is not
+// part of the one-workgroup bottom (split1).  Each does the arithmetic of
+// the generic per-cell functions term for term, so the cycle's result is
+// the same bit for bit in either layout.
 //
 // Every kernel returns at once when the Krylov solver's `done` flag is set,
 // so preconditioned iterations can be launched in batches like the plain
@@ -71,6 +73,7 @@ struct MGState {
   int small_from = 0;   // first level handled by the one-workgroup kernel
   int nu = 2;
   std::string shape;
+  bool split1 = false;  // level 1 is colour-split too (its sweeps run the finest level's kernels)
   int dense_n = 0;      // > 0: the coarsest level (dense_n cells) is solved by its dense inverse
   double *cinv64 = nullptr;
   float *cinv32 = nullptr;
@@ -322,7 +325,7 @@ enum SplitSweep { SWEEP_PLAIN, SWEEP_PROLONG, SWEEP_OUT };
 //     neighbours are read as x + xc[parent], as in rb_prolong_at.
 //   SWEEP_OUT: the cycle's last black half-sweep; it writes the new black
 //     value and the red one (its cen) to out in natural layout, precision V.
-template <typename T, typename V, int MODE>
+template <typename T, typename V, int MODE, typename CLay = Natural>
 __global__ void __launch_bounds__(256) mg_split_sweep_kernel(MGView L, MGView C, int colour, const T *__restrict__ b,
                                                              T *__restrict__ x, const T *__restrict__ xc,
                                                              V *__restrict__ out, const KrylovScalars *ks) {
@@ -350,7 +353,7 @@ __global__ void __launch_bounds__(256) mg_split_sweep_kernel(MGView L, MGView C,
   }
   // the parent's indices along y and z of a neighbour at (j', k'); x is per plane
   auto par = [&](int ip, int jp, int kp) -> int64_t {
-    return idx3(C, L.fx == 2 ? ip >> 1 : ip, L.fy == 2 ? jp >> 1 : jp, L.fz == 2 ? kp >> 1 : kp);
+    return CLay::at(C, L.fx == 2 ? ip >> 1 : ip, L.fy == 2 ? jp >> 1 : jp, L.fz == 2 ? kp >> 1 : kp);
   };
   T cen[MG_PLANES + 2];
 #pragma unroll
@@ -420,7 +423,7 @@ __global__ void __launch_bounds__(256) mg_split_sweep_kernel(MGView L, MGView C,
 // beside them and the single entries below (2K - 1) and above (2K + 2) each
 // column (40 loads for 8 children, with b), and sums the children's
 // residuals in restrict_at's order with child_res's arithmetic.
-template <typename T>
+template <typename T, typename CLay = Natural>
 __global__ void __launch_bounds__(256) mg_restrict_split_kernel(MGView F, MGView C, const T *__restrict__ bf,
                                                                 const T *__restrict__ xf, T *__restrict__ bc,
                                                                 T *__restrict__ xc, const KrylovScalars *ks) {
@@ -477,7 +480,7 @@ __global__ void __launch_bounds__(256) mg_restrict_split_kernel(MGView F, MGView
     const T r = B[di][dj][m] - (mg_diag<T>(F, i, j, k) * xv - s);
     acc += mg_vol<T>(F, i, j, k) * (double)r;
   }
-  set_rhs<T, Natural>(C, I, J, K, (T)(acc / mg_vol<T>(C, I, J, K)), bc, xc);
+  set_rhs<T, CLay>(C, I, J, K, (T)(acc / mg_vol<T>(C, I, J, K)), bc, xc);
 }
 
 template <typename T, typename Lay>
@@ -761,18 +764,22 @@ int mg_apply(Ctx &c, const V *r, V *out, const KrylovScalars *ks) {
       for (int colour = 0; colour < 2; ++colour) {
         const bool first = s == 0 && colour == 0, last = s == M.nu - 1 && colour == 1;
         if (first && pre) continue;
-        if (l == 0) {
+        if (l == 0 || (l == 1 && M.split1)) {
           Shape3 sh = shape3((v.nz + 1) / 2, v.ny, v.nx);
           sh.grid.z = (v.nx + MG_PLANES - 1) / MG_PLANES;
-          const MGView &C = M.lv[1].v;
-          if (first)
-            mg_split_sweep_kernel<T, T, SWEEP_PROLONG><<<sh.grid, sh.block, 0, c.stream>>>(v, C, 0, b0, x0, X(1),
+          const MGView &C = M.lv[l + 1].v;
+          T *bl = l == 0 ? b0 : lb<T>(M.lv[1]), *xl = X(l);
+          if (first && l == 0 && M.split1)  // the correction comes from the colour-split level 1
+            mg_split_sweep_kernel<T, T, SWEEP_PROLONG, ColourSplit><<<sh.grid, sh.block, 0, c.stream>>>(
+                v, C, 0, bl, xl, X(1), nullptr, ks);
+          else if (first)
+            mg_split_sweep_kernel<T, T, SWEEP_PROLONG><<<sh.grid, sh.block, 0, c.stream>>>(v, C, 0, bl, xl, X(l + 1),
                                                                                           nullptr, ks);
-          else if (last && !pre)
-            mg_split_sweep_kernel<T, V, SWEEP_OUT><<<sh.grid, sh.block, 0, c.stream>>>(v, C, 1, b0, x0, nullptr, out,
+          else if (last && !pre && l == 0)
+            mg_split_sweep_kernel<T, V, SWEEP_OUT><<<sh.grid, sh.block, 0, c.stream>>>(v, C, 1, bl, xl, nullptr, out,
                                                                                       ks);
           else
-            mg_split_sweep_kernel<T, T, SWEEP_PLAIN><<<sh.grid, sh.block, 0, c.stream>>>(v, C, colour, b0, x0,
+            mg_split_sweep_kernel<T, T, SWEEP_PLAIN><<<sh.grid, sh.block, 0, c.stream>>>(v, C, colour, bl, xl,
                                                                                         nullptr, nullptr, ks);
         } else {
           using Ops = LevelOps<T, Natural>;
@@ -786,12 +793,19 @@ int mg_apply(Ctx &c, const V *r, V *out, const KrylovScalars *ks) {
   for (int l = 0; l < ls; ++l) {
     smooth(l, true);
     const MGView &F = M.lv[l].v;
-    if (l == 0 && F.fx == 2 && F.fy == 2 && F.fz == 2) {
-      const MGView &C = M.lv[1].v;
+    const bool fsplit = l == 0 || (l == 1 && M.split1);
+    T *bf = l == 0 ? b0 : lb<T>(M.lv[l]);
+    if (fsplit && F.fx == 2 && F.fy == 2 && F.fz == 2) {
+      const MGView &C = M.lv[l + 1].v;
       const Shape3 sh = shape3(C.nz, C.ny, C.nx);
-      mg_restrict_split_kernel<T><<<sh.grid, sh.block, 0, c.stream>>>(F, C, b0, x0, lb<T>(M.lv[1]), X(1), ks);
-    } else if (l == 0)
-      LevelOps<T, ColourSplit>::restrict_to(F, M.lv[1].v, b0, x0, lb<T>(M.lv[1]), X(1), ks, c.stream);
+      if (l == 0 && M.split1)  // level 1's right-hand side in its colour-split layout
+        mg_restrict_split_kernel<T, ColourSplit><<<sh.grid, sh.block, 0, c.stream>>>(F, C, bf, X(l), lb<T>(M.lv[1]),
+                                                                                    X(1), ks);
+      else
+        mg_restrict_split_kernel<T><<<sh.grid, sh.block, 0, c.stream>>>(F, C, bf, X(l), lb<T>(M.lv[l + 1]),
+                                                                        X(l + 1), ks);
+    } else if (fsplit)  // a colour-split level into a natural one, cell by cell
+      LevelOps<T, ColourSplit>::restrict_to(F, M.lv[l + 1].v, bf, X(l), lb<T>(M.lv[l + 1]), X(l + 1), ks, c.stream);
     else
       LevelOps<T, Natural>::restrict_to(M.lv[l].v, M.lv[l + 1].v, lb<T>(M.lv[l]), X(l), lb<T>(M.lv[l + 1]),
                                         X(l + 1), ks, c.stream);
@@ -927,6 +941,10 @@ int mg_ensure(Ctx &c, bool fp32) {
     }
     M->small_from = (int)M->lv.size() - 1;
     while (M->small_from > 0 && M->lv[M->small_from - 1].v.n <= MG_SMALL) M->small_from--;
+    // level 1 runs colour-split when it has level kernels and the finest
+    // level's split restriction (every axis halved) produces its right-hand side
+    const MGView &L0 = M->lv[0].v;
+    M->split1 = M->small_from > 1 && L0.fx == 2 && L0.fy == 2 && L0.fz == 2;
     // exact coarsest solve: the dense inverse of the coarsest operator when a
     // Dirichlet face makes it nonsingular (all-Neumann keeps the sweeps)
     const MGView &Lc = M->lv.back().v;
@@ -996,8 +1014,10 @@ int mg_ensure(Ctx &c, bool fp32) {
   M.nu = c.prm.mg_nu > 0 ? c.prm.mg_nu : 4;
   for (size_t l = 0; l < M.lv.size(); ++l) {
     MGLevel &L = M.lv[l];
-    // the finest level's colour-split vectors hold nx ny ceil(nz / 2) cells per colour
-    const size_t n = l == 0 ? (size_t)L.v.nx * L.v.ny * (2 * ((L.v.nz + 1) / 2)) : (size_t)L.v.n;
+    // colour-split vectors (the finest level, and level 1 with split1) hold
+    // nx ny ceil(nz / 2) cells per colour
+    const bool sp = l == 0 || (l == 1 && M.split1);
+    const size_t n = sp ? (size_t)L.v.nx * L.v.ny * (2 * ((L.v.nz + 1) / 2)) : (size_t)L.v.n;
     int rc = CFD_OK;
     if (fp32) {
       if (!L.b32 && (rc = mg_alloc(&L.b32, n))) return rc;
