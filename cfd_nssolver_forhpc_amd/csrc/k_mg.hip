@@ -29,11 +29,12 @@
 // threads walk MG_PLANES planes and load each other-colour entry once for
 // its three uses, the post-smoothing's first sweep with the prolongation
 // folded in and its last one writing the result, and a restriction that
-// reads the children as colour pairs.  Level 1 runs the same kernels in the
-// same layout when the finest level halves every axis and level 1 is not
-// part of the one-workgroup bottom (split1).  Each does the arithmetic of
-// the generic per-cell functions term for term, so the cycle's result is
-// the same bit for bit in either layout.
+// reads the children as colour pairs.  The next levels run the same kernels
+// in the same layout while the level above halves every axis and they are
+// not part of the one-workgroup bottom, up to MG_SPLIT_MAX levels
+// (split_levels).  Each does the arithmetic of the generic per-cell
+// functions term for term, so the cycle's result is the same bit for bit in
+// either layout.
 //
 // Every kernel returns at once when the Krylov solver's `done` flag is set,
 // so preconditioned iterations can be launched in batches like the plain
@@ -52,6 +53,12 @@ namespace cfd {
 constexpr int MG_MAX_LEVELS = 16;
 constexpr int64_t MG_SMALL = 4096;
 constexpr int MG_COARSE_SWEEPS = 48;
+// colour-split levels at most (the finest one and the next ones).  At
+// 512x256x64, two (512x256x64, 256x128x32) ran the multigrid steps 1-2%
+// faster than the finest alone; a third level (128x64x16) gave that back
+// and a fourth cost 2% more: the small levels' sweeps are bound by their
+// launches, and fewer threads walking planes hide less latency.
+constexpr int MG_SPLIT_MAX = 2;
 
 // per-level device view: ax = cm[nx] cp[nx] dd[nx] w[nx] (and ay, az)
 struct MGView {
@@ -73,7 +80,7 @@ struct MGState {
   int small_from = 0;   // first level handled by the one-workgroup kernel
   int nu = 2;
   std::string shape;
-  bool split1 = false;  // level 1 is colour-split too (its sweeps run the finest level's kernels)
+  int split_levels = 0; // levels 0 .. split_levels - 1 are colour-split (they run the split kernels)
   int dense_n = 0;      // > 0: the coarsest level (dense_n cells) is solved by its dense inverse
   double *cinv64 = nullptr;
   float *cinv32 = nullptr;
@@ -764,14 +771,14 @@ int mg_apply(Ctx &c, const V *r, V *out, const KrylovScalars *ks) {
       for (int colour = 0; colour < 2; ++colour) {
         const bool first = s == 0 && colour == 0, last = s == M.nu - 1 && colour == 1;
         if (first && pre) continue;
-        if (l == 0 || (l == 1 && M.split1)) {
+        if (l < M.split_levels) {
           Shape3 sh = shape3((v.nz + 1) / 2, v.ny, v.nx);
           sh.grid.z = (v.nx + MG_PLANES - 1) / MG_PLANES;
           const MGView &C = M.lv[l + 1].v;
-          T *bl = l == 0 ? b0 : lb<T>(M.lv[1]), *xl = X(l);
-          if (first && l == 0 && M.split1)  // the correction comes from the colour-split level 1
+          T *bl = l == 0 ? b0 : lb<T>(M.lv[l]), *xl = X(l);
+          if (first && l + 1 < M.split_levels)  // the correction comes from a colour-split level
             mg_split_sweep_kernel<T, T, SWEEP_PROLONG, ColourSplit><<<sh.grid, sh.block, 0, c.stream>>>(
-                v, C, 0, bl, xl, X(1), nullptr, ks);
+                v, C, 0, bl, xl, X(l + 1), nullptr, ks);
           else if (first)
             mg_split_sweep_kernel<T, T, SWEEP_PROLONG><<<sh.grid, sh.block, 0, c.stream>>>(v, C, 0, bl, xl, X(l + 1),
                                                                                           nullptr, ks);
@@ -793,14 +800,14 @@ int mg_apply(Ctx &c, const V *r, V *out, const KrylovScalars *ks) {
   for (int l = 0; l < ls; ++l) {
     smooth(l, true);
     const MGView &F = M.lv[l].v;
-    const bool fsplit = l == 0 || (l == 1 && M.split1);
+    const bool fsplit = l < M.split_levels;
     T *bf = l == 0 ? b0 : lb<T>(M.lv[l]);
     if (fsplit && F.fx == 2 && F.fy == 2 && F.fz == 2) {
       const MGView &C = M.lv[l + 1].v;
       const Shape3 sh = shape3(C.nz, C.ny, C.nx);
-      if (l == 0 && M.split1)  // level 1's right-hand side in its colour-split layout
-        mg_restrict_split_kernel<T, ColourSplit><<<sh.grid, sh.block, 0, c.stream>>>(F, C, bf, X(l), lb<T>(M.lv[1]),
-                                                                                    X(1), ks);
+      if (l + 1 < M.split_levels)  // the coarse right-hand side in its colour-split layout
+        mg_restrict_split_kernel<T, ColourSplit><<<sh.grid, sh.block, 0, c.stream>>>(
+            F, C, bf, X(l), lb<T>(M.lv[l + 1]), X(l + 1), ks);
       else
         mg_restrict_split_kernel<T><<<sh.grid, sh.block, 0, c.stream>>>(F, C, bf, X(l), lb<T>(M.lv[l + 1]),
                                                                         X(l + 1), ks);
@@ -941,10 +948,15 @@ int mg_ensure(Ctx &c, bool fp32) {
     }
     M->small_from = (int)M->lv.size() - 1;
     while (M->small_from > 0 && M->lv[M->small_from - 1].v.n <= MG_SMALL) M->small_from--;
-    // level 1 runs colour-split when it has level kernels and the finest
-    // level's split restriction (every axis halved) produces its right-hand side
-    const MGView &L0 = M->lv[0].v;
-    M->split1 = M->small_from > 1 && L0.fx == 2 && L0.fy == 2 && L0.fz == 2;
+    // the finest level is colour-split whenever it has level kernels; each
+    // next one is too while the level above halves every axis (its split
+    // restriction then writes the coarse right-hand side in that layout)
+    M->split_levels = M->small_from > 0 ? 1 : 0;
+    while (M->split_levels < M->small_from && M->split_levels < MG_SPLIT_MAX) {
+      const MGView &P = M->lv[M->split_levels - 1].v;
+      if (!(P.fx == 2 && P.fy == 2 && P.fz == 2)) break;
+      M->split_levels++;
+    }
     // exact coarsest solve: the dense inverse of the coarsest operator when a
     // Dirichlet face makes it nonsingular (all-Neumann keeps the sweeps)
     const MGView &Lc = M->lv.back().v;
@@ -1014,9 +1026,8 @@ int mg_ensure(Ctx &c, bool fp32) {
   M.nu = c.prm.mg_nu > 0 ? c.prm.mg_nu : 4;
   for (size_t l = 0; l < M.lv.size(); ++l) {
     MGLevel &L = M.lv[l];
-    // colour-split vectors (the finest level, and level 1 with split1) hold
-    // nx ny ceil(nz / 2) cells per colour
-    const bool sp = l == 0 || (l == 1 && M.split1);
+    // colour-split vectors hold nx ny ceil(nz / 2) cells per colour
+    const bool sp = (int)l < M.split_levels;
     const size_t n = sp ? (size_t)L.v.nx * L.v.ny * (2 * ((L.v.nz + 1) / 2)) : (size_t)L.v.n;
     int rc = CFD_OK;
     if (fp32) {
