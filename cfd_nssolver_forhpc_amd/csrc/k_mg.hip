@@ -73,6 +73,10 @@ struct MGLevel {
   double *coef = nullptr;               // ax | ay | az
   double *x64 = nullptr, *b64 = nullptr;
   float *x32 = nullptr, *b32 = nullptr;
+  // a colour-split level below the finest: its x in natural layout, written
+  // by its last post-smoothing half-sweep for the prolongation above it
+  double *xn64 = nullptr;
+  float *xn32 = nullptr;
 };
 
 struct MGState {
@@ -330,8 +334,10 @@ enum SplitSweep { SWEEP_PLAIN, SWEEP_PROLONG, SWEEP_OUT };
 // plane-uniform.  The arithmetic is rb_at's, term for term.
 //   SWEEP_PROLONG: the first red half-sweep after the coarse correction: the
 //     neighbours are read as x + xc[parent], as in rb_prolong_at.
-//   SWEEP_OUT: the cycle's last black half-sweep; it writes the new black
-//     value and the red one (its cen) to out in natural layout, precision V.
+//   SWEEP_OUT: the last black half-sweep of the finest level (or of a
+//     colour-split coarse level, whose natural copy the prolongation above
+//     reads); it writes the new black value and the red one (its cen) to
+//     out in natural layout, precision V.
 template <typename T, typename V, int MODE, typename CLay = Natural>
 __global__ void __launch_bounds__(256) mg_split_sweep_kernel(MGView L, MGView C, int colour, const T *__restrict__ b,
                                                              T *__restrict__ x, const T *__restrict__ xc,
@@ -709,6 +715,12 @@ double *lx<double>(MGLevel &L) { return L.x64; }
 template <>
 float *lx<float>(MGLevel &L) { return L.x32; }
 template <typename T>
+T *lxn(MGLevel &L);
+template <>
+double *lxn<double>(MGLevel &L) { return L.xn64; }
+template <>
+float *lxn<float>(MGLevel &L) { return L.xn32; }
+template <typename T>
 T *lb(MGLevel &L);
 template <>
 double *lb<double>(MGLevel &L) { return L.b64; }
@@ -776,9 +788,13 @@ int mg_apply(Ctx &c, const V *r, V *out, const KrylovScalars *ks) {
           sh.grid.z = (v.nx + MG_PLANES - 1) / MG_PLANES;
           const MGView &C = M.lv[l + 1].v;
           T *bl = l == 0 ? b0 : lb<T>(M.lv[l]), *xl = X(l);
-          if (first && l + 1 < M.split_levels)  // the correction comes from a colour-split level
-            mg_split_sweep_kernel<T, T, SWEEP_PROLONG, ColourSplit><<<sh.grid, sh.block, 0, c.stream>>>(
-                v, C, 0, bl, xl, X(l + 1), nullptr, ks);
+          if (first && l + 1 < M.split_levels)  // the correction from a split level's natural copy
+            mg_split_sweep_kernel<T, T, SWEEP_PROLONG><<<sh.grid, sh.block, 0, c.stream>>>(v, C, 0, bl, xl,
+                                                                                          lxn<T>(M.lv[l + 1]),
+                                                                                          nullptr, ks);
+          else if (last && !pre && l > 0)  // a split level's last half-sweep writes its natural copy
+            mg_split_sweep_kernel<T, T, SWEEP_OUT><<<sh.grid, sh.block, 0, c.stream>>>(v, C, 1, bl, xl, nullptr,
+                                                                                      lxn<T>(M.lv[l]), ks);
           else if (first)
             mg_split_sweep_kernel<T, T, SWEEP_PROLONG><<<sh.grid, sh.block, 0, c.stream>>>(v, C, 0, bl, xl, X(l + 1),
                                                                                           nullptr, ks);
@@ -868,6 +884,8 @@ void mg_release(Ctx &c) {
     mg_free(L.b64);
     mg_free(L.x32);
     mg_free(L.b32);
+    mg_free(L.xn64);
+    mg_free(L.xn32);
   }
   mg_free(c.mg->cinv64);
   mg_free(c.mg->cinv32);
@@ -1029,13 +1047,16 @@ int mg_ensure(Ctx &c, bool fp32) {
     // colour-split vectors hold nx ny ceil(nz / 2) cells per colour
     const bool sp = (int)l < M.split_levels;
     const size_t n = sp ? (size_t)L.v.nx * L.v.ny * (2 * ((L.v.nz + 1) / 2)) : (size_t)L.v.n;
+    const bool nat = sp && l > 0;  // a split level below the finest keeps a natural copy of x
     int rc = CFD_OK;
     if (fp32) {
       if (!L.b32 && (rc = mg_alloc(&L.b32, n))) return rc;
       if (!L.x32 && (rc = mg_alloc(&L.x32, n))) return rc;
+      if (nat && !L.xn32 && (rc = mg_alloc(&L.xn32, (size_t)L.v.n))) return rc;
     } else {
       if (!L.b64 && (rc = mg_alloc(&L.b64, n))) return rc;
       if (!L.x64 && (rc = mg_alloc(&L.x64, n))) return rc;
+      if (nat && !L.xn64 && (rc = mg_alloc(&L.xn64, (size_t)L.v.n))) return rc;
     }
   }
   const size_t N = (size_t)c.g.N;
